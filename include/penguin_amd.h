@@ -1,0 +1,109 @@
+/*
+ * penguin_amd.h -- C ABI of the MI355X Godunov-sweep library.
+ *
+ * The reference's boundary for this path is a set of plain C++ host
+ * functions over Grid* arrays, all compiled against include/parameters.h
+ * (reference include/solver.h:4, timestep.h:4, orbit.h:4-7, planet.h:4-5,
+ * viscosity.h:4-5, boundary.h:4-14).  Each entry point below replaces one of
+ * them; the comment names the reference function and file:line.
+ *
+ * Conventions
+ *   - every function returns 0 on success or a negative PG_E_* code; the
+ *     message of the last failure on the calling thread is pg_last_error();
+ *     no C++ exception crosses this boundary;
+ *   - a pg_handle owns its device memory and one HIP stream; it is not
+ *     thread-safe: one host thread per handle;
+ *   - state buffers are host FP64 arrays of interior cells, x (radius)
+ *     fastest, one contiguous block per field: gas rho, p, u (radial),
+ *     v (azimuthal), w; dust rho, u, v, w -- the order of the reference's
+ *     snapshot writer (src/output/output.cu:95-180).
+ */
+#ifndef PENGUIN_AMD_H
+#define PENGUIN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pg_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_E_INVALID (-1)     /* bad argument or unsupported configuration */
+#define PG_E_HIP (-2)         /* a HIP runtime call failed                 */
+#define PG_E_NOMEM (-3)       /* device or host allocation failed          */
+#define PG_E_STATE (-4)       /* call not valid in the handle's state      */
+
+typedef struct pg_handle pg_handle;
+
+/* Last error message of the calling thread ("" if none). */
+const char* pg_last_error(void);
+/* Library version string. */
+const char* pg_version(void);
+
+/* Allocate a disk and fill it with the initial state: grid (init.cu:472-500),
+   profiles (init.cu:502-532), planet (planet.cu:319-403) and orbital-advection
+   rotation (orbit.cu:111-120); replaces main.cu:157-301. */
+int pg_create(const pg_params* params, pg_handle** out);
+/* Free everything the handle owns (main.cu:433-518). */
+int pg_destroy(pg_handle* h);
+
+/* Copy the interior state to/from host FP64 buffers (main.cu:61-104).
+   gas: 5*xres*yres doubles (rho,p,u,v,w); dust: 4*xres*yres (rho,u,v,w) or
+   NULL.  On set, p is ignored under EOS_flag 0 (it is c_s^2(R)*rho). */
+int pg_get_state(pg_handle* h, double* gas, double* dust);
+int pg_set_state(pg_handle* h, const double* gas, const double* dust);
+
+/* CFL step: dt = min(CFL / max signal rate, 1.1*dt_prev)
+   (global_dt, timestep.cu:113-140). */
+int pg_global_dt(pg_handle* h, double dt_prev, double* dt_out);
+/* One full step at time t with step dt (solve, solver.h:4 / solve.cu:279-292:
+   DS split order plus wave killing). */
+int pg_solve(pg_handle* h, double t, double dt);
+/* nsteps of the reference time loop (main.cu:320-359: dt from global_dt with
+   the 1.1x cap, clipped to end_time) entirely on the device; t and dt live
+   in device memory.  *t_io / *dt_io carry the time and the previous step in
+   and out; dts (nsteps doubles, may be NULL) receives every step size.
+   use_graph != 0 captures one step in a hipGraph and replays it. */
+int pg_run(pg_handle* h, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph);
+
+/* ---- module-level entry points (reference module boundaries) ---------- */
+/* viscous acceleration*rho of the current state (or of the predictor buffer
+   if on_T): viscosity_tensor_evaluation1/2 (viscosity.cu:489-549).
+   vis_out (2*xres*yres doubles, may be NULL) receives f_r, f_phi. */
+int pg_viscosity_eval(pg_handle* h, int on_T, double* vis_out);
+/* apply_source_terms(_inplace) (sweeps.cu:713-747) with the planet at
+   `t_planet`; inplace = 0 writes the predictor buffer. */
+int pg_sources(pg_handle* h, double t_planet, double mdt, double dt, int inplace);
+/* gas and dust directional sweeps (sweeps.cu:337-456, dust.cu:240-353) */
+int pg_sweep_x(pg_handle* h, double dt);
+int pg_sweep_y(pg_handle* h, double dt);
+int pg_sweep_x_dust(pg_handle* h, double dt);
+int pg_sweep_y_dust(pg_handle* h, double dt);
+/* set_OrbAdv + shift_OrbAdv + advecty(_dust) (orbit.cu:122-157,
+   advection.cu:321-357); inc/res (xres each, may be NULL) receive the
+   integer shift and residual speed per radius */
+int pg_orbital_advection(pg_handle* h, double dt, int32_t* inc, double* res);
+/* killwave (killwave.cu:105-117) */
+int pg_killwave(pg_handle* h, double dt);
+/* planet position and mass at time t (evolve_planet, planet.cu:293-306);
+   body10 = m,a,e,x,y,z,vx,vy,vz,rs (may be NULL) */
+int pg_planet(pg_handle* h, double t, double* body10);
+
+/* ---- introspection ----------------------------------------------------- */
+/* HIP stream the handle launches on (as a void*), for external capture. */
+int pg_stream(pg_handle* h, void** stream);
+/* Average device time (microseconds) per launch of each kernel over the last
+   pg_run call with profiling enabled; names are "sweep_x_gas", ... */
+int pg_profile_enable(pg_handle* h, int on);
+int pg_profile_get(pg_handle* h, const char* kernel, double* avg_us, int* count);
+/* Bytes of device memory owned by the handle. */
+int pg_device_bytes(pg_handle* h, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PENGUIN_AMD_H */

@@ -1,0 +1,259 @@
+"""ctypes binding of the FP64 host oracle (oracle/penguin_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: the checker.  Imported by tests/, by the smoke
+check in __graft_entry__.py and by bench.py's CPU-baseline leg; the product
+package penguin_amd never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from penguin_amd.params import Params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with g++/OpenMP (seconds)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "_build/liboracle.so"],
+                       cwd=_HERE, check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        pp = ctypes.POINTER(Params)
+        sig = {
+            "orc_create": (vp, [pp]),
+            "orc_destroy": (None, [vp]),
+            "orc_num_threads": (ctypes.c_int, []),
+            "orc_get_state": (None, [vp, _dp, _dp]),
+            "orc_set_state": (None, [vp, _dp, _dp]),
+            "orc_get_coords": (None, [vp, _dp, _dp, _dp, _dp]),
+            "orc_get_planet": (None, [vp, _dp]),
+            "orc_get_rot": (None, [vp, _dp]),
+            "orc_global_dt": (ctypes.c_double, [vp, ctypes.c_double]),
+            "orc_solve": (None, [vp, ctypes.c_double, ctypes.c_double]),
+            "orc_run": (None, [vp, ctypes.c_int, _dp, _dp, _dp]),
+            "orc_viscosity": (None, [vp, ctypes.c_int, _dp]),
+            "orc_sources": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+            "orc_sweepx": (None, [vp, ctypes.c_double]),
+            "orc_sweepy": (None, [vp, ctypes.c_double, ctypes.c_double]),
+            "orc_sweepx_dust": (None, [vp, ctypes.c_double]),
+            "orc_sweepy_dust": (None, [vp, ctypes.c_double, ctypes.c_double]),
+            "orc_orbital_advection": (None, [vp, ctypes.c_double]),
+            "orc_get_orbadv": (None, [vp, _ip, _dp]),
+            "orc_killwave": (None, [vp, ctypes.c_double]),
+            "orc_evolve_planet": (None, [vp, ctypes.c_double]),
+            "orc_fourth_parameters": (None, [ctypes.c_int, _dp, _dp, _dp]),
+            "orc_ppm_adjust": (None, [ctypes.c_double, _dp]),
+            "orc_ppm_aveL": (ctypes.c_double, [ctypes.c_double, _dp]),
+            "orc_ppm_aveR": (ctypes.c_double, [ctypes.c_double, _dp]),
+            "orc_flatten": (ctypes.c_double, [ctypes.c_double, _dp, _dp, _dp]),
+            "orc_set_state_row": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                         _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                         ctypes.c_double, _dp]),
+            "orc_riemann_hllc": (None, [ctypes.c_double, ctypes.c_int, _dp, _dp]),
+            "orc_riemann_row": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                       _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]),
+            "orc_dust_flux": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     ctypes.c_int, ctypes.c_double, _dp]),
+            "orc_orbit_solution": (None, [pp, ctypes.c_double, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def arr(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+class Oracle:
+    """One disk advanced by the FP64 host oracle."""
+
+    def __init__(self, p: Params):
+        self.p = p.copy()
+        self._h = lib().orc_create(ctypes.byref(self.p))
+        self.nx, self.ny = int(p.xres), int(p.yres)
+        self.t = 0.0
+        self.dt = float(p.end_time)   # main.cu:144: the first cap is end_time
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state: gas (5, ny, nx): rho, p, u, v, w; dust (4, ny, nx): rho, u, v, w
+    def get_state(self):
+        gas = np.zeros((5, self.ny, self.nx))
+        dust = np.zeros((4, self.ny, self.nx))
+        lib().orc_get_state(self._h, ptr(gas), ptr(dust))
+        return gas, dust
+
+    def set_state(self, gas=None, dust=None):
+        g = None if gas is None else arr(gas)
+        d = None if dust is None else arr(dust)
+        lib().orc_set_state(self._h, ptr(g) if g is not None else None,
+                            ptr(d) if d is not None else None)
+
+    def coords(self):
+        xa = np.zeros(self.nx + 7); xv = np.zeros(self.nx + 6)
+        ya = np.zeros(self.ny + 7); yv = np.zeros(self.ny + 6)
+        lib().orc_get_coords(self._h, ptr(xa), ptr(xv), ptr(ya), ptr(yv))
+        return xa, xv, ya, yv
+
+    def planet(self):
+        b = np.zeros(10)
+        lib().orc_get_planet(self._h, ptr(b))
+        return b
+
+    def rot(self):
+        r = np.zeros(self.nx + 6)
+        lib().orc_get_rot(self._h, ptr(r))
+        return r
+
+    def global_dt(self, dt_prev: float) -> float:
+        return lib().orc_global_dt(self._h, dt_prev)
+
+    def solve(self, t: float, dt: float):
+        lib().orc_solve(self._h, t, dt)
+
+    def run(self, nsteps: int) -> np.ndarray:
+        dts = np.zeros(nsteps)
+        t = ctypes.c_double(self.t)
+        dt = ctypes.c_double(self.dt)
+        lib().orc_run(self._h, nsteps, ctypes.byref(t), ctypes.byref(dt), ptr(dts))
+        self.t, self.dt = t.value, dt.value
+        return dts
+
+    def viscosity(self, on_T: bool = False) -> np.ndarray:
+        v = np.zeros((2, self.ny, self.nx))
+        lib().orc_viscosity(self._h, int(on_T), ptr(v))
+        return v
+
+    def sources(self, mdt: float, dt: float, inplace: bool = True):
+        lib().orc_sources(self._h, mdt, dt, int(inplace))
+
+    def sweepx(self, dt):
+        lib().orc_sweepx(self._h, dt)
+
+    def sweepy(self, t, dt):
+        lib().orc_sweepy(self._h, t, dt)
+
+    def sweepx_dust(self, dt):
+        lib().orc_sweepx_dust(self._h, dt)
+
+    def sweepy_dust(self, t, dt):
+        lib().orc_sweepy_dust(self._h, t, dt)
+
+    def orbital_advection(self, dt):
+        lib().orc_orbital_advection(self._h, dt)
+
+    def orbadv(self):
+        inc = np.zeros(self.nx, dtype=np.int32)
+        res = np.zeros(self.nx)
+        lib().orc_get_orbadv(self._h, inc.ctypes.data_as(_ip), ptr(res))
+        return inc, res
+
+    def killwave(self, dt):
+        lib().orc_killwave(self._h, dt)
+
+    def evolve_planet(self, t):
+        lib().orc_evolve_planet(self._h, t)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# ----------------------------------------------------------- unit functions
+def fourth_parameters(geom, x6, a5):
+    par = np.zeros(3)
+    lib().orc_fourth_parameters(geom, ptr(arr(x6)), ptr(arr(a5)), ptr(par))
+    return par
+
+
+def ppm_adjust(flat, par3):
+    par = arr(par3).copy()
+    lib().orc_ppm_adjust(flat, ptr(par))
+    return par
+
+
+def ppm_aveL(q, par3):
+    return lib().orc_ppm_aveL(q, ptr(arr(par3)))
+
+
+def ppm_aveR(q, par3):
+    return lib().orc_ppm_aveR(q, ptr(arr(par3)))
+
+
+def flatten(gam, r5, p5, u5):
+    return lib().orc_flatten(gam, ptr(arr(r5)), ptr(arr(p5)), ptr(arr(u5)))
+
+
+def set_state_row(geom, gam, xa, r, p, u, v, w, flat, i, dt):
+    n = len(r)
+    S = np.zeros(10)
+    xa, r, p, u, v, w, flat = map(arr, (xa, r, p, u, v, w, flat))
+    lib().orc_set_state_row(geom, gam, n, ptr(xa), None, ptr(r), ptr(p), ptr(u), ptr(v), ptr(w),
+                            ptr(flat), i, dt, ptr(S))
+    return S
+
+
+def riemann_hllc(gam, S10, eos=0):
+    out = np.zeros(10)
+    lib().orc_riemann_hllc(gam, eos, ptr(arr(S10)), ptr(out))
+    return out
+
+
+def riemann_row(geom, gam, xa, r, p, u, v, w, dt):
+    n = len(r)
+    D = np.zeros((5, n))
+    xa, r, p, u, v, w = map(arr, (xa, r, p, u, v, w))
+    lib().orc_riemann_row(geom, gam, n, ptr(xa), None, ptr(r), ptr(p), ptr(u), ptr(v), ptr(w),
+                          dt, ptr(D))
+    return D
+
+
+def dust_flux(geom, xa, r, u, v, w, i, dt):
+    n = len(r)
+    F = np.zeros(4)
+    xa, r, u, v, w = map(arr, (xa, r, u, v, w))
+    lib().orc_dust_flux(geom, n, ptr(xa), None, ptr(r), ptr(u), ptr(v), ptr(w), i, dt, ptr(F))
+    return F
+
+
+def orbit_solution(p: Params, t: float) -> np.ndarray:
+    b = np.zeros(10)
+    lib().orc_orbit_solution(ctypes.byref(p), t, ptr(b))
+    return b

@@ -1,0 +1,1416 @@
+// penguin_oracle.cpp -- FP64 host restatement of the reference solver's step.
+//
+// TEST INFRASTRUCTURE ONLY (the checker).  Nothing in the product library
+// links, loads or calls this file; tests/, the smoke check and bench.py's
+// CPU-baseline leg load it through ctypes.
+//
+// What it restates, and from where (reference = fakesuji/PEnGUIn):
+//   * grid and profiles        src/geom/geom.cu:3-33, src/init/init.cu:7-532
+//   * reconstruction           src/solver/recon/recon.cu:216-501, PPM.cu:7-113
+//   * traced states + Riemann  src/solver/riemann/states.cu:133-267,
+//                              wave_speeds.cu:24-35,288-333, HLLC.cu:2-77,
+//                              riemann.cu:5-131
+//   * sweeps + update          src/solver/sweeps.cu:218-456
+//   * dust                     src/solver/dust/dust.cu:1-353
+//   * orbital advection        src/orbit/orbit.cu:17-157,
+//                              src/solver/advection/advection.cu:1-357
+//   * viscosity                src/viscosity/viscosity.cu:9-197,489-549
+//   * source terms + forces    src/solver/sweeps.cu:555-747,
+//                              src/solver/force/forces.cu:41-58,110-327
+//   * CFL                      src/timestep/timestep.cu:18-140
+//   * ghost fills              src/boundary/boundary.cu:13-268,462-848
+//   * planet                   src/planet/planet.cu:7-187,293-317
+//   * wave killing             src/killwave/killwave.cu:8-117
+//   * split order              src/solver/solve.cu:212-292, main.cu:320-359
+//
+// The reference's block-cooperative device routines (flatten, riemann,
+// net_source, net_flux: shared memory + __syncthreads) are executed here as
+// ordered phases over a whole row, which is what the barriers enforce.  Rows
+// are independent and run under OpenMP.
+//
+// Scope: ndim = 2 (geomx = 1 cylindrical radius, geomy = 3 azimuth),
+// init_flag 2 disk, recon_flag 6 (PPM4), EOS_flag 0 (locally isothermal),
+// twobd_flag 0, cool_flag 0.  Resolutions of the reference's ambiguities:
+//   * dust sweeps and dust advection read the flattening coefficient from
+//     uninitialised shared memory under recon_flag 6 (dust.cu:116-122,
+//     advection.cu:186-192 vs PPM.cu:69-78); here it is 0, the convention the
+//     gas advection uses (advection.cu:12-18);
+//   * killwave's __sinf (killwave.cu:11) is sinf on a float argument.
+
+#include "penguin_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double kPi = 3.1415926535897932384626434;
+const double kHalfPi = 0.5 * kPi;
+const double kTwoPi = 2.0 * kPi;
+const double kSmallR = 1.0e-10;
+const double kSmallP = 1.0e-10;
+const int kPad = 3;  // npad for recon_flag > 4 (parameters.h:49-53)
+
+struct Cell { double r, p, u, v, w; };
+struct Dust { double r, u, v, w; };
+struct Body { double m, a, e, x, y, z, vx, vy, vz, rs; };
+
+// ---------------------------------------------------------------- profiles
+// init.cu:7-50, 72-349 (init_flag 2, geomx 1, ndim 2), 351-454.
+
+double cs2_at(const pg_params& P, double x) {
+    double coeff = P.sc_h * P.sc_h * (1.0 / P.planet_radius);
+    return coeff * std::pow(x / P.planet_radius, -P.p_beta);
+}
+double h_at(const pg_params& P, double x) { return std::sqrt(cs2_at(P, x) * std::pow(x, 3)); }
+double nu_at(const pg_params& P, double x) {
+    if (!P.visc_flag) return 0.0;
+    double rp = P.planet_radius;
+    double coeff = P.ss_alpha * std::sqrt(cs2_at(P, rp)) * h_at(P, rp);
+    return coeff * std::pow(x / rp, P.p_alpha);
+}
+double rho_init(const pg_params& P, double x) { return std::pow(x / P.planet_radius, -P.p_alpha); }
+double p_init(const pg_params& P, double x) { return cs2_at(P, x) * rho_init(P, x); }
+double u_init(const pg_params& P, double x) { return -1.5 * nu_at(P, x) / x; }
+double v_init(const pg_params& P, double x) {
+    double rho = rho_init(P, x);
+    double dr = 1.1514178e-7;
+    double dPdr = (p_init(P, x + dr) - p_init(P, x - dr)) / (2.0 * dr);
+    return std::sqrt(1.0 / x + (x / rho) * dPdr);
+}
+Cell cell_init(const pg_params& P, double x) {
+    return Cell{rho_init(P, x), p_init(P, x), u_init(P, x), v_init(P, x), 0.0};
+}
+Dust dust_init(const pg_params& P, double x) {
+    double St = P.Stokes;
+    double f_gas = v_init(P, x) / std::sqrt(1.0 / x);
+    double tmp = St * St * f_gas;
+    double L = ((1.0 + tmp) / tmp) *
+               (1.0 - std::sqrt(1.0 - 2.0 * tmp * (1.0 - f_gas) / (1.0 + tmp) / (1.0 + tmp)));
+    double u0 = St * (L * L - 2.0 * L);
+    double u = (1.0 - std::sqrt(1.0 - 2.0 * St * u0)) / St;
+    L = (1.0 - f_gas + St * u / 2.0) / (1.0 + St * u / 2.0);
+    u0 = St * (L * L - 2.0 * L);
+    u = (1.0 - std::sqrt(1.0 - 2.0 * St * u0)) / St;
+    Dust d;
+    d.r = P.D_G_ratio * std::pow(x / P.planet_radius, -P.p_alpha);
+    d.u = u * std::sqrt(1.0 / x);
+    d.v = std::sqrt(1.0 / x);
+    d.w = 0.0;
+    return d;
+}
+
+// ---------------------------------------------------------------- geometry
+// geom.cu:3-33, init.cu:472-500
+
+double dv_dr(int geom, double ra, double dr) {
+    if (geom == 5) return dr < 1.0e-4 ? std::sin(ra + 0.5 * dr) : (std::cos(ra) - std::cos(ra + dr)) / dr;
+    if (geom == 2) return ra * ra + ra * dr + dr * dr / 3.0;
+    if (geom == 1) return ra + dr / 2.0;
+    return 1.0;
+}
+double cell_volume(int geom, double ra, double dr) { return dr * dv_dr(geom, ra, dr); }
+
+void build_axis(std::vector<double>& a, std::vector<double>& v, double amin, double amax,
+                int res, int pad, int geom, int grid) {
+    a.assign(res + 2 * pad + 1, 0.0);
+    v.assign(res + 2 * pad, 0.0);
+    if (grid == 1) {
+        double t = std::log(amax / amin);
+        for (int i = 0; i <= res; ++i) a[pad + i] = amin * std::exp(t * (double)i / (double)res);
+    } else {
+        for (int i = 0; i <= res; ++i) a[pad + i] = amin + (amax - amin) * (double)i / (double)res;
+    }
+    for (int i = 0; i < pad; ++i) {
+        if (grid == 1) {
+            a[i] = amin * std::exp(std::log(amax / amin) * (double)(i - pad) / (double)res);
+            a[res + pad + 1 + i] = amin * std::exp(std::log(amax / amin) * (double)(res + 1 + i) / (double)res);
+        } else {
+            a[i] = a[pad] - (a[2 * pad - i] - a[pad]);
+            a[res + pad + 1 + i] = a[res + pad] + (a[res + pad] - a[res + pad - 1 - i]);
+        }
+    }
+    for (int i = 0; i < res + 2 * pad; ++i) v[i] = cell_volume(geom, a[i], a[i + 1] - a[i]);
+}
+
+// ---------------------------------------------------------- reconstruction
+inline double lim01(double a) { return std::fmin(std::fmax(a, 0.0), 1.0); }  // solve.cu:14-17
+inline double exp_lim(double x) { return std::fmax(1.0 + x, 1.0e-6); }       // solve.cu:29-34
+
+// Cell means of x, x^2, x^3 under the geometric weight (1, x, x^2 for
+// geom 0/3/4, 1, 2), recon.cu:242-301.
+inline void cell_moments(int geom, double xl, double xr, double m[3]) {
+    if (geom == 2) {
+        double d = xr * xr * xr - xl * xl * xl;
+        m[0] = (xr * xr * xr * xr - xl * xl * xl * xl) / d / (4.0 / 3.0);
+        m[1] = (xr * xr * xr * xr * xr - xl * xl * xl * xl * xl) / d / (5.0 / 3.0);
+        m[2] = (xr * xr * xr * xr * xr * xr - xl * xl * xl * xl * xl * xl) / d / 2.0;
+    } else if (geom == 1) {
+        double d = xr * xr - xl * xl;
+        m[0] = (xr * xr * xr - xl * xl * xl) / d / 1.5;
+        m[1] = (xr * xr * xr * xr - xl * xl * xl * xl) / d / 2.0;
+        m[2] = (xr * xr * xr * xr * xr - xl * xl * xl * xl * xl) / d / 2.5;
+    } else {
+        double d = xr - xl;
+        m[0] = (xr * xr - xl * xl) / d / 2.0;
+        m[1] = (xr * xr * xr - xl * xl * xl) / d / 3.0;
+        m[2] = (xr * xr * xr * xr - xl * xl * xl * xl) / d / 4.0;
+    }
+}
+
+// Coefficients c1..c3 of the cubic whose cell means over four consecutive
+// cells (moments M[0..3]) differ by d[0..2] (recon.cu:305-328).
+inline void cubic_from_means(const double M[4][3], const double d[3], double c[3]) {
+    double D0[3], D1[3], D2[3];
+    for (int k = 0; k < 3; ++k) {
+        D0[k] = M[1][k] - M[0][k];
+        D1[k] = M[2][k] - M[1][k];
+        D2[k] = M[3][k] - M[2][k];
+    }
+    // D?2 -> index 0, D?3 -> index 1, D?4 -> index 2
+    double den_a = D2[1] * D0[0] - D0[1] * D2[0];
+    double den_b = D1[1] * D0[0] - D0[1] * D1[0];
+    double c3 = (D0[0] * d[2] - D2[0] * d[0]) / den_a - (d[1] * D0[0] - d[0] * D1[0]) / den_b;
+    double t = (D0[0] * D2[2] - D2[0] * D0[2]) / den_a - (D1[2] * D0[0] - D0[2] * D1[0]) / den_b;
+    c3 /= t;
+    t = (d[1] * D0[0] - d[0] * D1[0]) / den_b;
+    double c2 = t - c3 * (D1[2] * D0[0] - D0[2] * D1[0]) / den_b;
+    t = d[0] / D0[0];
+    double c1 = t - c2 * D0[1] / D0[0] - c3 * D0[2] / D0[0];
+    c[0] = c1; c[1] = c2; c[2] = c3;
+}
+
+// get_4th_parameters (recon.cu:216-378): par = {sL, <a>, sR} of cell i, with
+// x[i-2..i+3] the edges and a[i-2..i+2] the cell means.
+void fourth_parameters(int i, int geom, const double* x, const double* a, double par[3]) {
+    double M[5][3];
+    for (int c = 0; c < 5; ++c) cell_moments(geom, x[i - 2 + c], x[i - 1 + c], M[c]);
+    double a0 = a[i - 2], a1 = a[i - 1], a2 = a[i], a3 = a[i + 1], a4 = a[i + 2];
+    double x2 = x[i], x3 = x[i + 1];
+    double c[3];
+    {
+        double Mw[4][3];
+        for (int m = 0; m < 4; ++m) for (int k = 0; k < 3; ++k) Mw[m][k] = M[m][k];
+        double d[3] = {a1 - a0, a2 - a1, a3 - a2};
+        cubic_from_means(Mw, d, c);
+    }
+    double sL = c[0] * (M[2][0] - x2) + c[1] * (M[2][1] - x2 * x2) + c[2] * (M[2][2] - x2 * x2 * x2);
+    {
+        double Mw[4][3];
+        for (int m = 0; m < 4; ++m) for (int k = 0; k < 3; ++k) Mw[m][k] = M[m + 1][k];
+        double d[3] = {a2 - a1, a3 - a2, a4 - a3};
+        cubic_from_means(Mw, d, c);
+    }
+    double sR = c[0] * (x3 - M[2][0]) + c[1] * (x3 * x3 - M[2][1]) + c[2] * (x3 * x3 * x3 - M[2][2]);
+    sL = std::copysign(std::fmin(std::fabs(sL), std::fabs(a2 - a1)), a2 - a1);
+    sR = std::copysign(std::fmin(std::fabs(sR), std::fabs(a3 - a2)), a3 - a2);
+    if (sR * sL <= 0.0) { par[0] = 0.0; par[1] = a2; par[2] = 0.0; }
+    else                { par[0] = sL;  par[1] = a2; par[2] = sR; }
+}
+
+// adjust_PPM_parameters (PPM.cu:64-91)
+void ppm_adjust(double flat, double par[3]) {
+    double sL = par[0], sR = par[2];
+    sR *= (1.0 - flat);
+    sL *= (1.0 - flat);
+    double t3 = sR + sL;
+    double t2 = t3 * t3;
+    double t1 = t3 * 3.0 * (sL - sR);
+    if (t2 < t1) sL = 2.0 * sR;
+    else if (t2 < -t1) sR = 2.0 * sL;
+    par[0] = sL;
+    par[2] = sR;
+}
+
+// get_PPM_aveR / aveL (PPM.cu:95-113)
+inline double ppm_aveR(double x, const double* par) {
+    double a6 = 3.0 * (par[0] - par[2]);
+    double aR = par[1] + par[2];
+    double S = par[2] + par[0];
+    x = 1.0 - x;
+    return aR - (x / 2.0) * (S - (1.0 - x / 1.5) * a6);
+}
+inline double ppm_aveL(double x, const double* par) {
+    double a6 = 3.0 * (par[0] - par[2]);
+    double aL = par[1] - par[0];
+    double S = par[2] + par[0];
+    return aL + (x / 2.0) * (S + (1.0 - x / 1.5) * a6);
+}
+
+// recon_flag 6: 4th-order parameters, then flattening + PPM monotonicity
+inline void recon_par(int i, int geom, const double* x, const double* a, const double* flat, double par[3]) {
+    fourth_parameters(i, geom, x, a, par);
+    ppm_adjust(flat[i], par);
+}
+
+// flatten (PPM.cu:7-62) for cell i of a row of n cells
+double flatten_cell(int i, int n, double gam, const double* r, const double* p, const double* u) {
+    double f = 0.0;
+    if (i >= 2 && i < n - 2) {
+        double pb = p[i - 1], p0 = p[i], pf = p[i + 1];
+        double cs = std::sqrt(gam * p0 / r[i]);
+        double dp1f = pf - p0;
+        double dp2f = p[i + 2] - pb;
+        double dp1b = p0 - pb;
+        double dp2b = pf - p[i - 2];
+        double Mf = (u[i] - u[i + 1]) / cs;
+        double Mb = (u[i - 1] - u[i]) / cs;
+        if (Mf > 0.0 || Mb > 0.0) {
+            if (dp2f != 0.0 && dp2b != 0.0) {
+                double ff = 10.0 * std::fmax(0.0, std::fabs(dp1f / dp2f - 0.5) - 0.25);
+                double fb = 10.0 * std::fmax(0.0, std::fabs(dp1b / dp2b - 0.5) - 0.25);
+                f = std::fmax(ff, fb);
+                f = std::fmin(0.5, f * std::fmax(std::fabs(dp1f), std::fabs(dp1b)) / p0);
+            }
+        }
+    }
+    return f;
+}
+
+// ------------------------------------------------------------ Riemann step
+struct State { double rl, pl, ul, vl, wl, rr, pr, ur, vr, wr; };
+
+// set_state (states.cu:133-267) for interface i (between cells i-1 and i)
+void trace_state(int i, int geom, double gam, const double* xa, const double* r, const double* p,
+                 const double* u, const double* v, const double* w, const double* flat,
+                 double dt, double us, State& S) {
+    double rp[3], pp[3], up[3], vp[3], wp[3];
+    double hdt = 0.5 * dt;
+    // right state: cell i, swept region adjacent to its left edge
+    recon_par(i, geom, xa, r, flat, rp);
+    recon_par(i, geom, xa, p, flat, pp);
+    recon_par(i, geom, xa, u, flat, up);
+    double xl = xa[i], xr = xa[i + 1];
+    double cs = std::sqrt(gam * p[i] / r[i]);
+    double tmp = xl - std::fmin(u[i], 0.0) * dt + cs * dt;
+    double q = lim01((tmp - xl) / (xr - xl));
+    double rmin = r[i] * 1.0e-10, pmin = p[i] * 1.0e-10;
+    double ur = ppm_aveL(q, up);
+    double dis = std::fmin(ur, 0.0) * dt;
+    tmp = xl - dis;
+    q = lim01((tmp - xl) / (xr - xl));
+    double r0 = ppm_aveL(q, rp), p0 = ppm_aveL(q, pp), u0 = ppm_aveL(q, up);
+    r0 = std::fmax(r0, rmin);
+    p0 = std::fmax(p0, pmin);
+    cs = std::sqrt(gam * p0 / r0);
+    tmp = xl - dis + cs * dt;
+    q = lim01((tmp - xl) / (xr - xl));
+    double u_ = ppm_aveL(q, up), p_ = ppm_aveL(q, pp);
+    u_ = 2.0 * (u_ - u0) / (cs * dt);
+    p_ = 2.0 * (p_ - p0) / (cs * dt);
+    u_ *= -hdt;
+    p_ *= -hdt;
+    S.rr = r0 * exp_lim(u_);
+    S.pr = p0 * exp_lim(gam * u_);
+    S.ur = u0 + p_ / r0 + us;
+    recon_par(i, geom, xa, v, flat, vp);
+    recon_par(i, geom, xa, w, flat, wp);
+    S.vr = ppm_aveL(q, vp);
+    S.wr = ppm_aveL(q, wp);
+    // left state: cell i-1, swept region adjacent to its right edge
+    recon_par(i - 1, geom, xa, r, flat, rp);
+    recon_par(i - 1, geom, xa, p, flat, pp);
+    recon_par(i - 1, geom, xa, u, flat, up);
+    xl = xa[i - 1];
+    xr = xa[i];
+    cs = std::sqrt(gam * p[i - 1] / r[i - 1]);
+    tmp = xr - std::fmax(u[i - 1], 0.0) * dt - cs * dt;
+    q = lim01((tmp - xl) / (xr - xl));
+    rmin = r[i - 1] * 1.0e-10;
+    pmin = p[i - 1] * 1.0e-10;
+    double ul = ppm_aveR(q, up);
+    dis = std::fmax(ul, 0.0) * dt;
+    tmp = xr - dis;
+    q = lim01((tmp - xl) / (xr - xl));
+    r0 = ppm_aveR(q, rp);
+    p0 = ppm_aveR(q, pp);
+    u0 = ppm_aveR(q, up);
+    r0 = std::fmax(r0, rmin);
+    p0 = std::fmax(p0, pmin);
+    cs = std::sqrt(gam * p0 / r0);
+    tmp = xr - dis - cs * dt;
+    q = lim01((tmp - xl) / (xr - xl));
+    u_ = ppm_aveR(q, up);
+    p_ = ppm_aveR(q, pp);
+    u_ = 2.0 * (u0 - u_) / (cs * dt);
+    p_ = 2.0 * (p0 - p_) / (cs * dt);
+    u_ *= -hdt;
+    p_ *= -hdt;
+    S.rl = r0 * exp_lim(u_);
+    S.pl = p0 * exp_lim(gam * u_);
+    S.ul = u0 + p_ / r0 + us;
+    recon_par(i - 1, geom, xa, v, flat, vp);
+    recon_par(i - 1, geom, xa, w, flat, wp);
+    S.vl = ppm_aveR(q, vp);
+    S.wl = ppm_aveR(q, wp);
+}
+
+// get_pm_simple, get_lr_speeds_Einfeldt, get_sm (wave_speeds.cu:24-35,288-315)
+void wave_speeds(const State& S, double gam, double us, double& pm, double& sl, double& sm, double& sr) {
+    double cl = std::sqrt(gam * S.pl / S.rl);
+    double cr = std::sqrt(gam * S.pr / S.rr);
+    double c_ = 0.5 * (cr + cl);
+    double p_ = 0.5 * (S.pr + S.pl);
+    pm = p_ * exp_lim(0.5 * gam * (S.ul - S.ur) / c_);
+    double srl = std::sqrt(S.rl), srr = std::sqrt(S.rr);
+    double ubar = (srl * S.ul + srr * S.ur) / (srl + srr);
+    double n2 = 0.5 * srl * srr / ((srl + srr) * (srl + srr));
+    double cbar = std::sqrt((cl * cl * srl + cr * cr * srr) / (srl + srr) + n2 * (S.ur - S.ul) * (S.ur - S.ul));
+    sl = ubar - cbar;
+    sr = ubar + cbar;
+    sm = (S.rl * S.ul * cl + S.rr * S.ur * cr) / (S.rl * cl + S.rr * cr);
+    sm += (S.pl - S.pr) / (S.rl + S.rr) / (0.5 * (cl + cr));
+    sm += us * (std::fmax(S.ul, 0.0) + std::fmax(-S.ur, 0.0)) / (cl + cr);
+}
+
+// upwind selection (HLLC.cu:2-8)
+inline double pick(double FL, double F_L, double F_R, double FR, double sl, double sm, double sr) {
+    if (!std::signbit(sl)) return FL;
+    if (std::signbit(sr)) return FR;
+    if (!std::signbit(sm)) return F_L;
+    return F_R;
+}
+
+double energy(int eos, double gam, double r, double p, double u, double v, double w) {  // EOS.cu:4-18
+    (void)u; (void)v; (void)w;
+    if (eos == 2) return p / ((gam - 1.0) * r);  // internal_e_flag 1
+    if (eos == 1) return p / std::pow(r, gam);
+    return p / r;
+}
+
+// HLLC_fluxes (HLLC.cu:10-77), internal_e_flag 1 for EOS_flag 2
+void hllc(const State& S, int eos, double gam, double pm, double sl, double sm, double sr,
+          Cell& F, double& pres, double& uprs) {
+    double fl = S.ul * S.rl;
+    double f_l = sm * S.rl * (sl - S.ul) / (sl - sm);
+    double f_r = sm * S.rr * (sr - S.ur) / (sr - sm);
+    double fr = S.ur * S.rr;
+    F.r = pick(fl, f_l, f_r, fr, sl, sm, sr);
+    F.u = pick(S.ul, S.ul, S.ur, S.ur, sl, sm, sr) * F.r;
+    F.v = pick(S.vl, S.vl, S.vr, S.vr, sl, sm, sr) * F.r;
+    F.w = pick(S.wl, S.wl, S.wr, S.wr, sl, sm, sr) * F.r;
+    if (eos > 0) {
+        double el = energy(eos, gam, S.rl, S.pl, S.ul, 0.0, 0.0);
+        double er = energy(eos, gam, S.rr, S.pr, S.ur, 0.0, 0.0);
+        F.p = pick(el, el, er, er, sl, sm, sr) * F.r;
+    } else {
+        F.p = 0.0;
+    }
+    pres = pick(S.pl, (sl * pm - sm * S.pl) / (sl - sm), (sr * pm - sm * S.pr) / (sr - sm), S.pr, sl, sm, sr);
+    if (eos == 2) {
+        uprs = pick(S.ul, sm * (sl * pm - S.ul * S.pl) / (sl * pm - sm * S.pl),
+                    sm * (sr * pm - S.ur * S.pr) / (sr * pm - sm * S.pr), S.ur, sl, sm, sr);
+    } else {
+        uprs = 0.0;
+    }
+}
+
+inline double geom_factor(int geom, double x) {  // eval_gfac (riemann.cu:5-11)
+    if (geom == 1) return x;
+    if (geom == 2) return x * x;
+    if (geom == 5) return std::sin(x);
+    return 1.0;
+}
+
+struct RowScratch {
+    std::vector<double> r, p, u, v, w, flat, pres, uprs, gfac;
+    std::vector<Cell> F, Del;
+    std::vector<Dust> FD, DelD;
+    void resize(int n) {
+        r.resize(n); p.resize(n); u.resize(n); v.resize(n); w.resize(n);
+        flat.resize(n); pres.resize(n + 1); uprs.resize(n + 1); gfac.resize(n + 1);
+        F.resize(n + 1); Del.resize(n); FD.resize(n + 1); DelD.resize(n);
+    }
+};
+
+// riemann() + net_source() over a whole row of n cells (riemann.cu:40-131).
+// Del[i] is valid for cells i in [kPad, n-kPad).
+void riemann_row(int geom, int eos, double gam, int n, const double* xa, RowScratch& s,
+                 double dt_state) {
+    for (int i = 0; i < n; ++i) s.flat[i] = flatten_cell(i, n, gam, s.r.data(), s.p.data(), s.u.data());
+    for (int i = kPad; i < n + 1 - kPad; ++i) {
+        State S;
+        trace_state(i, geom, gam, xa, s.r.data(), s.p.data(), s.u.data(), s.v.data(), s.w.data(),
+                    s.flat.data(), dt_state, 0.0, S);
+        double pm, sl, sm, sr;
+        wave_speeds(S, gam, 0.0, pm, sl, sm, sr);
+        hllc(S, eos, gam, pm, sl, sm, sr, s.F[i], s.pres[i], s.uprs[i]);
+        s.gfac[i] = geom_factor(geom, xa[i]);
+    }
+    for (int i = kPad; i < n - kPad; ++i) {
+        double g0 = s.gfac[i], g1 = s.gfac[i + 1];
+        double du = 0.5 * (g0 + g1) * (s.pres[i] - s.pres[i + 1]);
+        double de = 0.0;
+        if (eos == 2) de = 0.5 * (s.pres[i] + s.pres[i + 1]) * (s.uprs[i] * g0 - s.uprs[i + 1] * g1);
+        Cell& D = s.Del[i];
+        D.r = s.F[i].r * g0 - s.F[i + 1].r * g1;
+        D.p = s.F[i].p * g0 - s.F[i + 1].p * g1 + de;
+        D.u = s.F[i].u * g0 - s.F[i + 1].u * g1 + du;
+        D.v = s.F[i].v * g0 - s.F[i + 1].v * g1;
+        D.w = s.F[i].w * g0 - s.F[i + 1].w * g1;
+    }
+}
+
+// ------------------------------------------------------------------ dust
+// dust_flux (dust.cu:6-105), flattening coefficient 0
+Dust dust_interface(int i, int geom, const double* xa, const double* r, const double* u,
+                    const double* v, const double* w, const double* flat, double dt, double us) {
+    State S;
+    double rp[3], upar[3];
+    double xl = xa[i], xr = xa[i + 1];
+    recon_par(i, geom, xa, u, flat, upar);
+    double tmp = xl - std::fmin(upar[1] - upar[0], 0.0) * dt;
+    double q = lim01((tmp - xl) / (xr - xl));
+    double ur = ppm_aveL(q, upar);
+    double u_ = 0.5 * dt * (ur - u[i]) / (0.5 * (xr - tmp));
+    tmp = xl - std::fmin(ur, 0.0) * dt;
+    q = lim01((tmp - xl) / (xr - xl));
+    recon_par(i, geom, xa, r, flat, rp);
+    double r0 = ppm_aveL(q, rp);
+    double u0 = ppm_aveL(q, upar);
+    S.rr = r0 * exp_lim(u_);
+    S.ur = u0 + us;
+    recon_par(i, geom, xa, v, flat, rp);
+    recon_par(i, geom, xa, w, flat, upar);
+    S.vr = ppm_aveL(q, rp);
+    S.wr = ppm_aveL(q, upar);
+
+    xl = xa[i - 1];
+    xr = xa[i];
+    recon_par(i - 1, geom, xa, u, flat, upar);
+    tmp = xr - std::fmax(upar[1] + upar[2], 0.0) * dt;
+    q = lim01((tmp - xl) / (xr - xl));
+    double ul = ppm_aveR(q, upar);
+    u_ = 0.5 * dt * (u[i - 1] - ul) / (0.5 * (tmp - xl));
+    tmp = xr - std::fmax(ul, 0.0) * dt;
+    q = lim01((tmp - xl) / (xr - xl));
+    recon_par(i - 1, geom, xa, r, flat, rp);
+    r0 = ppm_aveR(q, rp);
+    u0 = ppm_aveR(q, upar);
+    S.rl = r0 * exp_lim(u_);
+    S.ul = u0 + us;
+    recon_par(i - 1, geom, xa, v, flat, rp);
+    recon_par(i - 1, geom, xa, w, flat, upar);
+    S.vl = ppm_aveR(q, rp);
+    S.wl = ppm_aveR(q, upar);
+
+    S.rl = std::fmin(r[i - 1] * dt / (xa[i] - xa[i - 1]), std::fmax(S.ul * S.rl, 0.0));
+    S.rr = std::fmax(r[i] * dt / (xa[i] - xa[i + 1]), std::fmin(S.ur * S.rr, 0.0));
+    Dust F;
+    F.r = S.rl + S.rr;
+    F.u = S.rl * S.ul + S.rr * S.ur;
+    F.v = S.rl * S.vl + S.rr * S.vr;
+    F.w = S.rl * S.wl + S.rr * S.wr;
+    return F;
+}
+
+// dust_flow (dust.cu:107-147) over a row; rows hold r,u,v,w in s.r,s.u,s.v,s.w
+void dust_row(int geom, int n, const double* xa, RowScratch& s, double dt_flux) {
+    std::fill(s.flat.begin(), s.flat.begin() + n, 0.0);
+    for (int i = kPad; i < n + 1 - kPad; ++i) {
+        s.FD[i] = dust_interface(i, geom, xa, s.r.data(), s.u.data(), s.v.data(), s.w.data(),
+                                 s.flat.data(), dt_flux, 0.0);
+        s.gfac[i] = geom_factor(geom, xa[i]);
+    }
+    for (int i = kPad; i < n - kPad; ++i) {
+        double g0 = s.gfac[i], g1 = s.gfac[i + 1];
+        Dust& D = s.DelD[i];
+        D.r = s.FD[i].r * g0 - s.FD[i + 1].r * g1;
+        D.u = s.FD[i].u * g0 - s.FD[i + 1].u * g1;
+        D.v = s.FD[i].v * g0 - s.FD[i + 1].v * g1;
+        D.w = s.FD[i].w * g0 - s.FD[i + 1].w * g1;
+    }
+}
+
+// ---------------------------------------------------------- cell updates
+Cell update_cell(const pg_params& P, double x, Cell Q, const Cell& D) {  // sweeps.cu:218-261
+    double r_min = std::fmax(Q.r * 1.0e-10, kSmallR);
+    double p_min = std::fmax(Q.p * 1.0e-10, kSmallP);
+    double uo = Q.u, vo = Q.v, wo = Q.w;
+    Q.p = energy(P.EOS_flag, P.gam, Q.r, Q.p, Q.u, Q.v, Q.w);
+    Q.p *= Q.r;
+    Q.u *= Q.r; Q.v *= Q.r; Q.w *= Q.r;
+    Q.r += D.r; Q.p += D.p; Q.u += D.u; Q.v += D.v; Q.w += D.w;
+    Q.u /= Q.r; Q.v /= Q.r; Q.w /= Q.r;
+    if (Q.r < r_min) {
+        Q.r = r_min; Q.p = p_min; Q.u = uo; Q.v = vo; Q.w = wo;
+    } else if (P.EOS_flag == 2) {
+        Q.p = std::fmax(Q.p * (P.gam - 1.0), p_min);
+    } else if (P.EOS_flag == 0) {
+        Q.p = cs2_at(P, x) * Q.r;
+    }
+    return Q;
+}
+
+Cell update_cell_advect(const pg_params& P, double x, Cell Q, const Cell& D) {  // advection.cu:88-105
+    Q.u *= Q.r; Q.v *= Q.r; Q.w *= Q.r;
+    Q.r += D.r; Q.p += D.p; Q.u += D.u; Q.v += D.v; Q.w += D.w;
+    Q.u /= Q.r; Q.v /= Q.r; Q.w /= Q.r;
+    if (P.EOS_flag == 0) Q.p = cs2_at(P, x) * Q.r;
+    return Q;
+}
+
+Dust update_dust(const pg_params& P, Dust Q, const Dust& D) {  // dust.cu:149-176
+    double r_min = std::fmax(Q.r * 1.0e-10, P.D_G_ratio * kSmallR);
+    double uo = Q.u, vo = Q.v, wo = Q.w;
+    Q.u *= Q.r; Q.v *= Q.r; Q.w *= Q.r;
+    Q.r += D.r; Q.u += D.u; Q.v += D.v; Q.w += D.w;
+    Q.u /= Q.r; Q.v /= Q.r; Q.w /= Q.r;
+    if (Q.r < r_min) { Q.r = r_min; Q.u = uo; Q.v = vo; Q.w = wo; }
+    return Q;
+}
+
+// ----------------------------------------------------------------- planet
+double ramp(double t, double tau, double val) { return t < tau ? val * std::sin(kHalfPi * t / tau) : val; }
+
+double ecc_anomaly(double n, double e, double t) {  // planet.cu:60-103
+    double l = std::fmod(n * t, kTwoPi);
+    double D = -e * std::sin(l);
+    if (e == 0.0 || l == 0.0 || l == kPi || l == kTwoPi) return l;
+    double tor = 1.0e-12, step;
+    int count = 0;
+    do {
+        double es = std::sin(l - D), ec = std::cos(l - D);
+        es *= e; ec *= e;
+        double x = D + es, dx = 1.0 - ec, d2x = -es;
+        double eps = 2.0 * x * d2x / dx / dx;
+        if (std::fabs(eps) < 1.0) step = -(x / dx) * (1.0 + eps / 4.0 + eps * eps / 8.0 + 5.0 * eps * eps * eps / 64.0);
+        else step = std::copysign(std::fmin(std::fabs(x / dx), kHalfPi / 2.0), -x);
+        D += step;
+        count++;
+    } while (std::fabs(step) > tor || count < 10);
+    return l - D;
+}
+
+void orbit_solution(const pg_params& P, Body& b, double t) {  // planet.cu:105-133
+    double a = b.a, e = b.e;
+    double n = std::sqrt(1.0 / a / a / a);
+    double efac = std::sqrt(1.0 - e * e);
+    double B = e / (1.0 + efac);
+    double u = ecc_anomaly(n, e, t);
+    double sinu = std::sin(u), cosu = std::cos(u);
+    double f = u + 2.0 * std::atan(B * sinu / (1.0 - B * cosu));
+    double esinf = std::sin(f) * e, ecosf = std::cos(f) * e;
+    b.m = ramp(t, P.ramp_time, P.planet_mass);
+    b.x = a * efac * efac / (1.0 + ecosf);
+    b.y = f - std::fmod(P.frame_omega * t, kTwoPi) + kPi;
+    b.z = kHalfPi;
+    b.vx = n * a * esinf / efac;
+    b.vy = n * (1.0 + ecosf) * (1.0 + ecosf) / efac / efac / efac;
+    b.vz = 0.0;
+}
+
+// star_planet_grav_cyl (forces.cu:41-58), evaluated with dt = 0
+void planet_force_cyl(double rad, double azi, double z, const Body& b, double& fx, double& fy, double& fz) {
+    double s = std::sin(azi - (b.y + 0.0 * b.vy)), c = std::cos(azi - (b.y + 0.0 * b.vy));
+    double Rp = std::sqrt(rad * rad + b.x * b.x - 2.0 * b.x * rad * c + z * z + b.rs * b.rs);
+    double k = 1.0 / Rp / Rp / Rp;
+    fx = -b.m * (rad - b.x * c) * k - b.m * c / b.x / b.x;
+    fy = -b.m * b.x * s * k + b.m * s / b.x / b.x;
+    fz = -b.m * z * k;
+}
+
+}  // namespace
+
+// ================================================================== grid
+struct orc_grid {
+    pg_params P;
+    int xres, yres, xpad, ypad, xarr, yarr;
+    std::vector<double> xa, xv, ya, yv, za, zv;
+    std::vector<Cell> C, T;
+    std::vector<Dust> CD, TD;
+    std::vector<double> vis;
+    std::vector<double> rot, res;
+    std::vector<int> inc;
+    Body planet;
+    size_t ind(int i, int j) const { return (size_t)i + (size_t)xarr * (size_t)j; }
+    double xc(int i) const { return 0.5 * (xa[i + 1] + xa[i]); }
+    double yc(int j) const { return 0.5 * (ya[j + 1] + ya[j]); }
+};
+
+namespace {
+
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+// uniform in [0,1) keyed by (seed, field, interior i, interior j)
+double keyed_uniform(uint64_t seed, int field, int i, int j) {
+    uint64_t key = splitmix64(seed) ^ splitmix64(((uint64_t)field << 48) ^ ((uint64_t)j << 24) ^ (uint64_t)i);
+    return (double)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// boundx / boundx_dust / boundx2 for one device with types 0 or 1
+// (boundary.cu:13-113, 462-560)
+void fill_x_ghosts(orc_grid& g, std::vector<Cell>& X) {
+    const pg_params& P = g.P;
+    for (int j = 0; j < g.yarr; ++j) {
+        for (int n = 0; n < g.xpad; ++n) {
+            size_t l = g.ind(n, j), r = g.ind(g.xres + g.xpad + n, j);
+            if (P.bound_lft == 0) X[l] = cell_init(P, g.xc(n));
+            else if (P.bound_lft == 1) X[l] = X[g.ind(g.xpad, j)];
+            if (P.bound_rgh == 0) X[r] = cell_init(P, g.xc(g.xres + g.xpad + n));
+            else if (P.bound_rgh == 1) X[r] = X[g.ind(g.xres + g.xpad - 1, j)];
+        }
+    }
+}
+void fill_x_ghosts_dust(orc_grid& g, std::vector<Dust>& X) {
+    const pg_params& P = g.P;
+    for (int j = 0; j < g.yarr; ++j) {
+        for (int n = 0; n < g.xpad; ++n) {
+            size_t l = g.ind(n, j), r = g.ind(g.xres + g.xpad + n, j);
+            if (P.bound_lft == 0) X[l] = dust_init(P, g.xc(n));
+            else if (P.bound_lft == 1) X[l] = X[g.ind(g.xpad, j)];
+            if (P.bound_rgh == 0) X[r] = dust_init(P, g.xc(g.xres + g.xpad + n));
+            else if (P.bound_rgh == 1) X[r] = X[g.ind(g.xres + g.xpad - 1, j)];
+        }
+    }
+}
+// boundy / boundy_dust, periodic (boundary.cu:141-268), all i
+template <class T>
+void fill_y_ghosts(orc_grid& g, std::vector<T>& X) {
+    for (int i = 0; i < g.xarr; ++i)
+        for (int n = 0; n < g.ypad; ++n) {
+            X[g.ind(i, n)] = X[g.ind(i, g.yres + n)];
+            X[g.ind(i, g.yres + g.ypad + n)] = X[g.ind(i, g.ypad + n)];
+        }
+}
+
+// ------------------------------------------------------------------ sweeps
+void sweep_x_gas(orc_grid& g, double dt) {  // sweeps.cu:263-358
+    const pg_params& P = g.P;
+    const int n = g.xarr;
+#pragma omp parallel
+    {
+        RowScratch s;
+        s.resize(n);
+#pragma omp for schedule(static)
+        for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
+            for (int i = 0; i < n; ++i) {
+                const Cell& c = g.C[g.ind(i, j)];
+                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                s.r[i] = c.r; s.p[i] = c.p; s.u[i] = c.u; s.v[i] = c.v * rad; s.w[i] = c.w;
+            }
+            riemann_row(P.geomx, P.EOS_flag, P.gam, n, g.xa.data(), s, dt / 1.0);
+            for (int i = g.xpad; i < n - g.xpad; ++i) {
+                Cell D = s.Del[i];
+                double f = dt / g.xv[i];
+                D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
+                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                D.v /= rad;
+                g.T[g.ind(i, j)] = update_cell(P, g.xc(i), g.C[g.ind(i, j)], D);
+            }
+        }
+    }
+    std::swap(g.C, g.T);
+}
+
+void sweep_y_gas(orc_grid& g, double dt) {  // sweeps.cu:361-456
+    const pg_params& P = g.P;
+    const int n = g.yarr;
+#pragma omp parallel
+    {
+        RowScratch s;
+        s.resize(n);
+#pragma omp for schedule(static)
+        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+            double R = g.xc(i);
+            double rot = g.rot[i];
+            for (int j = 0; j < n; ++j) {
+                const Cell& c = g.C[g.ind(i, j)];
+                s.r[j] = c.r; s.p[j] = c.p;
+                s.u[j] = c.v - rot;
+                s.u[j] -= R * P.frame_omega;
+                s.v[j] = c.w; s.w[j] = c.u;
+            }
+            riemann_row(P.geomy, P.EOS_flag, P.gam, n, g.ya.data(), s, dt / R);
+            for (int j = g.ypad; j < n - g.ypad; ++j) {
+                Cell D = s.Del[j];
+                double f = dt / g.yv[j] / R;
+                D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
+                D.u += (rot + R * P.frame_omega) * D.r;
+                double t = D.v;
+                D.v = D.u; D.u = D.w; D.w = t;
+                g.T[g.ind(i, j)] = update_cell(P, g.xc(i), g.C[g.ind(i, j)], D);
+            }
+        }
+    }
+    std::swap(g.C, g.T);
+}
+
+void sweep_x_dust(orc_grid& g, double dt) {  // dust.cu:178-257
+    const pg_params& P = g.P;
+    const int n = g.xarr;
+#pragma omp parallel
+    {
+        RowScratch s;
+        s.resize(n);
+#pragma omp for schedule(static)
+        for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
+            for (int i = 0; i < n; ++i) {
+                const Dust& c = g.CD[g.ind(i, j)];
+                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                s.r[i] = c.r; s.u[i] = c.u; s.v[i] = c.v * rad; s.w[i] = c.w;
+            }
+            dust_row(P.geomx, n, g.xa.data(), s, dt);
+            for (int i = g.xpad; i < n - g.xpad; ++i) {
+                Dust D = s.DelD[i];
+                double f = dt / g.xv[i];
+                D.r *= f; D.u *= f; D.v *= f; D.w *= f;
+                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                D.v /= rad;
+                g.TD[g.ind(i, j)] = update_dust(P, g.CD[g.ind(i, j)], D);
+            }
+        }
+    }
+    std::swap(g.CD, g.TD);
+}
+
+void sweep_y_dust(orc_grid& g, double dt) {  // dust.cu:260-353
+    const pg_params& P = g.P;
+    const int n = g.yarr;
+#pragma omp parallel
+    {
+        RowScratch s;
+        s.resize(n);
+#pragma omp for schedule(static)
+        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+            double R = g.xc(i);
+            double rot = g.rot[i];
+            for (int j = 0; j < n; ++j) {
+                const Dust& c = g.CD[g.ind(i, j)];
+                s.r[j] = c.r;
+                s.u[j] = c.v - rot;
+                s.u[j] -= R * P.frame_omega;
+                s.v[j] = c.w; s.w[j] = c.u;
+            }
+            dust_row(P.geomy, n, g.ya.data(), s, dt / R);
+            for (int j = g.ypad; j < n - g.ypad; ++j) {
+                Dust D = s.DelD[j];
+                double f = dt / g.yv[j];
+                D.r *= f; D.u *= f; D.v *= f; D.w *= f;
+                double f2 = 1.0 / R;
+                D.r *= f2; D.u *= f2; D.v *= f2; D.w *= f2;
+                D.u += (rot + R * P.frame_omega) * D.r;
+                double t = D.v;
+                D.v = D.u; D.u = D.w; D.w = t;
+                g.TD[g.ind(i, j)] = update_dust(P, g.CD[g.ind(i, j)], D);
+            }
+        }
+    }
+    std::swap(g.CD, g.TD);
+}
+
+// --------------------------------------------------------- orbital advection
+void set_orbital_advection(orc_grid& g, double dt) {  // orbit.cu:74-99
+    const pg_params& P = g.P;
+    double dy0 = (P.ymax - P.ymin) / (double)P.yres;
+    for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+        double dy = dy0 * g.xc(i);
+        double rot = g.rot[i];
+        int shift = (int)std::floor(rot * dt / dy + 0.5);
+        double speed = shift * dy / dt;
+        g.res[i] = rot - speed;
+        g.inc[i] = shift;
+    }
+}
+
+void shift_orbital_advection(orc_grid& g) {  // orbit.cu:17-49, 144-157
+    const int jmax = g.yres;
+#pragma omp parallel for schedule(static)
+    for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+        int inc = g.inc[i];
+        for (int j = 0; j < jmax; ++j) {
+            int jj = (j + inc) % jmax;
+            if (jj < 0) jj += jmax;
+            g.T[g.ind(i, jj + g.ypad)] = g.C[g.ind(i, j + g.ypad)];
+            if (g.P.dust_flag) g.TD[g.ind(i, jj + g.ypad)] = g.CD[g.ind(i, j + g.ypad)];
+        }
+    }
+    std::swap(g.C, g.T);
+    if (g.P.dust_flag) std::swap(g.CD, g.TD);
+}
+
+// advection() row phases (advection.cu:1-86 gas, 175-252 dust), flat = 0
+void advect_row(int geom, int n, const double* xa, RowScratch& s, double speed, double dt_eff,
+                bool gas, std::vector<Cell>& flux) {
+    std::fill(s.flat.begin(), s.flat.begin() + n, 0.0);
+    double par[3];
+    for (int i = kPad; i < n + 1 - kPad; ++i) {
+        Cell D{0.0, 0.0, 0.0, 0.0, 0.0};
+        if (speed > 0.0) {
+            double xl = xa[i - 1], xr = xa[i];
+            double x0 = xr - speed * dt_eff;
+            double q = lim01((x0 - xl) / (xr - xl));
+            recon_par(i - 1, geom, xa, s.r.data(), s.flat.data(), par); D.r = ppm_aveR(q, par);
+            if (gas) { recon_par(i - 1, geom, xa, s.p.data(), s.flat.data(), par); D.p = ppm_aveR(q, par); }
+            recon_par(i - 1, geom, xa, s.u.data(), s.flat.data(), par); D.u = ppm_aveR(q, par) * D.r;
+            recon_par(i - 1, geom, xa, s.v.data(), s.flat.data(), par); D.v = ppm_aveR(q, par) * D.r;
+            recon_par(i - 1, geom, xa, s.w.data(), s.flat.data(), par); D.w = ppm_aveR(q, par) * D.r;
+        } else {
+            double xl = xa[i], xr = xa[i + 1];
+            double x0 = xl - speed * dt_eff;
+            double q = lim01((x0 - xl) / (xr - xl));
+            recon_par(i, geom, xa, s.r.data(), s.flat.data(), par); D.r = ppm_aveL(q, par);
+            if (gas) { recon_par(i, geom, xa, s.p.data(), s.flat.data(), par); D.p = ppm_aveL(q, par); }
+            recon_par(i, geom, xa, s.u.data(), s.flat.data(), par); D.u = ppm_aveL(q, par) * D.r;
+            recon_par(i, geom, xa, s.v.data(), s.flat.data(), par); D.v = ppm_aveL(q, par) * D.r;
+            recon_par(i, geom, xa, s.w.data(), s.flat.data(), par); D.w = ppm_aveL(q, par) * D.r;
+        }
+        D.r *= speed; D.p *= speed; D.u *= speed; D.v *= speed; D.w *= speed;
+        flux[i] = D;
+        s.gfac[i] = geom_factor(geom, xa[i]);
+    }
+    for (int i = kPad; i < n - kPad; ++i) {
+        double g0 = s.gfac[i], g1 = s.gfac[i + 1];
+        Cell& D = s.Del[i];
+        D.r = flux[i].r * g0 - flux[i + 1].r * g1;
+        D.p = flux[i].p * g0 - flux[i + 1].p * g1;
+        D.u = flux[i].u * g0 - flux[i + 1].u * g1;
+        D.v = flux[i].v * g0 - flux[i + 1].v * g1;
+        D.w = flux[i].w * g0 - flux[i + 1].w * g1;
+    }
+}
+
+void advect_y_gas(orc_grid& g, double dt) {  // advection.cu:107-173, 321-338
+    const pg_params& P = g.P;
+    fill_y_ghosts(g, g.C);
+    const int n = g.yarr;
+#pragma omp parallel
+    {
+        RowScratch s;
+        s.resize(n);
+        std::vector<Cell> flux(n + 1);
+#pragma omp for schedule(static)
+        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+            double R = g.xc(i);
+            for (int j = 0; j < n; ++j) {
+                const Cell& c = g.C[g.ind(i, j)];
+                s.r[j] = c.r; s.p[j] = c.p; s.u[j] = c.v; s.v[j] = c.w; s.w[j] = c.u;
+            }
+            advect_row(P.geomy, n, g.ya.data(), s, g.res[i], dt / R, true, flux);
+            for (int j = g.ypad; j < n - g.ypad; ++j) {
+                Cell D = s.Del[j];
+                double f = dt / g.yv[j];
+                D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
+                double f2 = 1.0 / R;
+                D.r *= f2; D.p *= f2; D.u *= f2; D.v *= f2; D.w *= f2;
+                double t = D.v;
+                D.v = D.u; D.u = D.w; D.w = t;
+                g.T[g.ind(i, j)] = update_cell_advect(P, g.xc(i), g.C[g.ind(i, j)], D);
+            }
+        }
+    }
+    std::swap(g.C, g.T);
+}
+
+void advect_y_dust(orc_grid& g, double dt) {  // advection.cu:254-319, 340-357
+    const pg_params& P = g.P;
+    fill_y_ghosts(g, g.CD);
+    const int n = g.yarr;
+#pragma omp parallel
+    {
+        RowScratch s;
+        s.resize(n);
+        std::vector<Cell> flux(n + 1);
+#pragma omp for schedule(static)
+        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+            double R = g.xc(i);
+            for (int j = 0; j < n; ++j) {
+                const Dust& c = g.CD[g.ind(i, j)];
+                s.r[j] = c.r; s.u[j] = c.v; s.v[j] = c.w; s.w[j] = c.u;
+            }
+            advect_row(P.geomy, n, g.ya.data(), s, g.res[i], dt / R, false, flux);
+            for (int j = g.ypad; j < n - g.ypad; ++j) {
+                Cell D = s.Del[j];
+                double f = dt / g.yv[j];
+                D.r *= f; D.u *= f; D.v *= f; D.w *= f;
+                double f2 = 1.0 / R;
+                D.r *= f2; D.u *= f2; D.v *= f2; D.w *= f2;
+                double t = D.v;
+                D.v = D.u; D.u = D.w; D.w = t;
+                Dust DD{D.r, D.u, D.v, D.w};
+                g.TD[g.ind(i, j)] = update_dust(P, g.CD[g.ind(i, j)], DD);
+            }
+        }
+    }
+    std::swap(g.CD, g.TD);
+}
+
+// ---------------------------------------------------------------- viscosity
+void viscous_forces(orc_grid& g, const std::vector<Cell>& X) {  // viscosity.cu:34-197 (ndim 2)
+    const pg_params& P = g.P;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < g.yarr - 1; ++j) {
+        for (int i = 1; i < g.xarr - 1; ++i) {
+            double r[3], t[3], u[9], v[9], nr[9];
+            for (int n = 0; n < 3; ++n) { r[n] = g.xc(i + n - 1); t[n] = g.yc(j + n - 1); }
+            double nu0 = nu_at(P, r[0]), nu1 = nu_at(P, r[1]), nu2 = nu_at(P, r[2]);
+            for (int n = 0; n < 9; ++n) {
+                size_t id;
+                if (n < 3) id = g.ind(i + n - 1, j);
+                else if (n < 6) id = g.ind(i + n - 4, j - 1);
+                else id = g.ind(i + n - 7, j + 1);
+                nr[n] = X[id].r; u[n] = X[id].u; v[n] = X[id].v;
+                if (n == 0 || n == 3 || n == 6) nr[n] *= nu0;
+                else if (n == 2 || n == 5 || n == 8) nr[n] *= nu2;
+                else nr[n] *= nu1;
+            }
+            auto d1 = [&](const double* a, double& dx, double& dxx) {
+                dx = (a[2] - a[0]) / (r[2] - r[0]);
+                dxx = ((a[2] - a[1]) / (r[2] - r[1]) - (a[1] - a[0]) / (r[1] - r[0])) / (0.5 * (r[2] - r[0]));
+            };
+            auto d2 = [&](const double* a, double& dy, double& dyy, double& dxy) {
+                dy = (a[7] - a[4]) / (t[2] - t[0]) / r[1];
+                dyy = ((a[7] - a[1]) / (t[2] - t[1]) - (a[1] - a[4]) / (t[1] - t[0])) / (0.5 * (t[2] - t[0])) / r[1] / r[1];
+                dxy = (a[8] - a[6] - a[5] + a[3]) / (r[2] - r[0]) / (t[2] - t[0]) / r[1];
+            };
+            double dudx, d2udx2, dvdx, d2vdx2, dvdy, d2vdy2, d2vdxy, dudy, d2udy2, d2udxy;
+            d1(u, dudx, d2udx2);
+            double dndx = (nr[2] - nr[0]) / (r[2] - r[0]);
+            d1(v, dvdx, d2vdx2);
+            d2(v, dvdy, d2vdy2, d2vdxy);
+            d2(u, dudy, d2udy2, d2udxy);
+            double dndy = (nr[7] - nr[4]) / (t[2] - t[0]) / r[1];
+            double sxx = 2.0 * nr[1] * dudx;
+            double fx = 2.0 * dndx * dudx + 2.0 * nr[1] * d2udx2 + sxx / r[1];
+            double syy = 2.0 * nr[1] * (dvdy + u[1] / r[1]);
+            double sxy = nr[1] * (dudy + dvdx - v[1] / r[1]);
+            double fy = 2.0 * dndy * dvdy + 2.0 * nr[1] * d2vdy2 + 2.0 * nr[1] * dudy / r[1] + 2.0 * dndy * u[1] / r[1];
+            fx += dndy * (dudy + dvdx) + nr[1] * (d2udy2 + d2vdxy) - dndy * v[1] / r[1] - nr[1] * dvdy / r[1] - syy / r[1];
+            fy += dndx * (dudy + dvdx) + nr[1] * (d2udxy + d2vdx2) - dndx * v[1] / r[1] - nr[1] * dvdx / r[1] +
+                  nr[1] * v[1] / r[1] / r[1] + 2.0 * sxy / r[1];
+            size_t id = g.ind(i, j);
+            g.vis[2 * id + 0] = fx;
+            g.vis[2 * id + 1] = fy;
+        }
+    }
+}
+
+void viscosity_eval(orc_grid& g, bool on_T) {  // viscosity.cu:489-549
+    std::vector<Cell>& X = on_T ? g.T : g.C;
+    fill_y_ghosts(g, X);
+    fill_x_ghosts(g, X);
+    viscous_forces(g, X);
+}
+
+// ----------------------------------------------------------- source terms
+// apply_source_terms (sweeps.cu:597-711), ndim 2, geomx 1, cool_flag 0
+void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
+                  std::vector<Cell>& out, std::vector<Dust>& out_d, double mdt, double dt) {
+    const pg_params& P = g.P;
+    const bool dust = P.dust_flag != 0;
+    const bool visc = P.visc_flag != 0;
+    const Body b = g.planet;
+    const int np = P.n_planet;
+#pragma omp parallel for schedule(static)
+    for (int j = g.ypad - 1; j < g.yarr - g.ypad + 1; ++j) {
+        for (int i = g.xpad - 1; i < g.xarr - g.xpad + 1; ++i) {
+            size_t id = g.ind(i, j);
+            Cell C = in[id], T = C;
+            Dust Q{0, 0, 0, 0}, D{0, 0, 0, 0};
+            if (dust) { Q = in_d[id]; D = Q; }
+            double xc = g.xc(i);
+            double yc = g.yc(j) + g.rot[i] * 0.0;
+            double zc = 0.0;
+            // get_gx / get_gy / get_gz (forces.cu:182-327), twobd_flag 0
+            double dis = std::sqrt(xc * xc + zc * zc);
+            double gx = -xc / dis / dis / dis;
+            double gy = 0.0;
+            double gz = -zc / dis / dis / dis;
+            for (int m = 0; m < np; ++m) {
+                double fx, fy, fz;
+                planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
+                gx += fx;
+            }
+            for (int m = 0; m < np; ++m) {
+                double fx, fy, fz;
+                planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
+                gy += fy;
+            }
+            for (int m = 0; m < np; ++m) {
+                double fx, fy, fz;
+                planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
+                gz += fz;
+            }
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            double fxd = 0.0, fyd = 0.0, fzd = 0.0, ts = 0.0;
+            auto gas_forces = [&]() {  // get_g_forces (sweeps.cu:555-576)
+                fy = 0.0;
+                if (visc) fy += g.vis[2 * id + 1] / T.r;
+                fx = T.v * T.v / xc;
+                if (visc) fx += g.vis[2 * id + 0] / T.r;
+            };
+            auto dust_forces = [&]() {  // get_d_forces (sweeps.cu:578-594)
+                fyd = 0.0 * ts + T.v - Q.v;
+                fxd = (D.v * D.v / xc) * ts + T.u - Q.u;
+            };
+            if (dust) {
+                ts = P.Stokes / (std::sqrt(P.gam * T.p / T.r) / P.sc_h) / T.r;  // dust.cu:1-4
+                dust_forces();
+            }
+            gas_forces();
+            if (mdt != 0.0) {
+                T.u = C.u + fx * mdt + gx * mdt;
+                T.v = C.v + fy * mdt + gy * mdt;
+                T.w = C.w + fz * mdt + gz * mdt;
+                if (dust) {
+                    double dtau = 1.0 - std::exp(-mdt / ts);
+                    D.u = Q.u + fxd * dtau + gx * ts * dtau;
+                    D.v = Q.v + fyd * dtau + gy * ts * dtau;
+                    D.w = Q.w + fzd * dtau + gz * ts * dtau;
+                    ts = P.Stokes / (std::sqrt(P.gam * T.p / T.r) / P.sc_h) / T.r;
+                    dust_forces();
+                }
+                gas_forces();
+            }
+            Cell o;
+            o.u = C.u + fx * dt + gx * dt;
+            o.v = C.v + fy * dt + gy * dt;
+            o.w = C.w + fz * dt + gz * dt;
+            o.r = in[id].r;
+            o.p = in[id].p;
+            out[id] = o;
+            if (dust) {
+                double dtau = 1.0 - std::exp(-dt / ts);
+                Dust od;
+                od.u = Q.u + fxd * dtau + gx * ts * dtau;
+                od.v = Q.v + fyd * dtau + gy * ts * dtau;
+                od.w = Q.w + fzd * dtau + gz * ts * dtau;
+                od.r = in_d[id].r;
+                out_d[id] = od;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------- killwave
+void killwave(orc_grid& g, double dt) {  // killwave.cu:8-117
+    const pg_params& P = g.P;
+    double d_in = h_at(P, P.xmin) * P.kill_width;
+    double d_out = h_at(P, P.xmax) * P.kill_width;
+    double inner = P.xmin + d_in, outer = P.xmax - d_out;
+#pragma omp parallel for schedule(static)
+    for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
+        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+            double rad = g.xc(i);
+            Cell& c = g.C[g.ind(i, j)];
+            for (int side = 0; side < 2; ++side) {
+                bool in_zone = side == 0 ? rad < inner : rad > outer;
+                if (!in_zone) continue;
+                double f = side == 0 ? (inner - rad) / d_in : (rad - outer) / d_out;
+                double tau = std::pow(rad, 1.5);
+                double s = (double)sinf((float)(kHalfPi * f));
+                f = s * s;
+                f *= dt / tau;
+                Cell I = cell_init(P, rad);
+                c.r += f * (I.r - c.r);
+                c.p += f * (I.p - c.p);
+                c.u += f * (I.u - c.u);
+                c.v += f * (I.v - c.v);
+                c.w += f * (I.w - c.w);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------- CFL
+double cfl_rate(const orc_grid& g) {  // get_dt_lv1 (timestep.cu:18-97), ndim 2, geomy 3
+    const pg_params& P = g.P;
+    double best = 0.0;
+#pragma omp parallel for reduction(max : best) schedule(static)
+    for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
+        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
+            size_t id = g.ind(i, j);
+            const Cell& c = g.C[id];
+            double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+            double cs = std::sqrt(P.gam * c.p / c.r);
+            double spd = std::fabs(c.u) + cs;
+            if (P.dust_flag) spd = std::fmax(std::fabs(g.CD[id].u), spd);
+            double wid = std::fabs(g.xa[i + 1] - g.xa[i]);
+            double tmp = spd / wid;
+            spd = std::fabs(c.v - rad * P.frame_omega - g.rot[i]) + cs;
+            if (P.dust_flag) spd = std::fmax(std::fabs(g.CD[id].v - rad * P.frame_omega - g.rot[i]), spd);
+            wid = std::fabs(g.ya[j + 1] - g.ya[j]) * rad;
+            tmp = std::fmax(spd / wid, tmp);
+            best = std::fmax(best, tmp);
+        }
+    }
+    return best;
+}
+
+// -------------------------------------------------------------------- DS
+void ds_step(orc_grid& g, double time, double dt) {  // solve.cu:212-277
+    const pg_params& P = g.P;
+    double hdt = 0.5 * dt;
+    if (P.visc_flag) viscosity_eval(g, false);
+    fill_x_ghosts(g, g.C);
+    if (P.dust_flag) fill_x_ghosts_dust(g, g.CD);
+    source_terms(g, g.C, g.CD, g.C, g.CD, 0.0, hdt);
+    fill_x_ghosts(g, g.C);
+    sweep_x_gas(g, dt);
+    fill_y_ghosts(g, g.C);
+    sweep_y_gas(g, dt);
+    if (P.dust_flag) {
+        fill_x_ghosts_dust(g, g.CD);
+        sweep_x_dust(g, dt);
+        fill_y_ghosts(g, g.CD);
+        sweep_y_dust(g, dt);
+    }
+    if (P.OrbAdv_flag) {
+        set_orbital_advection(g, dt);
+        shift_orbital_advection(g);
+        advect_y_gas(g, dt);
+        if (P.dust_flag) advect_y_dust(g, dt);
+    }
+    orbit_solution(P, g.planet, time + dt);
+    if (P.visc_flag) {
+        source_terms(g, g.C, g.CD, g.T, g.TD, hdt, hdt);
+        viscosity_eval(g, true);
+        source_terms(g, g.C, g.CD, g.C, g.CD, hdt, hdt);
+    } else {
+        source_terms(g, g.C, g.CD, g.C, g.CD, hdt, hdt);
+    }
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+orc_grid* orc_create(const pg_params* Pin) {
+    orc_grid* g = new orc_grid();
+    g->P = *Pin;
+    const pg_params& P = g->P;
+    g->xres = P.xres; g->yres = P.yres;
+    g->xpad = kPad; g->ypad = kPad;
+    g->xarr = g->xres + 2 * g->xpad;
+    g->yarr = g->yres + 2 * g->ypad;
+    build_axis(g->xa, g->xv, P.xmin, P.xmax, P.xres, g->xpad, P.geomx, P.gridx);
+    build_axis(g->ya, g->yv, P.ymin, P.ymax, P.yres, g->ypad, P.geomy, P.gridy);
+    build_axis(g->za, g->zv, P.zmin, P.zmax, 1, 0, P.geomz, P.gridz);
+    size_t N = (size_t)g->xarr * g->yarr;
+    g->C.resize(N); g->T.resize(N); g->CD.resize(N); g->TD.resize(N);
+    g->vis.assign(2 * N, 0.0);
+    // fill_grid (init.cu:502-519): cell-centre values everywhere
+    for (int j = 0; j < g->yarr; ++j)
+        for (int i = 0; i < g->xarr; ++i) {
+            g->C[g->ind(i, j)] = cell_init(P, g->xc(i));
+            g->CD[g->ind(i, j)] = P.dust_flag ? dust_init(P, g->xc(i)) : Dust{0, 0, 0, 0};
+        }
+    // seeded perturbation of the interior (not in the reference)
+    for (int j = 0; j < P.yres; ++j)
+        for (int i = 0; i < P.xres; ++i) {
+            Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+            double x = g->xc(i + g->xpad);
+            if (P.noise_rho != 0.0) {
+                c.r *= 1.0 + P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, j) - 1.0);
+                if (P.EOS_flag == 0) c.p = cs2_at(P, x) * c.r;
+            }
+            if (P.noise_vr != 0.0)
+                c.u += P.noise_vr * std::sqrt(P.gam * cs2_at(P, x)) * (2.0 * keyed_uniform(P.seed, 1, i, j) - 1.0);
+        }
+    g->T = g->C;
+    g->TD = g->CD;
+    // init_planet (planet.cu:319-403), n_planet 1, 2D softening (planet.cu:308-317)
+    g->planet = Body{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    g->planet.a = P.planet_radius;
+    g->planet.e = P.planet_ecc;
+    orbit_solution(P, g->planet, 0.0);
+    g->planet.rs = P.rs_fac * P.sc_h * P.planet_radius;
+    // init_orbital_advection (orbit.cu:51-72)
+    g->rot.assign(g->xarr, 0.0);
+    g->res.assign(g->xarr, 0.0);
+    g->inc.assign(g->xarr, 0);
+    for (int i = 0; i < g->xarr; ++i) {
+        double rad = g->xc(i);
+        g->rot[i] = P.OrbAdv_flag ? v_init(P, rad) - rad * P.frame_omega : 0.0;
+    }
+    return g;
+}
+
+void orc_destroy(orc_grid* g) { delete g; }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_get_state(const orc_grid* g, double* gas, double* dust) {
+    size_t N = (size_t)g->xres * g->yres;
+    for (int j = 0; j < g->yres; ++j)
+        for (int i = 0; i < g->xres; ++i) {
+            size_t o = (size_t)i + (size_t)g->xres * j;
+            const Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+            if (gas) {
+                gas[o] = c.r; gas[N + o] = c.p; gas[2 * N + o] = c.u; gas[3 * N + o] = c.v; gas[4 * N + o] = c.w;
+            }
+            if (dust) {
+                const Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                dust[o] = d.r; dust[N + o] = d.u; dust[2 * N + o] = d.v; dust[3 * N + o] = d.w;
+            }
+        }
+}
+
+void orc_set_state(orc_grid* g, const double* gas, const double* dust) {
+    size_t N = (size_t)g->xres * g->yres;
+    for (int j = 0; j < g->yres; ++j)
+        for (int i = 0; i < g->xres; ++i) {
+            size_t o = (size_t)i + (size_t)g->xres * j;
+            if (gas) {
+                Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+                c.r = gas[o]; c.p = gas[N + o]; c.u = gas[2 * N + o]; c.v = gas[3 * N + o]; c.w = gas[4 * N + o];
+            }
+            if (dust) {
+                Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                d.r = dust[o]; d.u = dust[N + o]; d.v = dust[2 * N + o]; d.w = dust[3 * N + o];
+            }
+        }
+}
+
+void orc_get_coords(const orc_grid* g, double* xa, double* xv, double* ya, double* yv) {
+    if (xa) std::memcpy(xa, g->xa.data(), g->xa.size() * sizeof(double));
+    if (xv) std::memcpy(xv, g->xv.data(), g->xv.size() * sizeof(double));
+    if (ya) std::memcpy(ya, g->ya.data(), g->ya.size() * sizeof(double));
+    if (yv) std::memcpy(yv, g->yv.data(), g->yv.size() * sizeof(double));
+}
+
+void orc_get_planet(const orc_grid* g, double* b) { std::memcpy(b, &g->planet, sizeof(Body)); }
+
+void orc_get_rot(const orc_grid* g, double* rot) { std::memcpy(rot, g->rot.data(), g->rot.size() * sizeof(double)); }
+
+double orc_global_dt(orc_grid* g, double dt_prev) {  // timestep.cu:99-140
+    double dt = g->P.CFL / cfl_rate(*g);
+    double cap = dt_prev * 1.1;
+    return dt < cap ? dt : cap;
+}
+
+void orc_solve(orc_grid* g, double t, double dt) {  // solve.cu:279-292
+    ds_step(*g, t, dt);
+    if (g->P.kill_flag) killwave(*g, dt);
+}
+
+void orc_run(orc_grid* g, int nsteps, double* t_io, double* dt_io, double* dts) {  // main.cu:320-359
+    double t = *t_io, dt = *dt_io;
+    for (int n = 0; n < nsteps; ++n) {
+        dt = orc_global_dt(g, dt);
+        if (t + dt > g->P.end_time) dt = g->P.end_time - t;
+        orc_solve(g, t, dt);
+        t += dt;
+        if (dts) dts[n] = dt;
+    }
+    *t_io = t;
+    *dt_io = dt;
+}
+
+void orc_viscosity(orc_grid* g, int on_T, double* vis) {
+    viscosity_eval(*g, on_T != 0);
+    if (!vis) return;
+    size_t N = (size_t)g->xres * g->yres;
+    for (int j = 0; j < g->yres; ++j)
+        for (int i = 0; i < g->xres; ++i) {
+            size_t id = g->ind(i + g->xpad, j + g->ypad), o = (size_t)i + (size_t)g->xres * j;
+            vis[o] = g->vis[2 * id];
+            vis[N + o] = g->vis[2 * id + 1];
+        }
+}
+
+void orc_sources(orc_grid* g, double mdt, double dt, int inplace) {
+    if (inplace) source_terms(*g, g->C, g->CD, g->C, g->CD, mdt, dt);
+    else source_terms(*g, g->C, g->CD, g->T, g->TD, mdt, dt);
+}
+
+void orc_sweepx(orc_grid* g, double dt) { fill_x_ghosts(*g, g->C); sweep_x_gas(*g, dt); }
+void orc_sweepy(orc_grid* g, double t, double dt) { (void)t; fill_y_ghosts(*g, g->C); sweep_y_gas(*g, dt); }
+void orc_sweepx_dust(orc_grid* g, double dt) { fill_x_ghosts_dust(*g, g->CD); sweep_x_dust(*g, dt); }
+void orc_sweepy_dust(orc_grid* g, double t, double dt) { (void)t; fill_y_ghosts(*g, g->CD); sweep_y_dust(*g, dt); }
+
+void orc_orbital_advection(orc_grid* g, double dt) {
+    set_orbital_advection(*g, dt);
+    shift_orbital_advection(*g);
+    advect_y_gas(*g, dt);
+    if (g->P.dust_flag) advect_y_dust(*g, dt);
+}
+
+void orc_get_orbadv(const orc_grid* g, int* inc, double* res) {
+    for (int i = 0; i < g->xres; ++i) {
+        if (inc) inc[i] = g->inc[i + g->xpad];
+        if (res) res[i] = g->res[i + g->xpad];
+    }
+}
+
+void orc_killwave(orc_grid* g, double dt) { killwave(*g, dt); }
+void orc_evolve_planet(orc_grid* g, double t) { orbit_solution(g->P, g->planet, t); }
+
+// ------------------------------------------------------------ unit level
+void orc_fourth_parameters(int geom, const double* x6, const double* a5, double* par3) {
+    fourth_parameters(2, geom, x6, a5, par3);
+}
+void orc_ppm_adjust(double flat, double* par3) { ppm_adjust(flat, par3); }
+double orc_ppm_aveL(double q, const double* par3) { return ppm_aveL(q, par3); }
+double orc_ppm_aveR(double q, const double* par3) { return ppm_aveR(q, par3); }
+double orc_flatten(double gam, const double* r5, const double* p5, const double* u5) {
+    return flatten_cell(2, 5, gam, r5, p5, u5);
+}
+void orc_set_state_row(int geom, double gam, int n, const double* xa, const double* dv,
+                       const double* r, const double* p, const double* u, const double* v,
+                       const double* w, const double* flat, int i, double dt, double* S10) {
+    (void)n; (void)dv;
+    State S;
+    trace_state(i, geom, gam, xa, r, p, u, v, w, flat, dt, 0.0, S);
+    std::memcpy(S10, &S, sizeof(State));
+}
+void orc_riemann_hllc(double gam, int eos, const double* S10, double* out) {
+    State S;
+    std::memcpy(&S, S10, sizeof(State));
+    double pm, sl, sm, sr, pres, uprs;
+    wave_speeds(S, gam, 0.0, pm, sl, sm, sr);
+    Cell F;
+    hllc(S, eos, gam, pm, sl, sm, sr, F, pres, uprs);
+    double o[10] = {pm, sl, sm, sr, F.r, F.p, F.u, F.v, F.w, pres};
+    std::memcpy(out, o, sizeof(o));
+}
+void orc_riemann_row(int geom, double gam, int n, const double* xa, const double* dv,
+                     const double* r, const double* p, const double* u, const double* v,
+                     const double* w, double dt, double* Del5) {
+    (void)dv;
+    RowScratch s;
+    s.resize(n);
+    std::memcpy(s.r.data(), r, n * sizeof(double));
+    std::memcpy(s.p.data(), p, n * sizeof(double));
+    std::memcpy(s.u.data(), u, n * sizeof(double));
+    std::memcpy(s.v.data(), v, n * sizeof(double));
+    std::memcpy(s.w.data(), w, n * sizeof(double));
+    for (int i = 0; i < n; ++i) s.Del[i] = Cell{0, 0, 0, 0, 0};
+    riemann_row(geom, 0, gam, n, xa, s, dt);
+    for (int i = 0; i < n; ++i) {
+        Del5[i] = s.Del[i].r; Del5[n + i] = s.Del[i].p; Del5[2 * n + i] = s.Del[i].u;
+        Del5[3 * n + i] = s.Del[i].v; Del5[4 * n + i] = s.Del[i].w;
+    }
+}
+void orc_dust_flux(int geom, int n, const double* xa, const double* dv, const double* r,
+                   const double* u, const double* v, const double* w, int i, double dt, double* F4) {
+    (void)dv;
+    std::vector<double> flat(n, 0.0);
+    Dust F = dust_interface(i, geom, xa, r, u, v, w, flat.data(), dt, 0.0);
+    F4[0] = F.r; F4[1] = F.u; F4[2] = F.v; F4[3] = F.w;
+}
+void orc_orbit_solution(const pg_params* P, double t, double* b10) {
+    Body b{0, P->planet_radius, P->planet_ecc, 0, 0, 0, 0, 0, 0, P->rs_fac * P->sc_h * P->planet_radius};
+    orbit_solution(*P, b, t);
+    std::memcpy(b10, &b, sizeof(Body));
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// pg_internal.h -- data shared between the host module code (pg_host.cpp)
+// and the HIP kernels (pg_kernels.hip).
+//
+// Device data layout (one disk):
+//   state     SoA, interior cells only, radius fastest: field f of cell (i,j)
+//             at base[f*N + i + nx*j], N = nx*ny.
+//             gas  : rho, u (radial velocity), vp (azimuthal velocity minus
+//                    the background rotation vbg(R) = initial v_phi(R))
+//             dust : rho_d, u_d, vp_d (same background subtracted)
+//             Pressure is not stored: under the locally isothermal EOS the
+//             reference resets p = c_s^2(R) rho after every update
+//             (sweeps.cu:255-256, advection.cu:100-102).
+//   ghosts    radial ghost cells hold the analytic initial state (boundary
+//             type 0, boundary.cu:30-33) and are read from per-radius
+//             tables; azimuthal ghosts are periodic (type 3) and come from
+//             modular indexing.
+//   tables    per padded radius ip = i + NPAD (i interior, ip in
+//             [0, nx + 2*NPAD)), precomputed in FP64 on the host.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/pg_params.h"
+
+namespace pg {
+
+constexpr int NPAD = 3;
+constexpr int NGAS = 3;
+constexpr int NDUST = 3;
+
+struct Body {
+    double m, a, e, x, y, z, vx, vy, vz, rs;
+};
+
+// Per-step control block, device resident, always FP64.
+struct StepCtl {
+    double t;            // time at the start of the step
+    double dt;           // step size
+    double dt_prev;      // previous step size (1.1x cap)
+    double end_time;
+    unsigned long long rate_bits;  // max CFL rate of the state, bits of a non-negative double
+    int nstep;
+    int mode;            // 0: dt supplied by host, 1: dt from rate (device loop)
+    Body planet0;        // planet at t     (first half kick)
+    Body planet1;        // planet at t+dt  (predictor and corrector)
+};
+
+// Per-radius tables (FP64 on the host; cast to the storage type on upload).
+enum TabField {
+    T_XA = 0,     // left edge of padded cell ip (nx+7 entries used)
+    T_XV,         // cell volume per unit angle, (xr^2-xl^2)/2 for geomx 1
+    T_XC,         // centre 0.5*(xl+xr)
+    T_IDX,        // 1/(xr-xl)
+    T_CS2,        // c_s^2(R)
+    T_VBG,        // background rotation v_phi,init(R)
+    T_EBAL,       // vbg^2/R - 1/R^2 (centrifugal minus stellar gravity at vbg)
+    T_NU,         // kinematic viscosity nu(R)
+    T_DVBG,       // (vbg[ip+1]-vbg[ip-1])/(xc[ip+1]-xc[ip-1])
+    T_D2VBG,      // second difference of vbg on the centre grid (viscosity.cu:12)
+    T_WL0, T_WL1, T_WL2,   // 4th-order left-slope weights
+    T_WR0, T_WR1, T_WR2,   // 4th-order right-slope weights
+    T_G_R, T_G_U, T_G_V,   // gas initial state rho, u, vp (vp = 0)
+    T_D_R, T_D_U, T_D_V,   // dust initial state rho_d, u_d, vp_d
+    T_KILL,       // wave-killing rate without dt: sin^2(pi/2*d)/R^1.5, 0 outside
+    T_NFIELDS
+};
+
+template <typename real>
+struct DiskArgs {
+    int nx, ny;
+    int has_dust, visc, orbadv, kill;
+    int n_planet;
+    int tile_x;          // interior cells per x-sweep tile
+    int sweep_on;        // 0: the ring kernels skip the sweep (shift + advection only)
+    real* gas;           // 3*N current state
+    real* gasT;          // 3*N scratch / ping-pong
+    real* dust;          // 3*N
+    real* dustT;         // 3*N
+    real* vis;           // 2*N viscous acceleration*rho
+    const real* tab;     // T_NFIELDS * tab_stride
+    int tab_stride;      // nx + 2*NPAD + 1
+    real dy;             // azimuthal cell width (angle), uniform grid
+    real wLy[3], wRy[3]; // 4th-order weights of the uniform azimuthal grid
+    real gam, sc_h, Stokes, D_G_ratio, frame_omega;
+    double CFL;
+    double dy_d;         // dy in FP64 for the orbital shift
+    const double* rot;   // 2*nx FP64: [0,nx) vbg(R) - R*frame_omega
+                         // (orbit.cu:65), [nx,2nx) R of the interior radii
+    int* inc;            // nx, integer azimuthal shift of the step
+    real* res;           // nx, residual advection speed of the step
+    StepCtl* ctl;
+    double ramp_time, planet_mass, planet_ecc, planet_radius, rs;
+    double* rate_out;    // scratch (unused when null)
+};
+
+}  // namespace pg

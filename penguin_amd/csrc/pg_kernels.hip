@@ -1,0 +1,1144 @@
+// pg_kernels.hip -- gfx950 kernels of the Godunov step (2D, locally
+// isothermal gas + pressureless dust, PPM4 + HLLC).
+//
+// Every kernel is templated on the storage type `real` (float or double);
+// arithmetic runs in that type.  Data layout and tables: pg_internal.h.
+//
+// Kernels and the reference routines they compute:
+//   k_sweep_x_gas   sweepx_inplace  (sweeps.cu:263-358) + riemann()
+//   k_sweep_x_dust  sweepx_dust_inplace (dust.cu:178-257)
+//   k_ring_gas      sweepy_inplace (sweeps.cu:361-456), then the orbital
+//                   shift (orbit.cu:17-49) and the residual advection
+//                   (advection.cu:1-173) of one radius, fused: the ring of
+//                   one radius stays in LDS between the three stages
+//   k_ring_dust     the same for dust (dust.cu:260-353, advection.cu:175-319)
+//   k_viscosity     viscous_forces (viscosity.cu:34-197)
+//   k_sources       apply_source_terms (sweeps.cu:555-711), optionally
+//                   followed by killwave (killwave.cu:17-103) and the CFL
+//                   rate of the result (timestep.cu:18-97)
+//   k_cfl           get_dt_lv1/lv2 (timestep.cu:18-111)
+//   k_prologue      dt selection (timestep.cu:133-139, main.cu:322-329),
+//                   planet orbit (planet.cu:105-187), orbital shift and
+//                   residual speed per radius (orbit.cu:74-99)
+//
+// The reference evaluates the block-cooperative phases of riemann()
+// (flatten -> traced states -> HLLC -> flux difference) with shared-memory
+// barriers between them; here each phase writes LDS arrays over a tile and a
+// barrier separates the phases, and each cell's parabola is computed once
+// (the reference recomputes it for both of its interfaces).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pg_internal.h"
+
+namespace pg {
+
+#define TAB(f, ip) d.tab[(size_t)(f) * d.tab_stride + (ip)]
+
+// ---------------------------------------------------------------- helpers
+template <typename real>
+__device__ __forceinline__ real lim01(real a) {  // solve.cu:14-17
+    return fmin(fmax(a, real(0)), real(1));
+}
+template <typename real>
+__device__ __forceinline__ real exp_lim(real x) {  // solve.cu:29-34
+    return fmax(real(1) + x, real(1.0e-6));
+}
+
+// 4th-order slopes from precomputed geometric weights (the linear map
+// get_4th_parameters applies to the four neighbouring differences,
+// recon.cu:216-357), limiter (recon.cu:361-376), flattening and the PPM
+// monotonicity correction (PPM.cu:64-91).
+template <typename real>
+__device__ __forceinline__ void slopes(real am2, real am1, real a0, real ap1, real ap2,
+                                       real wl0, real wl1, real wl2, real wr0, real wr1, real wr2,
+                                       real flat, real& sL, real& sR) {
+    real d0 = am1 - am2, d1 = a0 - am1, d2 = ap1 - a0, d3 = ap2 - ap1;
+    real l = wl0 * d0 + wl1 * d1 + wl2 * d2;
+    real r = wr0 * d1 + wr1 * d2 + wr2 * d3;
+    l = copysign(fmin(fabs(l), fabs(d1)), d1);
+    r = copysign(fmin(fabs(r), fabs(d2)), d2);
+    if (r * l <= real(0)) {
+        l = real(0);
+        r = real(0);
+    }
+    real fk = real(1) - flat;
+    r *= fk;
+    l *= fk;
+    real t3 = r + l;
+    real t2 = t3 * t3;
+    real t1 = t3 * real(3) * (l - r);
+    if (t2 < t1) l = real(2) * r;
+    else if (t2 < -t1) r = real(2) * l;
+    sL = l;
+    sR = r;
+}
+
+// Mean of the cell parabola over its first fraction q (get_PPM_aveL,
+// PPM.cu:106-113) and over its last fraction x (get_PPM_aveR with x = 1-q,
+// PPM.cu:95-104).
+template <typename real>
+__device__ __forceinline__ real aveL(real q, real sL, real a, real sR) {
+    real a6 = real(3) * (sL - sR);
+    return (a - sL) + (q * real(0.5)) * ((sR + sL) + (real(1) - q * real(2.0 / 3.0)) * a6);
+}
+template <typename real>
+__device__ __forceinline__ real aveR(real x, real sL, real a, real sR) {
+    real a6 = real(3) * (sL - sR);
+    return (a + sR) - (x * real(0.5)) * ((sR + sL) - (real(1) - x * real(2.0 / 3.0)) * a6);
+}
+
+// Flattening coefficient of the centre cell of a 5-cell window
+// (flatten, PPM.cu:7-62).  cs is the cell's sound speed.
+template <typename real>
+__device__ __forceinline__ real flatten5(real pm2, real pm1, real p0, real pp1, real pp2,
+                                         real um1, real u0, real up1, real cs) {
+    real f = real(0);
+    real dp1f = pp1 - p0, dp2f = pp2 - pm1, dp1b = p0 - pm1, dp2b = pp1 - pm2;
+    real Mf = (u0 - up1) / cs, Mb = (um1 - u0) / cs;
+    if (Mf > real(0) || Mb > real(0)) {
+        if (dp2f != real(0) && dp2b != real(0)) {
+            real ff = real(10) * fmax(real(0), fabs(dp1f / dp2f - real(0.5)) - real(0.25));
+            real fb = real(10) * fmax(real(0), fabs(dp1b / dp2b - real(0.5)) - real(0.25));
+            f = fmax(ff, fb);
+            f = fmin(real(0.5), f * fmax(fabs(dp1f), fabs(dp1b)) / p0);
+        }
+    }
+    return f;
+}
+
+// Characteristic-traced half-step state on the right of an interface from
+// the cell right of it (set_state, states.cu:151-206).  dtx = dt/dx.
+template <typename real>
+__device__ __forceinline__ void trace_right(real gam, real dt, real idx, real r, real p, real u,
+                                            real rsL, real rsR, real psL, real psR, real usL, real usR,
+                                            real vsL, real v, real vsR,
+                                            real& rr, real& pr, real& ur, real& vr) {
+    real cs = sqrt(gam * p / r);
+    real q = lim01((cs - fmin(u, real(0))) * dt * idx);
+    real ua = aveL(q, usL, u, usR);
+    real dis = fmin(ua, real(0)) * dt;
+    q = lim01(-dis * idx);
+    real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
+    real p0 = fmax(aveL(q, psL, p, psR), p * real(1.0e-10));
+    real u0 = aveL(q, usL, u, usR);
+    cs = sqrt(gam * p0 / r0);
+    q = lim01((cs * dt - dis) * idx);
+    real u_ = aveL(q, usL, u, usR);
+    real p_ = aveL(q, psL, p, psR);
+    u_ = -(u_ - u0) / cs;
+    p_ = -(p_ - p0) / cs;
+    rr = r0 * exp_lim(u_);
+    pr = p0 * exp_lim(gam * u_);
+    ur = u0 + p_ / r0;
+    vr = aveL(q, vsL, v, vsR);
+}
+
+// ... and on the left from the cell left of it (states.cu:208-264).
+template <typename real>
+__device__ __forceinline__ void trace_left(real gam, real dt, real idx, real r, real p, real u,
+                                           real rsL, real rsR, real psL, real psR, real usL, real usR,
+                                           real vsL, real v, real vsR,
+                                           real& rl, real& pl, real& ul, real& vl) {
+    real cs = sqrt(gam * p / r);
+    real x = lim01((fmax(u, real(0)) + cs) * dt * idx);
+    real ua = aveR(x, usL, u, usR);
+    real dis = fmax(ua, real(0)) * dt;
+    x = lim01(dis * idx);
+    real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
+    real p0 = fmax(aveR(x, psL, p, psR), p * real(1.0e-10));
+    real u0 = aveR(x, usL, u, usR);
+    cs = sqrt(gam * p0 / r0);
+    x = lim01((dis + cs * dt) * idx);
+    real u_ = aveR(x, usL, u, usR);
+    real p_ = aveR(x, psL, p, psR);
+    u_ = -(u0 - u_) / cs;
+    p_ = -(p0 - p_) / cs;
+    rl = r0 * exp_lim(u_);
+    pl = p0 * exp_lim(gam * u_);
+    ul = u0 + p_ / r0;
+    vl = aveR(x, vsL, v, vsR);
+}
+
+// Wave speeds (get_pm_simple, get_lr_speeds_Einfeldt, get_sm;
+// wave_speeds.cu:24-35,288-333) and the HLLC fluxes of an isothermal state
+// (HLLC.cu:10-77, EOS_flag 0): mass flux, upwinded normal and transverse
+// velocity, interface pressure.
+template <typename real>
+__device__ __forceinline__ void hllc_iso(real gam, real rl, real pl, real ul, real vl,
+                                         real rr, real pr, real ur, real vr,
+                                         real& Fr, real& us, real& vs, real& pres) {
+    real cl = sqrt(gam * pl / rl), cr = sqrt(gam * pr / rr);
+    real pm = real(0.5) * (pr + pl) * exp_lim(real(0.5) * gam * (ul - ur) / (real(0.5) * (cr + cl)));
+    real srl = sqrt(rl), srr = sqrt(rr);
+    real ssum = srl + srr;
+    real ubar = (srl * ul + srr * ur) / ssum;
+    real n2 = real(0.5) * srl * srr / (ssum * ssum);
+    real cbar = sqrt((cl * cl * srl + cr * cr * srr) / ssum + n2 * (ur - ul) * (ur - ul));
+    real sl = ubar - cbar, sr = ubar + cbar;
+    real sm = (rl * ul * cl + rr * ur * cr) / (rl * cl + rr * cr);
+    sm += (pl - pr) / (rl + rr) / (real(0.5) * (cl + cr));
+    // upwind selection of HLLC.cu:2-8
+    if (!signbit(sl)) {
+        Fr = ul * rl; us = ul; vs = vl; pres = pl;
+    } else if (signbit(sr)) {
+        Fr = ur * rr; us = ur; vs = vr; pres = pr;
+    } else if (!signbit(sm)) {
+        Fr = sm * rl * (sl - ul) / (sl - sm); us = ul; vs = vl; pres = (sl * pm - sm * pl) / (sl - sm);
+    } else {
+        Fr = sm * rr * (sr - ur) / (sr - sm); us = ur; vs = vr; pres = (sr * pm - sm * pr) / (sr - sm);
+    }
+    us *= Fr;
+    vs *= Fr;
+}
+
+// Pressureless dust flux through an interface (dust_flux, dust.cu:6-105)
+// from the parabolas of the cells left (L) and right (R) of it.
+// idxL, idxR: 1/dx of those cells; dt: the sweep's time step in the
+// coordinate's units.
+template <typename real>
+__device__ __forceinline__ void dust_iface(real dt,
+                                           real idxL, real rL, real rLsL, real rLsR, real uL, real uLsL,
+                                           real uLsR, real vL, real vLsL, real vLsR,
+                                           real idxR, real rR, real rRsL, real rRsR, real uR, real uRsL,
+                                           real uRsR, real vR, real vRsL, real vRsR,
+                                           real& Fr, real& Fu, real& Fv) {
+    // right state from the right cell
+    real y = -fmin(uR - uRsL, real(0)) * dt * idxR;
+    real q = lim01(y);
+    real ur = aveL(q, uRsL, uR, uRsR);
+    real du = dt * (ur - uR) * idxR / (real(1) - y);
+    q = lim01(-fmin(ur, real(0)) * dt * idxR);
+    real srr = aveL(q, rRsL, rR, rRsR) * exp_lim(du);
+    real sur = aveL(q, uRsL, uR, uRsR);
+    real svr = aveL(q, vRsL, vR, vRsR);
+    // left state from the left cell
+    y = fmax(uL + uLsR, real(0)) * dt * idxL;
+    real x = lim01(y);
+    real ul = aveR(x, uLsL, uL, uLsR);
+    du = dt * (uL - ul) * idxL / (real(1) - y);
+    x = lim01(fmax(ul, real(0)) * dt * idxL);
+    real srl = aveR(x, rLsL, rL, rLsR) * exp_lim(du);
+    real sul = aveR(x, uLsL, uL, uLsR);
+    real svl = aveR(x, vLsL, vL, vLsR);
+    // donor-cell caps (dust.cu:96-97)
+    srl = fmin(rL * dt * idxL, fmax(sul * srl, real(0)));
+    srr = fmax(-rR * dt * idxR, fmin(sur * srr, real(0)));
+    Fr = srl + srr;
+    Fu = srl * sul + srr * sur;
+    Fv = srl * svl + srr * svr;
+}
+
+// ------------------------------------------------------------ planet (FP64)
+__host__ __device__ inline double ramp_function(double t, double tau, double val) {  // planet.cu:7-17
+    return t < tau ? val * sin(0.5 * 3.1415926535897932384626434 * t / tau) : val;
+}
+
+__host__ __device__ inline double ecc_anomaly(double n, double e, double t) {  // planet.cu:60-103
+    const double twopi = 2.0 * 3.1415926535897932384626434;
+    const double pi = 3.1415926535897932384626434;
+    double l = fmod(n * t, twopi);
+    double D = -e * sin(l);
+    if (e == 0.0 || l == 0.0 || l == pi || l == twopi) return l;
+    double step;
+    int count = 0;
+    do {
+        double es = e * sin(l - D), ec = e * cos(l - D);
+        double x = D + es, dx = 1.0 - ec, d2x = -es;
+        double eps = 2.0 * x * d2x / dx / dx;
+        if (fabs(eps) < 1.0) step = -(x / dx) * (1.0 + eps / 4.0 + eps * eps / 8.0 + 5.0 * eps * eps * eps / 64.0);
+        else step = copysign(fmin(fabs(x / dx), 0.25 * pi), -x);
+        D += step;
+        count++;
+    } while ((fabs(step) > 1.0e-12 || count < 10) && count < 200);
+    return l - D;
+}
+
+// orbit_solution (planet.cu:105-133): analytic Kepler orbit in the frame
+// rotating at frame_omega; mass ramp of planet.cu:7-17.
+__host__ __device__ inline void orbit_solution(Body& b, double t, double ramp_time, double mass,
+                                               double frame_omega) {
+    const double pi = 3.1415926535897932384626434;
+    double a = b.a, e = b.e;
+    double n = sqrt(1.0 / a / a / a);
+    double efac = sqrt(1.0 - e * e);
+    double B = e / (1.0 + efac);
+    double u = ecc_anomaly(n, e, t);
+    double f = u + 2.0 * atan(B * sin(u) / (1.0 - B * cos(u)));
+    double esinf = e * sin(f), ecosf = e * cos(f);
+    b.m = ramp_function(t, ramp_time, mass);
+    b.x = a * efac * efac / (1.0 + ecosf);
+    b.y = f - fmod(frame_omega * t, 2.0 * pi) + pi;
+    b.z = 0.5 * pi;
+    b.vx = n * a * esinf / efac;
+    b.vy = n * (1.0 + ecosf) * (1.0 + ecosf) / efac / efac / efac;
+    b.vz = 0.0;
+}
+
+// ------------------------------------------------------------ x sweep, gas
+// One block = one tile of `tile_x` interior cells of one azimuthal row, plus
+// 3 ghost/halo cells each side, staged in LDS.  Phases (barrier between):
+// load -> flatten -> parabolas -> interface fluxes -> conservative update.
+template <typename real>
+__global__ __launch_bounds__(256) void k_sweep_x_gas(DiskArgs<real> d) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    real* sm = reinterpret_cast<real*>(smem_raw);
+    const int nx = d.nx;
+    const size_t N = (size_t)nx * d.ny;
+    const int TX = d.tile_x;
+    const int j = blockIdx.y;
+    const int x0 = blockIdx.x * TX;
+    const int nt = min(TX, nx - x0);
+    const int ns = nt + 2 * NPAD;
+    const int S = TX + 2 * NPAD;  // LDS array stride
+    real* s_r = sm;
+    real* s_p = sm + S;
+    real* s_u = sm + 2 * S;
+    real* s_L = sm + 3 * S;
+    real* s_f = sm + 4 * S;                 // flattening, later Fr
+    real* s_sl = sm + 5 * S;                // 4 fields: r, p, u, L
+    real* s_sr = sm + 9 * S;
+    real* s_Fu = sm + 13 * S;
+    real* s_FL = sm + 14 * S;
+    real* s_pr = sm + 15 * S;
+    real* s_Fr = sm + 16 * S;
+    real* s_vp = sm + 17 * S;
+    const real dt = (real)d.ctl->dt;
+    const real gam = d.gam;
+    const real* __restrict__ in = d.gas;
+
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        int i = x0 - NPAD + s;
+        int ip = i + NPAD;
+        real r, u, vp;
+        if (i < 0 || i >= nx) {
+            r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); vp = TAB(T_G_V, ip);
+        } else {
+            size_t o = (size_t)j * nx + i;
+            r = in[o]; u = in[N + o]; vp = in[2 * N + o];
+        }
+        s_r[s] = r;
+        s_p[s] = TAB(T_CS2, ip) * r;
+        s_u[s] = u;
+        s_vp[s] = vp;
+        s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x + 2; s < ns - 2; s += blockDim.x) {
+        real cs = sqrt(gam * s_p[s] / s_r[s]);
+        s_f[s] = flatten5(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2],
+                          s_u[s - 1], s_u[s], s_u[s + 1], cs);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x + 2; s < ns - 2; s += blockDim.x) {
+        int ip = x0 + s;
+        real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
+        real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
+        real fl = s_f[s];
+        const real* fields[4] = {s_r, s_p, s_u, s_L};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const real* a = fields[f];
+            real l, r;
+            slopes(a[s - 2], a[s - 1], a[s], a[s + 1], a[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+            s_sl[f * S + s] = l;
+            s_sr[f * S + s] = r;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x + NPAD; k < ns - 2; k += blockDim.x) {
+        int ipk = x0 + k;
+        real rl, pl, ul, vl, rr, pr, ur, vr;
+        trace_left(gam, dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_p[k - 1], s_u[k - 1],
+                   s_sl[k - 1], s_sr[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
+                   s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
+                   s_sl[3 * S + k - 1], s_L[k - 1], s_sr[3 * S + k - 1], rl, pl, ul, vl);
+        trace_right(gam, dt, TAB(T_IDX, ipk), s_r[k], s_p[k], s_u[k],
+                    s_sl[k], s_sr[k], s_sl[S + k], s_sr[S + k], s_sl[2 * S + k], s_sr[2 * S + k],
+                    s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, pr, ur, vr);
+        real Fr, Fu, FL, pres;
+        hllc_iso(gam, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, FL, pres);
+        real g = TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
+        s_Fr[k] = Fr * g;
+        s_Fu[k] = Fu * g;
+        s_FL[k] = FL * g;
+        s_pr[k] = pres;
+    }
+    __syncthreads();
+    real* __restrict__ out = d.gasT;
+    for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += blockDim.x) {
+        int ip = x0 + s;
+        int i = ip - NPAD;
+        real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
+        real f = dt / TAB(T_XV, ip);
+        real R = TAB(T_XC, ip);
+        real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
+        real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1])) * f;
+        real DL = (s_FL[s] - s_FL[s + 1]) * f;
+        real r = s_r[s], u = s_u[s], vp = s_vp[s];
+        size_t o = (size_t)j * nx + i;
+        real rn = r + Dr;
+        real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
+        real un = u, vn = vp;
+        if (rn < rmin) {
+            rn = rmin;
+        } else {
+            un = (r * u + Du) / rn;
+            vn = (r * vp + (DL / R - TAB(T_VBG, ip) * Dr)) / rn;
+        }
+        out[o] = rn;
+        out[N + o] = un;
+        out[2 * N + o] = vn;
+    }
+}
+
+// ----------------------------------------------------------- x sweep, dust
+template <typename real>
+__global__ __launch_bounds__(256) void k_sweep_x_dust(DiskArgs<real> d) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    real* sm = reinterpret_cast<real*>(smem_raw);
+    const int nx = d.nx;
+    const size_t N = (size_t)nx * d.ny;
+    const int TX = d.tile_x;
+    const int j = blockIdx.y;
+    const int x0 = blockIdx.x * TX;
+    const int nt = min(TX, nx - x0);
+    const int ns = nt + 2 * NPAD;
+    const int S = TX + 2 * NPAD;
+    real* s_r = sm;
+    real* s_u = sm + S;
+    real* s_L = sm + 2 * S;
+    real* s_sl = sm + 3 * S;  // 3 fields: r, u, L
+    real* s_sr = sm + 6 * S;
+    real* s_Fr = sm + 9 * S;
+    real* s_Fu = sm + 10 * S;
+    real* s_FL = sm + 11 * S;
+    real* s_vp = sm + 12 * S;
+    const real dt = (real)d.ctl->dt;
+    const real* __restrict__ in = d.dust;
+
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        int i = x0 - NPAD + s;
+        int ip = i + NPAD;
+        real r, u, vp;
+        if (i < 0 || i >= nx) {
+            r = TAB(T_D_R, ip); u = TAB(T_D_U, ip); vp = TAB(T_D_V, ip);
+        } else {
+            size_t o = (size_t)j * nx + i;
+            r = in[o]; u = in[N + o]; vp = in[2 * N + o];
+        }
+        s_r[s] = r;
+        s_u[s] = u;
+        s_vp[s] = vp;
+        s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x + 2; s < ns - 2; s += blockDim.x) {
+        int ip = x0 + s;
+        real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
+        real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
+        const real* fields[3] = {s_r, s_u, s_L};
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const real* a = fields[f];
+            real l, r;
+            slopes(a[s - 2], a[s - 1], a[s], a[s + 1], a[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            s_sl[f * S + s] = l;
+            s_sr[f * S + s] = r;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x + NPAD; k < ns - 2; k += blockDim.x) {
+        int ipk = x0 + k;
+        real Fr, Fu, FL;
+        dust_iface(dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_sl[k - 1], s_sr[k - 1],
+                   s_u[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
+                   s_L[k - 1], s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
+                   TAB(T_IDX, ipk), s_r[k], s_sl[k], s_sr[k], s_u[k], s_sl[S + k], s_sr[S + k],
+                   s_L[k], s_sl[2 * S + k], s_sr[2 * S + k], Fr, Fu, FL);
+        real g = TAB(T_XA, ipk);
+        s_Fr[k] = Fr * g;
+        s_Fu[k] = Fu * g;
+        s_FL[k] = FL * g;
+    }
+    __syncthreads();
+    real* __restrict__ out = d.dustT;
+    for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += blockDim.x) {
+        int ip = x0 + s;
+        int i = ip - NPAD;
+        real f = dt / TAB(T_XV, ip);
+        real R = TAB(T_XC, ip);
+        real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
+        real Du = (s_Fu[s] - s_Fu[s + 1]) * f;
+        real DL = (s_FL[s] - s_FL[s + 1]) * f;
+        size_t o = (size_t)j * nx + i;
+        real r = s_r[s], u = s_u[s], vp = s_vp[s];
+        real rn = r + Dr;
+        real rmin = fmax(r * real(1.0e-10), d.D_G_ratio * real(1.0e-10));
+        real un = u, vn = vp;
+        if (rn < rmin) {
+            rn = rmin;
+        } else {
+            un = (r * u + Du) / rn;
+            vn = (r * vp + (DL / R - TAB(T_VBG, ip) * Dr)) / rn;
+        }
+        out[o] = rn;
+        out[N + o] = un;
+        out[2 * N + o] = vn;
+    }
+}
+
+// ------------------------------------------------------- azimuthal ring
+// One block = one radius.  Stage A sweeps the ring in chunks of blockDim
+// cells (periodic halo) and writes each updated cell to its orbitally
+// shifted position (j + inc) mod ny of an LDS ring; stage B advects that
+// ring by the residual speed and writes the result to HBM.
+template <typename real>
+struct RingLds {
+    real* ring;  // 3*ny
+    real *c0, *c1, *c2;   // chunk state (slots)
+    real* fl;
+    real *sl0, *sr0, *sl1, *sr1, *sl2, *sr2;
+    real *F0, *F1, *F2, *F3;
+};
+
+template <typename real>
+__device__ __forceinline__ RingLds<real> ring_lds(real* sm, int ny, int S) {
+    RingLds<real> L;
+    L.ring = sm;
+    real* b = sm + 3 * ny;
+    L.c0 = b; L.c1 = b + S; L.c2 = b + 2 * S; L.fl = b + 3 * S;
+    L.sl0 = b + 4 * S; L.sr0 = b + 5 * S; L.sl1 = b + 6 * S; L.sr1 = b + 7 * S;
+    L.sl2 = b + 8 * S; L.sr2 = b + 9 * S;
+    L.F0 = b + 10 * S; L.F1 = b + 11 * S; L.F2 = b + 12 * S; L.F3 = b + 13 * S;
+    return L;
+}
+
+__device__ __forceinline__ int wrap(int j, int n) {
+    j %= n;
+    return j < 0 ? j + n : j;
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_ring_gas(DiskArgs<real> d) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    real* sm = reinterpret_cast<real*>(smem_raw);
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int B = blockDim.x;
+    const int S = B + 2 * NPAD;
+    RingLds<real> L = ring_lds(sm, ny, S);
+    const int i = blockIdx.x;
+    const int ip = i + NPAD;
+    const real R = TAB(T_XC, ip);
+    const real cs2 = TAB(T_CS2, ip);
+    const real gam = d.gam;
+    const real dt = (real)d.ctl->dt;
+    const real dts = dt / R;          // set_state receives dt/rad (riemann.cu:111)
+    const real idy = real(1) / d.dy;
+    const real fup = dt / d.dy / R;   // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
+    const real cs0 = sqrt(gam * cs2);
+    const int inc = d.orbadv ? d.inc[i] : 0;
+    const real* __restrict__ in = d.gas;
+    const real wl0 = d.wLy[0], wl1 = d.wLy[1], wl2 = d.wLy[2];
+    const real wr0 = d.wRy[0], wr1 = d.wRy[1], wr2 = d.wRy[2];
+
+    // ---- stage A: azimuthal sweep (rho, vp normal, u transverse) ----
+    for (int c0 = 0; c0 < ny; c0 += B) {
+        const int nt = min(B, ny - c0);
+        const int ns = nt + 2 * NPAD;
+        for (int s = threadIdx.x; s < ns; s += B) {
+            int jj = wrap(c0 - NPAD + s, ny);
+            size_t o = (size_t)jj * nx + i;
+            L.c0[s] = in[o];            // rho
+            L.c1[s] = in[2 * N + o];    // vp: azimuthal velocity in the frame of rot + R*Omega_f
+            L.c2[s] = in[N + o];        // u: radial velocity (transverse here)
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
+            // under p = c_s^2 rho with c_s^2 fixed on the ring, flatten's
+            // pressure ratios equal the density ratios
+            L.fl[s] = flatten5(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2],
+                               L.c1[s - 1], L.c1[s], L.c1[s + 1], cs0);
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
+            real fl = L.fl[s], l, r;
+            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+            L.sl0[s] = l; L.sr0[s] = r;
+            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+            L.sl1[s] = l; L.sr1[s] = r;
+            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+            L.sl2[s] = l; L.sr2[s] = r;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
+            real rl, pl, ul, vl, rr, pr, ur, vr;
+            // the pressure parabola is c_s^2 times the density parabola
+            trace_left(gam, dts, idy, L.c0[k - 1], cs2 * L.c0[k - 1], L.c1[k - 1],
+                       L.sl0[k - 1], L.sr0[k - 1], cs2 * L.sl0[k - 1], cs2 * L.sr0[k - 1],
+                       L.sl1[k - 1], L.sr1[k - 1], L.sl2[k - 1], L.c2[k - 1], L.sr2[k - 1], rl, pl, ul, vl);
+            trace_right(gam, dts, idy, L.c0[k], cs2 * L.c0[k], L.c1[k],
+                        L.sl0[k], L.sr0[k], cs2 * L.sl0[k], cs2 * L.sr0[k],
+                        L.sl1[k], L.sr1[k], L.sl2[k], L.c2[k], L.sr2[k], rr, pr, ur, vr);
+            real Fr, Fu, Fw, pres;
+            hllc_iso(gam, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, Fw, pres);
+            L.F0[k] = Fr; L.F1[k] = Fu; L.F2[k] = Fw; L.F3[k] = pres;
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
+            int jj = c0 - NPAD + s;
+            real Dr = (L.F0[s] - L.F0[s + 1]) * fup;
+            real Du = (L.F1[s] - L.F1[s + 1] + (L.F3[s] - L.F3[s + 1])) * fup;
+            real Dw = (L.F2[s] - L.F2[s + 1]) * fup;
+            real r = L.c0[s], vp = L.c1[s], u = L.c2[s];
+            real rn = r + Dr;
+            real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
+            real vn = vp, un = u;
+            if (!d.sweep_on) {
+                rn = r;
+            } else if (rn < rmin) {
+                rn = rmin;
+            } else {
+                // (rho v + Du + (rot + R Omega_f) Dr) / rho' - vbg == (rho vp + Du) / rho'
+                vn = (r * vp + Du) / rn;
+                un = (r * u + Dw) / rn;
+            }
+            int js = wrap(jj + inc, ny);
+            L.ring[js] = rn;
+            L.ring[ny + js] = vn;
+            L.ring[2 * ny + js] = un;
+        }
+        __syncthreads();
+    }
+    real* __restrict__ out = d.gasT;
+    if (!d.orbadv) {
+        for (int jj = threadIdx.x; jj < ny; jj += B) {
+            size_t o = (size_t)jj * nx + i;
+            out[o] = L.ring[jj]; out[N + o] = L.ring[2 * ny + jj]; out[2 * N + o] = L.ring[ny + jj];
+        }
+        return;
+    }
+    // ---- stage B: residual advection at speed res (advection.cu:1-105) ----
+    const real speed = d.res[i];
+    const real frac = fabs(speed) * dts * idy;   // |speed| dt / (R dy)
+    const real fad = dt / d.dy / R;              // dt/yv then 1/rad (advection.cu:154-158)
+    for (int c0 = 0; c0 < ny; c0 += B) {
+        const int nt = min(B, ny - c0);
+        const int ns = nt + 2 * NPAD;
+        for (int s = threadIdx.x; s < ns; s += B) {
+            int jj = wrap(c0 - NPAD + s, ny);
+            L.c0[s] = L.ring[jj];
+            L.c1[s] = L.ring[ny + jj];
+            L.c2[s] = L.ring[2 * ny + jj];
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
+            real l, r;
+            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl0[s] = l; L.sr0[s] = r;
+            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl1[s] = l; L.sr1[s] = r;
+            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl2[s] = l; L.sr2[s] = r;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
+            real Dr, Dv, Du;
+            if (speed > real(0)) {
+                int c = k - 1;
+                real x = lim01(frac);
+                Dr = aveR(x, L.sl0[c], L.c0[c], L.sr0[c]);
+                Dv = aveR(x, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
+                Du = aveR(x, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
+            } else {
+                int c = k;
+                real q = lim01(frac);
+                Dr = aveL(q, L.sl0[c], L.c0[c], L.sr0[c]);
+                Dv = aveL(q, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
+                Du = aveL(q, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
+            }
+            L.F0[k] = Dr * speed; L.F1[k] = Dv * speed; L.F2[k] = Du * speed;
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
+            int jj = c0 - NPAD + s;
+            real Dr = (L.F0[s] - L.F0[s + 1]) * fad;
+            real Dv = (L.F1[s] - L.F1[s + 1]) * fad;
+            real Du = (L.F2[s] - L.F2[s + 1]) * fad;
+            real r = L.c0[s];
+            real rn = r + Dr;   // update_Cell_advect has no floor
+            size_t o = (size_t)jj * nx + i;
+            out[o] = rn;
+            out[N + o] = (r * L.c2[s] + Du) / rn;
+            out[2 * N + o] = (r * L.c1[s] + Dv) / rn;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_ring_dust(DiskArgs<real> d) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    real* sm = reinterpret_cast<real*>(smem_raw);
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int B = blockDim.x;
+    const int S = B + 2 * NPAD;
+    RingLds<real> L = ring_lds(sm, ny, S);
+    const int i = blockIdx.x;
+    const int ip = i + NPAD;
+    const real R = TAB(T_XC, ip);
+    const real dt = (real)d.ctl->dt;
+    const real dts = dt / R;          // dust_flow: dt /= rad for geom > 2 (dust.cu:127)
+    const real idy = real(1) / d.dy;
+    const real f1 = dt / d.dy, f2 = real(1) / R;  // dust.cu:309-313
+    const int inc = d.orbadv ? d.inc[i] : 0;
+    const real* __restrict__ in = d.dust;
+    const real rfloor = d.D_G_ratio * real(1.0e-10);
+    const real wl0 = d.wLy[0], wl1 = d.wLy[1], wl2 = d.wLy[2];
+    const real wr0 = d.wRy[0], wr1 = d.wRy[1], wr2 = d.wRy[2];
+
+    for (int c0 = 0; c0 < ny; c0 += B) {
+        const int nt = min(B, ny - c0);
+        const int ns = nt + 2 * NPAD;
+        for (int s = threadIdx.x; s < ns; s += B) {
+            int jj = wrap(c0 - NPAD + s, ny);
+            size_t o = (size_t)jj * nx + i;
+            L.c0[s] = in[o];
+            L.c1[s] = in[2 * N + o];
+            L.c2[s] = in[N + o];
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
+            real l, r;
+            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl0[s] = l; L.sr0[s] = r;
+            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl1[s] = l; L.sr1[s] = r;
+            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl2[s] = l; L.sr2[s] = r;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
+            real Fr, Fu, Fw;
+            dust_iface(dts, idy, L.c0[k - 1], L.sl0[k - 1], L.sr0[k - 1], L.c1[k - 1], L.sl1[k - 1],
+                       L.sr1[k - 1], L.c2[k - 1], L.sl2[k - 1], L.sr2[k - 1],
+                       idy, L.c0[k], L.sl0[k], L.sr0[k], L.c1[k], L.sl1[k], L.sr1[k],
+                       L.c2[k], L.sl2[k], L.sr2[k], Fr, Fu, Fw);
+            L.F0[k] = Fr; L.F1[k] = Fu; L.F2[k] = Fw;
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
+            int jj = c0 - NPAD + s;
+            real Dr = (L.F0[s] - L.F0[s + 1]) * f1 * f2;
+            real Du = (L.F1[s] - L.F1[s + 1]) * f1 * f2;
+            real Dw = (L.F2[s] - L.F2[s + 1]) * f1 * f2;
+            real r = L.c0[s], vp = L.c1[s], u = L.c2[s];
+            real rn = r + Dr;
+            real rmin = fmax(r * real(1.0e-10), rfloor);
+            real vn = vp, un = u;
+            if (!d.sweep_on) {
+                rn = r;
+            } else if (rn < rmin) {
+                rn = rmin;
+            } else {
+                vn = (r * vp + Du) / rn;
+                un = (r * u + Dw) / rn;
+            }
+            int js = wrap(jj + inc, ny);
+            L.ring[js] = rn;
+            L.ring[ny + js] = vn;
+            L.ring[2 * ny + js] = un;
+        }
+        __syncthreads();
+    }
+    real* __restrict__ out = d.dustT;
+    if (!d.orbadv) {
+        for (int jj = threadIdx.x; jj < ny; jj += B) {
+            size_t o = (size_t)jj * nx + i;
+            out[o] = L.ring[jj]; out[N + o] = L.ring[2 * ny + jj]; out[2 * N + o] = L.ring[ny + jj];
+        }
+        return;
+    }
+    const real speed = d.res[i];
+    const real frac = fabs(speed) * dts * idy;
+    for (int c0 = 0; c0 < ny; c0 += B) {
+        const int nt = min(B, ny - c0);
+        const int ns = nt + 2 * NPAD;
+        for (int s = threadIdx.x; s < ns; s += B) {
+            int jj = wrap(c0 - NPAD + s, ny);
+            L.c0[s] = L.ring[jj];
+            L.c1[s] = L.ring[ny + jj];
+            L.c2[s] = L.ring[2 * ny + jj];
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
+            real l, r;
+            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl0[s] = l; L.sr0[s] = r;
+            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl1[s] = l; L.sr1[s] = r;
+            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+            L.sl2[s] = l; L.sr2[s] = r;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
+            real Dr, Dv, Du;
+            if (speed > real(0)) {
+                int c = k - 1;
+                real x = lim01(frac);
+                Dr = aveR(x, L.sl0[c], L.c0[c], L.sr0[c]);
+                Dv = aveR(x, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
+                Du = aveR(x, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
+            } else {
+                int c = k;
+                real q = lim01(frac);
+                Dr = aveL(q, L.sl0[c], L.c0[c], L.sr0[c]);
+                Dv = aveL(q, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
+                Du = aveL(q, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
+            }
+            L.F0[k] = Dr * speed; L.F1[k] = Dv * speed; L.F2[k] = Du * speed;
+        }
+        __syncthreads();
+        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
+            int jj = c0 - NPAD + s;
+            real Dr = (L.F0[s] - L.F0[s + 1]) * f1 * f2;
+            real Dv = (L.F1[s] - L.F1[s + 1]) * f1 * f2;
+            real Du = (L.F2[s] - L.F2[s + 1]) * f1 * f2;
+            real r = L.c0[s];
+            real rn = r + Dr;
+            real rmin = fmax(r * real(1.0e-10), rfloor);
+            real vn = L.c1[s], un = L.c2[s];
+            if (rn < rmin) {
+                rn = rmin;
+            } else {
+                vn = (r * L.c1[s] + Dv) / rn;
+                un = (r * L.c2[s] + Du) / rn;
+            }
+            size_t o = (size_t)jj * nx + i;
+            out[o] = rn;
+            out[N + o] = un;
+            out[2 * N + o] = vn;
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------- viscosity
+// 9-point stencil of viscous_forces (viscosity.cu:34-197, ndim 2).  The
+// derivatives of v_phi = vbg(R) + vp are split into the tabulated
+// background part and the stored deviation (the difference operators are
+// linear, so this is the same arithmetic rearranged).
+template <typename real>
+__global__ __launch_bounds__(256) void k_viscosity(DiskArgs<real> d, const real* __restrict__ st) {
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nx) return;
+    const int ip = i + NPAD;
+    real r[3], nu[3];
+    for (int n = 0; n < 3; ++n) { r[n] = TAB(T_XC, ip - 1 + n); nu[n] = TAB(T_NU, ip - 1 + n); }
+    real nr[9], u[9], v[9];
+    const int jm = wrap(j - 1, ny), jp = wrap(j + 1, ny);
+    const int rows[3] = {j, jm, jp};
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int n = 3 * b + a;
+            int ii = i - 1 + a;
+            real rr, uu, vv;
+            if (ii < 0 || ii >= nx) {
+                rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
+            } else {
+                size_t o = (size_t)rows[b] * nx + ii;
+                rr = st[o]; uu = st[N + o]; vv = st[2 * N + o];
+            }
+            nr[n] = rr * nu[a];
+            u[n] = uu;
+            v[n] = vv;  // deviation vp
+        }
+    }
+    const real dy = d.dy;
+    const real r1 = r[1];
+    const real dr20 = r[2] - r[0], dr21 = r[2] - r[1], dr10 = r[1] - r[0];
+    const real dt20 = real(2) * dy;
+    // radial derivatives
+    real dudx = (u[2] - u[0]) / dr20;
+    real d2udx2 = ((u[2] - u[1]) / dr21 - (u[1] - u[0]) / dr10) / (real(0.5) * dr20);
+    real dvdx = TAB(T_DVBG, ip) + (v[2] - v[0]) / dr20;
+    real d2vdx2 = TAB(T_D2VBG, ip) + ((v[2] - v[1]) / dr21 - (v[1] - v[0]) / dr10) / (real(0.5) * dr20);
+    real dndx = (nr[2] - nr[0]) / dr20;
+    // azimuthal and mixed derivatives
+    real dvdy = (v[7] - v[4]) / dt20 / r1;
+    real d2vdy2 = ((v[7] - v[1]) / dy - (v[1] - v[4]) / dy) / (real(0.5) * dt20) / r1 / r1;
+    real d2vdxy = (v[8] - v[6] - v[5] + v[3]) / dr20 / dt20 / r1;
+    real dudy = (u[7] - u[4]) / dt20 / r1;
+    real d2udy2 = ((u[7] - u[1]) / dy - (u[1] - u[4]) / dy) / (real(0.5) * dt20) / r1 / r1;
+    real d2udxy = (u[8] - u[6] - u[5] + u[3]) / dr20 / dt20 / r1;
+    real dndy = (nr[7] - nr[4]) / dt20 / r1;
+    const real v1 = TAB(T_VBG, ip) + v[1];
+    const real n1 = nr[1];
+    real sxx = real(2) * n1 * dudx;
+    real fx = real(2) * dndx * dudx + real(2) * n1 * d2udx2 + sxx / r1;
+    real syy = real(2) * n1 * (dvdy + u[1] / r1);
+    real sxy = n1 * (dudy + dvdx - v1 / r1);
+    real fy = real(2) * dndy * dvdy + real(2) * n1 * d2vdy2 + real(2) * n1 * dudy / r1 + real(2) * dndy * u[1] / r1;
+    fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 / r1 - n1 * dvdy / r1 - syy / r1;
+    fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 / r1 - n1 * dvdx / r1 + n1 * v1 / r1 / r1 +
+          real(2) * sxy / r1;
+    size_t o = (size_t)j * nx + i;
+    d.vis[o] = fx;
+    d.vis[N + o] = fy;
+}
+
+// ------------------------------------------------------------- sources
+// Planet acceleration (star_planet_grav_cyl, forces.cu:41-58, z = 0).
+template <typename real>
+__device__ __forceinline__ void planet_acc(const Body& b, real R, real phi, real& ax, real& ay) {
+    real dphi = phi - (real)b.y;
+    real s = sin(dphi), c = cos(dphi), h = sin(real(0.5) * dphi);
+    real px = (real)b.x, m = (real)b.m, rs = (real)b.rs;
+    // R^2 + px^2 - 2 px R cos(dphi) + rs^2, written without cancellation
+    real Rp2 = (R - px) * (R - px) + real(4) * px * R * h * h + rs * rs;
+    real Rp = sqrt(Rp2);
+    real k = real(1) / (Rp * Rp * Rp);
+    real ind = m / (px * px);
+    ax = -m * (R - px * c) * k - c * ind;
+    ay = -m * px * s * k + s * ind;
+}
+
+// Warp/block max of a non-negative value, then one atomic per block.
+template <typename real>
+__device__ __forceinline__ void block_max_to(unsigned long long* dst, real v) {
+    double x = (double)v;
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    __shared__ double wmax[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wmax[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) m = fmax(m, wmax[k]);
+        atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// Pointwise source terms.  mode: 0 first half kick (planet at t, in place),
+// 1 predictor (planet at t+dt, gas written to gasT), 2 corrector (planet at
+// t+dt, in place) optionally followed by wave killing and the CFL rate,
+// 3 wave killing alone.
+template <typename real>
+__global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int do_kill, int do_rate) {
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    real rate = real(0);
+    if (i < nx) {
+        const int ip = i + NPAD;
+        const size_t o = (size_t)j * nx + i;
+        const double hdt = 0.5 * d.ctl->dt;
+        const real dt = (real)hdt;
+        const real mdt = mode == 0 ? real(0) : (real)hdt;
+        const Body& b = mode == 0 ? d.ctl->planet0 : d.ctl->planet1;
+        const real R = TAB(T_XC, ip);
+        const real vbg = TAB(T_VBG, ip);
+        const real ebal = TAB(T_EBAL, ip);
+        const real phi = (real)(d.dy_d * ((double)j + 0.5));
+        real px = real(0), py = real(0);
+        if (d.n_planet > 0) planet_acc(b, R, phi, px, py);
+        const real* g = d.gas;
+        real rho = g[o], cu = g[N + o], cvp = g[2 * N + o];
+        real vx = real(0), vy = real(0);
+        if (d.visc) { vx = d.vis[o] / rho; vy = d.vis[N + o] / rho; }
+        const real ay = vy + py;               // fy + gy, independent of the state
+        real Tu = cu, Tvp = cvp;
+        real Qr = 0, Qu = 0, Qvp = 0, Du = 0, Dvp = 0, ts = 0;
+        if (d.has_dust) {
+            Qr = d.dust[o]; Qu = d.dust[N + o]; Qvp = d.dust[2 * N + o];
+            Du = Qu; Dvp = Qvp;
+            ts = d.Stokes / (sqrt(d.gam * TAB(T_CS2, ip)) / d.sc_h) / rho;   // dust.cu:1-4
+        }
+        // fx + gx at azimuthal velocity vbg + vp: centrifugal - stellar
+        // gravity (tabulated at vbg) + deviation part + viscosity + planet
+        auto axd = [&](real vp) { return ebal + vp * (real(2) * vbg + vp) / R + px; };
+        auto axg = [&](real vp) { return axd(vp) + vx; };
+        if (mode != 0) {
+            Tu = cu + axg(cvp) * mdt;
+            Tvp = cvp + ay * mdt;
+            if (d.has_dust) {
+                real dtau = -expm1(-mdt / ts);
+                Du = Qu + (axd(Dvp) * ts + cu - Qu) * dtau;
+                Dvp = Qvp + ((cvp - Qvp) + py * ts) * dtau;
+            }
+        }
+        real nu_ = cu + axg(Tvp) * dt;
+        real nvp = cvp + ay * dt;
+        real nr_ = rho;
+        if (mode == 3) { nu_ = cu; nvp = cvp; }   // wave killing alone
+        if (mode == 1) {
+            d.gasT[o] = nr_; d.gasT[N + o] = nu_; d.gasT[2 * N + o] = nvp;
+        } else {
+            real ndu = Qu, ndvp = Qvp;
+            if (d.has_dust && mode != 3) {
+                real dtau = -expm1(-dt / ts);
+                ndu = Qu + (axd(Dvp) * ts + Tu - Qu) * dtau;
+                ndvp = Qvp + ((Tvp - Qvp) + py * ts) * dtau;
+            }
+            if (do_kill) {   // killwave on the corrected state (solve.cu:287-289)
+                real k = TAB(T_KILL, ip) * (real)d.ctl->dt;
+                nr_ += k * (TAB(T_G_R, ip) - nr_);
+                nu_ += k * (TAB(T_G_U, ip) - nu_);
+                nvp += k * (real(0) - nvp);
+            }
+            d.gas[o] = nr_; d.gas[N + o] = nu_; d.gas[2 * N + o] = nvp;
+            if (d.has_dust) { d.dust[N + o] = ndu; d.dust[2 * N + o] = ndvp; }
+            if (do_rate) {
+                real cs = sqrt(d.gam * TAB(T_CS2, ip));
+                real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
+                if (d.has_dust) { sx = fmax(fabs(ndu), sx); sy = fmax(fabs(ndvp), sy); }
+                rate = fmax(sx * TAB(T_IDX, ip), sy / (d.dy * R));
+            }
+        }
+    }
+    if (do_rate) block_max_to(&d.ctl->rate_bits, rate);
+}
+
+// CFL rate of the current state (get_dt_lv1, timestep.cu:18-97).
+template <typename real>
+__global__ __launch_bounds__(256) void k_cfl(DiskArgs<real> d) {
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    real rate = real(0);
+    if (i < nx) {
+        const int ip = i + NPAD;
+        const size_t o = (size_t)j * nx + i;
+        real cs = sqrt(d.gam * TAB(T_CS2, ip));
+        real sx = fabs(d.gas[N + o]) + cs, sy = fabs(d.gas[2 * N + o]) + cs;
+        if (d.has_dust) { sx = fmax(fabs(d.dust[N + o]), sx); sy = fmax(fabs(d.dust[2 * N + o]), sy); }
+        rate = fmax(sx * TAB(T_IDX, ip), sy / (d.dy * TAB(T_XC, ip)));
+    }
+    block_max_to(&d.ctl->rate_bits, rate);
+}
+
+// Step prologue, one block.  mode 0: t and dt were set by the host.
+// mode 1 (device loop): finish the previous step (t += dt), take
+// dt = min(CFL/rate, 1.1 dt_prev) clipped to end_time, record it.  Then the
+// planet at t and t+dt and the orbital shift of every radius.
+template <typename real>
+__global__ __launch_bounds__(256) void k_prologue(DiskArgs<real> d, int mode, double* dts) {
+    StepCtl* ctl = d.ctl;
+    __shared__ double s_dt;
+    if (threadIdx.x == 0) {
+        double t = ctl->t, dt = ctl->dt;
+        if (mode == 1) {
+            if (ctl->nstep > 0) t += dt;
+            double rate = __longlong_as_double((long long)ctl->rate_bits);
+            double dtn = d.CFL / rate;
+            double cap = dt * 1.1;
+            dtn = dtn < cap ? dtn : cap;
+            if (t + dtn > ctl->end_time) dtn = ctl->end_time - t;
+            dt = dtn;
+            ctl->t = t;
+            ctl->dt = dt;
+            ctl->dt_prev = dt;
+            if (dts) dts[ctl->nstep] = dt;
+            ctl->nstep += 1;
+            ctl->rate_bits = 0ull;
+        }
+        Body b0 = ctl->planet0;
+        orbit_solution(b0, t, d.ramp_time, d.planet_mass, (double)d.frame_omega);
+        ctl->planet0 = b0;
+        Body b1 = ctl->planet1;
+        orbit_solution(b1, t + dt, d.ramp_time, d.planet_mass, (double)d.frame_omega);
+        ctl->planet1 = b1;
+        s_dt = dt;
+    }
+    __syncthreads();
+    if (d.orbadv) {
+        const double dt = s_dt;
+        for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
+            // set_orbital_advection (orbit.cu:74-99), FP64
+            double dy = d.dy_d * d.rot[d.nx + i];
+            double rt = d.rot[i];
+            int shift = (int)floor(rt * dt / dy + 0.5);
+            double speed = shift * dy / dt;
+            d.inc[i] = shift;
+            d.res[i] = (real)(rt - speed);
+        }
+    }
+}
+
+// ------------------------------------------------------------- launchers
+template <typename real>
+struct Launch {
+    static hipError_t sweep_x_gas(const DiskArgs<real>& d, hipStream_t s) {
+        dim3 grid((d.nx + d.tile_x - 1) / d.tile_x, d.ny);
+        size_t lds = (size_t)18 * (d.tile_x + 2 * NPAD) * sizeof(real);
+        hipLaunchKernelGGL(k_sweep_x_gas<real>, grid, dim3(256), lds, s, d);
+        return hipGetLastError();
+    }
+    static hipError_t sweep_x_dust(const DiskArgs<real>& d, hipStream_t s) {
+        dim3 grid((d.nx + d.tile_x - 1) / d.tile_x, d.ny);
+        size_t lds = (size_t)13 * (d.tile_x + 2 * NPAD) * sizeof(real);
+        hipLaunchKernelGGL(k_sweep_x_dust<real>, grid, dim3(256), lds, s, d);
+        return hipGetLastError();
+    }
+    static size_t ring_lds(int ny, int B) { return ((size_t)3 * ny + (size_t)14 * (B + 2 * NPAD)) * sizeof(real); }
+    static hipError_t ring_gas(const DiskArgs<real>& d, hipStream_t s) {
+        const int B = 256;
+        size_t lds = ring_lds(d.ny, B);
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_ring_gas<real>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_ring_gas<real>, dim3(d.nx), dim3(B), lds, s, d);
+        return hipGetLastError();
+    }
+    static hipError_t ring_dust(const DiskArgs<real>& d, hipStream_t s) {
+        const int B = 256;
+        size_t lds = ring_lds(d.ny, B);
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_ring_dust<real>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_ring_dust<real>, dim3(d.nx), dim3(B), lds, s, d);
+        return hipGetLastError();
+    }
+    static hipError_t viscosity(const DiskArgs<real>& d, const real* st, hipStream_t s) {
+        dim3 grid((d.nx + 255) / 256, d.ny);
+        hipLaunchKernelGGL(k_viscosity<real>, grid, dim3(256), 0, s, d, st);
+        return hipGetLastError();
+    }
+    static hipError_t sources(const DiskArgs<real>& d, int mode, int kill, int rate, hipStream_t s) {
+        dim3 grid((d.nx + 255) / 256, d.ny);
+        hipLaunchKernelGGL(k_sources<real>, grid, dim3(256), 0, s, d, mode, kill, rate);
+        return hipGetLastError();
+    }
+    static hipError_t cfl(const DiskArgs<real>& d, hipStream_t s) {
+        dim3 grid((d.nx + 255) / 256, d.ny);
+        hipLaunchKernelGGL(k_cfl<real>, grid, dim3(256), 0, s, d);
+        return hipGetLastError();
+    }
+};
+
+template struct Launch<float>;
+template struct Launch<double>;
+
+template <typename real>
+hipError_t launch_prologue(const DiskArgs<real>& d, int mode, double* dts, hipStream_t s) {
+    hipLaunchKernelGGL(k_prologue<real>, dim3(1), dim3(256), 0, s, d, mode, dts);
+    return hipGetLastError();
+}
+template hipError_t launch_prologue<float>(const DiskArgs<float>&, int, double*, hipStream_t);
+template hipError_t launch_prologue<double>(const DiskArgs<double>&, int, double*, hipStream_t);
+
+}  // namespace pg

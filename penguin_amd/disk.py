@@ -1,0 +1,210 @@
+"""ctypes binding of libpenguin_amd.so (include/penguin_amd.h) and the host
+driver that mirrors the reference's main loop.
+
+``Disk`` plays the role of the reference's ``Grid* dev`` plus the loop in
+main.cu:320-359: ``global_dt`` -> ``solve`` -> ``t += dt``.  The numerics run
+in the HIP library; this module only moves arguments across the C ABI.
+The library must have been built (``python -m penguin_amd.build``); there is
+no fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import Params
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpenguin_amd.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes) of every exported entry point of penguin_amd.h
+SIGNATURES = {
+    "pg_last_error": (ctypes.c_char_p, []),
+    "pg_version": (ctypes.c_char_p, []),
+    "pg_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(_vp)]),
+    "pg_destroy": (ctypes.c_int, [_vp]),
+    "pg_get_state": (ctypes.c_int, [_vp, _dp, _dp]),
+    "pg_set_state": (ctypes.c_int, [_vp, _dp, _dp]),
+    "pg_global_dt": (ctypes.c_int, [_vp, ctypes.c_double, _dp]),
+    "pg_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "pg_run": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
+    "pg_viscosity_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pg_sources": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "pg_sweep_x": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "pg_sweep_y": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "pg_sweep_x_dust": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "pg_sweep_y_dust": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "pg_orbital_advection": (ctypes.c_int, [_vp, ctypes.c_double, _ip, _dp]),
+    "pg_killwave": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "pg_planet": (ctypes.c_int, [_vp, ctypes.c_double, _dp]),
+    "pg_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "pg_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pg_profile_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "pg_device_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+}
+
+
+class PenguinError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"penguin_amd error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library (built in-tree); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"{path} not found: build it with `python -m penguin_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise PenguinError(rc, load_library().pg_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+KERNELS = ("prologue", "viscosity", "sources_kick", "sweep_x_gas", "ring_gas", "sweep_x_dust",
+           "ring_dust", "sources_predictor", "sources_corrector", "cfl")
+
+
+class Disk:
+    """One disk on the GPU: the reference's Grid plus its time loop."""
+
+    def __init__(self, p: Params):
+        L = load_library()
+        self.p = p.copy()
+        h = _vp()
+        _check(L.pg_create(ctypes.byref(self.p), ctypes.byref(h)))
+        self._h = h
+        self.nx, self.ny = int(p.xres), int(p.yres)
+        self.t = 0.0
+        self.dt = float(p.end_time)   # main.cu:144: the first cap is end_time
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().pg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state -----------------------------------------------------------
+    def get_state(self):
+        """gas (5, ny, nx): rho, p, u, v, w; dust (4, ny, nx) or None."""
+        gas = np.zeros((5, self.ny, self.nx))
+        dust = np.zeros((4, self.ny, self.nx)) if self.p.dust_flag else None
+        _check(load_library().pg_get_state(self._h, _ptr(gas), _ptr(dust) if dust is not None else None))
+        return gas, dust
+
+    def set_state(self, gas=None, dust=None):
+        g = None if gas is None else np.ascontiguousarray(gas, dtype=np.float64)
+        d = None if dust is None else np.ascontiguousarray(dust, dtype=np.float64)
+        _check(load_library().pg_set_state(self._h, _ptr(g) if g is not None else None,
+                                           _ptr(d) if d is not None else None))
+
+    # ---- stepping (timestep.cu:113-140, solve.cu:279-292, main.cu:320-359)
+    def global_dt(self, dt_prev: float) -> float:
+        out = ctypes.c_double()
+        _check(load_library().pg_global_dt(self._h, dt_prev, ctypes.byref(out)))
+        return out.value
+
+    def solve(self, t: float, dt: float):
+        _check(load_library().pg_solve(self._h, t, dt))
+
+    def step_host(self) -> float:
+        """One iteration of the reference loop driven from the host."""
+        dt = self.global_dt(self.dt)
+        if self.t + dt > self.p.end_time:
+            dt = self.p.end_time - self.t
+        self.solve(self.t, dt)
+        self.t += dt
+        self.dt = dt
+        return dt
+
+    def run(self, nsteps: int, use_graph: bool = False) -> np.ndarray:
+        """nsteps of the loop on the device; returns the step sizes."""
+        dts = np.zeros(max(nsteps, 1))
+        t = ctypes.c_double(self.t)
+        dt = ctypes.c_double(self.dt)
+        _check(load_library().pg_run(self._h, nsteps, ctypes.byref(t), ctypes.byref(dt), _ptr(dts),
+                                     int(use_graph)))
+        self.t, self.dt = t.value, dt.value
+        return dts[:nsteps]
+
+    # ---- module-level entry points --------------------------------------
+    def viscosity(self, on_T: bool = False) -> np.ndarray:
+        v = np.zeros((2, self.ny, self.nx))
+        _check(load_library().pg_viscosity_eval(self._h, int(on_T), _ptr(v)))
+        return v
+
+    def sources(self, t_planet: float, mdt: float, dt: float, inplace: bool = True):
+        _check(load_library().pg_sources(self._h, t_planet, mdt, dt, int(inplace)))
+
+    def sweep_x(self, dt: float):
+        _check(load_library().pg_sweep_x(self._h, dt))
+
+    def sweep_y(self, dt: float):
+        _check(load_library().pg_sweep_y(self._h, dt))
+
+    def sweep_x_dust(self, dt: float):
+        _check(load_library().pg_sweep_x_dust(self._h, dt))
+
+    def sweep_y_dust(self, dt: float):
+        _check(load_library().pg_sweep_y_dust(self._h, dt))
+
+    def orbital_advection(self, dt: float):
+        inc = np.zeros(self.nx, dtype=np.int32)
+        res = np.zeros(self.nx)
+        _check(load_library().pg_orbital_advection(self._h, dt, inc.ctypes.data_as(_ip), _ptr(res)))
+        return inc, res
+
+    def killwave(self, dt: float):
+        _check(load_library().pg_killwave(self._h, dt))
+
+    def planet(self, t: float) -> np.ndarray:
+        b = np.zeros(10)
+        _check(load_library().pg_planet(self._h, t, _ptr(b)))
+        return b
+
+    # ---- introspection ---------------------------------------------------
+    def profile(self, on: bool = True):
+        _check(load_library().pg_profile_enable(self._h, int(on)))
+
+    def kernel_time_us(self, name: str):
+        avg = ctypes.c_double()
+        cnt = ctypes.c_int()
+        _check(load_library().pg_profile_get(self._h, name.encode(), ctypes.byref(avg), ctypes.byref(cnt)))
+        return avg.value, cnt.value
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        _check(load_library().pg_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
