@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s of the Godunov step on one MI355X.
+
+Workload: the 2D gas + pressureless dust disk, nr = 1024 x nphi = 3072, FP32
+storage (St 0.01, dust/gas 0.01, q = 3e-4, alpha = 1e-3, wave killing, 1e-4
+seeded noise on Sigma and v_r), synthetic initial state from the analytic
+profiles.  A step is one full DS step (viscosity, half kick, x sweeps,
+azimuthal sweeps + orbital advection, predictor/corrector kick, wave
+killing) with the CFL dt chosen on the device, exactly as the reference
+time loop does it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  `value` is interior cell updates per second over the
+K timed steps (state resident in HBM before timing starts).  `roofline`
+describes the dominant kernel: algorithmic bytes per launch over its average
+HIP-event duration in the timed region, against 8 TB/s.  `cpu_baseline`
+times the FP64 host oracle (OpenMP) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
+
+# algorithmic HBM bytes per interior cell per launch, in units of the storage
+# word: (fields read + fields written) that the kernel must move at minimum
+WORDS_PER_CELL = {
+    "sweep_x_gas": 6,        # rho,u,vp in and out
+    "ring_gas": 6,           # azimuthal sweep + orbital shift + residual advection
+    "sweep_x_dust": 6,
+    "ring_dust": 6,
+    "viscosity": 5,          # rho,u,vp in; two accelerations out
+    "sources_kick": 12,      # gas 3 + dust 3 + vis 2 in; gas u,vp + dust u,vp out
+    "sources_predictor": 8,  # gas 3 + vis 2 in; gas 3 out
+    "sources_corrector": 12,
+    "cfl": 6,
+    "prologue": 0,
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", type=int, default=4, choices=(4, 8))
+    ap.add_argument("--nr", type=int, default=1024)
+    ap.add_argument("--nphi", type=int, default=3072)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--graph", action="store_true", help="also time the hipGraph-replayed loop")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: the whole disk lives on one MI355X; --gpus {a.gpus} is not "
+              "supported (multi-GPU radial slabs are not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from penguin_amd import Disk, config_c2
+
+    p = config_c2(precision=a.precision, xres=a.nr, yres=a.nphi)
+    word = a.precision
+    cells = p.xres * p.yres
+    d = Disk(p)
+    d.run(a.warmup, use_graph=False)
+
+    # timed region: K device-loop steps, HIP events around every launch
+    d.profile(True)
+    t0 = time.perf_counter()
+    dts = d.run(a.steps, use_graph=False)
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = cells * a.steps / elapsed
+    kernels = {}
+    for name, words in WORDS_PER_CELL.items():
+        avg_us, cnt = d.kernel_time_us(name)
+        if cnt:
+            kernels[name] = {"avg_us": round(avg_us, 2), "launches": cnt,
+                             "GBps": round(words * word * cells / (avg_us * 1e-6) / 1e9, 1) if words else None}
+    d.profile(False)
+    dom = max((k for k in kernels if WORDS_PER_CELL[k]),
+              key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    bytes_launch = WORDS_PER_CELL[dom] * word * cells
+    achieved = bytes_launch / (kernels[dom]["avg_us"] * 1e-6) / 1e9
+    # full-step roofline: 3 read+write passes of 6 fields (x, phi+OA, sources)
+    step_bytes = 3 * 2 * 6 * word
+    graph_value = None
+    if a.graph:
+        d.run(2, use_graph=True)
+        g0 = time.perf_counter()
+        d.run(a.steps, use_graph=True)
+        graph_value = cells * a.steps / (time.perf_counter() - g0)
+
+    cpu = None
+    if not a.no_cpu and a.cpu_steps > 0:
+        import oracle
+        oracle.build()
+        o = oracle.Oracle(config_c2(precision=8, xres=a.nr, yres=a.nphi))
+        c0 = time.perf_counter()
+        o.run(a.cpu_steps)
+        cdt = time.perf_counter() - c0
+        cpu = {"value": round(cells * a.cpu_steps / cdt, 1), "unit": "cell-updates/s",
+               "cores": oracle.num_threads(), "kind": "port",
+               "sample": f"{a.cpu_steps} steps of the {a.nr}x{a.nphi} gas+dust disk from its initial state "
+                         f"(FP64 oracle, OpenMP over rows, {cdt:.1f} s)"}
+
+    out = {
+        "metric": "cell-updates/s on 1 GPU",
+        "value": round(value, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if a.precision == 4 else "fp64",
+        "data": "synthetic",
+        "config": {"workload": f"2D gas+dust disk {a.nr}x{a.nphi} (PPM4+HLLC, orbital advection, "
+                               f"viscosity, planet, wave killing), device-side CFL dt",
+                   "nr": a.nr, "nphi": a.nphi, "mean_dt": float(sum(dts) / len(dts))},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
+                     "bytes_per_launch": bytes_launch},
+        "step_roofline": {"bytes_per_cell_update": step_bytes,
+                          "roofline_cell_updates_per_s": PEAK_HBM / step_bytes,
+                          "frac": round(value / (PEAK_HBM / step_bytes), 4)},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    if graph_value is not None:
+        out["graph_value"] = round(graph_value, 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -25,7 +25,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 def build(force: bool = False) -> str:
     """Compile the oracle with g++/OpenMP (seconds)."""
     if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-s", "-B" if force else "-s", "_build/liboracle.so"],
+        subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["_build/liboracle.so"],
                        cwd=_HERE, check=True)
     return LIB_PATH
 

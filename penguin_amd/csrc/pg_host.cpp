@@ -218,10 +218,14 @@ struct pg_handle {
     double wly[3]{}, wry[3]{};
     Body planet_init{};
     bool rate_valid = false;
-    // profiling
+    // profiling: event pairs recorded around launches without a host wait,
+    // resolved after the stream drains
     bool profile = false;
     std::map<std::string, KernelTimer> timers;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kernel;   // kernel id of pair k (events 2k, 2k+1)
+    size_t ev_used = 0;
     // graph of one device-loop step, and the buffers it was captured with
     hipGraphExec_t graph = nullptr;
     void* graph_gas = nullptr;
@@ -443,18 +447,36 @@ const char* kname[] = {"prologue", "viscosity", "sources_kick", "sweep_x_gas", "
 enum KId { K_PRO, K_VIS, K_KICK, K_SXG, K_RG, K_SXD, K_RD, K_PRED, K_CORR, K_CFL };
 
 int timed_begin(pg_handle* h) {
-    if (h->profile) HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    if (!h->profile) return PG_OK;
+    size_t need = 2 * (h->ev_used + 1);
+    while (h->ev_pool.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        h->ev_pool.push_back(e);
+    }
+    HIP_TRY(hipEventRecord(h->ev_pool[2 * h->ev_used], h->stream));
     return PG_OK;
 }
 int timed_end(pg_handle* h, int k) {
     if (!h->profile) return PG_OK;
-    HIP_TRY(hipEventRecord(h->ev1, h->stream));
-    HIP_TRY(hipEventSynchronize(h->ev1));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    KernelTimer& t = h->timers[kname[k]];
-    t.total_ms += ms;
-    t.count += 1;
+    HIP_TRY(hipEventRecord(h->ev_pool[2 * h->ev_used + 1], h->stream));
+    if (h->ev_kernel.size() <= h->ev_used) h->ev_kernel.resize(h->ev_used + 1);
+    h->ev_kernel[h->ev_used] = k;
+    h->ev_used += 1;
+    return PG_OK;
+}
+// After the stream has drained: fold the recorded pairs into the timers.
+int timers_resolve(pg_handle* h) {
+    if (!h->profile) return PG_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (size_t k = 0; k < h->ev_used; ++k) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev_pool[2 * k], h->ev_pool[2 * k + 1]));
+        KernelTimer& t = h->timers[kname[h->ev_kernel[k]]];
+        t.total_ms += ms;
+        t.count += 1;
+    }
+    h->ev_used = 0;
     return PG_OK;
 }
 
@@ -615,6 +637,7 @@ int pg_destroy(pg_handle* h) {
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->ctl_host) hipHostFree(h->ctl_host);
+    for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -713,7 +736,7 @@ int pg_solve(pg_handle* h, double t, double dt) {
     if ((rc = prologue(h, 0, nullptr))) return rc;
     if ((rc = passes(h))) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
-    return PG_OK;
+    return timers_resolve(h);
 }
 
 int pg_run(pg_handle* h, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph) {
@@ -772,6 +795,7 @@ int pg_run(pg_handle* h, int nsteps, double* t_io, double* dt_io, double* dts, i
         }
     }
     if ((rc = pull_ctl(h))) return rc;
+    if ((rc = timers_resolve(h))) return rc;
     if (dts) HIP_TRY(hipMemcpy(dts, h->dts, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
     *t_io = c.t + c.dt;
     *dt_io = c.dt;
@@ -988,6 +1012,7 @@ int pg_profile_enable(pg_handle* h, int on) {
     CHECK_H(h);
     h->profile = on != 0;
     h->timers.clear();
+    h->ev_used = 0;
     return PG_OK;
 }
 
