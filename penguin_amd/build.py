@@ -8,6 +8,10 @@ hipcc cross-compiles without a GPU.  Flags:
                                         no packed FP32 instructions in device
                                         code (the host pass warns about the
                                         unknown feature; harmless)
+  -fno-hip-fp32-correctly-rounded-divide-sqrt
+                                        FP32 division and square root by the
+                                        hardware approximations (~1 ulp);
+                                        FP64 is unaffected
 """
 from __future__ import annotations
 
@@ -33,6 +37,7 @@ def hipcc() -> str:
 def flags() -> list:
     return ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-fno-hip-fp32-correctly-rounded-divide-sqrt",
             "-Wall", "-Wno-unused-result", "-Wno-unused-variable", "-Wno-unused-function"]
 
 

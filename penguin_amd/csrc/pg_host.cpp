@@ -208,6 +208,8 @@ struct pg_handle {
     double* rot = nullptr;
     double* dts = nullptr;
     int dts_cap = 0;
+    double* rate_part = nullptr;
+    int rate_nblk = 0;
     StepCtl* ctl = nullptr;
     StepCtl* ctl_host = nullptr;  // pinned mirror
     size_t bytes = 0;
@@ -269,6 +271,8 @@ struct pg_handle {
         d.planet_ecc = P.planet_ecc;
         d.planet_radius = P.planet_radius;
         d.rs = planet_init.rs;
+        d.rate_part = rate_part;
+        d.rate_nblk = rate_nblk;
         return d;
     }
 };
@@ -284,6 +288,7 @@ int validate(const pg_params& P) {
     if (P.gridx != 0 && P.gridx != 1) return fail(PG_E_INVALID, "gridx=%d: supported radial grids are 0 and 1", P.gridx);
     if (P.recon_flag != 6) return fail(PG_E_INVALID, "recon_flag=%d: supported reconstruction is 6 (PPM4)", P.recon_flag);
     if (P.EOS_flag != 0) return fail(PG_E_INVALID, "EOS_flag=%d: supported EOS is 0 (locally isothermal)", P.EOS_flag);
+    if (P.gam != 1.0) return fail(PG_E_INVALID, "gam=%g: EOS_flag 0 fixes gam = 1 (parameters.h:118-119)", P.gam);
     if (!P.OrbAdv_flag) return fail(PG_E_INVALID, "OrbAdv_flag=0 is not supported");
     if (P.bound_lft != 0 || P.bound_rgh != 0)
         return fail(PG_E_INVALID, "radial boundary types %d/%d: supported type is 0 (analytic)", P.bound_lft, P.bound_rgh);
@@ -294,7 +299,7 @@ int validate(const pg_params& P) {
     if (P.n_planet != 0 && P.n_planet != 1) return fail(PG_E_INVALID, "n_planet=%d: 0 or 1", P.n_planet);
     if (!(P.xmin > 0.0 && P.xmax > P.xmin)) return fail(PG_E_INVALID, "bad radial domain");
     size_t esz = P.precision;
-    size_t lds = (3 * (size_t)P.yres + 14 * (256 + 2 * NPAD)) * esz;
+    size_t lds = (3 * (size_t)P.yres + 14 * (RING_THREADS + 2 * NPAD)) * esz;
     if (lds > 160 * 1024)
         return fail(PG_E_INVALID, "yres=%d at precision %d needs %zu B of LDS for the azimuthal ring (max 163840)",
                     P.yres, P.precision, lds);
@@ -373,8 +378,15 @@ void build_tables_and_state(pg_handle* h, std::vector<double>& gas, std::vector<
             T(T_D_V, ip) = pr.v_d(xc) - vb;
         }
     }
+    for (int ip = 0; ip < np; ++ip) {
+        T(T_C0, ip) = std::sqrt(P.gam * T(T_CS2, ip));
+        T(T_IR, ip) = 1.0 / T(T_XC, ip);
+    }
     for (int ip = 1; ip < np - 1; ++ip) {
         double xm = T(T_XC, ip - 1), x0 = T(T_XC, ip), xp = T(T_XC, ip + 1);
+        T(T_I20, ip) = 1.0 / (xp - xm);
+        T(T_I21, ip) = 1.0 / (xp - x0);
+        T(T_I10, ip) = 1.0 / (x0 - xm);
         double vm = T(T_VBG, ip - 1), v0 = T(T_VBG, ip), vpp = T(T_VBG, ip + 1);
         T(T_DVBG, ip) = (vpp - vm) / (xp - xm);
         T(T_D2VBG, ip) = ((vpp - v0) / (xp - x0) - (v0 - vm) / (x0 - xm)) / (0.5 * (xp - xm));
@@ -555,11 +567,12 @@ int pull_ctl(pg_handle* h) {
 }
 
 int cfl_rate(pg_handle* h) {
-    h->ctl_host->rate_bits = 0ull;
-    HIP_TRY(hipMemsetAsync(&h->ctl->rate_bits, 0, sizeof(unsigned long long), h->stream));
     hipError_t e = h->prec == 8 ? Launch<double>::cfl(h->args<double>(), h->stream)
                                 : Launch<float>::cfl(h->args<float>(), h->stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch cfl failed: %s", hipGetErrorString(e));
+    e = h->prec == 8 ? Launch<double>::rate_reduce(h->args<double>(), h->stream)
+                     : Launch<float>::rate_reduce(h->args<float>(), h->stream);
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch rate_reduce failed: %s", hipGetErrorString(e));
     h->rate_valid = true;
     return PG_OK;
 }
@@ -605,6 +618,8 @@ int pg_create(const pg_params* params, pg_handle** out) {
         (rc = dev_alloc(h, &h->inc, h->nx * sizeof(int))) || (rc = dev_alloc(h, &h->rot, rot.size() * sizeof(double))) ||
         (rc = dev_alloc(h, &h->ctl, sizeof(StepCtl))))
         return cleanup(rc);
+    h->rate_nblk = ((h->nx + 255) / 256) * h->ny;
+    if ((rc = dev_alloc(h, &h->rate_part, sizeof(double) * h->rate_nblk))) return cleanup(rc);
     if (h->P.dust_flag) {
         if ((rc = dev_alloc(h, &h->dust, 3 * N * es)) || (rc = dev_alloc(h, &h->dustT, 3 * N * es))) return cleanup(rc);
     }
@@ -633,7 +648,7 @@ int pg_destroy(pg_handle* h) {
     if (!h) return PG_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->graph) hipGraphExecDestroy(h->graph);
-    void* bufs[] = {h->gas, h->gasT, h->dust, h->dustT, h->vis, h->tab, h->res, h->inc, h->rot, h->dts, h->ctl};
+    void* bufs[] = {h->gas, h->gasT, h->dust, h->dustT, h->vis, h->tab, h->res, h->inc, h->rot, h->dts, h->ctl, h->rate_part};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->ctl_host) hipHostFree(h->ctl_host);

@@ -28,6 +28,7 @@ namespace pg {
 constexpr int NPAD = 3;
 constexpr int NGAS = 3;
 constexpr int NDUST = 3;
+constexpr int RING_THREADS = 512;   // threads per azimuthal-ring block
 
 struct Body {
     double m, a, e, x, y, z, vx, vy, vz, rs;
@@ -63,6 +64,11 @@ enum TabField {
     T_G_R, T_G_U, T_G_V,   // gas initial state rho, u, vp (vp = 0)
     T_D_R, T_D_U, T_D_V,   // dust initial state rho_d, u_d, vp_d
     T_KILL,       // wave-killing rate without dt: sin^2(pi/2*d)/R^1.5, 0 outside
+    T_C0,         // isothermal sound speed sqrt(c_s^2)
+    T_I20,        // 1/(xc[ip+1]-xc[ip-1])
+    T_I21,        // 1/(xc[ip+1]-xc[ip])
+    T_I10,        // 1/(xc[ip]-xc[ip-1])
+    T_IR,         // 1/xc
     T_NFIELDS
 };
 
@@ -91,7 +97,8 @@ struct DiskArgs {
     real* res;           // nx, residual advection speed of the step
     StepCtl* ctl;
     double ramp_time, planet_mass, planet_ecc, planet_radius, rs;
-    double* rate_out;    // scratch (unused when null)
+    double* rate_part;   // per-block maxima of the CFL rate (rate_nblk entries)
+    int rate_nblk;
 };
 
 }  // namespace pg

@@ -109,88 +109,161 @@ __device__ __forceinline__ real flatten5(real pm2, real pm1, real p0, real pp1, 
 }
 
 // Characteristic-traced half-step state on the right of an interface from
-// the cell right of it (set_state, states.cu:151-206).  dtx = dt/dx.
+// the cell right of it (set_state, states.cu:151-206), locally isothermal
+// (EOS_flag 0 fixes gam = 1, parameters.h:118-119, so exp_lim(gam*x) =
+// exp_lim(x) and the sound speed of the traced state is sqrt(p0/r0)).
+// c: the cell's sound speed; idx = 1/dx.  Returns the state and its sound
+// speed cout (= the one HLLC recomputes from pl/rl, wave_speeds.cu:290).
 template <typename real>
-__device__ __forceinline__ void trace_right(real gam, real dt, real idx, real r, real p, real u,
+__device__ __forceinline__ void trace_right(real c, real dt, real idx, real r, real p, real u,
                                             real rsL, real rsR, real psL, real psR, real usL, real usR,
                                             real vsL, real v, real vsR,
-                                            real& rr, real& pr, real& ur, real& vr) {
-    real cs = sqrt(gam * p / r);
-    real q = lim01((cs - fmin(u, real(0))) * dt * idx);
+                                            real& rr, real& pr, real& ur, real& vr, real& cout) {
+    real q = lim01((c - fmin(u, real(0))) * dt * idx);
     real ua = aveL(q, usL, u, usR);
     real dis = fmin(ua, real(0)) * dt;
     q = lim01(-dis * idx);
     real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
     real p0 = fmax(aveL(q, psL, p, psR), p * real(1.0e-10));
     real u0 = aveL(q, usL, u, usR);
-    cs = sqrt(gam * p0 / r0);
+    real cs = sqrt(p0 / r0);
     q = lim01((cs * dt - dis) * idx);
     real u_ = aveL(q, usL, u, usR);
     real p_ = aveL(q, psL, p, psR);
-    u_ = -(u_ - u0) / cs;
-    p_ = -(p_ - p0) / cs;
-    rr = r0 * exp_lim(u_);
-    pr = p0 * exp_lim(gam * u_);
-    ur = u0 + p_ / r0;
+    real ics = real(1) / cs;
+    real e = exp_lim((u0 - u_) * ics);
+    rr = r0 * e;
+    pr = p0 * e;
+    ur = u0 + (p0 - p_) * ics / r0;
     vr = aveL(q, vsL, v, vsR);
+    cout = cs;
 }
 
 // ... and on the left from the cell left of it (states.cu:208-264).
 template <typename real>
-__device__ __forceinline__ void trace_left(real gam, real dt, real idx, real r, real p, real u,
+__device__ __forceinline__ void trace_left(real c, real dt, real idx, real r, real p, real u,
                                            real rsL, real rsR, real psL, real psR, real usL, real usR,
                                            real vsL, real v, real vsR,
-                                           real& rl, real& pl, real& ul, real& vl) {
-    real cs = sqrt(gam * p / r);
-    real x = lim01((fmax(u, real(0)) + cs) * dt * idx);
+                                           real& rl, real& pl, real& ul, real& vl, real& cout) {
+    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
     real ua = aveR(x, usL, u, usR);
     real dis = fmax(ua, real(0)) * dt;
     x = lim01(dis * idx);
     real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
     real p0 = fmax(aveR(x, psL, p, psR), p * real(1.0e-10));
     real u0 = aveR(x, usL, u, usR);
-    cs = sqrt(gam * p0 / r0);
+    real cs = sqrt(p0 / r0);
     x = lim01((dis + cs * dt) * idx);
     real u_ = aveR(x, usL, u, usR);
     real p_ = aveR(x, psL, p, psR);
-    u_ = -(u0 - u_) / cs;
-    p_ = -(p0 - p_) / cs;
-    rl = r0 * exp_lim(u_);
-    pl = p0 * exp_lim(gam * u_);
-    ul = u0 + p_ / r0;
+    real ics = real(1) / cs;
+    real e = exp_lim((u_ - u0) * ics);
+    rl = r0 * e;
+    pl = p0 * e;
+    ul = u0 + (p_ - p0) * ics / r0;
+    vl = aveR(x, vsL, v, vsR);
+    cout = cs;
+}
+
+// The same with one sound speed c for every cell of the row (p = c^2 rho
+// with c fixed: the azimuthal sweep of a locally isothermal disk); pressure
+// is c^2 times density throughout, so only rho, u, v are traced.
+template <typename real>
+__device__ __forceinline__ void trace_right_c(real c, real dt, real idx, real r, real u,
+                                              real rsL, real rsR, real usL, real usR,
+                                              real vsL, real v, real vsR,
+                                              real& rr, real& ur, real& vr) {
+    real q = lim01((c - fmin(u, real(0))) * dt * idx);
+    real ua = aveL(q, usL, u, usR);
+    real dis = fmin(ua, real(0)) * dt;
+    q = lim01(-dis * idx);
+    real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
+    real u0 = aveL(q, usL, u, usR);
+    q = lim01((c * dt - dis) * idx);
+    real u_ = aveL(q, usL, u, usR);
+    real ra = aveL(q, rsL, r, rsR);
+    rr = r0 * exp_lim((u0 - u_) / c);
+    ur = u0 + c * (r0 - ra) / r0;
+    vr = aveL(q, vsL, v, vsR);
+}
+
+template <typename real>
+__device__ __forceinline__ void trace_left_c(real c, real dt, real idx, real r, real u,
+                                             real rsL, real rsR, real usL, real usR,
+                                             real vsL, real v, real vsR,
+                                             real& rl, real& ul, real& vl) {
+    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
+    real ua = aveR(x, usL, u, usR);
+    real dis = fmax(ua, real(0)) * dt;
+    x = lim01(dis * idx);
+    real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
+    real u0 = aveR(x, usL, u, usR);
+    x = lim01((dis + c * dt) * idx);
+    real u_ = aveR(x, usL, u, usR);
+    real ra = aveR(x, rsL, r, rsR);
+    rl = r0 * exp_lim((u_ - u0) / c);
+    ul = u0 + c * (ra - r0) / r0;
     vl = aveR(x, vsL, v, vsR);
 }
 
-// Wave speeds (get_pm_simple, get_lr_speeds_Einfeldt, get_sm;
-// wave_speeds.cu:24-35,288-333) and the HLLC fluxes of an isothermal state
-// (HLLC.cu:10-77, EOS_flag 0): mass flux, upwinded normal and transverse
-// velocity, interface pressure.
+// HLLC selection (HLLC.cu:2-77, EOS_flag 0) given the wave speeds.
 template <typename real>
-__device__ __forceinline__ void hllc_iso(real gam, real rl, real pl, real ul, real vl,
-                                         real rr, real pr, real ur, real vr,
-                                         real& Fr, real& us, real& vs, real& pres) {
-    real cl = sqrt(gam * pl / rl), cr = sqrt(gam * pr / rr);
-    real pm = real(0.5) * (pr + pl) * exp_lim(real(0.5) * gam * (ul - ur) / (real(0.5) * (cr + cl)));
-    real srl = sqrt(rl), srr = sqrt(rr);
-    real ssum = srl + srr;
-    real ubar = (srl * ul + srr * ur) / ssum;
-    real n2 = real(0.5) * srl * srr / (ssum * ssum);
-    real cbar = sqrt((cl * cl * srl + cr * cr * srr) / ssum + n2 * (ur - ul) * (ur - ul));
-    real sl = ubar - cbar, sr = ubar + cbar;
-    real sm = (rl * ul * cl + rr * ur * cr) / (rl * cl + rr * cr);
-    sm += (pl - pr) / (rl + rr) / (real(0.5) * (cl + cr));
-    // upwind selection of HLLC.cu:2-8
+__device__ __forceinline__ void hllc_select(real pm, real sl, real sm, real sr,
+                                            real rl, real pl, real ul, real vl,
+                                            real rr, real pr, real ur, real vr,
+                                            real& Fr, real& us, real& vs, real& pres) {
     if (!signbit(sl)) {
         Fr = ul * rl; us = ul; vs = vl; pres = pl;
     } else if (signbit(sr)) {
         Fr = ur * rr; us = ur; vs = vr; pres = pr;
     } else if (!signbit(sm)) {
-        Fr = sm * rl * (sl - ul) / (sl - sm); us = ul; vs = vl; pres = (sl * pm - sm * pl) / (sl - sm);
+        real k = real(1) / (sl - sm);
+        Fr = sm * rl * (sl - ul) * k; us = ul; vs = vl; pres = (sl * pm - sm * pl) * k;
     } else {
-        Fr = sm * rr * (sr - ur) / (sr - sm); us = ur; vs = vr; pres = (sr * pm - sm * pr) / (sr - sm);
+        real k = real(1) / (sr - sm);
+        Fr = sm * rr * (sr - ur) * k; us = ur; vs = vr; pres = (sr * pm - sm * pr) * k;
     }
     us *= Fr;
     vs *= Fr;
+}
+
+// Wave speeds (get_pm_simple, get_lr_speeds_Einfeldt, get_sm;
+// wave_speeds.cu:24-35,288-333; gam = 1) with the states' sound speeds cl,
+// cr, then the HLLC fluxes: mass flux, upwinded normal and transverse
+// momentum flux, interface pressure.
+template <typename real>
+__device__ __forceinline__ void hllc_iso(real cl, real cr, real rl, real pl, real ul, real vl,
+                                         real rr, real pr, real ur, real vr,
+                                         real& Fr, real& us, real& vs, real& pres) {
+    real csum = cl + cr;
+    real pm = real(0.5) * (pr + pl) * exp_lim((ul - ur) / csum);
+    real srl = sqrt(rl), srr = sqrt(rr);
+    real iss = real(1) / (srl + srr);
+    real ubar = (srl * ul + srr * ur) * iss;
+    real n2 = real(0.5) * srl * srr * iss * iss;
+    real du = ur - ul;
+    real cbar = sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
+    real sl = ubar - cbar, sr = ubar + cbar;
+    real sm = (rl * ul * cl + rr * ur * cr) / (rl * cl + rr * cr) + real(2) * (pl - pr) / ((rl + rr) * csum);
+    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
+}
+
+// The same with one sound speed c on both sides (azimuthal isothermal sweep):
+// cbar^2 = c^2 + n2 du^2 and sm = (rl (ul + c) + rr (ur - c)) / (rl + rr).
+template <typename real>
+__device__ __forceinline__ void hllc_c(real c, real c2, real rl, real ul, real vl, real rr, real ur, real vr,
+                                       real& Fr, real& us, real& vs, real& pres) {
+    real pl = c2 * rl, pr = c2 * rr;
+    real pm = real(0.5) * (pr + pl) * exp_lim(real(0.5) * (ul - ur) / c);
+    real srl = sqrt(rl), srr = sqrt(rr);
+    real iss = real(1) / (srl + srr);
+    real ubar = (srl * ul + srr * ur) * iss;
+    real n2 = real(0.5) * srl * srr * iss * iss;
+    real du = ur - ul;
+    real cbar = sqrt(c2 + n2 * du * du);
+    real sl = ubar - cbar, sr = ubar + cbar;
+    real sm = (rl * (ul + c) + rr * (ur - c)) / (rl + rr);
+    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
 }
 
 // Pressureless dust flux through an interface (dust_flux, dust.cu:6-105)
@@ -326,7 +399,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_gas(DiskArgs<real> d) {
     }
     __syncthreads();
     for (int s = threadIdx.x + 2; s < ns - 2; s += blockDim.x) {
-        real cs = sqrt(gam * s_p[s] / s_r[s]);
+        real cs = TAB(T_C0, x0 + s);   // sqrt(gam p / rho) with p = c_s^2 rho, gam = 1
         s_f[s] = flatten5(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2],
                           s_u[s - 1], s_u[s], s_u[s + 1], cs);
     }
@@ -349,16 +422,16 @@ __global__ __launch_bounds__(256) void k_sweep_x_gas(DiskArgs<real> d) {
     __syncthreads();
     for (int k = threadIdx.x + NPAD; k < ns - 2; k += blockDim.x) {
         int ipk = x0 + k;
-        real rl, pl, ul, vl, rr, pr, ur, vr;
-        trace_left(gam, dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_p[k - 1], s_u[k - 1],
+        real rl, pl, ul, vl, rr, pr, ur, vr, cl, cr;
+        trace_left(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_p[k - 1], s_u[k - 1],
                    s_sl[k - 1], s_sr[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
                    s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
-                   s_sl[3 * S + k - 1], s_L[k - 1], s_sr[3 * S + k - 1], rl, pl, ul, vl);
-        trace_right(gam, dt, TAB(T_IDX, ipk), s_r[k], s_p[k], s_u[k],
+                   s_sl[3 * S + k - 1], s_L[k - 1], s_sr[3 * S + k - 1], rl, pl, ul, vl, cl);
+        trace_right(TAB(T_C0, ipk), dt, TAB(T_IDX, ipk), s_r[k], s_p[k], s_u[k],
                     s_sl[k], s_sr[k], s_sl[S + k], s_sr[S + k], s_sl[2 * S + k], s_sr[2 * S + k],
-                    s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, pr, ur, vr);
+                    s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, pr, ur, vr, cr);
         real Fr, Fu, FL, pres;
-        hllc_iso(gam, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, FL, pres);
+        hllc_iso(cl, cr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, FL, pres);
         real g = TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
         s_Fr[k] = Fr * g;
         s_Fu[k] = Fu * g;
@@ -515,13 +588,23 @@ __device__ __forceinline__ RingLds<real> ring_lds(real* sm, int ny, int S) {
     return L;
 }
 
+// Radius handled by ring block b.  Workgroups are dispatched round-robin
+// over the 8 XCDs (each with its own L2); giving XCD x the contiguous radii
+// [x*per, (x+1)*per) lets neighbouring radii, which share cache lines of the
+// radius-fastest layout, hit in the same L2.  Returns -1 for padding blocks.
+__device__ __forceinline__ int ring_radius(int b, int nx) {
+    const int per = (nx + 7) >> 3;
+    const int r = (b & 7) * per + (b >> 3);
+    return r < nx ? r : -1;
+}
+
 __device__ __forceinline__ int wrap(int j, int n) {
     j %= n;
     return j < 0 ? j + n : j;
 }
 
 template <typename real>
-__global__ __launch_bounds__(256) void k_ring_gas(DiskArgs<real> d) {
+__global__ __launch_bounds__(512) void k_ring_gas(DiskArgs<real> d) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
     const int nx = d.nx, ny = d.ny;
@@ -529,7 +612,8 @@ __global__ __launch_bounds__(256) void k_ring_gas(DiskArgs<real> d) {
     const int B = blockDim.x;
     const int S = B + 2 * NPAD;
     RingLds<real> L = ring_lds(sm, ny, S);
-    const int i = blockIdx.x;
+    const int i = ring_radius(blockIdx.x, nx);
+    if (i < 0) return;
     const int ip = i + NPAD;
     const real R = TAB(T_XC, ip);
     const real cs2 = TAB(T_CS2, ip);
@@ -538,7 +622,7 @@ __global__ __launch_bounds__(256) void k_ring_gas(DiskArgs<real> d) {
     const real dts = dt / R;          // set_state receives dt/rad (riemann.cu:111)
     const real idy = real(1) / d.dy;
     const real fup = dt / d.dy / R;   // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
-    const real cs0 = sqrt(gam * cs2);
+    const real cs0 = TAB(T_C0, ip);
     const int inc = d.orbadv ? d.inc[i] : 0;
     const real* __restrict__ in = d.gas;
     const real wl0 = d.wLy[0], wl1 = d.wLy[1], wl2 = d.wLy[2];
@@ -574,16 +658,15 @@ __global__ __launch_bounds__(256) void k_ring_gas(DiskArgs<real> d) {
         }
         __syncthreads();
         for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
-            real rl, pl, ul, vl, rr, pr, ur, vr;
-            // the pressure parabola is c_s^2 times the density parabola
-            trace_left(gam, dts, idy, L.c0[k - 1], cs2 * L.c0[k - 1], L.c1[k - 1],
-                       L.sl0[k - 1], L.sr0[k - 1], cs2 * L.sl0[k - 1], cs2 * L.sr0[k - 1],
-                       L.sl1[k - 1], L.sr1[k - 1], L.sl2[k - 1], L.c2[k - 1], L.sr2[k - 1], rl, pl, ul, vl);
-            trace_right(gam, dts, idy, L.c0[k], cs2 * L.c0[k], L.c1[k],
-                        L.sl0[k], L.sr0[k], cs2 * L.sl0[k], cs2 * L.sr0[k],
-                        L.sl1[k], L.sr1[k], L.sl2[k], L.c2[k], L.sr2[k], rr, pr, ur, vr);
+            real rl, ul, vl, rr, ur, vr;
+            // p = c_s^2 rho with c_s^2 fixed on the ring: the pressure
+            // parabola is c_s^2 times the density parabola
+            trace_left_c(cs0, dts, idy, L.c0[k - 1], L.c1[k - 1], L.sl0[k - 1], L.sr0[k - 1],
+                         L.sl1[k - 1], L.sr1[k - 1], L.sl2[k - 1], L.c2[k - 1], L.sr2[k - 1], rl, ul, vl);
+            trace_right_c(cs0, dts, idy, L.c0[k], L.c1[k], L.sl0[k], L.sr0[k],
+                          L.sl1[k], L.sr1[k], L.sl2[k], L.c2[k], L.sr2[k], rr, ur, vr);
             real Fr, Fu, Fw, pres;
-            hllc_iso(gam, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, Fw, pres);
+            hllc_c(cs0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
             L.F0[k] = Fr; L.F1[k] = Fu; L.F2[k] = Fw; L.F3[k] = pres;
         }
         __syncthreads();
@@ -679,7 +762,7 @@ __global__ __launch_bounds__(256) void k_ring_gas(DiskArgs<real> d) {
 }
 
 template <typename real>
-__global__ __launch_bounds__(256) void k_ring_dust(DiskArgs<real> d) {
+__global__ __launch_bounds__(512) void k_ring_dust(DiskArgs<real> d) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
     const int nx = d.nx, ny = d.ny;
@@ -687,7 +770,8 @@ __global__ __launch_bounds__(256) void k_ring_dust(DiskArgs<real> d) {
     const int B = blockDim.x;
     const int S = B + 2 * NPAD;
     RingLds<real> L = ring_lds(sm, ny, S);
-    const int i = blockIdx.x;
+    const int i = ring_radius(blockIdx.x, nx);
+    if (i < 0) return;
     const int ip = i + NPAD;
     const real R = TAB(T_XC, ip);
     const real dt = (real)d.ctl->dt;
@@ -839,8 +923,8 @@ __global__ __launch_bounds__(256) void k_viscosity(DiskArgs<real> d, const real*
     const int j = blockIdx.y;
     if (i >= nx) return;
     const int ip = i + NPAD;
-    real r[3], nu[3];
-    for (int n = 0; n < 3; ++n) { r[n] = TAB(T_XC, ip - 1 + n); nu[n] = TAB(T_NU, ip - 1 + n); }
+    real nu[3];
+    for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
     real nr[9], u[9], v[9];
     const int jm = wrap(j - 1, ny), jp = wrap(j + 1, ny);
     const int rows[3] = {j, jm, jp};
@@ -862,34 +946,35 @@ __global__ __launch_bounds__(256) void k_viscosity(DiskArgs<real> d, const real*
             v[n] = vv;  // deviation vp
         }
     }
-    const real dy = d.dy;
-    const real r1 = r[1];
-    const real dr20 = r[2] - r[0], dr21 = r[2] - r[1], dr10 = r[1] - r[0];
-    const real dt20 = real(2) * dy;
+    // reciprocal differences of the centre grid (viscosity.cu:9-31)
+    const real i20 = TAB(T_I20, ip), i21 = TAB(T_I21, ip), i10 = TAB(T_I10, ip);
+    const real ir = TAB(T_IR, ip);
+    const real idy = real(1) / d.dy;
+    const real hdy = real(0.5) * idy;            // 1/(t2 - t0)
     // radial derivatives
-    real dudx = (u[2] - u[0]) / dr20;
-    real d2udx2 = ((u[2] - u[1]) / dr21 - (u[1] - u[0]) / dr10) / (real(0.5) * dr20);
-    real dvdx = TAB(T_DVBG, ip) + (v[2] - v[0]) / dr20;
-    real d2vdx2 = TAB(T_D2VBG, ip) + ((v[2] - v[1]) / dr21 - (v[1] - v[0]) / dr10) / (real(0.5) * dr20);
-    real dndx = (nr[2] - nr[0]) / dr20;
+    real dudx = (u[2] - u[0]) * i20;
+    real d2udx2 = ((u[2] - u[1]) * i21 - (u[1] - u[0]) * i10) * (real(2) * i20);
+    real dvdx = TAB(T_DVBG, ip) + (v[2] - v[0]) * i20;
+    real d2vdx2 = TAB(T_D2VBG, ip) + ((v[2] - v[1]) * i21 - (v[1] - v[0]) * i10) * (real(2) * i20);
+    real dndx = (nr[2] - nr[0]) * i20;
     // azimuthal and mixed derivatives
-    real dvdy = (v[7] - v[4]) / dt20 / r1;
-    real d2vdy2 = ((v[7] - v[1]) / dy - (v[1] - v[4]) / dy) / (real(0.5) * dt20) / r1 / r1;
-    real d2vdxy = (v[8] - v[6] - v[5] + v[3]) / dr20 / dt20 / r1;
-    real dudy = (u[7] - u[4]) / dt20 / r1;
-    real d2udy2 = ((u[7] - u[1]) / dy - (u[1] - u[4]) / dy) / (real(0.5) * dt20) / r1 / r1;
-    real d2udxy = (u[8] - u[6] - u[5] + u[3]) / dr20 / dt20 / r1;
-    real dndy = (nr[7] - nr[4]) / dt20 / r1;
+    real dvdy = (v[7] - v[4]) * hdy * ir;
+    real d2vdy2 = (v[7] - real(2) * v[1] + v[4]) * (idy * idy) * ir * ir;
+    real d2vdxy = (v[8] - v[6] - v[5] + v[3]) * i20 * hdy * ir;
+    real dudy = (u[7] - u[4]) * hdy * ir;
+    real d2udy2 = (u[7] - real(2) * u[1] + u[4]) * (idy * idy) * ir * ir;
+    real d2udxy = (u[8] - u[6] - u[5] + u[3]) * i20 * hdy * ir;
+    real dndy = (nr[7] - nr[4]) * hdy * ir;
     const real v1 = TAB(T_VBG, ip) + v[1];
     const real n1 = nr[1];
     real sxx = real(2) * n1 * dudx;
-    real fx = real(2) * dndx * dudx + real(2) * n1 * d2udx2 + sxx / r1;
-    real syy = real(2) * n1 * (dvdy + u[1] / r1);
-    real sxy = n1 * (dudy + dvdx - v1 / r1);
-    real fy = real(2) * dndy * dvdy + real(2) * n1 * d2vdy2 + real(2) * n1 * dudy / r1 + real(2) * dndy * u[1] / r1;
-    fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 / r1 - n1 * dvdy / r1 - syy / r1;
-    fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 / r1 - n1 * dvdx / r1 + n1 * v1 / r1 / r1 +
-          real(2) * sxy / r1;
+    real fx = real(2) * dndx * dudx + real(2) * n1 * d2udx2 + sxx * ir;
+    real syy = real(2) * n1 * (dvdy + u[1] * ir);
+    real sxy = n1 * (dudy + dvdx - v1 * ir);
+    real fy = real(2) * dndy * dvdy + real(2) * n1 * d2vdy2 + real(2) * n1 * dudy * ir + real(2) * dndy * u[1] * ir;
+    fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 * ir - n1 * dvdy * ir - syy * ir;
+    fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 * ir - n1 * dvdx * ir + n1 * v1 * ir * ir +
+          real(2) * sxy * ir;
     size_t o = (size_t)j * nx + i;
     d.vis[o] = fx;
     d.vis[N + o] = fy;
@@ -911,9 +996,11 @@ __device__ __forceinline__ void planet_acc(const Body& b, real R, real phi, real
     ay = -m * px * s * k + s * ind;
 }
 
-// Warp/block max of a non-negative value, then one atomic per block.
+// Block max of a non-negative value, written to this block's slot of the
+// partial-max array (reduced by the prologue or k_rate_reduce): one store per
+// block instead of a contended atomic on one address.
 template <typename real>
-__device__ __forceinline__ void block_max_to(unsigned long long* dst, real v) {
+__device__ __forceinline__ void block_max_store(double* part, real v) {
     double x = (double)v;
     for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
     __shared__ double wmax[16];
@@ -923,8 +1010,31 @@ __device__ __forceinline__ void block_max_to(unsigned long long* dst, real v) {
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) m = fmax(m, wmax[k]);
-        atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+        part[blockIdx.x + gridDim.x * blockIdx.y] = m;
     }
+}
+
+// Max over the partial array by one block; result in thread 0.  Eight
+// independent loads per iteration keep the memory pipeline full.
+__device__ __forceinline__ double reduce_partials(const double* part, int n) {
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int stride = blockDim.x * 8;
+    int k0 = threadIdx.x;
+    for (; k0 + 7 * (int)blockDim.x < n; k0 += stride) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = fmax(a[u], part[k0 + u * blockDim.x]);
+    }
+    for (int k = k0; k < n; k += blockDim.x) a[0] = fmax(a[0], part[k]);
+    double x = fmax(fmax(fmax(a[0], a[1]), fmax(a[2], a[3])), fmax(fmax(a[4], a[5]), fmax(a[6], a[7])));
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    __shared__ double wmax[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wmax[w] = x;
+    __syncthreads();
+    double m = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) m = fmax(m, wmax[k]);
+    return m;
 }
 
 // Pointwise source terms.  mode: 0 first half kick (planet at t, in place),
@@ -1005,7 +1115,7 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
             }
         }
     }
-    if (do_rate) block_max_to(&d.ctl->rate_bits, rate);
+    if (do_rate) block_max_store(d.rate_part, rate);
 }
 
 // CFL rate of the current state (get_dt_lv1, timestep.cu:18-97).
@@ -1024,7 +1134,14 @@ __global__ __launch_bounds__(256) void k_cfl(DiskArgs<real> d) {
         if (d.has_dust) { sx = fmax(fabs(d.dust[N + o]), sx); sy = fmax(fabs(d.dust[2 * N + o]), sy); }
         rate = fmax(sx * TAB(T_IDX, ip), sy / (d.dy * TAB(T_XC, ip)));
     }
-    block_max_to(&d.ctl->rate_bits, rate);
+    block_max_store(d.rate_part, rate);
+}
+
+// Max of the partials into the control block (host-driven global_dt).
+template <typename real>
+__global__ __launch_bounds__(1024) void k_rate_reduce(DiskArgs<real> d) {
+    double m = reduce_partials(d.rate_part, d.rate_nblk);
+    if (threadIdx.x == 0) d.ctl->rate_bits = (unsigned long long)__double_as_longlong(m);
 }
 
 // Step prologue, one block.  mode 0: t and dt were set by the host.
@@ -1032,14 +1149,15 @@ __global__ __launch_bounds__(256) void k_cfl(DiskArgs<real> d) {
 // dt = min(CFL/rate, 1.1 dt_prev) clipped to end_time, record it.  Then the
 // planet at t and t+dt and the orbital shift of every radius.
 template <typename real>
-__global__ __launch_bounds__(256) void k_prologue(DiskArgs<real> d, int mode, double* dts) {
+__global__ __launch_bounds__(1024) void k_prologue(DiskArgs<real> d, int mode, double* dts) {
     StepCtl* ctl = d.ctl;
     __shared__ double s_dt;
+    double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
     if (threadIdx.x == 0) {
         double t = ctl->t, dt = ctl->dt;
         if (mode == 1) {
             if (ctl->nstep > 0) t += dt;
-            double rate = __longlong_as_double((long long)ctl->rate_bits);
+            ctl->rate_bits = (unsigned long long)__double_as_longlong(rate);
             double dtn = d.CFL / rate;
             double cap = dt * 1.1;
             dtn = dtn < cap ? dtn : cap;
@@ -1050,7 +1168,6 @@ __global__ __launch_bounds__(256) void k_prologue(DiskArgs<real> d, int mode, do
             ctl->dt_prev = dt;
             if (dts) dts[ctl->nstep] = dt;
             ctl->nstep += 1;
-            ctl->rate_bits = 0ull;
         }
         Body b0 = ctl->planet0;
         orbit_solution(b0, t, d.ramp_time, d.planet_mass, (double)d.frame_omega);
@@ -1092,25 +1209,25 @@ struct Launch {
     }
     static size_t ring_lds(int ny, int B) { return ((size_t)3 * ny + (size_t)14 * (B + 2 * NPAD)) * sizeof(real); }
     static hipError_t ring_gas(const DiskArgs<real>& d, hipStream_t s) {
-        const int B = 256;
+        const int B = RING_THREADS;
         size_t lds = ring_lds(d.ny, B);
         if (lds > 65536) {
             hipError_t e = hipFuncSetAttribute((const void*)k_ring_gas<real>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL(k_ring_gas<real>, dim3(d.nx), dim3(B), lds, s, d);
+        hipLaunchKernelGGL(k_ring_gas<real>, dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
         return hipGetLastError();
     }
     static hipError_t ring_dust(const DiskArgs<real>& d, hipStream_t s) {
-        const int B = 256;
+        const int B = RING_THREADS;
         size_t lds = ring_lds(d.ny, B);
         if (lds > 65536) {
             hipError_t e = hipFuncSetAttribute((const void*)k_ring_dust<real>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
         }
-        hipLaunchKernelGGL(k_ring_dust<real>, dim3(d.nx), dim3(B), lds, s, d);
+        hipLaunchKernelGGL(k_ring_dust<real>, dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
         return hipGetLastError();
     }
     static hipError_t viscosity(const DiskArgs<real>& d, const real* st, hipStream_t s) {
@@ -1128,6 +1245,10 @@ struct Launch {
         hipLaunchKernelGGL(k_cfl<real>, grid, dim3(256), 0, s, d);
         return hipGetLastError();
     }
+    static hipError_t rate_reduce(const DiskArgs<real>& d, hipStream_t s) {
+        hipLaunchKernelGGL(k_rate_reduce<real>, dim3(1), dim3(1024), 0, s, d);
+        return hipGetLastError();
+    }
 };
 
 template struct Launch<float>;
@@ -1135,7 +1256,7 @@ template struct Launch<double>;
 
 template <typename real>
 hipError_t launch_prologue(const DiskArgs<real>& d, int mode, double* dts, hipStream_t s) {
-    hipLaunchKernelGGL(k_prologue<real>, dim3(1), dim3(256), 0, s, d, mode, dts);
+    hipLaunchKernelGGL(k_prologue<real>, dim3(1), dim3(1024), 0, s, d, mode, dts);
     return hipGetLastError();
 }
 template hipError_t launch_prologue<float>(const DiskArgs<float>&, int, double*, hipStream_t);

@@ -18,6 +18,7 @@ struct Launch {
     static hipError_t viscosity(const DiskArgs<real>& d, const real* st, hipStream_t s);
     static hipError_t sources(const DiskArgs<real>& d, int mode, int kill, int rate, hipStream_t s);
     static hipError_t cfl(const DiskArgs<real>& d, hipStream_t s);
+    static hipError_t rate_reduce(const DiskArgs<real>& d, hipStream_t s);
 };
 
 template <typename real>

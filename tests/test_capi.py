@@ -1,0 +1,80 @@
+"""CPU checks of the C ABI (no GPU needed): the built library exports every
+entry point declared in include/penguin_amd.h, the ctypes mirror of
+pg_params has the C layout, and invalid configurations fail with an error
+code and message instead of crashing."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from penguin_amd.disk import LIB_PATH, SIGNATURES, PenguinError, load_library
+from penguin_amd.params import Params, config_c2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "penguin_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(pg_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_is_built():
+    assert os.path.exists(LIB_PATH), "run python -m penguin_amd.build"
+
+
+def test_every_declared_entry_point_is_exported_and_bound():
+    names = declared_functions()
+    assert len(names) >= 20
+    L = load_library()
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in SIGNATURES, f"{n} has no ctypes signature"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (pg_\w+)", out))
+    assert set(names) <= exported
+    # nothing else with the pg_ prefix leaks from the library
+    assert exported <= set(names)
+
+
+def test_params_layout_matches_c(tmp_path):
+    fields = [f for f, _ in Params._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{os.path.join(ROOT, "include", "pg_params.h")}"',
+           'int main(void){', 'printf("%zu\\n", sizeof(pg_params));']
+    for f in fields:
+        src.append(f'printf("%zu\\n", offsetof(pg_params, {f}));')
+    src.append("return 0;}")
+    c = tmp_path / "probe.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-o", str(exe), str(c)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert vals[0] == ctypes.sizeof(Params)
+    for f, off in zip(fields, vals[1:]):
+        assert getattr(Params, f).offset == off, f
+
+
+def test_invalid_configuration_is_rejected():
+    L = load_library()
+    h = ctypes.c_void_p()
+    for bad in (dict(ndim=3), dict(EOS_flag=2), dict(recon_flag=5), dict(precision=2),
+                dict(bound_bak=2), dict(gam=1.4), dict(xres=4)):
+        p = config_c2(xres=64, yres=64).replace(**bad)
+        rc = L.pg_create(ctypes.byref(p), ctypes.byref(h))
+        assert rc == -1, bad
+        assert L.pg_last_error().decode(), bad
+    with pytest.raises(PenguinError):
+        from penguin_amd import Disk
+        Disk(config_c2(xres=64, yres=64).replace(ndim=3))
+
+
+def test_null_handle_is_an_error_not_a_crash():
+    L = load_library()
+    dt = ctypes.c_double()
+    assert L.pg_global_dt(None, 1.0, ctypes.byref(dt)) == -1
+    assert L.pg_solve(None, 0.0, 1e-3) == -1
+    assert L.pg_destroy(None) == 0
+    assert L.pg_version().decode().startswith("penguin_amd")
