@@ -92,6 +92,18 @@ int pg_killwave(pg_handle* h, double dt);
    body10 = m,a,e,x,y,z,vx,vy,vz,rs (may be NULL) */
 int pg_planet(pg_handle* h, double t, double* body10);
 
+/* ---- snapshots (the reference's checkpoint files) ------------------------ */
+/* Write the current state in the layout of write_check_point
+   (output.cu:125-180): FP64 t, xa[xres+1], ya[yres+1], rho, p, u, v
+   (xres*yres each, x fastest), then with dust rho_d, u_d, v_d, w_d. */
+int pg_write_snapshot(pg_handle* h, const char* filename, double t);
+/* Restart from such a file (load_grid, output.cu:182-248); unlike the
+   reference this also restores the dust fields.  *t_out receives the time. */
+int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out);
+/* Run label of create_label (output.cu:14-93), e.g. "256x512_h50_1p33E_e0_a-30_OA_PPM4";
+   the snapshot of frame n is "binary_" + label + "_" + 5-digit n (main.cu:403). */
+int pg_snapshot_label(const pg_params* params, char* buf, size_t len);
+
 /* ---- introspection ----------------------------------------------------- */
 /* HIP stream the handle launches on (as a void*), for external capture. */
 int pg_stream(pg_handle* h, void** stream);

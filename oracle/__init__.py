@@ -59,6 +59,7 @@ def lib():
             "orc_orbital_advection": (None, [vp, ctypes.c_double]),
             "orc_get_orbadv": (None, [vp, _ip, _dp]),
             "orc_killwave": (None, [vp, ctypes.c_double]),
+            "orc_write_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_double]),
             "orc_evolve_planet": (None, [vp, ctypes.c_double]),
             "orc_fourth_parameters": (None, [ctypes.c_int, _dp, _dp, _dp]),
             "orc_ppm_adjust": (None, [ctypes.c_double, _dp]),
@@ -190,6 +191,10 @@ class Oracle:
 
     def evolve_planet(self, t):
         lib().orc_evolve_planet(self._h, t)
+
+    def write_snapshot(self, filename: str, t: float):
+        if lib().orc_write_snapshot(self._h, filename.encode(), t) != 0:
+            raise OSError(filename)
 
 
 def num_threads() -> int:

@@ -43,6 +43,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <cstdio>
 #include <vector>
 
 #ifdef _OPENMP
@@ -1352,6 +1353,42 @@ void orc_get_orbadv(const orc_grid* g, int* inc, double* res) {
 }
 
 void orc_killwave(orc_grid* g, double dt) { killwave(*g, dt); }
+
+// write_check_point (output.cu:125-180) for ndim 2, with restructure_data's
+// field order (output.cu:95-123); the dust w field is written too (ndim > 1
+// guard of output.cu:171)
+int orc_write_snapshot(const orc_grid* g, const char* filename, double t) {
+    FILE* f = std::fopen(filename, "wb");
+    if (!f) return -1;
+    std::fwrite(&t, sizeof(double), 1, f);
+    std::fwrite(&g->xa[g->xpad], sizeof(double), g->xres + 1, f);
+    std::fwrite(&g->ya[g->ypad], sizeof(double), g->yres + 1, f);
+    std::vector<double> tmp((size_t)g->xres * g->yres);
+    int nf = g->P.dust_flag ? 8 : 4;
+    for (int field = 0; field < nf; ++field) {
+        for (int j = 0; j < g->yres; ++j)
+            for (int i = 0; i < g->xres; ++i) {
+                size_t id = g->ind(i + g->xpad, j + g->ypad);
+                const Cell& c = g->C[id];
+                const Dust& d = g->CD[id];
+                double v = 0.0;
+                switch (field) {
+                    case 0: v = c.r; break;
+                    case 1: v = c.p; break;
+                    case 2: v = c.u; break;
+                    case 3: v = c.v; break;
+                    case 4: v = d.r; break;
+                    case 5: v = d.u; break;
+                    case 6: v = d.v; break;
+                    default: v = d.w; break;
+                }
+                tmp[(size_t)i + (size_t)g->xres * j] = v;
+            }
+        std::fwrite(tmp.data(), sizeof(double), tmp.size(), f);
+    }
+    std::fclose(f);
+    return 0;
+}
 void orc_evolve_planet(orc_grid* g, double t) { orbit_solution(g->P, g->planet, t); }
 
 // ------------------------------------------------------------ unit level

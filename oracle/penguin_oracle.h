@@ -54,6 +54,7 @@ void orc_sweepy_dust(orc_grid* g, double t, double dt);
 void orc_orbital_advection(orc_grid* g, double dt);
 void orc_get_orbadv(const orc_grid* g, int* inc_xres, double* res_xres);
 void orc_killwave(orc_grid* g, double dt);
+int  orc_write_snapshot(const orc_grid* g, const char* filename, double t);
 void orc_evolve_planet(orc_grid* g, double t);
 
 /* unit-level functions, on caller-supplied row arrays */

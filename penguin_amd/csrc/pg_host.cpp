@@ -19,7 +19,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <string>
 #include <vector>
@@ -203,6 +205,7 @@ struct pg_handle {
     void* dustT = nullptr;
     void* vis = nullptr;
     void* tab = nullptr;
+    void* phitab = nullptr;
     void* res = nullptr;
     int* inc = nullptr;
     double* rot = nullptr;
@@ -210,11 +213,13 @@ struct pg_handle {
     int dts_cap = 0;
     double* rate_part = nullptr;
     int rate_nblk = 0;
+    int tile_x = 256;   // x-sweep tile width (PG_TILE_X overrides)
     StepCtl* ctl = nullptr;
     StepCtl* ctl_host = nullptr;  // pinned mirror
     size_t bytes = 0;
     // host copies of what the state conversion needs
-    std::vector<double> tabh;  // T_NFIELDS * stride
+    std::vector<double> tabh;  // per-radius tables, field-major
+    std::vector<double> phih;  // 4*ny azimuthal sine/cosine table
     int tab_stride = 0;
     double dy = 0.0;
     double wly[3]{}, wry[3]{};
@@ -244,7 +249,7 @@ struct pg_handle {
         d.orbadv = P.OrbAdv_flag;
         d.kill = P.kill_flag;
         d.n_planet = P.n_planet;
-        d.tile_x = std::min(nx, 256);
+        d.tile_x = tile_x;
         d.sweep_on = 1;
         d.gas = (real*)gas;
         d.gasT = (real*)gasT;
@@ -254,6 +259,8 @@ struct pg_handle {
         d.tab = (const real*)tab;
         d.tab_stride = tab_stride;
         d.dy = (real)dy;
+        d.idy = (real)(1.0 / dy);
+        d.phitab = (const real*)phitab;
         for (int k = 0; k < 3; ++k) { d.wLy[k] = (real)wly[k]; d.wRy[k] = (real)wry[k]; }
         d.gam = (real)P.gam;
         d.sc_h = (real)P.sc_h;
@@ -380,7 +387,11 @@ void build_tables_and_state(pg_handle* h, std::vector<double>& gas, std::vector<
     }
     for (int ip = 0; ip < np; ++ip) {
         T(T_C0, ip) = std::sqrt(P.gam * T(T_CS2, ip));
+        T(T_IC0, ip) = 1.0 / T(T_C0, ip);
         T(T_IR, ip) = 1.0 / T(T_XC, ip);
+        T(T_IXV, ip) = 1.0 / T(T_XV, ip);
+        T(T_TS, ip) = P.Stokes / (T(T_C0, ip) / P.sc_h);
+        T(T_ITS, ip) = 1.0 / T(T_TS, ip);
     }
     for (int ip = 1; ip < np - 1; ++ip) {
         double xm = T(T_XC, ip - 1), x0 = T(T_XC, ip), xp = T(T_XC, ip + 1);
@@ -399,6 +410,15 @@ void build_tables_and_state(pg_handle* h, std::vector<double>& gas, std::vector<
     }
     slope_weights(P.geomy, &ya[NPAD - 2], h->wly, h->wry);
     h->dy = (P.ymax - P.ymin) / (double)ny;
+    // azimuthal cell centres (0.5*(ya[j]+ya[j+1]), init.cu via Grid::get_yc)
+    h->phih.assign(4 * (size_t)ny, 0.0);
+    for (int j = 0; j < ny; ++j) {
+        double phi = 0.5 * (ya[j + NPAD] + ya[j + NPAD + 1]);
+        h->phih[4 * j + 0] = std::sin(phi);
+        h->phih[4 * j + 1] = std::cos(phi);
+        h->phih[4 * j + 2] = std::sin(0.5 * phi);
+        h->phih[4 * j + 3] = std::cos(0.5 * phi);
+    }
     // wave-killing rate (killwave.cu:8-103): sin^2(pi/2 d) / R^1.5, with the
     // single-precision sine of the reference (killwave.cu:11)
     if (P.kill_flag) {
@@ -599,6 +619,11 @@ int pg_create(const pg_params* params, pg_handle** out) {
     h->prec = params->precision;
     h->nx = params->xres;
     h->ny = params->yres;
+    h->tile_x = (h->nx % 512 == 0) ? 512 : (h->nx == 384 ? 384 : (h->nx == 128 ? 128 : 256));
+    if (const char* e = std::getenv("PG_TILE_X")) {
+        int v = std::atoi(e);
+        if (v == 128 || v == 256 || v == 384 || v == 512 || v == 1024) h->tile_x = v;
+    }
     h->N = (size_t)h->nx * h->ny;
     auto cleanup = [&](int code) {
         pg_destroy(h);
@@ -614,7 +639,8 @@ int pg_create(const pg_params* params, pg_handle** out) {
     build_tables_and_state(h, gas, dust, rot);
     if ((rc = dev_alloc(h, &h->gas, 3 * N * es)) || (rc = dev_alloc(h, &h->gasT, 3 * N * es)) ||
         (rc = dev_alloc(h, &h->vis, 2 * N * es)) ||
-        (rc = dev_alloc(h, &h->tab, h->tabh.size() * es)) || (rc = dev_alloc(h, &h->res, h->nx * es)) ||
+        (rc = dev_alloc(h, &h->tab, h->tabh.size() * es)) || (rc = dev_alloc(h, &h->phitab, h->phih.size() * es)) ||
+        (rc = dev_alloc(h, &h->res, h->nx * es)) ||
         (rc = dev_alloc(h, &h->inc, h->nx * sizeof(int))) || (rc = dev_alloc(h, &h->rot, rot.size() * sizeof(double))) ||
         (rc = dev_alloc(h, &h->ctl, sizeof(StepCtl))))
         return cleanup(rc);
@@ -626,7 +652,8 @@ int pg_create(const pg_params* params, pg_handle** out) {
     if (hipHostMalloc((void**)&h->ctl_host, sizeof(StepCtl), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(PG_E_NOMEM, "hipHostMalloc failed"));
     std::memset(h->ctl_host, 0, sizeof(StepCtl));
-    if ((rc = upload(h, h->tabh, h->tab)) || (rc = upload(h, gas, h->gas)) || (rc = upload(h, gas, h->gasT)))
+    if ((rc = upload(h, h->tabh, h->tab)) || (rc = upload(h, h->phih, h->phitab)) || (rc = upload(h, gas, h->gas)) ||
+        (rc = upload(h, gas, h->gasT)))
         return cleanup(rc);
     if (h->P.dust_flag && ((rc = upload(h, dust, h->dust)) || (rc = upload(h, dust, h->dustT)))) return cleanup(rc);
     std::vector<double> zero(2 * N, 0.0);
@@ -648,7 +675,7 @@ int pg_destroy(pg_handle* h) {
     if (!h) return PG_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->graph) hipGraphExecDestroy(h->graph);
-    void* bufs[] = {h->gas, h->gasT, h->dust, h->dustT, h->vis, h->tab, h->res, h->inc, h->rot, h->dts, h->ctl, h->rate_part};
+    void* bufs[] = {h->gas, h->gasT, h->dust, h->dustT, h->vis, h->tab, h->phitab, h->res, h->inc, h->rot, h->dts, h->ctl, h->rate_part};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->ctl_host) hipHostFree(h->ctl_host);
@@ -872,6 +899,7 @@ int pg_sources(pg_handle* h, double t_planet, double mdt, double dt, int inplace
     // planet0 holds the planet at t_planet; modes 1/2 read planet1, so copy
     if ((rc = pull_ctl(h))) return rc;
     h->ctl_host->planet1 = h->ctl_host->planet0;
+    h->ctl_host->pk1 = h->ctl_host->pk0;
     if ((rc = push_ctl(h))) return rc;
     int mode;
     if (mdt == 0.0) {
@@ -1013,6 +1041,85 @@ int pg_killwave(pg_handle* h, double dt) {
                                 : Launch<float>::sources(h->args<float>(), 3, 1, 0, h->stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch killwave failed: %s", hipGetErrorString(e));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return PG_OK;
+}
+
+int pg_snapshot_label(const pg_params* P, char* buf, size_t len) {
+    if (!P || !buf || len == 0) return fail(PG_E_INVALID, "null argument");
+    // create_label (output.cu:14-93): each number passes through an int,
+    // i.e. truncation toward zero
+    auto I = [](double x) { return std::to_string((int)x); };
+    const double earth = 0.000003, jupiter = 0.0009543;
+    std::string s;
+    if (P->ndim == 1) s = I(P->xres);
+    else if (P->ndim == 2) s = I(P->xres) + "x" + I(P->yres);
+    else s = I(P->xres) + "x" + I(P->yres) + "x" + I(P->zres);
+    s += "_h" + I(1000.0 * P->sc_h);
+    if (P->n_planet > 0) {
+        if (P->planet_mass >= jupiter) s += "_" + I(P->n_planet) + "p" + I(P->planet_mass / jupiter) + "J";
+        else s += "_" + I(P->n_planet) + "p" + I(P->planet_mass / earth) + "E";
+        s += "_e" + I(P->planet_ecc * 1000.0);
+    }
+    if (P->visc_flag == 1) s += "_a" + I(std::log10(P->ss_alpha) * 10.0);
+    if (P->OrbAdv_flag) s += "_OA";
+    if (P->dust_flag) s += "_St" + I(std::log10(P->Stokes) * 10.0);
+    static const char* rec[] = {"_VAN", "_MOC", "_PEM2", "_PEM3", "_PPM3", "_PEM4", "_PPM4"};
+    if (P->recon_flag >= 0 && P->recon_flag <= 6) s += rec[P->recon_flag];
+    if (s.size() + 1 > len) return fail(PG_E_INVALID, "label buffer too small (%zu needed)", s.size() + 1);
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return PG_OK;
+}
+
+int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
+    CHECK_H(h);
+    if (!filename) return fail(PG_E_INVALID, "null filename");
+    const size_t N = h->N;
+    std::vector<double> gas(5 * N), dust(h->P.dust_flag ? 4 * N : 0);
+    int rc = pg_get_state(h, gas.data(), h->P.dust_flag ? dust.data() : nullptr);
+    if (rc) return rc;
+    std::ofstream f(filename, std::ios::out | std::ios::binary | std::ios::trunc);
+    if (!f) return fail(PG_E_INVALID, "cannot open %s for writing", filename);
+    std::vector<double> xa(h->nx + 1), ya(h->ny + 1);
+    for (int i = 0; i <= h->nx; ++i) xa[i] = tabv(h, T_XA, i + NPAD);
+    for (int j = 0; j <= h->ny; ++j) ya[j] = h->P.ymin + (h->P.ymax - h->P.ymin) * (double)j / (double)h->ny;
+    f.write((const char*)&t, sizeof(double));
+    f.write((const char*)xa.data(), sizeof(double) * xa.size());
+    f.write((const char*)ya.data(), sizeof(double) * ya.size());
+    f.write((const char*)gas.data(), sizeof(double) * 4 * N);  // rho, p, u, v (ndim 2: no w)
+    if (h->P.dust_flag) f.write((const char*)dust.data(), sizeof(double) * 4 * N);  // w_d too (output.cu:171)
+    if (!f) return fail(PG_E_INVALID, "write to %s failed", filename);
+    return PG_OK;
+}
+
+int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out) {
+    CHECK_H(h);
+    if (!filename) return fail(PG_E_INVALID, "null filename");
+    const size_t N = h->N;
+    const size_t nfield = 4 + (h->P.dust_flag ? 4 : 0);
+    const size_t expect = sizeof(double) * (1 + (h->nx + 1) + (h->ny + 1) + nfield * N);
+    std::ifstream f(filename, std::ios::in | std::ios::binary);
+    if (!f) return fail(PG_E_INVALID, "cannot open %s", filename);
+    f.seekg(0, std::ios::end);
+    size_t size = (size_t)f.tellg();
+    if (size != expect)
+        return fail(PG_E_INVALID, "%s has %zu bytes, a %dx%d%s snapshot has %zu", filename, size, h->nx, h->ny,
+                    h->P.dust_flag ? " gas+dust" : "", expect);
+    f.seekg(0);
+    double t = 0.0;
+    f.read((char*)&t, sizeof(double));
+    std::vector<double> xa(h->nx + 1), ya(h->ny + 1);
+    f.read((char*)xa.data(), sizeof(double) * xa.size());
+    f.read((char*)ya.data(), sizeof(double) * ya.size());
+    for (int i = 0; i <= h->nx; ++i)
+        if (std::fabs(xa[i] - tabv(h, T_XA, i + NPAD)) > 1e-12 * std::fabs(xa[i]))
+            return fail(PG_E_INVALID, "%s: radial grid does not match this disk", filename);
+    std::vector<double> gas(5 * N, 0.0), dust(h->P.dust_flag ? 4 * N : 0);
+    f.read((char*)gas.data(), sizeof(double) * 4 * N);
+    if (h->P.dust_flag) f.read((char*)dust.data(), sizeof(double) * 4 * N);
+    if (!f) return fail(PG_E_INVALID, "read from %s failed", filename);
+    int rc = pg_set_state(h, gas.data(), h->P.dust_flag ? dust.data() : nullptr);
+    if (rc) return rc;
+    if (t_out) *t_out = t;
     return PG_OK;
 }
 

@@ -34,6 +34,12 @@ struct Body {
     double m, a, e, x, y, z, vx, vy, vz, rs;
 };
 
+// A planet as the source kernel uses it: mass, orbital radius, softening,
+// m/x^2 (indirect term) and the sine/cosine of its angle and half angle.
+struct PlanetK {
+    double m, x, rs, m_over_x2, sphi, cphi, shphi, chphi;
+};
+
 // Per-step control block, device resident, always FP64.
 struct StepCtl {
     double t;            // time at the start of the step
@@ -45,6 +51,7 @@ struct StepCtl {
     int mode;            // 0: dt supplied by host, 1: dt from rate (device loop)
     Body planet0;        // planet at t     (first half kick)
     Body planet1;        // planet at t+dt  (predictor and corrector)
+    PlanetK pk0, pk1;    // the same, as the source kernel reads them
 };
 
 // Per-radius tables (FP64 on the host; cast to the storage type on upload).
@@ -69,6 +76,10 @@ enum TabField {
     T_I21,        // 1/(xc[ip+1]-xc[ip])
     T_I10,        // 1/(xc[ip]-xc[ip-1])
     T_IR,         // 1/xc
+    T_IXV,        // 1/xv
+    T_IC0,        // 1/c_s
+    T_TS,         // St * h0 / c_s: dust stopping time times rho (dust.cu:1-4)
+    T_ITS,        // 1/T_TS
     T_NFIELDS
 };
 
@@ -84,9 +95,11 @@ struct DiskArgs {
     real* dust;          // 3*N
     real* dustT;         // 3*N
     real* vis;           // 2*N viscous acceleration*rho
-    const real* tab;     // T_NFIELDS * tab_stride
-    int tab_stride;      // nx + 2*NPAD + 1
+    const real* tab;     // per-radius tables, field-major: tab[f*tab_stride + ip]
+    int tab_stride;      // nx + 2*NPAD + 1 radii
     real dy;             // azimuthal cell width (angle), uniform grid
+    real idy;            // 1/dy
+    const real* phitab;  // 4*ny: sin(phi_j), cos(phi_j), sin(phi_j/2), cos(phi_j/2)
     real wLy[3], wRy[3]; // 4th-order weights of the uniform azimuthal grid
     real gam, sc_h, Stokes, D_G_ratio, frame_omega;
     double CFL;

@@ -34,7 +34,25 @@
 
 namespace pg {
 
+// per-radius tables, field-major: lanes that differ in radius read
+// consecutive addresses
 #define TAB(f, ip) d.tab[(size_t)(f) * d.tab_stride + (ip)]
+
+// Division, reciprocal, square root and exponential.  FP32: the hardware
+// approximations (v_rcp_f32, v_sqrt_f32, v_exp_f32; about 1 ulp); FP64: IEEE.
+template <typename real> struct FM;
+template <> struct FM<float> {
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+};
+template <> struct FM<double> {
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+};
 
 // ---------------------------------------------------------------- helpers
 template <typename real>
@@ -96,13 +114,14 @@ __device__ __forceinline__ real flatten5(real pm2, real pm1, real p0, real pp1, 
                                          real um1, real u0, real up1, real cs) {
     real f = real(0);
     real dp1f = pp1 - p0, dp2f = pp2 - pm1, dp1b = p0 - pm1, dp2b = pp1 - pm2;
-    real Mf = (u0 - up1) / cs, Mb = (um1 - u0) / cs;
-    if (Mf > real(0) || Mb > real(0)) {
+    // Mf > 0 || Mb > 0 with Mf = (u0-up1)/cs, Mb = (um1-u0)/cs and cs > 0
+    (void)cs;
+    if (u0 - up1 > real(0) || um1 - u0 > real(0)) {
         if (dp2f != real(0) && dp2b != real(0)) {
-            real ff = real(10) * fmax(real(0), fabs(dp1f / dp2f - real(0.5)) - real(0.25));
-            real fb = real(10) * fmax(real(0), fabs(dp1b / dp2b - real(0.5)) - real(0.25));
+            real ff = real(10) * fmax(real(0), fabs(FM<real>::div(dp1f, dp2f) - real(0.5)) - real(0.25));
+            real fb = real(10) * fmax(real(0), fabs(FM<real>::div(dp1b, dp2b) - real(0.5)) - real(0.25));
             f = fmax(ff, fb);
-            f = fmin(real(0.5), f * fmax(fabs(dp1f), fabs(dp1b)) / p0);
+            f = fmin(real(0.5), FM<real>::div(f * fmax(fabs(dp1f), fabs(dp1b)), p0));
         }
     }
     return f;
@@ -126,15 +145,16 @@ __device__ __forceinline__ void trace_right(real c, real dt, real idx, real r, r
     real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
     real p0 = fmax(aveL(q, psL, p, psR), p * real(1.0e-10));
     real u0 = aveL(q, usL, u, usR);
-    real cs = sqrt(p0 / r0);
+    real ir0 = FM<real>::rcp(r0);
+    real cs = FM<real>::sqrt(p0 * ir0);
     q = lim01((cs * dt - dis) * idx);
     real u_ = aveL(q, usL, u, usR);
     real p_ = aveL(q, psL, p, psR);
-    real ics = real(1) / cs;
+    real ics = FM<real>::rcp(cs);
     real e = exp_lim((u0 - u_) * ics);
     rr = r0 * e;
     pr = p0 * e;
-    ur = u0 + (p0 - p_) * ics / r0;
+    ur = u0 + (p0 - p_) * ics * ir0;
     vr = aveL(q, vsL, v, vsR);
     cout = cs;
 }
@@ -152,15 +172,16 @@ __device__ __forceinline__ void trace_left(real c, real dt, real idx, real r, re
     real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
     real p0 = fmax(aveR(x, psL, p, psR), p * real(1.0e-10));
     real u0 = aveR(x, usL, u, usR);
-    real cs = sqrt(p0 / r0);
+    real ir0 = FM<real>::rcp(r0);
+    real cs = FM<real>::sqrt(p0 * ir0);
     x = lim01((dis + cs * dt) * idx);
     real u_ = aveR(x, usL, u, usR);
     real p_ = aveR(x, psL, p, psR);
-    real ics = real(1) / cs;
+    real ics = FM<real>::rcp(cs);
     real e = exp_lim((u_ - u0) * ics);
     rl = r0 * e;
     pl = p0 * e;
-    ul = u0 + (p_ - p0) * ics / r0;
+    ul = u0 + (p_ - p0) * ics * ir0;
     vl = aveR(x, vsL, v, vsR);
     cout = cs;
 }
@@ -169,7 +190,7 @@ __device__ __forceinline__ void trace_left(real c, real dt, real idx, real r, re
 // with c fixed: the azimuthal sweep of a locally isothermal disk); pressure
 // is c^2 times density throughout, so only rho, u, v are traced.
 template <typename real>
-__device__ __forceinline__ void trace_right_c(real c, real dt, real idx, real r, real u,
+__device__ __forceinline__ void trace_right_c(real c, real ic, real dt, real idx, real r, real u,
                                               real rsL, real rsR, real usL, real usR,
                                               real vsL, real v, real vsR,
                                               real& rr, real& ur, real& vr) {
@@ -182,13 +203,13 @@ __device__ __forceinline__ void trace_right_c(real c, real dt, real idx, real r,
     q = lim01((c * dt - dis) * idx);
     real u_ = aveL(q, usL, u, usR);
     real ra = aveL(q, rsL, r, rsR);
-    rr = r0 * exp_lim((u0 - u_) / c);
-    ur = u0 + c * (r0 - ra) / r0;
+    rr = r0 * exp_lim((u0 - u_) * ic);
+    ur = u0 + c * FM<real>::div(r0 - ra, r0);
     vr = aveL(q, vsL, v, vsR);
 }
 
 template <typename real>
-__device__ __forceinline__ void trace_left_c(real c, real dt, real idx, real r, real u,
+__device__ __forceinline__ void trace_left_c(real c, real ic, real dt, real idx, real r, real u,
                                              real rsL, real rsR, real usL, real usR,
                                              real vsL, real v, real vsR,
                                              real& rl, real& ul, real& vl) {
@@ -201,8 +222,8 @@ __device__ __forceinline__ void trace_left_c(real c, real dt, real idx, real r, 
     x = lim01((dis + c * dt) * idx);
     real u_ = aveR(x, usL, u, usR);
     real ra = aveR(x, rsL, r, rsR);
-    rl = r0 * exp_lim((u_ - u0) / c);
-    ul = u0 + c * (ra - r0) / r0;
+    rl = r0 * exp_lim((u_ - u0) * ic);
+    ul = u0 + c * FM<real>::div(ra - r0, r0);
     vl = aveR(x, vsL, v, vsR);
 }
 
@@ -217,10 +238,10 @@ __device__ __forceinline__ void hllc_select(real pm, real sl, real sm, real sr,
     } else if (signbit(sr)) {
         Fr = ur * rr; us = ur; vs = vr; pres = pr;
     } else if (!signbit(sm)) {
-        real k = real(1) / (sl - sm);
+        real k = FM<real>::rcp(sl - sm);
         Fr = sm * rl * (sl - ul) * k; us = ul; vs = vl; pres = (sl * pm - sm * pl) * k;
     } else {
-        real k = real(1) / (sr - sm);
+        real k = FM<real>::rcp(sr - sm);
         Fr = sm * rr * (sr - ur) * k; us = ur; vs = vr; pres = (sr * pm - sm * pr) * k;
     }
     us *= Fr;
@@ -236,33 +257,35 @@ __device__ __forceinline__ void hllc_iso(real cl, real cr, real rl, real pl, rea
                                          real rr, real pr, real ur, real vr,
                                          real& Fr, real& us, real& vs, real& pres) {
     real csum = cl + cr;
-    real pm = real(0.5) * (pr + pl) * exp_lim((ul - ur) / csum);
-    real srl = sqrt(rl), srr = sqrt(rr);
-    real iss = real(1) / (srl + srr);
+    real ics = FM<real>::rcp(csum);
+    real pm = real(0.5) * (pr + pl) * exp_lim((ul - ur) * ics);
+    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
+    real iss = FM<real>::rcp(srl + srr);
     real ubar = (srl * ul + srr * ur) * iss;
     real n2 = real(0.5) * srl * srr * iss * iss;
     real du = ur - ul;
-    real cbar = sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
+    real cbar = FM<real>::sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
     real sl = ubar - cbar, sr = ubar + cbar;
-    real sm = (rl * ul * cl + rr * ur * cr) / (rl * cl + rr * cr) + real(2) * (pl - pr) / ((rl + rr) * csum);
+    real sm = FM<real>::div(rl * ul * cl + rr * ur * cr, rl * cl + rr * cr) +
+              real(2) * (pl - pr) * ics * FM<real>::rcp(rl + rr);
     hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
 }
 
 // The same with one sound speed c on both sides (azimuthal isothermal sweep):
 // cbar^2 = c^2 + n2 du^2 and sm = (rl (ul + c) + rr (ur - c)) / (rl + rr).
 template <typename real>
-__device__ __forceinline__ void hllc_c(real c, real c2, real rl, real ul, real vl, real rr, real ur, real vr,
+__device__ __forceinline__ void hllc_c(real c, real ic, real c2, real rl, real ul, real vl, real rr, real ur, real vr,
                                        real& Fr, real& us, real& vs, real& pres) {
     real pl = c2 * rl, pr = c2 * rr;
-    real pm = real(0.5) * (pr + pl) * exp_lim(real(0.5) * (ul - ur) / c);
-    real srl = sqrt(rl), srr = sqrt(rr);
-    real iss = real(1) / (srl + srr);
+    real pm = real(0.5) * (pr + pl) * exp_lim(real(0.5) * (ul - ur) * ic);
+    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
+    real iss = FM<real>::rcp(srl + srr);
     real ubar = (srl * ul + srr * ur) * iss;
     real n2 = real(0.5) * srl * srr * iss * iss;
     real du = ur - ul;
-    real cbar = sqrt(c2 + n2 * du * du);
+    real cbar = FM<real>::sqrt(c2 + n2 * du * du);
     real sl = ubar - cbar, sr = ubar + cbar;
-    real sm = (rl * (ul + c) + rr * (ur - c)) / (rl + rr);
+    real sm = FM<real>::div(rl * (ul + c) + rr * (ur - c), rl + rr);
     hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
 }
 
@@ -281,7 +304,7 @@ __device__ __forceinline__ void dust_iface(real dt,
     real y = -fmin(uR - uRsL, real(0)) * dt * idxR;
     real q = lim01(y);
     real ur = aveL(q, uRsL, uR, uRsR);
-    real du = dt * (ur - uR) * idxR / (real(1) - y);
+    real du = FM<real>::div(dt * (ur - uR) * idxR, real(1) - y);
     q = lim01(-fmin(ur, real(0)) * dt * idxR);
     real srr = aveL(q, rRsL, rR, rRsR) * exp_lim(du);
     real sur = aveL(q, uRsL, uR, uRsR);
@@ -290,7 +313,7 @@ __device__ __forceinline__ void dust_iface(real dt,
     y = fmax(uL + uLsR, real(0)) * dt * idxL;
     real x = lim01(y);
     real ul = aveR(x, uLsL, uL, uLsR);
-    du = dt * (uL - ul) * idxL / (real(1) - y);
+    du = FM<real>::div(dt * (uL - ul) * idxL, real(1) - y);
     x = lim01(fmax(ul, real(0)) * dt * idxL);
     real srl = aveR(x, rLsL, rL, rLsR) * exp_lim(du);
     real sul = aveR(x, uLsL, uL, uLsR);
@@ -350,46 +373,47 @@ __host__ __device__ inline void orbit_solution(Body& b, double t, double ramp_ti
 }
 
 // ------------------------------------------------------------ x sweep, gas
-// One block = one tile of `tile_x` interior cells of one azimuthal row, plus
-// 3 ghost/halo cells each side, staged in LDS.  Phases (barrier between):
-// load -> flatten -> parabolas -> interface fluxes -> conservative update.
-template <typename real>
-__global__ __launch_bounds__(256) void k_sweep_x_gas(DiskArgs<real> d) {
+// One block = TX interior cells of one azimuthal row (the whole row when
+// nr <= 1024) plus 3 ghost/halo cells each side, staged in LDS; thread t
+// owns slot t (interior cell t-3 of the tile).  Phases (barrier between):
+// load -> flattening -> parabolas -> interface fluxes -> conservative update.
+// TX is a compile-time constant so every LDS array offset is an immediate.
+template <typename real, int TX>
+__global__ __launch_bounds__(TX) void k_sweep_x_gas(DiskArgs<real> d) {
+    constexpr int S = TX + 2 * NPAD;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
-    const int nx = d.nx;
-    const size_t N = (size_t)nx * d.ny;
-    const int TX = d.tile_x;
-    const int j = blockIdx.y;
-    const int x0 = blockIdx.x * TX;
-    const int nt = min(TX, nx - x0);
-    const int ns = nt + 2 * NPAD;
-    const int S = TX + 2 * NPAD;  // LDS array stride
     real* s_r = sm;
     real* s_p = sm + S;
     real* s_u = sm + 2 * S;
     real* s_L = sm + 3 * S;
-    real* s_f = sm + 4 * S;                 // flattening, later Fr
-    real* s_sl = sm + 5 * S;                // 4 fields: r, p, u, L
-    real* s_sr = sm + 9 * S;
-    real* s_Fu = sm + 13 * S;
-    real* s_FL = sm + 14 * S;
-    real* s_pr = sm + 15 * S;
-    real* s_Fr = sm + 16 * S;
-    real* s_vp = sm + 17 * S;
+    real* s_vp = sm + 4 * S;
+    real* s_f = sm + 5 * S;
+    real* s_sl = sm + 6 * S;    // 4 fields: r, p, u, L
+    real* s_sr = sm + 10 * S;
+    real* s_Fr = sm + 14 * S;
+    real* s_Fu = sm + 15 * S;
+    real* s_FL = sm + 16 * S;
+    real* s_pr = sm + 17 * S;
+    const int nx = d.nx;
+    const size_t N = (size_t)nx * d.ny;
+    const int t = threadIdx.x;
+    const int j = blockIdx.y;
+    const int x0 = blockIdx.x * TX;
+    const int nt = min(TX, nx - x0);
+    const int ns = nt + 2 * NPAD;
     const real dt = (real)d.ctl->dt;
-    const real gam = d.gam;
     const real* __restrict__ in = d.gas;
+    const size_t row = (size_t)j * nx;
 
-    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-        int i = x0 - NPAD + s;
-        int ip = i + NPAD;
+    for (int s = t; s < ns; s += TX) {
+        const int i = x0 - NPAD + s;
+        const int ip = i + NPAD;
         real r, u, vp;
         if (i < 0 || i >= nx) {
             r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); vp = TAB(T_G_V, ip);
         } else {
-            size_t o = (size_t)j * nx + i;
-            r = in[o]; u = in[N + o]; vp = in[2 * N + o];
+            r = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
         }
         s_r[s] = r;
         s_p[s] = TAB(T_CS2, ip) * r;
@@ -398,30 +422,29 @@ __global__ __launch_bounds__(256) void k_sweep_x_gas(DiskArgs<real> d) {
         s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
     }
     __syncthreads();
-    for (int s = threadIdx.x + 2; s < ns - 2; s += blockDim.x) {
-        real cs = TAB(T_C0, x0 + s);   // sqrt(gam p / rho) with p = c_s^2 rho, gam = 1
+    for (int s = t + 2; s < ns - 2; s += TX) {
         s_f[s] = flatten5(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2],
-                          s_u[s - 1], s_u[s], s_u[s + 1], cs);
+                          s_u[s - 1], s_u[s], s_u[s + 1], real(1));
     }
     __syncthreads();
-    for (int s = threadIdx.x + 2; s < ns - 2; s += blockDim.x) {
-        int ip = x0 + s;
-        real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
-        real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
-        real fl = s_f[s];
-        const real* fields[4] = {s_r, s_p, s_u, s_L};
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const real* a = fields[f];
-            real l, r;
-            slopes(a[s - 2], a[s - 1], a[s], a[s + 1], a[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
-            s_sl[f * S + s] = l;
-            s_sr[f * S + s] = r;
-        }
+    for (int s = t + 2; s < ns - 2; s += TX) {
+        const int ip = x0 + s;
+        const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
+        const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
+        const real fl = s_f[s];
+        real l, r;
+        slopes(s_r[s - 2], s_r[s - 1], s_r[s], s_r[s + 1], s_r[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+        s_sl[s] = l; s_sr[s] = r;
+        slopes(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+        s_sl[S + s] = l; s_sr[S + s] = r;
+        slopes(s_u[s - 2], s_u[s - 1], s_u[s], s_u[s + 1], s_u[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+        s_sl[2 * S + s] = l; s_sr[2 * S + s] = r;
+        slopes(s_L[s - 2], s_L[s - 1], s_L[s], s_L[s + 1], s_L[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+        s_sl[3 * S + s] = l; s_sr[3 * S + s] = r;
     }
     __syncthreads();
-    for (int k = threadIdx.x + NPAD; k < ns - 2; k += blockDim.x) {
-        int ipk = x0 + k;
+    for (int k = t + NPAD; k < ns - 2; k += TX) {
+        const int ipk = x0 + k;
         real rl, pl, ul, vl, rr, pr, ur, vr, cl, cr;
         trace_left(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_p[k - 1], s_u[k - 1],
                    s_sl[k - 1], s_sr[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
@@ -432,7 +455,7 @@ __global__ __launch_bounds__(256) void k_sweep_x_gas(DiskArgs<real> d) {
                     s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, pr, ur, vr, cr);
         real Fr, Fu, FL, pres;
         hllc_iso(cl, cr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, FL, pres);
-        real g = TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
+        const real g = TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
         s_Fr[k] = Fr * g;
         s_Fu[k] = Fu * g;
         s_FL[k] = FL * g;
@@ -440,66 +463,65 @@ __global__ __launch_bounds__(256) void k_sweep_x_gas(DiskArgs<real> d) {
     }
     __syncthreads();
     real* __restrict__ out = d.gasT;
-    for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += blockDim.x) {
-        int ip = x0 + s;
-        int i = ip - NPAD;
-        real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
-        real f = dt / TAB(T_XV, ip);
-        real R = TAB(T_XC, ip);
-        real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
-        real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1])) * f;
-        real DL = (s_FL[s] - s_FL[s + 1]) * f;
-        real r = s_r[s], u = s_u[s], vp = s_vp[s];
-        size_t o = (size_t)j * nx + i;
+    for (int s = t + NPAD; s < ns - NPAD; s += TX) {
+        const int ip = x0 + s;
+        const int i = ip - NPAD;
+        const real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
+        const real f = dt * TAB(T_IXV, ip);
+        const real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
+        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1])) * f;
+        const real DL = (s_FL[s] - s_FL[s + 1]) * f;
+        const real r = s_r[s], u = s_u[s], vp = s_vp[s];
         real rn = r + Dr;
-        real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
+        const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
         real un = u, vn = vp;
         if (rn < rmin) {
             rn = rmin;
         } else {
-            un = (r * u + Du) / rn;
-            vn = (r * vp + (DL / R - TAB(T_VBG, ip) * Dr)) / rn;
+            const real irn = FM<real>::rcp(rn);
+            un = (r * u + Du) * irn;
+            vn = (r * vp + (DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr)) * irn;
         }
-        out[o] = rn;
-        out[N + o] = un;
-        out[2 * N + o] = vn;
+        out[row + i] = rn;
+        out[N + row + i] = un;
+        out[2 * N + row + i] = vn;
     }
 }
 
 // ----------------------------------------------------------- x sweep, dust
-template <typename real>
-__global__ __launch_bounds__(256) void k_sweep_x_dust(DiskArgs<real> d) {
+template <typename real, int TX>
+__global__ __launch_bounds__(TX) void k_sweep_x_dust(DiskArgs<real> d) {
+    constexpr int S = TX + 2 * NPAD;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
+    real* s_r = sm;
+    real* s_u = sm + S;
+    real* s_L = sm + 2 * S;
+    real* s_vp = sm + 3 * S;
+    real* s_sl = sm + 4 * S;  // 3 fields: r, u, L
+    real* s_sr = sm + 7 * S;
+    real* s_Fr = sm + 10 * S;
+    real* s_Fu = sm + 11 * S;
+    real* s_FL = sm + 12 * S;
     const int nx = d.nx;
     const size_t N = (size_t)nx * d.ny;
-    const int TX = d.tile_x;
+    const int t = threadIdx.x;
     const int j = blockIdx.y;
     const int x0 = blockIdx.x * TX;
     const int nt = min(TX, nx - x0);
     const int ns = nt + 2 * NPAD;
-    const int S = TX + 2 * NPAD;
-    real* s_r = sm;
-    real* s_u = sm + S;
-    real* s_L = sm + 2 * S;
-    real* s_sl = sm + 3 * S;  // 3 fields: r, u, L
-    real* s_sr = sm + 6 * S;
-    real* s_Fr = sm + 9 * S;
-    real* s_Fu = sm + 10 * S;
-    real* s_FL = sm + 11 * S;
-    real* s_vp = sm + 12 * S;
     const real dt = (real)d.ctl->dt;
     const real* __restrict__ in = d.dust;
+    const size_t row = (size_t)j * nx;
 
-    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-        int i = x0 - NPAD + s;
-        int ip = i + NPAD;
+    for (int s = t; s < ns; s += TX) {
+        const int i = x0 - NPAD + s;
+        const int ip = i + NPAD;
         real r, u, vp;
         if (i < 0 || i >= nx) {
             r = TAB(T_D_R, ip); u = TAB(T_D_U, ip); vp = TAB(T_D_V, ip);
         } else {
-            size_t o = (size_t)j * nx + i;
-            r = in[o]; u = in[N + o]; vp = in[2 * N + o];
+            r = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
         }
         s_r[s] = r;
         s_u[s] = u;
@@ -507,86 +529,84 @@ __global__ __launch_bounds__(256) void k_sweep_x_dust(DiskArgs<real> d) {
         s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
     }
     __syncthreads();
-    for (int s = threadIdx.x + 2; s < ns - 2; s += blockDim.x) {
-        int ip = x0 + s;
-        real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
-        real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
-        const real* fields[3] = {s_r, s_u, s_L};
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            const real* a = fields[f];
-            real l, r;
-            slopes(a[s - 2], a[s - 1], a[s], a[s + 1], a[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            s_sl[f * S + s] = l;
-            s_sr[f * S + s] = r;
-        }
+    for (int s = t + 2; s < ns - 2; s += TX) {
+        const int ip = x0 + s;
+        const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
+        const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
+        real l, r;
+        slopes(s_r[s - 2], s_r[s - 1], s_r[s], s_r[s + 1], s_r[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+        s_sl[s] = l; s_sr[s] = r;
+        slopes(s_u[s - 2], s_u[s - 1], s_u[s], s_u[s + 1], s_u[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+        s_sl[S + s] = l; s_sr[S + s] = r;
+        slopes(s_L[s - 2], s_L[s - 1], s_L[s], s_L[s + 1], s_L[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
+        s_sl[2 * S + s] = l; s_sr[2 * S + s] = r;
     }
     __syncthreads();
-    for (int k = threadIdx.x + NPAD; k < ns - 2; k += blockDim.x) {
-        int ipk = x0 + k;
+    for (int k = t + NPAD; k < ns - 2; k += TX) {
+        const int ipk = x0 + k;
         real Fr, Fu, FL;
         dust_iface(dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_sl[k - 1], s_sr[k - 1],
                    s_u[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
                    s_L[k - 1], s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
                    TAB(T_IDX, ipk), s_r[k], s_sl[k], s_sr[k], s_u[k], s_sl[S + k], s_sr[S + k],
                    s_L[k], s_sl[2 * S + k], s_sr[2 * S + k], Fr, Fu, FL);
-        real g = TAB(T_XA, ipk);
+        const real g = TAB(T_XA, ipk);
         s_Fr[k] = Fr * g;
         s_Fu[k] = Fu * g;
         s_FL[k] = FL * g;
     }
     __syncthreads();
     real* __restrict__ out = d.dustT;
-    for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += blockDim.x) {
-        int ip = x0 + s;
-        int i = ip - NPAD;
-        real f = dt / TAB(T_XV, ip);
-        real R = TAB(T_XC, ip);
-        real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
-        real Du = (s_Fu[s] - s_Fu[s + 1]) * f;
-        real DL = (s_FL[s] - s_FL[s + 1]) * f;
-        size_t o = (size_t)j * nx + i;
-        real r = s_r[s], u = s_u[s], vp = s_vp[s];
+    const real rfloor = d.D_G_ratio * real(1.0e-10);
+    for (int s = t + NPAD; s < ns - NPAD; s += TX) {
+        const int ip = x0 + s;
+        const int i = ip - NPAD;
+        const real f = dt * TAB(T_IXV, ip);
+        const real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
+        const real Du = (s_Fu[s] - s_Fu[s + 1]) * f;
+        const real DL = (s_FL[s] - s_FL[s + 1]) * f;
+        const real r = s_r[s], u = s_u[s], vp = s_vp[s];
         real rn = r + Dr;
-        real rmin = fmax(r * real(1.0e-10), d.D_G_ratio * real(1.0e-10));
+        const real rmin = fmax(r * real(1.0e-10), rfloor);
         real un = u, vn = vp;
         if (rn < rmin) {
             rn = rmin;
         } else {
-            un = (r * u + Du) / rn;
-            vn = (r * vp + (DL / R - TAB(T_VBG, ip) * Dr)) / rn;
+            const real irn = FM<real>::rcp(rn);
+            un = (r * u + Du) * irn;
+            vn = (r * vp + (DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr)) * irn;
         }
-        out[o] = rn;
-        out[N + o] = un;
-        out[2 * N + o] = vn;
+        out[row + i] = rn;
+        out[N + row + i] = un;
+        out[2 * N + row + i] = vn;
     }
 }
 
 // ------------------------------------------------------- azimuthal ring
-// One block = one radius.  Stage A sweeps the ring in chunks of blockDim
-// cells (periodic halo) and writes each updated cell to its orbitally
-// shifted position (j + inc) mod ny of an LDS ring; stage B advects that
-// ring by the residual speed and writes the result to HBM.
-template <typename real>
+// One block = one radius.  Stage A sweeps the ring in chunks of B cells
+// (periodic halo) and writes each updated cell to its orbitally shifted
+// position (j + inc) mod ny of an LDS ring; stage B advects that ring by the
+// residual speed and writes the result to HBM.
+template <typename real, int B>
 struct RingLds {
+    static constexpr int S = B + 2 * NPAD;
     real* ring;  // 3*ny
-    real *c0, *c1, *c2;   // chunk state (slots)
-    real* fl;
-    real *sl0, *sr0, *sl1, *sr1, *sl2, *sr2;
-    real *F0, *F1, *F2, *F3;
+    real* b;     // chunk scratch: 14 arrays of S
+    __device__ __forceinline__ real* c0() const { return b; }
+    __device__ __forceinline__ real* c1() const { return b + S; }
+    __device__ __forceinline__ real* c2() const { return b + 2 * S; }
+    __device__ __forceinline__ real* fl() const { return b + 3 * S; }
+    __device__ __forceinline__ real* sl0() const { return b + 4 * S; }
+    __device__ __forceinline__ real* sr0() const { return b + 5 * S; }
+    __device__ __forceinline__ real* sl1() const { return b + 6 * S; }
+    __device__ __forceinline__ real* sr1() const { return b + 7 * S; }
+    __device__ __forceinline__ real* sl2() const { return b + 8 * S; }
+    __device__ __forceinline__ real* sr2() const { return b + 9 * S; }
+    __device__ __forceinline__ real* F0() const { return b + 10 * S; }
+    __device__ __forceinline__ real* F1() const { return b + 11 * S; }
+    __device__ __forceinline__ real* F2() const { return b + 12 * S; }
+    __device__ __forceinline__ real* F3() const { return b + 13 * S; }
 };
-
-template <typename real>
-__device__ __forceinline__ RingLds<real> ring_lds(real* sm, int ny, int S) {
-    RingLds<real> L;
-    L.ring = sm;
-    real* b = sm + 3 * ny;
-    L.c0 = b; L.c1 = b + S; L.c2 = b + 2 * S; L.fl = b + 3 * S;
-    L.sl0 = b + 4 * S; L.sr0 = b + 5 * S; L.sl1 = b + 6 * S; L.sr1 = b + 7 * S;
-    L.sl2 = b + 8 * S; L.sr2 = b + 9 * S;
-    L.F0 = b + 10 * S; L.F1 = b + 11 * S; L.F2 = b + 12 * S; L.F3 = b + 13 * S;
-    return L;
-}
 
 // Radius handled by ring block b.  Workgroups are dispatched round-robin
 // over the 8 XCDs (each with its own L2); giving XCD x the contiguous radii
@@ -603,81 +623,160 @@ __device__ __forceinline__ int wrap(int j, int n) {
     return j < 0 ? j + n : j;
 }
 
-template <typename real>
-__global__ __launch_bounds__(512) void k_ring_gas(DiskArgs<real> d) {
+// parabolas of the three ring fields of slot s (flattening coefficient fl)
+template <typename real, int B>
+__device__ __forceinline__ void ring_slopes(const RingLds<real, B>& L, int s, real fl, const real* w) {
+    real l, r;
+    const real* c0 = L.c0();
+    const real* c1 = L.c1();
+    const real* c2 = L.c2();
+    slopes(c0[s - 2], c0[s - 1], c0[s], c0[s + 1], c0[s + 2], w[0], w[1], w[2], w[3], w[4], w[5], fl, l, r);
+    L.sl0()[s] = l; L.sr0()[s] = r;
+    slopes(c1[s - 2], c1[s - 1], c1[s], c1[s + 1], c1[s + 2], w[0], w[1], w[2], w[3], w[4], w[5], fl, l, r);
+    L.sl1()[s] = l; L.sr1()[s] = r;
+    slopes(c2[s - 2], c2[s - 1], c2[s], c2[s + 1], c2[s + 2], w[0], w[1], w[2], w[3], w[4], w[5], fl, l, r);
+    L.sl2()[s] = l; L.sr2()[s] = r;
+}
+
+// Stage B (residual advection, advection.cu:1-105) over the shifted ring;
+// `dust` selects update_Dust's floor (advection.cu:315) over
+// update_Cell_advect (advection.cu:88-105, no floor).
+template <typename real, int B>
+__device__ __forceinline__ void ring_advect(const DiskArgs<real>& d, const RingLds<real, B>& L, int i, real* __restrict__ out,
+                                            real speed, real frac, real fad, bool dust, real rfloor, const real* w) {
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int t = threadIdx.x;
+    const real q = lim01(frac);
+    for (int c0 = 0; c0 < ny; c0 += B) {
+        const int nt = min(B, ny - c0);
+        const int ns = nt + 2 * NPAD;
+        for (int s = t; s < ns; s += B) {
+            const int jj = wrap(c0 - NPAD + s, ny);
+            L.c0()[s] = L.ring[jj];
+            L.c1()[s] = L.ring[ny + jj];
+            L.c2()[s] = L.ring[2 * ny + jj];
+        }
+        __syncthreads();
+        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, real(0), w);
+        __syncthreads();
+        for (int k = t + NPAD; k < ns - 2; k += B) {
+            real Dr, Dv, Du;
+            if (speed > real(0)) {
+                const int c = k - 1;
+                Dr = aveR(q, L.sl0()[c], L.c0()[c], L.sr0()[c]);
+                Dv = aveR(q, L.sl1()[c], L.c1()[c], L.sr1()[c]) * Dr;
+                Du = aveR(q, L.sl2()[c], L.c2()[c], L.sr2()[c]) * Dr;
+            } else {
+                const int c = k;
+                Dr = aveL(q, L.sl0()[c], L.c0()[c], L.sr0()[c]);
+                Dv = aveL(q, L.sl1()[c], L.c1()[c], L.sr1()[c]) * Dr;
+                Du = aveL(q, L.sl2()[c], L.c2()[c], L.sr2()[c]) * Dr;
+            }
+            L.F0()[k] = Dr * speed; L.F1()[k] = Dv * speed; L.F2()[k] = Du * speed;
+        }
+        __syncthreads();
+        for (int s = t + NPAD; s < ns - NPAD; s += B) {
+            const int jj = c0 - NPAD + s;
+            const real Dr = (L.F0()[s] - L.F0()[s + 1]) * fad;
+            const real Dv = (L.F1()[s] - L.F1()[s + 1]) * fad;
+            const real Du = (L.F2()[s] - L.F2()[s + 1]) * fad;
+            const real r = L.c0()[s];
+            real rn = r + Dr;
+            real vn = L.c1()[s], un = L.c2()[s];
+            if (dust && rn < fmax(r * real(1.0e-10), rfloor)) {
+                rn = fmax(r * real(1.0e-10), rfloor);
+            } else {
+                const real irn = FM<real>::rcp(rn);
+                vn = (r * L.c1()[s] + Dv) * irn;
+                un = (r * L.c2()[s] + Du) * irn;
+            }
+            const size_t o = (size_t)jj * nx + i;
+            out[o] = rn;
+            out[N + o] = un;
+            out[2 * N + o] = vn;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename real, int B>
+__device__ __forceinline__ void ring_copy_out(const DiskArgs<real>& d, const RingLds<real, B>& L, int i, real* __restrict__ out) {
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    for (int jj = threadIdx.x; jj < ny; jj += B) {
+        const size_t o = (size_t)jj * nx + i;
+        out[o] = L.ring[jj]; out[N + o] = L.ring[2 * ny + jj]; out[2 * N + o] = L.ring[ny + jj];
+    }
+}
+
+template <typename real, int B>
+__global__ __launch_bounds__(B) void k_ring_gas(DiskArgs<real> d) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int B = blockDim.x;
-    const int S = B + 2 * NPAD;
-    RingLds<real> L = ring_lds(sm, ny, S);
+    const int t = threadIdx.x;
     const int i = ring_radius(blockIdx.x, nx);
     if (i < 0) return;
+    RingLds<real, B> L{sm, sm + 3 * ny};
     const int ip = i + NPAD;
-    const real R = TAB(T_XC, ip);
     const real cs2 = TAB(T_CS2, ip);
-    const real gam = d.gam;
-    const real dt = (real)d.ctl->dt;
-    const real dts = dt / R;          // set_state receives dt/rad (riemann.cu:111)
-    const real idy = real(1) / d.dy;
-    const real fup = dt / d.dy / R;   // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
     const real cs0 = TAB(T_C0, ip);
+    const real ic0 = TAB(T_IC0, ip);
+    const real iR = TAB(T_IR, ip);
+    const real dt = (real)d.ctl->dt;
+    const real dts = dt * iR;          // set_state receives dt/rad (riemann.cu:111)
+    const real idy = d.idy;
+    const real fup = dt * idy * iR;    // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
     const int inc = d.orbadv ? d.inc[i] : 0;
     const real* __restrict__ in = d.gas;
-    const real wl0 = d.wLy[0], wl1 = d.wLy[1], wl2 = d.wLy[2];
-    const real wr0 = d.wRy[0], wr1 = d.wRy[1], wr2 = d.wRy[2];
+    const real w[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
 
     // ---- stage A: azimuthal sweep (rho, vp normal, u transverse) ----
     for (int c0 = 0; c0 < ny; c0 += B) {
         const int nt = min(B, ny - c0);
         const int ns = nt + 2 * NPAD;
-        for (int s = threadIdx.x; s < ns; s += B) {
-            int jj = wrap(c0 - NPAD + s, ny);
-            size_t o = (size_t)jj * nx + i;
-            L.c0[s] = in[o];            // rho
-            L.c1[s] = in[2 * N + o];    // vp: azimuthal velocity in the frame of rot + R*Omega_f
-            L.c2[s] = in[N + o];        // u: radial velocity (transverse here)
+        for (int s = t; s < ns; s += B) {
+            const int jj = wrap(c0 - NPAD + s, ny);
+            const size_t o = (size_t)jj * nx + i;
+            L.c0()[s] = in[o];            // rho
+            L.c1()[s] = in[2 * N + o];    // vp: azimuthal velocity in the frame of rot + R*Omega_f
+            L.c2()[s] = in[N + o];        // u: radial velocity (transverse here)
         }
         __syncthreads();
-        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
+        for (int s = t + 2; s < ns - 2; s += B) {
             // under p = c_s^2 rho with c_s^2 fixed on the ring, flatten's
             // pressure ratios equal the density ratios
-            L.fl[s] = flatten5(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2],
-                               L.c1[s - 1], L.c1[s], L.c1[s + 1], cs0);
+            const real* c0p = L.c0();
+            const real* c1p = L.c1();
+            L.fl()[s] = flatten5(c0p[s - 2], c0p[s - 1], c0p[s], c0p[s + 1], c0p[s + 2],
+                                 c1p[s - 1], c1p[s], c1p[s + 1], cs0);
         }
         __syncthreads();
-        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
-            real fl = L.fl[s], l, r;
-            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
-            L.sl0[s] = l; L.sr0[s] = r;
-            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
-            L.sl1[s] = l; L.sr1[s] = r;
-            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
-            L.sl2[s] = l; L.sr2[s] = r;
-        }
+        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, L.fl()[s], w);
         __syncthreads();
-        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
+        for (int k = t + NPAD; k < ns - 2; k += B) {
             real rl, ul, vl, rr, ur, vr;
             // p = c_s^2 rho with c_s^2 fixed on the ring: the pressure
             // parabola is c_s^2 times the density parabola
-            trace_left_c(cs0, dts, idy, L.c0[k - 1], L.c1[k - 1], L.sl0[k - 1], L.sr0[k - 1],
-                         L.sl1[k - 1], L.sr1[k - 1], L.sl2[k - 1], L.c2[k - 1], L.sr2[k - 1], rl, ul, vl);
-            trace_right_c(cs0, dts, idy, L.c0[k], L.c1[k], L.sl0[k], L.sr0[k],
-                          L.sl1[k], L.sr1[k], L.sl2[k], L.c2[k], L.sr2[k], rr, ur, vr);
+            trace_left_c(cs0, ic0, dts, idy, L.c0()[k - 1], L.c1()[k - 1], L.sl0()[k - 1], L.sr0()[k - 1],
+                         L.sl1()[k - 1], L.sr1()[k - 1], L.sl2()[k - 1], L.c2()[k - 1], L.sr2()[k - 1], rl, ul, vl);
+            trace_right_c(cs0, ic0, dts, idy, L.c0()[k], L.c1()[k], L.sl0()[k], L.sr0()[k],
+                          L.sl1()[k], L.sr1()[k], L.sl2()[k], L.c2()[k], L.sr2()[k], rr, ur, vr);
             real Fr, Fu, Fw, pres;
-            hllc_c(cs0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
-            L.F0[k] = Fr; L.F1[k] = Fu; L.F2[k] = Fw; L.F3[k] = pres;
+            hllc_c(cs0, ic0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
+            L.F0()[k] = Fr; L.F1()[k] = Fu; L.F2()[k] = Fw; L.F3()[k] = pres;
         }
         __syncthreads();
-        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
-            int jj = c0 - NPAD + s;
-            real Dr = (L.F0[s] - L.F0[s + 1]) * fup;
-            real Du = (L.F1[s] - L.F1[s + 1] + (L.F3[s] - L.F3[s + 1])) * fup;
-            real Dw = (L.F2[s] - L.F2[s + 1]) * fup;
-            real r = L.c0[s], vp = L.c1[s], u = L.c2[s];
+        for (int s = t + NPAD; s < ns - NPAD; s += B) {
+            const int jj = c0 - NPAD + s;
+            const real Dr = (L.F0()[s] - L.F0()[s + 1]) * fup;
+            const real Du = (L.F1()[s] - L.F1()[s + 1] + (L.F3()[s] - L.F3()[s + 1])) * fup;
+            const real Dw = (L.F2()[s] - L.F2()[s + 1]) * fup;
+            const real r = L.c0()[s], vp = L.c1()[s], u = L.c2()[s];
             real rn = r + Dr;
-            real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
+            const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
             real vn = vp, un = u;
             if (!d.sweep_on) {
                 rn = r;
@@ -685,229 +784,102 @@ __global__ __launch_bounds__(512) void k_ring_gas(DiskArgs<real> d) {
                 rn = rmin;
             } else {
                 // (rho v + Du + (rot + R Omega_f) Dr) / rho' - vbg == (rho vp + Du) / rho'
-                vn = (r * vp + Du) / rn;
-                un = (r * u + Dw) / rn;
+                const real irn = FM<real>::rcp(rn);
+                vn = (r * vp + Du) * irn;
+                un = (r * u + Dw) * irn;
             }
-            int js = wrap(jj + inc, ny);
+            const int js = wrap(jj + inc, ny);
             L.ring[js] = rn;
             L.ring[ny + js] = vn;
             L.ring[2 * ny + js] = un;
         }
         __syncthreads();
     }
-    real* __restrict__ out = d.gasT;
     if (!d.orbadv) {
-        for (int jj = threadIdx.x; jj < ny; jj += B) {
-            size_t o = (size_t)jj * nx + i;
-            out[o] = L.ring[jj]; out[N + o] = L.ring[2 * ny + jj]; out[2 * N + o] = L.ring[ny + jj];
-        }
+        ring_copy_out(d, L, i, d.gasT);
         return;
     }
     // ---- stage B: residual advection at speed res (advection.cu:1-105) ----
     const real speed = d.res[i];
     const real frac = fabs(speed) * dts * idy;   // |speed| dt / (R dy)
-    const real fad = dt / d.dy / R;              // dt/yv then 1/rad (advection.cu:154-158)
-    for (int c0 = 0; c0 < ny; c0 += B) {
-        const int nt = min(B, ny - c0);
-        const int ns = nt + 2 * NPAD;
-        for (int s = threadIdx.x; s < ns; s += B) {
-            int jj = wrap(c0 - NPAD + s, ny);
-            L.c0[s] = L.ring[jj];
-            L.c1[s] = L.ring[ny + jj];
-            L.c2[s] = L.ring[2 * ny + jj];
-        }
-        __syncthreads();
-        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
-            real l, r;
-            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl0[s] = l; L.sr0[s] = r;
-            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl1[s] = l; L.sr1[s] = r;
-            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl2[s] = l; L.sr2[s] = r;
-        }
-        __syncthreads();
-        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
-            real Dr, Dv, Du;
-            if (speed > real(0)) {
-                int c = k - 1;
-                real x = lim01(frac);
-                Dr = aveR(x, L.sl0[c], L.c0[c], L.sr0[c]);
-                Dv = aveR(x, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
-                Du = aveR(x, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
-            } else {
-                int c = k;
-                real q = lim01(frac);
-                Dr = aveL(q, L.sl0[c], L.c0[c], L.sr0[c]);
-                Dv = aveL(q, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
-                Du = aveL(q, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
-            }
-            L.F0[k] = Dr * speed; L.F1[k] = Dv * speed; L.F2[k] = Du * speed;
-        }
-        __syncthreads();
-        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
-            int jj = c0 - NPAD + s;
-            real Dr = (L.F0[s] - L.F0[s + 1]) * fad;
-            real Dv = (L.F1[s] - L.F1[s + 1]) * fad;
-            real Du = (L.F2[s] - L.F2[s + 1]) * fad;
-            real r = L.c0[s];
-            real rn = r + Dr;   // update_Cell_advect has no floor
-            size_t o = (size_t)jj * nx + i;
-            out[o] = rn;
-            out[N + o] = (r * L.c2[s] + Du) / rn;
-            out[2 * N + o] = (r * L.c1[s] + Dv) / rn;
-        }
-        __syncthreads();
-    }
+    ring_advect(d, L, i, d.gasT, speed, frac, fup, false, real(0), w);
 }
 
-template <typename real>
-__global__ __launch_bounds__(512) void k_ring_dust(DiskArgs<real> d) {
+template <typename real, int B>
+__global__ __launch_bounds__(B) void k_ring_dust(DiskArgs<real> d) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int B = blockDim.x;
-    const int S = B + 2 * NPAD;
-    RingLds<real> L = ring_lds(sm, ny, S);
+    const int t = threadIdx.x;
     const int i = ring_radius(blockIdx.x, nx);
     if (i < 0) return;
+    RingLds<real, B> L{sm, sm + 3 * ny};
     const int ip = i + NPAD;
-    const real R = TAB(T_XC, ip);
+    const real iR = TAB(T_IR, ip);
     const real dt = (real)d.ctl->dt;
-    const real dts = dt / R;          // dust_flow: dt /= rad for geom > 2 (dust.cu:127)
-    const real idy = real(1) / d.dy;
-    const real f1 = dt / d.dy, f2 = real(1) / R;  // dust.cu:309-313
+    const real dts = dt * iR;          // dust_flow: dt /= rad for geom > 2 (dust.cu:127)
+    const real idy = d.idy;
+    const real f12 = dt * idy * iR;    // dust.cu:309-313
     const int inc = d.orbadv ? d.inc[i] : 0;
     const real* __restrict__ in = d.dust;
     const real rfloor = d.D_G_ratio * real(1.0e-10);
-    const real wl0 = d.wLy[0], wl1 = d.wLy[1], wl2 = d.wLy[2];
-    const real wr0 = d.wRy[0], wr1 = d.wRy[1], wr2 = d.wRy[2];
+    const real w[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
 
     for (int c0 = 0; c0 < ny; c0 += B) {
         const int nt = min(B, ny - c0);
         const int ns = nt + 2 * NPAD;
-        for (int s = threadIdx.x; s < ns; s += B) {
-            int jj = wrap(c0 - NPAD + s, ny);
-            size_t o = (size_t)jj * nx + i;
-            L.c0[s] = in[o];
-            L.c1[s] = in[2 * N + o];
-            L.c2[s] = in[N + o];
+        for (int s = t; s < ns; s += B) {
+            const int jj = wrap(c0 - NPAD + s, ny);
+            const size_t o = (size_t)jj * nx + i;
+            L.c0()[s] = in[o];
+            L.c1()[s] = in[2 * N + o];
+            L.c2()[s] = in[N + o];
         }
         __syncthreads();
-        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
-            real l, r;
-            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl0[s] = l; L.sr0[s] = r;
-            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl1[s] = l; L.sr1[s] = r;
-            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl2[s] = l; L.sr2[s] = r;
-        }
+        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, real(0), w);
         __syncthreads();
-        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
+        for (int k = t + NPAD; k < ns - 2; k += B) {
             real Fr, Fu, Fw;
-            dust_iface(dts, idy, L.c0[k - 1], L.sl0[k - 1], L.sr0[k - 1], L.c1[k - 1], L.sl1[k - 1],
-                       L.sr1[k - 1], L.c2[k - 1], L.sl2[k - 1], L.sr2[k - 1],
-                       idy, L.c0[k], L.sl0[k], L.sr0[k], L.c1[k], L.sl1[k], L.sr1[k],
-                       L.c2[k], L.sl2[k], L.sr2[k], Fr, Fu, Fw);
-            L.F0[k] = Fr; L.F1[k] = Fu; L.F2[k] = Fw;
+            dust_iface(dts, idy, L.c0()[k - 1], L.sl0()[k - 1], L.sr0()[k - 1], L.c1()[k - 1], L.sl1()[k - 1],
+                       L.sr1()[k - 1], L.c2()[k - 1], L.sl2()[k - 1], L.sr2()[k - 1],
+                       idy, L.c0()[k], L.sl0()[k], L.sr0()[k], L.c1()[k], L.sl1()[k], L.sr1()[k],
+                       L.c2()[k], L.sl2()[k], L.sr2()[k], Fr, Fu, Fw);
+            L.F0()[k] = Fr; L.F1()[k] = Fu; L.F2()[k] = Fw;
         }
         __syncthreads();
-        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
-            int jj = c0 - NPAD + s;
-            real Dr = (L.F0[s] - L.F0[s + 1]) * f1 * f2;
-            real Du = (L.F1[s] - L.F1[s + 1]) * f1 * f2;
-            real Dw = (L.F2[s] - L.F2[s + 1]) * f1 * f2;
-            real r = L.c0[s], vp = L.c1[s], u = L.c2[s];
+        for (int s = t + NPAD; s < ns - NPAD; s += B) {
+            const int jj = c0 - NPAD + s;
+            const real Dr = (L.F0()[s] - L.F0()[s + 1]) * f12;
+            const real Du = (L.F1()[s] - L.F1()[s + 1]) * f12;
+            const real Dw = (L.F2()[s] - L.F2()[s + 1]) * f12;
+            const real r = L.c0()[s], vp = L.c1()[s], u = L.c2()[s];
             real rn = r + Dr;
-            real rmin = fmax(r * real(1.0e-10), rfloor);
+            const real rmin = fmax(r * real(1.0e-10), rfloor);
             real vn = vp, un = u;
             if (!d.sweep_on) {
                 rn = r;
             } else if (rn < rmin) {
                 rn = rmin;
             } else {
-                vn = (r * vp + Du) / rn;
-                un = (r * u + Dw) / rn;
+                const real irn = FM<real>::rcp(rn);
+                vn = (r * vp + Du) * irn;
+                un = (r * u + Dw) * irn;
             }
-            int js = wrap(jj + inc, ny);
+            const int js = wrap(jj + inc, ny);
             L.ring[js] = rn;
             L.ring[ny + js] = vn;
             L.ring[2 * ny + js] = un;
         }
         __syncthreads();
     }
-    real* __restrict__ out = d.dustT;
     if (!d.orbadv) {
-        for (int jj = threadIdx.x; jj < ny; jj += B) {
-            size_t o = (size_t)jj * nx + i;
-            out[o] = L.ring[jj]; out[N + o] = L.ring[2 * ny + jj]; out[2 * N + o] = L.ring[ny + jj];
-        }
+        ring_copy_out(d, L, i, d.dustT);
         return;
     }
     const real speed = d.res[i];
     const real frac = fabs(speed) * dts * idy;
-    for (int c0 = 0; c0 < ny; c0 += B) {
-        const int nt = min(B, ny - c0);
-        const int ns = nt + 2 * NPAD;
-        for (int s = threadIdx.x; s < ns; s += B) {
-            int jj = wrap(c0 - NPAD + s, ny);
-            L.c0[s] = L.ring[jj];
-            L.c1[s] = L.ring[ny + jj];
-            L.c2[s] = L.ring[2 * ny + jj];
-        }
-        __syncthreads();
-        for (int s = threadIdx.x + 2; s < ns - 2; s += B) {
-            real l, r;
-            slopes(L.c0[s - 2], L.c0[s - 1], L.c0[s], L.c0[s + 1], L.c0[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl0[s] = l; L.sr0[s] = r;
-            slopes(L.c1[s - 2], L.c1[s - 1], L.c1[s], L.c1[s + 1], L.c1[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl1[s] = l; L.sr1[s] = r;
-            slopes(L.c2[s - 2], L.c2[s - 1], L.c2[s], L.c2[s + 1], L.c2[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
-            L.sl2[s] = l; L.sr2[s] = r;
-        }
-        __syncthreads();
-        for (int k = threadIdx.x + NPAD; k < ns - 2; k += B) {
-            real Dr, Dv, Du;
-            if (speed > real(0)) {
-                int c = k - 1;
-                real x = lim01(frac);
-                Dr = aveR(x, L.sl0[c], L.c0[c], L.sr0[c]);
-                Dv = aveR(x, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
-                Du = aveR(x, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
-            } else {
-                int c = k;
-                real q = lim01(frac);
-                Dr = aveL(q, L.sl0[c], L.c0[c], L.sr0[c]);
-                Dv = aveL(q, L.sl1[c], L.c1[c], L.sr1[c]) * Dr;
-                Du = aveL(q, L.sl2[c], L.c2[c], L.sr2[c]) * Dr;
-            }
-            L.F0[k] = Dr * speed; L.F1[k] = Dv * speed; L.F2[k] = Du * speed;
-        }
-        __syncthreads();
-        for (int s = threadIdx.x + NPAD; s < ns - NPAD; s += B) {
-            int jj = c0 - NPAD + s;
-            real Dr = (L.F0[s] - L.F0[s + 1]) * f1 * f2;
-            real Dv = (L.F1[s] - L.F1[s + 1]) * f1 * f2;
-            real Du = (L.F2[s] - L.F2[s + 1]) * f1 * f2;
-            real r = L.c0[s];
-            real rn = r + Dr;
-            real rmin = fmax(r * real(1.0e-10), rfloor);
-            real vn = L.c1[s], un = L.c2[s];
-            if (rn < rmin) {
-                rn = rmin;
-            } else {
-                vn = (r * L.c1[s] + Dv) / rn;
-                un = (r * L.c2[s] + Du) / rn;
-            }
-            size_t o = (size_t)jj * nx + i;
-            out[o] = rn;
-            out[N + o] = un;
-            out[2 * N + o] = vn;
-        }
-        __syncthreads();
-    }
+    ring_advect(d, L, i, d.dustT, speed, frac, f12, true, rfloor, w);
 }
 
 // ----------------------------------------------------------------- viscosity
@@ -949,7 +921,7 @@ __global__ __launch_bounds__(256) void k_viscosity(DiskArgs<real> d, const real*
     // reciprocal differences of the centre grid (viscosity.cu:9-31)
     const real i20 = TAB(T_I20, ip), i21 = TAB(T_I21, ip), i10 = TAB(T_I10, ip);
     const real ir = TAB(T_IR, ip);
-    const real idy = real(1) / d.dy;
+    const real idy = d.idy;
     const real hdy = real(0.5) * idy;            // 1/(t2 - t0)
     // radial derivatives
     real dudx = (u[2] - u[0]) * i20;
@@ -981,17 +953,21 @@ __global__ __launch_bounds__(256) void k_viscosity(DiskArgs<real> d, const real*
 }
 
 // ------------------------------------------------------------- sources
-// Planet acceleration (star_planet_grav_cyl, forces.cu:41-58, z = 0).
+// Planet acceleration (star_planet_grav_cyl, forces.cu:41-58, z = 0) at
+// angle phi_j from the tabulated sin/cos of phi_j and phi_j/2 and the
+// planet's (sin, cos) of phi_p and phi_p/2.  The squared distance is written
+// without cancellation: (R - a)^2 + 4 a R sin^2((phi - phi_p)/2) + rs^2.
 template <typename real>
-__device__ __forceinline__ void planet_acc(const Body& b, real R, real phi, real& ax, real& ay) {
-    real dphi = phi - (real)b.y;
-    real s = sin(dphi), c = cos(dphi), h = sin(real(0.5) * dphi);
-    real px = (real)b.x, m = (real)b.m, rs = (real)b.rs;
-    // R^2 + px^2 - 2 px R cos(dphi) + rs^2, written without cancellation
-    real Rp2 = (R - px) * (R - px) + real(4) * px * R * h * h + rs * rs;
-    real Rp = sqrt(Rp2);
-    real k = real(1) / (Rp * Rp * Rp);
-    real ind = m / (px * px);
+__device__ __forceinline__ void planet_acc(const PlanetK& pk, const real* pt, real R, real& ax, real& ay) {
+    const real sp = (real)pk.sphi, cp = (real)pk.cphi, shp = (real)pk.shphi, chp = (real)pk.chphi;
+    const real s = pt[0] * cp - pt[1] * sp;      // sin(phi - phi_p)
+    const real c = pt[1] * cp + pt[0] * sp;      // cos(phi - phi_p)
+    const real h = pt[2] * chp - pt[3] * shp;    // sin((phi - phi_p)/2)
+    const real px = (real)pk.x, m = (real)pk.m, rs = (real)pk.rs;
+    const real Rp2 = (R - px) * (R - px) + real(4) * px * R * h * h + rs * rs;
+    const real iRp = FM<real>::rcp(FM<real>::sqrt(Rp2));
+    const real k = iRp * iRp * iRp;
+    const real ind = (real)pk.m_over_x2;
     ax = -m * (R - px * c) * k - c * ind;
     ay = -m * px * s * k + s * ind;
 }
@@ -1037,6 +1013,14 @@ __device__ __forceinline__ double reduce_partials(const double* part, int n) {
     return m;
 }
 
+// 1 - exp(-x) (the drag factor dtau, sweeps.cu:673,702)
+template <typename real>
+__device__ __forceinline__ real one_minus_exp_neg(real x);
+template <>
+__device__ __forceinline__ float one_minus_exp_neg(float x) { return 1.0f - __expf(-x); }
+template <>
+__device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x); }
+
 // Pointwise source terms.  mode: 0 first half kick (planet at t, in place),
 // 1 predictor (planet at t+dt, gas written to gasT), 2 corrector (planet at
 // t+dt, in place) optionally followed by wave killing and the CFL rate,
@@ -1051,37 +1035,38 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
     if (i < nx) {
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
-        const double hdt = 0.5 * d.ctl->dt;
-        const real dt = (real)hdt;
-        const real mdt = mode == 0 ? real(0) : (real)hdt;
-        const Body& b = mode == 0 ? d.ctl->planet0 : d.ctl->planet1;
+        const real dt = (real)(0.5 * d.ctl->dt);
+        const real mdt = mode == 0 ? real(0) : dt;
+        const PlanetK& pk = mode == 0 ? d.ctl->pk0 : d.ctl->pk1;
         const real R = TAB(T_XC, ip);
+        const real iR = TAB(T_IR, ip);
         const real vbg = TAB(T_VBG, ip);
         const real ebal = TAB(T_EBAL, ip);
-        const real phi = (real)(d.dy_d * ((double)j + 0.5));
         real px = real(0), py = real(0);
-        if (d.n_planet > 0) planet_acc(b, R, phi, px, py);
+        if (d.n_planet > 0) planet_acc(pk, d.phitab + 4 * j, R, px, py);
         const real* g = d.gas;
-        real rho = g[o], cu = g[N + o], cvp = g[2 * N + o];
+        const real rho = g[o], cu = g[N + o], cvp = g[2 * N + o];
+        const real irho = FM<real>::rcp(rho);
         real vx = real(0), vy = real(0);
-        if (d.visc) { vx = d.vis[o] / rho; vy = d.vis[N + o] / rho; }
+        if (d.visc) { vx = d.vis[o] * irho; vy = d.vis[N + o] * irho; }
         const real ay = vy + py;               // fy + gy, independent of the state
         real Tu = cu, Tvp = cvp;
-        real Qr = 0, Qu = 0, Qvp = 0, Du = 0, Dvp = 0, ts = 0;
+        real Qu = 0, Qvp = 0, Du = 0, Dvp = 0, ts = 0, its = 0;
         if (d.has_dust) {
-            Qr = d.dust[o]; Qu = d.dust[N + o]; Qvp = d.dust[2 * N + o];
+            Qu = d.dust[N + o]; Qvp = d.dust[2 * N + o];
             Du = Qu; Dvp = Qvp;
-            ts = d.Stokes / (sqrt(d.gam * TAB(T_CS2, ip)) / d.sc_h) / rho;   // dust.cu:1-4
+            ts = TAB(T_TS, ip) * irho;            // St / (c_s/h0) / rho (dust.cu:1-4)
+            its = rho * TAB(T_ITS, ip);
         }
         // fx + gx at azimuthal velocity vbg + vp: centrifugal - stellar
-        // gravity (tabulated at vbg) + deviation part + viscosity + planet
-        auto axd = [&](real vp) { return ebal + vp * (real(2) * vbg + vp) / R + px; };
+        // gravity (tabulated at vbg) + deviation part + planet (+ viscosity)
+        auto axd = [&](real vp) { return ebal + vp * (real(2) * vbg + vp) * iR + px; };
         auto axg = [&](real vp) { return axd(vp) + vx; };
-        if (mode != 0) {
+        if (mode == 1 || mode == 2) {
             Tu = cu + axg(cvp) * mdt;
             Tvp = cvp + ay * mdt;
             if (d.has_dust) {
-                real dtau = -expm1(-mdt / ts);
+                const real dtau = one_minus_exp_neg(mdt * its);
                 Du = Qu + (axd(Dvp) * ts + cu - Qu) * dtau;
                 Dvp = Qvp + ((cvp - Qvp) + py * ts) * dtau;
             }
@@ -1095,23 +1080,24 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
         } else {
             real ndu = Qu, ndvp = Qvp;
             if (d.has_dust && mode != 3) {
-                real dtau = -expm1(-dt / ts);
+                const real dtau = one_minus_exp_neg(dt * its);
                 ndu = Qu + (axd(Dvp) * ts + Tu - Qu) * dtau;
                 ndvp = Qvp + ((Tvp - Qvp) + py * ts) * dtau;
             }
             if (do_kill) {   // killwave on the corrected state (solve.cu:287-289)
-                real k = TAB(T_KILL, ip) * (real)d.ctl->dt;
-                nr_ += k * (TAB(T_G_R, ip) - nr_);
-                nu_ += k * (TAB(T_G_U, ip) - nu_);
-                nvp += k * (real(0) - nvp);
+                const real kk = TAB(T_KILL, ip) * (real)d.ctl->dt;
+                nr_ += kk * (TAB(T_G_R, ip) - nr_);
+                nu_ += kk * (TAB(T_G_U, ip) - nu_);
+                nvp += kk * (real(0) - nvp);
             }
-            d.gas[o] = nr_; d.gas[N + o] = nu_; d.gas[2 * N + o] = nvp;
+            if (do_kill) d.gas[o] = nr_;   // rho changes only by wave killing
+            d.gas[N + o] = nu_; d.gas[2 * N + o] = nvp;
             if (d.has_dust) { d.dust[N + o] = ndu; d.dust[2 * N + o] = ndvp; }
             if (do_rate) {
-                real cs = sqrt(d.gam * TAB(T_CS2, ip));
+                const real cs = TAB(T_C0, ip);
                 real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
                 if (d.has_dust) { sx = fmax(fabs(ndu), sx); sy = fmax(fabs(ndvp), sy); }
-                rate = fmax(sx * TAB(T_IDX, ip), sy / (d.dy * R));
+                rate = fmax(sx * TAB(T_IDX, ip), sy * d.idy * iR);
             }
         }
     }
@@ -1129,10 +1115,10 @@ __global__ __launch_bounds__(256) void k_cfl(DiskArgs<real> d) {
     if (i < nx) {
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
-        real cs = sqrt(d.gam * TAB(T_CS2, ip));
+        const real cs = TAB(T_C0, ip);
         real sx = fabs(d.gas[N + o]) + cs, sy = fabs(d.gas[2 * N + o]) + cs;
         if (d.has_dust) { sx = fmax(fabs(d.dust[N + o]), sx); sy = fmax(fabs(d.dust[2 * N + o]), sy); }
-        rate = fmax(sx * TAB(T_IDX, ip), sy / (d.dy * TAB(T_XC, ip)));
+        rate = fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip));
     }
     block_max_store(d.rate_part, rate);
 }
@@ -1142,6 +1128,20 @@ template <typename real>
 __global__ __launch_bounds__(1024) void k_rate_reduce(DiskArgs<real> d) {
     double m = reduce_partials(d.rate_part, d.rate_nblk);
     if (threadIdx.x == 0) d.ctl->rate_bits = (unsigned long long)__double_as_longlong(m);
+}
+
+// What the source kernel needs of a planet, precomputed once per step.
+__host__ __device__ inline PlanetK planet_kernel_view(const Body& b) {
+    PlanetK k;
+    k.m = b.m;
+    k.x = b.x;
+    k.rs = b.rs;
+    k.m_over_x2 = b.m / (b.x * b.x);
+    k.sphi = sin(b.y);
+    k.cphi = cos(b.y);
+    k.shphi = sin(0.5 * b.y);
+    k.chphi = cos(0.5 * b.y);
+    return k;
 }
 
 // Step prologue, one block.  mode 0: t and dt were set by the host.
@@ -1172,9 +1172,11 @@ __global__ __launch_bounds__(1024) void k_prologue(DiskArgs<real> d, int mode, d
         Body b0 = ctl->planet0;
         orbit_solution(b0, t, d.ramp_time, d.planet_mass, (double)d.frame_omega);
         ctl->planet0 = b0;
+        ctl->pk0 = planet_kernel_view(b0);
         Body b1 = ctl->planet1;
         orbit_solution(b1, t + dt, d.ramp_time, d.planet_mass, (double)d.frame_omega);
         ctl->planet1 = b1;
+        ctl->pk1 = planet_kernel_view(b1);
         s_dt = dt;
     }
     __syncthreads();
@@ -1193,41 +1195,60 @@ __global__ __launch_bounds__(1024) void k_prologue(DiskArgs<real> d, int mode, d
 }
 
 // ------------------------------------------------------------- launchers
+template <typename Kern>
+static hipError_t set_lds(Kern k, size_t lds) {
+    if (lds <= 65536) return hipSuccess;
+    return hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+template <typename real, int TX>
+static hipError_t launch_sweep_x(const DiskArgs<real>& d, bool dust, hipStream_t s) {
+    dim3 grid((d.nx + TX - 1) / TX, d.ny);
+    const size_t S = TX + 2 * NPAD;
+    if (dust) {
+        size_t lds = 13 * S * sizeof(real);
+        hipError_t e = set_lds(k_sweep_x_dust<real, TX>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), grid, dim3(TX), lds, s, d);
+    } else {
+        size_t lds = 18 * S * sizeof(real);
+        hipError_t e = set_lds(k_sweep_x_gas<real, TX>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), grid, dim3(TX), lds, s, d);
+    }
+    return hipGetLastError();
+}
+
+template <typename real>
+static hipError_t sweep_x_any(const DiskArgs<real>& d, bool dust, hipStream_t s) {
+    switch (d.tile_x) {
+        case 128: return launch_sweep_x<real, 128>(d, dust, s);
+        case 384: return launch_sweep_x<real, 384>(d, dust, s);
+        case 512: return launch_sweep_x<real, 512>(d, dust, s);
+        case 1024: return launch_sweep_x<real, 1024>(d, dust, s);
+        default: return launch_sweep_x<real, 256>(d, dust, s);
+    }
+}
+
 template <typename real>
 struct Launch {
-    static hipError_t sweep_x_gas(const DiskArgs<real>& d, hipStream_t s) {
-        dim3 grid((d.nx + d.tile_x - 1) / d.tile_x, d.ny);
-        size_t lds = (size_t)18 * (d.tile_x + 2 * NPAD) * sizeof(real);
-        hipLaunchKernelGGL(k_sweep_x_gas<real>, grid, dim3(256), lds, s, d);
-        return hipGetLastError();
-    }
-    static hipError_t sweep_x_dust(const DiskArgs<real>& d, hipStream_t s) {
-        dim3 grid((d.nx + d.tile_x - 1) / d.tile_x, d.ny);
-        size_t lds = (size_t)13 * (d.tile_x + 2 * NPAD) * sizeof(real);
-        hipLaunchKernelGGL(k_sweep_x_dust<real>, grid, dim3(256), lds, s, d);
-        return hipGetLastError();
-    }
+    static hipError_t sweep_x_gas(const DiskArgs<real>& d, hipStream_t s) { return sweep_x_any(d, false, s); }
+    static hipError_t sweep_x_dust(const DiskArgs<real>& d, hipStream_t s) { return sweep_x_any(d, true, s); }
     static size_t ring_lds(int ny, int B) { return ((size_t)3 * ny + (size_t)14 * (B + 2 * NPAD)) * sizeof(real); }
     static hipError_t ring_gas(const DiskArgs<real>& d, hipStream_t s) {
-        const int B = RING_THREADS;
+        constexpr int B = RING_THREADS;
         size_t lds = ring_lds(d.ny, B);
-        if (lds > 65536) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_ring_gas<real>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL(k_ring_gas<real>, dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
+        hipError_t e = set_lds(k_ring_gas<real, B>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_ring_gas<real, B>), dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
         return hipGetLastError();
     }
     static hipError_t ring_dust(const DiskArgs<real>& d, hipStream_t s) {
-        const int B = RING_THREADS;
+        constexpr int B = RING_THREADS;
         size_t lds = ring_lds(d.ny, B);
-        if (lds > 65536) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_ring_dust<real>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL(k_ring_dust<real>, dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
+        hipError_t e = set_lds(k_ring_dust<real, B>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_ring_dust<real, B>), dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
         return hipGetLastError();
     }
     static hipError_t viscosity(const DiskArgs<real>& d, const real* st, hipStream_t s) {

@@ -47,6 +47,9 @@ SIGNATURES = {
     "pg_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pg_profile_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int)]),
     "pg_device_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "pg_write_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
+    "pg_read_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "pg_snapshot_label": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_size_t]),
 }
 
 
@@ -193,6 +196,16 @@ class Disk:
         b = np.zeros(10)
         _check(load_library().pg_planet(self._h, t, _ptr(b)))
         return b
+
+    # ---- snapshots (output.cu:125-248) ----------------------------------
+    def write_snapshot(self, filename: str, t: float = None):
+        _check(load_library().pg_write_snapshot(self._h, filename.encode(), self.t if t is None else t))
+
+    def read_snapshot(self, filename: str) -> float:
+        t = ctypes.c_double()
+        _check(load_library().pg_read_snapshot(self._h, filename.encode(), ctypes.byref(t)))
+        self.t = t.value
+        return t.value
 
     # ---- introspection ---------------------------------------------------
     def profile(self, on: bool = True):
