@@ -114,6 +114,41 @@ int pg_profile_get(pg_handle* h, const char* kernel, double* avg_us, int* count)
 /* Bytes of device memory owned by the handle. */
 int pg_device_bytes(pg_handle* h, size_t* bytes);
 
+/* ---- batches of disks ------------------------------------------------------
+   No counterpart in the reference, which runs one Grid per process
+   (main.cu:157-359).  A batch advances n independent disks of uneven sizes
+   and parameters together: each kernel launch covers every disk through a
+   work list of (disk, block) pairs, every disk keeps its own CFL step and
+   time in its device control block (the loop of main.cu:320-359 per disk),
+   and one hipGraph replays a whole step of the batch.  Per disk the
+   arithmetic is that of pg_run, so each disk's state is bitwise what a
+   pg_handle holding it alone would produce.  All disks share precision,
+   dust_flag, visc_flag and kill_flag; each must pass pg_create's checks. */
+typedef struct pg_batch pg_batch;
+
+int pg_batch_create(int n, const pg_params* params, pg_batch** out);
+int pg_batch_destroy(pg_batch* b);
+int pg_batch_size(pg_batch* b, int* n);
+/* state of disk `disk` in pg_get_state's layout */
+int pg_batch_get_state(pg_batch* b, int disk, double* gas, double* dust);
+int pg_batch_set_state(pg_batch* b, int disk, const double* gas, const double* dust);
+/* nsteps of every disk; t_io, dt_io: n entries each (time and previous step
+   per disk, in and out); dts (n*nsteps, disk-major) may be NULL */
+int pg_batch_run(pg_batch* b, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph);
+/* launch schedule: x-sweep tile width (128, 256, 384, 512, 1024), threads
+   per azimuthal ring block (256, 512, 1024), ring order (0: longest rings
+   first, XCD-contiguous radii; 1: disk order, XCD-contiguous; 2: longest
+   first, plain radius order) */
+int pg_batch_set_schedule(pg_batch* b, int tile_x, int ring_b, int order);
+int pg_batch_get_schedule(pg_batch* b, int* tile_x, int* ring_b, int* order);
+/* time `steps` captured steps of each schedule candidate on the batch's
+   own state (restored afterwards) and keep the fastest; *best_ms (may be
+   NULL) receives its device time per step */
+int pg_batch_autotune(pg_batch* b, int steps, double* best_ms);
+int pg_batch_profile_enable(pg_batch* b, int on);
+int pg_batch_profile_get(pg_batch* b, const char* kernel, double* avg_us, int* count);
+int pg_batch_device_bytes(pg_batch* b, size_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

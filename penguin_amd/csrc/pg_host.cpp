@@ -190,19 +190,16 @@ struct KernelTimer {
     int count = 0;
 };
 
-}  // namespace
-
-struct pg_handle {
+// One disk: its device buffers and the host copies of the tables its state
+// conversion needs.  Fluid buffers come in pairs; the owning group's parity
+// says which of the two holds the current state.
+struct DiskDev {
     pg_params P{};
     int prec = 4;
     int nx = 0, ny = 0;
     size_t N = 0;
-    hipStream_t stream = nullptr;
-    // device memory
-    void* gas = nullptr;
-    void* gasT = nullptr;
-    void* dust = nullptr;
-    void* dustT = nullptr;
+    void* gas[2] = {nullptr, nullptr};
+    void* dust[2] = {nullptr, nullptr};
     void* vis = nullptr;
     void* tab = nullptr;
     void* phitab = nullptr;
@@ -213,31 +210,15 @@ struct pg_handle {
     int dts_cap = 0;
     double* rate_part = nullptr;
     int rate_nblk = 0;
-    int tile_x = 256;   // x-sweep tile width (PG_TILE_X overrides)
     StepCtl* ctl = nullptr;
     StepCtl* ctl_host = nullptr;  // pinned mirror
     size_t bytes = 0;
-    // host copies of what the state conversion needs
     std::vector<double> tabh;  // per-radius tables, field-major
     std::vector<double> phih;  // 4*ny azimuthal sine/cosine table
     int tab_stride = 0;
     double dy = 0.0;
     double wly[3]{}, wry[3]{};
     Body planet_init{};
-    bool rate_valid = false;
-    // profiling: event pairs recorded around launches without a host wait,
-    // resolved after the stream drains
-    bool profile = false;
-    std::map<std::string, KernelTimer> timers;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<int> ev_kernel;   // kernel id of pair k (events 2k, 2k+1)
-    size_t ev_used = 0;
-    // graph of one device-loop step, and the buffers it was captured with
-    hipGraphExec_t graph = nullptr;
-    void* graph_gas = nullptr;
-    void* graph_dust = nullptr;
-    double* graph_dts = nullptr;
 
     template <typename real>
     DiskArgs<real> args() const {
@@ -249,12 +230,10 @@ struct pg_handle {
         d.orbadv = P.OrbAdv_flag;
         d.kill = P.kill_flag;
         d.n_planet = P.n_planet;
-        d.tile_x = tile_x;
-        d.sweep_on = 1;
-        d.gas = (real*)gas;
-        d.gasT = (real*)gasT;
-        d.dust = (real*)dust;
-        d.dustT = (real*)dustT;
+        for (int k = 0; k < 2; ++k) {
+            d.gas[k] = (real*)gas[k];
+            d.dust[k] = (real*)dust[k];
+        }
         d.vis = (real*)vis;
         d.tab = (const real*)tab;
         d.tab_stride = tab_stride;
@@ -280,8 +259,73 @@ struct pg_handle {
         d.rs = planet_init.rs;
         d.rate_part = rate_part;
         d.rate_nblk = rate_nblk;
+        d.dts = dts;
+        d.dts_cap = dts_cap;
         return d;
     }
+};
+
+}  // namespace
+
+// A set of disks launched together: a pg_handle holds one, a pg_batch many.
+// Every kernel launch covers all disks of the set through a work list of
+// (disk, local block) pairs; the per-disk descriptors live in device memory,
+// so a captured step reads the current descriptors when it replays.
+struct Group {
+    int prec = 4;
+    std::vector<DiskDev*> disks;
+    hipStream_t stream = nullptr;
+    bool dust = false, visc = false, kill = false;
+    int gpar = 0, dpar = 0;      // which buffer of each fluid pair is current
+    void* d_args = nullptr;      // DiskArgs<real>[ndisk]
+    int2* d_wl = nullptr;        // work lists: x sweeps | rings | pointwise kernels
+    size_t wl_cap = 0;
+    int n_x = 0, n_ring = 0, n_pt = 0;
+    int tile_x = 256, ring_b = RING_THREADS, order = 0;
+    int ny_max = 0;
+    // profiling: event pairs recorded around launches without a host wait,
+    // resolved after the stream drains
+    bool profile = false;
+    std::map<std::string, KernelTimer> timers;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kernel;   // kernel id of pair k (events 2k, 2k+1)
+    size_t ev_used = 0;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // graph of one device-loop step and the parity it was captured at
+    hipGraphExec_t graph = nullptr;
+    int graph_par = -1;
+    size_t bytes = 0;
+
+    int par() const { return gpar | (dpar << 1); }
+    int ndisk() const { return (int)disks.size(); }
+    template <typename real>
+    LaunchSet<real> ls() const {
+        LaunchSet<real> L{};
+        L.disks = (const DiskArgs<real>*)d_args;
+        L.ndisk = ndisk();
+        L.wl_x = d_wl;
+        L.n_x = n_x;
+        L.tile_x = tile_x;
+        L.wl_ring = d_wl + n_x;
+        L.n_ring = n_ring;
+        L.ring_b = ring_b;
+        L.ny_max = ny_max;
+        L.wl_pt = d_wl + n_x + n_ring;
+        L.n_pt = n_pt;
+        return L;
+    }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_par = -1;
+    }
+};
+
+struct pg_handle {
+    Group g;
+};
+struct pg_batch {
+    Group g;
 };
 
 namespace {
@@ -314,10 +358,13 @@ int validate(const pg_params& P) {
 }
 
 template <typename T>
-int dev_alloc(pg_handle* h, T** p, size_t bytes) {
+int dev_alloc(size_t& counter, T** p, size_t bytes) {
     hipError_t e = hipMalloc((void**)p, bytes);
-    if (e != hipSuccess) return fail(PG_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    h->bytes += bytes;
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(PG_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    counter += bytes;
     return PG_OK;
 }
 
@@ -330,29 +377,28 @@ int upload_cast(const std::vector<double>& src, void* dst, hipStream_t s) {
     return PG_OK;
 }
 
-int upload(pg_handle* h, const std::vector<double>& src, void* dst) {
-    return h->prec == 8 ? upload_cast<double>(src, dst, h->stream) : upload_cast<float>(src, dst, h->stream);
+int upload(int prec, hipStream_t s, const std::vector<double>& src, void* dst) {
+    return prec == 8 ? upload_cast<double>(src, dst, s) : upload_cast<float>(src, dst, s);
 }
 
 template <typename real>
-int download_cast(pg_handle* h, const void* src, std::vector<double>& dst, size_t n) {
+int download_cast(hipStream_t s, const void* src, std::vector<double>& dst, size_t n) {
     std::vector<real> tmp(n);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * sizeof(real), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * sizeof(real), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     dst.resize(n);
     for (size_t k = 0; k < n; ++k) dst[k] = (double)tmp[k];
     return PG_OK;
 }
 
-int download(pg_handle* h, const void* src, std::vector<double>& dst, size_t n) {
-    return h->prec == 8 ? download_cast<double>(h, src, dst, n) : download_cast<float>(h, src, dst, n);
+int download(int prec, hipStream_t s, const void* src, std::vector<double>& dst, size_t n) {
+    return prec == 8 ? download_cast<double>(s, src, dst, n) : download_cast<float>(s, src, dst, n);
 }
 
-double tabv(const pg_handle* h, int f, int ip) { return h->tabh[(size_t)f * h->tab_stride + ip]; }
-
+double tabv(const DiskDev* h, int f, int ip) { return h->tabh[(size_t)f * h->tab_stride + ip]; }
 // Build the per-radius tables and the FP64 initial state (interior cells,
 // device field order: gas rho,u,vp; dust rho,u,vp).
-void build_tables_and_state(pg_handle* h, std::vector<double>& gas, std::vector<double>& dust,
+void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<double>& dust,
                             std::vector<double>& rot) {
     const pg_params& P = h->P;
     Profiles pr{P};
@@ -473,127 +519,603 @@ void build_tables_and_state(pg_handle* h, std::vector<double>& gas, std::vector<
     h->planet_init.rs = P.rs_fac * P.sc_h * P.planet_radius;
 }
 
+// ------------------------------------------------------------ disks
+void disk_free(DiskDev* d) {
+    if (!d) return;
+    void* bufs[] = {d->gas[0], d->gas[1], d->dust[0], d->dust[1], d->vis, d->tab, d->phitab,
+                    d->res, d->inc, d->rot, d->dts, d->ctl, d->rate_part};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (d->ctl_host) (void)hipHostFree(d->ctl_host);
+    delete d;
+}
+
+// Allocate a disk and upload its tables and initial state (init.cu:472-532,
+// planet.cu:319-403, orbit.cu:111-120).
+int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
+    *out = nullptr;
+    DiskDev* h = new DiskDev();
+    h->P = P;
+    h->prec = P.precision;
+    h->nx = P.xres;
+    h->ny = P.yres;
+    h->N = (size_t)h->nx * h->ny;
+    auto cleanup = [&](int code) {
+        disk_free(h);
+        return code;
+    };
+    const size_t es = h->prec;
+    const size_t N = h->N;
+    std::vector<double> gas, dust, rot;
+    build_tables_and_state(h, gas, dust, rot);
+    int rc;
+    size_t& B = h->bytes;
+    if ((rc = dev_alloc(B, &h->gas[0], 3 * N * es)) || (rc = dev_alloc(B, &h->gas[1], 3 * N * es)) ||
+        (rc = dev_alloc(B, &h->vis, 2 * N * es)) ||
+        (rc = dev_alloc(B, &h->tab, h->tabh.size() * es)) || (rc = dev_alloc(B, &h->phitab, h->phih.size() * es)) ||
+        (rc = dev_alloc(B, &h->res, h->nx * es)) ||
+        (rc = dev_alloc(B, &h->inc, h->nx * sizeof(int))) || (rc = dev_alloc(B, &h->rot, rot.size() * sizeof(double))) ||
+        (rc = dev_alloc(B, &h->ctl, sizeof(StepCtl))))
+        return cleanup(rc);
+    h->rate_nblk = ((h->nx + 255) / 256) * h->ny;
+    if ((rc = dev_alloc(B, &h->rate_part, sizeof(double) * h->rate_nblk))) return cleanup(rc);
+    if (P.dust_flag) {
+        if ((rc = dev_alloc(B, &h->dust[0], 3 * N * es)) || (rc = dev_alloc(B, &h->dust[1], 3 * N * es)))
+            return cleanup(rc);
+    }
+    if (hipHostMalloc((void**)&h->ctl_host, sizeof(StepCtl), hipHostMallocDefault) != hipSuccess) {
+        h->ctl_host = nullptr;
+        return cleanup(fail(PG_E_NOMEM, "hipHostMalloc failed"));
+    }
+    std::memset(h->ctl_host, 0, sizeof(StepCtl));
+    if ((rc = upload(h->prec, s, h->tabh, h->tab)) || (rc = upload(h->prec, s, h->phih, h->phitab)) ||
+        (rc = upload(h->prec, s, gas, h->gas[0])) || (rc = upload(h->prec, s, gas, h->gas[1])))
+        return cleanup(rc);
+    if (P.dust_flag &&
+        ((rc = upload(h->prec, s, dust, h->dust[0])) || (rc = upload(h->prec, s, dust, h->dust[1]))))
+        return cleanup(rc);
+    std::vector<double> zero(2 * N, 0.0);
+    if ((rc = upload(h->prec, s, zero, h->vis))) return cleanup(rc);
+    if (hipMemcpy(h->rot, rot.data(), rot.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(PG_E_HIP, "rot upload failed"));
+    if (hipMemset(h->inc, 0, h->nx * sizeof(int)) != hipSuccess) return cleanup(fail(PG_E_HIP, "memset failed"));
+    StepCtl& c = *h->ctl_host;
+    c.planet0 = h->planet_init;
+    c.planet1 = h->planet_init;
+    c.end_time = P.end_time;
+    if (hipMemcpyAsync(h->ctl, h->ctl_host, sizeof(StepCtl), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return cleanup(fail(PG_E_HIP, "control block upload failed"));
+    *out = h;
+    return PG_OK;
+}
+
+// ------------------------------------------------------------ groups
+// Work lists of one launch schedule.
+//   x sweeps / pointwise kernels: every (tile or 256-radius segment, row) of
+//   every disk, disk by disk.
+//   rings: one entry per radius.  Workgroups are dispatched round-robin over
+//   the 8 XCDs, each with its own L2; neighbouring radii share cache lines of
+//   the radius-fastest layout, so with `order` 0 or 1 each disk's radii are
+//   cut into 8 contiguous runs and XCD x receives run x (entry 8k+x of the
+//   list).  Order 0 takes the disks by decreasing ring length (longest
+//   processing time first), order 1 in their own order, order 2 longest
+//   first without the XCD split.  Entries with radius -1 pad the list.
+void build_worklists(const Group& g, int tile_x, int order, std::vector<int2>& wx, std::vector<int2>& wr,
+                     std::vector<int2>& wp) {
+    wx.clear();
+    wr.clear();
+    wp.clear();
+    const int nd = g.ndisk();
+    for (int k = 0; k < nd; ++k) {
+        const DiskDev* d = g.disks[k];
+        const int tiles = (d->nx + tile_x - 1) / tile_x;
+        for (int b = 0; b < tiles * d->ny; ++b) wx.push_back(make_int2(k, b));
+        const int segs = (d->nx + 255) / 256;
+        for (int b = 0; b < segs * d->ny; ++b) wp.push_back(make_int2(k, b));
+    }
+    std::vector<int> ord(nd);
+    for (int k = 0; k < nd; ++k) ord[k] = k;
+    if (order != 1)
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return g.disks[a]->ny > g.disks[b]->ny; });
+    if (order == 2) {
+        for (int k : ord)
+            for (int i = 0; i < g.disks[k]->nx; ++i) wr.push_back(make_int2(k, i));
+        return;
+    }
+    std::vector<int2> q[8];
+    for (int k : ord) {
+        const int nx = g.disks[k]->nx;
+        const int per = (nx + 7) / 8;
+        for (int x = 0; x < 8; ++x)
+            for (int i = x * per; i < std::min((x + 1) * per, nx); ++i) q[x].push_back(make_int2(k, i));
+    }
+    size_t len = 0;
+    for (int x = 0; x < 8; ++x) len = std::max(len, q[x].size());
+    wr.assign(8 * len, make_int2(0, -1));
+    for (int x = 0; x < 8; ++x)
+        for (size_t k = 0; k < q[x].size(); ++k) wr[8 * k + x] = q[x][k];
+}
+
+bool tile_ok(int t) { return t == 128 || t == 256 || t == 384 || t == 512 || t == 1024; }
+bool ring_ok(int b) { return b == 256 || b == 512 || b == 1024; }
+
+// Install a launch schedule: rebuild and upload the work lists.
+int group_schedule(Group& g, int tile_x, int ring_b, int order) {
+    if (!tile_ok(tile_x)) return fail(PG_E_INVALID, "tile_x=%d: use 128, 256, 384, 512 or 1024", tile_x);
+    if (!ring_ok(ring_b)) return fail(PG_E_INVALID, "ring_b=%d: use 256, 512 or 1024", ring_b);
+    if (order < 0 || order > 2) return fail(PG_E_INVALID, "order=%d: use 0, 1 or 2", order);
+    const size_t lds = (3 * (size_t)g.ny_max + 14 * (size_t)(ring_b + 2 * NPAD)) * g.prec;
+    if (lds > 160 * 1024)
+        return fail(PG_E_INVALID, "ring_b=%d with yres %d needs %zu B of LDS (max 163840)", ring_b, g.ny_max, lds);
+    std::vector<int2> wx, wr, wp;
+    build_worklists(g, tile_x, order, wx, wr, wp);
+    const size_t need = wx.size() + wr.size() + wp.size();
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (need > g.wl_cap) {
+        if (g.d_wl) (void)hipFree(g.d_wl);
+        g.d_wl = nullptr;
+        g.wl_cap = 0;
+        int rc = dev_alloc(g.bytes, &g.d_wl, need * sizeof(int2));
+        if (rc) return rc;
+        g.wl_cap = need;
+    }
+    std::vector<int2> all;
+    all.reserve(need);
+    all.insert(all.end(), wx.begin(), wx.end());
+    all.insert(all.end(), wr.begin(), wr.end());
+    all.insert(all.end(), wp.begin(), wp.end());
+    HIP_TRY(hipMemcpy(g.d_wl, all.data(), need * sizeof(int2), hipMemcpyHostToDevice));
+    g.n_x = (int)wx.size();
+    g.n_ring = (int)wr.size();
+    g.n_pt = (int)wp.size();
+    g.tile_x = tile_x;
+    g.ring_b = ring_b;
+    g.order = order;
+    g.drop_graph();
+    return PG_OK;
+}
+
+// Upload the per-disk descriptors (after creation or a reallocation).
+int group_upload_args(Group& g) {
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (g.prec == 8) {
+        std::vector<DiskArgs<double>> a;
+        for (DiskDev* d : g.disks) a.push_back(d->args<double>());
+        HIP_TRY(hipMemcpy(g.d_args, a.data(), a.size() * sizeof(a[0]), hipMemcpyHostToDevice));
+    } else {
+        std::vector<DiskArgs<float>> a;
+        for (DiskDev* d : g.disks) a.push_back(d->args<float>());
+        HIP_TRY(hipMemcpy(g.d_args, a.data(), a.size() * sizeof(a[0]), hipMemcpyHostToDevice));
+    }
+    return PG_OK;
+}
+
+void group_release(Group& g) {
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    g.drop_graph();
+    for (DiskDev* d : g.disks) disk_free(d);
+    g.disks.clear();
+    if (g.d_args) (void)hipFree(g.d_args);
+    if (g.d_wl) (void)hipFree(g.d_wl);
+    g.d_args = nullptr;
+    g.d_wl = nullptr;
+    for (hipEvent_t e : g.ev_pool) (void)hipEventDestroy(e);
+    g.ev_pool.clear();
+    if (g.ev_a) (void)hipEventDestroy(g.ev_a);
+    if (g.ev_b) (void)hipEventDestroy(g.ev_b);
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    g.stream = nullptr;
+}
+
+int env_int(const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+// Create the disks of a group.  All disks of a group share the storage
+// precision and the dust / viscosity / wave-killing switches (each kernel
+// launch then runs the same passes for every disk).
+int group_init(Group& g, int n, const pg_params* params) {
+    for (int k = 0; k < n; ++k) {
+        int rc = validate(params[k]);
+        if (rc) return n > 1 ? fail(rc, "disk %d: %s", k, g_err.c_str()) : rc;
+        const pg_params& a = params[0];
+        const pg_params& b = params[k];
+        if (b.precision != a.precision || b.dust_flag != a.dust_flag || b.visc_flag != a.visc_flag ||
+            b.kill_flag != a.kill_flag)
+            return fail(PG_E_INVALID,
+                        "disk %d: precision, dust_flag, visc_flag and kill_flag must agree across a batch", k);
+    }
+    g.prec = params[0].precision;
+    g.dust = params[0].dust_flag;
+    g.visc = params[0].visc_flag;
+    g.kill = params[0].kill_flag;
+    if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
+        g.stream = nullptr;
+        return fail(PG_E_HIP, "hipStreamCreate failed");
+    }
+    if (hipEventCreate(&g.ev_a) != hipSuccess || hipEventCreate(&g.ev_b) != hipSuccess)
+        return fail(PG_E_HIP, "hipEventCreate failed");
+    int rc;
+    for (int k = 0; k < n; ++k) {
+        DiskDev* d = nullptr;
+        if ((rc = disk_create(params[k], g.stream, &d))) return n > 1 ? fail(rc, "disk %d: %s", k, g_err.c_str()) : rc;
+        g.disks.push_back(d);
+        g.bytes += d->bytes;
+        g.ny_max = std::max(g.ny_max, d->ny);
+    }
+    const size_t asz = g.prec == 8 ? sizeof(DiskArgs<double>) : sizeof(DiskArgs<float>);
+    if ((rc = dev_alloc(g.bytes, &g.d_args, asz * n))) return rc;
+    if ((rc = group_upload_args(g))) return rc;
+    // default schedule: widest x tile that divides every disk's radii
+    int tile = 512;
+    for (DiskDev* d : g.disks)
+        if (d->nx % tile) tile = 256;
+    if (n == 1) {
+        const int nx = g.disks[0]->nx;
+        tile = (nx % 512 == 0) ? 512 : (nx == 384 ? 384 : (nx == 128 ? 128 : 256));
+    }
+    int ringb = RING_THREADS;
+    tile = env_int("PG_TILE_X", tile);
+    ringb = env_int("PG_RING_B", ringb);
+    if (!tile_ok(tile)) tile = 256;
+    if (!ring_ok(ringb)) ringb = RING_THREADS;
+    return group_schedule(g, tile, ringb, 0);
+}
+
 // ------------------------------------------------------------ launching
 const char* kname[] = {"prologue", "viscosity", "sources_kick", "sweep_x_gas", "ring_gas", "sweep_x_dust",
                        "ring_dust", "sources_predictor", "sources_corrector", "cfl"};
 enum KId { K_PRO, K_VIS, K_KICK, K_SXG, K_RG, K_SXD, K_RD, K_PRED, K_CORR, K_CFL };
 
-int timed_begin(pg_handle* h) {
-    if (!h->profile) return PG_OK;
-    size_t need = 2 * (h->ev_used + 1);
-    while (h->ev_pool.size() < need) {
+int timed_begin(Group& g) {
+    if (!g.profile) return PG_OK;
+    size_t need = 2 * (g.ev_used + 1);
+    while (g.ev_pool.size() < need) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
-        h->ev_pool.push_back(e);
+        g.ev_pool.push_back(e);
     }
-    HIP_TRY(hipEventRecord(h->ev_pool[2 * h->ev_used], h->stream));
+    HIP_TRY(hipEventRecord(g.ev_pool[2 * g.ev_used], g.stream));
     return PG_OK;
 }
-int timed_end(pg_handle* h, int k) {
-    if (!h->profile) return PG_OK;
-    HIP_TRY(hipEventRecord(h->ev_pool[2 * h->ev_used + 1], h->stream));
-    if (h->ev_kernel.size() <= h->ev_used) h->ev_kernel.resize(h->ev_used + 1);
-    h->ev_kernel[h->ev_used] = k;
-    h->ev_used += 1;
+int timed_end(Group& g, int k) {
+    if (!g.profile) return PG_OK;
+    HIP_TRY(hipEventRecord(g.ev_pool[2 * g.ev_used + 1], g.stream));
+    if (g.ev_kernel.size() <= g.ev_used) g.ev_kernel.resize(g.ev_used + 1);
+    g.ev_kernel[g.ev_used] = k;
+    g.ev_used += 1;
     return PG_OK;
 }
 // After the stream has drained: fold the recorded pairs into the timers.
-int timers_resolve(pg_handle* h) {
-    if (!h->profile) return PG_OK;
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    for (size_t k = 0; k < h->ev_used; ++k) {
+int timers_resolve(Group& g) {
+    if (!g.profile) return PG_OK;
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    for (size_t k = 0; k < g.ev_used; ++k) {
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, h->ev_pool[2 * k], h->ev_pool[2 * k + 1]));
-        KernelTimer& t = h->timers[kname[h->ev_kernel[k]]];
+        HIP_TRY(hipEventElapsedTime(&ms, g.ev_pool[2 * k], g.ev_pool[2 * k + 1]));
+        KernelTimer& t = g.timers[kname[g.ev_kernel[k]]];
         t.total_ms += ms;
         t.count += 1;
     }
-    h->ev_used = 0;
+    g.ev_used = 0;
     return PG_OK;
 }
 
-#define LAUNCH(h, k, expr)                                                                       \
+#define LAUNCH(g, k, expr)                                                                       \
     do {                                                                                         \
-        int rc_ = timed_begin(h);                                                                \
+        int rc_ = timed_begin(g);                                                                \
         if (rc_) return rc_;                                                                     \
         hipError_t e_ = (expr);                                                                  \
         if (e_ != hipSuccess) return fail(PG_E_HIP, "launch %s failed: %s", kname[k], hipGetErrorString(e_)); \
-        rc_ = timed_end(h, k);                                                                   \
+        rc_ = timed_end(g, k);                                                                   \
         if (rc_) return rc_;                                                                     \
     } while (0)
 
-// One step (DS order, solve.cu:212-292) after the prologue; dt, t and the
-// planets are in the device control block.
+// One step (DS order, solve.cu:212-292) of every disk of the group after the
+// prologue; each disk's dt, t and planets are in its device control block.
+// A full step flips each fluid's buffer pair twice, so the parity it ends
+// with is the one it started with.
 template <typename real>
-int step_passes(pg_handle* h) {
-    const pg_params& P = h->P;
-    hipStream_t s = h->stream;
-    if (P.visc_flag) {
-        DiskArgs<real> d = h->args<real>();
-        LAUNCH(h, K_VIS, Launch<real>::viscosity(d, d.gas, s));
+int step_passes(Group& g) {
+    const LaunchSet<real> L = g.ls<real>();
+    hipStream_t s = g.stream;
+    if (g.visc) LAUNCH(g, K_VIS, Launch<real>::viscosity(L, g.par(), 0, s));
+    LAUNCH(g, K_KICK, Launch<real>::sources(L, g.par(), 0, 0, 0, s));
+    LAUNCH(g, K_SXG, Launch<real>::sweep_x_gas(L, g.par(), s));
+    g.gpar ^= 1;
+    LAUNCH(g, K_RG, Launch<real>::ring_gas(L, g.par(), 3, s));
+    g.gpar ^= 1;
+    if (g.dust) {
+        LAUNCH(g, K_SXD, Launch<real>::sweep_x_dust(L, g.par(), s));
+        g.dpar ^= 1;
+        LAUNCH(g, K_RD, Launch<real>::ring_dust(L, g.par(), 3, s));
+        g.dpar ^= 1;
     }
-    {
-        DiskArgs<real> d = h->args<real>();
-        LAUNCH(h, K_KICK, Launch<real>::sources(d, 0, 0, 0, s));
+    if (g.visc) {
+        LAUNCH(g, K_PRED, Launch<real>::sources(L, g.par(), 1, 0, 0, s));
+        LAUNCH(g, K_VIS, Launch<real>::viscosity(L, g.par(), 1, s));
     }
-    {
-        DiskArgs<real> d = h->args<real>();
-        LAUNCH(h, K_SXG, Launch<real>::sweep_x_gas(d, s));
-        std::swap(h->gas, h->gasT);
-    }
-    {
-        DiskArgs<real> d = h->args<real>();
-        LAUNCH(h, K_RG, Launch<real>::ring_gas(d, s));
-        std::swap(h->gas, h->gasT);
-    }
-    if (P.dust_flag) {
-        DiskArgs<real> d = h->args<real>();
-        LAUNCH(h, K_SXD, Launch<real>::sweep_x_dust(d, s));
-        std::swap(h->dust, h->dustT);
-        d = h->args<real>();
-        LAUNCH(h, K_RD, Launch<real>::ring_dust(d, s));
-        std::swap(h->dust, h->dustT);
-    }
-    if (P.visc_flag) {
-        DiskArgs<real> d = h->args<real>();
-        LAUNCH(h, K_PRED, Launch<real>::sources(d, 1, 0, 0, s));
-        LAUNCH(h, K_VIS, Launch<real>::viscosity(d, d.gasT, s));
-    }
-    {
-        DiskArgs<real> d = h->args<real>();
-        LAUNCH(h, K_CORR, Launch<real>::sources(d, 2, P.kill_flag, 1, s));
-    }
-    h->rate_valid = true;
+    LAUNCH(g, K_CORR, Launch<real>::sources(L, g.par(), 2, g.kill ? 1 : 0, 1, s));
     return PG_OK;
 }
 
-int passes(pg_handle* h) { return h->prec == 8 ? step_passes<double>(h) : step_passes<float>(h); }
+int passes(Group& g) { return g.prec == 8 ? step_passes<double>(g) : step_passes<float>(g); }
 
-int prologue(pg_handle* h, int mode, double* dts) {
-    hipError_t e = h->prec == 8 ? launch_prologue<double>(h->args<double>(), mode, dts, h->stream)
-                                : launch_prologue<float>(h->args<float>(), mode, dts, h->stream);
+int prologue(Group& g, int mode, int do_orbadv) {
+    hipError_t e = g.prec == 8 ? Launch<double>::prologue(g.ls<double>(), mode, do_orbadv, g.stream)
+                               : Launch<float>::prologue(g.ls<float>(), mode, do_orbadv, g.stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
     return PG_OK;
 }
 
-int push_ctl(pg_handle* h) {
-    HIP_TRY(hipMemcpyAsync(h->ctl, h->ctl_host, sizeof(StepCtl), hipMemcpyHostToDevice, h->stream));
+int push_ctl(Group& g) {
+    for (DiskDev* d : g.disks)
+        HIP_TRY(hipMemcpyAsync(d->ctl, d->ctl_host, sizeof(StepCtl), hipMemcpyHostToDevice, g.stream));
     return PG_OK;
 }
-int pull_ctl(pg_handle* h) {
-    HIP_TRY(hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+int pull_ctl(Group& g) {
+    for (DiskDev* d : g.disks)
+        HIP_TRY(hipMemcpyAsync(d->ctl_host, d->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 
-int cfl_rate(pg_handle* h) {
-    hipError_t e = h->prec == 8 ? Launch<double>::cfl(h->args<double>(), h->stream)
-                                : Launch<float>::cfl(h->args<float>(), h->stream);
+int launch_cfl(Group& g) {
+    hipError_t e = g.prec == 8 ? Launch<double>::cfl(g.ls<double>(), g.par(), g.stream)
+                               : Launch<float>::cfl(g.ls<float>(), g.par(), g.stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch cfl failed: %s", hipGetErrorString(e));
-    e = h->prec == 8 ? Launch<double>::rate_reduce(h->args<double>(), h->stream)
-                     : Launch<float>::rate_reduce(h->args<float>(), h->stream);
+    return PG_OK;
+}
+
+int cfl_rate(Group& g) {
+    int rc = launch_cfl(g);
+    if (rc) return rc;
+    hipError_t e = g.prec == 8 ? Launch<double>::rate_reduce(g.ls<double>(), g.stream)
+                               : Launch<float>::rate_reduce(g.ls<float>(), g.stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch rate_reduce failed: %s", hipGetErrorString(e));
-    h->rate_valid = true;
+    return PG_OK;
+}
+
+// nsteps of the reference loop (main.cu:320-359) for every disk of the
+// group, each with its own dt.  t_io/dt_io hold one entry per disk; dts
+// (ndisk*nsteps, disk-major, may be null) receives the step sizes;
+// loop_ms (may be null) the device time of the step loop.
+int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph, float* loop_ms) {
+    if (nsteps < 0 || !t_io || !dt_io) return fail(PG_E_INVALID, "bad arguments");
+    if (nsteps == 0) return PG_OK;
+    int rc;
+    bool realloc = false;
+    for (DiskDev* d : g.disks) {
+        if (d->dts_cap >= nsteps) continue;
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (d->dts) (void)hipFree(d->dts);
+        d->dts = nullptr;
+        d->dts_cap = 0;
+        if ((rc = dev_alloc(g.bytes, &d->dts, sizeof(double) * nsteps))) return rc;
+        d->dts_cap = nsteps;
+        realloc = true;
+    }
+    if (realloc && (rc = group_upload_args(g))) return rc;
+    if ((rc = pull_ctl(g))) return rc;
+    for (int k = 0; k < g.ndisk(); ++k) {
+        StepCtl& c = *g.disks[k]->ctl_host;
+        c.t = t_io[k];
+        c.dt = dt_io[k];
+        c.dt_prev = dt_io[k];
+        c.nstep = 0;
+        c.mode = 1;
+        c.end_time = g.disks[k]->P.end_time;
+        c.rate_bits = 0ull;
+    }
+    if ((rc = push_ctl(g))) return rc;
+    // rate of the current state for the first step (later steps get it from
+    // the corrector)
+    if ((rc = launch_cfl(g))) return rc;
+    if (use_graph && !g.profile) {
+        if (g.graph && g.graph_par != g.par()) g.drop_graph();
+        if (!g.graph) {
+            hipGraph_t gr;
+            const int par0 = g.par();
+            HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            int rc1 = prologue(g, 1, 1);
+            int rc2 = rc1 ? rc1 : passes(g);
+            hipError_t ec = hipStreamEndCapture(g.stream, &gr);
+            if (rc2) return rc2;
+            if (ec != hipSuccess) return fail(PG_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+            ec = hipGraphInstantiate(&g.graph, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ec != hipSuccess) {
+                g.graph = nullptr;
+                return fail(PG_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ec));
+            }
+            g.graph_par = par0;
+        }
+        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
+        for (int n = 0; n < nsteps; ++n) HIP_TRY(hipGraphLaunch(g.graph, g.stream));
+        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
+    } else {
+        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
+        for (int n = 0; n < nsteps; ++n) {
+            if ((rc = timed_begin(g))) return rc;
+            if ((rc = prologue(g, 1, 1))) return rc;
+            if ((rc = timed_end(g, K_PRO))) return rc;
+            if ((rc = passes(g))) return rc;
+        }
+        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
+    }
+    if ((rc = pull_ctl(g))) return rc;
+    if (loop_ms) HIP_TRY(hipEventElapsedTime(loop_ms, g.ev_a, g.ev_b));
+    if ((rc = timers_resolve(g))) return rc;
+    for (int k = 0; k < g.ndisk(); ++k) {
+        const DiskDev* d = g.disks[k];
+        if (dts) HIP_TRY(hipMemcpy(dts + (size_t)k * nsteps, d->dts, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+        t_io[k] = d->ctl_host->t + d->ctl_host->dt;
+        dt_io[k] = d->ctl_host->dt;
+    }
+    return PG_OK;
+}
+
+// ------------------------------------------------------------ state I/O
+int disk_get_state(Group& g, int k, double* gas, double* dust) {
+    DiskDev* h = g.disks[k];
+    const size_t N = h->N;
+    const int nx = h->nx;
+    std::vector<double> gv, dv;
+    int rc;
+    if (gas) {
+        if ((rc = download(h->prec, g.stream, h->gas[g.gpar], gv, 3 * N))) return rc;
+        for (size_t o = 0; o < N; ++o) {
+            int ip = (int)(o % nx) + NPAD;
+            double rho = gv[o];
+            gas[o] = rho;
+            gas[N + o] = tabv(h, T_CS2, ip) * rho;
+            gas[2 * N + o] = gv[N + o];
+            gas[3 * N + o] = tabv(h, T_VBG, ip) + gv[2 * N + o];
+            gas[4 * N + o] = 0.0;
+        }
+    }
+    if (dust) {
+        if (!h->P.dust_flag) return fail(PG_E_STATE, "the disk has no dust");
+        if ((rc = download(h->prec, g.stream, h->dust[g.dpar], dv, 3 * N))) return rc;
+        for (size_t o = 0; o < N; ++o) {
+            int ip = (int)(o % nx) + NPAD;
+            dust[o] = dv[o];
+            dust[N + o] = dv[N + o];
+            dust[2 * N + o] = tabv(h, T_VBG, ip) + dv[2 * N + o];
+            dust[3 * N + o] = 0.0;
+        }
+    }
+    return PG_OK;
+}
+
+int disk_set_state(Group& g, int k, const double* gas, const double* dust) {
+    DiskDev* h = g.disks[k];
+    const size_t N = h->N;
+    const int nx = h->nx;
+    int rc;
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (dust && !h->P.dust_flag) return fail(PG_E_STATE, "the disk has no dust");
+    if (gas) {
+        std::vector<double> v(3 * N);
+        for (size_t o = 0; o < N; ++o) {
+            int ip = (int)(o % nx) + NPAD;
+            v[o] = gas[o];
+            v[N + o] = gas[2 * N + o];
+            v[2 * N + o] = gas[3 * N + o] - tabv(h, T_VBG, ip);
+        }
+        if ((rc = upload(h->prec, g.stream, v, h->gas[g.gpar]))) return rc;
+    }
+    if (dust) {
+        std::vector<double> v(3 * N);
+        for (size_t o = 0; o < N; ++o) {
+            int ip = (int)(o % nx) + NPAD;
+            v[o] = dust[o];
+            v[N + o] = dust[N + o];
+            v[2 * N + o] = dust[2 * N + o] - tabv(h, T_VBG, ip);
+        }
+        if ((rc = upload(h->prec, g.stream, v, h->dust[g.dpar]))) return rc;
+    }
+    return PG_OK;
+}
+
+int group_profile_get(Group& g, const char* kernel, double* avg_us, int* count) {
+    if (!kernel || !avg_us || !count) return fail(PG_E_INVALID, "null argument");
+    auto it = g.timers.find(kernel);
+    if (it == g.timers.end()) {
+        *avg_us = 0.0;
+        *count = 0;
+        return PG_OK;
+    }
+    *count = it->second.count;
+    *avg_us = it->second.count ? 1000.0 * it->second.total_ms / it->second.count : 0.0;
+    return PG_OK;
+}
+
+// Time `steps` captured steps of every schedule candidate on the batch's own
+// state and keep the fastest; the state, times and control blocks are
+// restored after each trial.
+int group_autotune(Group& g, int steps, double* best_ms) {
+    if (steps < 1) return fail(PG_E_INVALID, "steps must be >= 1");
+    int rc;
+    const int nd = g.ndisk();
+    // save the current state and control blocks
+    std::vector<void*> sg(nd, nullptr), sd(nd, nullptr);
+    auto free_saved = [&]() {
+        for (void* p : sg)
+            if (p) (void)hipFree(p);
+        for (void* p : sd)
+            if (p) (void)hipFree(p);
+    };
+    size_t scratch = 0;
+    std::vector<StepCtl> saved_ctl(nd);
+    if ((rc = pull_ctl(g))) return rc;
+    for (int k = 0; k < nd; ++k) {
+        DiskDev* d = g.disks[k];
+        saved_ctl[k] = *d->ctl_host;
+        const size_t bytes = 3 * d->N * d->prec;
+        if ((rc = dev_alloc(scratch, &sg[k], bytes))) return free_saved(), rc;
+        if (hipMemcpyAsync(sg[k], d->gas[g.gpar], bytes, hipMemcpyDeviceToDevice, g.stream) != hipSuccess)
+            return free_saved(), fail(PG_E_HIP, "state copy failed");
+        if (g.dust) {
+            if ((rc = dev_alloc(scratch, &sd[k], bytes))) return free_saved(), rc;
+            if (hipMemcpyAsync(sd[k], d->dust[g.dpar], bytes, hipMemcpyDeviceToDevice, g.stream) != hipSuccess)
+                return free_saved(), fail(PG_E_HIP, "state copy failed");
+        }
+    }
+    auto restore = [&]() -> int {
+        for (int k = 0; k < nd; ++k) {
+            DiskDev* d = g.disks[k];
+            const size_t bytes = 3 * d->N * d->prec;
+            HIP_TRY(hipMemcpyAsync(d->gas[g.gpar], sg[k], bytes, hipMemcpyDeviceToDevice, g.stream));
+            if (g.dust) HIP_TRY(hipMemcpyAsync(d->dust[g.dpar], sd[k], bytes, hipMemcpyDeviceToDevice, g.stream));
+            *d->ctl_host = saved_ctl[k];
+        }
+        int r = push_ctl(g);
+        if (r) return r;
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        return PG_OK;
+    };
+    const bool prof = g.profile;
+    g.profile = false;
+    const int tiles[] = {128, 256, 512};
+    const int rings[] = {256, 512};
+    double best = 1e300;
+    int bt = g.tile_x, bb = g.ring_b, bo = g.order;
+    std::vector<double> t(nd), dt(nd);
+    for (int tx : tiles)
+        for (int rb : rings)
+            for (int order = 0; order < 3; ++order) {
+                if ((rc = group_schedule(g, tx, rb, order))) {
+                    if (rc == PG_E_INVALID) continue;  // does not fit this batch
+                    g.profile = prof;
+                    return free_saved(), rc;
+                }
+                float ms = 0.f;
+                for (int k = 0; k < nd; ++k) {
+                    t[k] = saved_ctl[k].t;
+                    dt[k] = saved_ctl[k].dt > 0.0 ? saved_ctl[k].dt : g.disks[k]->P.end_time;
+                }
+                // first call captures the graph and warms up; the second is timed
+                if ((rc = group_run(g, 1, t.data(), dt.data(), nullptr, 1, nullptr)) ||
+                    (rc = group_run(g, steps, t.data(), dt.data(), nullptr, 1, &ms)) || (rc = restore())) {
+                    g.profile = prof;
+                    return free_saved(), rc;
+                }
+                const double per = ms / steps;
+                if (per < best) {
+                    best = per;
+                    bt = tx;
+                    bb = rb;
+                    bo = order;
+                }
+            }
+    g.profile = prof;
+    free_saved();
+    g_err.clear();
+    if ((rc = group_schedule(g, bt, bb, bo))) return rc;
+    if (best_ms) *best_ms = best;
     return PG_OK;
 }
 
@@ -606,160 +1128,52 @@ int cfl_rate(pg_handle* h) {
 extern "C" {
 
 const char* pg_last_error(void) { return g_err.c_str(); }
-const char* pg_version(void) { return "penguin_amd 0.1 (gfx950)"; }
+const char* pg_version(void) { return "penguin_amd 0.2 (gfx950)"; }
 
 int pg_create(const pg_params* params, pg_handle** out) {
     g_err.clear();
     if (!params || !out) return fail(PG_E_INVALID, "null argument");
     *out = nullptr;
-    int rc = validate(*params);
-    if (rc) return rc;
     pg_handle* h = new pg_handle();
-    h->P = *params;
-    h->prec = params->precision;
-    h->nx = params->xres;
-    h->ny = params->yres;
-    h->tile_x = (h->nx % 512 == 0) ? 512 : (h->nx == 384 ? 384 : (h->nx == 128 ? 128 : 256));
-    if (const char* e = std::getenv("PG_TILE_X")) {
-        int v = std::atoi(e);
-        if (v == 128 || v == 256 || v == 384 || v == 512 || v == 1024) h->tile_x = v;
+    int rc = group_init(h->g, 1, params);
+    if (rc) {
+        group_release(h->g);
+        delete h;
+        return rc;
     }
-    h->N = (size_t)h->nx * h->ny;
-    auto cleanup = [&](int code) {
-        pg_destroy(h);
-        return code;
-    };
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-        return cleanup(fail(PG_E_HIP, "hipStreamCreate failed"));
-    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-        return cleanup(fail(PG_E_HIP, "hipEventCreate failed"));
-    const size_t es = h->prec;
-    const size_t N = h->N;
-    std::vector<double> gas, dust, rot;
-    build_tables_and_state(h, gas, dust, rot);
-    if ((rc = dev_alloc(h, &h->gas, 3 * N * es)) || (rc = dev_alloc(h, &h->gasT, 3 * N * es)) ||
-        (rc = dev_alloc(h, &h->vis, 2 * N * es)) ||
-        (rc = dev_alloc(h, &h->tab, h->tabh.size() * es)) || (rc = dev_alloc(h, &h->phitab, h->phih.size() * es)) ||
-        (rc = dev_alloc(h, &h->res, h->nx * es)) ||
-        (rc = dev_alloc(h, &h->inc, h->nx * sizeof(int))) || (rc = dev_alloc(h, &h->rot, rot.size() * sizeof(double))) ||
-        (rc = dev_alloc(h, &h->ctl, sizeof(StepCtl))))
-        return cleanup(rc);
-    h->rate_nblk = ((h->nx + 255) / 256) * h->ny;
-    if ((rc = dev_alloc(h, &h->rate_part, sizeof(double) * h->rate_nblk))) return cleanup(rc);
-    if (h->P.dust_flag) {
-        if ((rc = dev_alloc(h, &h->dust, 3 * N * es)) || (rc = dev_alloc(h, &h->dustT, 3 * N * es))) return cleanup(rc);
-    }
-    if (hipHostMalloc((void**)&h->ctl_host, sizeof(StepCtl), hipHostMallocDefault) != hipSuccess)
-        return cleanup(fail(PG_E_NOMEM, "hipHostMalloc failed"));
-    std::memset(h->ctl_host, 0, sizeof(StepCtl));
-    if ((rc = upload(h, h->tabh, h->tab)) || (rc = upload(h, h->phih, h->phitab)) || (rc = upload(h, gas, h->gas)) ||
-        (rc = upload(h, gas, h->gasT)))
-        return cleanup(rc);
-    if (h->P.dust_flag && ((rc = upload(h, dust, h->dust)) || (rc = upload(h, dust, h->dustT)))) return cleanup(rc);
-    std::vector<double> zero(2 * N, 0.0);
-    if ((rc = upload(h, zero, h->vis))) return cleanup(rc);
-    if (hipMemcpy(h->rot, rot.data(), rot.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(fail(PG_E_HIP, "rot upload failed"));
-    if (hipMemset(h->inc, 0, h->nx * sizeof(int)) != hipSuccess) return cleanup(fail(PG_E_HIP, "memset failed"));
-    StepCtl& c = *h->ctl_host;
-    c.planet0 = h->planet_init;
-    c.planet1 = h->planet_init;
-    c.end_time = h->P.end_time;
-    if ((rc = push_ctl(h))) return cleanup(rc);
-    if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail(PG_E_HIP, "sync failed"));
     *out = h;
     return PG_OK;
 }
 
 int pg_destroy(pg_handle* h) {
     if (!h) return PG_OK;
-    if (h->stream) hipStreamSynchronize(h->stream);
-    if (h->graph) hipGraphExecDestroy(h->graph);
-    void* bufs[] = {h->gas, h->gasT, h->dust, h->dustT, h->vis, h->tab, h->phitab, h->res, h->inc, h->rot, h->dts, h->ctl, h->rate_part};
-    for (void* b : bufs)
-        if (b) hipFree(b);
-    if (h->ctl_host) hipHostFree(h->ctl_host);
-    for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
-    if (h->ev0) hipEventDestroy(h->ev0);
-    if (h->ev1) hipEventDestroy(h->ev1);
-    if (h->stream) hipStreamDestroy(h->stream);
+    group_release(h->g);
     delete h;
     return PG_OK;
 }
 
 int pg_get_state(pg_handle* h, double* gas, double* dust) {
     CHECK_H(h);
-    const size_t N = h->N;
-    const int nx = h->nx;
-    std::vector<double> g, dd;
-    int rc;
-    if (gas) {
-        if ((rc = download(h, h->gas, g, 3 * N))) return rc;
-        for (size_t o = 0; o < N; ++o) {
-            int ip = (int)(o % nx) + NPAD;
-            double rho = g[o];
-            gas[o] = rho;
-            gas[N + o] = tabv(h, T_CS2, ip) * rho;
-            gas[2 * N + o] = g[N + o];
-            gas[3 * N + o] = tabv(h, T_VBG, ip) + g[2 * N + o];
-            gas[4 * N + o] = 0.0;
-        }
-    }
-    if (dust) {
-        if (!h->P.dust_flag) return fail(PG_E_STATE, "the disk has no dust");
-        if ((rc = download(h, h->dust, dd, 3 * N))) return rc;
-        for (size_t o = 0; o < N; ++o) {
-            int ip = (int)(o % nx) + NPAD;
-            dust[o] = dd[o];
-            dust[N + o] = dd[N + o];
-            dust[2 * N + o] = tabv(h, T_VBG, ip) + dd[2 * N + o];
-            dust[3 * N + o] = 0.0;
-        }
-    }
-    return PG_OK;
+    return disk_get_state(h->g, 0, gas, dust);
 }
 
 int pg_set_state(pg_handle* h, const double* gas, const double* dust) {
     CHECK_H(h);
-    const size_t N = h->N;
-    const int nx = h->nx;
-    int rc;
-    if (gas) {
-        std::vector<double> g(3 * N);
-        for (size_t o = 0; o < N; ++o) {
-            int ip = (int)(o % nx) + NPAD;
-            g[o] = gas[o];
-            g[N + o] = gas[2 * N + o];
-            g[2 * N + o] = gas[3 * N + o] - tabv(h, T_VBG, ip);
-        }
-        if ((rc = upload(h, g, h->gas))) return rc;
-    }
-    if (dust) {
-        if (!h->P.dust_flag) return fail(PG_E_STATE, "the disk has no dust");
-        std::vector<double> d(3 * N);
-        for (size_t o = 0; o < N; ++o) {
-            int ip = (int)(o % nx) + NPAD;
-            d[o] = dust[o];
-            d[N + o] = dust[N + o];
-            d[2 * N + o] = dust[2 * N + o] - tabv(h, T_VBG, ip);
-        }
-        if ((rc = upload(h, d, h->dust))) return rc;
-    }
-    h->rate_valid = false;
-    return PG_OK;
+    return disk_set_state(h->g, 0, gas, dust);
 }
 
 int pg_global_dt(pg_handle* h, double dt_prev, double* dt_out) {
     CHECK_H(h);
     if (!dt_out) return fail(PG_E_INVALID, "null dt_out");
+    Group& g = h->g;
     int rc;
-    if ((rc = cfl_rate(h))) return rc;
-    if ((rc = pull_ctl(h))) return rc;
+    if ((rc = cfl_rate(g))) return rc;
+    if ((rc = pull_ctl(g))) return rc;
     double rate = 0.0;
-    unsigned long long bits = h->ctl_host->rate_bits;
+    unsigned long long bits = g.disks[0]->ctl_host->rate_bits;
     std::memcpy(&rate, &bits, sizeof(double));
     if (!(rate > 0.0) || !std::isfinite(rate)) return fail(PG_E_STATE, "CFL rate is %g (state not finite?)", rate);
-    double dt = h->P.CFL / rate;
+    double dt = g.disks[0]->P.CFL / rate;
     double cap = dt_prev * 1.1;
     *dt_out = dt < cap ? dt : cap;
     return PG_OK;
@@ -768,139 +1182,62 @@ int pg_global_dt(pg_handle* h, double dt_prev, double* dt_out) {
 int pg_solve(pg_handle* h, double t, double dt) {
     CHECK_H(h);
     if (!(dt > 0.0)) return fail(PG_E_INVALID, "dt must be positive");
+    Group& g = h->g;
     int rc;
-    if ((rc = pull_ctl(h))) return rc;
-    h->ctl_host->t = t;
-    h->ctl_host->dt = dt;
-    h->ctl_host->mode = 0;
-    h->ctl_host->rate_bits = 0ull;
-    if ((rc = push_ctl(h))) return rc;
-    if ((rc = prologue(h, 0, nullptr))) return rc;
-    if ((rc = passes(h))) return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return timers_resolve(h);
+    if ((rc = pull_ctl(g))) return rc;
+    StepCtl& c = *g.disks[0]->ctl_host;
+    c.t = t;
+    c.dt = dt;
+    c.mode = 0;
+    c.rate_bits = 0ull;
+    if ((rc = push_ctl(g))) return rc;
+    if ((rc = prologue(g, 0, 1))) return rc;
+    if ((rc = passes(g))) return rc;
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return timers_resolve(g);
 }
 
 int pg_run(pg_handle* h, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph) {
     CHECK_H(h);
-    if (nsteps < 0 || !t_io || !dt_io) return fail(PG_E_INVALID, "bad arguments");
-    int rc;
-    if (nsteps == 0) return PG_OK;
-    if (h->dts_cap < nsteps) {
-        if (h->dts) hipFree(h->dts);
-        h->dts = nullptr;
-        if ((rc = dev_alloc(h, &h->dts, sizeof(double) * nsteps))) return rc;
-        h->dts_cap = nsteps;
-    }
-    if ((rc = pull_ctl(h))) return rc;
-    StepCtl& c = *h->ctl_host;
-    c.t = *t_io;
-    c.dt = *dt_io;
-    c.dt_prev = *dt_io;
-    c.nstep = 0;
-    c.mode = 1;
-    c.end_time = h->P.end_time;
-    c.rate_bits = 0ull;
-    if ((rc = push_ctl(h))) return rc;
-    // rate of the current state for the first step (later steps get it from
-    // the corrector)
-    hipError_t e = h->prec == 8 ? Launch<double>::cfl(h->args<double>(), h->stream)
-                                : Launch<float>::cfl(h->args<float>(), h->stream);
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch cfl failed: %s", hipGetErrorString(e));
-    if (use_graph && !h->profile) {
-        if (h->graph && (h->graph_gas != h->gas || h->graph_dust != h->dust || h->graph_dts != h->dts)) {
-            hipGraphExecDestroy(h->graph);
-            h->graph = nullptr;
-        }
-        if (!h->graph) {
-            h->graph_gas = h->gas;
-            h->graph_dust = h->dust;
-            h->graph_dts = h->dts;
-            hipGraph_t g;
-            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            int rc1 = prologue(h, 1, h->dts);
-            int rc2 = rc1 ? rc1 : passes(h);
-            hipError_t ec = hipStreamEndCapture(h->stream, &g);
-            if (rc2) return rc2;
-            if (ec != hipSuccess) return fail(PG_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
-            ec = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
-            if (ec != hipSuccess) return fail(PG_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ec));
-        }
-        for (int n = 0; n < nsteps; ++n) HIP_TRY(hipGraphLaunch(h->graph, h->stream));
-    } else {
-        for (int n = 0; n < nsteps; ++n) {
-            if ((rc = timed_begin(h))) return rc;
-            if ((rc = prologue(h, 1, h->dts))) return rc;
-            if ((rc = timed_end(h, K_PRO))) return rc;
-            if ((rc = passes(h))) return rc;
-        }
-    }
-    if ((rc = pull_ctl(h))) return rc;
-    if ((rc = timers_resolve(h))) return rc;
-    if (dts) HIP_TRY(hipMemcpy(dts, h->dts, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
-    *t_io = c.t + c.dt;
-    *dt_io = c.dt;
-    return PG_OK;
+    return group_run(h->g, nsteps, t_io, dt_io, dts, use_graph, nullptr);
 }
 
 int pg_viscosity_eval(pg_handle* h, int on_T, double* vis_out) {
     CHECK_H(h);
-    hipError_t e = h->prec == 8
-                       ? Launch<double>::viscosity(h->args<double>(), on_T ? (const double*)h->gasT : (const double*)h->gas, h->stream)
-                       : Launch<float>::viscosity(h->args<float>(), on_T ? (const float*)h->gasT : (const float*)h->gas, h->stream);
+    Group& g = h->g;
+    hipError_t e = g.prec == 8 ? Launch<double>::viscosity(g.ls<double>(), g.par(), on_T ? 1 : 0, g.stream)
+                               : Launch<float>::viscosity(g.ls<float>(), g.par(), on_T ? 1 : 0, g.stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch viscosity failed: %s", hipGetErrorString(e));
+    const DiskDev* d = g.disks[0];
     if (vis_out) {
         std::vector<double> v;
-        int rc = download(h, h->vis, v, 2 * h->N);
+        int rc = download(g.prec, g.stream, d->vis, v, 2 * d->N);
         if (rc) return rc;
-        std::memcpy(vis_out, v.data(), 2 * h->N * sizeof(double));
+        std::memcpy(vis_out, v.data(), 2 * d->N * sizeof(double));
     }
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 
 int pg_planet(pg_handle* h, double t, double* body10) {
     CHECK_H(h);
+    Group& g = h->g;
     int rc;
-    if ((rc = pull_ctl(h))) return rc;
-    h->ctl_host->t = t;
-    h->ctl_host->dt = 0.0;
-    if ((rc = push_ctl(h))) return rc;
-    // mode 0 with dt = 0 would divide by zero in the orbital shift: evaluate
-    // only the planet (host copy of the same closed form would do as well)
-    DiskArgs<double> dd = h->args<double>();
-    DiskArgs<float> df = h->args<float>();
-    dd.orbadv = 0;
-    df.orbadv = 0;
-    hipError_t e = h->prec == 8 ? launch_prologue<double>(dd, 0, nullptr, h->stream)
-                                : launch_prologue<float>(df, 0, nullptr, h->stream);
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
-    if ((rc = pull_ctl(h))) return rc;
-    if (body10) std::memcpy(body10, &h->ctl_host->planet0, sizeof(Body));
+    if ((rc = pull_ctl(g))) return rc;
+    StepCtl& c = *g.disks[0]->ctl_host;
+    c.t = t;
+    c.dt = 0.0;
+    if ((rc = push_ctl(g))) return rc;
+    // the orbital shift divides by dt: evaluate only the planet
+    if ((rc = prologue(g, 0, 0))) return rc;
+    if ((rc = pull_ctl(g))) return rc;
+    if (body10) std::memcpy(body10, &c.planet0, sizeof(Body));
     return PG_OK;
 }
 
 int pg_sources(pg_handle* h, double t_planet, double mdt, double dt, int inplace) {
     CHECK_H(h);
-    // the kernel takes its half step from the control block: dt_block = 2*dt
-    int rc;
-    if ((rc = pull_ctl(h))) return rc;
-    h->ctl_host->t = t_planet;
-    h->ctl_host->dt = 2.0 * dt;
-    if ((rc = push_ctl(h))) return rc;
-    DiskArgs<double> dd = h->args<double>();
-    DiskArgs<float> df = h->args<float>();
-    dd.orbadv = 0;
-    df.orbadv = 0;
-    hipError_t e = h->prec == 8 ? launch_prologue<double>(dd, 0, nullptr, h->stream)
-                                : launch_prologue<float>(df, 0, nullptr, h->stream);
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
-    // planet0 holds the planet at t_planet; modes 1/2 read planet1, so copy
-    if ((rc = pull_ctl(h))) return rc;
-    h->ctl_host->planet1 = h->ctl_host->planet0;
-    h->ctl_host->pk1 = h->ctl_host->pk0;
-    if ((rc = push_ctl(h))) return rc;
+    Group& g = h->g;
     int mode;
     if (mdt == 0.0) {
         if (!inplace) return fail(PG_E_INVALID, "mdt == 0 is only supported in place");
@@ -910,94 +1247,69 @@ int pg_sources(pg_handle* h, double t_planet, double mdt, double dt, int inplace
             return fail(PG_E_INVALID, "mdt must be 0 or equal to dt (the two uses in DS)");
         mode = inplace ? 2 : 1;
     }
-    e = h->prec == 8 ? Launch<double>::sources(h->args<double>(), mode, 0, 0, h->stream)
-                     : Launch<float>::sources(h->args<float>(), mode, 0, 0, h->stream);
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch sources failed: %s", hipGetErrorString(e));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return PG_OK;
-}
-
-static int set_dt_and_prologue(pg_handle* h, double dt, int orbadv) {
+    // the kernel takes its half step from the control block: dt_block = 2*dt
     int rc;
-    if ((rc = pull_ctl(h))) return rc;
-    h->ctl_host->dt = dt;
-    if ((rc = push_ctl(h))) return rc;
-    DiskArgs<double> dd = h->args<double>();
-    DiskArgs<float> df = h->args<float>();
-    dd.orbadv = orbadv;
-    df.orbadv = orbadv;
-    hipError_t e = h->prec == 8 ? launch_prologue<double>(dd, 0, nullptr, h->stream)
-                                : launch_prologue<float>(df, 0, nullptr, h->stream);
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
+    if ((rc = pull_ctl(g))) return rc;
+    StepCtl& c = *g.disks[0]->ctl_host;
+    c.t = t_planet;
+    c.dt = 2.0 * dt;
+    if ((rc = push_ctl(g))) return rc;
+    if ((rc = prologue(g, 0, 0))) return rc;
+    // planet0 holds the planet at t_planet; modes 1/2 read planet1, so copy
+    if ((rc = pull_ctl(g))) return rc;
+    c.planet1 = c.planet0;
+    c.pk1 = c.pk0;
+    if ((rc = push_ctl(g))) return rc;
+    hipError_t e = g.prec == 8 ? Launch<double>::sources(g.ls<double>(), g.par(), mode, 0, 0, g.stream)
+                               : Launch<float>::sources(g.ls<float>(), g.par(), mode, 0, 0, g.stream);
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch sources failed: %s", hipGetErrorString(e));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 
-int pg_sweep_x(pg_handle* h, double dt) {
+static int set_dt_and_prologue(Group& g, double dt, int orbadv) {
+    int rc;
+    if ((rc = pull_ctl(g))) return rc;
+    g.disks[0]->ctl_host->dt = dt;
+    if ((rc = push_ctl(g))) return rc;
+    return prologue(g, 0, orbadv);
+}
+
+static int sweep_module(pg_handle* h, double dt, int which) {
     CHECK_H(h);
-    int rc = set_dt_and_prologue(h, dt, 0);
+    Group& g = h->g;
+    const bool dust = which == 1 || which == 3;
+    if (dust && !g.dust) return fail(PG_E_STATE, "the disk has no dust");
+    if (!(dt > 0.0)) return fail(PG_E_INVALID, "dt must be positive");
+    int rc = set_dt_and_prologue(g, dt, 0);
     if (rc) return rc;
-    hipError_t e = h->prec == 8 ? Launch<double>::sweep_x_gas(h->args<double>(), h->stream)
-                                : Launch<float>::sweep_x_gas(h->args<float>(), h->stream);
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch sweep_x_gas failed: %s", hipGetErrorString(e));
-    std::swap(h->gas, h->gasT);
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    hipError_t e;
+    const int par = g.par();
+    if (g.prec == 8) {
+        const LaunchSet<double> L = g.ls<double>();
+        e = which == 0 ? Launch<double>::sweep_x_gas(L, par, g.stream)
+            : which == 1 ? Launch<double>::sweep_x_dust(L, par, g.stream)
+            : which == 2 ? Launch<double>::ring_gas(L, par, 1, g.stream)
+                         : Launch<double>::ring_dust(L, par, 1, g.stream);
+    } else {
+        const LaunchSet<float> L = g.ls<float>();
+        e = which == 0 ? Launch<float>::sweep_x_gas(L, par, g.stream)
+            : which == 1 ? Launch<float>::sweep_x_dust(L, par, g.stream)
+            : which == 2 ? Launch<float>::ring_gas(L, par, 1, g.stream)
+                         : Launch<float>::ring_dust(L, par, 1, g.stream);
+    }
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch sweep failed: %s", hipGetErrorString(e));
+    if (dust) g.dpar ^= 1;
+    else g.gpar ^= 1;
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 
-int pg_sweep_x_dust(pg_handle* h, double dt) {
-    CHECK_H(h);
-    if (!h->P.dust_flag) return fail(PG_E_STATE, "the disk has no dust");
-    int rc = set_dt_and_prologue(h, dt, 0);
-    if (rc) return rc;
-    hipError_t e = h->prec == 8 ? Launch<double>::sweep_x_dust(h->args<double>(), h->stream)
-                                : Launch<float>::sweep_x_dust(h->args<float>(), h->stream);
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch sweep_x_dust failed: %s", hipGetErrorString(e));
-    std::swap(h->dust, h->dustT);
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return PG_OK;
-}
-
+int pg_sweep_x(pg_handle* h, double dt) { return sweep_module(h, dt, 0); }
+int pg_sweep_x_dust(pg_handle* h, double dt) { return sweep_module(h, dt, 1); }
 // azimuthal sweep alone: the ring kernels with the orbital stages switched off
-int pg_sweep_y(pg_handle* h, double dt) {
-    CHECK_H(h);
-    int rc = set_dt_and_prologue(h, dt, 0);
-    if (rc) return rc;
-    hipError_t e;
-    if (h->prec == 8) {
-        DiskArgs<double> d = h->args<double>();
-        d.orbadv = 0;
-        e = Launch<double>::ring_gas(d, h->stream);
-    } else {
-        DiskArgs<float> d = h->args<float>();
-        d.orbadv = 0;
-        e = Launch<float>::ring_gas(d, h->stream);
-    }
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch ring_gas failed: %s", hipGetErrorString(e));
-    std::swap(h->gas, h->gasT);
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return PG_OK;
-}
-
-int pg_sweep_y_dust(pg_handle* h, double dt) {
-    CHECK_H(h);
-    if (!h->P.dust_flag) return fail(PG_E_STATE, "the disk has no dust");
-    int rc = set_dt_and_prologue(h, dt, 0);
-    if (rc) return rc;
-    hipError_t e;
-    if (h->prec == 8) {
-        DiskArgs<double> d = h->args<double>();
-        d.orbadv = 0;
-        e = Launch<double>::ring_dust(d, h->stream);
-    } else {
-        DiskArgs<float> d = h->args<float>();
-        d.orbadv = 0;
-        e = Launch<float>::ring_dust(d, h->stream);
-    }
-    if (e != hipSuccess) return fail(PG_E_HIP, "launch ring_dust failed: %s", hipGetErrorString(e));
-    std::swap(h->dust, h->dustT);
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return PG_OK;
-}
+int pg_sweep_y(pg_handle* h, double dt) { return sweep_module(h, dt, 2); }
+int pg_sweep_y_dust(pg_handle* h, double dt) { return sweep_module(h, dt, 3); }
 
 // set_OrbAdv + shift_OrbAdv + advecty(_dust) alone: the ring kernels with
 // their sweep stage switched off (each cell is carried unchanged to its
@@ -1005,42 +1317,43 @@ int pg_sweep_y_dust(pg_handle* h, double dt) {
 int pg_orbital_advection(pg_handle* h, double dt, int32_t* inc, double* res) {
     CHECK_H(h);
     if (!(dt > 0.0)) return fail(PG_E_INVALID, "dt must be positive");
-    int rc = set_dt_and_prologue(h, dt, 1);
+    Group& g = h->g;
+    int rc = set_dt_and_prologue(g, dt, 1);
     if (rc) return rc;
-    hipError_t e = hipSuccess;
-    for (int fluid = 0; fluid < (h->P.dust_flag ? 2 : 1); ++fluid) {
-        if (h->prec == 8) {
-            DiskArgs<double> d = h->args<double>();
-            d.sweep_on = 0;
-            e = fluid ? Launch<double>::ring_dust(d, h->stream) : Launch<double>::ring_gas(d, h->stream);
+    for (int fluid = 0; fluid < (g.dust ? 2 : 1); ++fluid) {
+        hipError_t e;
+        if (g.prec == 8) {
+            const LaunchSet<double> L = g.ls<double>();
+            e = fluid ? Launch<double>::ring_dust(L, g.par(), 2, g.stream) : Launch<double>::ring_gas(L, g.par(), 2, g.stream);
         } else {
-            DiskArgs<float> d = h->args<float>();
-            d.sweep_on = 0;
-            e = fluid ? Launch<float>::ring_dust(d, h->stream) : Launch<float>::ring_gas(d, h->stream);
+            const LaunchSet<float> L = g.ls<float>();
+            e = fluid ? Launch<float>::ring_dust(L, g.par(), 2, g.stream) : Launch<float>::ring_gas(L, g.par(), 2, g.stream);
         }
         if (e != hipSuccess) return fail(PG_E_HIP, "launch ring failed: %s", hipGetErrorString(e));
-        if (fluid) std::swap(h->dust, h->dustT);
-        else std::swap(h->gas, h->gasT);
+        if (fluid) g.dpar ^= 1;
+        else g.gpar ^= 1;
     }
-    if (inc) HIP_TRY(hipMemcpyAsync(inc, h->inc, h->nx * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    const DiskDev* d = g.disks[0];
+    if (inc) HIP_TRY(hipMemcpyAsync(inc, d->inc, d->nx * sizeof(int), hipMemcpyDeviceToHost, g.stream));
     if (res) {
         std::vector<double> r;
-        if ((rc = download(h, h->res, r, h->nx))) return rc;
-        std::memcpy(res, r.data(), h->nx * sizeof(double));
+        if ((rc = download(g.prec, g.stream, d->res, r, d->nx))) return rc;
+        std::memcpy(res, r.data(), d->nx * sizeof(double));
     }
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 
 int pg_killwave(pg_handle* h, double dt) {
     CHECK_H(h);
-    if (!h->P.kill_flag) return PG_OK;
-    int rc = set_dt_and_prologue(h, dt, 0);
+    Group& g = h->g;
+    if (!g.kill) return PG_OK;
+    int rc = set_dt_and_prologue(g, dt, 0);
     if (rc) return rc;
-    hipError_t e = h->prec == 8 ? Launch<double>::sources(h->args<double>(), 3, 1, 0, h->stream)
-                                : Launch<float>::sources(h->args<float>(), 3, 1, 0, h->stream);
+    hipError_t e = g.prec == 8 ? Launch<double>::sources(g.ls<double>(), g.par(), 3, 1, 0, g.stream)
+                               : Launch<float>::sources(g.ls<float>(), g.par(), 3, 1, 0, g.stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch killwave failed: %s", hipGetErrorString(e));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 
@@ -1073,20 +1386,21 @@ int pg_snapshot_label(const pg_params* P, char* buf, size_t len) {
 int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
     CHECK_H(h);
     if (!filename) return fail(PG_E_INVALID, "null filename");
-    const size_t N = h->N;
-    std::vector<double> gas(5 * N), dust(h->P.dust_flag ? 4 * N : 0);
-    int rc = pg_get_state(h, gas.data(), h->P.dust_flag ? dust.data() : nullptr);
+    const DiskDev* d = h->g.disks[0];
+    const size_t N = d->N;
+    std::vector<double> gas(5 * N), dust(d->P.dust_flag ? 4 * N : 0);
+    int rc = disk_get_state(h->g, 0, gas.data(), d->P.dust_flag ? dust.data() : nullptr);
     if (rc) return rc;
     std::ofstream f(filename, std::ios::out | std::ios::binary | std::ios::trunc);
     if (!f) return fail(PG_E_INVALID, "cannot open %s for writing", filename);
-    std::vector<double> xa(h->nx + 1), ya(h->ny + 1);
-    for (int i = 0; i <= h->nx; ++i) xa[i] = tabv(h, T_XA, i + NPAD);
-    for (int j = 0; j <= h->ny; ++j) ya[j] = h->P.ymin + (h->P.ymax - h->P.ymin) * (double)j / (double)h->ny;
+    std::vector<double> xa(d->nx + 1), ya(d->ny + 1);
+    for (int i = 0; i <= d->nx; ++i) xa[i] = tabv(d, T_XA, i + NPAD);
+    for (int j = 0; j <= d->ny; ++j) ya[j] = d->P.ymin + (d->P.ymax - d->P.ymin) * (double)j / (double)d->ny;
     f.write((const char*)&t, sizeof(double));
     f.write((const char*)xa.data(), sizeof(double) * xa.size());
     f.write((const char*)ya.data(), sizeof(double) * ya.size());
     f.write((const char*)gas.data(), sizeof(double) * 4 * N);  // rho, p, u, v (ndim 2: no w)
-    if (h->P.dust_flag) f.write((const char*)dust.data(), sizeof(double) * 4 * N);  // w_d too (output.cu:171)
+    if (d->P.dust_flag) f.write((const char*)dust.data(), sizeof(double) * 4 * N);  // w_d too (output.cu:171)
     if (!f) return fail(PG_E_INVALID, "write to %s failed", filename);
     return PG_OK;
 }
@@ -1094,30 +1408,31 @@ int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
 int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out) {
     CHECK_H(h);
     if (!filename) return fail(PG_E_INVALID, "null filename");
-    const size_t N = h->N;
-    const size_t nfield = 4 + (h->P.dust_flag ? 4 : 0);
-    const size_t expect = sizeof(double) * (1 + (h->nx + 1) + (h->ny + 1) + nfield * N);
+    const DiskDev* d = h->g.disks[0];
+    const size_t N = d->N;
+    const size_t nfield = 4 + (d->P.dust_flag ? 4 : 0);
+    const size_t expect = sizeof(double) * (1 + (d->nx + 1) + (d->ny + 1) + nfield * N);
     std::ifstream f(filename, std::ios::in | std::ios::binary);
     if (!f) return fail(PG_E_INVALID, "cannot open %s", filename);
     f.seekg(0, std::ios::end);
     size_t size = (size_t)f.tellg();
     if (size != expect)
-        return fail(PG_E_INVALID, "%s has %zu bytes, a %dx%d%s snapshot has %zu", filename, size, h->nx, h->ny,
-                    h->P.dust_flag ? " gas+dust" : "", expect);
+        return fail(PG_E_INVALID, "%s has %zu bytes, a %dx%d%s snapshot has %zu", filename, size, d->nx, d->ny,
+                    d->P.dust_flag ? " gas+dust" : "", expect);
     f.seekg(0);
     double t = 0.0;
     f.read((char*)&t, sizeof(double));
-    std::vector<double> xa(h->nx + 1), ya(h->ny + 1);
+    std::vector<double> xa(d->nx + 1), ya(d->ny + 1);
     f.read((char*)xa.data(), sizeof(double) * xa.size());
     f.read((char*)ya.data(), sizeof(double) * ya.size());
-    for (int i = 0; i <= h->nx; ++i)
-        if (std::fabs(xa[i] - tabv(h, T_XA, i + NPAD)) > 1e-12 * std::fabs(xa[i]))
+    for (int i = 0; i <= d->nx; ++i)
+        if (std::fabs(xa[i] - tabv(d, T_XA, i + NPAD)) > 1e-12 * std::fabs(xa[i]))
             return fail(PG_E_INVALID, "%s: radial grid does not match this disk", filename);
-    std::vector<double> gas(5 * N, 0.0), dust(h->P.dust_flag ? 4 * N : 0);
+    std::vector<double> gas(5 * N, 0.0), dust(d->P.dust_flag ? 4 * N : 0);
     f.read((char*)gas.data(), sizeof(double) * 4 * N);
-    if (h->P.dust_flag) f.read((char*)dust.data(), sizeof(double) * 4 * N);
+    if (d->P.dust_flag) f.read((char*)dust.data(), sizeof(double) * 4 * N);
     if (!f) return fail(PG_E_INVALID, "read from %s failed", filename);
-    int rc = pg_set_state(h, gas.data(), h->P.dust_flag ? dust.data() : nullptr);
+    int rc = disk_set_state(h->g, 0, gas.data(), d->P.dust_flag ? dust.data() : nullptr);
     if (rc) return rc;
     if (t_out) *t_out = t;
     return PG_OK;
@@ -1126,36 +1441,113 @@ int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out) {
 int pg_stream(pg_handle* h, void** stream) {
     CHECK_H(h);
     if (!stream) return fail(PG_E_INVALID, "null argument");
-    *stream = (void*)h->stream;
+    *stream = (void*)h->g.stream;
     return PG_OK;
 }
 
 int pg_profile_enable(pg_handle* h, int on) {
     CHECK_H(h);
-    h->profile = on != 0;
-    h->timers.clear();
-    h->ev_used = 0;
+    h->g.profile = on != 0;
+    h->g.timers.clear();
+    h->g.ev_used = 0;
     return PG_OK;
 }
 
 int pg_profile_get(pg_handle* h, const char* kernel, double* avg_us, int* count) {
     CHECK_H(h);
-    if (!kernel || !avg_us || !count) return fail(PG_E_INVALID, "null argument");
-    auto it = h->timers.find(kernel);
-    if (it == h->timers.end()) {
-        *avg_us = 0.0;
-        *count = 0;
-        return PG_OK;
-    }
-    *count = it->second.count;
-    *avg_us = it->second.count ? 1000.0 * it->second.total_ms / it->second.count : 0.0;
-    return PG_OK;
+    return group_profile_get(h->g, kernel, avg_us, count);
 }
 
 int pg_device_bytes(pg_handle* h, size_t* bytes) {
     CHECK_H(h);
     if (!bytes) return fail(PG_E_INVALID, "null argument");
-    *bytes = h->bytes;
+    *bytes = h->g.bytes;
+    return PG_OK;
+}
+
+// ---- batches ---------------------------------------------------------------
+int pg_batch_create(int n, const pg_params* params, pg_batch** out) {
+    g_err.clear();
+    if (!params || !out || n < 1) return fail(PG_E_INVALID, "bad arguments");
+    *out = nullptr;
+    pg_batch* b = new pg_batch();
+    int rc = group_init(b->g, n, params);
+    if (rc) {
+        group_release(b->g);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PG_OK;
+}
+
+int pg_batch_destroy(pg_batch* b) {
+    if (!b) return PG_OK;
+    group_release(b->g);
+    delete b;
+    return PG_OK;
+}
+
+int pg_batch_size(pg_batch* b, int* n) {
+    CHECK_H(b);
+    if (!n) return fail(PG_E_INVALID, "null argument");
+    *n = b->g.ndisk();
+    return PG_OK;
+}
+
+int pg_batch_get_state(pg_batch* b, int disk, double* gas, double* dust) {
+    CHECK_H(b);
+    if (disk < 0 || disk >= b->g.ndisk()) return fail(PG_E_INVALID, "disk %d out of range", disk);
+    return disk_get_state(b->g, disk, gas, dust);
+}
+
+int pg_batch_set_state(pg_batch* b, int disk, const double* gas, const double* dust) {
+    CHECK_H(b);
+    if (disk < 0 || disk >= b->g.ndisk()) return fail(PG_E_INVALID, "disk %d out of range", disk);
+    return disk_set_state(b->g, disk, gas, dust);
+}
+
+int pg_batch_run(pg_batch* b, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph) {
+    CHECK_H(b);
+    return group_run(b->g, nsteps, t_io, dt_io, dts, use_graph, nullptr);
+}
+
+int pg_batch_set_schedule(pg_batch* b, int tile_x, int ring_b, int order) {
+    CHECK_H(b);
+    return group_schedule(b->g, tile_x, ring_b, order);
+}
+
+int pg_batch_get_schedule(pg_batch* b, int* tile_x, int* ring_b, int* order) {
+    CHECK_H(b);
+    if (!tile_x || !ring_b || !order) return fail(PG_E_INVALID, "null argument");
+    *tile_x = b->g.tile_x;
+    *ring_b = b->g.ring_b;
+    *order = b->g.order;
+    return PG_OK;
+}
+
+int pg_batch_autotune(pg_batch* b, int steps, double* best_ms) {
+    CHECK_H(b);
+    return group_autotune(b->g, steps, best_ms);
+}
+
+int pg_batch_profile_enable(pg_batch* b, int on) {
+    CHECK_H(b);
+    b->g.profile = on != 0;
+    b->g.timers.clear();
+    b->g.ev_used = 0;
+    return PG_OK;
+}
+
+int pg_batch_profile_get(pg_batch* b, const char* kernel, double* avg_us, int* count) {
+    CHECK_H(b);
+    return group_profile_get(b->g, kernel, avg_us, count);
+}
+
+int pg_batch_device_bytes(pg_batch* b, size_t* bytes) {
+    CHECK_H(b);
+    if (!bytes) return fail(PG_E_INVALID, "null argument");
+    *bytes = b->g.bytes;
     return PG_OK;
 }
 

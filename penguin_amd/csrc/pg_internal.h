@@ -28,7 +28,7 @@ namespace pg {
 constexpr int NPAD = 3;
 constexpr int NGAS = 3;
 constexpr int NDUST = 3;
-constexpr int RING_THREADS = 512;   // threads per azimuthal-ring block
+constexpr int RING_THREADS = 512;   // default threads per azimuthal-ring block
 
 struct Body {
     double m, a, e, x, y, z, vx, vy, vz, rs;
@@ -88,18 +88,14 @@ struct DiskArgs {
     int nx, ny;
     int has_dust, visc, orbadv, kill;
     int n_planet;
-    int tile_x;          // interior cells per x-sweep tile
-    int sweep_on;        // 0: the ring kernels skip the sweep (shift + advection only)
-    real* gas;           // 3*N current state
-    real* gasT;          // 3*N scratch / ping-pong
-    real* dust;          // 3*N
-    real* dustT;         // 3*N
+    real* gas[2];        // 3*N state, ping-pong pair; the launch's parity picks the current one
+    real* dust[2];       // 3*N
     real* vis;           // 2*N viscous acceleration*rho
     const real* tab;     // per-radius tables, field-major: tab[f*tab_stride + ip]
     int tab_stride;      // nx + 2*NPAD + 1 radii
+    const real* phitab;  // 4*ny: sin(phi_j), cos(phi_j), sin(phi_j/2), cos(phi_j/2)
     real dy;             // azimuthal cell width (angle), uniform grid
     real idy;            // 1/dy
-    const real* phitab;  // 4*ny: sin(phi_j), cos(phi_j), sin(phi_j/2), cos(phi_j/2)
     real wLy[3], wRy[3]; // 4th-order weights of the uniform azimuthal grid
     real gam, sc_h, Stokes, D_G_ratio, frame_omega;
     double CFL;
@@ -112,6 +108,26 @@ struct DiskArgs {
     double ramp_time, planet_mass, planet_ecc, planet_radius, rs;
     double* rate_part;   // per-block maxima of the CFL rate (rate_nblk entries)
     int rate_nblk;
+    double* dts;         // step sizes recorded by the device loop (or null)
+    int dts_cap;         // entries of dts
+};
+
+// A set of disks launched together (one disk, or a batch).  Each kernel's
+// grid walks a work list of (disk, local block) pairs, so one launch covers
+// every disk of the set whatever their sizes.
+template <typename real>
+struct LaunchSet {
+    const DiskArgs<real>* disks;
+    int ndisk;
+    const int2* wl_x;    // x sweeps: local block = tile + tiles_per_row * row
+    int n_x;
+    int tile_x;          // x-sweep tile width (compile-time variants)
+    const int2* wl_ring; // rings: local block = radius (-1: padding)
+    int n_ring;
+    int ring_b;          // ring chunk (threads per block)
+    int ny_max;          // largest ring, sizes the LDS
+    const int2* wl_pt;   // pointwise kernels: local block = 256-radius segment + segments_per_row * row
+    int n_pt;
 };
 
 }  // namespace pg

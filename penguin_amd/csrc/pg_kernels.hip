@@ -34,6 +34,13 @@
 
 namespace pg {
 
+// A launch's parity word: bit 0 selects the current gas buffer of every disk
+// in the set, bit 1 the current dust buffer; a kernel that writes a fluid
+// writes the other buffer of its pair.
+__device__ __forceinline__ int gsel(int par) { return par & 1; }
+__device__ __forceinline__ int dsel(int par) { return (par >> 1) & 1; }
+
+
 // per-radius tables, field-major: lanes that differ in radius read
 // consecutive addresses
 #define TAB(f, ip) d.tab[(size_t)(f) * d.tab_stride + (ip)]
@@ -379,7 +386,9 @@ __host__ __device__ inline void orbit_solution(Body& b, double t, double ramp_ti
 // load -> flattening -> parabolas -> interface fluxes -> conservative update.
 // TX is a compile-time constant so every LDS array offset is an immediate.
 template <typename real, int TX>
-__global__ __launch_bounds__(TX) void k_sweep_x_gas(DiskArgs<real> d) {
+__global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par) {
+    const int2 w = ls.wl_x[blockIdx.x];
+    const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
@@ -398,12 +407,13 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(DiskArgs<real> d) {
     const int nx = d.nx;
     const size_t N = (size_t)nx * d.ny;
     const int t = threadIdx.x;
-    const int j = blockIdx.y;
-    const int x0 = blockIdx.x * TX;
+    const int tiles = (nx + TX - 1) / TX;
+    const int j = w.y / tiles;
+    const int x0 = (w.y - j * tiles) * TX;
     const int nt = min(TX, nx - x0);
     const int ns = nt + 2 * NPAD;
     const real dt = (real)d.ctl->dt;
-    const real* __restrict__ in = d.gas;
+    const real* __restrict__ in = d.gas[gsel(par)];
     const size_t row = (size_t)j * nx;
 
     for (int s = t; s < ns; s += TX) {
@@ -462,7 +472,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(DiskArgs<real> d) {
         s_pr[k] = pres;
     }
     __syncthreads();
-    real* __restrict__ out = d.gasT;
+    real* __restrict__ out = d.gas[gsel(par) ^ 1];
     for (int s = t + NPAD; s < ns - NPAD; s += TX) {
         const int ip = x0 + s;
         const int i = ip - NPAD;
@@ -490,7 +500,9 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(DiskArgs<real> d) {
 
 // ----------------------------------------------------------- x sweep, dust
 template <typename real, int TX>
-__global__ __launch_bounds__(TX) void k_sweep_x_dust(DiskArgs<real> d) {
+__global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par) {
+    const int2 w = ls.wl_x[blockIdx.x];
+    const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
@@ -506,12 +518,13 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(DiskArgs<real> d) {
     const int nx = d.nx;
     const size_t N = (size_t)nx * d.ny;
     const int t = threadIdx.x;
-    const int j = blockIdx.y;
-    const int x0 = blockIdx.x * TX;
+    const int tiles = (nx + TX - 1) / TX;
+    const int j = w.y / tiles;
+    const int x0 = (w.y - j * tiles) * TX;
     const int nt = min(TX, nx - x0);
     const int ns = nt + 2 * NPAD;
     const real dt = (real)d.ctl->dt;
-    const real* __restrict__ in = d.dust;
+    const real* __restrict__ in = d.dust[dsel(par)];
     const size_t row = (size_t)j * nx;
 
     for (int s = t; s < ns; s += TX) {
@@ -556,7 +569,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(DiskArgs<real> d) {
         s_FL[k] = FL * g;
     }
     __syncthreads();
-    real* __restrict__ out = d.dustT;
+    real* __restrict__ out = d.dust[dsel(par) ^ 1];
     const real rfloor = d.D_G_ratio * real(1.0e-10);
     for (int s = t + NPAD; s < ns - NPAD; s += TX) {
         const int ip = x0 + s;
@@ -607,16 +620,6 @@ struct RingLds {
     __device__ __forceinline__ real* F2() const { return b + 12 * S; }
     __device__ __forceinline__ real* F3() const { return b + 13 * S; }
 };
-
-// Radius handled by ring block b.  Workgroups are dispatched round-robin
-// over the 8 XCDs (each with its own L2); giving XCD x the contiguous radii
-// [x*per, (x+1)*per) lets neighbouring radii, which share cache lines of the
-// radius-fastest layout, hit in the same L2.  Returns -1 for padding blocks.
-__device__ __forceinline__ int ring_radius(int b, int nx) {
-    const int per = (nx + 7) >> 3;
-    const int r = (b & 7) * per + (b >> 3);
-    return r < nx ? r : -1;
-}
 
 __device__ __forceinline__ int wrap(int j, int n) {
     j %= n;
@@ -710,15 +713,20 @@ __device__ __forceinline__ void ring_copy_out(const DiskArgs<real>& d, const Rin
     }
 }
 
+// flags: bit 0 = sweep stage, bit 1 = orbital shift + residual advection
 template <typename real, int B>
-__global__ __launch_bounds__(B) void k_ring_gas(DiskArgs<real> d) {
+__global__ __launch_bounds__(B) void k_ring_gas(LaunchSet<real> ls, int par, int flags) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
+    const int2 w = ls.wl_ring[blockIdx.x];
+    const int i = w.y;
+    if (i < 0) return;
+    const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
     const int t = threadIdx.x;
-    const int i = ring_radius(blockIdx.x, nx);
-    if (i < 0) return;
+    const bool sweep_on = flags & 1;
+    const bool orbadv = (flags & 2) && d.orbadv;
     RingLds<real, B> L{sm, sm + 3 * ny};
     const int ip = i + NPAD;
     const real cs2 = TAB(T_CS2, ip);
@@ -729,9 +737,9 @@ __global__ __launch_bounds__(B) void k_ring_gas(DiskArgs<real> d) {
     const real dts = dt * iR;          // set_state receives dt/rad (riemann.cu:111)
     const real idy = d.idy;
     const real fup = dt * idy * iR;    // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
-    const int inc = d.orbadv ? d.inc[i] : 0;
-    const real* __restrict__ in = d.gas;
-    const real w[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
+    const int inc = orbadv ? d.inc[i] : 0;
+    const real* __restrict__ in = d.gas[gsel(par)];
+    const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
 
     // ---- stage A: azimuthal sweep (rho, vp normal, u transverse) ----
     for (int c0 = 0; c0 < ny; c0 += B) {
@@ -754,7 +762,7 @@ __global__ __launch_bounds__(B) void k_ring_gas(DiskArgs<real> d) {
                                  c1p[s - 1], c1p[s], c1p[s + 1], cs0);
         }
         __syncthreads();
-        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, L.fl()[s], w);
+        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, L.fl()[s], wy);
         __syncthreads();
         for (int k = t + NPAD; k < ns - 2; k += B) {
             real rl, ul, vl, rr, ur, vr;
@@ -778,7 +786,7 @@ __global__ __launch_bounds__(B) void k_ring_gas(DiskArgs<real> d) {
             real rn = r + Dr;
             const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
             real vn = vp, un = u;
-            if (!d.sweep_on) {
+            if (!sweep_on) {
                 rn = r;
             } else if (rn < rmin) {
                 rn = rmin;
@@ -795,25 +803,30 @@ __global__ __launch_bounds__(B) void k_ring_gas(DiskArgs<real> d) {
         }
         __syncthreads();
     }
-    if (!d.orbadv) {
-        ring_copy_out(d, L, i, d.gasT);
+    if (!orbadv) {
+        ring_copy_out(d, L, i, d.gas[gsel(par) ^ 1]);
         return;
     }
     // ---- stage B: residual advection at speed res (advection.cu:1-105) ----
     const real speed = d.res[i];
     const real frac = fabs(speed) * dts * idy;   // |speed| dt / (R dy)
-    ring_advect(d, L, i, d.gasT, speed, frac, fup, false, real(0), w);
+    ring_advect(d, L, i, d.gas[gsel(par) ^ 1], speed, frac, fup, false, real(0), wy);
 }
 
+// flags: bit 0 = sweep stage, bit 1 = orbital shift + residual advection
 template <typename real, int B>
-__global__ __launch_bounds__(B) void k_ring_dust(DiskArgs<real> d) {
+__global__ __launch_bounds__(B) void k_ring_dust(LaunchSet<real> ls, int par, int flags) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
+    const int2 w = ls.wl_ring[blockIdx.x];
+    const int i = w.y;
+    if (i < 0) return;
+    const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
     const int t = threadIdx.x;
-    const int i = ring_radius(blockIdx.x, nx);
-    if (i < 0) return;
+    const bool sweep_on = flags & 1;
+    const bool orbadv = (flags & 2) && d.orbadv;
     RingLds<real, B> L{sm, sm + 3 * ny};
     const int ip = i + NPAD;
     const real iR = TAB(T_IR, ip);
@@ -821,10 +834,10 @@ __global__ __launch_bounds__(B) void k_ring_dust(DiskArgs<real> d) {
     const real dts = dt * iR;          // dust_flow: dt /= rad for geom > 2 (dust.cu:127)
     const real idy = d.idy;
     const real f12 = dt * idy * iR;    // dust.cu:309-313
-    const int inc = d.orbadv ? d.inc[i] : 0;
-    const real* __restrict__ in = d.dust;
+    const int inc = orbadv ? d.inc[i] : 0;
+    const real* __restrict__ in = d.dust[dsel(par)];
     const real rfloor = d.D_G_ratio * real(1.0e-10);
-    const real w[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
+    const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
 
     for (int c0 = 0; c0 < ny; c0 += B) {
         const int nt = min(B, ny - c0);
@@ -837,7 +850,7 @@ __global__ __launch_bounds__(B) void k_ring_dust(DiskArgs<real> d) {
             L.c2()[s] = in[N + o];
         }
         __syncthreads();
-        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, real(0), w);
+        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, real(0), wy);
         __syncthreads();
         for (int k = t + NPAD; k < ns - 2; k += B) {
             real Fr, Fu, Fw;
@@ -857,7 +870,7 @@ __global__ __launch_bounds__(B) void k_ring_dust(DiskArgs<real> d) {
             real rn = r + Dr;
             const real rmin = fmax(r * real(1.0e-10), rfloor);
             real vn = vp, un = u;
-            if (!d.sweep_on) {
+            if (!sweep_on) {
                 rn = r;
             } else if (rn < rmin) {
                 rn = rmin;
@@ -873,13 +886,13 @@ __global__ __launch_bounds__(B) void k_ring_dust(DiskArgs<real> d) {
         }
         __syncthreads();
     }
-    if (!d.orbadv) {
-        ring_copy_out(d, L, i, d.dustT);
+    if (!orbadv) {
+        ring_copy_out(d, L, i, d.dust[dsel(par) ^ 1]);
         return;
     }
     const real speed = d.res[i];
     const real frac = fabs(speed) * dts * idy;
-    ring_advect(d, L, i, d.dustT, speed, frac, f12, true, rfloor, w);
+    ring_advect(d, L, i, d.dust[dsel(par) ^ 1], speed, frac, f12, true, rfloor, wy);
 }
 
 // ----------------------------------------------------------------- viscosity
@@ -887,12 +900,18 @@ __global__ __launch_bounds__(B) void k_ring_dust(DiskArgs<real> d) {
 // derivatives of v_phi = vbg(R) + vp are split into the tabulated
 // background part and the stored deviation (the difference operators are
 // linear, so this is the same arithmetic rearranged).
+// which = 0: viscosity of the current state; 1: of the other buffer (the
+// predictor, viscosity_tensor_evaluation2)
 template <typename real>
-__global__ __launch_bounds__(256) void k_viscosity(DiskArgs<real> d, const real* __restrict__ st) {
+__global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, int which) {
+    const int2 w = ls.wl_pt[blockIdx.x];
+    const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int segs = (nx + 255) >> 8;
+    const int j = w.y / segs;
+    const int i = (w.y - j * segs) * 256 + threadIdx.x;
+    const real* __restrict__ st = d.gas[gsel(par) ^ which];
     if (i >= nx) return;
     const int ip = i + NPAD;
     real nu[3];
@@ -986,7 +1005,7 @@ __device__ __forceinline__ void block_max_store(double* part, real v) {
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) m = fmax(m, wmax[k]);
-        part[blockIdx.x + gridDim.x * blockIdx.y] = m;
+        part[0] = m;
     }
 }
 
@@ -1026,11 +1045,17 @@ __device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x
 // t+dt, in place) optionally followed by wave killing and the CFL rate,
 // 3 wave killing alone.
 template <typename real>
-__global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int do_kill, int do_rate) {
+__global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, int mode, int do_kill, int do_rate) {
+    const int2 w = ls.wl_pt[blockIdx.x];
+    const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int segs = (nx + 255) >> 8;
+    const int j = w.y / segs;
+    const int i = (w.y - j * segs) * 256 + threadIdx.x;
+    real* __restrict__ gas = d.gas[gsel(par)];
+    real* __restrict__ gasT = d.gas[gsel(par) ^ 1];
+    real* __restrict__ dust = d.dust[dsel(par)];
     real rate = real(0);
     if (i < nx) {
         const int ip = i + NPAD;
@@ -1044,8 +1069,7 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
         const real ebal = TAB(T_EBAL, ip);
         real px = real(0), py = real(0);
         if (d.n_planet > 0) planet_acc(pk, d.phitab + 4 * j, R, px, py);
-        const real* g = d.gas;
-        const real rho = g[o], cu = g[N + o], cvp = g[2 * N + o];
+        const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
         const real irho = FM<real>::rcp(rho);
         real vx = real(0), vy = real(0);
         if (d.visc) { vx = d.vis[o] * irho; vy = d.vis[N + o] * irho; }
@@ -1053,7 +1077,7 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
         real Tu = cu, Tvp = cvp;
         real Qu = 0, Qvp = 0, Du = 0, Dvp = 0, ts = 0, its = 0;
         if (d.has_dust) {
-            Qu = d.dust[N + o]; Qvp = d.dust[2 * N + o];
+            Qu = dust[N + o]; Qvp = dust[2 * N + o];
             Du = Qu; Dvp = Qvp;
             ts = TAB(T_TS, ip) * irho;            // St / (c_s/h0) / rho (dust.cu:1-4)
             its = rho * TAB(T_ITS, ip);
@@ -1076,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
         real nr_ = rho;
         if (mode == 3) { nu_ = cu; nvp = cvp; }   // wave killing alone
         if (mode == 1) {
-            d.gasT[o] = nr_; d.gasT[N + o] = nu_; d.gasT[2 * N + o] = nvp;
+            gasT[o] = nr_; gasT[N + o] = nu_; gasT[2 * N + o] = nvp;
         } else {
             real ndu = Qu, ndvp = Qvp;
             if (d.has_dust && mode != 3) {
@@ -1090,9 +1114,9 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
                 nu_ += kk * (TAB(T_G_U, ip) - nu_);
                 nvp += kk * (real(0) - nvp);
             }
-            if (do_kill) d.gas[o] = nr_;   // rho changes only by wave killing
-            d.gas[N + o] = nu_; d.gas[2 * N + o] = nvp;
-            if (d.has_dust) { d.dust[N + o] = ndu; d.dust[2 * N + o] = ndvp; }
+            if (do_kill) gas[o] = nr_;   // rho changes only by wave killing
+            gas[N + o] = nu_; gas[2 * N + o] = nvp;
+            if (d.has_dust) { dust[N + o] = ndu; dust[2 * N + o] = ndvp; }
             if (do_rate) {
                 const real cs = TAB(T_C0, ip);
                 real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
@@ -1101,31 +1125,37 @@ __global__ __launch_bounds__(256) void k_sources(DiskArgs<real> d, int mode, int
             }
         }
     }
-    if (do_rate) block_max_store(d.rate_part, rate);
+    if (do_rate) block_max_store(d.rate_part + w.y, rate);
 }
 
 // CFL rate of the current state (get_dt_lv1, timestep.cu:18-97).
 template <typename real>
-__global__ __launch_bounds__(256) void k_cfl(DiskArgs<real> d) {
+__global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
+    const int2 w = ls.wl_pt[blockIdx.x];
+    const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int segs = (nx + 255) >> 8;
+    const int j = w.y / segs;
+    const int i = (w.y - j * segs) * 256 + threadIdx.x;
+    const real* __restrict__ gas = d.gas[gsel(par)];
+    const real* __restrict__ dust = d.dust[dsel(par)];
     real rate = real(0);
     if (i < nx) {
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
         const real cs = TAB(T_C0, ip);
-        real sx = fabs(d.gas[N + o]) + cs, sy = fabs(d.gas[2 * N + o]) + cs;
-        if (d.has_dust) { sx = fmax(fabs(d.dust[N + o]), sx); sy = fmax(fabs(d.dust[2 * N + o]), sy); }
+        real sx = fabs(gas[N + o]) + cs, sy = fabs(gas[2 * N + o]) + cs;
+        if (d.has_dust) { sx = fmax(fabs(dust[N + o]), sx); sy = fmax(fabs(dust[2 * N + o]), sy); }
         rate = fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip));
     }
-    block_max_store(d.rate_part, rate);
+    block_max_store(d.rate_part + w.y, rate);
 }
 
 // Max of the partials into the control block (host-driven global_dt).
 template <typename real>
-__global__ __launch_bounds__(1024) void k_rate_reduce(DiskArgs<real> d) {
+__global__ __launch_bounds__(1024) void k_rate_reduce(LaunchSet<real> ls) {
+    const DiskArgs<real>& d = ls.disks[blockIdx.x];
     double m = reduce_partials(d.rate_part, d.rate_nblk);
     if (threadIdx.x == 0) d.ctl->rate_bits = (unsigned long long)__double_as_longlong(m);
 }
@@ -1149,7 +1179,9 @@ __host__ __device__ inline PlanetK planet_kernel_view(const Body& b) {
 // dt = min(CFL/rate, 1.1 dt_prev) clipped to end_time, record it.  Then the
 // planet at t and t+dt and the orbital shift of every radius.
 template <typename real>
-__global__ __launch_bounds__(1024) void k_prologue(DiskArgs<real> d, int mode, double* dts) {
+__global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode, int do_orbadv) {
+    const DiskArgs<real>& d = ls.disks[blockIdx.x];
+    double* dts = d.dts;
     StepCtl* ctl = d.ctl;
     __shared__ double s_dt;
     double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
@@ -1166,7 +1198,7 @@ __global__ __launch_bounds__(1024) void k_prologue(DiskArgs<real> d, int mode, d
             ctl->t = t;
             ctl->dt = dt;
             ctl->dt_prev = dt;
-            if (dts) dts[ctl->nstep] = dt;
+            if (dts && ctl->nstep < d.dts_cap) dts[ctl->nstep] = dt;
             ctl->nstep += 1;
         }
         Body b0 = ctl->planet0;
@@ -1180,7 +1212,7 @@ __global__ __launch_bounds__(1024) void k_prologue(DiskArgs<real> d, int mode, d
         s_dt = dt;
     }
     __syncthreads();
-    if (d.orbadv) {
+    if (d.orbadv && do_orbadv) {
         const double dt = s_dt;
         for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
             // set_orbital_advection (orbit.cu:74-99), FP64
@@ -1202,85 +1234,87 @@ static hipError_t set_lds(Kern k, size_t lds) {
 }
 
 template <typename real, int TX>
-static hipError_t launch_sweep_x(const DiskArgs<real>& d, bool dust, hipStream_t s) {
-    dim3 grid((d.nx + TX - 1) / TX, d.ny);
+static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, bool dust, hipStream_t s) {
     const size_t S = TX + 2 * NPAD;
     if (dust) {
         size_t lds = 13 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_dust<real, TX>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), grid, dim3(TX), lds, s, d);
+        hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par);
     } else {
         size_t lds = 18 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_gas<real, TX>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), grid, dim3(TX), lds, s, d);
+        hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par);
     }
     return hipGetLastError();
 }
 
 template <typename real>
-static hipError_t sweep_x_any(const DiskArgs<real>& d, bool dust, hipStream_t s) {
-    switch (d.tile_x) {
-        case 128: return launch_sweep_x<real, 128>(d, dust, s);
-        case 384: return launch_sweep_x<real, 384>(d, dust, s);
-        case 512: return launch_sweep_x<real, 512>(d, dust, s);
-        case 1024: return launch_sweep_x<real, 1024>(d, dust, s);
-        default: return launch_sweep_x<real, 256>(d, dust, s);
+static hipError_t sweep_x_any(const LaunchSet<real>& ls, int par, bool dust, hipStream_t s) {
+    switch (ls.tile_x) {
+        case 128: return launch_sweep_x<real, 128>(ls, par, dust, s);
+        case 384: return launch_sweep_x<real, 384>(ls, par, dust, s);
+        case 512: return launch_sweep_x<real, 512>(ls, par, dust, s);
+        case 1024: return launch_sweep_x<real, 1024>(ls, par, dust, s);
+        default: return launch_sweep_x<real, 256>(ls, par, dust, s);
+    }
+}
+
+template <typename real, int B>
+static hipError_t launch_ring(const LaunchSet<real>& ls, int par, int flags, bool dust, hipStream_t s) {
+    size_t lds = ((size_t)3 * ls.ny_max + (size_t)14 * (B + 2 * NPAD)) * sizeof(real);
+    if (dust) {
+        hipError_t e = set_lds(k_ring_dust<real, B>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_ring_dust<real, B>), dim3(ls.n_ring), dim3(B), lds, s, ls, par, flags);
+    } else {
+        hipError_t e = set_lds(k_ring_gas<real, B>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_ring_gas<real, B>), dim3(ls.n_ring), dim3(B), lds, s, ls, par, flags);
+    }
+    return hipGetLastError();
+}
+
+template <typename real>
+static hipError_t ring_any(const LaunchSet<real>& ls, int par, int flags, bool dust, hipStream_t s) {
+    switch (ls.ring_b) {
+        case 256: return launch_ring<real, 256>(ls, par, flags, dust, s);
+        case 1024: return launch_ring<real, 1024>(ls, par, flags, dust, s);
+        default: return launch_ring<real, 512>(ls, par, flags, dust, s);
     }
 }
 
 template <typename real>
 struct Launch {
-    static hipError_t sweep_x_gas(const DiskArgs<real>& d, hipStream_t s) { return sweep_x_any(d, false, s); }
-    static hipError_t sweep_x_dust(const DiskArgs<real>& d, hipStream_t s) { return sweep_x_any(d, true, s); }
     static size_t ring_lds(int ny, int B) { return ((size_t)3 * ny + (size_t)14 * (B + 2 * NPAD)) * sizeof(real); }
-    static hipError_t ring_gas(const DiskArgs<real>& d, hipStream_t s) {
-        constexpr int B = RING_THREADS;
-        size_t lds = ring_lds(d.ny, B);
-        hipError_t e = set_lds(k_ring_gas<real, B>, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_ring_gas<real, B>), dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
+    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, hipStream_t s) { return sweep_x_any(ls, par, false, s); }
+    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, hipStream_t s) { return sweep_x_any(ls, par, true, s); }
+    static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, false, s); }
+    static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, true, s); }
+    static hipError_t viscosity(const LaunchSet<real>& ls, int par, int which, hipStream_t s) {
+        hipLaunchKernelGGL(k_viscosity<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par, which);
         return hipGetLastError();
     }
-    static hipError_t ring_dust(const DiskArgs<real>& d, hipStream_t s) {
-        constexpr int B = RING_THREADS;
-        size_t lds = ring_lds(d.ny, B);
-        hipError_t e = set_lds(k_ring_dust<real, B>, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_ring_dust<real, B>), dim3(8 * ((d.nx + 7) / 8)), dim3(B), lds, s, d);
+    static hipError_t sources(const LaunchSet<real>& ls, int par, int mode, int kill, int rate, hipStream_t s) {
+        hipLaunchKernelGGL(k_sources<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par, mode, kill, rate);
         return hipGetLastError();
     }
-    static hipError_t viscosity(const DiskArgs<real>& d, const real* st, hipStream_t s) {
-        dim3 grid((d.nx + 255) / 256, d.ny);
-        hipLaunchKernelGGL(k_viscosity<real>, grid, dim3(256), 0, s, d, st);
+    static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s) {
+        hipLaunchKernelGGL(k_cfl<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par);
         return hipGetLastError();
     }
-    static hipError_t sources(const DiskArgs<real>& d, int mode, int kill, int rate, hipStream_t s) {
-        dim3 grid((d.nx + 255) / 256, d.ny);
-        hipLaunchKernelGGL(k_sources<real>, grid, dim3(256), 0, s, d, mode, kill, rate);
+    static hipError_t rate_reduce(const LaunchSet<real>& ls, hipStream_t s) {
+        hipLaunchKernelGGL(k_rate_reduce<real>, dim3(ls.ndisk), dim3(1024), 0, s, ls);
         return hipGetLastError();
     }
-    static hipError_t cfl(const DiskArgs<real>& d, hipStream_t s) {
-        dim3 grid((d.nx + 255) / 256, d.ny);
-        hipLaunchKernelGGL(k_cfl<real>, grid, dim3(256), 0, s, d);
-        return hipGetLastError();
-    }
-    static hipError_t rate_reduce(const DiskArgs<real>& d, hipStream_t s) {
-        hipLaunchKernelGGL(k_rate_reduce<real>, dim3(1), dim3(1024), 0, s, d);
+    static hipError_t prologue(const LaunchSet<real>& ls, int mode, int do_orbadv, hipStream_t s) {
+        hipLaunchKernelGGL(k_prologue<real>, dim3(ls.ndisk), dim3(1024), 0, s, ls, mode, do_orbadv);
         return hipGetLastError();
     }
 };
 
 template struct Launch<float>;
 template struct Launch<double>;
-
-template <typename real>
-hipError_t launch_prologue(const DiskArgs<real>& d, int mode, double* dts, hipStream_t s) {
-    hipLaunchKernelGGL(k_prologue<real>, dim3(1), dim3(1024), 0, s, d, mode, dts);
-    return hipGetLastError();
-}
-template hipError_t launch_prologue<float>(const DiskArgs<float>&, int, double*, hipStream_t);
-template hipError_t launch_prologue<double>(const DiskArgs<double>&, int, double*, hipStream_t);
 
 }  // namespace pg

@@ -10,18 +10,16 @@ namespace pg {
 
 template <typename real>
 struct Launch {
-    static hipError_t sweep_x_gas(const DiskArgs<real>& d, hipStream_t s);
-    static hipError_t sweep_x_dust(const DiskArgs<real>& d, hipStream_t s);
     static size_t ring_lds(int ny, int B);
-    static hipError_t ring_gas(const DiskArgs<real>& d, hipStream_t s);
-    static hipError_t ring_dust(const DiskArgs<real>& d, hipStream_t s);
-    static hipError_t viscosity(const DiskArgs<real>& d, const real* st, hipStream_t s);
-    static hipError_t sources(const DiskArgs<real>& d, int mode, int kill, int rate, hipStream_t s);
-    static hipError_t cfl(const DiskArgs<real>& d, hipStream_t s);
-    static hipError_t rate_reduce(const DiskArgs<real>& d, hipStream_t s);
+    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, hipStream_t s);
+    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, hipStream_t s);
+    static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
+    static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
+    static hipError_t viscosity(const LaunchSet<real>& ls, int par, int which, hipStream_t s);
+    static hipError_t sources(const LaunchSet<real>& ls, int par, int mode, int kill, int rate, hipStream_t s);
+    static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s);
+    static hipError_t rate_reduce(const LaunchSet<real>& ls, hipStream_t s);
+    static hipError_t prologue(const LaunchSet<real>& ls, int mode, int do_orbadv, hipStream_t s);
 };
-
-template <typename real>
-hipError_t launch_prologue(const DiskArgs<real>& d, int mode, double* dts, hipStream_t s);
 
 }  // namespace pg

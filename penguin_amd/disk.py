@@ -50,6 +50,18 @@ SIGNATURES = {
     "pg_write_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
     "pg_read_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
     "pg_snapshot_label": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_size_t]),
+    "pg_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(_vp)]),
+    "pg_batch_destroy": (ctypes.c_int, [_vp]),
+    "pg_batch_size": (ctypes.c_int, [_vp, _ip]),
+    "pg_batch_get_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "pg_batch_set_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "pg_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
+    "pg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pg_batch_get_schedule": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
+    "pg_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "pg_batch_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pg_batch_profile_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "pg_batch_device_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
 }
 
 
@@ -220,4 +232,94 @@ class Disk:
     def device_bytes(self) -> int:
         b = ctypes.c_size_t()
         _check(load_library().pg_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+
+class Batch:
+    """Many independent disks advanced together (pg_batch_* entry points):
+    one launch per pass covers every disk, each disk keeps its own CFL step,
+    and a hipGraph replays a whole step of the batch."""
+
+    def __init__(self, params):
+        L = load_library()
+        self.params = [p.copy() for p in params]
+        arr = (Params * len(self.params))(*self.params)
+        h = _vp()
+        _check(L.pg_batch_create(len(self.params), arr, ctypes.byref(h)))
+        self._h = h
+        self.n = len(self.params)
+        self.t = np.zeros(self.n)
+        self.dt = np.array([float(p.end_time) for p in self.params])  # main.cu:144
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().pg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return self.n
+
+    def cells(self) -> int:
+        return int(sum(int(p.xres) * int(p.yres) for p in self.params))
+
+    def get_state(self, k: int):
+        p = self.params[k]
+        gas = np.zeros((5, int(p.yres), int(p.xres)))
+        dust = np.zeros((4, int(p.yres), int(p.xres))) if p.dust_flag else None
+        _check(load_library().pg_batch_get_state(self._h, k, _ptr(gas), _ptr(dust) if dust is not None else None))
+        return gas, dust
+
+    def set_state(self, k: int, gas=None, dust=None):
+        g = None if gas is None else np.ascontiguousarray(gas, dtype=np.float64)
+        d = None if dust is None else np.ascontiguousarray(dust, dtype=np.float64)
+        _check(load_library().pg_batch_set_state(self._h, k, _ptr(g) if g is not None else None,
+                                                 _ptr(d) if d is not None else None))
+
+    def run(self, nsteps: int, use_graph: bool = True) -> np.ndarray:
+        """nsteps of every disk; returns the step sizes, shape (n, nsteps)."""
+        dts = np.zeros((self.n, max(nsteps, 1)))
+        t = np.ascontiguousarray(self.t, dtype=np.float64)
+        dt = np.ascontiguousarray(self.dt, dtype=np.float64)
+        _check(load_library().pg_batch_run(self._h, nsteps, _ptr(t), _ptr(dt), _ptr(dts), int(use_graph)))
+        self.t, self.dt = t, dt
+        return dts[:, :nsteps]
+
+    def set_schedule(self, tile_x: int, ring_b: int, order: int = 0):
+        _check(load_library().pg_batch_set_schedule(self._h, tile_x, ring_b, order))
+
+    def schedule(self):
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(load_library().pg_batch_get_schedule(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, steps: int = 5) -> float:
+        """Pick the fastest schedule on this batch; returns its ms per step."""
+        ms = ctypes.c_double()
+        _check(load_library().pg_batch_autotune(self._h, steps, ctypes.byref(ms)))
+        return ms.value
+
+    def profile(self, on: bool = True):
+        _check(load_library().pg_batch_profile_enable(self._h, int(on)))
+
+    def kernel_time_us(self, name: str):
+        avg = ctypes.c_double()
+        cnt = ctypes.c_int()
+        _check(load_library().pg_batch_profile_get(self._h, name.encode(), ctypes.byref(avg), ctypes.byref(cnt)))
+        return avg.value, cnt.value
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        _check(load_library().pg_batch_device_bytes(self._h, ctypes.byref(b)))
         return b.value

@@ -78,3 +78,28 @@ def test_null_handle_is_an_error_not_a_crash():
     assert L.pg_solve(None, 0.0, 1e-3) == -1
     assert L.pg_destroy(None) == 0
     assert L.pg_version().decode().startswith("penguin_amd")
+
+
+def test_batch_configuration_is_checked_before_any_device_work():
+    from penguin_amd import Batch, batch_configs
+    L = load_library()
+    h = ctypes.c_void_p()
+    ps = batch_configs(3)
+    # a member that fails the single-disk checks is named by its index
+    bad = [ps[0], ps[1].replace(EOS_flag=2), ps[2]]
+    arr = (Params * 3)(*bad)
+    assert L.pg_batch_create(3, arr, ctypes.byref(h)) == -1
+    assert "disk 1" in L.pg_last_error().decode()
+    # members must share precision and the dust / viscosity / killing switches
+    for change in (dict(precision=8), dict(dust_flag=1), dict(visc_flag=0), dict(kill_flag=1)):
+        mixed = [ps[0], ps[1].replace(**change), ps[2]]
+        arr = (Params * 3)(*mixed)
+        assert L.pg_batch_create(3, arr, ctypes.byref(h)) == -1, change
+        assert "must agree" in L.pg_last_error().decode(), change
+    assert L.pg_batch_create(0, arr, ctypes.byref(h)) == -1
+    with pytest.raises(PenguinError):
+        Batch([ps[0], ps[1].replace(precision=8)])
+    assert L.pg_batch_destroy(None) == 0
+    t = (ctypes.c_double * 1)()
+    assert L.pg_batch_run(None, 1, t, t, None, 1) == -1
+    assert L.pg_batch_set_schedule(None, 256, 512, 0) == -1
