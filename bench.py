@@ -14,9 +14,12 @@ time loop does it.
 Prints one JSON line.  `value` is interior cell updates per second over the
 K timed steps (state resident in HBM before timing starts).  `roofline`
 describes the dominant kernel: algorithmic bytes per launch over its average
-HIP-event duration in the timed region, against 8 TB/s.  `cpu_baseline`
-times the FP64 host oracle (OpenMP) on a bounded sample of the same
-workload.
+HIP-event duration in the timed region, against 8 TB/s; its `traffic` is the
+HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+passes (profiles/).  `cpu_baseline` times the FP64 host oracle (OpenMP) on a
+bounded sample of the same workload.  `batched` times 64 uneven disks
+advanced together (one launch per pass, captured step, autotuned schedule)
+against looping the single-disk path over the same disks.
 """
 from __future__ import annotations
 
@@ -55,10 +58,77 @@ def parse():
     ap.add_argument("--precision", type=int, default=4, choices=(4, 8))
     ap.add_argument("--nr", type=int, default=1024)
     ap.add_argument("--nphi", type=int, default=3072)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--graph", action="store_true", help="also time the hipGraph-replayed loop")
+    ap.add_argument("--batch-disks", type=int, default=64, help="disks of the batched leg (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "c2_gas_dust_1024x3072_hbm_counters.md")
+
+
+def measured_traffic(kernel: str, nr: int, nphi: int, precision: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE passes (copy-calibrated), for the workload they
+    were taken on (1024x3072 FP32); None for any other workload."""
+    if (nr, nphi, precision) != (1024, 3072, 4) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
+def copy_probe(mib: int):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+
+
+def batched_leg(ndisk: int, steps: int, precision: int) -> dict:
+    """The batched workload (ndisk uneven 2D gas disks, own CFL dt each,
+    captured step, autotuned schedule) against looping the single-disk path
+    (pg_run with its own captured step) over the same disks, same steps."""
+    from penguin_amd import Batch, Disk, batch_configs
+
+    ps = batch_configs(ndisk, seed=1, precision=precision)
+    cells = sum(int(p.xres) * int(p.yres) for p in ps)
+    b = Batch(ps)
+    tune_ms = b.autotune(steps=3)
+    b.run(5)
+    t0 = time.perf_counter()
+    b.run(steps)
+    tb = time.perf_counter() - t0
+    sched = b.schedule()
+    b.close()
+    tl = 0.0
+    for p in ps:
+        d = Disk(p)
+        d.run(5, use_graph=True)
+        t0 = time.perf_counter()
+        d.run(steps, use_graph=True)
+        tl += time.perf_counter() - t0
+        d.close()
+    vb = cells * steps / tb
+    vl = cells * steps / tl
+    return {"disks": ndisk, "steps": steps, "cells": cells, "value": round(vb, 1), "unit": "cell-updates/s",
+            "ms_per_step": round(1e3 * tb / steps, 4), "looped_value": round(vl, 1),
+            "looped_ms_per_step": round(1e3 * tl / steps, 4), "speedup": round(vb / vl, 3),
+            "schedule": {"tile_x": sched[0], "ring_threads": sched[1], "ring_order": sched[2],
+                         "autotune_ms_per_step": round(tune_ms, 4)},
+            "workload": f"{ndisk} 2D gas disks, nr in {{128,256,384,512}}, nphi in {{256,512,1024,1536}}, "
+                        "q logU[1e-5,1e-3], alpha logU[1e-4,1e-2], h U[0.03,0.08], seed 1"}
 
 
 def main():
@@ -67,6 +137,9 @@ def main():
         print(f"bench.py measures one GPU: the whole disk lives on one MI355X; --gpus {a.gpus} is not "
               "supported (multi-GPU radial slabs are not built)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
 
     from penguin_amd import Disk, config_c2
 
@@ -103,6 +176,8 @@ def main():
         d.run(a.steps, use_graph=True)
         graph_value = cells * a.steps / (time.perf_counter() - g0)
 
+    batched = batched_leg(a.batch_disks, a.batch_steps, a.precision) if a.batch_disks > 0 else None
+
     cpu = None
     if not a.no_cpu and a.cpu_steps > 0:
         import oracle
@@ -133,7 +208,8 @@ def main():
                                f"viscosity, planet, wave killing), device-side CFL dt",
                    "nr": a.nr, "nphi": a.nphi, "mean_dt": float(sum(dts) / len(dts))},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4),
+                     "traffic": measured_traffic(dom, a.nr, a.nphi, a.precision),
                      "bytes_per_launch": bytes_launch},
         "step_roofline": {"bytes_per_cell_update": step_bytes,
                           "roofline_cell_updates_per_s": PEAK_HBM / step_bytes,
@@ -143,6 +219,8 @@ def main():
     }
     if graph_value is not None:
         out["graph_value"] = round(graph_value, 1)
+    if batched is not None:
+        out["batched"] = batched
     print(json.dumps(out))
 
 
