@@ -19,7 +19,9 @@ HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 passes (profiles/).  `cpu_baseline` times the FP64 host oracle (OpenMP) on a
 bounded sample of the same workload.  `batched` times 64 uneven disks
 advanced together (one launch per pass, captured step, autotuned schedule)
-against looping the single-disk path over the same disks.
+against looping the single-disk path over the same disks.  `adjoint` times
+the backward sweep of the gap-profile loss on the 512x1024 disk against the
+plain forward run.
 """
 from __future__ import annotations
 
@@ -63,9 +65,49 @@ def parse():
     ap.add_argument("--graph", action="store_true", help="also time the hipGraph-replayed loop")
     ap.add_argument("--batch-disks", type=int, default=64, help="disks of the batched leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the adjoint leg (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def adjoint_leg(steps: int, precision: int) -> dict:
+    """Gap-profile loss on the 512x1024 gas disk (q 1e-4, alpha 1e-3):
+    forward `steps` with checkpoints every 20, backward through the adjoint;
+    cost = backward wall time (recompute included) / plain forward wall time
+    (pg_run with its captured step) over the same steps."""
+    import numpy as np
+    from penguin_amd import Disk, config_c5
+    from penguin_amd.adjoint import gap_profile
+
+    p = config_c5(precision=precision)
+    target = gap_profile(p)
+    d = Disk(p)
+    d.run(3, use_graph=True)                     # warm up kernels and the graph
+    d.close()
+    d = Disk(p)
+    t0 = time.perf_counter()
+    d.run(steps, use_graph=True)
+    t_fwd = time.perf_counter() - t0
+    d.close()
+    d = Disk(p)
+    t0 = time.perf_counter()
+    d.adjoint_forward(steps, 20)
+    t_afwd = time.perf_counter() - t0
+    g, _ = d.get_state()
+    prof = g[0].mean(axis=0)
+    loss = float(np.sum((prof - target) ** 2))
+    dL = np.zeros((5, d.ny, d.nx))
+    dL[0] = np.broadcast_to(2.0 * (prof - target) / d.ny, (d.ny, d.nx))
+    t0 = time.perf_counter()
+    dq, dal, d0 = d.adjoint_backward(dL)
+    t_bwd = time.perf_counter() - t0
+    d.close()
+    return {"workload": f"512x1024 gas disk, q 1e-4, alpha 1e-3, {steps} steps, checkpoints every 20, "
+                        "loss = L2 of <Sigma>_phi(r) to a Gaussian-gap target",
+            "forward_s": round(t_fwd, 4), "forward_with_checkpoints_s": round(t_afwd, 4),
+            "backward_s": round(t_bwd, 4), "cost_ratio": round(t_bwd / t_fwd, 3),
+            "loss": loss, "dL_dq": dq, "dL_dalpha": dal, "dL_dsigma0_norm": float(np.linalg.norm(d0[0]))}
 
 
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "c2_gas_dust_1024x3072_hbm_counters.md")
@@ -177,6 +219,7 @@ def main():
         graph_value = cells * a.steps / (time.perf_counter() - g0)
 
     batched = batched_leg(a.batch_disks, a.batch_steps, a.precision) if a.batch_disks > 0 else None
+    adjoint = adjoint_leg(a.adjoint_steps, a.precision) if a.adjoint_steps > 0 else None
 
     cpu = None
     if not a.no_cpu and a.cpu_steps > 0:
@@ -221,6 +264,8 @@ def main():
         out["graph_value"] = round(graph_value, 1)
     if batched is not None:
         out["batched"] = batched
+    if adjoint is not None:
+        out["adjoint"] = adjoint
     print(json.dumps(out))
 
 

@@ -114,6 +114,45 @@ int pg_profile_get(pg_handle* h, const char* kernel, double* avg_us, int* count)
 /* Bytes of device memory owned by the handle. */
 int pg_device_bytes(pg_handle* h, size_t* bytes);
 
+/* ---- adjoint of the step ---------------------------------------------------
+   No counterpart in the reference (which has no derivatives).  Reverse mode
+   of the whole DS step of a gas disk without dust or wave killing: source
+   terms with the planet, viscosity, x sweep (PPM4 + HLLC), azimuthal sweep,
+   orbital shift and residual advection, each transposed at the forward
+   point with the step sizes, the integer shifts and every branch of the
+   forward arithmetic frozen.  Parameter derivatives are for planet_mass
+   (the planet's force is proportional to it) and ss_alpha (the viscous
+   forces and the radial velocity -1.5 nu(R)/R of the initial state and of
+   the radial ghost cells are proportional to it). */
+typedef struct pg_grads {
+    double dq;         /* dL/d planet_mass                                     */
+    double dalpha;     /* dL/d ss_alpha                                        */
+    double* dstate0;   /* 5*xres*yres (pg_get_state layout) or NULL: dL/d of   */
+                       /* the start state, rho carrying p = c_s^2 rho          */
+} pg_grads;
+/* Run nsteps of the device loop from the current state (time t0, previous
+   step dt0, as pg_run), keeping the state every ckpt_every steps and the
+   step sizes (dts, nsteps doubles, may be NULL). */
+int pg_adjoint_forward(pg_handle* h, int nsteps, int ckpt_every, double t0, double dt0, double* dts);
+/* Given dL/d(final state) in pg_get_state layout (5*xres*yres; the p entry
+   is folded in as c_s^2 dL/dp on rho), propagate back through every step of
+   the last pg_adjoint_forward, recomputing each segment from its
+   checkpoint; fills *grads.  The handle's state is left at the forward's
+   final state. */
+int pg_adjoint_backward(pg_handle* h, const double* dL_dstate, pg_grads* grads);
+/* Adjoint of one module call at the current state: the transpose of
+   pass 0 pg_sweep_x, 1 pg_sweep_y, 2 pg_orbital_advection (gas),
+   3 pg_viscosity_eval (lam_out has the 2 force fields), 4/5/6 pg_sources
+   kick (mdt = 0) / predictor / corrector with the viscous forces the handle
+   holds.  t and dt are the module call's arguments (t: the planet's time
+   for the sources).  lam_out, lam_in: FP64, fields rho, u (radial), v
+   (azimuthal), xres*yres each (the derivative with respect to v equals the
+   one with respect to the stored deviation from the background rotation);
+   dparam (2 doubles, may be NULL) receives dL/d planet_mass and
+   dL/d ss_alpha of the pass. */
+int pg_adjoint_pass(pg_handle* h, int pass, double t, double dt, const double* lam_out, double* lam_in,
+                    double* dparam);
+
 /* ---- batches of disks ------------------------------------------------------
    No counterpart in the reference, which runs one Grid per process
    (main.cu:157-359).  A batch advances n independent disks of uneven sizes

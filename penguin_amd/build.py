@@ -23,8 +23,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpenguin_amd.so")
-SOURCES = ["pg_kernels.hip", "pg_host.cpp"]
-HEADERS = ["pg_internal.h", "pg_launch.h", "../../include/penguin_amd.h", "../../include/pg_params.h"]
+SOURCES = ["pg_kernels.hip", "pg_adjoint.hip", "pg_host.cpp"]
+HEADERS = ["pg_internal.h", "pg_launch.h", "pg_device.h", "pg_adjoint_fn.h", "../../include/penguin_amd.h", "../../include/pg_params.h"]
 
 
 def hipcc() -> str:
@@ -64,6 +64,25 @@ def build(verbose: bool = False, out: str = LIB) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp_out, out)
+    build_adjcheck(verbose)
+    return out
+
+
+ADJCHECK = os.path.join(HERE, "libpg_adjcheck.so")
+
+
+def build_adjcheck(verbose: bool = False, out: str = ADJCHECK) -> str:
+    """Host-only library evaluating the per-cell functions and their
+    adjoints in FP64 (pg_adjcheck.cpp), for the derivative checks of the
+    CPU test suite."""
+    cmd = [hipcc()] + flags() + ["-shared", "-x", "hip", os.path.join(CSRC, "pg_adjcheck.cpp"), "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed on pg_adjcheck.cpp")
+    os.replace(out + ".tmp", out)
     return out
 
 

@@ -1,0 +1,865 @@
+// pg_adjoint.hip -- reverse-mode (adjoint) kernels of the Godunov step of a
+// 2D locally isothermal gas disk (no dust, no wave killing).
+//
+// The step is split as the forward step runs it (solve.cu:212-292):
+//   V0 = viscosity(S0); S1 = kick(S0, V0); S2 = x-sweep(S1);
+//   SA = azimuthal sweep(S2); S3 = shift + residual advection(SA);
+//   T = predictor(S3, V0); V1 = viscosity(T); S4 = corrector(S3, V1)
+// and each kernel here is the transpose of one of these maps at the forward
+// point, with the step size, the orbital shift and every branch of the
+// forward arithmetic (limiters, floors, the HLLC upwind choice) frozen.
+// Each kernel recomputes the forward values of its tile from the stored
+// input of its pass, then runs the phases backwards.  Adjoints of stencil
+// inputs are gathered (each thread sums what its cell receives), so results
+// do not depend on thread order; only the two parameter adjoints (planet
+// mass, viscosity alpha) are summed with atomics.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pg_adjoint_fn.h"
+#include "pg_device.h"
+#include "pg_internal.h"
+
+namespace pg {
+
+// Sum a per-thread value over the block and add it to *dst (one atomic per
+// block).
+__device__ __forceinline__ void block_sum_add(double* dst, double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __shared__ double wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) s += wsum[k];
+        if (s != 0.0) atomicAdd(dst, s);
+    }
+}
+
+template <typename real>
+__device__ __forceinline__ void put(real* li, size_t o, real v, int acc) {
+    li[o] = acc ? li[o] + v : v;
+}
+
+// ------------------------------------------------------- x sweep adjoint
+// Tile of TX output cells of one row; forward values on cells x0-6 ..
+// x0+TX+5 (LDS slot s <-> cell x0-6+s).  Cells beyond the radial ghost
+// zone take the outermost ghost's values: they feed nothing the outputs
+// depend on, and finite values keep their zero adjoints zero.
+template <typename real, int TX>
+__global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
+    const DiskArgs<real>& d = *A.d;
+    constexpr int S = TX + 4 * NPAD;
+    __shared__ real r_[S], p_[S], u_[S], vp_[S], L_[S], fl_[S];
+    __shared__ real sl_[4][S], sr_[4][S];
+    __shared__ real Fr_[S], Fu_[S], FL_[S], pr_[S];
+    __shared__ real ar_[S], ap_[S], au_[S], aL_[S], avp_[S];
+    __shared__ real asl_[4][S], asr_[4][S];
+    __shared__ real cA_[S], cB_[S], cC_[S], cP_[S];
+    __shared__ real ld_[4][4][S];
+    __shared__ real lfl_[S];
+    __shared__ real lfp_[5][S];
+    const int nx = d.nx;
+    const size_t N = (size_t)nx * d.ny;
+    const int t = threadIdx.x;
+    const int j = blockIdx.y;
+    const int x0 = blockIdx.x * TX;
+    const int nt = min(TX, nx - x0);
+    const int ns = nt + 4 * NPAD;
+    const real dt = (real)d.ctl->dt;
+    const size_t row = (size_t)j * nx;
+    const int ipmax = nx + 2 * NPAD - 1;
+    auto tip = [&](int s) { return min(max(x0 - 2 * NPAD + s + NPAD, 0), ipmax); };   // clamped table index
+    auto inside = [&](int s) { const int g = x0 - 2 * NPAD + s; return g >= 0 && g < nx; };
+
+    // ---- forward recompute (k_sweep_x_gas) ----
+    for (int s = t; s < ns; s += TX) {
+        const int g = x0 - 2 * NPAD + s;
+        const int ip = tip(s);
+        real r, u, vp;
+        if (g < 0 || g >= nx) {
+            r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); vp = TAB(T_G_V, ip);
+        } else {
+            r = A.s[row + g]; u = A.s[N + row + g]; vp = A.s[2 * N + row + g];
+        }
+        r_[s] = r; p_[s] = TAB(T_CS2, ip) * r; u_[s] = u; vp_[s] = vp;
+        L_[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
+    }
+    __syncthreads();
+    for (int s = t + 2; s < ns - 2; s += TX)
+        fl_[s] = flatten5(p_[s - 2], p_[s - 1], p_[s], p_[s + 1], p_[s + 2], u_[s - 1], u_[s], u_[s + 1], real(1));
+    __syncthreads();
+    for (int s = t + 2; s < ns - 2; s += TX) {
+        const int ip = tip(s);
+        const real w0 = TAB(T_WL0, ip), w1 = TAB(T_WL1, ip), w2 = TAB(T_WL2, ip);
+        const real v0 = TAB(T_WR0, ip), v1 = TAB(T_WR1, ip), v2 = TAB(T_WR2, ip);
+        const real* f4[4] = {r_, p_, u_, L_};
+        for (int f = 0; f < 4; ++f) {
+            const real* a = f4[f];
+            real l, r;
+            slopes(a[s - 2], a[s - 1], a[s], a[s + 1], a[s + 2], w0, w1, w2, v0, v1, v2, fl_[s], l, r);
+            sl_[f][s] = l; sr_[f][s] = r;
+        }
+    }
+    __syncthreads();
+    auto state_in = [&](int s, real* in) {
+        in[TI_R] = r_[s]; in[TI_P] = p_[s]; in[TI_U] = u_[s];
+        in[TI_RSL] = sl_[0][s]; in[TI_RSR] = sr_[0][s];
+        in[TI_PSL] = sl_[1][s]; in[TI_PSR] = sr_[1][s];
+        in[TI_USL] = sl_[2][s]; in[TI_USR] = sr_[2][s];
+        in[TI_VSL] = sl_[3][s]; in[TI_V] = L_[s]; in[TI_VSR] = sr_[3][s];
+    };
+    for (int k = t + NPAD; k < ns - 2; k += TX) {
+        const int ipl = tip(k - 1), ipr = tip(k);
+        real inL[TI_N], inR[TI_N];
+        state_in(k - 1, inL);
+        state_in(k, inR);
+        real rl, pl, ul, vl, cl, rr, pr, ur, vr, cr;
+        trace_left(TAB(T_C0, ipl), dt, TAB(T_IDX, ipl), inL[0], inL[1], inL[2], inL[3], inL[4], inL[5], inL[6],
+                   inL[7], inL[8], inL[9], inL[10], inL[11], rl, pl, ul, vl, cl);
+        trace_right(TAB(T_C0, ipr), dt, TAB(T_IDX, ipr), inR[0], inR[1], inR[2], inR[3], inR[4], inR[5], inR[6],
+                    inR[7], inR[8], inR[9], inR[10], inR[11], rr, pr, ur, vr, cr);
+        real Fr, Fu, FL, pres;
+        hllc_iso(cl, cr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, FL, pres);
+        const real g = TAB(T_XA, ipr);
+        Fr_[k] = Fr * g; Fu_[k] = Fu * g; FL_[k] = FL * g; pr_[k] = pres;
+    }
+    // ---- reverse of the update (cells [3, ns-3)) ----
+    for (int s = t; s < S; s += TX) {
+        ap_[s] = real(0); aL_[s] = real(0);
+        for (int f = 0; f < 4; ++f) { asl_[f][s] = real(0); asr_[f][s] = real(0); }
+    }
+    __syncthreads();
+    for (int s = t; s < S; s += TX) {
+        real lr = 0, lu = 0, lvp = 0, cA = 0, cB = 0, cC = 0, cP = 0;
+        if (s >= NPAD && s < ns - NPAD && inside(s)) {
+            const int g = x0 - 2 * NPAD + s;
+            const int ip = g + NPAD;
+            const real lrn = A.lo[row + g], lun = A.lo[N + row + g], lvn = A.lo[2 * N + row + g];
+            const real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
+            const real f = dt * TAB(T_IXV, ip);
+            const real Dr = (Fr_[s] - Fr_[s + 1]) * f;
+            const real Du = (Fu_[s] - Fu_[s + 1] + real(0.5) * (g0 + g1) * (pr_[s] - pr_[s + 1])) * f;
+            const real DL = (FL_[s] - FL_[s + 1]) * f;
+            const real r = r_[s], u = u_[s], vp = vp_[s];
+            const real rn = r + Dr;
+            const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
+            if (rn < rmin) {
+                lr = r * real(1.0e-10) > real(1.0e-10) ? lrn * real(1.0e-10) : real(0);
+                lu = lun;
+                lvp = lvn;
+            } else {
+                const real irn = FM<real>::rcp(rn);
+                const real iR = TAB(T_IR, ip), vbg = TAB(T_VBG, ip);
+                const real un = (r * u + Du) * irn;
+                const real vn = (r * vp + (DL * iR - vbg * Dr)) * irn;
+                const real lrt = lrn - (lun * un + lvn * vn) * irn;
+                lr = lrt + (lun * u + lvn * vp) * irn;
+                lu = lun * r * irn;
+                lvp = lvn * r * irn;
+                const real lDr = lrt - lvn * vbg * irn, lDu = lun * irn, lDL = lvn * iR * irn;
+                cA = lDr * f; cB = lDu * f; cC = lDL * f; cP = lDu * real(0.5) * (g0 + g1) * f;
+            }
+        }
+        ar_[s] = lr; au_[s] = lu; avp_[s] = lvp;
+        cA_[s] = cA; cB_[s] = cB; cC_[s] = cC; cP_[s] = cP;
+    }
+    __syncthreads();
+    // ---- reverse of the interface fluxes: even interfaces, then odd ones,
+    //      so that the two cells each touches are never shared in a pass ----
+    for (int par = 0; par < 2; ++par) {
+        for (int k = NPAD + par + 2 * t; k < ns - 2; k += 2 * TX) {
+            const real lFs = cA_[k] - cA_[k - 1], lUs = cB_[k] - cB_[k - 1];
+            const real lLs = cC_[k] - cC_[k - 1], lpres = cP_[k] - cP_[k - 1];
+            if (lFs == real(0) && lUs == real(0) && lLs == real(0) && lpres == real(0)) continue;
+            const int ipl = tip(k - 1), ipr = tip(k);
+            const real g = TAB(T_XA, ipr);
+            real inL[TI_N], inR[TI_N];
+            state_in(k - 1, inL);
+            state_in(k, inR);
+            const real cL0 = TAB(T_C0, ipl), cR0 = TAB(T_C0, ipr), idxL = TAB(T_IDX, ipl), idxR = TAB(T_IDX, ipr);
+            real S8[8], cl, cr;
+            trace_left(cL0, dt, idxL, inL[0], inL[1], inL[2], inL[3], inL[4], inL[5], inL[6], inL[7], inL[8],
+                       inL[9], inL[10], inL[11], S8[0], S8[1], S8[2], S8[3], cl);
+            trace_right(cR0, dt, idxR, inR[0], inR[1], inR[2], inR[3], inR[4], inR[5], inR[6], inR[7], inR[8],
+                        inR[9], inR[10], inR[11], S8[4], S8[5], S8[6], S8[7], cr);
+            real lS8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lcl = 0, lcr = 0;
+            hllc_iso_adj(cl, cr, S8, lFs * g, lUs * g, lLs * g, lpres, lS8, lcl, lcr);
+            real lL[TI_N], lR[TI_N];
+            for (int q = 0; q < TI_N; ++q) { lL[q] = real(0); lR[q] = real(0); }
+            trace_left_adj(cL0, dt, idxL, inL, lS8[0], lS8[1], lS8[2], lS8[3], lcl, lL);
+            trace_right_adj(cR0, dt, idxR, inR, lS8[4], lS8[5], lS8[6], lS8[7], lcr, lR);
+            const int m = k - 1;
+            ar_[m] += lL[TI_R]; ap_[m] += lL[TI_P]; au_[m] += lL[TI_U]; aL_[m] += lL[TI_V];
+            asl_[0][m] += lL[TI_RSL]; asr_[0][m] += lL[TI_RSR]; asl_[1][m] += lL[TI_PSL]; asr_[1][m] += lL[TI_PSR];
+            asl_[2][m] += lL[TI_USL]; asr_[2][m] += lL[TI_USR]; asl_[3][m] += lL[TI_VSL]; asr_[3][m] += lL[TI_VSR];
+            ar_[k] += lR[TI_R]; ap_[k] += lR[TI_P]; au_[k] += lR[TI_U]; aL_[k] += lR[TI_V];
+            asl_[0][k] += lR[TI_RSL]; asr_[0][k] += lR[TI_RSR]; asl_[1][k] += lR[TI_PSL]; asr_[1][k] += lR[TI_PSR];
+            asl_[2][k] += lR[TI_USL]; asr_[2][k] += lR[TI_USR]; asl_[3][k] += lR[TI_VSL]; asr_[3][k] += lR[TI_VSR];
+        }
+        __syncthreads();
+    }
+    // ---- reverse of the parabolas: adjoints of the neighbour differences ----
+    for (int s = t; s < S; s += TX) {
+        real lflat = real(0);
+        real ldf[4][4];
+        for (int f = 0; f < 4; ++f)
+            for (int q = 0; q < 4; ++q) ldf[f][q] = real(0);
+        if (s >= 2 && s < ns - 2) {
+            const int ip = tip(s);
+            const real w0 = TAB(T_WL0, ip), w1 = TAB(T_WL1, ip), w2 = TAB(T_WL2, ip);
+            const real v0 = TAB(T_WR0, ip), v1 = TAB(T_WR1, ip), v2 = TAB(T_WR2, ip);
+            const real* f4[4] = {r_, p_, u_, L_};
+            for (int f = 0; f < 4; ++f) {
+                if (asl_[f][s] == real(0) && asr_[f][s] == real(0)) continue;
+                const real* a = f4[f];
+                slopes_adj(a[s - 2], a[s - 1], a[s], a[s + 1], a[s + 2], w0, w1, w2, v0, v1, v2, fl_[s],
+                           asl_[f][s], asr_[f][s], ldf[f], lflat);
+            }
+        }
+        for (int f = 0; f < 4; ++f)
+            for (int q = 0; q < 4; ++q) ld_[f][q][s] = ldf[f][q];
+        lfl_[s] = lflat;
+    }
+    __syncthreads();
+    // gather the differences into the cells; flattening adjoint per cell
+    for (int s = t; s < S; s += TX) {
+        real add[4];
+        for (int f = 0; f < 4; ++f) {
+            real v = real(0);
+            if (s + 2 < S) v -= ld_[f][0][s + 2];
+            if (s + 1 < S) v += ld_[f][0][s + 1] - ld_[f][1][s + 1];
+            v += ld_[f][1][s] - ld_[f][2][s];
+            if (s >= 1) v += ld_[f][2][s - 1] - ld_[f][3][s - 1];
+            if (s >= 2) v += ld_[f][3][s - 2];
+            add[f] = v;
+        }
+        ar_[s] += add[0]; ap_[s] += add[1]; au_[s] += add[2]; aL_[s] += add[3];
+        real lp5[5] = {0, 0, 0, 0, 0};
+        if (s >= 2 && s < ns - 2 && lfl_[s] != real(0))
+            flatten5_adj(p_[s - 2], p_[s - 1], p_[s], p_[s + 1], p_[s + 2], u_[s - 1], u_[s], u_[s + 1], lfl_[s], lp5);
+        for (int q = 0; q < 5; ++q) lfp_[q][s] = lp5[q];
+    }
+    __syncthreads();
+    // ---- gather the flattening adjoints, chain p = cs2 rho and L = R (vbg + vp), store ----
+    double lalpha = 0.0;
+    for (int s = t; s < ns; s += TX) {
+        real lp = ap_[s];
+        if (s + 2 < S) lp += lfp_[0][s + 2];
+        if (s + 1 < S) lp += lfp_[1][s + 1];
+        lp += lfp_[2][s];
+        if (s >= 1) lp += lfp_[3][s - 1];
+        if (s >= 2) lp += lfp_[4][s - 2];
+        const int g = x0 - 2 * NPAD + s;
+        const int ip = tip(s);
+        if (s >= 2 * NPAD && s < 2 * NPAD + nt) {
+            const size_t o = row + g;
+            put(A.li, o, ar_[s] + TAB(T_CS2, ip) * lp, A.accumulate);
+            put(A.li, N + o, au_[s], A.accumulate);
+            put(A.li, 2 * N + o, avp_[s] + TAB(T_XC, ip) * aL_[s], A.accumulate);
+        } else if ((g < 0 && g >= -NPAD) || (g >= nx && g < nx + NPAD)) {
+            // radial ghost: u = -1.5 nu(R)/R is proportional to alpha
+            lalpha += (double)au_[s] * (double)TAB(T_G_U, ip);
+        }
+    }
+    if (A.alpha > 0.0) block_sum_add(A.lparam + 1, lalpha / A.alpha);
+}
+
+// ------------------------------------------------- azimuthal sweep adjoint
+// Tile of TR radii x TY output rows, no shift (the pass SA = sweep(S2));
+// LDS row k <-> row y0-6+k, rows [0, TY+12).
+template <typename real> struct AdjPhiTile { static constexpr int TR = sizeof(real) == 8 ? 8 : 16, TY = 16; };
+
+template <typename real>
+__global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k_adj_phi(AdjArgs<real> A) {
+    constexpr int TR = AdjPhiTile<real>::TR, TY = AdjPhiTile<real>::TY, B = TR * TY;
+    constexpr int NK = TY + 4 * NPAD, E = NK * TR;
+    __shared__ real c0[E], c1[E], c2[E], fl[E];
+    __shared__ real sl[3][E], sr[3][E];
+    __shared__ real F0[E], F1[E], F2[E], F3[E];
+    __shared__ real a0[E], a1[E], a2[E];
+    __shared__ real asl[3][E], asr[3][E];
+    __shared__ real cA[E], cB[E], cC[E];
+    __shared__ real ld[3][4][E];
+    __shared__ real lfl[E];
+    __shared__ real lfp[5][E];
+    const DiskArgs<real>& d = *A.d;
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int t = threadIdx.x;
+    const int ri = t % TR;
+    const int i = blockIdx.x * TR + ri;
+    const bool live = i < nx;
+    const int ic = live ? i : nx - 1;
+    const int ip = ic + NPAD;
+    const int y0 = blockIdx.y * TY;
+    const real iR = TAB(T_IR, ip);
+    const real dt = (real)d.ctl->dt;
+    const real dts = dt * iR, idy = d.idy, fup = dt * idy * iR;
+    const real cs2 = TAB(T_CS2, ip), cs0 = TAB(T_C0, ip), ic0 = TAB(T_IC0, ip);
+    const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
+    const int jbase = y0 - 2 * NPAD;
+    for (int e = t; e < E; e += B) {
+        real r = real(1), vp = real(0), u = real(0);
+        if (live) {
+            const size_t o = (size_t)wrap(jbase + e / TR, ny) * nx + i;
+            r = A.s[o]; vp = A.s[2 * N + o]; u = A.s[N + o];
+        }
+        c0[e] = r; c1[e] = vp; c2[e] = u;
+    }
+    __syncthreads();
+    for (int e = t + 2 * TR; e < (NK - 2) * TR; e += B)
+        fl[e] = flatten5(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR], c1[e - TR], c1[e], c1[e + TR], cs0);
+    __syncthreads();
+    real* cf[3] = {c0, c1, c2};
+    for (int e = t + 2 * TR; e < (NK - 2) * TR; e += B) {
+        for (int f = 0; f < 3; ++f) {
+            const real* a = cf[f];
+            real l, r;
+            slopes(a[e - 2 * TR], a[e - TR], a[e], a[e + TR], a[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], fl[e], l, r);
+            sl[f][e] = l; sr[f][e] = r;
+        }
+    }
+    __syncthreads();
+    auto state_in = [&](int e, real* in) {
+        in[TC_R] = c0[e]; in[TC_U] = c1[e]; in[TC_RSL] = sl[0][e]; in[TC_RSR] = sr[0][e];
+        in[TC_USL] = sl[1][e]; in[TC_USR] = sr[1][e]; in[TC_VSL] = sl[2][e]; in[TC_V] = c2[e]; in[TC_VSR] = sr[2][e];
+    };
+    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
+        real inL[TC_N], inR[TC_N];
+        state_in(e - TR, inL);
+        state_in(e, inR);
+        real rl, ul, vl, rr, ur, vr;
+        trace_left_c(cs0, ic0, dts, idy, inL[0], inL[1], inL[2], inL[3], inL[4], inL[5], inL[6], inL[7], inL[8], rl, ul, vl);
+        trace_right_c(cs0, ic0, dts, idy, inR[0], inR[1], inR[2], inR[3], inR[4], inR[5], inR[6], inR[7], inR[8], rr, ur, vr);
+        real Fr, Fu, Fw, pres;
+        hllc_c(cs0, ic0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
+        F0[e] = Fr; F1[e] = Fu; F2[e] = Fw; F3[e] = pres;
+    }
+    for (int e = t; e < E; e += B) {
+        for (int f = 0; f < 3; ++f) { asl[f][e] = real(0); asr[f][e] = real(0); }
+    }
+    __syncthreads();
+    // reverse of the update, cells [3, NK-3)
+    for (int e = t; e < E; e += B) {
+        const int k = e / TR;
+        real l0 = 0, l1 = 0, l2 = 0, ca = 0, cb = 0, cc = 0;
+        if (live && k >= NPAD && k < NK - NPAD) {
+            const size_t o = (size_t)wrap(jbase + k, ny) * nx + i;
+            const real lrn = A.lo[o], lun = A.lo[N + o], lvn = A.lo[2 * N + o];
+            const real Dr = (F0[e] - F0[e + TR]) * fup;
+            const real Du = (F1[e] - F1[e + TR] + (F3[e] - F3[e + TR])) * fup;
+            const real Dw = (F2[e] - F2[e + TR]) * fup;
+            const real r = c0[e], vp = c1[e], u = c2[e];
+            const real rn = r + Dr;
+            const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
+            if (rn < rmin) {
+                l0 = r * real(1.0e-10) > real(1.0e-10) ? lrn * real(1.0e-10) : real(0);
+                l1 = lvn;
+                l2 = lun;
+            } else {
+                const real irn = FM<real>::rcp(rn);
+                const real vn = (r * vp + Du) * irn, un = (r * u + Dw) * irn;
+                const real lrt = lrn - (lvn * vn + lun * un) * irn;
+                l0 = lrt + (lvn * vp + lun * u) * irn;
+                l1 = lvn * r * irn;
+                l2 = lun * r * irn;
+                ca = lrt * fup; cb = lvn * irn * fup; cc = lun * irn * fup;
+            }
+        }
+        a0[e] = l0; a1[e] = l1; a2[e] = l2;
+        cA[e] = ca; cB[e] = cb; cC[e] = cc;
+    }
+    __syncthreads();
+    // reverse of the interfaces, even rows then odd rows
+    for (int par = 0; par < 2; ++par) {
+        for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
+            const int k = e / TR;
+            if ((k & 1) != par) continue;
+            const real lFr = cA[e] - cA[e - TR], lFu = cB[e] - cB[e - TR], lFw = cC[e] - cC[e - TR];
+            const real lpres = lFu;   // the pressure difference enters Du like the momentum flux
+            if (lFr == real(0) && lFu == real(0) && lFw == real(0)) continue;
+            real inL[TC_N], inR[TC_N];
+            state_in(e - TR, inL);
+            state_in(e, inR);
+            real Sc[6];
+            trace_left_c(cs0, ic0, dts, idy, inL[0], inL[1], inL[2], inL[3], inL[4], inL[5], inL[6], inL[7], inL[8],
+                         Sc[0], Sc[1], Sc[2]);
+            trace_right_c(cs0, ic0, dts, idy, inR[0], inR[1], inR[2], inR[3], inR[4], inR[5], inR[6], inR[7], inR[8],
+                          Sc[3], Sc[4], Sc[5]);
+            real lSc[6] = {0, 0, 0, 0, 0, 0};
+            hllc_c_adj(cs0, ic0, cs2, Sc, lFr, lFu, lFw, lpres, lSc);
+            real lL[TC_N], lR[TC_N];
+            for (int q = 0; q < TC_N; ++q) { lL[q] = real(0); lR[q] = real(0); }
+            trace_left_c_adj(cs0, ic0, dts, idy, inL, lSc[0], lSc[1], lSc[2], lL);
+            trace_right_c_adj(cs0, ic0, dts, idy, inR, lSc[3], lSc[4], lSc[5], lR);
+            const int m = e - TR;
+            a0[m] += lL[TC_R]; a1[m] += lL[TC_U]; a2[m] += lL[TC_V];
+            asl[0][m] += lL[TC_RSL]; asr[0][m] += lL[TC_RSR]; asl[1][m] += lL[TC_USL]; asr[1][m] += lL[TC_USR];
+            asl[2][m] += lL[TC_VSL]; asr[2][m] += lL[TC_VSR];
+            a0[e] += lR[TC_R]; a1[e] += lR[TC_U]; a2[e] += lR[TC_V];
+            asl[0][e] += lR[TC_RSL]; asr[0][e] += lR[TC_RSR]; asl[1][e] += lR[TC_USL]; asr[1][e] += lR[TC_USR];
+            asl[2][e] += lR[TC_VSL]; asr[2][e] += lR[TC_VSR];
+        }
+        __syncthreads();
+    }
+    // reverse of the parabolas
+    for (int e = t; e < E; e += B) {
+        const int k = e / TR;
+        real lflat = real(0);
+        real ldf[3][4];
+        for (int f = 0; f < 3; ++f)
+            for (int q = 0; q < 4; ++q) ldf[f][q] = real(0);
+        if (k >= 2 && k < NK - 2) {
+            for (int f = 0; f < 3; ++f) {
+                if (asl[f][e] == real(0) && asr[f][e] == real(0)) continue;
+                const real* a = cf[f];
+                slopes_adj(a[e - 2 * TR], a[e - TR], a[e], a[e + TR], a[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4],
+                           wy[5], fl[e], asl[f][e], asr[f][e], ldf[f], lflat);
+            }
+        }
+        for (int f = 0; f < 3; ++f)
+            for (int q = 0; q < 4; ++q) ld[f][q][e] = ldf[f][q];
+        lfl[e] = lflat;
+    }
+    __syncthreads();
+    for (int e = t; e < E; e += B) {
+        const int k = e / TR;
+        real add[3];
+        for (int f = 0; f < 3; ++f) {
+            real v = real(0);
+            if (k + 2 < NK) v -= ld[f][0][e + 2 * TR];
+            if (k + 1 < NK) v += ld[f][0][e + TR] - ld[f][1][e + TR];
+            v += ld[f][1][e] - ld[f][2][e];
+            if (k >= 1) v += ld[f][2][e - TR] - ld[f][3][e - TR];
+            if (k >= 2) v += ld[f][3][e - 2 * TR];
+            add[f] = v;
+        }
+        a0[e] += add[0]; a1[e] += add[1]; a2[e] += add[2];
+        real lp5[5] = {0, 0, 0, 0, 0};
+        if (k >= 2 && k < NK - 2 && lfl[e] != real(0))
+            flatten5_adj(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR], c1[e - TR], c1[e], c1[e + TR],
+                         lfl[e], lp5);
+        for (int q = 0; q < 5; ++q) lfp[q][e] = lp5[q];
+    }
+    __syncthreads();
+    const int rows = min(TY, ny - y0);
+    for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
+        const int k = e / TR;
+        const int y = k - 2 * NPAD;
+        if (!live || y >= rows) continue;
+        real lr = a0[e] + lfp[0][e + 2 * TR] + lfp[1][e + TR] + lfp[2][e] + lfp[3][e - TR] + lfp[4][e - 2 * TR];
+        const size_t o = (size_t)(y0 + y) * nx + i;
+        put(A.li, o, lr, A.accumulate);
+        put(A.li, N + o, a2[e], A.accumulate);
+        put(A.li, 2 * N + o, a1[e], A.accumulate);
+    }
+}
+
+// ----------------------------- orbital shift + residual advection adjoint
+// Forward: S3[y] = advect(SA shifted by inc)[y], i.e. post-shift row y
+// takes pre-shift row y - inc.  Tile of TR radii x TY pre-shift rows y0..;
+// LDS row k <-> pre-shift row y0-6+k <-> post-shift row y0-6+k+inc.
+template <typename real>
+__global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k_adj_adv(AdjArgs<real> A) {
+    constexpr int TR = AdjPhiTile<real>::TR, TY = AdjPhiTile<real>::TY, B = TR * TY;
+    constexpr int NK = TY + 4 * NPAD, E = NK * TR;
+    __shared__ real c0[E], c1[E], c2[E];
+    __shared__ real sl[3][E], sr[3][E];
+    __shared__ real F0[E], F1[E], F2[E];
+    __shared__ real a0[E], a1[E], a2[E];
+    __shared__ real cA[E], cB[E], cC[E];
+    __shared__ real lsl[3][E], lsr[3][E];
+    __shared__ real ld[3][4][E];
+    const DiskArgs<real>& d = *A.d;
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int t = threadIdx.x;
+    const int ri = t % TR;
+    const int i = blockIdx.x * TR + ri;
+    const bool live = i < nx;
+    const int ic = live ? i : nx - 1;
+    const int ip = ic + NPAD;
+    const int y0 = blockIdx.y * TY;
+    const real iR = TAB(T_IR, ip);
+    const real dt = (real)d.ctl->dt;
+    const real dts = dt * iR, idy = d.idy, fup = dt * idy * iR;
+    const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
+    const int inc = d.orbadv ? d.inc[ic] : 0;
+    const real speed = d.res[ic];
+    const real q = lim01(fabs(speed) * dts * idy);
+    const int jbase = y0 - 2 * NPAD;
+    for (int e = t; e < E; e += B) {
+        real r = real(1), vp = real(0), u = real(0);
+        if (live) {
+            const size_t o = (size_t)wrap(jbase + e / TR, ny) * nx + i;
+            r = A.s[o]; vp = A.s[2 * N + o]; u = A.s[N + o];
+        }
+        c0[e] = r; c1[e] = vp; c2[e] = u;
+    }
+    __syncthreads();
+    real* cf[3] = {c0, c1, c2};
+    for (int e = t + 2 * TR; e < (NK - 2) * TR; e += B) {
+        for (int f = 0; f < 3; ++f) {
+            const real* a = cf[f];
+            real l, r;
+            slopes(a[e - 2 * TR], a[e - TR], a[e], a[e + TR], a[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], real(0), l, r);
+            sl[f][e] = l; sr[f][e] = r;
+        }
+    }
+    __syncthreads();
+    // upwind cell of interface e: e-TR when speed > 0, else e
+    const int up = speed > real(0) ? -TR : 0;
+    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
+        const int m = e + up;
+        real Dr, Dv, Du;
+        if (speed > real(0)) {
+            Dr = aveR(q, sl[0][m], c0[m], sr[0][m]);
+            Dv = aveR(q, sl[1][m], c1[m], sr[1][m]) * Dr;
+            Du = aveR(q, sl[2][m], c2[m], sr[2][m]) * Dr;
+        } else {
+            Dr = aveL(q, sl[0][m], c0[m], sr[0][m]);
+            Dv = aveL(q, sl[1][m], c1[m], sr[1][m]) * Dr;
+            Du = aveL(q, sl[2][m], c2[m], sr[2][m]) * Dr;
+        }
+        F0[e] = Dr * speed; F1[e] = Dv * speed; F2[e] = Du * speed;
+    }
+    __syncthreads();
+    // reverse of the update (update_Cell_advect, no floor), cells [3, NK-3)
+    for (int e = t; e < E; e += B) {
+        const int k = e / TR;
+        real l0 = 0, l1 = 0, l2 = 0, ca = 0, cb = 0, cc = 0;
+        if (live && k >= NPAD && k < NK - NPAD) {
+            const size_t o = (size_t)wrap(jbase + k + inc, ny) * nx + i;
+            const real lrn = A.lo[o], lun = A.lo[N + o], lvn = A.lo[2 * N + o];
+            const real Dr = (F0[e] - F0[e + TR]) * fup;
+            const real Dv = (F1[e] - F1[e + TR]) * fup;
+            const real Du = (F2[e] - F2[e + TR]) * fup;
+            const real r = c0[e];
+            const real rn = r + Dr;
+            const real irn = FM<real>::rcp(rn);
+            const real vn = (r * c1[e] + Dv) * irn, un = (r * c2[e] + Du) * irn;
+            const real lrt = lrn - (lvn * vn + lun * un) * irn;
+            l0 = lrt + (lvn * c1[e] + lun * c2[e]) * irn;
+            l1 = lvn * r * irn;
+            l2 = lun * r * irn;
+            ca = lrt * fup; cb = lvn * irn * fup; cc = lun * irn * fup;
+        }
+        a0[e] = l0; a1[e] = l1; a2[e] = l2;
+        cA[e] = ca; cB[e] = cb; cC[e] = cc;
+        for (int f = 0; f < 3; ++f) { lsl[f][e] = real(0); lsr[f][e] = real(0); }
+    }
+    __syncthreads();
+    // reverse of the fluxes: each cell is the upwind cell of exactly one
+    // interface, so the accumulation has no conflicts
+    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
+        const int m = e + up;
+        const real lF0 = (cA[e] - cA[e - TR]) * speed;
+        const real lF1 = (cB[e] - cB[e - TR]) * speed;
+        const real lF2 = (cC[e] - cC[e - TR]) * speed;
+        real A0, A1, A2;
+        if (speed > real(0)) {
+            A0 = aveR(q, sl[0][m], c0[m], sr[0][m]);
+            A1 = aveR(q, sl[1][m], c1[m], sr[1][m]);
+            A2 = aveR(q, sl[2][m], c2[m], sr[2][m]);
+        } else {
+            A0 = aveL(q, sl[0][m], c0[m], sr[0][m]);
+            A1 = aveL(q, sl[1][m], c1[m], sr[1][m]);
+            A2 = aveL(q, sl[2][m], c2[m], sr[2][m]);
+        }
+        // F0 = A0 s, F1 = A1 A0 s, F2 = A2 A0 s
+        const real lA0 = lF0 + lF1 * A1 + lF2 * A2;
+        const real lA1 = lF1 * A0, lA2 = lF2 * A0;
+        real lq = 0, ls[3][2] = {{0, 0}, {0, 0}, {0, 0}}, la[3] = {0, 0, 0};
+        const real lA[3] = {lA0, lA1, lA2};
+        for (int f = 0; f < 3; ++f) {
+            const real* a = cf[f];
+            if (speed > real(0)) aveR_adj(q, sl[f][m], a[m], sr[f][m], lA[f], lq, ls[f][0], la[f], ls[f][1]);
+            else aveL_adj(q, sl[f][m], a[m], sr[f][m], lA[f], lq, ls[f][0], la[f], ls[f][1]);
+        }
+        a0[m] += la[0]; a1[m] += la[1]; a2[m] += la[2];
+        for (int f = 0; f < 3; ++f) { lsl[f][m] = ls[f][0]; lsr[f][m] = ls[f][1]; }
+    }
+    __syncthreads();
+    for (int e = t; e < E; e += B) {
+        const int k = e / TR;
+        real ldf[3][4];
+        for (int f = 0; f < 3; ++f)
+            for (int qq = 0; qq < 4; ++qq) ldf[f][qq] = real(0);
+        if (k >= 2 && k < NK - 2) {
+            real lflat = 0;
+            for (int f = 0; f < 3; ++f) {
+                if (lsl[f][e] == real(0) && lsr[f][e] == real(0)) continue;
+                const real* a = cf[f];
+                slopes_adj(a[e - 2 * TR], a[e - TR], a[e], a[e + TR], a[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4],
+                           wy[5], real(0), lsl[f][e], lsr[f][e], ldf[f], lflat);
+            }
+        }
+        for (int f = 0; f < 3; ++f)
+            for (int qq = 0; qq < 4; ++qq) ld[f][qq][e] = ldf[f][qq];
+    }
+    __syncthreads();
+    const int rows = min(TY, ny - y0);
+    for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
+        const int k = e / TR;
+        const int y = k - 2 * NPAD;
+        if (!live || y >= rows) continue;
+        real add[3];
+        for (int f = 0; f < 3; ++f) {
+            add[f] = -ld[f][0][e + 2 * TR] + ld[f][0][e + TR] - ld[f][1][e + TR] + ld[f][1][e] - ld[f][2][e] +
+                     ld[f][2][e - TR] - ld[f][3][e - TR] + ld[f][3][e - 2 * TR];
+        }
+        const size_t o = (size_t)(y0 + y) * nx + i;
+        put(A.li, o, a0[e] + add[0], A.accumulate);
+        put(A.li, N + o, a2[e] + add[2], A.accumulate);
+        put(A.li, 2 * N + o, a1[e] + add[1], A.accumulate);
+    }
+}
+
+// ------------------------------------------------------ viscosity adjoint
+// Forward: k_viscosity, a 9-point stencil of (rho nu, u, vp).  The partial
+// kernel writes, per output cell, the adjoints it sends to each input of its
+// stencil (slot n: u of neighbour n, 9+n: vp, 18..22: rho nu of neighbours
+// 0,1,2,4,7; neighbour n = 3b + a at radius i-1+a, row j, j-1, j+1 for
+// b = 0, 1, 2); the gather kernel sums them per input cell.
+template <typename real>
+__global__ __launch_bounds__(256) void k_adj_vis_part(AdjArgs<real> A) {
+    const DiskArgs<real>& d = *A.d;
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    double lal = 0.0;
+    if (i < nx) {
+        const int ip = i + NPAD;
+        real nu[3];
+        for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
+        real nr[9], u[9], v[9];
+        const int rows[3] = {j, wrap(j - 1, ny), wrap(j + 1, ny)};
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+                const int n = 3 * b + a, ii = i - 1 + a;
+                real rr, uu, vv;
+                if (ii < 0 || ii >= nx) {
+                    rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
+                } else {
+                    const size_t o = (size_t)rows[b] * nx + ii;
+                    rr = A.s[o]; uu = A.s[N + o]; vv = A.s[2 * N + o];
+                }
+                nr[n] = rr * nu[a]; u[n] = uu; v[n] = vv;
+            }
+        const real i20 = TAB(T_I20, ip), i21 = TAB(T_I21, ip), i10 = TAB(T_I10, ip);
+        const real ir = TAB(T_IR, ip), idy = d.idy, hdy = real(0.5) * idy;
+        const real dudx = (u[2] - u[0]) * i20;
+        const real d2udx2 = ((u[2] - u[1]) * i21 - (u[1] - u[0]) * i10) * (real(2) * i20);
+        const real dvdx = TAB(T_DVBG, ip) + (v[2] - v[0]) * i20;
+        const real d2vdx2 = TAB(T_D2VBG, ip) + ((v[2] - v[1]) * i21 - (v[1] - v[0]) * i10) * (real(2) * i20);
+        const real dndx = (nr[2] - nr[0]) * i20;
+        const real ky = hdy * ir, kyy = (idy * idy) * ir * ir, kxy = i20 * hdy * ir;
+        const real dvdy = (v[7] - v[4]) * ky;
+        const real d2vdy2 = (v[7] - real(2) * v[1] + v[4]) * kyy;
+        const real d2vdxy = (v[8] - v[6] - v[5] + v[3]) * kxy;
+        const real dudy = (u[7] - u[4]) * ky;
+        const real d2udy2 = (u[7] - real(2) * u[1] + u[4]) * kyy;
+        const real d2udxy = (u[8] - u[6] - u[5] + u[3]) * kxy;
+        const real dndy = (nr[7] - nr[4]) * ky;
+        const real v1 = TAB(T_VBG, ip) + v[1];
+        const real n1 = nr[1];
+        const size_t o = (size_t)j * nx + i;
+        const real X = A.lvis[o], Y = A.lvis[N + o];
+        // forward values (for the alpha adjoint: the forces are linear in nu)
+        {
+            real sxx = real(2) * n1 * dudx;
+            real fx = real(2) * dndx * dudx + real(2) * n1 * d2udx2 + sxx * ir;
+            real syy = real(2) * n1 * (dvdy + u[1] * ir);
+            real sxy = n1 * (dudy + dvdx - v1 * ir);
+            real fy = real(2) * dndy * dvdy + real(2) * n1 * d2vdy2 + real(2) * n1 * dudy * ir + real(2) * dndy * u[1] * ir;
+            fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 * ir - n1 * dvdy * ir - syy * ir;
+            fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 * ir - n1 * dvdx * ir + n1 * v1 * ir * ir +
+                  real(2) * sxy * ir;
+            lal = (double)X * (double)fx + (double)Y * (double)fy;
+        }
+        // adjoints of the derivative terms
+        const real l_dndx = X * (real(2) * dudx) + Y * ((dudy + dvdx) - v1 * ir);
+        const real l_dudx = X * (real(2) * dndx + real(2) * n1 * ir);
+        const real l_n1 = X * (real(2) * d2udx2 + real(2) * dudx * ir + (d2udy2 + d2vdxy) - dvdy * ir -
+                               real(2) * (dvdy + u[1] * ir) * ir) +
+                          Y * (real(2) * d2vdy2 + real(2) * dudy * ir + (d2udxy + d2vdx2) - dvdx * ir + v1 * ir * ir +
+                               real(2) * (dudy + dvdx - v1 * ir) * ir);
+        const real l_d2udx2 = X * real(2) * n1;
+        const real l_dndy = X * ((dudy + dvdx) - v1 * ir) + Y * (real(2) * dvdy + real(2) * u[1] * ir);
+        const real l_dudy = X * dndy + Y * (real(4) * n1 * ir + dndx);
+        const real l_dvdx = X * dndy + Y * (dndx + n1 * ir);
+        const real l_d2udy2 = X * n1, l_d2vdxy = X * n1;
+        const real l_v1 = X * (-dndy * ir) + Y * (-dndx * ir - n1 * ir * ir);
+        const real l_dvdy = X * (real(-3) * n1 * ir) + Y * (real(2) * dndy);
+        const real l_u1 = X * (real(-2) * n1 * ir * ir) + Y * (real(2) * dndy * ir);
+        const real l_d2vdy2 = Y * real(2) * n1, l_d2udxy = Y * n1, l_d2vdx2 = Y * n1;
+        real lu[9], lv[9], ln[9];
+        for (int n = 0; n < 9; ++n) { lu[n] = real(0); lv[n] = real(0); ln[n] = real(0); }
+        lu[2] += l_dudx * i20; lu[0] -= l_dudx * i20;
+        {
+            const real c = real(2) * i20 * l_d2udx2;
+            lu[2] += c * i21; lu[1] -= c * (i21 + i10); lu[0] += c * i10;
+        }
+        lv[2] += l_dvdx * i20; lv[0] -= l_dvdx * i20;
+        {
+            const real c = real(2) * i20 * l_d2vdx2;
+            lv[2] += c * i21; lv[1] -= c * (i21 + i10); lv[0] += c * i10;
+        }
+        ln[2] += l_dndx * i20; ln[0] -= l_dndx * i20;
+        lv[7] += l_dvdy * ky; lv[4] -= l_dvdy * ky;
+        lv[7] += l_d2vdy2 * kyy; lv[1] -= real(2) * l_d2vdy2 * kyy; lv[4] += l_d2vdy2 * kyy;
+        lv[8] += l_d2vdxy * kxy; lv[6] -= l_d2vdxy * kxy; lv[5] -= l_d2vdxy * kxy; lv[3] += l_d2vdxy * kxy;
+        lu[7] += l_dudy * ky; lu[4] -= l_dudy * ky;
+        lu[7] += l_d2udy2 * kyy; lu[1] -= real(2) * l_d2udy2 * kyy; lu[4] += l_d2udy2 * kyy;
+        lu[8] += l_d2udxy * kxy; lu[6] -= l_d2udxy * kxy; lu[5] -= l_d2udxy * kxy; lu[3] += l_d2udxy * kxy;
+        ln[7] += l_dndy * ky; ln[4] -= l_dndy * ky;
+        lv[1] += l_v1;
+        ln[1] += l_n1;
+        lu[1] += l_u1;
+        for (int n = 0; n < 9; ++n) {
+            A.part[(size_t)n * N + o] = lu[n];
+            A.part[(size_t)(9 + n) * N + o] = lv[n];
+        }
+        const int rn[5] = {0, 1, 2, 4, 7};
+        for (int k = 0; k < 5; ++k) A.part[(size_t)(18 + k) * N + o] = ln[rn[k]] * nu[rn[k] % 3];
+    }
+    if (A.alpha > 0.0) block_sum_add(A.lparam + 1, lal / A.alpha);
+}
+
+// Sum of the partials each input cell received; columns -1 and nx are the
+// radial ghosts, whose u = -1.5 nu(R)/R makes them alpha-dependent.
+template <typename real>
+__global__ __launch_bounds__(256) void k_adj_vis_gather(AdjArgs<real> A) {
+    const DiskArgs<real>& d = *A.d;
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int i = blockIdx.x * 256 + threadIdx.x - 1;   // -1 .. nx
+    const int j = blockIdx.y;
+    double lal = 0.0;
+    if (i <= nx) {
+        real lr = 0, lu = 0, lv = 0;
+        const int db[3] = {0, -1, 1};
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+                const int n = 3 * b + a;
+                const int io = i - (a - 1);
+                if (io < 0 || io >= nx) continue;
+                const int jo = wrap(j - db[b], ny);
+                const size_t o = (size_t)jo * nx + io;
+                lu += A.part[(size_t)n * N + o];
+                lv += A.part[(size_t)(9 + n) * N + o];
+                int slot = -1;
+                if (n == 0) slot = 18;
+                else if (n == 1) slot = 19;
+                else if (n == 2) slot = 20;
+                else if (n == 4) slot = 21;
+                else if (n == 7) slot = 22;
+                if (slot >= 0) lr += A.part[(size_t)slot * N + o];
+            }
+        if (i >= 0 && i < nx) {
+            const size_t o = (size_t)j * nx + i;
+            put(A.li, o, lr, A.accumulate & 1);
+            put(A.li, N + o, lu, A.accumulate & 1);
+            put(A.li, 2 * N + o, lv, A.accumulate & 1);
+        } else {
+            lal = (double)lu * (double)TAB(T_G_U, i + NPAD);
+        }
+    }
+    if (A.alpha > 0.0) block_sum_add(A.lparam + 1, lal / A.alpha);
+}
+
+// ---------------------------------------------------- source-term adjoint
+// Forward: k_sources for gas without dust or wave killing, h = dt/2:
+//   mode 0:  u' = u + (ebal + vp (2 vbg + vp)/R + px + Vr/rho) h,
+//            vp' = vp + (Vphi/rho + py) h
+//   mode 1/2: the same with vp in the centrifugal term replaced by the
+//            predicted W = vp + (Vphi/rho + py) h.
+// accumulate: bit 0 -> li +=, bit 1 -> lvis +=.
+template <typename real>
+__global__ __launch_bounds__(256) void k_adj_src(AdjArgs<real> A) {
+    const DiskArgs<real>& d = *A.d;
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    double lq = 0.0;
+    if (i < nx) {
+        const int ip = i + NPAD;
+        const size_t o = (size_t)j * nx + i;
+        const int mode = A.mode;
+        const real h = (real)(0.5 * d.ctl->dt);
+        const PlanetK& pk = mode == 0 ? d.ctl->pk0 : d.ctl->pk1;
+        const real iR = TAB(T_IR, ip), vbg = TAB(T_VBG, ip);
+        real px = real(0), py = real(0);
+        if (d.n_planet > 0) planet_acc(pk, d.phitab + 4 * j, TAB(T_XC, ip), px, py);
+        const real rho = A.s[o], cvp = A.s[2 * N + o];
+        const real irho = FM<real>::rcp(rho);
+        real Vr = 0, Vp = 0;
+        if (d.visc) { Vr = A.vis[o]; Vp = A.vis[N + o]; }
+        const real vy = Vp * irho;
+        const real W = mode == 0 ? cvp : cvp + (vy + py) * h;
+        const real lnr = A.lo[o], lnu = A.lo[N + o], lnv = A.lo[2 * N + o];
+        const real lW = lnu * h * (real(2) * vbg + real(2) * W) * iR;
+        const real lpx = lnu * h, lvx = lnu * h;
+        real lcvp = lnv + lW;
+        real lvy = lnv * h, lpy = lnv * h;
+        if (mode != 0) { lvy += lW * h; lpy += lW * h; }
+        real lrho = lnr;
+        real lVr = 0, lVp = 0;
+        if (d.visc) {
+            lVr = lvx * irho;
+            lVp = lvy * irho;
+            lrho -= (lvx * Vr + lvy * Vp) * irho * irho;
+        }
+        put(A.li, o, lrho, A.accumulate & 1);
+        put(A.li, N + o, lnu, A.accumulate & 1);
+        put(A.li, 2 * N + o, lcvp, A.accumulate & 1);
+        if (d.visc) {
+            put(A.lvis, o, lVr, A.accumulate & 2);
+            put(A.lvis, N + o, lVp, A.accumulate & 2);
+        }
+        lq = (double)lpx * (double)px + (double)lpy * (double)py;
+    }
+    if (A.q > 0.0) block_sum_add(A.lparam, lq / A.q);
+}
+
+// ------------------------------------------------------------- launchers
+template <typename real>
+struct AdjLaunch {
+    static constexpr int TXA = 128;
+    static hipError_t x(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
+        hipLaunchKernelGGL((k_adj_x<real, TXA>), dim3((nx + TXA - 1) / TXA, ny), dim3(TXA), 0, s, A);
+        return hipGetLastError();
+    }
+    static hipError_t phi(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
+        constexpr int TR = AdjPhiTile<real>::TR, TY = AdjPhiTile<real>::TY;
+        hipLaunchKernelGGL(k_adj_phi<real>, dim3((nx + TR - 1) / TR, (ny + TY - 1) / TY), dim3(TR * TY), 0, s, A);
+        return hipGetLastError();
+    }
+    static hipError_t adv(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
+        constexpr int TR = AdjPhiTile<real>::TR, TY = AdjPhiTile<real>::TY;
+        hipLaunchKernelGGL(k_adj_adv<real>, dim3((nx + TR - 1) / TR, (ny + TY - 1) / TY), dim3(TR * TY), 0, s, A);
+        return hipGetLastError();
+    }
+    static hipError_t vis(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
+        hipLaunchKernelGGL(k_adj_vis_part<real>, dim3((nx + 255) / 256, ny), dim3(256), 0, s, A);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_adj_vis_gather<real>, dim3((nx + 2 + 255) / 256, ny), dim3(256), 0, s, A);
+        return hipGetLastError();
+    }
+    static hipError_t src(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
+        hipLaunchKernelGGL(k_adj_src<real>, dim3((nx + 255) / 256, ny), dim3(256), 0, s, A);
+        return hipGetLastError();
+    }
+};
+
+template struct AdjLaunch<float>;
+template struct AdjLaunch<double>;
+
+}  // namespace pg

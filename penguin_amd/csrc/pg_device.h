@@ -1,0 +1,390 @@
+// pg_device.h -- per-cell arithmetic of the Godunov step shared by the
+// forward kernels (pg_kernels.hip) and their adjoints (pg_adjoint.hip).
+//
+// The pure functions are __host__ __device__ so that the host can evaluate
+// them in FP64 (local derivative checks); FP32 uses the hardware
+// approximations of FM<float> and exists on the device only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pg_internal.h"
+
+#define PG_HD __host__ __device__ __forceinline__
+
+namespace pg {
+
+// A launch's parity word: bit 0 selects the current gas buffer of every disk
+// in the set, bit 1 the current dust buffer; a kernel that writes a fluid
+// writes the other buffer of its pair.
+__device__ __forceinline__ int gsel(int par) { return par & 1; }
+__device__ __forceinline__ int dsel(int par) { return (par >> 1) & 1; }
+
+
+// per-radius tables, field-major: lanes that differ in radius read
+// consecutive addresses
+#define TAB(f, ip) d.tab[(size_t)(f) * d.tab_stride + (ip)]
+
+// Division, reciprocal, square root and exponential.  FP32: the hardware
+// approximations (v_rcp_f32, v_sqrt_f32, v_exp_f32; about 1 ulp); FP64: IEEE.
+template <typename real> struct FM;
+template <> struct FM<float> {
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float exp(float x) { return __expf(x); }
+};
+template <> struct FM<double> {
+    static PG_HD double rcp(double x) { return 1.0 / x; }
+    static PG_HD double div(double a, double b) { return a / b; }
+    static PG_HD double sqrt(double x) { return ::sqrt(x); }
+    static PG_HD double exp(double x) { return ::exp(x); }
+};
+
+// ---------------------------------------------------------------- helpers
+template <typename real>
+PG_HD real lim01(real a) {  // solve.cu:14-17
+    return fmin(fmax(a, real(0)), real(1));
+}
+template <typename real>
+PG_HD real exp_lim(real x) {  // solve.cu:29-34
+    return fmax(real(1) + x, real(1.0e-6));
+}
+
+// 4th-order slopes from precomputed geometric weights (the linear map
+// get_4th_parameters applies to the four neighbouring differences,
+// recon.cu:216-357), limiter (recon.cu:361-376), flattening and the PPM
+// monotonicity correction (PPM.cu:64-91).
+template <typename real>
+PG_HD void slopes(real am2, real am1, real a0, real ap1, real ap2,
+                                       real wl0, real wl1, real wl2, real wr0, real wr1, real wr2,
+                                       real flat, real& sL, real& sR) {
+    real d0 = am1 - am2, d1 = a0 - am1, d2 = ap1 - a0, d3 = ap2 - ap1;
+    real l = wl0 * d0 + wl1 * d1 + wl2 * d2;
+    real r = wr0 * d1 + wr1 * d2 + wr2 * d3;
+    l = copysign(fmin(fabs(l), fabs(d1)), d1);
+    r = copysign(fmin(fabs(r), fabs(d2)), d2);
+    if (r * l <= real(0)) {
+        l = real(0);
+        r = real(0);
+    }
+    real fk = real(1) - flat;
+    r *= fk;
+    l *= fk;
+    real t3 = r + l;
+    real t2 = t3 * t3;
+    real t1 = t3 * real(3) * (l - r);
+    if (t2 < t1) l = real(2) * r;
+    else if (t2 < -t1) r = real(2) * l;
+    sL = l;
+    sR = r;
+}
+
+// Mean of the cell parabola over its first fraction q (get_PPM_aveL,
+// PPM.cu:106-113) and over its last fraction x (get_PPM_aveR with x = 1-q,
+// PPM.cu:95-104).
+template <typename real>
+PG_HD real aveL(real q, real sL, real a, real sR) {
+    real a6 = real(3) * (sL - sR);
+    return (a - sL) + (q * real(0.5)) * ((sR + sL) + (real(1) - q * real(2.0 / 3.0)) * a6);
+}
+template <typename real>
+PG_HD real aveR(real x, real sL, real a, real sR) {
+    real a6 = real(3) * (sL - sR);
+    return (a + sR) - (x * real(0.5)) * ((sR + sL) - (real(1) - x * real(2.0 / 3.0)) * a6);
+}
+
+// Flattening coefficient of the centre cell of a 5-cell window
+// (flatten, PPM.cu:7-62).  cs is the cell's sound speed.
+template <typename real>
+PG_HD real flatten5(real pm2, real pm1, real p0, real pp1, real pp2,
+                                         real um1, real u0, real up1, real cs) {
+    real f = real(0);
+    real dp1f = pp1 - p0, dp2f = pp2 - pm1, dp1b = p0 - pm1, dp2b = pp1 - pm2;
+    // Mf > 0 || Mb > 0 with Mf = (u0-up1)/cs, Mb = (um1-u0)/cs and cs > 0
+    (void)cs;
+    if (u0 - up1 > real(0) || um1 - u0 > real(0)) {
+        if (dp2f != real(0) && dp2b != real(0)) {
+            real ff = real(10) * fmax(real(0), fabs(FM<real>::div(dp1f, dp2f) - real(0.5)) - real(0.25));
+            real fb = real(10) * fmax(real(0), fabs(FM<real>::div(dp1b, dp2b) - real(0.5)) - real(0.25));
+            f = fmax(ff, fb);
+            f = fmin(real(0.5), FM<real>::div(f * fmax(fabs(dp1f), fabs(dp1b)), p0));
+        }
+    }
+    return f;
+}
+
+// Characteristic-traced half-step state on the right of an interface from
+// the cell right of it (set_state, states.cu:151-206), locally isothermal
+// (EOS_flag 0 fixes gam = 1, parameters.h:118-119, so exp_lim(gam*x) =
+// exp_lim(x) and the sound speed of the traced state is sqrt(p0/r0)).
+// c: the cell's sound speed; idx = 1/dx.  Returns the state and its sound
+// speed cout (= the one HLLC recomputes from pl/rl, wave_speeds.cu:290).
+template <typename real>
+PG_HD void trace_right(real c, real dt, real idx, real r, real p, real u,
+                                            real rsL, real rsR, real psL, real psR, real usL, real usR,
+                                            real vsL, real v, real vsR,
+                                            real& rr, real& pr, real& ur, real& vr, real& cout) {
+    real q = lim01((c - fmin(u, real(0))) * dt * idx);
+    real ua = aveL(q, usL, u, usR);
+    real dis = fmin(ua, real(0)) * dt;
+    q = lim01(-dis * idx);
+    real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
+    real p0 = fmax(aveL(q, psL, p, psR), p * real(1.0e-10));
+    real u0 = aveL(q, usL, u, usR);
+    real ir0 = FM<real>::rcp(r0);
+    real cs = FM<real>::sqrt(p0 * ir0);
+    q = lim01((cs * dt - dis) * idx);
+    real u_ = aveL(q, usL, u, usR);
+    real p_ = aveL(q, psL, p, psR);
+    real ics = FM<real>::rcp(cs);
+    real e = exp_lim((u0 - u_) * ics);
+    rr = r0 * e;
+    pr = p0 * e;
+    ur = u0 + (p0 - p_) * ics * ir0;
+    vr = aveL(q, vsL, v, vsR);
+    cout = cs;
+}
+
+// ... and on the left from the cell left of it (states.cu:208-264).
+template <typename real>
+PG_HD void trace_left(real c, real dt, real idx, real r, real p, real u,
+                                           real rsL, real rsR, real psL, real psR, real usL, real usR,
+                                           real vsL, real v, real vsR,
+                                           real& rl, real& pl, real& ul, real& vl, real& cout) {
+    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
+    real ua = aveR(x, usL, u, usR);
+    real dis = fmax(ua, real(0)) * dt;
+    x = lim01(dis * idx);
+    real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
+    real p0 = fmax(aveR(x, psL, p, psR), p * real(1.0e-10));
+    real u0 = aveR(x, usL, u, usR);
+    real ir0 = FM<real>::rcp(r0);
+    real cs = FM<real>::sqrt(p0 * ir0);
+    x = lim01((dis + cs * dt) * idx);
+    real u_ = aveR(x, usL, u, usR);
+    real p_ = aveR(x, psL, p, psR);
+    real ics = FM<real>::rcp(cs);
+    real e = exp_lim((u_ - u0) * ics);
+    rl = r0 * e;
+    pl = p0 * e;
+    ul = u0 + (p_ - p0) * ics * ir0;
+    vl = aveR(x, vsL, v, vsR);
+    cout = cs;
+}
+
+// The same with one sound speed c for every cell of the row (p = c^2 rho
+// with c fixed: the azimuthal sweep of a locally isothermal disk); pressure
+// is c^2 times density throughout, so only rho, u, v are traced.
+template <typename real>
+PG_HD void trace_right_c(real c, real ic, real dt, real idx, real r, real u,
+                                              real rsL, real rsR, real usL, real usR,
+                                              real vsL, real v, real vsR,
+                                              real& rr, real& ur, real& vr) {
+    real q = lim01((c - fmin(u, real(0))) * dt * idx);
+    real ua = aveL(q, usL, u, usR);
+    real dis = fmin(ua, real(0)) * dt;
+    q = lim01(-dis * idx);
+    real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
+    real u0 = aveL(q, usL, u, usR);
+    q = lim01((c * dt - dis) * idx);
+    real u_ = aveL(q, usL, u, usR);
+    real ra = aveL(q, rsL, r, rsR);
+    rr = r0 * exp_lim((u0 - u_) * ic);
+    ur = u0 + c * FM<real>::div(r0 - ra, r0);
+    vr = aveL(q, vsL, v, vsR);
+}
+
+template <typename real>
+PG_HD void trace_left_c(real c, real ic, real dt, real idx, real r, real u,
+                                             real rsL, real rsR, real usL, real usR,
+                                             real vsL, real v, real vsR,
+                                             real& rl, real& ul, real& vl) {
+    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
+    real ua = aveR(x, usL, u, usR);
+    real dis = fmax(ua, real(0)) * dt;
+    x = lim01(dis * idx);
+    real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
+    real u0 = aveR(x, usL, u, usR);
+    x = lim01((dis + c * dt) * idx);
+    real u_ = aveR(x, usL, u, usR);
+    real ra = aveR(x, rsL, r, rsR);
+    rl = r0 * exp_lim((u_ - u0) * ic);
+    ul = u0 + c * FM<real>::div(ra - r0, r0);
+    vl = aveR(x, vsL, v, vsR);
+}
+
+// HLLC selection (HLLC.cu:2-77, EOS_flag 0) given the wave speeds.
+template <typename real>
+PG_HD void hllc_select(real pm, real sl, real sm, real sr,
+                                            real rl, real pl, real ul, real vl,
+                                            real rr, real pr, real ur, real vr,
+                                            real& Fr, real& us, real& vs, real& pres) {
+    if (!__builtin_signbit(sl)) {
+        Fr = ul * rl; us = ul; vs = vl; pres = pl;
+    } else if (__builtin_signbit(sr)) {
+        Fr = ur * rr; us = ur; vs = vr; pres = pr;
+    } else if (!__builtin_signbit(sm)) {
+        real k = FM<real>::rcp(sl - sm);
+        Fr = sm * rl * (sl - ul) * k; us = ul; vs = vl; pres = (sl * pm - sm * pl) * k;
+    } else {
+        real k = FM<real>::rcp(sr - sm);
+        Fr = sm * rr * (sr - ur) * k; us = ur; vs = vr; pres = (sr * pm - sm * pr) * k;
+    }
+    us *= Fr;
+    vs *= Fr;
+}
+
+// Wave speeds (get_pm_simple, get_lr_speeds_Einfeldt, get_sm;
+// wave_speeds.cu:24-35,288-333; gam = 1) with the states' sound speeds cl,
+// cr, then the HLLC fluxes: mass flux, upwinded normal and transverse
+// momentum flux, interface pressure.
+template <typename real>
+PG_HD void hllc_iso(real cl, real cr, real rl, real pl, real ul, real vl,
+                                         real rr, real pr, real ur, real vr,
+                                         real& Fr, real& us, real& vs, real& pres) {
+    real csum = cl + cr;
+    real ics = FM<real>::rcp(csum);
+    real pm = real(0.5) * (pr + pl) * exp_lim((ul - ur) * ics);
+    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
+    real iss = FM<real>::rcp(srl + srr);
+    real ubar = (srl * ul + srr * ur) * iss;
+    real n2 = real(0.5) * srl * srr * iss * iss;
+    real du = ur - ul;
+    real cbar = FM<real>::sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
+    real sl = ubar - cbar, sr = ubar + cbar;
+    real sm = FM<real>::div(rl * ul * cl + rr * ur * cr, rl * cl + rr * cr) +
+              real(2) * (pl - pr) * ics * FM<real>::rcp(rl + rr);
+    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
+}
+
+// The same with one sound speed c on both sides (azimuthal isothermal sweep):
+// cbar^2 = c^2 + n2 du^2 and sm = (rl (ul + c) + rr (ur - c)) / (rl + rr).
+template <typename real>
+PG_HD void hllc_c(real c, real ic, real c2, real rl, real ul, real vl, real rr, real ur, real vr,
+                                       real& Fr, real& us, real& vs, real& pres) {
+    real pl = c2 * rl, pr = c2 * rr;
+    real pm = real(0.5) * (pr + pl) * exp_lim(real(0.5) * (ul - ur) * ic);
+    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
+    real iss = FM<real>::rcp(srl + srr);
+    real ubar = (srl * ul + srr * ur) * iss;
+    real n2 = real(0.5) * srl * srr * iss * iss;
+    real du = ur - ul;
+    real cbar = FM<real>::sqrt(c2 + n2 * du * du);
+    real sl = ubar - cbar, sr = ubar + cbar;
+    real sm = FM<real>::div(rl * (ul + c) + rr * (ur - c), rl + rr);
+    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
+}
+
+// Pressureless dust flux through an interface (dust_flux, dust.cu:6-105)
+// from the parabolas of the cells left (L) and right (R) of it.
+// idxL, idxR: 1/dx of those cells; dt: the sweep's time step in the
+// coordinate's units.
+template <typename real>
+PG_HD void dust_iface(real dt,
+                                           real idxL, real rL, real rLsL, real rLsR, real uL, real uLsL,
+                                           real uLsR, real vL, real vLsL, real vLsR,
+                                           real idxR, real rR, real rRsL, real rRsR, real uR, real uRsL,
+                                           real uRsR, real vR, real vRsL, real vRsR,
+                                           real& Fr, real& Fu, real& Fv) {
+    // right state from the right cell
+    real y = -fmin(uR - uRsL, real(0)) * dt * idxR;
+    real q = lim01(y);
+    real ur = aveL(q, uRsL, uR, uRsR);
+    real du = FM<real>::div(dt * (ur - uR) * idxR, real(1) - y);
+    q = lim01(-fmin(ur, real(0)) * dt * idxR);
+    real srr = aveL(q, rRsL, rR, rRsR) * exp_lim(du);
+    real sur = aveL(q, uRsL, uR, uRsR);
+    real svr = aveL(q, vRsL, vR, vRsR);
+    // left state from the left cell
+    y = fmax(uL + uLsR, real(0)) * dt * idxL;
+    real x = lim01(y);
+    real ul = aveR(x, uLsL, uL, uLsR);
+    du = FM<real>::div(dt * (uL - ul) * idxL, real(1) - y);
+    x = lim01(fmax(ul, real(0)) * dt * idxL);
+    real srl = aveR(x, rLsL, rL, rLsR) * exp_lim(du);
+    real sul = aveR(x, uLsL, uL, uLsR);
+    real svl = aveR(x, vLsL, vL, vLsR);
+    // donor-cell caps (dust.cu:96-97)
+    srl = fmin(rL * dt * idxL, fmax(sul * srl, real(0)));
+    srr = fmax(-rR * dt * idxR, fmin(sur * srr, real(0)));
+    Fr = srl + srr;
+    Fu = srl * sul + srr * sur;
+    Fv = srl * svl + srr * svr;
+}
+
+// ------------------------------------------------------------ planet (FP64)
+__host__ __device__ inline double ramp_function(double t, double tau, double val) {  // planet.cu:7-17
+    return t < tau ? val * sin(0.5 * 3.1415926535897932384626434 * t / tau) : val;
+}
+
+__host__ __device__ inline double ecc_anomaly(double n, double e, double t) {  // planet.cu:60-103
+    const double twopi = 2.0 * 3.1415926535897932384626434;
+    const double pi = 3.1415926535897932384626434;
+    double l = fmod(n * t, twopi);
+    double D = -e * sin(l);
+    if (e == 0.0 || l == 0.0 || l == pi || l == twopi) return l;
+    double step;
+    int count = 0;
+    do {
+        double es = e * sin(l - D), ec = e * cos(l - D);
+        double x = D + es, dx = 1.0 - ec, d2x = -es;
+        double eps = 2.0 * x * d2x / dx / dx;
+        if (fabs(eps) < 1.0) step = -(x / dx) * (1.0 + eps / 4.0 + eps * eps / 8.0 + 5.0 * eps * eps * eps / 64.0);
+        else step = copysign(fmin(fabs(x / dx), 0.25 * pi), -x);
+        D += step;
+        count++;
+    } while ((fabs(step) > 1.0e-12 || count < 10) && count < 200);
+    return l - D;
+}
+
+// orbit_solution (planet.cu:105-133): analytic Kepler orbit in the frame
+// rotating at frame_omega; mass ramp of planet.cu:7-17.
+__host__ __device__ inline void orbit_solution(Body& b, double t, double ramp_time, double mass,
+                                               double frame_omega) {
+    const double pi = 3.1415926535897932384626434;
+    double a = b.a, e = b.e;
+    double n = sqrt(1.0 / a / a / a);
+    double efac = sqrt(1.0 - e * e);
+    double B = e / (1.0 + efac);
+    double u = ecc_anomaly(n, e, t);
+    double f = u + 2.0 * atan(B * sin(u) / (1.0 - B * cos(u)));
+    double esinf = e * sin(f), ecosf = e * cos(f);
+    b.m = ramp_function(t, ramp_time, mass);
+    b.x = a * efac * efac / (1.0 + ecosf);
+    b.y = f - fmod(frame_omega * t, 2.0 * pi) + pi;
+    b.z = 0.5 * pi;
+    b.vx = n * a * esinf / efac;
+    b.vy = n * (1.0 + ecosf) * (1.0 + ecosf) / efac / efac / efac;
+    b.vz = 0.0;
+}
+
+__device__ __forceinline__ int wrap(int j, int n) {
+    j %= n;
+    return j < 0 ? j + n : j;
+}
+
+
+// ------------------------------------------------------- planet force
+// Planet acceleration (star_planet_grav_cyl, forces.cu:41-58, z = 0) at
+// angle phi_j from the tabulated sin/cos of phi_j and phi_j/2 and the
+// planet's (sin, cos) of phi_p and phi_p/2.  The squared distance is written
+// without cancellation: (R - a)^2 + 4 a R sin^2((phi - phi_p)/2) + rs^2.
+template <typename real>
+__device__ __forceinline__ void planet_acc(const PlanetK& pk, const real* pt, real R, real& ax, real& ay) {
+    const real sp = (real)pk.sphi, cp = (real)pk.cphi, shp = (real)pk.shphi, chp = (real)pk.chphi;
+    const real s = pt[0] * cp - pt[1] * sp;      // sin(phi - phi_p)
+    const real c = pt[1] * cp + pt[0] * sp;      // cos(phi - phi_p)
+    const real h = pt[2] * chp - pt[3] * shp;    // sin((phi - phi_p)/2)
+    const real px = (real)pk.x, m = (real)pk.m, rs = (real)pk.rs;
+    const real Rp2 = (R - px) * (R - px) + real(4) * px * R * h * h + rs * rs;
+    const real iRp = FM<real>::rcp(FM<real>::sqrt(Rp2));
+    const real k = iRp * iRp * iRp;
+    const real ind = (real)pk.m_over_x2;
+    ax = -m * (R - px * c) * k - c * ind;
+    ay = -m * px * s * k + s * ind;
+}
+
+}  // namespace pg

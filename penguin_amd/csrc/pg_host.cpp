@@ -321,8 +321,10 @@ struct Group {
     }
 };
 
+struct AdjWork;
 struct pg_handle {
     Group g;
+    AdjWork* adj = nullptr;   // adjoint workspace, allocated by the first adjoint call
 };
 struct pg_batch {
     Group g;
@@ -349,11 +351,6 @@ int validate(const pg_params& P) {
     if (P.xres < 8 || P.yres < 8) return fail(PG_E_INVALID, "resolution %dx%d too small (min 8x8)", P.xres, P.yres);
     if (P.n_planet != 0 && P.n_planet != 1) return fail(PG_E_INVALID, "n_planet=%d: 0 or 1", P.n_planet);
     if (!(P.xmin > 0.0 && P.xmax > P.xmin)) return fail(PG_E_INVALID, "bad radial domain");
-    size_t esz = P.precision;
-    size_t lds = (3 * (size_t)P.yres + 14 * (RING_THREADS + 2 * NPAD)) * esz;
-    if (lds > 160 * 1024)
-        return fail(PG_E_INVALID, "yres=%d at precision %d needs %zu B of LDS for the azimuthal ring (max 163840)",
-                    P.yres, P.precision, lds);
     return PG_OK;
 }
 
@@ -594,15 +591,16 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
 // Work lists of one launch schedule.
 //   x sweeps / pointwise kernels: every (tile or 256-radius segment, row) of
 //   every disk, disk by disk.
-//   rings: one entry per radius.  Workgroups are dispatched round-robin over
-//   the 8 XCDs, each with its own L2; neighbouring radii share cache lines of
-//   the radius-fastest layout, so with `order` 0 or 1 each disk's radii are
-//   cut into 8 contiguous runs and XCD x receives run x (entry 8k+x of the
-//   list).  Order 0 takes the disks by decreasing ring length (longest
-//   processing time first), order 1 in their own order, order 2 longest
-//   first without the XCD split.  Entries with radius -1 pad the list.
-void build_worklists(const Group& g, int tile_x, int order, std::vector<int2>& wx, std::vector<int2>& wr,
-                     std::vector<int2>& wp) {
+//   azimuthal pass: tiles of TR radii x TY rows (tile = ty * ntr + tr).
+//   Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+//   L2.  Tiles that are neighbours in azimuth share their 6-row halos and
+//   tiles that are neighbours in radius share cache lines, so with order 0
+//   each disk's tile rows are cut into 8 contiguous runs and XCD x receives
+//   run x (entry 8k+x of the list), radius strips innermost.  Order 1 lists
+//   the tiles row by row, order 2 strip by strip.  Entries with tile -1 pad
+//   the list.
+void build_worklists(const Group& g, int tile_x, int ring_b, int order, std::vector<int2>& wx,
+                     std::vector<int2>& wr, std::vector<int2>& wp) {
     wx.clear();
     wr.clear();
     wp.clear();
@@ -614,21 +612,26 @@ void build_worklists(const Group& g, int tile_x, int order, std::vector<int2>& w
         const int segs = (d->nx + 255) / 256;
         for (int b = 0; b < segs * d->ny; ++b) wp.push_back(make_int2(k, b));
     }
-    std::vector<int> ord(nd);
-    for (int k = 0; k < nd; ++k) ord[k] = k;
-    if (order != 1)
-        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return g.disks[a]->ny > g.disks[b]->ny; });
-    if (order == 2) {
-        for (int k : ord)
-            for (int i = 0; i < g.disks[k]->nx; ++i) wr.push_back(make_int2(k, i));
+    const int TR = Launch<float>::phi_tile_radii(ring_b), TY = Launch<float>::phi_tile_rows(ring_b);
+    if (order != 0) {
+        for (int k = 0; k < nd; ++k) {
+            const int ntr = (g.disks[k]->nx + TR - 1) / TR, nty = (g.disks[k]->ny + TY - 1) / TY;
+            if (order == 1) {
+                for (int b = 0; b < ntr * nty; ++b) wr.push_back(make_int2(k, b));
+            } else {
+                for (int tr = 0; tr < ntr; ++tr)
+                    for (int ty = 0; ty < nty; ++ty) wr.push_back(make_int2(k, ty * ntr + tr));
+            }
+        }
         return;
     }
     std::vector<int2> q[8];
-    for (int k : ord) {
-        const int nx = g.disks[k]->nx;
-        const int per = (nx + 7) / 8;
+    for (int k = 0; k < nd; ++k) {
+        const int ntr = (g.disks[k]->nx + TR - 1) / TR, nty = (g.disks[k]->ny + TY - 1) / TY;
+        const int per = (nty + 7) / 8;
         for (int x = 0; x < 8; ++x)
-            for (int i = x * per; i < std::min((x + 1) * per, nx); ++i) q[x].push_back(make_int2(k, i));
+            for (int ty = x * per; ty < std::min((x + 1) * per, nty); ++ty)
+                for (int tr = 0; tr < ntr; ++tr) q[x].push_back(make_int2(k, ty * ntr + tr));
     }
     size_t len = 0;
     for (int x = 0; x < 8; ++x) len = std::max(len, q[x].size());
@@ -645,11 +648,8 @@ int group_schedule(Group& g, int tile_x, int ring_b, int order) {
     if (!tile_ok(tile_x)) return fail(PG_E_INVALID, "tile_x=%d: use 128, 256, 384, 512 or 1024", tile_x);
     if (!ring_ok(ring_b)) return fail(PG_E_INVALID, "ring_b=%d: use 256, 512 or 1024", ring_b);
     if (order < 0 || order > 2) return fail(PG_E_INVALID, "order=%d: use 0, 1 or 2", order);
-    const size_t lds = (3 * (size_t)g.ny_max + 14 * (size_t)(ring_b + 2 * NPAD)) * g.prec;
-    if (lds > 160 * 1024)
-        return fail(PG_E_INVALID, "ring_b=%d with yres %d needs %zu B of LDS (max 163840)", ring_b, g.ny_max, lds);
     std::vector<int2> wx, wr, wp;
-    build_worklists(g, tile_x, order, wx, wr, wp);
+    build_worklists(g, tile_x, ring_b, order, wx, wr, wp);
     const size_t need = wx.size() + wr.size() + wp.size();
     HIP_TRY(hipStreamSynchronize(g.stream));
     if (need > g.wl_cap) {
@@ -1119,6 +1119,314 @@ int group_autotune(Group& g, int steps, double* best_ms) {
     return PG_OK;
 }
 
+// ------------------------------------------------------------ adjoint
+// Per step the backward sweep needs every intermediate of the forward step
+// (the inputs of each pass).  The forward run keeps a checkpoint every
+// `ckpt` steps and the step sizes; the backward sweep, segment by segment
+// from the last, recomputes the segment once from its checkpoint, storing
+// each step's intermediates in a slot, then runs the adjoint passes of its
+// steps in reverse order.
+constexpr int SLOT_WORDS = 22;   // S0, S1, S2, SA, S3, T (3 fields each), V0, V1 (2 each)
+enum { SL_S0 = 0, SL_S1 = 3, SL_S2 = 6, SL_SA = 9, SL_S3 = 12, SL_T = 15, SL_V0 = 18, SL_V1 = 20 };
+// forward passes of one slot, each with its own disk descriptor (the
+// buffers it reads and writes)
+enum { AA_V0, AA_KICK, AA_X, AA_PA, AA_PB, AA_PRED, AA_V1, AA_CORR, AA_N };
+
+}  // namespace
+
+struct AdjWork {
+    int nsteps = 0, ckpt = 0, nseg = 0, slots = 0, prec = 4;
+    size_t N = 0;
+    std::vector<double> ts, dts;   // start time and size of every forward step
+    void* ckpts = nullptr;         // nseg states (3N)
+    void* slot = nullptr;          // slots x SLOT_WORDS*N
+    void* lam = nullptr;           // lamA, lamB, lamT (3N each), lamV0, lamV1 (2N each)
+    void* part = nullptr;          // VIS_SLOTS*N
+    double* lparam = nullptr;      // dL/dq, dL/dalpha
+    double* td = nullptr;          // (t, dt) of every step, on the device
+    void* args = nullptr;          // slots x AA_N disk descriptors
+    bool forward_done = false;
+    size_t bytes = 0;
+};
+
+namespace {
+
+void adj_free(AdjWork* w) {
+    if (!w) return;
+    void* bufs[] = {w->ckpts, w->slot, w->lam, w->part, w->lparam, w->td, w->args};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete w;
+}
+
+template <typename real>
+int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
+    Group& g = h->g;
+    DiskDev* d = g.disks[0];
+    if (h->adj && h->adj->nsteps == nsteps && h->adj->ckpt == ckpt) return PG_OK;
+    adj_free(h->adj);
+    h->adj = nullptr;
+    AdjWork* w = new AdjWork();
+    w->nsteps = nsteps;
+    w->ckpt = ckpt;
+    w->nseg = (nsteps + ckpt - 1) / ckpt;
+    w->slots = std::min(ckpt, nsteps) + 1;
+    w->prec = g.prec;
+    w->N = d->N;
+    const size_t N = w->N, es = sizeof(real);
+    int rc;
+    auto cleanup = [&](int code) {
+        adj_free(w);
+        return code;
+    };
+    if ((rc = dev_alloc(w->bytes, &w->ckpts, (size_t)w->nseg * 3 * N * es)) ||
+        (rc = dev_alloc(w->bytes, &w->slot, (size_t)w->slots * SLOT_WORDS * N * es)) ||
+        (rc = dev_alloc(w->bytes, &w->lam, (size_t)13 * N * es)) ||
+        (rc = dev_alloc(w->bytes, &w->part, (size_t)VIS_SLOTS * N * es)) ||
+        (rc = dev_alloc(w->bytes, &w->lparam, 2 * sizeof(double))) ||
+        (rc = dev_alloc(w->bytes, &w->td, 2 * (size_t)nsteps * sizeof(double))) ||
+        (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))))
+        return cleanup(rc);
+    // descriptors of the recompute passes: the disk's own tables and control
+    // block, with the state and viscosity pointers of the slot
+    std::vector<DiskArgs<real>> a((size_t)w->slots * AA_N);
+    real* base = (real*)w->slot;
+    for (int m = 0; m < w->slots; ++m) {
+        real* sl = base + (size_t)m * SLOT_WORDS * N;
+        real* nx = base + (size_t)std::min(m + 1, w->slots - 1) * SLOT_WORDS * N;
+        const DiskArgs<real> t = d->args<real>();
+        auto set = [&](int k, real* g0, real* g1, real* vis) {
+            DiskArgs<real> x = t;
+            x.gas[0] = g0;
+            x.gas[1] = g1;
+            x.dust[0] = x.dust[1] = nullptr;
+            x.vis = vis;
+            x.dts = nullptr;
+            x.dts_cap = 0;
+            a[(size_t)m * AA_N + k] = x;
+        };
+        set(AA_V0, sl + SL_S0 * N, sl + SL_S0 * N, sl + SL_V0 * N);
+        set(AA_KICK, sl + SL_S1 * N, sl + SL_S1 * N, sl + SL_V0 * N);
+        set(AA_X, sl + SL_S1 * N, sl + SL_S2 * N, sl + SL_V0 * N);
+        set(AA_PA, sl + SL_S2 * N, sl + SL_SA * N, sl + SL_V0 * N);
+        set(AA_PB, sl + SL_SA * N, sl + SL_S3 * N, sl + SL_V0 * N);
+        set(AA_PRED, sl + SL_S3 * N, sl + SL_T * N, sl + SL_V0 * N);
+        set(AA_V1, sl + SL_T * N, sl + SL_T * N, sl + SL_V1 * N);
+        set(AA_CORR, nx + SL_S0 * N, nx + SL_S0 * N, sl + SL_V1 * N);
+    }
+    if (hipMemcpy(w->args, a.data(), a.size() * sizeof(a[0]), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(PG_E_HIP, "adjoint descriptor upload failed"));
+    h->adj = w;
+    return PG_OK;
+}
+
+// prologue of recorded step n: t, dt from the device copy, then the planets
+// and the orbital shift as in the forward step
+int set_step(Group& g, const AdjWork* w, int n) {
+    hipError_t e = g.prec == 8 ? Launch<double>::prologue_replay(g.ls<double>(), w->td + 2 * (size_t)n, g.stream)
+                               : Launch<float>::prologue_replay(g.ls<float>(), w->td + 2 * (size_t)n, g.stream);
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
+    return PG_OK;
+}
+
+#define HIPL(expr)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return fail(PG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));         \
+    } while (0)
+
+// Recompute step n into slot m (and its end state into slot m+1's S0).
+template <typename real>
+int adj_recompute(pg_handle* h, int m, int n, bool write_next) {
+    Group& g = h->g;
+    AdjWork* w = h->adj;
+    const size_t N = w->N;
+    hipStream_t s = g.stream;
+    int rc = set_step(g, w, n);
+    if (rc) return rc;
+    LaunchSet<real> L = g.ls<real>();
+    const DiskArgs<real>* a = (const DiskArgs<real>*)w->args + (size_t)m * AA_N;
+    auto at = [&](int k) {
+        LaunchSet<real> x = L;
+        x.disks = a + k;
+        x.ndisk = 1;
+        return x;
+    };
+    real* sl = (real*)w->slot + (size_t)m * SLOT_WORDS * N;
+    real* nxt = (real*)w->slot + (size_t)(m + 1) * SLOT_WORDS * N;
+    const bool visc = g.visc;
+    if (visc) HIPL(Launch<real>::viscosity(at(AA_V0), 0, 0, s));
+    HIPL(hipMemcpyAsync(sl + SL_S1 * N, sl + SL_S0 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
+    HIPL(Launch<real>::sources(at(AA_KICK), 0, 0, 0, 0, s));
+    HIPL(Launch<real>::sweep_x_gas(at(AA_X), 0, s));
+    HIPL(Launch<real>::ring_gas(at(AA_PA), 0, 1, s));
+    HIPL(Launch<real>::ring_gas(at(AA_PB), 0, 2, s));
+    if (visc) {
+        HIPL(Launch<real>::sources(at(AA_PRED), 0, 1, 0, 0, s));
+        HIPL(Launch<real>::viscosity(at(AA_V1), 0, 0, s));
+    }
+    if (write_next) {
+        HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
+        HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 0, s));
+    }
+    return PG_OK;
+}
+
+// Adjoint passes of step n held in slot m; lam[0] holds the adjoint of the
+// step's end state on entry and of its start state on exit.
+template <typename real>
+int adj_step(pg_handle* h, int m, int n, real*& lamA, real*& lamB) {
+    Group& g = h->g;
+    AdjWork* w = h->adj;
+    DiskDev* d = g.disks[0];
+    const size_t N = w->N;
+    const int nx = d->nx, ny = d->ny;
+    hipStream_t s = g.stream;
+    int rc = set_step(g, w, n);
+    if (rc) return rc;
+    real* sl = (real*)w->slot + (size_t)m * SLOT_WORDS * N;
+    real* lamT = (real*)w->lam + 6 * N;
+    real* lamV0 = (real*)w->lam + 9 * N;
+    real* lamV1 = (real*)w->lam + 11 * N;
+    AdjArgs<real> A{};
+    A.d = (const DiskArgs<real>*)g.d_args;
+    A.part = (real*)w->part;
+    A.lparam = w->lparam;
+    A.q = d->P.n_planet > 0 ? d->P.planet_mass : 0.0;
+    A.alpha = d->P.visc_flag ? d->P.ss_alpha : 0.0;
+    const bool visc = g.visc;
+    // corrector: S4 = C(S3, V1)
+    A.mode = 2; A.s = sl + SL_S3 * N; A.vis = sl + SL_V1 * N; A.lo = lamA; A.li = lamB; A.lvis = lamV1; A.accumulate = 0;
+    HIPL(AdjLaunch<real>::src(A, nx, ny, s));
+    if (visc) {
+        // V1 = vis(T)
+        A.s = sl + SL_T * N; A.lvis = lamV1; A.li = lamT; A.accumulate = 0;
+        HIPL(AdjLaunch<real>::vis(A, nx, ny, s));
+        // T = P(S3, V0)
+        A.mode = 1; A.s = sl + SL_S3 * N; A.vis = sl + SL_V0 * N; A.lo = lamT; A.li = lamB; A.lvis = lamV0;
+        A.accumulate = 1;
+        HIPL(AdjLaunch<real>::src(A, nx, ny, s));
+    }
+    // S3 = shift + advection(SA)
+    A.s = sl + SL_SA * N; A.lo = lamB; A.li = lamA; A.accumulate = 0;
+    HIPL(AdjLaunch<real>::adv(A, nx, ny, s));
+    // SA = azimuthal sweep(S2)
+    A.s = sl + SL_S2 * N; A.lo = lamA; A.li = lamB; A.accumulate = 0;
+    HIPL(AdjLaunch<real>::phi(A, nx, ny, s));
+    // S2 = x sweep(S1)
+    A.s = sl + SL_S1 * N; A.lo = lamB; A.li = lamA; A.accumulate = 0;
+    HIPL(AdjLaunch<real>::x(A, nx, ny, s));
+    // S1 = kick(S0, V0)
+    A.mode = 0; A.s = sl + SL_S0 * N; A.vis = sl + SL_V0 * N; A.lo = lamA; A.li = lamB; A.lvis = lamV0;
+    A.accumulate = visc ? 2 : 0;
+    HIPL(AdjLaunch<real>::src(A, nx, ny, s));
+    if (visc) {
+        // V0 = vis(S0)
+        A.s = sl + SL_S0 * N; A.lvis = lamV0; A.li = lamB; A.accumulate = 1;
+        HIPL(AdjLaunch<real>::vis(A, nx, ny, s));
+    }
+    std::swap(lamA, lamB);
+    return PG_OK;
+}
+
+int adj_check_config(const pg_params& P) {
+    if (P.dust_flag || P.kill_flag)
+        return fail(PG_E_INVALID, "the adjoint covers gas disks without dust and wave killing");
+    return PG_OK;
+}
+
+template <typename real>
+int adj_forward(pg_handle* h, int nsteps, int ckpt, double t0, double dt0, double* dts_out) {
+    Group& g = h->g;
+    int rc;
+    if ((rc = adj_alloc<real>(h, nsteps, ckpt))) return rc;
+    AdjWork* w = h->adj;
+    w->forward_done = false;
+    const size_t bytes = 3 * w->N * sizeof(real);
+    w->dts.assign(nsteps, 0.0);
+    w->ts.assign(nsteps, 0.0);
+    double t = t0, dt = dt0;
+    for (int k = 0; k < w->nseg; ++k) {
+        const int n0 = k * ckpt, len = std::min(ckpt, nsteps - n0);
+        HIPL(hipMemcpyAsync((char*)w->ckpts + (size_t)k * bytes, g.disks[0]->gas[g.gpar], bytes,
+                            hipMemcpyDeviceToDevice, g.stream));
+        if ((rc = group_run(g, len, &t, &dt, w->dts.data() + n0, 1, nullptr))) return rc;
+    }
+    // start times, summed in the order the device loop sums them
+    double tt = t0;
+    for (int n = 0; n < nsteps; ++n) {
+        w->ts[n] = tt;
+        tt += w->dts[n];
+    }
+    std::vector<double> td(2 * (size_t)nsteps);
+    for (int n = 0; n < nsteps; ++n) {
+        td[2 * n] = w->ts[n];
+        td[2 * n + 1] = w->dts[n];
+    }
+    HIPL(hipMemcpy(w->td, td.data(), td.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (dts_out) std::memcpy(dts_out, w->dts.data(), sizeof(double) * nsteps);
+    w->forward_done = true;
+    return PG_OK;
+}
+
+template <typename real>
+int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
+    Group& g = h->g;
+    AdjWork* w = h->adj;
+    DiskDev* d = g.disks[0];
+    const size_t N = w->N;
+    const int nx = d->nx;
+    int rc;
+    // adjoint of the final state in the device representation:
+    // rho (p = cs2 rho), u, vp (v = vbg + vp)
+    std::vector<double> l0(3 * N);
+    for (size_t o = 0; o < N; ++o) {
+        const int ip = (int)(o % nx) + NPAD;
+        l0[o] = dL[o] + tabv(d, T_CS2, ip) * dL[N + o];
+        l0[N + o] = dL[2 * N + o];
+        l0[2 * N + o] = dL[3 * N + o];
+    }
+    real* lamA = (real*)w->lam;
+    real* lamB = (real*)w->lam + 3 * N;
+    if ((rc = upload(g.prec, g.stream, l0, lamA))) return rc;
+    HIPL(hipMemsetAsync(w->lparam, 0, 2 * sizeof(double), g.stream));
+    HIPL(hipMemsetAsync((real*)w->lam + 9 * N, 0, 4 * N * sizeof(real), g.stream));
+    const size_t bytes = 3 * N * sizeof(real);
+    for (int k = w->nseg - 1; k >= 0; --k) {
+        const int n0 = k * w->ckpt, len = std::min(w->ckpt, w->nsteps - n0);
+        HIPL(hipMemcpyAsync(w->slot, (char*)w->ckpts + (size_t)k * bytes, bytes, hipMemcpyDeviceToDevice, g.stream));
+        for (int m = 0; m < len; ++m)
+            if ((rc = adj_recompute<real>(h, m, n0 + m, m + 1 < len))) return rc;
+        for (int m = len - 1; m >= 0; --m)
+            if ((rc = adj_step<real>(h, m, n0 + m, lamA, lamB))) return rc;
+    }
+    std::vector<double> lam0;
+    if ((rc = download(g.prec, g.stream, lamA, lam0, 3 * N))) return rc;
+    double lp[2] = {0.0, 0.0};
+    HIPL(hipMemcpy(lp, w->lparam, sizeof(lp), hipMemcpyDeviceToHost));
+    // the initial radial velocity -1.5 nu(R)/R is proportional to alpha
+    double dal = lp[1];
+    if (d->P.visc_flag && d->P.ss_alpha > 0.0) {
+        double acc = 0.0;
+        for (size_t o = 0; o < N; ++o) acc += lam0[N + o] * tabv(d, T_G_U, (int)(o % nx) + NPAD);
+        dal += acc / d->P.ss_alpha;
+    }
+    out->dq = lp[0];
+    out->dalpha = dal;
+    if (out->dstate0) {
+        double* s0 = out->dstate0;
+        for (size_t o = 0; o < N; ++o) {
+            s0[o] = lam0[o];
+            s0[N + o] = 0.0;
+            s0[2 * N + o] = lam0[N + o];
+            s0[3 * N + o] = lam0[2 * N + o];
+            s0[4 * N + o] = 0.0;
+        }
+    }
+    HIPL(hipStreamSynchronize(g.stream));
+    return PG_OK;
+}
+
 #define CHECK_H(h) \
     if (!(h)) return fail(PG_E_INVALID, "null handle")
 
@@ -1147,6 +1455,8 @@ int pg_create(const pg_params* params, pg_handle** out) {
 
 int pg_destroy(pg_handle* h) {
     if (!h) return PG_OK;
+    if (h->g.stream) (void)hipStreamSynchronize(h->g.stream);
+    adj_free(h->adj);
     group_release(h->g);
     delete h;
     return PG_OK;
@@ -1463,6 +1773,105 @@ int pg_device_bytes(pg_handle* h, size_t* bytes) {
     if (!bytes) return fail(PG_E_INVALID, "null argument");
     *bytes = h->g.bytes;
     return PG_OK;
+}
+
+// ---- adjoint ---------------------------------------------------------------
+int pg_adjoint_forward(pg_handle* h, int nsteps, int ckpt_every, double t0, double dt0, double* dts) {
+    CHECK_H(h);
+    if (nsteps < 1 || ckpt_every < 1) return fail(PG_E_INVALID, "nsteps and ckpt_every must be >= 1");
+    int rc = adj_check_config(h->g.disks[0]->P);
+    if (rc) return rc;
+    return h->g.prec == 8 ? adj_forward<double>(h, nsteps, ckpt_every, t0, dt0, dts)
+                          : adj_forward<float>(h, nsteps, ckpt_every, t0, dt0, dts);
+}
+
+int pg_adjoint_backward(pg_handle* h, const double* dL_dstate, pg_grads* grads) {
+    CHECK_H(h);
+    if (!dL_dstate || !grads) return fail(PG_E_INVALID, "null argument");
+    if (!h->adj || !h->adj->forward_done) return fail(PG_E_STATE, "pg_adjoint_forward has not run");
+    return h->g.prec == 8 ? adj_backward<double>(h, dL_dstate, grads) : adj_backward<float>(h, dL_dstate, grads);
+}
+
+}  // extern "C"
+
+template <typename real>
+static int adjoint_pass(pg_handle* h, int pass, double t, double dt, const double* lam_out, double* lam_in,
+                        double* dparam) {
+    Group& g = h->g;
+    DiskDev* d = g.disks[0];
+    const size_t N = d->N;
+    int rc;
+    // control block as the module entry point sets it
+    if ((rc = pull_ctl(g))) return rc;
+    StepCtl& c = *d->ctl_host;
+    const bool src = pass >= 4;
+    c.t = t;
+    c.dt = src ? 2.0 * dt : dt;
+    if ((rc = push_ctl(g))) return rc;
+    if ((rc = prologue(g, 0, pass == 2 ? 1 : 0))) return rc;
+    if (src) {
+        if ((rc = pull_ctl(g))) return rc;
+        c.planet1 = c.planet0;
+        c.pk1 = c.pk0;
+        if ((rc = push_ctl(g))) return rc;
+    }
+    size_t scratch = 0;
+    real *lo = nullptr, *li = nullptr, *lv = nullptr, *part = nullptr;
+    double* lp = nullptr;
+    auto cleanup = [&](int code) {
+        void* b[] = {lo, li, lv, part, lp};
+        for (void* x : b)
+            if (x) (void)hipFree(x);
+        return code;
+    };
+    if ((rc = dev_alloc(scratch, &lo, 3 * N * sizeof(real))) || (rc = dev_alloc(scratch, &li, 3 * N * sizeof(real))) ||
+        (rc = dev_alloc(scratch, &lv, 2 * N * sizeof(real))) ||
+        (rc = dev_alloc(scratch, &part, (size_t)VIS_SLOTS * N * sizeof(real))) ||
+        (rc = dev_alloc(scratch, &lp, 2 * sizeof(double))))
+        return cleanup(rc);
+    std::vector<double> lin(lam_out, lam_out + (pass == 3 ? 2 : 3) * N);
+    if ((rc = upload(g.prec, g.stream, lin, pass == 3 ? (void*)lv : (void*)lo))) return cleanup(rc);
+    if (hipMemsetAsync(lp, 0, 2 * sizeof(double), g.stream) != hipSuccess) return cleanup(fail(PG_E_HIP, "memset"));
+    AdjArgs<real> A{};
+    A.d = (const DiskArgs<real>*)g.d_args;
+    A.s = (const real*)d->gas[g.gpar];
+    A.vis = (const real*)d->vis;
+    A.lo = lo;
+    A.li = li;
+    A.lvis = lv;
+    A.part = part;
+    A.lparam = lp;
+    A.q = d->P.n_planet > 0 ? d->P.planet_mass : 0.0;
+    A.alpha = d->P.visc_flag ? d->P.ss_alpha : 0.0;
+    A.accumulate = 0;
+    hipError_t e = hipSuccess;
+    switch (pass) {
+        case 0: e = AdjLaunch<real>::x(A, d->nx, d->ny, g.stream); break;
+        case 1: e = AdjLaunch<real>::phi(A, d->nx, d->ny, g.stream); break;
+        case 2: e = AdjLaunch<real>::adv(A, d->nx, d->ny, g.stream); break;
+        case 3: e = AdjLaunch<real>::vis(A, d->nx, d->ny, g.stream); break;
+        default: A.mode = pass - 4; e = AdjLaunch<real>::src(A, d->nx, d->ny, g.stream); break;
+    }
+    if (e != hipSuccess) return cleanup(fail(PG_E_HIP, "adjoint pass launch: %s", hipGetErrorString(e)));
+    std::vector<double> out;
+    if ((rc = download(g.prec, g.stream, li, out, 3 * N))) return cleanup(rc);
+    std::memcpy(lam_in, out.data(), 3 * N * sizeof(double));
+    if (dparam && hipMemcpy(dparam, lp, 2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return cleanup(fail(PG_E_HIP, "download"));
+    return cleanup(PG_OK);
+}
+
+extern "C" {
+
+int pg_adjoint_pass(pg_handle* h, int pass, double t, double dt, const double* lam_out, double* lam_in,
+                    double* dparam) {
+    CHECK_H(h);
+    if (pass < 0 || pass > 6) return fail(PG_E_INVALID, "pass %d: use 0..6", pass);
+    if (!lam_out || !lam_in) return fail(PG_E_INVALID, "null argument");
+    int rc = adj_check_config(h->g.disks[0]->P);
+    if (rc) return rc;
+    return h->g.prec == 8 ? adjoint_pass<double>(h, pass, t, dt, lam_out, lam_in, dparam)
+                          : adjoint_pass<float>(h, pass, t, dt, lam_out, lam_in, dparam);
 }
 
 // ---- batches ---------------------------------------------------------------

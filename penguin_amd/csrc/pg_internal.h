@@ -28,7 +28,7 @@ namespace pg {
 constexpr int NPAD = 3;
 constexpr int NGAS = 3;
 constexpr int NDUST = 3;
-constexpr int RING_THREADS = 512;   // default threads per azimuthal-ring block
+constexpr int RING_THREADS = 512;   // default threads per azimuthal-pass block
 
 struct Body {
     double m, a, e, x, y, z, vx, vy, vz, rs;
@@ -111,6 +111,25 @@ struct DiskArgs {
     double* dts;         // step sizes recorded by the device loop (or null)
     int dts_cap;         // entries of dts
 };
+
+// Buffers of one adjoint pass (pg_adjoint.hip): the forward inputs the
+// pass recomputes from, the adjoint of its output and where the adjoint of
+// its inputs goes.  One disk.
+template <typename real>
+struct AdjArgs {
+    const DiskArgs<real>* d;   // the disk: tables, control block (dt, planets, shift)
+    const real* s;             // forward input state of the pass (3N)
+    const real* vis;           // viscous forces the pass read (2N)
+    const real* lo;            // adjoint of the pass output (3N)
+    real* li;                  // adjoint of the pass input state (3N)
+    real* lvis;                // adjoint of vis (2N)
+    real* part;                // viscosity: per-output partial adjoints (VIS_SLOTS*N)
+    double* lparam;            // [0] dL/d planet_mass, [1] dL/d ss_alpha
+    int accumulate;            // 1: li += adjoint, 0: li = adjoint
+    int mode;                  // source-term mode, 0 kick / 1 predictor / 2 corrector
+    double q, alpha;           // planet_mass, ss_alpha
+};
+constexpr int VIS_SLOTS = 23;
 
 // A set of disks launched together (one disk, or a batch).  Each kernel's
 // grid walks a work list of (disk, local block) pairs, so one launch covers

@@ -7,11 +7,12 @@
 // Kernels and the reference routines they compute:
 //   k_sweep_x_gas   sweepx_inplace  (sweeps.cu:263-358) + riemann()
 //   k_sweep_x_dust  sweepx_dust_inplace (dust.cu:178-257)
-//   k_ring_gas      sweepy_inplace (sweeps.cu:361-456), then the orbital
+//   k_phi           sweepy_inplace (sweeps.cu:361-456) or the dust
+//                   azimuthal sweep (dust.cu:260-353), then the orbital
 //                   shift (orbit.cu:17-49) and the residual advection
-//                   (advection.cu:1-173) of one radius, fused: the ring of
-//                   one radius stays in LDS between the three stages
-//   k_ring_dust     the same for dust (dust.cu:260-353, advection.cu:175-319)
+//                   (advection.cu:1-319), fused over a tile of radii x rows:
+//                   the shift becomes a per-radius offset of the tile's
+//                   loads, so nothing but the tile is held in LDS
 //   k_viscosity     viscous_forces (viscosity.cu:34-197)
 //   k_sources       apply_source_terms (sweeps.cu:555-711), optionally
 //                   followed by killwave (killwave.cu:17-103) and the CFL
@@ -30,354 +31,10 @@
 
 #include <cmath>
 
+#include "pg_device.h"
 #include "pg_internal.h"
 
 namespace pg {
-
-// A launch's parity word: bit 0 selects the current gas buffer of every disk
-// in the set, bit 1 the current dust buffer; a kernel that writes a fluid
-// writes the other buffer of its pair.
-__device__ __forceinline__ int gsel(int par) { return par & 1; }
-__device__ __forceinline__ int dsel(int par) { return (par >> 1) & 1; }
-
-
-// per-radius tables, field-major: lanes that differ in radius read
-// consecutive addresses
-#define TAB(f, ip) d.tab[(size_t)(f) * d.tab_stride + (ip)]
-
-// Division, reciprocal, square root and exponential.  FP32: the hardware
-// approximations (v_rcp_f32, v_sqrt_f32, v_exp_f32; about 1 ulp); FP64: IEEE.
-template <typename real> struct FM;
-template <> struct FM<float> {
-    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-    static __device__ __forceinline__ float exp(float x) { return __expf(x); }
-};
-template <> struct FM<double> {
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
-    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
-    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
-};
-
-// ---------------------------------------------------------------- helpers
-template <typename real>
-__device__ __forceinline__ real lim01(real a) {  // solve.cu:14-17
-    return fmin(fmax(a, real(0)), real(1));
-}
-template <typename real>
-__device__ __forceinline__ real exp_lim(real x) {  // solve.cu:29-34
-    return fmax(real(1) + x, real(1.0e-6));
-}
-
-// 4th-order slopes from precomputed geometric weights (the linear map
-// get_4th_parameters applies to the four neighbouring differences,
-// recon.cu:216-357), limiter (recon.cu:361-376), flattening and the PPM
-// monotonicity correction (PPM.cu:64-91).
-template <typename real>
-__device__ __forceinline__ void slopes(real am2, real am1, real a0, real ap1, real ap2,
-                                       real wl0, real wl1, real wl2, real wr0, real wr1, real wr2,
-                                       real flat, real& sL, real& sR) {
-    real d0 = am1 - am2, d1 = a0 - am1, d2 = ap1 - a0, d3 = ap2 - ap1;
-    real l = wl0 * d0 + wl1 * d1 + wl2 * d2;
-    real r = wr0 * d1 + wr1 * d2 + wr2 * d3;
-    l = copysign(fmin(fabs(l), fabs(d1)), d1);
-    r = copysign(fmin(fabs(r), fabs(d2)), d2);
-    if (r * l <= real(0)) {
-        l = real(0);
-        r = real(0);
-    }
-    real fk = real(1) - flat;
-    r *= fk;
-    l *= fk;
-    real t3 = r + l;
-    real t2 = t3 * t3;
-    real t1 = t3 * real(3) * (l - r);
-    if (t2 < t1) l = real(2) * r;
-    else if (t2 < -t1) r = real(2) * l;
-    sL = l;
-    sR = r;
-}
-
-// Mean of the cell parabola over its first fraction q (get_PPM_aveL,
-// PPM.cu:106-113) and over its last fraction x (get_PPM_aveR with x = 1-q,
-// PPM.cu:95-104).
-template <typename real>
-__device__ __forceinline__ real aveL(real q, real sL, real a, real sR) {
-    real a6 = real(3) * (sL - sR);
-    return (a - sL) + (q * real(0.5)) * ((sR + sL) + (real(1) - q * real(2.0 / 3.0)) * a6);
-}
-template <typename real>
-__device__ __forceinline__ real aveR(real x, real sL, real a, real sR) {
-    real a6 = real(3) * (sL - sR);
-    return (a + sR) - (x * real(0.5)) * ((sR + sL) - (real(1) - x * real(2.0 / 3.0)) * a6);
-}
-
-// Flattening coefficient of the centre cell of a 5-cell window
-// (flatten, PPM.cu:7-62).  cs is the cell's sound speed.
-template <typename real>
-__device__ __forceinline__ real flatten5(real pm2, real pm1, real p0, real pp1, real pp2,
-                                         real um1, real u0, real up1, real cs) {
-    real f = real(0);
-    real dp1f = pp1 - p0, dp2f = pp2 - pm1, dp1b = p0 - pm1, dp2b = pp1 - pm2;
-    // Mf > 0 || Mb > 0 with Mf = (u0-up1)/cs, Mb = (um1-u0)/cs and cs > 0
-    (void)cs;
-    if (u0 - up1 > real(0) || um1 - u0 > real(0)) {
-        if (dp2f != real(0) && dp2b != real(0)) {
-            real ff = real(10) * fmax(real(0), fabs(FM<real>::div(dp1f, dp2f) - real(0.5)) - real(0.25));
-            real fb = real(10) * fmax(real(0), fabs(FM<real>::div(dp1b, dp2b) - real(0.5)) - real(0.25));
-            f = fmax(ff, fb);
-            f = fmin(real(0.5), FM<real>::div(f * fmax(fabs(dp1f), fabs(dp1b)), p0));
-        }
-    }
-    return f;
-}
-
-// Characteristic-traced half-step state on the right of an interface from
-// the cell right of it (set_state, states.cu:151-206), locally isothermal
-// (EOS_flag 0 fixes gam = 1, parameters.h:118-119, so exp_lim(gam*x) =
-// exp_lim(x) and the sound speed of the traced state is sqrt(p0/r0)).
-// c: the cell's sound speed; idx = 1/dx.  Returns the state and its sound
-// speed cout (= the one HLLC recomputes from pl/rl, wave_speeds.cu:290).
-template <typename real>
-__device__ __forceinline__ void trace_right(real c, real dt, real idx, real r, real p, real u,
-                                            real rsL, real rsR, real psL, real psR, real usL, real usR,
-                                            real vsL, real v, real vsR,
-                                            real& rr, real& pr, real& ur, real& vr, real& cout) {
-    real q = lim01((c - fmin(u, real(0))) * dt * idx);
-    real ua = aveL(q, usL, u, usR);
-    real dis = fmin(ua, real(0)) * dt;
-    q = lim01(-dis * idx);
-    real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
-    real p0 = fmax(aveL(q, psL, p, psR), p * real(1.0e-10));
-    real u0 = aveL(q, usL, u, usR);
-    real ir0 = FM<real>::rcp(r0);
-    real cs = FM<real>::sqrt(p0 * ir0);
-    q = lim01((cs * dt - dis) * idx);
-    real u_ = aveL(q, usL, u, usR);
-    real p_ = aveL(q, psL, p, psR);
-    real ics = FM<real>::rcp(cs);
-    real e = exp_lim((u0 - u_) * ics);
-    rr = r0 * e;
-    pr = p0 * e;
-    ur = u0 + (p0 - p_) * ics * ir0;
-    vr = aveL(q, vsL, v, vsR);
-    cout = cs;
-}
-
-// ... and on the left from the cell left of it (states.cu:208-264).
-template <typename real>
-__device__ __forceinline__ void trace_left(real c, real dt, real idx, real r, real p, real u,
-                                           real rsL, real rsR, real psL, real psR, real usL, real usR,
-                                           real vsL, real v, real vsR,
-                                           real& rl, real& pl, real& ul, real& vl, real& cout) {
-    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
-    real ua = aveR(x, usL, u, usR);
-    real dis = fmax(ua, real(0)) * dt;
-    x = lim01(dis * idx);
-    real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
-    real p0 = fmax(aveR(x, psL, p, psR), p * real(1.0e-10));
-    real u0 = aveR(x, usL, u, usR);
-    real ir0 = FM<real>::rcp(r0);
-    real cs = FM<real>::sqrt(p0 * ir0);
-    x = lim01((dis + cs * dt) * idx);
-    real u_ = aveR(x, usL, u, usR);
-    real p_ = aveR(x, psL, p, psR);
-    real ics = FM<real>::rcp(cs);
-    real e = exp_lim((u_ - u0) * ics);
-    rl = r0 * e;
-    pl = p0 * e;
-    ul = u0 + (p_ - p0) * ics * ir0;
-    vl = aveR(x, vsL, v, vsR);
-    cout = cs;
-}
-
-// The same with one sound speed c for every cell of the row (p = c^2 rho
-// with c fixed: the azimuthal sweep of a locally isothermal disk); pressure
-// is c^2 times density throughout, so only rho, u, v are traced.
-template <typename real>
-__device__ __forceinline__ void trace_right_c(real c, real ic, real dt, real idx, real r, real u,
-                                              real rsL, real rsR, real usL, real usR,
-                                              real vsL, real v, real vsR,
-                                              real& rr, real& ur, real& vr) {
-    real q = lim01((c - fmin(u, real(0))) * dt * idx);
-    real ua = aveL(q, usL, u, usR);
-    real dis = fmin(ua, real(0)) * dt;
-    q = lim01(-dis * idx);
-    real r0 = fmax(aveL(q, rsL, r, rsR), r * real(1.0e-10));
-    real u0 = aveL(q, usL, u, usR);
-    q = lim01((c * dt - dis) * idx);
-    real u_ = aveL(q, usL, u, usR);
-    real ra = aveL(q, rsL, r, rsR);
-    rr = r0 * exp_lim((u0 - u_) * ic);
-    ur = u0 + c * FM<real>::div(r0 - ra, r0);
-    vr = aveL(q, vsL, v, vsR);
-}
-
-template <typename real>
-__device__ __forceinline__ void trace_left_c(real c, real ic, real dt, real idx, real r, real u,
-                                             real rsL, real rsR, real usL, real usR,
-                                             real vsL, real v, real vsR,
-                                             real& rl, real& ul, real& vl) {
-    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
-    real ua = aveR(x, usL, u, usR);
-    real dis = fmax(ua, real(0)) * dt;
-    x = lim01(dis * idx);
-    real r0 = fmax(aveR(x, rsL, r, rsR), r * real(1.0e-10));
-    real u0 = aveR(x, usL, u, usR);
-    x = lim01((dis + c * dt) * idx);
-    real u_ = aveR(x, usL, u, usR);
-    real ra = aveR(x, rsL, r, rsR);
-    rl = r0 * exp_lim((u_ - u0) * ic);
-    ul = u0 + c * FM<real>::div(ra - r0, r0);
-    vl = aveR(x, vsL, v, vsR);
-}
-
-// HLLC selection (HLLC.cu:2-77, EOS_flag 0) given the wave speeds.
-template <typename real>
-__device__ __forceinline__ void hllc_select(real pm, real sl, real sm, real sr,
-                                            real rl, real pl, real ul, real vl,
-                                            real rr, real pr, real ur, real vr,
-                                            real& Fr, real& us, real& vs, real& pres) {
-    if (!signbit(sl)) {
-        Fr = ul * rl; us = ul; vs = vl; pres = pl;
-    } else if (signbit(sr)) {
-        Fr = ur * rr; us = ur; vs = vr; pres = pr;
-    } else if (!signbit(sm)) {
-        real k = FM<real>::rcp(sl - sm);
-        Fr = sm * rl * (sl - ul) * k; us = ul; vs = vl; pres = (sl * pm - sm * pl) * k;
-    } else {
-        real k = FM<real>::rcp(sr - sm);
-        Fr = sm * rr * (sr - ur) * k; us = ur; vs = vr; pres = (sr * pm - sm * pr) * k;
-    }
-    us *= Fr;
-    vs *= Fr;
-}
-
-// Wave speeds (get_pm_simple, get_lr_speeds_Einfeldt, get_sm;
-// wave_speeds.cu:24-35,288-333; gam = 1) with the states' sound speeds cl,
-// cr, then the HLLC fluxes: mass flux, upwinded normal and transverse
-// momentum flux, interface pressure.
-template <typename real>
-__device__ __forceinline__ void hllc_iso(real cl, real cr, real rl, real pl, real ul, real vl,
-                                         real rr, real pr, real ur, real vr,
-                                         real& Fr, real& us, real& vs, real& pres) {
-    real csum = cl + cr;
-    real ics = FM<real>::rcp(csum);
-    real pm = real(0.5) * (pr + pl) * exp_lim((ul - ur) * ics);
-    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
-    real iss = FM<real>::rcp(srl + srr);
-    real ubar = (srl * ul + srr * ur) * iss;
-    real n2 = real(0.5) * srl * srr * iss * iss;
-    real du = ur - ul;
-    real cbar = FM<real>::sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
-    real sl = ubar - cbar, sr = ubar + cbar;
-    real sm = FM<real>::div(rl * ul * cl + rr * ur * cr, rl * cl + rr * cr) +
-              real(2) * (pl - pr) * ics * FM<real>::rcp(rl + rr);
-    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
-}
-
-// The same with one sound speed c on both sides (azimuthal isothermal sweep):
-// cbar^2 = c^2 + n2 du^2 and sm = (rl (ul + c) + rr (ur - c)) / (rl + rr).
-template <typename real>
-__device__ __forceinline__ void hllc_c(real c, real ic, real c2, real rl, real ul, real vl, real rr, real ur, real vr,
-                                       real& Fr, real& us, real& vs, real& pres) {
-    real pl = c2 * rl, pr = c2 * rr;
-    real pm = real(0.5) * (pr + pl) * exp_lim(real(0.5) * (ul - ur) * ic);
-    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
-    real iss = FM<real>::rcp(srl + srr);
-    real ubar = (srl * ul + srr * ur) * iss;
-    real n2 = real(0.5) * srl * srr * iss * iss;
-    real du = ur - ul;
-    real cbar = FM<real>::sqrt(c2 + n2 * du * du);
-    real sl = ubar - cbar, sr = ubar + cbar;
-    real sm = FM<real>::div(rl * (ul + c) + rr * (ur - c), rl + rr);
-    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
-}
-
-// Pressureless dust flux through an interface (dust_flux, dust.cu:6-105)
-// from the parabolas of the cells left (L) and right (R) of it.
-// idxL, idxR: 1/dx of those cells; dt: the sweep's time step in the
-// coordinate's units.
-template <typename real>
-__device__ __forceinline__ void dust_iface(real dt,
-                                           real idxL, real rL, real rLsL, real rLsR, real uL, real uLsL,
-                                           real uLsR, real vL, real vLsL, real vLsR,
-                                           real idxR, real rR, real rRsL, real rRsR, real uR, real uRsL,
-                                           real uRsR, real vR, real vRsL, real vRsR,
-                                           real& Fr, real& Fu, real& Fv) {
-    // right state from the right cell
-    real y = -fmin(uR - uRsL, real(0)) * dt * idxR;
-    real q = lim01(y);
-    real ur = aveL(q, uRsL, uR, uRsR);
-    real du = FM<real>::div(dt * (ur - uR) * idxR, real(1) - y);
-    q = lim01(-fmin(ur, real(0)) * dt * idxR);
-    real srr = aveL(q, rRsL, rR, rRsR) * exp_lim(du);
-    real sur = aveL(q, uRsL, uR, uRsR);
-    real svr = aveL(q, vRsL, vR, vRsR);
-    // left state from the left cell
-    y = fmax(uL + uLsR, real(0)) * dt * idxL;
-    real x = lim01(y);
-    real ul = aveR(x, uLsL, uL, uLsR);
-    du = FM<real>::div(dt * (uL - ul) * idxL, real(1) - y);
-    x = lim01(fmax(ul, real(0)) * dt * idxL);
-    real srl = aveR(x, rLsL, rL, rLsR) * exp_lim(du);
-    real sul = aveR(x, uLsL, uL, uLsR);
-    real svl = aveR(x, vLsL, vL, vLsR);
-    // donor-cell caps (dust.cu:96-97)
-    srl = fmin(rL * dt * idxL, fmax(sul * srl, real(0)));
-    srr = fmax(-rR * dt * idxR, fmin(sur * srr, real(0)));
-    Fr = srl + srr;
-    Fu = srl * sul + srr * sur;
-    Fv = srl * svl + srr * svr;
-}
-
-// ------------------------------------------------------------ planet (FP64)
-__host__ __device__ inline double ramp_function(double t, double tau, double val) {  // planet.cu:7-17
-    return t < tau ? val * sin(0.5 * 3.1415926535897932384626434 * t / tau) : val;
-}
-
-__host__ __device__ inline double ecc_anomaly(double n, double e, double t) {  // planet.cu:60-103
-    const double twopi = 2.0 * 3.1415926535897932384626434;
-    const double pi = 3.1415926535897932384626434;
-    double l = fmod(n * t, twopi);
-    double D = -e * sin(l);
-    if (e == 0.0 || l == 0.0 || l == pi || l == twopi) return l;
-    double step;
-    int count = 0;
-    do {
-        double es = e * sin(l - D), ec = e * cos(l - D);
-        double x = D + es, dx = 1.0 - ec, d2x = -es;
-        double eps = 2.0 * x * d2x / dx / dx;
-        if (fabs(eps) < 1.0) step = -(x / dx) * (1.0 + eps / 4.0 + eps * eps / 8.0 + 5.0 * eps * eps * eps / 64.0);
-        else step = copysign(fmin(fabs(x / dx), 0.25 * pi), -x);
-        D += step;
-        count++;
-    } while ((fabs(step) > 1.0e-12 || count < 10) && count < 200);
-    return l - D;
-}
-
-// orbit_solution (planet.cu:105-133): analytic Kepler orbit in the frame
-// rotating at frame_omega; mass ramp of planet.cu:7-17.
-__host__ __device__ inline void orbit_solution(Body& b, double t, double ramp_time, double mass,
-                                               double frame_omega) {
-    const double pi = 3.1415926535897932384626434;
-    double a = b.a, e = b.e;
-    double n = sqrt(1.0 / a / a / a);
-    double efac = sqrt(1.0 - e * e);
-    double B = e / (1.0 + efac);
-    double u = ecc_anomaly(n, e, t);
-    double f = u + 2.0 * atan(B * sin(u) / (1.0 - B * cos(u)));
-    double esinf = e * sin(f), ecosf = e * cos(f);
-    b.m = ramp_function(t, ramp_time, mass);
-    b.x = a * efac * efac / (1.0 + ecosf);
-    b.y = f - fmod(frame_omega * t, 2.0 * pi) + pi;
-    b.z = 0.5 * pi;
-    b.vx = n * a * esinf / efac;
-    b.vy = n * (1.0 + ecosf) * (1.0 + ecosf) / efac / efac / efac;
-    b.vz = 0.0;
-}
 
 // ------------------------------------------------------------ x sweep, gas
 // One block = TX interior cells of one azimuthal row (the whole row when
@@ -595,304 +252,208 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
     }
 }
 
-// ------------------------------------------------------- azimuthal ring
-// One block = one radius.  Stage A sweeps the ring in chunks of B cells
-// (periodic halo) and writes each updated cell to its orbitally shifted
-// position (j + inc) mod ny of an LDS ring; stage B advects that ring by the
-// residual speed and writes the result to HBM.
-template <typename real, int B>
-struct RingLds {
-    static constexpr int S = B + 2 * NPAD;
-    real* ring;  // 3*ny
-    real* b;     // chunk scratch: 14 arrays of S
-    __device__ __forceinline__ real* c0() const { return b; }
-    __device__ __forceinline__ real* c1() const { return b + S; }
-    __device__ __forceinline__ real* c2() const { return b + 2 * S; }
-    __device__ __forceinline__ real* fl() const { return b + 3 * S; }
-    __device__ __forceinline__ real* sl0() const { return b + 4 * S; }
-    __device__ __forceinline__ real* sr0() const { return b + 5 * S; }
-    __device__ __forceinline__ real* sl1() const { return b + 6 * S; }
-    __device__ __forceinline__ real* sr1() const { return b + 7 * S; }
-    __device__ __forceinline__ real* sl2() const { return b + 8 * S; }
-    __device__ __forceinline__ real* sr2() const { return b + 9 * S; }
-    __device__ __forceinline__ real* F0() const { return b + 10 * S; }
-    __device__ __forceinline__ real* F1() const { return b + 11 * S; }
-    __device__ __forceinline__ real* F2() const { return b + 12 * S; }
-    __device__ __forceinline__ real* F3() const { return b + 13 * S; }
-};
 
-__device__ __forceinline__ int wrap(int j, int n) {
-    j %= n;
-    return j < 0 ? j + n : j;
-}
+// Tile geometry of the azimuthal pass for a thread count: TR radii x TY
+// output rows.  The rows of one radius are independent of the other radii,
+// so a tile is TR independent azimuthal strips side by side; TR consecutive
+// radii make each row of the tile one contiguous segment in HBM.
+template <int B> struct PhiTile;
+template <> struct PhiTile<256> { static constexpr int TR = 16, TY = 16; };
+template <> struct PhiTile<512> { static constexpr int TR = 16, TY = 32; };
+template <> struct PhiTile<1024> { static constexpr int TR = 32, TY = 32; };
 
-// parabolas of the three ring fields of slot s (flattening coefficient fl)
-template <typename real, int B>
-__device__ __forceinline__ void ring_slopes(const RingLds<real, B>& L, int s, real fl, const real* w) {
-    real l, r;
-    const real* c0 = L.c0();
-    const real* c1 = L.c1();
-    const real* c2 = L.c2();
-    slopes(c0[s - 2], c0[s - 1], c0[s], c0[s + 1], c0[s + 2], w[0], w[1], w[2], w[3], w[4], w[5], fl, l, r);
-    L.sl0()[s] = l; L.sr0()[s] = r;
-    slopes(c1[s - 2], c1[s - 1], c1[s], c1[s + 1], c1[s + 2], w[0], w[1], w[2], w[3], w[4], w[5], fl, l, r);
-    L.sl1()[s] = l; L.sr1()[s] = r;
-    slopes(c2[s - 2], c2[s - 1], c2[s], c2[s + 1], c2[s + 2], w[0], w[1], w[2], w[3], w[4], w[5], fl, l, r);
-    L.sl2()[s] = l; L.sr2()[s] = r;
-}
-
-// Stage B (residual advection, advection.cu:1-105) over the shifted ring;
-// `dust` selects update_Dust's floor (advection.cu:315) over
-// update_Cell_advect (advection.cu:88-105, no floor).
-template <typename real, int B>
-__device__ __forceinline__ void ring_advect(const DiskArgs<real>& d, const RingLds<real, B>& L, int i, real* __restrict__ out,
-                                            real speed, real frac, real fad, bool dust, real rfloor, const real* w) {
-    const int nx = d.nx, ny = d.ny;
-    const size_t N = (size_t)nx * ny;
-    const int t = threadIdx.x;
-    const real q = lim01(frac);
-    for (int c0 = 0; c0 < ny; c0 += B) {
-        const int nt = min(B, ny - c0);
-        const int ns = nt + 2 * NPAD;
-        for (int s = t; s < ns; s += B) {
-            const int jj = wrap(c0 - NPAD + s, ny);
-            L.c0()[s] = L.ring[jj];
-            L.c1()[s] = L.ring[ny + jj];
-            L.c2()[s] = L.ring[2 * ny + jj];
-        }
-        __syncthreads();
-        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, real(0), w);
-        __syncthreads();
-        for (int k = t + NPAD; k < ns - 2; k += B) {
-            real Dr, Dv, Du;
-            if (speed > real(0)) {
-                const int c = k - 1;
-                Dr = aveR(q, L.sl0()[c], L.c0()[c], L.sr0()[c]);
-                Dv = aveR(q, L.sl1()[c], L.c1()[c], L.sr1()[c]) * Dr;
-                Du = aveR(q, L.sl2()[c], L.c2()[c], L.sr2()[c]) * Dr;
-            } else {
-                const int c = k;
-                Dr = aveL(q, L.sl0()[c], L.c0()[c], L.sr0()[c]);
-                Dv = aveL(q, L.sl1()[c], L.c1()[c], L.sr1()[c]) * Dr;
-                Du = aveL(q, L.sl2()[c], L.c2()[c], L.sr2()[c]) * Dr;
-            }
-            L.F0()[k] = Dr * speed; L.F1()[k] = Dv * speed; L.F2()[k] = Du * speed;
-        }
-        __syncthreads();
-        for (int s = t + NPAD; s < ns - NPAD; s += B) {
-            const int jj = c0 - NPAD + s;
-            const real Dr = (L.F0()[s] - L.F0()[s + 1]) * fad;
-            const real Dv = (L.F1()[s] - L.F1()[s + 1]) * fad;
-            const real Du = (L.F2()[s] - L.F2()[s + 1]) * fad;
-            const real r = L.c0()[s];
-            real rn = r + Dr;
-            real vn = L.c1()[s], un = L.c2()[s];
-            if (dust && rn < fmax(r * real(1.0e-10), rfloor)) {
-                rn = fmax(r * real(1.0e-10), rfloor);
-            } else {
-                const real irn = FM<real>::rcp(rn);
-                vn = (r * L.c1()[s] + Dv) * irn;
-                un = (r * L.c2()[s] + Du) * irn;
-            }
-            const size_t o = (size_t)jj * nx + i;
-            out[o] = rn;
-            out[N + o] = un;
-            out[2 * N + o] = vn;
-        }
-        __syncthreads();
-    }
-}
-
-template <typename real, int B>
-__device__ __forceinline__ void ring_copy_out(const DiskArgs<real>& d, const RingLds<real, B>& L, int i, real* __restrict__ out) {
-    const int nx = d.nx, ny = d.ny;
-    const size_t N = (size_t)nx * ny;
-    for (int jj = threadIdx.x; jj < ny; jj += B) {
-        const size_t o = (size_t)jj * nx + i;
-        out[o] = L.ring[jj]; out[N + o] = L.ring[2 * ny + jj]; out[2 * N + o] = L.ring[ny + jj];
-    }
-}
-
-// flags: bit 0 = sweep stage, bit 1 = orbital shift + residual advection
-template <typename real, int B>
-__global__ __launch_bounds__(B) void k_ring_gas(LaunchSet<real> ls, int par, int flags) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    real* sm = reinterpret_cast<real*>(smem_raw);
+// Azimuthal pass of one fluid over a tile, fused with the orbital shift and
+// the residual advection: sweepy_inplace (sweeps.cu:361-456) or
+// sweepy_dust_inplace (dust.cu:260-353), then shift_orbital_advection
+// (orbit.cu:17-49) and advecty / advecty_dust (advection.cu:1-357).
+//
+// Output row y (after the shift) of radius i is the residual advection of
+// shifted cells y-3..y+3, i.e. of swept cells y-inc-3..y-inc+3, which need
+// input rows y-inc-6..y-inc+6.  So the tile loads, for each radius, TY+12
+// input rows starting at y0-6-inc(i): LDS row k of that radius holds input
+// row y0-6-inc+k, which the shift carries to row y0-6+k.  Stage A sweeps LDS
+// rows [3, TY+9); stage B advects rows [6, TY+6) and writes them to rows
+// y0..y0+TY-1, the same rows for every radius, so the stores are contiguous.
+// flags: bit 0 = azimuthal sweep, bit 1 = orbital shift + residual advection.
+template <typename real, bool DUST, int B>
+__global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flags) {
+    constexpr int TR = PhiTile<B>::TR, TY = PhiTile<B>::TY;
+    constexpr int NK = TY + 4 * NPAD;          // LDS rows per radius
+    constexpr int E = NK * TR;                 // LDS cells per array
+    __shared__ real c0[E], c1[E], c2[E], fl[E];
+    __shared__ real sl0[E], sr0[E], sl1[E], sr1[E], sl2[E], sr2[E];
+    __shared__ real F0[E], F1[E], F2[E], F3[E];
     const int2 w = ls.wl_ring[blockIdx.x];
-    const int i = w.y;
-    if (i < 0) return;
+    if (w.y < 0) return;   // padding entry of the work list (whole block)
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
+    const int ntr = (nx + TR - 1) / TR;
+    const int ty = w.y / ntr;
+    const int i0 = (w.y - ty * ntr) * TR;
+    const int y0 = ty * TY;
     const int t = threadIdx.x;
     const bool sweep_on = flags & 1;
     const bool orbadv = (flags & 2) && d.orbadv;
-    RingLds<real, B> L{sm, sm + 3 * ny};
-    const int ip = i + NPAD;
-    const real cs2 = TAB(T_CS2, ip);
-    const real cs0 = TAB(T_C0, ip);
-    const real ic0 = TAB(T_IC0, ip);
+    // per-lane radius: lanes of one wave cover TR radii x (64/TR) rows
+    const int ri = t % TR;
+    const int i = i0 + ri;
+    const bool live = i < nx;
+    const int ic = live ? i : nx - 1;          // clamped for table reads
+    const int ip = ic + NPAD;
     const real iR = TAB(T_IR, ip);
     const real dt = (real)d.ctl->dt;
-    const real dts = dt * iR;          // set_state receives dt/rad (riemann.cu:111)
+    const real dts = dt * iR;                  // set_state / dust_flow receive dt/rad
     const real idy = d.idy;
-    const real fup = dt * idy * iR;    // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
-    const int inc = orbadv ? d.inc[i] : 0;
-    const real* __restrict__ in = d.gas[gsel(par)];
+    const real fup = dt * idy * iR;            // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
+    const int inc = orbadv ? d.inc[ic] : 0;
+    const real* __restrict__ in = DUST ? d.dust[dsel(par)] : d.gas[gsel(par)];
+    real* __restrict__ out = DUST ? d.dust[dsel(par) ^ 1] : d.gas[gsel(par) ^ 1];
     const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
+    const real rfloor = d.D_G_ratio * real(1.0e-10);
+    // B is a multiple of TR, so lane t keeps radius ri in every loop below
+    static_assert(B % TR == 0, "tile");
 
-    // ---- stage A: azimuthal sweep (rho, vp normal, u transverse) ----
-    for (int c0 = 0; c0 < ny; c0 += B) {
-        const int nt = min(B, ny - c0);
-        const int ns = nt + 2 * NPAD;
-        for (int s = t; s < ns; s += B) {
-            const int jj = wrap(c0 - NPAD + s, ny);
-            const size_t o = (size_t)jj * nx + i;
-            L.c0()[s] = in[o];            // rho
-            L.c1()[s] = in[2 * N + o];    // vp: azimuthal velocity in the frame of rot + R*Omega_f
-            L.c2()[s] = in[N + o];        // u: radial velocity (transverse here)
+    // ---- load: TY+12 rows of each radius, shifted back by its inc ----
+    const int jbase = y0 - 2 * NPAD - inc;
+    for (int e = t; e < E; e += B) {
+        const int kk = e / TR;
+        real r = real(1), vp = real(0), u = real(0);
+        if (live) {
+            const size_t o = (size_t)wrap(jbase + kk, ny) * nx + i;
+            r = in[o];
+            vp = in[2 * N + o];   // azimuthal: normal to the interfaces
+            u = in[N + o];        // radial: transverse here
         }
-        __syncthreads();
-        for (int s = t + 2; s < ns - 2; s += B) {
-            // under p = c_s^2 rho with c_s^2 fixed on the ring, flatten's
-            // pressure ratios equal the density ratios
-            const real* c0p = L.c0();
-            const real* c1p = L.c1();
-            L.fl()[s] = flatten5(c0p[s - 2], c0p[s - 1], c0p[s], c0p[s + 1], c0p[s + 2],
-                                 c1p[s - 1], c1p[s], c1p[s + 1], cs0);
-        }
-        __syncthreads();
-        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, L.fl()[s], wy);
-        __syncthreads();
-        for (int k = t + NPAD; k < ns - 2; k += B) {
-            real rl, ul, vl, rr, ur, vr;
-            // p = c_s^2 rho with c_s^2 fixed on the ring: the pressure
-            // parabola is c_s^2 times the density parabola
-            trace_left_c(cs0, ic0, dts, idy, L.c0()[k - 1], L.c1()[k - 1], L.sl0()[k - 1], L.sr0()[k - 1],
-                         L.sl1()[k - 1], L.sr1()[k - 1], L.sl2()[k - 1], L.c2()[k - 1], L.sr2()[k - 1], rl, ul, vl);
-            trace_right_c(cs0, ic0, dts, idy, L.c0()[k], L.c1()[k], L.sl0()[k], L.sr0()[k],
-                          L.sl1()[k], L.sr1()[k], L.sl2()[k], L.c2()[k], L.sr2()[k], rr, ur, vr);
-            real Fr, Fu, Fw, pres;
-            hllc_c(cs0, ic0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
-            L.F0()[k] = Fr; L.F1()[k] = Fu; L.F2()[k] = Fw; L.F3()[k] = pres;
-        }
-        __syncthreads();
-        for (int s = t + NPAD; s < ns - NPAD; s += B) {
-            const int jj = c0 - NPAD + s;
-            const real Dr = (L.F0()[s] - L.F0()[s + 1]) * fup;
-            const real Du = (L.F1()[s] - L.F1()[s + 1] + (L.F3()[s] - L.F3()[s + 1])) * fup;
-            const real Dw = (L.F2()[s] - L.F2()[s + 1]) * fup;
-            const real r = L.c0()[s], vp = L.c1()[s], u = L.c2()[s];
-            real rn = r + Dr;
-            const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
-            real vn = vp, un = u;
-            if (!sweep_on) {
-                rn = r;
-            } else if (rn < rmin) {
-                rn = rmin;
-            } else {
-                // (rho v + Du + (rot + R Omega_f) Dr) / rho' - vbg == (rho vp + Du) / rho'
-                const real irn = FM<real>::rcp(rn);
-                vn = (r * vp + Du) * irn;
-                un = (r * u + Dw) * irn;
-            }
-            const int js = wrap(jj + inc, ny);
-            L.ring[js] = rn;
-            L.ring[ny + js] = vn;
-            L.ring[2 * ny + js] = un;
+        c0[e] = r; c1[e] = vp; c2[e] = u;
+    }
+    __syncthreads();
+    // ---- stage A: azimuthal sweep of LDS rows [3, TY+9) ----
+    if (!DUST) {
+        const real cs0 = TAB(T_C0, ip);
+        for (int e = t + 2 * TR; e < (NK - 2) * TR; e += B) {
+            // p = c_s^2 rho with c_s^2 fixed along the strip: flatten's
+            // pressure ratios are the density ratios
+            fl[e] = flatten5(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR],
+                             c1[e - TR], c1[e], c1[e + TR], cs0);
         }
         __syncthreads();
     }
+    for (int e = t + 2 * TR; e < (NK - 2) * TR; e += B) {
+        const real f = DUST ? real(0) : fl[e];
+        real l, r;
+        slopes(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], f, l, r);
+        sl0[e] = l; sr0[e] = r;
+        slopes(c1[e - 2 * TR], c1[e - TR], c1[e], c1[e + TR], c1[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], f, l, r);
+        sl1[e] = l; sr1[e] = r;
+        slopes(c2[e - 2 * TR], c2[e - TR], c2[e], c2[e + TR], c2[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], f, l, r);
+        sl2[e] = l; sr2[e] = r;
+    }
+    __syncthreads();
+    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
+        const int m = e - TR;   // the cell left of interface e
+        if (DUST) {
+            real Fr, Fu, Fw;
+            dust_iface(dts, idy, c0[m], sl0[m], sr0[m], c1[m], sl1[m], sr1[m], c2[m], sl2[m], sr2[m],
+                       idy, c0[e], sl0[e], sr0[e], c1[e], sl1[e], sr1[e], c2[e], sl2[e], sr2[e], Fr, Fu, Fw);
+            F0[e] = Fr; F1[e] = Fu; F2[e] = Fw;
+        } else {
+            const real cs2 = TAB(T_CS2, ip), cs0 = TAB(T_C0, ip), ic0 = TAB(T_IC0, ip);
+            real rl, ul, vl, rr, ur, vr;
+            // the pressure parabola is c_s^2 times the density parabola
+            trace_left_c(cs0, ic0, dts, idy, c0[m], c1[m], sl0[m], sr0[m], sl1[m], sr1[m], sl2[m], c2[m], sr2[m],
+                         rl, ul, vl);
+            trace_right_c(cs0, ic0, dts, idy, c0[e], c1[e], sl0[e], sr0[e], sl1[e], sr1[e], sl2[e], c2[e], sr2[e],
+                          rr, ur, vr);
+            real Fr, Fu, Fw, pres;
+            hllc_c(cs0, ic0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
+            F0[e] = Fr; F1[e] = Fu; F2[e] = Fw; F3[e] = pres;
+        }
+    }
+    __syncthreads();
+    for (int e = t + NPAD * TR; e < (NK - NPAD) * TR; e += B) {
+        const real Dr = (F0[e] - F0[e + TR]) * fup;
+        const real Du = DUST ? (F1[e] - F1[e + TR]) * fup
+                             : (F1[e] - F1[e + TR] + (F3[e] - F3[e + TR])) * fup;
+        const real Dw = (F2[e] - F2[e + TR]) * fup;
+        const real r = c0[e], vp = c1[e], u = c2[e];
+        real rn = r + Dr;
+        const real rmin = fmax(r * real(1.0e-10), DUST ? rfloor : real(1.0e-10));
+        real vn = vp, un = u;
+        if (!sweep_on) {
+            rn = r;
+        } else if (rn < rmin) {
+            rn = rmin;
+        } else {
+            // gas: (rho v + Du + (rot + R Omega_f) Dr) / rho' - vbg == (rho vp + Du) / rho'
+            const real irn = FM<real>::rcp(rn);
+            vn = (r * vp + Du) * irn;
+            un = (r * u + Dw) * irn;
+        }
+        c0[e] = rn; c1[e] = vn; c2[e] = un;   // each thread rewrites only its own cells
+    }
+    __syncthreads();
+    const int rows = min(TY, ny - y0);
     if (!orbadv) {
-        ring_copy_out(d, L, i, d.gas[gsel(par) ^ 1]);
+        for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
+            const int y = e / TR - 2 * NPAD;
+            if (!live || y >= rows) continue;
+            const size_t o = (size_t)(y0 + y) * nx + i;
+            out[o] = c0[e]; out[N + o] = c2[e]; out[2 * N + o] = c1[e];
+        }
         return;
     }
     // ---- stage B: residual advection at speed res (advection.cu:1-105) ----
-    const real speed = d.res[i];
-    const real frac = fabs(speed) * dts * idy;   // |speed| dt / (R dy)
-    ring_advect(d, L, i, d.gas[gsel(par) ^ 1], speed, frac, fup, false, real(0), wy);
-}
-
-// flags: bit 0 = sweep stage, bit 1 = orbital shift + residual advection
-template <typename real, int B>
-__global__ __launch_bounds__(B) void k_ring_dust(LaunchSet<real> ls, int par, int flags) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    real* sm = reinterpret_cast<real*>(smem_raw);
-    const int2 w = ls.wl_ring[blockIdx.x];
-    const int i = w.y;
-    if (i < 0) return;
-    const DiskArgs<real>& d = ls.disks[w.x];
-    const int nx = d.nx, ny = d.ny;
-    const size_t N = (size_t)nx * ny;
-    const int t = threadIdx.x;
-    const bool sweep_on = flags & 1;
-    const bool orbadv = (flags & 2) && d.orbadv;
-    RingLds<real, B> L{sm, sm + 3 * ny};
-    const int ip = i + NPAD;
-    const real iR = TAB(T_IR, ip);
-    const real dt = (real)d.ctl->dt;
-    const real dts = dt * iR;          // dust_flow: dt /= rad for geom > 2 (dust.cu:127)
-    const real idy = d.idy;
-    const real f12 = dt * idy * iR;    // dust.cu:309-313
-    const int inc = orbadv ? d.inc[i] : 0;
-    const real* __restrict__ in = d.dust[dsel(par)];
-    const real rfloor = d.D_G_ratio * real(1.0e-10);
-    const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
-
-    for (int c0 = 0; c0 < ny; c0 += B) {
-        const int nt = min(B, ny - c0);
-        const int ns = nt + 2 * NPAD;
-        for (int s = t; s < ns; s += B) {
-            const int jj = wrap(c0 - NPAD + s, ny);
-            const size_t o = (size_t)jj * nx + i;
-            L.c0()[s] = in[o];
-            L.c1()[s] = in[2 * N + o];
-            L.c2()[s] = in[N + o];
-        }
-        __syncthreads();
-        for (int s = t + 2; s < ns - 2; s += B) ring_slopes(L, s, real(0), wy);
-        __syncthreads();
-        for (int k = t + NPAD; k < ns - 2; k += B) {
-            real Fr, Fu, Fw;
-            dust_iface(dts, idy, L.c0()[k - 1], L.sl0()[k - 1], L.sr0()[k - 1], L.c1()[k - 1], L.sl1()[k - 1],
-                       L.sr1()[k - 1], L.c2()[k - 1], L.sl2()[k - 1], L.sr2()[k - 1],
-                       idy, L.c0()[k], L.sl0()[k], L.sr0()[k], L.c1()[k], L.sl1()[k], L.sr1()[k],
-                       L.c2()[k], L.sl2()[k], L.sr2()[k], Fr, Fu, Fw);
-            L.F0()[k] = Fr; L.F1()[k] = Fu; L.F2()[k] = Fw;
-        }
-        __syncthreads();
-        for (int s = t + NPAD; s < ns - NPAD; s += B) {
-            const int jj = c0 - NPAD + s;
-            const real Dr = (L.F0()[s] - L.F0()[s + 1]) * f12;
-            const real Du = (L.F1()[s] - L.F1()[s + 1]) * f12;
-            const real Dw = (L.F2()[s] - L.F2()[s + 1]) * f12;
-            const real r = L.c0()[s], vp = L.c1()[s], u = L.c2()[s];
-            real rn = r + Dr;
-            const real rmin = fmax(r * real(1.0e-10), rfloor);
-            real vn = vp, un = u;
-            if (!sweep_on) {
-                rn = r;
-            } else if (rn < rmin) {
-                rn = rmin;
-            } else {
-                const real irn = FM<real>::rcp(rn);
-                vn = (r * vp + Du) * irn;
-                un = (r * u + Dw) * irn;
-            }
-            const int js = wrap(jj + inc, ny);
-            L.ring[js] = rn;
-            L.ring[ny + js] = vn;
-            L.ring[2 * ny + js] = un;
-        }
-        __syncthreads();
+    const real speed = d.res[ic];
+    const real q = lim01(fabs(speed) * dts * idy);   // |speed| dt / (R dy)
+    for (int e = t + (2 * NPAD - 1) * TR; e < (2 * NPAD + TY + 1) * TR; e += B) {
+        real l, r;
+        slopes(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], real(0), l, r);
+        sl0[e] = l; sr0[e] = r;
+        slopes(c1[e - 2 * TR], c1[e - TR], c1[e], c1[e + TR], c1[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], real(0), l, r);
+        sl1[e] = l; sr1[e] = r;
+        slopes(c2[e - 2 * TR], c2[e - TR], c2[e], c2[e + TR], c2[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], real(0), l, r);
+        sl2[e] = l; sr2[e] = r;
     }
-    if (!orbadv) {
-        ring_copy_out(d, L, i, d.dust[dsel(par) ^ 1]);
-        return;
+    __syncthreads();
+    for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY + 1) * TR; e += B) {
+        real Dr, Dv, Du;
+        if (speed > real(0)) {
+            const int m = e - TR;
+            Dr = aveR(q, sl0[m], c0[m], sr0[m]);
+            Dv = aveR(q, sl1[m], c1[m], sr1[m]) * Dr;
+            Du = aveR(q, sl2[m], c2[m], sr2[m]) * Dr;
+        } else {
+            Dr = aveL(q, sl0[e], c0[e], sr0[e]);
+            Dv = aveL(q, sl1[e], c1[e], sr1[e]) * Dr;
+            Du = aveL(q, sl2[e], c2[e], sr2[e]) * Dr;
+        }
+        F0[e] = Dr * speed; F1[e] = Dv * speed; F2[e] = Du * speed;
     }
-    const real speed = d.res[i];
-    const real frac = fabs(speed) * dts * idy;
-    ring_advect(d, L, i, d.dust[dsel(par) ^ 1], speed, frac, f12, true, rfloor, wy);
+    __syncthreads();
+    for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
+        const int y = e / TR - 2 * NPAD;
+        if (!live || y >= rows) continue;
+        const real Dr = (F0[e] - F0[e + TR]) * fup;
+        const real Dv = (F1[e] - F1[e + TR]) * fup;
+        const real Du = (F2[e] - F2[e + TR]) * fup;
+        const real r = c0[e];
+        real rn = r + Dr;
+        real vn = c1[e], un = c2[e];
+        // update_Dust floors (advection.cu:315); update_Cell_advect has none
+        // (advection.cu:88-105)
+        if (DUST && rn < fmax(r * real(1.0e-10), rfloor)) {
+            rn = fmax(r * real(1.0e-10), rfloor);
+        } else {
+            const real irn = FM<real>::rcp(rn);
+            vn = (r * c1[e] + Dv) * irn;
+            un = (r * c2[e] + Du) * irn;
+        }
+        const size_t o = (size_t)(y0 + y) * nx + i;
+        out[o] = rn;
+        out[N + o] = un;
+        out[2 * N + o] = vn;
+    }
 }
 
 // ----------------------------------------------------------------- viscosity
@@ -970,26 +531,7 @@ __global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, 
     d.vis[o] = fx;
     d.vis[N + o] = fy;
 }
-
 // ------------------------------------------------------------- sources
-// Planet acceleration (star_planet_grav_cyl, forces.cu:41-58, z = 0) at
-// angle phi_j from the tabulated sin/cos of phi_j and phi_j/2 and the
-// planet's (sin, cos) of phi_p and phi_p/2.  The squared distance is written
-// without cancellation: (R - a)^2 + 4 a R sin^2((phi - phi_p)/2) + rs^2.
-template <typename real>
-__device__ __forceinline__ void planet_acc(const PlanetK& pk, const real* pt, real R, real& ax, real& ay) {
-    const real sp = (real)pk.sphi, cp = (real)pk.cphi, shp = (real)pk.shphi, chp = (real)pk.chphi;
-    const real s = pt[0] * cp - pt[1] * sp;      // sin(phi - phi_p)
-    const real c = pt[1] * cp + pt[0] * sp;      // cos(phi - phi_p)
-    const real h = pt[2] * chp - pt[3] * shp;    // sin((phi - phi_p)/2)
-    const real px = (real)pk.x, m = (real)pk.m, rs = (real)pk.rs;
-    const real Rp2 = (R - px) * (R - px) + real(4) * px * R * h * h + rs * rs;
-    const real iRp = FM<real>::rcp(FM<real>::sqrt(Rp2));
-    const real k = iRp * iRp * iRp;
-    const real ind = (real)pk.m_over_x2;
-    ax = -m * (R - px * c) * k - c * ind;
-    ay = -m * px * s * k + s * ind;
-}
 
 // Block max of a non-negative value, written to this block's slot of the
 // partial-max array (reduced by the prologue or k_rate_reduce): one store per
@@ -1178,8 +720,9 @@ __host__ __device__ inline PlanetK planet_kernel_view(const Body& b) {
 // mode 1 (device loop): finish the previous step (t += dt), take
 // dt = min(CFL/rate, 1.1 dt_prev) clipped to end_time, record it.  Then the
 // planet at t and t+dt and the orbital shift of every radius.
+// mode 2 (replay of a recorded step, one disk): t = td[0], dt = td[1].
 template <typename real>
-__global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode, int do_orbadv) {
+__global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode, int do_orbadv, const double* td) {
     const DiskArgs<real>& d = ls.disks[blockIdx.x];
     double* dts = d.dts;
     StepCtl* ctl = d.ctl;
@@ -1187,6 +730,12 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
     double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
     if (threadIdx.x == 0) {
         double t = ctl->t, dt = ctl->dt;
+        if (mode == 2) {
+            t = td[0];
+            dt = td[1];
+            ctl->t = t;
+            ctl->dt = dt;
+        }
         if (mode == 1) {
             if (ctl->nstep > 0) t += dt;
             ctl->rate_bits = (unsigned long long)__double_as_longlong(rate);
@@ -1263,16 +812,10 @@ static hipError_t sweep_x_any(const LaunchSet<real>& ls, int par, bool dust, hip
 
 template <typename real, int B>
 static hipError_t launch_ring(const LaunchSet<real>& ls, int par, int flags, bool dust, hipStream_t s) {
-    size_t lds = ((size_t)3 * ls.ny_max + (size_t)14 * (B + 2 * NPAD)) * sizeof(real);
-    if (dust) {
-        hipError_t e = set_lds(k_ring_dust<real, B>, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_ring_dust<real, B>), dim3(ls.n_ring), dim3(B), lds, s, ls, par, flags);
-    } else {
-        hipError_t e = set_lds(k_ring_gas<real, B>, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_ring_gas<real, B>), dim3(ls.n_ring), dim3(B), lds, s, ls, par, flags);
-    }
+    if (dust)
+        hipLaunchKernelGGL((k_phi<real, true, B>), dim3(ls.n_ring), dim3(B), 0, s, ls, par, flags);
+    else
+        hipLaunchKernelGGL((k_phi<real, false, B>), dim3(ls.n_ring), dim3(B), 0, s, ls, par, flags);
     return hipGetLastError();
 }
 
@@ -1287,7 +830,8 @@ static hipError_t ring_any(const LaunchSet<real>& ls, int par, int flags, bool d
 
 template <typename real>
 struct Launch {
-    static size_t ring_lds(int ny, int B) { return ((size_t)3 * ny + (size_t)14 * (B + 2 * NPAD)) * sizeof(real); }
+    static int phi_tile_radii(int B) { return B == 256 ? PhiTile<256>::TR : B == 1024 ? PhiTile<1024>::TR : PhiTile<512>::TR; }
+    static int phi_tile_rows(int B) { return B == 256 ? PhiTile<256>::TY : B == 1024 ? PhiTile<1024>::TY : PhiTile<512>::TY; }
     static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, hipStream_t s) { return sweep_x_any(ls, par, false, s); }
     static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, hipStream_t s) { return sweep_x_any(ls, par, true, s); }
     static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, false, s); }
@@ -1309,7 +853,11 @@ struct Launch {
         return hipGetLastError();
     }
     static hipError_t prologue(const LaunchSet<real>& ls, int mode, int do_orbadv, hipStream_t s) {
-        hipLaunchKernelGGL(k_prologue<real>, dim3(ls.ndisk), dim3(1024), 0, s, ls, mode, do_orbadv);
+        hipLaunchKernelGGL(k_prologue<real>, dim3(ls.ndisk), dim3(1024), 0, s, ls, mode, do_orbadv, (const double*)nullptr);
+        return hipGetLastError();
+    }
+    static hipError_t prologue_replay(const LaunchSet<real>& ls, const double* td, hipStream_t s) {
+        hipLaunchKernelGGL(k_prologue<real>, dim3(1), dim3(1024), 0, s, ls, 2, 1, td);
         return hipGetLastError();
     }
 };

@@ -10,7 +10,8 @@ namespace pg {
 
 template <typename real>
 struct Launch {
-    static size_t ring_lds(int ny, int B);
+    static int phi_tile_radii(int B);
+    static int phi_tile_rows(int B);
     static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
@@ -20,6 +21,18 @@ struct Launch {
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t rate_reduce(const LaunchSet<real>& ls, hipStream_t s);
     static hipError_t prologue(const LaunchSet<real>& ls, int mode, int do_orbadv, hipStream_t s);
+    // replay of a recorded step: t = td[0], dt = td[1] (one disk)
+    static hipError_t prologue_replay(const LaunchSet<real>& ls, const double* td, hipStream_t s);
+};
+
+// Adjoint passes (pg_adjoint.hip), one disk, grid from nx, ny.
+template <typename real>
+struct AdjLaunch {
+    static hipError_t x(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
+    static hipError_t phi(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
+    static hipError_t adv(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
+    static hipError_t vis(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
+    static hipError_t src(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
 };
 
 }  // namespace pg

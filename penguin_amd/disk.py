@@ -50,6 +50,9 @@ SIGNATURES = {
     "pg_write_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
     "pg_read_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
     "pg_snapshot_label": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_size_t]),
+    "pg_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]),
+    "pg_adjoint_backward": (ctypes.c_int, [_vp, _dp, _vp]),
+    "pg_adjoint_pass": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
     "pg_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(_vp)]),
     "pg_batch_destroy": (ctypes.c_int, [_vp]),
     "pg_batch_size": (ctypes.c_int, [_vp, _ip]),
@@ -63,6 +66,11 @@ SIGNATURES = {
     "pg_batch_profile_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int)]),
     "pg_batch_device_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
 }
+
+
+class Grads(ctypes.Structure):
+    """pg_grads (include/penguin_amd.h)."""
+    _fields_ = [("dq", ctypes.c_double), ("dalpha", ctypes.c_double), ("dstate0", _dp)]
 
 
 class PenguinError(RuntimeError):
@@ -208,6 +216,38 @@ class Disk:
         b = np.zeros(10)
         _check(load_library().pg_planet(self._h, t, _ptr(b)))
         return b
+
+    # ---- adjoint (pg_adjoint_forward / pg_adjoint_backward) ---------------
+    def adjoint_forward(self, nsteps: int, ckpt_every: int = 20) -> np.ndarray:
+        """nsteps of the device loop, keeping checkpoints for the adjoint;
+        returns the step sizes (frozen for the backward sweep)."""
+        dts = np.zeros(nsteps)
+        _check(load_library().pg_adjoint_forward(self._h, nsteps, ckpt_every, self.t, self.dt, _ptr(dts)))
+        self.t = self.t + float(np.sum(dts))
+        self.dt = float(dts[-1])
+        return dts
+
+    def adjoint_backward(self, dL_dgas: np.ndarray):
+        """dL/d(final gas state) (5, ny, nx) -> (dL/dq, dL/dalpha, dL/d(start
+        state) (5, ny, nx))."""
+        dL = np.ascontiguousarray(dL_dgas, dtype=np.float64)
+        assert dL.shape == (5, self.ny, self.nx)
+        d0 = np.zeros((5, self.ny, self.nx))
+        g = Grads(0.0, 0.0, _ptr(d0))
+        _check(load_library().pg_adjoint_backward(self._h, _ptr(dL), ctypes.byref(g)))
+        return g.dq, g.dalpha, d0
+
+    ADJOINT_PASSES = ("sweep_x", "sweep_y", "orbital_advection", "viscosity", "kick", "predictor", "corrector")
+
+    def adjoint_pass(self, name: str, t: float, dt: float, lam_out: np.ndarray):
+        """Transpose of one module call at the current state: lam_out
+        (3 or 2, ny, nx) -> (lam_in (3, ny, nx) over rho, u, v; (dL/dq, dL/dalpha))."""
+        k = self.ADJOINT_PASSES.index(name)
+        lo = np.ascontiguousarray(lam_out, dtype=np.float64)
+        li = np.zeros((3, self.ny, self.nx))
+        dp = np.zeros(2)
+        _check(load_library().pg_adjoint_pass(self._h, k, t, dt, _ptr(lo), _ptr(li), _ptr(dp)))
+        return li, dp
 
     # ---- snapshots (output.cu:125-248) ----------------------------------
     def write_snapshot(self, filename: str, t: float = None):
