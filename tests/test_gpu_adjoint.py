@@ -6,7 +6,9 @@ finite differences of the forward run with the same (frozen) step sizes;
 FP64 within 1e-5 relative, the FP32 adjoint within 1e-3 of the FP64 one;
 and the autograd wrapper returns the same numbers.  The configurations
 switch the planet and the viscosity on and off so that a wrong pass shows
-up in the case that isolates it.
+up in the case that isolates it.  The state is smooth and resolved and
+p_alpha = 0.5 (distinct radial ghost velocities), so that no limiter or
+flattening switch sits within a finite-difference step of its threshold.
 """
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _params(prec, nr=64, nphi=96, **kw):
-    return config_c5(precision=prec, xres=nr, yres=nphi).replace(noise_rho=0.0, **kw)
+    return config_c5(precision=prec, xres=nr, yres=nphi).replace(noise_rho=0.0, p_alpha=0.5, **kw)
 
 
 def _grid(p):
@@ -47,9 +49,15 @@ def _weights(p, seed=3):
     return rng.uniform(-1.0, 1.0, (int(p.yres), int(p.xres)))
 
 
-def _adjoint(p, nsteps, ckpt, w, sigma0=None):
+def _adjoint(p, nsteps, ckpt, w, sigma0=None, smooth=True):
     d = Disk(p)
-    s0 = _smooth_state(d, p, sigma0)
+    if smooth:
+        s0 = _smooth_state(d, p, sigma0)
+    else:   # only Sigma replaced, as DiskEvolve does
+        g, _ = d.get_state()
+        g[0] = sigma0
+        d.set_state(g)
+        s0 = g[0].copy()
     dts = d.adjoint_forward(nsteps, ckpt)
     g, _ = d.get_state()
     L = float(np.sum(w * g[0]))
@@ -58,6 +66,13 @@ def _adjoint(p, nsteps, ckpt, w, sigma0=None):
     dq, dal, d0 = d.adjoint_backward(dL)
     d.close()
     return L, dts, dq, dal, d0, s0
+
+
+def _smooth_weights(p):
+    """Weights of a smooth loss (a function of radius, like the gradient of
+    the gap-profile loss)."""
+    R, PHI = _grid(p)
+    return np.sin(3 * np.log(R)) / int(p.yres)
 
 
 def _frozen(p, sigma0, dts, w):
@@ -72,25 +87,29 @@ def _frozen(p, sigma0, dts, w):
     return float(np.sum(w * g[0]))
 
 
-def _fd_checks(p, nsteps, ckpt, tol):
-    w = _weights(p)
+def _fd_checks(p, nsteps, ckpt, tol, w=None, p_fd=None):
+    """Adjoint of p against central differences of frozen-dt forward runs of
+    p_fd (default p; an FP64 twin to check an FP32 adjoint)."""
+    w = _weights(p) if w is None else w
+    pf = p if p_fd is None else p_fd
     L, dts, dq, dal, d0, s0 = _adjoint(p, nsteps, ckpt, w)
-    assert abs(_frozen(p, s0, dts, w) - L) <= 1e-12 * abs(L) + 1e-14, "frozen-dt replay differs from the forward"
+    if p_fd is None:
+        assert abs(_frozen(p, s0, dts, w) - L) <= 1e-12 * abs(L) + 1e-14, "frozen-dt replay differs from the forward"
     out = {}
     if p.n_planet and p.planet_mass > 0:
-        e = 1e-6 * p.planet_mass
-        fd = (_frozen(p.replace(planet_mass=p.planet_mass + e), s0, dts, w) -
-              _frozen(p.replace(planet_mass=p.planet_mass - e), s0, dts, w)) / (2 * e)
+        e = 1e-4 * p.planet_mass
+        fd = (_frozen(pf.replace(planet_mass=p.planet_mass + e), s0, dts, w) -
+              _frozen(pf.replace(planet_mass=p.planet_mass - e), s0, dts, w)) / (2 * e)
         out["q"] = (dq, fd)
     if p.visc_flag:
-        e = 1e-6 * p.ss_alpha
-        fd = (_frozen(p.replace(ss_alpha=p.ss_alpha + e), s0, dts, w) -
-              _frozen(p.replace(ss_alpha=p.ss_alpha - e), s0, dts, w)) / (2 * e)
+        e = 1e-4 * p.ss_alpha
+        fd = (_frozen(pf.replace(ss_alpha=p.ss_alpha + e), s0, dts, w) -
+              _frozen(pf.replace(ss_alpha=p.ss_alpha - e), s0, dts, w)) / (2 * e)
         out["alpha"] = (dal, fd)
     R, PHI = _grid(p)
     delta = s0 * (0.5 + 0.5 * np.sin(PHI + 3 * R))
     e = 1e-6
-    fd = (_frozen(p, s0 + e * delta, dts, w) - _frozen(p, s0 - e * delta, dts, w)) / (2 * e)
+    fd = (_frozen(pf, s0 + e * delta, dts, w) - _frozen(pf, s0 - e * delta, dts, w)) / (2 * e)
     out["sigma0"] = (float(np.sum(d0[0] * delta)), fd)
     for k, (a, f) in out.items():
         rel = abs(a - f) / max(abs(f), 1e-300)
@@ -111,16 +130,12 @@ def test_adjoint_segments_fp64():
     _fd_checks(_params(8), 7, 3, 1e-5)
 
 
-def test_adjoint_fp32_matches_fp64():
-    p8, p4 = _params(8), _params(4)
-    w = _weights(p8)
-    _, _, dq8, da8, d08, s0 = _adjoint(p8, 10, 4, w)
-    _, _, dq4, da4, d04, _ = _adjoint(p4, 10, 4, w, sigma0=s0)
-    rq = abs(dq4 - dq8) / abs(dq8)
-    ra = abs(da4 - da8) / abs(da8)
-    rs = float(np.linalg.norm(d04[0] - d08[0]) / np.linalg.norm(d08[0]))
-    print(f"fp32 vs fp64: q {rq:.2e} alpha {ra:.2e} sigma0 {rs:.2e}", flush=True)
-    assert rq <= 1e-3 and ra <= 1e-3 and rs <= 1e-3
+def test_adjoint_fp32_against_fp64_fd():
+    # the FP32 adjoint (forward, checkpoints and backward in FP32) against
+    # FP64 central differences with the FP32 run's step sizes, for a smooth
+    # loss; the bar is 1e-3 relative
+    p4 = _params(4)
+    _fd_checks(p4, 20, 5, 1e-3, w=_smooth_weights(p4), p_fd=_params(8))
 
 
 def test_autograd_wrapper():
@@ -129,7 +144,10 @@ def test_autograd_wrapper():
 
     p = _params(8)
     w = _weights(p)
-    L, dts, dq, dal, d0, s0 = _adjoint(p, 5, 2, w)
+    d = Disk(p)
+    s_in = _smooth_state(d, p)
+    d.close()
+    L, dts, dq, dal, d0, s0 = _adjoint(p, 5, 2, w, sigma0=s_in, smooth=False)
     sig = torch.tensor(s0, requires_grad=True)
     q = torch.tensor(p.planet_mass, dtype=torch.float64, requires_grad=True)
     al = torch.tensor(p.ss_alpha, dtype=torch.float64, requires_grad=True)
