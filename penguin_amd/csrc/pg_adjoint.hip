@@ -11,8 +11,8 @@
 // Each kernel recomputes the forward values of its tile from the stored
 // input of its pass, then runs the phases backwards.  Adjoints of stencil
 // inputs are gathered (each thread sums what its cell receives), so results
-// do not depend on thread order; only the two parameter adjoints (planet
-// mass, viscosity alpha) are summed with atomics.
+// do not depend on thread order; the two parameter adjoints (planet mass,
+// viscosity alpha) are summed per workgroup into slots that the host adds up.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -23,9 +23,11 @@
 
 namespace pg {
 
-// Sum a per-thread value over the block and add it to *dst (one atomic per
-// block).
-__device__ __forceinline__ void block_sum_add(double* dst, double v) {
+// Sum a per-thread value over the block and add it to this block's slot of
+// a parameter-adjoint array (slot = linear block index; launches on one
+// stream run one after the other, so each slot has one writer at a time and
+// no atomics are needed).  The host sums the slots at the end.
+__device__ __forceinline__ void block_sum_add(double* slots, double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __shared__ double wsum[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -35,7 +37,7 @@ __device__ __forceinline__ void block_sum_add(double* dst, double v) {
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) s += wsum[k];
-        if (s != 0.0) atomicAdd(dst, s);
+        if (s != 0.0) slots[blockIdx.x + (size_t)gridDim.x * blockIdx.y] += s;
     }
 }
 
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
             lalpha += (double)au_[s] * (double)TAB(T_G_U, ip);
         }
     }
-    if (A.alpha > 0.0) block_sum_add(A.lparam + 1, lalpha / A.alpha);
+    if (A.alpha > 0.0) block_sum_add(A.lparam + A.nslot, lalpha / A.alpha);
 }
 
 // ------------------------------------------------- azimuthal sweep adjoint
@@ -728,7 +730,7 @@ __global__ __launch_bounds__(256) void k_adj_vis_part(AdjArgs<real> A) {
         const int rn[5] = {0, 1, 2, 4, 7};
         for (int k = 0; k < 5; ++k) A.part[(size_t)(18 + k) * N + o] = ln[rn[k]] * nu[rn[k] % 3];
     }
-    if (A.alpha > 0.0) block_sum_add(A.lparam + 1, lal / A.alpha);
+    if (A.alpha > 0.0) block_sum_add(A.lparam + A.nslot, lal / A.alpha);
 }
 
 // Sum of the partials each input cell received; columns -1 and nx are the
@@ -770,7 +772,7 @@ __global__ __launch_bounds__(256) void k_adj_vis_gather(AdjArgs<real> A) {
             lal = (double)lu * (double)TAB(T_G_U, i + NPAD);
         }
     }
-    if (A.alpha > 0.0) block_sum_add(A.lparam + 1, lal / A.alpha);
+    if (A.alpha > 0.0) block_sum_add(A.lparam + A.nslot, lal / A.alpha);
 }
 
 // ---------------------------------------------------- source-term adjoint

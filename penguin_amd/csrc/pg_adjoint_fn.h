@@ -22,27 +22,21 @@ PG_HD real exp_lim_d(real x) { return (real(1) + x > real(1.0e-6)) ? real(1) : r
 template <typename real>
 PG_HD real sbit(real x) { return __builtin_signbit(x) ? real(-1) : real(1); }
 
-// aveL (PPM.cu:106-113): y = (a - sL) + q/2 ((sR + sL) + (1 - 2q/3) a6), a6 = 3 (sL - sR)
+// aveL (PPM.cu:106-113): y = (a - sL) + q ((2 sL - sR) - q (sL - sR))
 template <typename real>
 PG_HD void aveL_adj(real q, real sL, real a, real sR, real ly, real& lq, real& lsL, real& la, real& lsR) {
-    const real g = real(1) - q * real(2.0 / 3.0);
-    const real a6 = real(3) * (sL - sR);
-    const real hq = q * real(0.5);
     la += ly;
-    lsL += ly * (real(-1) + hq + real(3) * hq * g);
-    lsR += ly * (hq - real(3) * hq * g);
-    lq += ly * (real(0.5) * ((sR + sL) + g * a6) - q * real(1.0 / 3.0) * a6);
+    lsL += ly * (real(-1) + q * (real(2) - q));
+    lsR += ly * (q * (q - real(1)));
+    lq += ly * ((real(2) * sL - sR) - real(2) * q * (sL - sR));
 }
-// aveR (PPM.cu:95-104): y = (a + sR) - x/2 ((sR + sL) - (1 - 2x/3) a6)
+// aveR (PPM.cu:95-104): y = (a + sR) + x ((sL - 2 sR) - x (sL - sR))
 template <typename real>
 PG_HD void aveR_adj(real x, real sL, real a, real sR, real ly, real& lx, real& lsL, real& la, real& lsR) {
-    const real g = real(1) - x * real(2.0 / 3.0);
-    const real a6 = real(3) * (sL - sR);
-    const real hx = x * real(0.5);
     la += ly;
-    lsL += ly * (-hx + real(3) * hx * g);
-    lsR += ly * (real(1) - hx - real(3) * hx * g);
-    lx += ly * (real(-0.5) * ((sR + sL) - g * a6) - x * real(1.0 / 3.0) * a6);
+    lsL += ly * (x * (real(1) - x));
+    lsR += ly * (real(1) + x * (x - real(2)));
+    lx += ly * ((sL - real(2) * sR) - real(2) * x * (sL - sR));
 }
 
 // slopes(): adjoints of the four neighbour differences d0..d3
@@ -176,12 +170,13 @@ PG_HD void trace_right_adj(real c, real dt, real idx, const real* in, real lrr, 
     const real p0 = fmax(p0a, p * real(1.0e-10));
     const real u0 = aveL(q2, usL, u, usR);
     const real ir0 = FM<real>::rcp(r0);
-    const real cs = FM<real>::sqrt(p0 * ir0);
+    const real w0 = p0 * ir0;
+    const real ics = FM<real>::rsqrt(w0);
+    const real cs = w0 * ics;
     const real z3 = (cs * dt - dis) * idx;
     const real q3 = lim01(z3);
     const real u_ = aveL(q3, usL, u, usR);
     const real p_ = aveL(q3, psL, p, psR);
-    const real ics = FM<real>::rcp(cs);
     const real earg = (u0 - u_) * ics;
     const real e = exp_lim(earg);
     // reverse
@@ -239,12 +234,13 @@ PG_HD void trace_left_adj(real c, real dt, real idx, const real* in, real lrl, r
     const real p0 = fmax(p0a, p * real(1.0e-10));
     const real u0 = aveR(x2, usL, u, usR);
     const real ir0 = FM<real>::rcp(r0);
-    const real cs = FM<real>::sqrt(p0 * ir0);
+    const real w0 = p0 * ir0;
+    const real ics = FM<real>::rsqrt(w0);
+    const real cs = w0 * ics;
     const real z3 = (dis + cs * dt) * idx;
     const real x3 = lim01(z3);
     const real u_ = aveR(x3, usL, u, usR);
     const real p_ = aveR(x3, psL, p, psR);
-    const real ics = FM<real>::rcp(cs);
     const real earg = (u_ - u0) * ics;
     const real e = exp_lim(earg);
     real lx3 = 0, lx2 = 0, lx1 = 0;
@@ -459,15 +455,16 @@ PG_HD void hllc_iso_adj(real cl, real cr, const real* S, real lFr, real lus, rea
     const real cbar = FM<real>::sqrt(cq * iss + n2 * du * du);
     const real sl = ubar - cbar, sr = ubar + cbar;
     const real num = rl * ul * cl + rr * ur * cr, den = rl * cl + rr * cr;
-    const real irs = FM<real>::rcp(rl + rr);
-    const real smA = FM<real>::div(num, den);
-    const real sm = smA + real(2) * (pl - pr) * ics * irs;
+    const real X = FM<real>::rcp(den * (rl + rr));
+    const real irs = den * X;
+    const real smA = num * (rl + rr) * X;
+    const real sm = smA + real(2) * (pl - pr) * ics * den * X;
     real lpm = 0, lsl = 0, lsm = 0, lsr = 0;
     hllc_select_adj(pm, sl, sm, sr, S, lFr, lus, lvs, lpres, lS, lpm, lsl, lsm, lsr);
     const real lubar = lsl + lsr;
     const real lcbar = lsr - lsl;
     // sm
-    const real iden = FM<real>::rcp(den);
+    const real iden = (rl + rr) * X;
     const real lnum = lsm * iden, lden = -lsm * smA * iden;
     lS[0] += lnum * ul * cl + lden * cl;
     lS[2] += lnum * rl * cl;

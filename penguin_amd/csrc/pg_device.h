@@ -23,23 +23,42 @@ __device__ __forceinline__ int gsel(int par) { return par & 1; }
 __device__ __forceinline__ int dsel(int par) { return (par >> 1) & 1; }
 
 
+// Device-memory (global address space) view of a pointer held in a
+// descriptor: loads through it compile to global loads instead of flat
+// loads, which would also count against the LDS counter the sweeps wait on.
+#define PG_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const PG_GLOBAL T* gmem(const T* p) { return (const PG_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ PG_GLOBAL T* gmem(T* p) { return (PG_GLOBAL T*)p; }
+
+// Load at a uniform base plus an unsigned 32-bit byte offset: the form the
+// scalar-base global load takes, so no 64-bit address arithmetic per load.
+template <typename T>
+__device__ __forceinline__ T ld_off(const PG_GLOBAL T* base, unsigned byte_off) {
+    return *(const PG_GLOBAL T*)((const PG_GLOBAL char*)base + byte_off);
+}
+
 // per-radius tables, field-major: lanes that differ in radius read
 // consecutive addresses
-#define TAB(f, ip) d.tab[(size_t)(f) * d.tab_stride + (ip)]
+#define TAB(f, ip) ld_off(gmem(d.tab) + (size_t)(f) * (size_t)d.tab_stride, (unsigned)(ip) * (unsigned)sizeof(real))
 
-// Division, reciprocal, square root and exponential.  FP32: the hardware
-// approximations (v_rcp_f32, v_sqrt_f32, v_exp_f32; about 1 ulp); FP64: IEEE.
+// Division, reciprocal, square root, reciprocal square root and exponential.
+// FP32: the hardware approximations (v_rcp_f32, v_sqrt_f32, v_rsq_f32,
+// v_exp_f32; about 1 ulp); FP64: IEEE.
 template <typename real> struct FM;
 template <> struct FM<float> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
     static __device__ __forceinline__ float exp(float x) { return __expf(x); }
 };
 template <> struct FM<double> {
     static PG_HD double rcp(double x) { return 1.0 / x; }
     static PG_HD double div(double a, double b) { return a / b; }
     static PG_HD double sqrt(double x) { return ::sqrt(x); }
+    static PG_HD double rsqrt(double x) { return 1.0 / ::sqrt(x); }
     static PG_HD double exp(double x) { return ::exp(x); }
 };
 
@@ -84,16 +103,17 @@ PG_HD void slopes(real am2, real am1, real a0, real ap1, real ap2,
 
 // Mean of the cell parabola over its first fraction q (get_PPM_aveL,
 // PPM.cu:106-113) and over its last fraction x (get_PPM_aveR with x = 1-q,
-// PPM.cu:95-104).
+// PPM.cu:95-104).  The reference's form a - sL + q/2 (sR + sL + (1 - 2q/3) a6)
+// with a6 = 3 (sL - sR) is the quadratic (a - sL) + q ((2 sL - sR) - q (sL - sR));
+// written so, each evaluation is two FMAs on coefficients that the callers'
+// repeated evaluations of one parabola share.
 template <typename real>
 PG_HD real aveL(real q, real sL, real a, real sR) {
-    real a6 = real(3) * (sL - sR);
-    return (a - sL) + (q * real(0.5)) * ((sR + sL) + (real(1) - q * real(2.0 / 3.0)) * a6);
+    return fma(q, fma(-q, sL - sR, real(2) * sL - sR), a - sL);
 }
 template <typename real>
 PG_HD real aveR(real x, real sL, real a, real sR) {
-    real a6 = real(3) * (sL - sR);
-    return (a + sR) - (x * real(0.5)) * ((sR + sL) - (real(1) - x * real(2.0 / 3.0)) * a6);
+    return fma(x, fma(-x, sL - sR, sL - real(2) * sR), a + sR);
 }
 
 // Flattening coefficient of the centre cell of a 5-cell window
@@ -135,11 +155,12 @@ PG_HD void trace_right(real c, real dt, real idx, real r, real p, real u,
     real p0 = fmax(aveL(q, psL, p, psR), p * real(1.0e-10));
     real u0 = aveL(q, usL, u, usR);
     real ir0 = FM<real>::rcp(r0);
-    real cs = FM<real>::sqrt(p0 * ir0);
+    real w = p0 * ir0;
+    real ics = FM<real>::rsqrt(w);
+    real cs = w * ics;   // sqrt(p0/r0) and its reciprocal from one rsqrt
     q = lim01((cs * dt - dis) * idx);
     real u_ = aveL(q, usL, u, usR);
     real p_ = aveL(q, psL, p, psR);
-    real ics = FM<real>::rcp(cs);
     real e = exp_lim((u0 - u_) * ics);
     rr = r0 * e;
     pr = p0 * e;
@@ -162,11 +183,12 @@ PG_HD void trace_left(real c, real dt, real idx, real r, real p, real u,
     real p0 = fmax(aveR(x, psL, p, psR), p * real(1.0e-10));
     real u0 = aveR(x, usL, u, usR);
     real ir0 = FM<real>::rcp(r0);
-    real cs = FM<real>::sqrt(p0 * ir0);
+    real w = p0 * ir0;
+    real ics = FM<real>::rsqrt(w);
+    real cs = w * ics;
     x = lim01((dis + cs * dt) * idx);
     real u_ = aveR(x, usL, u, usR);
     real p_ = aveR(x, psL, p, psR);
-    real ics = FM<real>::rcp(cs);
     real e = exp_lim((u_ - u0) * ics);
     rl = r0 * e;
     pl = p0 * e;
@@ -216,25 +238,29 @@ PG_HD void trace_left_c(real c, real ic, real dt, real idx, real r, real u,
     vl = aveR(x, vsL, v, vsR);
 }
 
-// HLLC selection (HLLC.cu:2-77, EOS_flag 0) given the wave speeds.
+// HLLC selection (HLLC.cu:2-77, EOS_flag 0) given the wave speeds: left
+// supersonic (sl >= 0) -> F(U_L); right supersonic (sr < 0) -> F(U_R);
+// otherwise the star state on the side of the contact (sm >= 0: left).
+// Written without branches: the side is selected first, then one
+// reciprocal serves the star state of that side.
 template <typename real>
 PG_HD void hllc_select(real pm, real sl, real sm, real sr,
-                                            real rl, real pl, real ul, real vl,
-                                            real rr, real pr, real ur, real vr,
-                                            real& Fr, real& us, real& vs, real& pres) {
-    if (!__builtin_signbit(sl)) {
-        Fr = ul * rl; us = ul; vs = vl; pres = pl;
-    } else if (__builtin_signbit(sr)) {
-        Fr = ur * rr; us = ur; vs = vr; pres = pr;
-    } else if (!__builtin_signbit(sm)) {
-        real k = FM<real>::rcp(sl - sm);
-        Fr = sm * rl * (sl - ul) * k; us = ul; vs = vl; pres = (sl * pm - sm * pl) * k;
-    } else {
-        real k = FM<real>::rcp(sr - sm);
-        Fr = sm * rr * (sr - ur) * k; us = ur; vs = vr; pres = (sr * pm - sm * pr) * k;
-    }
-    us *= Fr;
-    vs *= Fr;
+                       real rl, real pl, real ul, real vl,
+                       real rr, real pr, real ur, real vr,
+                       real& Fr, real& us, real& vs, real& pres) {
+    const bool sup_l = !__builtin_signbit(sl);
+    const bool sup_r = !sup_l && __builtin_signbit(sr);
+    const bool left = sup_l || (!sup_r && !__builtin_signbit(sm));
+    const real r_ = left ? rl : rr, p_ = left ? pl : pr, u_ = left ? ul : ur, v_ = left ? vl : vr;
+    const real s_ = left ? sl : sr;
+    const real k = FM<real>::rcp(s_ - sm);
+    const real Fs = sm * r_ * (s_ - u_) * k;
+    const real ps = (s_ * pm - sm * p_) * k;
+    const bool sup = sup_l || sup_r;
+    Fr = sup ? u_ * r_ : Fs;
+    pres = sup ? p_ : ps;
+    us = u_ * Fr;
+    vs = v_ * Fr;
 }
 
 // Wave speeds (get_pm_simple, get_lr_speeds_Einfeldt, get_sm;
@@ -255,8 +281,11 @@ PG_HD void hllc_iso(real cl, real cr, real rl, real pl, real ul, real vl,
     real du = ur - ul;
     real cbar = FM<real>::sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
     real sl = ubar - cbar, sr = ubar + cbar;
-    real sm = FM<real>::div(rl * ul * cl + rr * ur * cr, rl * cl + rr * cr) +
-              real(2) * (pl - pr) * ics * FM<real>::rcp(rl + rr);
+    // sm = num/den + 2 (pl - pr) ics / (rl + rr), both divisions from one
+    // reciprocal of den (rl + rr)
+    const real den = rl * cl + rr * cr, rs = rl + rr;
+    const real X = FM<real>::rcp(den * rs);
+    real sm = (rl * ul * cl + rr * ur * cr) * rs * X + real(2) * (pl - pr) * ics * den * X;
     hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
 }
 

@@ -589,8 +589,8 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
 
 // ------------------------------------------------------------ groups
 // Work lists of one launch schedule.
-//   x sweeps / pointwise kernels: every (tile or 256-radius segment, row) of
-//   every disk, disk by disk.
+//   x sweeps: every (tile, row) of every disk, disk by disk; pointwise
+//   kernels: every (256-radius segment, row), rows in XCD-contiguous bands.
 //   azimuthal pass: tiles of TR radii x TY rows (tile = ty * ntr + tr).
 //   Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 //   L2.  Tiles that are neighbours in azimuth share their 6-row halos and
@@ -605,12 +605,26 @@ void build_worklists(const Group& g, int tile_x, int ring_b, int order, std::vec
     wr.clear();
     wp.clear();
     const int nd = g.ndisk();
+    // pointwise kernels (the viscosity stencil reads rows j-1, j, j+1): each
+    // XCD receives a contiguous band of rows of every disk, so neighbouring
+    // rows are read through one L2
+    std::vector<int2> qp[8];
     for (int k = 0; k < nd; ++k) {
         const DiskDev* d = g.disks[k];
         const int tiles = (d->nx + tile_x - 1) / tile_x;
         for (int b = 0; b < tiles * d->ny; ++b) wx.push_back(make_int2(k, b));
         const int segs = (d->nx + 255) / 256;
-        for (int b = 0; b < segs * d->ny; ++b) wp.push_back(make_int2(k, b));
+        const int per = (d->ny + 7) / 8;
+        for (int x = 0; x < 8; ++x)
+            for (int j = x * per; j < std::min((x + 1) * per, d->ny); ++j)
+                for (int sg = 0; sg < segs; ++sg) qp[x].push_back(make_int2(k, sg + segs * j));
+    }
+    {
+        size_t len = 0;
+        for (int x = 0; x < 8; ++x) len = std::max(len, qp[x].size());
+        wp.assign(8 * len, make_int2(0, -1));
+        for (int x = 0; x < 8; ++x)
+            for (size_t k = 0; k < qp[x].size(); ++k) wp[8 * k + x] = qp[x][k];
     }
     const int TR = Launch<float>::phi_tile_radii(ring_b), TY = Launch<float>::phi_tile_rows(ring_b);
     if (order != 0) {
@@ -1142,7 +1156,8 @@ struct AdjWork {
     void* slot = nullptr;          // slots x SLOT_WORDS*N
     void* lam = nullptr;           // lamA, lamB, lamT (3N each), lamV0, lamV1 (2N each)
     void* part = nullptr;          // VIS_SLOTS*N
-    double* lparam = nullptr;      // dL/dq, dL/dalpha
+    double* lparam = nullptr;      // per-workgroup partial sums of dL/dq, dL/dalpha
+    int nslot = 0;                 // slots per parameter
     double* td = nullptr;          // (t, dt) of every step, on the device
     void* args = nullptr;          // slots x AA_N disk descriptors
     bool forward_done = false;
@@ -1150,6 +1165,16 @@ struct AdjWork {
 };
 
 namespace {
+
+// workgroups of the largest adjoint grid (x adjoint: 128-radius tiles x
+// rows; viscosity gather: 256-column segments of nx+2 columns x rows;
+// azimuthal adjoints: 8-radius x 16-row tiles at most)
+int adj_slots(int nx, int ny) {
+    const int a = ((nx + 127) / 128) * ny;
+    const int b = ((nx + 2 + 255) / 256) * ny;
+    const int c = ((nx + 7) / 8) * ((ny + 15) / 16);
+    return std::max(a, std::max(b, c));
+}
 
 void adj_free(AdjWork* w) {
     if (!w) return;
@@ -1173,6 +1198,7 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
     w->slots = std::min(ckpt, nsteps) + 1;
     w->prec = g.prec;
     w->N = d->N;
+    w->nslot = adj_slots(d->nx, d->ny);
     const size_t N = w->N, es = sizeof(real);
     int rc;
     auto cleanup = [&](int code) {
@@ -1183,7 +1209,7 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
         (rc = dev_alloc(w->bytes, &w->slot, (size_t)w->slots * SLOT_WORDS * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->lam, (size_t)13 * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->part, (size_t)VIS_SLOTS * N * es)) ||
-        (rc = dev_alloc(w->bytes, &w->lparam, 2 * sizeof(double))) ||
+        (rc = dev_alloc(w->bytes, &w->lparam, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->td, 2 * (size_t)nsteps * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))))
         return cleanup(rc);
@@ -1292,6 +1318,7 @@ int adj_step(pg_handle* h, int m, int n, real*& lamA, real*& lamB) {
     A.d = (const DiskArgs<real>*)g.d_args;
     A.part = (real*)w->part;
     A.lparam = w->lparam;
+    A.nslot = w->nslot;
     A.q = d->P.n_planet > 0 ? d->P.planet_mass : 0.0;
     A.alpha = d->P.visc_flag ? d->P.ss_alpha : 0.0;
     const bool visc = g.visc;
@@ -1389,7 +1416,7 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     real* lamA = (real*)w->lam;
     real* lamB = (real*)w->lam + 3 * N;
     if ((rc = upload(g.prec, g.stream, l0, lamA))) return rc;
-    HIPL(hipMemsetAsync(w->lparam, 0, 2 * sizeof(double), g.stream));
+    HIPL(hipMemsetAsync(w->lparam, 0, 2 * (size_t)w->nslot * sizeof(double), g.stream));
     HIPL(hipMemsetAsync((real*)w->lam + 9 * N, 0, 4 * N * sizeof(real), g.stream));
     const size_t bytes = 3 * N * sizeof(real);
     for (int k = w->nseg - 1; k >= 0; --k) {
@@ -1402,8 +1429,13 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     }
     std::vector<double> lam0;
     if ((rc = download(g.prec, g.stream, lamA, lam0, 3 * N))) return rc;
+    std::vector<double> slots(2 * (size_t)w->nslot);
+    HIPL(hipMemcpy(slots.data(), w->lparam, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
     double lp[2] = {0.0, 0.0};
-    HIPL(hipMemcpy(lp, w->lparam, sizeof(lp), hipMemcpyDeviceToHost));
+    for (int k = 0; k < w->nslot; ++k) {
+        lp[0] += slots[k];
+        lp[1] += slots[(size_t)w->nslot + k];
+    }
     // the initial radial velocity -1.5 nu(R)/R is proportional to alpha
     double dal = lp[1];
     if (d->P.visc_flag && d->P.ss_alpha > 0.0) {
@@ -1827,11 +1859,13 @@ static int adjoint_pass(pg_handle* h, int pass, double t, double dt, const doubl
     if ((rc = dev_alloc(scratch, &lo, 3 * N * sizeof(real))) || (rc = dev_alloc(scratch, &li, 3 * N * sizeof(real))) ||
         (rc = dev_alloc(scratch, &lv, 2 * N * sizeof(real))) ||
         (rc = dev_alloc(scratch, &part, (size_t)VIS_SLOTS * N * sizeof(real))) ||
-        (rc = dev_alloc(scratch, &lp, 2 * sizeof(double))))
+        (rc = dev_alloc(scratch, &lp, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))))
         return cleanup(rc);
+    const int nslot = adj_slots(d->nx, d->ny);
     std::vector<double> lin(lam_out, lam_out + (pass == 3 ? 2 : 3) * N);
     if ((rc = upload(g.prec, g.stream, lin, pass == 3 ? (void*)lv : (void*)lo))) return cleanup(rc);
-    if (hipMemsetAsync(lp, 0, 2 * sizeof(double), g.stream) != hipSuccess) return cleanup(fail(PG_E_HIP, "memset"));
+    if (hipMemsetAsync(lp, 0, 2 * (size_t)nslot * sizeof(double), g.stream) != hipSuccess)
+        return cleanup(fail(PG_E_HIP, "memset"));
     AdjArgs<real> A{};
     A.d = (const DiskArgs<real>*)g.d_args;
     A.s = (const real*)d->gas[g.gpar];
@@ -1841,6 +1875,7 @@ static int adjoint_pass(pg_handle* h, int pass, double t, double dt, const doubl
     A.lvis = lv;
     A.part = part;
     A.lparam = lp;
+    A.nslot = nslot;
     A.q = d->P.n_planet > 0 ? d->P.planet_mass : 0.0;
     A.alpha = d->P.visc_flag ? d->P.ss_alpha : 0.0;
     A.accumulate = 0;
@@ -1856,8 +1891,16 @@ static int adjoint_pass(pg_handle* h, int pass, double t, double dt, const doubl
     std::vector<double> out;
     if ((rc = download(g.prec, g.stream, li, out, 3 * N))) return cleanup(rc);
     std::memcpy(lam_in, out.data(), 3 * N * sizeof(double));
-    if (dparam && hipMemcpy(dparam, lp, 2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-        return cleanup(fail(PG_E_HIP, "download"));
+    if (dparam) {
+        std::vector<double> sl(2 * (size_t)nslot);
+        if (hipMemcpy(sl.data(), lp, sl.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return cleanup(fail(PG_E_HIP, "download"));
+        dparam[0] = dparam[1] = 0.0;
+        for (int k = 0; k < nslot; ++k) {
+            dparam[0] += sl[k];
+            dparam[1] += sl[(size_t)nslot + k];
+        }
+    }
     return cleanup(PG_OK);
 }
 

@@ -124,7 +124,9 @@ struct AdjArgs {
     real* li;                  // adjoint of the pass input state (3N)
     real* lvis;                // adjoint of vis (2N)
     real* part;                // viscosity: per-output partial adjoints (VIS_SLOTS*N)
-    double* lparam;            // [0] dL/d planet_mass, [1] dL/d ss_alpha
+    double* lparam;            // per-workgroup partial sums: [0, nslot) dL/d planet_mass,
+                               // [nslot, 2 nslot) dL/d ss_alpha
+    int nslot;                 // slots per parameter (>= the largest adjoint grid)
     int accumulate;            // 1: li += adjoint, 0: li = adjoint
     int mode;                  // source-term mode, 0 kick / 1 predictor / 2 corrector
     double q, alpha;           // planet_mass, ss_alpha

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par)
     const int nt = min(TX, nx - x0);
     const int ns = nt + 2 * NPAD;
     const real dt = (real)d.ctl->dt;
-    const real* __restrict__ in = d.gas[gsel(par)];
+    const PG_GLOBAL real* __restrict__ in = gmem(d.gas[gsel(par)]);
     const size_t row = (size_t)j * nx;
 
     for (int s = t; s < ns; s += TX) {
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par)
         s_pr[k] = pres;
     }
     __syncthreads();
-    real* __restrict__ out = d.gas[gsel(par) ^ 1];
+    PG_GLOBAL real* __restrict__ out = gmem(d.gas[gsel(par) ^ 1]);
     for (int s = t + NPAD; s < ns - NPAD; s += TX) {
         const int ip = x0 + s;
         const int i = ip - NPAD;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
     const int nt = min(TX, nx - x0);
     const int ns = nt + 2 * NPAD;
     const real dt = (real)d.ctl->dt;
-    const real* __restrict__ in = d.dust[dsel(par)];
+    const PG_GLOBAL real* __restrict__ in = gmem(d.dust[dsel(par)]);
     const size_t row = (size_t)j * nx;
 
     for (int s = t; s < ns; s += TX) {
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
         s_FL[k] = FL * g;
     }
     __syncthreads();
-    real* __restrict__ out = d.dust[dsel(par) ^ 1];
+    PG_GLOBAL real* __restrict__ out = gmem(d.dust[dsel(par) ^ 1]);
     const real rfloor = d.D_G_ratio * real(1.0e-10);
     for (int s = t + NPAD; s < ns - NPAD; s += TX) {
         const int ip = x0 + s;
@@ -306,9 +306,9 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
     const real dts = dt * iR;                  // set_state / dust_flow receive dt/rad
     const real idy = d.idy;
     const real fup = dt * idy * iR;            // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
-    const int inc = orbadv ? d.inc[ic] : 0;
-    const real* __restrict__ in = DUST ? d.dust[dsel(par)] : d.gas[gsel(par)];
-    real* __restrict__ out = DUST ? d.dust[dsel(par) ^ 1] : d.gas[gsel(par) ^ 1];
+    const int inc = orbadv ? gmem(d.inc)[ic] : 0;
+    const PG_GLOBAL real* __restrict__ in = gmem(DUST ? d.dust[dsel(par)] : d.gas[gsel(par)]);
+    PG_GLOBAL real* __restrict__ out = gmem(DUST ? d.dust[dsel(par) ^ 1] : d.gas[gsel(par) ^ 1]);
     const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
     const real rfloor = d.D_G_ratio * real(1.0e-10);
     // B is a multiple of TR, so lane t keeps radius ri in every loop below
@@ -316,16 +316,20 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
 
     // ---- load: TY+12 rows of each radius, shifted back by its inc ----
     const int jbase = y0 - 2 * NPAD - inc;
+    const int jb = wrap(jbase + t / TR, ny);   // row of this lane's first element
+    constexpr int KSTEP = B / TR;              // rows advanced per loop trip
+    int jj = jb;
     for (int e = t; e < E; e += B) {
-        const int kk = e / TR;
         real r = real(1), vp = real(0), u = real(0);
         if (live) {
-            const size_t o = (size_t)wrap(jbase + kk, ny) * nx + i;
+            const size_t o = (size_t)jj * nx + i;
             r = in[o];
             vp = in[2 * N + o];   // azimuthal: normal to the interfaces
             u = in[N + o];        // radial: transverse here
         }
         c0[e] = r; c1[e] = vp; c2[e] = u;
+        jj += KSTEP;
+        while (jj >= ny) jj -= ny;   // periodic in azimuth (more than once only when ny < TY+12)
     }
     __syncthreads();
     // ---- stage A: azimuthal sweep of LDS rows [3, TY+9) ----
@@ -404,7 +408,7 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
         return;
     }
     // ---- stage B: residual advection at speed res (advection.cu:1-105) ----
-    const real speed = d.res[ic];
+    const real speed = gmem(d.res)[ic];
     const real q = lim01(fabs(speed) * dts * idy);   // |speed| dt / (R dy)
     for (int e = t + (2 * NPAD - 1) * TR; e < (2 * NPAD + TY + 1) * TR; e += B) {
         real l, r;
@@ -466,13 +470,14 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
 template <typename real>
 __global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, int which) {
     const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;   // padding entry of the work list (whole block)
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
     const int segs = (nx + 255) >> 8;
     const int j = w.y / segs;
     const int i = (w.y - j * segs) * 256 + threadIdx.x;
-    const real* __restrict__ st = d.gas[gsel(par) ^ which];
+    const PG_GLOBAL real* __restrict__ st = gmem(d.gas[gsel(par) ^ which]);
     if (i >= nx) return;
     const int ip = i + NPAD;
     real nu[3];
@@ -528,8 +533,9 @@ __global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, 
     fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 * ir - n1 * dvdx * ir + n1 * v1 * ir * ir +
           real(2) * sxy * ir;
     size_t o = (size_t)j * nx + i;
-    d.vis[o] = fx;
-    d.vis[N + o] = fy;
+    PG_GLOBAL real* vis = gmem(d.vis);
+    vis[o] = fx;
+    vis[N + o] = fy;
 }
 // ------------------------------------------------------------- sources
 
@@ -589,15 +595,16 @@ __device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x
 template <typename real>
 __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, int mode, int do_kill, int do_rate) {
     const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;   // padding entry of the work list (whole block)
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
     const int segs = (nx + 255) >> 8;
     const int j = w.y / segs;
     const int i = (w.y - j * segs) * 256 + threadIdx.x;
-    real* __restrict__ gas = d.gas[gsel(par)];
-    real* __restrict__ gasT = d.gas[gsel(par) ^ 1];
-    real* __restrict__ dust = d.dust[dsel(par)];
+    PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
+    PG_GLOBAL real* __restrict__ gasT = gmem(d.gas[gsel(par) ^ 1]);
+    PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
     real rate = real(0);
     if (i < nx) {
         const int ip = i + NPAD;
@@ -614,7 +621,7 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
         const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
         const real irho = FM<real>::rcp(rho);
         real vx = real(0), vy = real(0);
-        if (d.visc) { vx = d.vis[o] * irho; vy = d.vis[N + o] * irho; }
+        if (d.visc) { vx = gmem(d.vis)[o] * irho; vy = gmem(d.vis)[N + o] * irho; }
         const real ay = vy + py;               // fy + gy, independent of the state
         real Tu = cu, Tvp = cvp;
         real Qu = 0, Qvp = 0, Du = 0, Dvp = 0, ts = 0, its = 0;
@@ -674,14 +681,15 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
 template <typename real>
 __global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
     const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;   // padding entry of the work list (whole block)
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
     const int segs = (nx + 255) >> 8;
     const int j = w.y / segs;
     const int i = (w.y - j * segs) * 256 + threadIdx.x;
-    const real* __restrict__ gas = d.gas[gsel(par)];
-    const real* __restrict__ dust = d.dust[dsel(par)];
+    const PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
+    const PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
     real rate = real(0);
     if (i < nx) {
         const int ip = i + NPAD;
@@ -726,7 +734,7 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
     const DiskArgs<real>& d = ls.disks[blockIdx.x];
     double* dts = d.dts;
     StepCtl* ctl = d.ctl;
-    __shared__ double s_dt;
+    __shared__ double s_dt, s_t;
     double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
     if (threadIdx.x == 0) {
         double t = ctl->t, dt = ctl->dt;
@@ -750,15 +758,18 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
             if (dts && ctl->nstep < d.dts_cap) dts[ctl->nstep] = dt;
             ctl->nstep += 1;
         }
-        Body b0 = ctl->planet0;
-        orbit_solution(b0, t, d.ramp_time, d.planet_mass, (double)d.frame_omega);
-        ctl->planet0 = b0;
-        ctl->pk0 = planet_kernel_view(b0);
-        Body b1 = ctl->planet1;
-        orbit_solution(b1, t + dt, d.ramp_time, d.planet_mass, (double)d.frame_omega);
-        ctl->planet1 = b1;
-        ctl->pk1 = planet_kernel_view(b1);
+        s_t = t;
         s_dt = dt;
+    }
+    __syncthreads();
+    // the planet at t and at t + dt, on two lanes
+    if (threadIdx.x < 2) {
+        const int k = threadIdx.x;
+        Body b = k == 0 ? ctl->planet0 : ctl->planet1;
+        orbit_solution(b, k == 0 ? s_t : s_t + s_dt, d.ramp_time, d.planet_mass, (double)d.frame_omega);
+        const PlanetK v = planet_kernel_view(b);
+        if (k == 0) { ctl->planet0 = b; ctl->pk0 = v; }
+        else { ctl->planet1 = b; ctl->pk1 = v; }
     }
     __syncthreads();
     if (d.orbadv && do_orbadv) {
