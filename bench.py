@@ -39,14 +39,14 @@ PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
 # algorithmic HBM bytes per interior cell per launch, in units of the storage
 # word: (fields read + fields written) that the kernel must move at minimum
 WORDS_PER_CELL = {
-    "sweep_x_gas": 6,        # rho,u,vp in and out
-    "ring_gas": 6,           # azimuthal sweep + orbital shift + residual advection
-    "sweep_x_dust": 6,
+    "sweep_x_gas": 8,            # rho,u,vp in and out; + the viscous forces of the fused first half kick
+    "ring_gas": 6,               # azimuthal sweep + orbital shift + residual advection
+    "sweep_x_dust": 9,           # dust in and out; + gas rho,u,vp for the fused kick's drag
     "ring_dust": 6,
-    "viscosity": 5,          # rho,u,vp in; two accelerations out
-    "sources_kick": 12,      # gas 3 + dust 3 + vis 2 in; gas u,vp + dust u,vp out
-    "sources_predictor": 8,  # gas 3 + vis 2 in; gas 3 out
-    "sources_corrector": 12,
+    "viscosity": 5,              # rho,u,vp in; two forces out (once per run)
+    "predictor_viscosity": 7,    # gas 3 + forces 2 in; forces of the predicted state 2 out
+    "corrector_viscosity": 14,   # gas 3 + forces 2 + dust u,vp in; gas 3 (wave killing) + dust 2 + forces 2 out
+    "sources_corrector": 12,     # (without viscosity) gas 3 + dust 3 in; gas and dust velocities out
     "cfl": 6,
     "prologue": 0,
 }

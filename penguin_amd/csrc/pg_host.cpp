@@ -201,6 +201,7 @@ struct DiskDev {
     void* gas[2] = {nullptr, nullptr};
     void* dust[2] = {nullptr, nullptr};
     void* vis = nullptr;
+    void* vis2 = nullptr;
     void* tab = nullptr;
     void* phitab = nullptr;
     void* res = nullptr;
@@ -235,6 +236,7 @@ struct DiskDev {
             d.dust[k] = (real*)dust[k];
         }
         d.vis = (real*)vis;
+        d.vis2 = (real*)vis2;
         d.tab = (const real*)tab;
         d.tab_stride = tab_stride;
         d.dy = (real)dy;
@@ -291,9 +293,10 @@ struct Group {
     std::vector<int> ev_kernel;   // kernel id of pair k (events 2k, 2k+1)
     size_t ev_used = 0;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    // graph of one device-loop step and the parity it was captured at
-    hipGraphExec_t graph = nullptr;
-    int graph_par = -1;
+    // graphs of one device-loop step, one per start parity (a step may end
+    // at another parity), and the parity each ends at
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    int graph_next[4] = {0, 0, 0, 0};
     size_t bytes = 0;
 
     int par() const { return gpar | (dpar << 1); }
@@ -314,10 +317,15 @@ struct Group {
         L.n_pt = n_pt;
         return L;
     }
+    void set_par(int p) {
+        gpar = p & 1;
+        dpar = (p >> 1) & 1;
+    }
     void drop_graph() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
-        graph_par = -1;
+        for (auto& x : graph) {
+            if (x) (void)hipGraphExecDestroy(x);
+            x = nullptr;
+        }
     }
 };
 
@@ -519,7 +527,7 @@ void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<do
 // ------------------------------------------------------------ disks
 void disk_free(DiskDev* d) {
     if (!d) return;
-    void* bufs[] = {d->gas[0], d->gas[1], d->dust[0], d->dust[1], d->vis, d->tab, d->phitab,
+    void* bufs[] = {d->gas[0], d->gas[1], d->dust[0], d->dust[1], d->vis, d->vis2, d->tab, d->phitab,
                     d->res, d->inc, d->rot, d->dts, d->ctl, d->rate_part};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -548,13 +556,14 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
     int rc;
     size_t& B = h->bytes;
     if ((rc = dev_alloc(B, &h->gas[0], 3 * N * es)) || (rc = dev_alloc(B, &h->gas[1], 3 * N * es)) ||
-        (rc = dev_alloc(B, &h->vis, 2 * N * es)) ||
+        (rc = dev_alloc(B, &h->vis, 2 * N * es)) || (rc = dev_alloc(B, &h->vis2, 2 * N * es)) ||
         (rc = dev_alloc(B, &h->tab, h->tabh.size() * es)) || (rc = dev_alloc(B, &h->phitab, h->phih.size() * es)) ||
         (rc = dev_alloc(B, &h->res, h->nx * es)) ||
         (rc = dev_alloc(B, &h->inc, h->nx * sizeof(int))) || (rc = dev_alloc(B, &h->rot, rot.size() * sizeof(double))) ||
         (rc = dev_alloc(B, &h->ctl, sizeof(StepCtl))))
         return cleanup(rc);
-    h->rate_nblk = ((h->nx + 255) / 256) * h->ny;
+    // one rate partial per 64-radius x 4-row tile of the pointwise kernels
+    h->rate_nblk = ((h->nx + 63) / 64) * ((h->ny + 3) / 4);
     if ((rc = dev_alloc(B, &h->rate_part, sizeof(double) * h->rate_nblk))) return cleanup(rc);
     if (P.dust_flag) {
         if ((rc = dev_alloc(B, &h->dust[0], 3 * N * es)) || (rc = dev_alloc(B, &h->dust[1], 3 * N * es)))
@@ -572,7 +581,7 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
         ((rc = upload(h->prec, s, dust, h->dust[0])) || (rc = upload(h->prec, s, dust, h->dust[1]))))
         return cleanup(rc);
     std::vector<double> zero(2 * N, 0.0);
-    if ((rc = upload(h->prec, s, zero, h->vis))) return cleanup(rc);
+    if ((rc = upload(h->prec, s, zero, h->vis)) || (rc = upload(h->prec, s, zero, h->vis2))) return cleanup(rc);
     if (hipMemcpy(h->rot, rot.data(), rot.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(PG_E_HIP, "rot upload failed"));
     if (hipMemset(h->inc, 0, h->nx * sizeof(int)) != hipSuccess) return cleanup(fail(PG_E_HIP, "memset failed"));
@@ -589,8 +598,9 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
 
 // ------------------------------------------------------------ groups
 // Work lists of one launch schedule.
-//   x sweeps: every (tile, row) of every disk, disk by disk; pointwise
-//   kernels: every (256-radius segment, row), rows in XCD-contiguous bands.
+//   x sweeps: every (tile, row) of every disk, disk by disk; pointwise and
+//   stencil kernels: every 64-radius x 4-row tile, tile rows in
+//   XCD-contiguous bands.
 //   azimuthal pass: tiles of TR radii x TY rows (tile = ty * ntr + tr).
 //   Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 //   L2.  Tiles that are neighbours in azimuth share their 6-row halos and
@@ -613,11 +623,11 @@ void build_worklists(const Group& g, int tile_x, int ring_b, int order, std::vec
         const DiskDev* d = g.disks[k];
         const int tiles = (d->nx + tile_x - 1) / tile_x;
         for (int b = 0; b < tiles * d->ny; ++b) wx.push_back(make_int2(k, b));
-        const int segs = (d->nx + 255) / 256;
-        const int per = (d->ny + 7) / 8;
+        const int ntx = (d->nx + 63) / 64, nty = (d->ny + 3) / 4;   // 64-radius x 4-row tiles
+        const int per = (nty + 7) / 8;
         for (int x = 0; x < 8; ++x)
-            for (int j = x * per; j < std::min((x + 1) * per, d->ny); ++j)
-                for (int sg = 0; sg < segs; ++sg) qp[x].push_back(make_int2(k, sg + segs * j));
+            for (int ty = x * per; ty < std::min((x + 1) * per, nty); ++ty)
+                for (int tx = 0; tx < ntx; ++tx) qp[x].push_back(make_int2(k, tx + ntx * ty));
     }
     {
         size_t len = 0;
@@ -780,7 +790,7 @@ int group_init(Group& g, int n, const pg_params* params) {
 
 // ------------------------------------------------------------ launching
 const char* kname[] = {"prologue", "viscosity", "sources_kick", "sweep_x_gas", "ring_gas", "sweep_x_dust",
-                       "ring_dust", "sources_predictor", "sources_corrector", "cfl"};
+                       "ring_dust", "predictor_viscosity", "corrector_viscosity", "cfl"};
 enum KId { K_PRO, K_VIS, K_KICK, K_SXG, K_RG, K_SXD, K_RD, K_PRED, K_CORR, K_CFL };
 
 int timed_begin(Group& g) {
@@ -829,33 +839,55 @@ int timers_resolve(Group& g) {
 
 // One step (DS order, solve.cu:212-292) of every disk of the group after the
 // prologue; each disk's dt, t and planets are in its device control block.
-// A full step flips each fluid's buffer pair twice, so the parity it ends
-// with is the one it started with.
+// With viscosity a step flips the gas pair three times (x sweep, azimuthal
+// pass, corrector), so consecutive steps alternate between two parities.
+// With viscosity, the forces of the step start (d.vis) come from the
+// previous step's corrector kernel; a step whose start state did not come
+// from one (vis_ready false) computes them first.
 template <typename real>
-int step_passes(Group& g) {
+int step_passes(Group& g, bool vis_ready) {
     const LaunchSet<real> L = g.ls<real>();
     hipStream_t s = g.stream;
-    if (g.visc) LAUNCH(g, K_VIS, Launch<real>::viscosity(L, g.par(), 0, s));
-    LAUNCH(g, K_KICK, Launch<real>::sources(L, g.par(), 0, 0, 0, s));
-    LAUNCH(g, K_SXG, Launch<real>::sweep_x_gas(L, g.par(), s));
+    if (g.visc && !vis_ready) LAUNCH(g, K_VIS, Launch<real>::viscosity(L, g.par(), 0, s));
+    // first half kick fused into the x sweeps; the dust x sweep runs before
+    // the gas azimuthal pass (gas and dust sweeps do not interact) so that
+    // it still finds the gas state the kick's drag term needs
+    LAUNCH(g, K_SXG, Launch<real>::sweep_x_gas(L, g.par(), 1, s));
     g.gpar ^= 1;
+    if (g.dust) {
+        LAUNCH(g, K_SXD, Launch<real>::sweep_x_dust(L, g.par(), 1, s));
+        g.dpar ^= 1;
+    }
     LAUNCH(g, K_RG, Launch<real>::ring_gas(L, g.par(), 3, s));
     g.gpar ^= 1;
     if (g.dust) {
-        LAUNCH(g, K_SXD, Launch<real>::sweep_x_dust(L, g.par(), s));
-        g.dpar ^= 1;
         LAUNCH(g, K_RD, Launch<real>::ring_dust(L, g.par(), 3, s));
         g.dpar ^= 1;
     }
     if (g.visc) {
-        LAUNCH(g, K_PRED, Launch<real>::sources(L, g.par(), 1, 0, 0, s));
-        LAUNCH(g, K_VIS, Launch<real>::viscosity(L, g.par(), 1, s));
+        // predictor + viscosity of the predicted state; corrector + wave
+        // killing + CFL rate + viscosity of the corrected state
+        LAUNCH(g, K_PRED, Launch<real>::predvis(L, g.par(), s));
+        LAUNCH(g, K_CORR, Launch<real>::corrvis(L, g.par(), g.kill ? 1 : 0, s));
+        g.gpar ^= 1;   // the corrected gas is in the other buffer
+    } else {
+        LAUNCH(g, K_CORR, Launch<real>::sources(L, g.par(), 2, g.kill ? 1 : 0, 1, s));
     }
-    LAUNCH(g, K_CORR, Launch<real>::sources(L, g.par(), 2, g.kill ? 1 : 0, 1, s));
     return PG_OK;
 }
 
-int passes(Group& g) { return g.prec == 8 ? step_passes<double>(g) : step_passes<float>(g); }
+int passes(Group& g, bool vis_ready) {
+    return g.prec == 8 ? step_passes<double>(g, vis_ready) : step_passes<float>(g, vis_ready);
+}
+
+// viscous forces of the current state into d.vis (start of a run)
+int vis_start(Group& g) {
+    if (!g.visc) return PG_OK;
+    hipError_t e = g.prec == 8 ? Launch<double>::viscosity(g.ls<double>(), g.par(), 0, g.stream)
+                               : Launch<float>::viscosity(g.ls<float>(), g.par(), 0, g.stream);
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch viscosity failed: %s", hipGetErrorString(e));
+    return PG_OK;
+}
 
 int prologue(Group& g, int mode, int do_orbadv) {
     hipError_t e = g.prec == 8 ? Launch<double>::prologue(g.ls<double>(), mode, do_orbadv, g.stream)
@@ -925,29 +957,35 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
     }
     if ((rc = push_ctl(g))) return rc;
     // rate of the current state for the first step (later steps get it from
-    // the corrector)
+    // the corrector), and its viscous forces
     if ((rc = launch_cfl(g))) return rc;
+    if ((rc = vis_start(g))) return rc;
     if (use_graph && !g.profile) {
-        if (g.graph && g.graph_par != g.par()) g.drop_graph();
-        if (!g.graph) {
-            hipGraph_t gr;
-            const int par0 = g.par();
-            HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-            int rc1 = prologue(g, 1, 1);
-            int rc2 = rc1 ? rc1 : passes(g);
-            hipError_t ec = hipStreamEndCapture(g.stream, &gr);
-            if (rc2) return rc2;
-            if (ec != hipSuccess) return fail(PG_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
-            ec = hipGraphInstantiate(&g.graph, gr, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(gr);
-            if (ec != hipSuccess) {
-                g.graph = nullptr;
-                return fail(PG_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ec));
-            }
-            g.graph_par = par0;
-        }
         if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
-        for (int n = 0; n < nsteps; ++n) HIP_TRY(hipGraphLaunch(g.graph, g.stream));
+        for (int n = 0; n < nsteps; ++n) {
+            const int p = g.par();
+            if (!g.graph[p]) {
+                // capture records the step without running it; the host's
+                // parity bookkeeping advances as if it had run
+                hipGraph_t gr;
+                HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+                int rc1 = prologue(g, 1, 1);
+                int rc2 = rc1 ? rc1 : passes(g, true);
+                hipError_t ec = hipStreamEndCapture(g.stream, &gr);
+                if (rc2) return rc2;
+                if (ec != hipSuccess) return fail(PG_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+                ec = hipGraphInstantiate(&g.graph[p], gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (ec != hipSuccess) {
+                    g.graph[p] = nullptr;
+                    return fail(PG_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ec));
+                }
+                g.graph_next[p] = g.par();
+            } else {
+                g.set_par(g.graph_next[p]);
+            }
+            HIP_TRY(hipGraphLaunch(g.graph[p], g.stream));
+        }
         if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
     } else {
         if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
@@ -955,7 +993,7 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
             if ((rc = timed_begin(g))) return rc;
             if ((rc = prologue(g, 1, 1))) return rc;
             if ((rc = timed_end(g, K_PRO))) return rc;
-            if ((rc = passes(g))) return rc;
+            if ((rc = passes(g, true))) return rc;
         }
         if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
     }
@@ -1284,7 +1322,7 @@ int adj_recompute(pg_handle* h, int m, int n, bool write_next) {
     if (visc) HIPL(Launch<real>::viscosity(at(AA_V0), 0, 0, s));
     HIPL(hipMemcpyAsync(sl + SL_S1 * N, sl + SL_S0 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
     HIPL(Launch<real>::sources(at(AA_KICK), 0, 0, 0, 0, s));
-    HIPL(Launch<real>::sweep_x_gas(at(AA_X), 0, s));
+    HIPL(Launch<real>::sweep_x_gas(at(AA_X), 0, 0, s));
     HIPL(Launch<real>::ring_gas(at(AA_PA), 0, 1, s));
     HIPL(Launch<real>::ring_gas(at(AA_PB), 0, 2, s));
     if (visc) {
@@ -1534,7 +1572,7 @@ int pg_solve(pg_handle* h, double t, double dt) {
     c.rate_bits = 0ull;
     if ((rc = push_ctl(g))) return rc;
     if ((rc = prologue(g, 0, 1))) return rc;
-    if ((rc = passes(g))) return rc;
+    if ((rc = passes(g, false))) return rc;
     HIP_TRY(hipStreamSynchronize(g.stream));
     return timers_resolve(g);
 }
@@ -1629,14 +1667,14 @@ static int sweep_module(pg_handle* h, double dt, int which) {
     const int par = g.par();
     if (g.prec == 8) {
         const LaunchSet<double> L = g.ls<double>();
-        e = which == 0 ? Launch<double>::sweep_x_gas(L, par, g.stream)
-            : which == 1 ? Launch<double>::sweep_x_dust(L, par, g.stream)
+        e = which == 0 ? Launch<double>::sweep_x_gas(L, par, 0, g.stream)
+            : which == 1 ? Launch<double>::sweep_x_dust(L, par, 0, g.stream)
             : which == 2 ? Launch<double>::ring_gas(L, par, 1, g.stream)
                          : Launch<double>::ring_dust(L, par, 1, g.stream);
     } else {
         const LaunchSet<float> L = g.ls<float>();
-        e = which == 0 ? Launch<float>::sweep_x_gas(L, par, g.stream)
-            : which == 1 ? Launch<float>::sweep_x_dust(L, par, g.stream)
+        e = which == 0 ? Launch<float>::sweep_x_gas(L, par, 0, g.stream)
+            : which == 1 ? Launch<float>::sweep_x_dust(L, par, 0, g.stream)
             : which == 2 ? Launch<float>::ring_gas(L, par, 1, g.stream)
                          : Launch<float>::ring_dust(L, par, 1, g.stream);
     }

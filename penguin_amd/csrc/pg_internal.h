@@ -90,7 +90,8 @@ struct DiskArgs {
     int n_planet;
     real* gas[2];        // 3*N state, ping-pong pair; the launch's parity picks the current one
     real* dust[2];       // 3*N
-    real* vis;           // 2*N viscous acceleration*rho
+    real* vis;           // 2*N viscous acceleration*rho of the step start (V0)
+    real* vis2;          // 2*N the same of the predicted state (V1)
     const real* tab;     // per-radius tables, field-major: tab[f*tab_stride + ip]
     int tab_stride;      // nx + 2*NPAD + 1 radii
     const real* phitab;  // 4*ny: sin(phi_j), cos(phi_j), sin(phi_j/2), cos(phi_j/2)

@@ -36,14 +36,80 @@
 
 namespace pg {
 
+// ------------------------------------------------------- source terms
+// 1 - exp(-x) (the drag factor dtau, sweeps.cu:673,702)
+template <typename real>
+__device__ __forceinline__ real one_minus_exp_neg(real x);
+template <>
+__device__ __forceinline__ float one_minus_exp_neg(float x) { return 1.0f - __expf(-x); }
+template <>
+__device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x); }
+
+// Source terms of one cell (apply_source_terms, sweeps.cu:555-711; the
+// dust does not act back on the gas).  mode 0: the first half kick with the
+// planet at t; 1/2: predictor / corrector with the planet at t+dt, the
+// velocity-dependent forces evaluated at the state predicted over h.
+// h = dt/2.  Gas inputs rho, cu, cvp (vp = deviation from vbg), viscous
+// forces Vr, Vp; dust Qu, Qvp.  GAS / DUST select the outputs computed.
+// Used by k_sources and by the x sweeps, which apply the first half kick to
+// the cells they load.
+template <typename real, bool GAS, bool DUST>
+__device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, int ip, int j, real h, real rho,
+                                             real cu, real cvp, real Vr, real Vp, real Qu, real Qvp, real& nu,
+                                             real& nvp, real& ndu, real& ndvp) {
+    const real mdt = mode == 0 ? real(0) : h;
+    const PlanetK& pk = mode == 0 ? d.ctl->pk0 : d.ctl->pk1;
+    const real R = TAB(T_XC, ip);
+    const real iR = TAB(T_IR, ip);
+    const real vbg = TAB(T_VBG, ip);
+    const real ebal = TAB(T_EBAL, ip);
+    real px = real(0), py = real(0);
+    if (d.n_planet > 0) planet_acc(pk, d.phitab + 4 * j, R, px, py);
+    const real irho = FM<real>::rcp(rho);
+    real vx = real(0), vy = real(0);
+    if (d.visc) { vx = Vr * irho; vy = Vp * irho; }
+    const real ay = vy + py;               // fy + gy, independent of the state
+    real Tu = cu, Tvp = cvp;
+    real Du = Qu, Dvp = Qvp, ts = 0, its = 0;
+    if (DUST) {
+        ts = TAB(T_TS, ip) * irho;            // St / (c_s/h0) / rho (dust.cu:1-4)
+        its = rho * TAB(T_ITS, ip);
+    }
+    // fx + gx at azimuthal velocity vbg + vp: centrifugal - stellar
+    // gravity (tabulated at vbg) + deviation part + planet (+ viscosity)
+    auto axd = [&](real vp) { return ebal + vp * (real(2) * vbg + vp) * iR + px; };
+    auto axg = [&](real vp) { return axd(vp) + vx; };
+    if (mode == 1 || mode == 2) {
+        Tu = cu + axg(cvp) * mdt;
+        Tvp = cvp + ay * mdt;
+        if (DUST) {
+            const real dtau = one_minus_exp_neg(mdt * its);
+            Du = Qu + (axd(Dvp) * ts + cu - Qu) * dtau;
+            Dvp = Qvp + ((cvp - Qvp) + py * ts) * dtau;
+        }
+    }
+    if (GAS) {
+        nu = cu + axg(Tvp) * h;
+        nvp = cvp + ay * h;
+    }
+    if (DUST) {
+        const real dtau = one_minus_exp_neg(h * its);
+        ndu = Qu + (axd(Dvp) * ts + Tu - Qu) * dtau;
+        ndvp = Qvp + ((Tvp - Qvp) + py * ts) * dtau;
+    }
+}
+
 // ------------------------------------------------------------ x sweep, gas
 // One block = TX interior cells of one azimuthal row (the whole row when
 // nr <= 1024) plus 3 ghost/halo cells each side, staged in LDS; thread t
 // owns slot t (interior cell t-3 of the tile).  Phases (barrier between):
 // load -> flattening -> parabolas -> interface fluxes -> conservative update.
 // TX is a compile-time constant so every LDS array offset is an immediate.
+// kick != 0: apply the first half kick (cell_sources mode 0: planet at t,
+// the viscous forces of the step start) to every interior cell as it is
+// loaded, in place of a separate source-term pass.
 template <typename real, int TX>
-__global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par) {
+__global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par, int kick) {
     const int2 w = ls.wl_x[blockIdx.x];
     const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
@@ -81,6 +147,14 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par)
             r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); vp = TAB(T_G_V, ip);
         } else {
             r = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
+            if (kick) {
+                real Vr = real(0), Vp = real(0), nu, nvp, unused0, unused1;
+                if (d.visc) { Vr = gmem(d.vis)[row + i]; Vp = gmem(d.vis)[N + row + i]; }
+                cell_sources<real, true, false>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), r, u, vp, Vr, Vp, real(0),
+                                                real(0), nu, nvp, unused0, unused1);
+                u = nu;
+                vp = nvp;
+            }
         }
         s_r[s] = r;
         s_p[s] = TAB(T_CS2, ip) * r;
@@ -156,8 +230,12 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par)
 }
 
 // ----------------------------------------------------------- x sweep, dust
+// kick != 0: apply the first half kick (drag toward the gas and the
+// planet's pull, cell_sources mode 0) to every interior cell as it is
+// loaded; the gas state before its own kick and sweep is the buffer
+// opposite the current one (the gas x sweep has just run).
 template <typename real, int TX>
-__global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par) {
+__global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par, int kick) {
     const int2 w = ls.wl_x[blockIdx.x];
     const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
@@ -192,6 +270,15 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
             r = TAB(T_D_R, ip); u = TAB(T_D_U, ip); vp = TAB(T_D_V, ip);
         } else {
             r = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
+            if (kick) {
+                const PG_GLOBAL real* __restrict__ g0 = gmem(d.gas[gsel(par) ^ 1]);
+                const real rho = g0[row + i], cu = g0[N + row + i], cvp = g0[2 * N + row + i];
+                real ndu, ndvp, unused0, unused1;
+                cell_sources<real, false, true>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), rho, cu, cvp, real(0), real(0),
+                                                u, vp, unused0, unused1, ndu, ndvp);
+                u = ndu;
+                vp = ndvp;
+            }
         }
         s_r[s] = r;
         s_u[s] = u;
@@ -461,48 +548,20 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
 }
 
 // ----------------------------------------------------------------- viscosity
-// 9-point stencil of viscous_forces (viscosity.cu:34-197, ndim 2).  The
-// derivatives of v_phi = vbg(R) + vp are split into the tabulated
-// background part and the stored deviation (the difference operators are
-// linear, so this is the same arithmetic rearranged).
-// which = 0: viscosity of the current state; 1: of the other buffer (the
-// predictor, viscosity_tensor_evaluation2)
+// Pointwise and stencil kernels work on tiles of PT_X radii x PT_Y rows,
+// one cell per thread; the tile id of a work-list entry is ty * ntx + tx.
+constexpr int PT_X = 64, PT_Y = 4;
+__device__ __forceinline__ int pt_ntx(int nx) { return (nx + PT_X - 1) / PT_X; }
+
+// 9-point stencil of viscous_forces (viscosity.cu:34-197, ndim 2) at the cell
+// of padded radius ip from its neighbourhood (n = 3b + a: radius i-1+a, row
+// j, j-1, j+1 for b = 0, 1, 2) of rho*nu, u and vp.  The derivatives of
+// v_phi = vbg(R) + vp are split into the tabulated background part and the
+// stored deviation (the difference operators are linear, so this is the
+// same arithmetic rearranged).
 template <typename real>
-__global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, int which) {
-    const int2 w = ls.wl_pt[blockIdx.x];
-    if (w.y < 0) return;   // padding entry of the work list (whole block)
-    const DiskArgs<real>& d = ls.disks[w.x];
-    const int nx = d.nx, ny = d.ny;
-    const size_t N = (size_t)nx * ny;
-    const int segs = (nx + 255) >> 8;
-    const int j = w.y / segs;
-    const int i = (w.y - j * segs) * 256 + threadIdx.x;
-    const PG_GLOBAL real* __restrict__ st = gmem(d.gas[gsel(par) ^ which]);
-    if (i >= nx) return;
-    const int ip = i + NPAD;
-    real nu[3];
-    for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
-    real nr[9], u[9], v[9];
-    const int jm = wrap(j - 1, ny), jp = wrap(j + 1, ny);
-    const int rows[3] = {j, jm, jp};
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            int n = 3 * b + a;
-            int ii = i - 1 + a;
-            real rr, uu, vv;
-            if (ii < 0 || ii >= nx) {
-                rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
-            } else {
-                size_t o = (size_t)rows[b] * nx + ii;
-                rr = st[o]; uu = st[N + o]; vv = st[2 * N + o];
-            }
-            nr[n] = rr * nu[a];
-            u[n] = uu;
-            v[n] = vv;  // deviation vp
-        }
-    }
+__device__ __forceinline__ void vis_cell(const DiskArgs<real>& d, int ip, const real* nr, const real* u,
+                                         const real* v, real& fx_out, real& fy_out) {
     // reciprocal differences of the centre grid (viscosity.cu:9-31)
     const real i20 = TAB(T_I20, ip), i21 = TAB(T_I21, ip), i10 = TAB(T_I10, ip);
     const real ir = TAB(T_IR, ip);
@@ -532,11 +591,57 @@ __global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, 
     fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 * ir - n1 * dvdy * ir - syy * ir;
     fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 * ir - n1 * dvdx * ir + n1 * v1 * ir * ir +
           real(2) * sxy * ir;
+    fx_out = fx;
+    fy_out = fy;
+}
+
+// which = 0: viscosity of the current state; 1: of the other buffer (the
+// predictor, viscosity_tensor_evaluation2).  Result in d.vis.
+template <typename real>
+__global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, int which) {
+    const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;   // padding entry of the work list (whole block)
+    const DiskArgs<real>& d = ls.disks[w.x];
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int ntx = pt_ntx(nx);
+    const int ty = w.y / ntx;
+    const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
+    const int j = ty * PT_Y + (int)(threadIdx.x / PT_X);
+    const PG_GLOBAL real* __restrict__ st = gmem(d.gas[gsel(par) ^ which]);
+    if (i >= nx || j >= ny) return;
+    const int ip = i + NPAD;
+    real nu[3];
+    for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
+    real nr[9], u[9], v[9];
+    const int jm = wrap(j - 1, ny), jp = wrap(j + 1, ny);
+    const int rows[3] = {j, jm, jp};
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int n = 3 * b + a;
+            int ii = i - 1 + a;
+            real rr, uu, vv;
+            if (ii < 0 || ii >= nx) {
+                rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
+            } else {
+                size_t o = (size_t)rows[b] * nx + ii;
+                rr = st[o]; uu = st[N + o]; vv = st[2 * N + o];
+            }
+            nr[n] = rr * nu[a];
+            u[n] = uu;
+            v[n] = vv;  // deviation vp
+        }
+    }
+    real fx, fy;
+    vis_cell(d, ip, nr, u, v, fx, fy);
     size_t o = (size_t)j * nx + i;
     PG_GLOBAL real* vis = gmem(d.vis);
     vis[o] = fx;
     vis[N + o] = fy;
 }
+
 // ------------------------------------------------------------- sources
 
 // Block max of a non-negative value, written to this block's slot of the
@@ -580,14 +685,6 @@ __device__ __forceinline__ double reduce_partials(const double* part, int n) {
     return m;
 }
 
-// 1 - exp(-x) (the drag factor dtau, sweeps.cu:673,702)
-template <typename real>
-__device__ __forceinline__ real one_minus_exp_neg(real x);
-template <>
-__device__ __forceinline__ float one_minus_exp_neg(float x) { return 1.0f - __expf(-x); }
-template <>
-__device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x); }
-
 // Pointwise source terms.  mode: 0 first half kick (planet at t, in place),
 // 1 predictor (planet at t+dt, gas written to gasT), 2 corrector (planet at
 // t+dt, in place) optionally followed by wave killing and the CFL rate,
@@ -599,64 +696,33 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int segs = (nx + 255) >> 8;
-    const int j = w.y / segs;
-    const int i = (w.y - j * segs) * 256 + threadIdx.x;
+    const int ntx = pt_ntx(nx);
+    const int ty = w.y / ntx;
+    const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
+    const int j = ty * PT_Y + (int)(threadIdx.x / PT_X);
     PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
     PG_GLOBAL real* __restrict__ gasT = gmem(d.gas[gsel(par) ^ 1]);
     PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
     real rate = real(0);
-    if (i < nx) {
+    if (i < nx && j < ny) {
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
-        const real dt = (real)(0.5 * d.ctl->dt);
-        const real mdt = mode == 0 ? real(0) : dt;
-        const PlanetK& pk = mode == 0 ? d.ctl->pk0 : d.ctl->pk1;
-        const real R = TAB(T_XC, ip);
-        const real iR = TAB(T_IR, ip);
-        const real vbg = TAB(T_VBG, ip);
-        const real ebal = TAB(T_EBAL, ip);
-        real px = real(0), py = real(0);
-        if (d.n_planet > 0) planet_acc(pk, d.phitab + 4 * j, R, px, py);
+        const real h = (real)(0.5 * d.ctl->dt);
         const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
-        const real irho = FM<real>::rcp(rho);
-        real vx = real(0), vy = real(0);
-        if (d.visc) { vx = gmem(d.vis)[o] * irho; vy = gmem(d.vis)[N + o] * irho; }
-        const real ay = vy + py;               // fy + gy, independent of the state
-        real Tu = cu, Tvp = cvp;
-        real Qu = 0, Qvp = 0, Du = 0, Dvp = 0, ts = 0, its = 0;
-        if (d.has_dust) {
-            Qu = dust[N + o]; Qvp = dust[2 * N + o];
-            Du = Qu; Dvp = Qvp;
-            ts = TAB(T_TS, ip) * irho;            // St / (c_s/h0) / rho (dust.cu:1-4)
-            its = rho * TAB(T_ITS, ip);
+        real Vr = real(0), Vp = real(0), Qu = real(0), Qvp = real(0);
+        if (d.visc) { Vr = gmem(d.vis)[o]; Vp = gmem(d.vis)[N + o]; }
+        if (d.has_dust) { Qu = dust[N + o]; Qvp = dust[2 * N + o]; }
+        real nu_ = cu, nvp = cvp, ndu = Qu, ndvp = Qvp;
+        if (mode != 3) {
+            if (d.has_dust && mode != 1)
+                cell_sources<real, true, true>(d, mode, ip, j, h, rho, cu, cvp, Vr, Vp, Qu, Qvp, nu_, nvp, ndu, ndvp);
+            else
+                cell_sources<real, true, false>(d, mode, ip, j, h, rho, cu, cvp, Vr, Vp, Qu, Qvp, nu_, nvp, ndu, ndvp);
         }
-        // fx + gx at azimuthal velocity vbg + vp: centrifugal - stellar
-        // gravity (tabulated at vbg) + deviation part + planet (+ viscosity)
-        auto axd = [&](real vp) { return ebal + vp * (real(2) * vbg + vp) * iR + px; };
-        auto axg = [&](real vp) { return axd(vp) + vx; };
-        if (mode == 1 || mode == 2) {
-            Tu = cu + axg(cvp) * mdt;
-            Tvp = cvp + ay * mdt;
-            if (d.has_dust) {
-                const real dtau = one_minus_exp_neg(mdt * its);
-                Du = Qu + (axd(Dvp) * ts + cu - Qu) * dtau;
-                Dvp = Qvp + ((cvp - Qvp) + py * ts) * dtau;
-            }
-        }
-        real nu_ = cu + axg(Tvp) * dt;
-        real nvp = cvp + ay * dt;
         real nr_ = rho;
-        if (mode == 3) { nu_ = cu; nvp = cvp; }   // wave killing alone
         if (mode == 1) {
             gasT[o] = nr_; gasT[N + o] = nu_; gasT[2 * N + o] = nvp;
         } else {
-            real ndu = Qu, ndvp = Qvp;
-            if (d.has_dust && mode != 3) {
-                const real dtau = one_minus_exp_neg(dt * its);
-                ndu = Qu + (axd(Dvp) * ts + Tu - Qu) * dtau;
-                ndvp = Qvp + ((Tvp - Qvp) + py * ts) * dtau;
-            }
             if (do_kill) {   // killwave on the corrected state (solve.cu:287-289)
                 const real kk = TAB(T_KILL, ip) * (real)d.ctl->dt;
                 nr_ += kk * (TAB(T_G_R, ip) - nr_);
@@ -670,11 +736,160 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
                 const real cs = TAB(T_C0, ip);
                 real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
                 if (d.has_dust) { sx = fmax(fabs(ndu), sx); sy = fmax(fabs(ndvp), sy); }
-                rate = fmax(sx * TAB(T_IDX, ip), sy * d.idy * iR);
+                rate = fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip));
             }
         }
     }
     if (do_rate) block_max_store(d.rate_part + w.y, rate);
+}
+
+// Stencil tile of the fused kernels: the PT_X x PT_Y cells of the tile and
+// a one-cell halo, (PT_X+2) x (PT_Y+2) LDS slots; slot (lx, ly) is radius
+// i0-1+lx, row j0-1+ly (periodic).  Radial ghosts hold the analytic state.
+constexpr int ST_X = PT_X + 2, ST_Y = PT_Y + 2, ST_N = ST_X * ST_Y;
+
+// The 9-point viscosity of the tile's own cells from a staged state.
+template <typename real>
+__device__ __forceinline__ void vis_from_tile(const DiskArgs<real>& d, const real* r_, const real* u_,
+                                              const real* v_, int i0, int j0, PG_GLOBAL real* out) {
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int lx = (int)(threadIdx.x % PT_X) + 1, ly = (int)(threadIdx.x / PT_X) + 1;
+    const int i = i0 + lx - 1, j = j0 + ly - 1;
+    if (i >= nx || j >= ny) return;
+    const int ip = i + NPAD;
+    real nu[3];
+    for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
+    const int rows[3] = {ly, ly - 1, ly + 1};
+    real nr[9], u[9], v[9];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int n = 3 * b + a, e = rows[b] * ST_X + lx - 1 + a;
+            nr[n] = r_[e] * nu[a];
+            u[n] = u_[e];
+            v[n] = v_[e];
+        }
+    real fx, fy;
+    vis_cell(d, ip, nr, u, v, fx, fy);
+    const size_t o = (size_t)j * nx + i;
+    out[o] = fx;
+    out[N + o] = fy;
+}
+
+// Predictor + viscosity of the predicted state (apply_source_terms with
+// mdt != 0 into T, then viscosity_tensor_evaluation2; solve.cu:282-285):
+// the predicted state of the tile and its halo is formed in LDS from the
+// corrected-sweep state S and the viscous forces of the step start
+// (d.vis), and only its viscous forces (into d.vis2) leave the kernel.
+template <typename real>
+__global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
+    __shared__ real Tr[ST_N], Tu[ST_N], Tv[ST_N];
+    const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;
+    const DiskArgs<real>& d = ls.disks[w.x];
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int ntx = pt_ntx(nx);
+    const int ty = w.y / ntx;
+    const int i0 = (w.y - ty * ntx) * PT_X, j0 = ty * PT_Y;
+    const PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
+    const PG_GLOBAL real* __restrict__ vis = gmem(d.vis);
+    const real h = (real)(0.5 * d.ctl->dt);
+    for (int e = threadIdx.x; e < ST_N; e += blockDim.x) {
+        const int lx = e % ST_X, ly = e / ST_X;
+        const int i = i0 - 1 + lx, j = wrap(j0 - 1 + ly, ny);
+        real r, u, v;
+        if (i < 0 || i >= nx) {
+            const int ip = min(i, nx) + NPAD;   // slots past the ghost feed no output
+            r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); v = TAB(T_G_V, ip);
+        } else {
+            const size_t o = (size_t)j * nx + i;
+            const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
+            real nu_, nvp, unused0, unused1;
+            cell_sources<real, true, false>(d, 1, i + NPAD, j, h, rho, cu, cvp, vis[o], vis[N + o], real(0), real(0),
+                                            nu_, nvp, unused0, unused1);
+            r = rho; u = nu_; v = nvp;
+        }
+        Tr[e] = r; Tu[e] = u; Tv[e] = v;
+    }
+    __syncthreads();
+    vis_from_tile(d, Tr, Tu, Tv, i0, j0, gmem(d.vis2));
+}
+
+// Corrector (apply_source_terms_inplace with the viscous forces of the
+// predicted state, d.vis2; solve.cu:286-287), wave killing
+// (solve.cu:288-289) and the CFL rate of the result (timestep.cu:18-97),
+// then the viscous forces of the corrected state for the next step's first
+// half kick (into d.vis).  The corrected state of the halo is recomputed
+// from its own inputs, so the tile needs no neighbour's result; the gas
+// result goes to the other buffer of the pair, because neighbouring tiles
+// read this tile's cells as their halo input.  (The dust is corrected in
+// place: its halo results are not used.)
+template <typename real>
+__global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, int do_kill) {
+    __shared__ real Sr[ST_N], Su[ST_N], Sv[ST_N];
+    const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;
+    const DiskArgs<real>& d = ls.disks[w.x];
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int ntx = pt_ntx(nx);
+    const int ty = w.y / ntx;
+    const int i0 = (w.y - ty * ntx) * PT_X, j0 = ty * PT_Y;
+    const PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
+    PG_GLOBAL real* __restrict__ gasN = gmem(d.gas[gsel(par) ^ 1]);
+    PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
+    const PG_GLOBAL real* __restrict__ vis2 = gmem(d.vis2);
+    const real h = (real)(0.5 * d.ctl->dt);
+    real rate = real(0);
+    for (int e = threadIdx.x; e < ST_N; e += blockDim.x) {
+        const int lx = e % ST_X, ly = e / ST_X;
+        const int i = i0 - 1 + lx, jr = j0 - 1 + ly, j = wrap(jr, ny);
+        real r, u, v;
+        if (i < 0 || i >= nx) {
+            const int ip = min(i, nx) + NPAD;
+            r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); v = TAB(T_G_V, ip);
+        } else {
+            const int ip = i + NPAD;
+            const size_t o = (size_t)j * nx + i;
+            const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
+            real Qu = real(0), Qvp = real(0);
+            if (d.has_dust) { Qu = dust[N + o]; Qvp = dust[2 * N + o]; }
+            real nu_ = cu, nvp = cvp, ndu = Qu, ndvp = Qvp;
+            if (d.has_dust)
+                cell_sources<real, true, true>(d, 2, ip, j, h, rho, cu, cvp, vis2[o], vis2[N + o], Qu, Qvp, nu_, nvp,
+                                               ndu, ndvp);
+            else
+                cell_sources<real, true, false>(d, 2, ip, j, h, rho, cu, cvp, vis2[o], vis2[N + o], Qu, Qvp, nu_, nvp,
+                                                ndu, ndvp);
+            real nr_ = rho;
+            if (do_kill) {
+                const real kk = TAB(T_KILL, ip) * (real)d.ctl->dt;
+                nr_ += kk * (TAB(T_G_R, ip) - nr_);
+                nu_ += kk * (TAB(T_G_U, ip) - nu_);
+                nvp += kk * (real(0) - nvp);
+            }
+            r = nr_; u = nu_; v = nvp;
+            // the tile's own cells are written back, with their CFL rate
+            const bool own = lx >= 1 && lx <= PT_X && ly >= 1 && ly <= PT_Y && jr < ny;
+            if (own) {
+                gasN[o] = nr_;
+                gasN[N + o] = nu_;
+                gasN[2 * N + o] = nvp;
+                if (d.has_dust) { dust[N + o] = ndu; dust[2 * N + o] = ndvp; }
+                const real cs = TAB(T_C0, ip);
+                real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
+                if (d.has_dust) { sx = fmax(fabs(ndu), sx); sy = fmax(fabs(ndvp), sy); }
+                rate = fmax(rate, fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip)));
+            }
+        }
+        Sr[e] = r; Su[e] = u; Sv[e] = v;
+    }
+    block_max_store(d.rate_part + w.y, rate);   // includes the barrier the tile needs
+    __syncthreads();
+    vis_from_tile(d, Sr, Su, Sv, i0, j0, gmem(d.vis));
 }
 
 // CFL rate of the current state (get_dt_lv1, timestep.cu:18-97).
@@ -685,13 +900,14 @@ __global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int segs = (nx + 255) >> 8;
-    const int j = w.y / segs;
-    const int i = (w.y - j * segs) * 256 + threadIdx.x;
+    const int ntx = pt_ntx(nx);
+    const int ty = w.y / ntx;
+    const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
+    const int j = ty * PT_Y + (int)(threadIdx.x / PT_X);
     const PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
     const PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
     real rate = real(0);
-    if (i < nx) {
+    if (i < nx && j < ny) {
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
         const real cs = TAB(T_C0, ip);
@@ -794,30 +1010,30 @@ static hipError_t set_lds(Kern k, size_t lds) {
 }
 
 template <typename real, int TX>
-static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, bool dust, hipStream_t s) {
+static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, bool dust, int kick, hipStream_t s) {
     const size_t S = TX + 2 * NPAD;
     if (dust) {
         size_t lds = 13 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_dust<real, TX>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par);
+        hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par, kick);
     } else {
         size_t lds = 18 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_gas<real, TX>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par);
+        hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par, kick);
     }
     return hipGetLastError();
 }
 
 template <typename real>
-static hipError_t sweep_x_any(const LaunchSet<real>& ls, int par, bool dust, hipStream_t s) {
+static hipError_t sweep_x_any(const LaunchSet<real>& ls, int par, bool dust, int kick, hipStream_t s) {
     switch (ls.tile_x) {
-        case 128: return launch_sweep_x<real, 128>(ls, par, dust, s);
-        case 384: return launch_sweep_x<real, 384>(ls, par, dust, s);
-        case 512: return launch_sweep_x<real, 512>(ls, par, dust, s);
-        case 1024: return launch_sweep_x<real, 1024>(ls, par, dust, s);
-        default: return launch_sweep_x<real, 256>(ls, par, dust, s);
+        case 128: return launch_sweep_x<real, 128>(ls, par, dust, kick, s);
+        case 384: return launch_sweep_x<real, 384>(ls, par, dust, kick, s);
+        case 512: return launch_sweep_x<real, 512>(ls, par, dust, kick, s);
+        case 1024: return launch_sweep_x<real, 1024>(ls, par, dust, kick, s);
+        default: return launch_sweep_x<real, 256>(ls, par, dust, kick, s);
     }
 }
 
@@ -843,8 +1059,8 @@ template <typename real>
 struct Launch {
     static int phi_tile_radii(int B) { return B == 256 ? PhiTile<256>::TR : B == 1024 ? PhiTile<1024>::TR : PhiTile<512>::TR; }
     static int phi_tile_rows(int B) { return B == 256 ? PhiTile<256>::TY : B == 1024 ? PhiTile<1024>::TY : PhiTile<512>::TY; }
-    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, hipStream_t s) { return sweep_x_any(ls, par, false, s); }
-    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, hipStream_t s) { return sweep_x_any(ls, par, true, s); }
+    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, false, kick, s); }
+    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, true, kick, s); }
     static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, false, s); }
     static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, true, s); }
     static hipError_t viscosity(const LaunchSet<real>& ls, int par, int which, hipStream_t s) {
@@ -853,6 +1069,14 @@ struct Launch {
     }
     static hipError_t sources(const LaunchSet<real>& ls, int par, int mode, int kill, int rate, hipStream_t s) {
         hipLaunchKernelGGL(k_sources<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par, mode, kill, rate);
+        return hipGetLastError();
+    }
+    static hipError_t predvis(const LaunchSet<real>& ls, int par, hipStream_t s) {
+        hipLaunchKernelGGL(k_predvis<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par);
+        return hipGetLastError();
+    }
+    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s) {
+        hipLaunchKernelGGL(k_corrvis<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par, kill);
         return hipGetLastError();
     }
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s) {

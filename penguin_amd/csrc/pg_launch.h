@@ -12,12 +12,18 @@ template <typename real>
 struct Launch {
     static int phi_tile_radii(int B);
     static int phi_tile_rows(int B);
-    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, hipStream_t s);
-    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, hipStream_t s);
+    // kick != 0: the first half kick applied to the loaded cells (dust: the
+    // pre-kick gas is in the buffer opposite the current one)
+    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, int kick, hipStream_t s);
+    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, int kick, hipStream_t s);
     static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
     static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
     static hipError_t viscosity(const LaunchSet<real>& ls, int par, int which, hipStream_t s);
     static hipError_t sources(const LaunchSet<real>& ls, int par, int mode, int kill, int rate, hipStream_t s);
+    // fused: predictor + viscosity of T (d.vis -> d.vis2); corrector + kill +
+    // CFL rate + viscosity of the corrected state (d.vis2 -> d.vis)
+    static hipError_t predvis(const LaunchSet<real>& ls, int par, hipStream_t s);
+    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s);
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t rate_reduce(const LaunchSet<real>& ls, hipStream_t s);
     static hipError_t prologue(const LaunchSet<real>& ls, int mode, int do_orbadv, hipStream_t s);

@@ -105,8 +105,8 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
-KERNELS = ("prologue", "viscosity", "sources_kick", "sweep_x_gas", "ring_gas", "sweep_x_dust",
-           "ring_dust", "sources_predictor", "sources_corrector", "cfl")
+KERNELS = ("prologue", "viscosity", "sweep_x_gas", "ring_gas", "sweep_x_dust", "ring_dust",
+           "predictor_viscosity", "corrector_viscosity", "sources_corrector", "cfl")
 
 
 class Disk:

@@ -124,6 +124,6 @@ def test_batch_profile_and_introspection():
     assert b.device_bytes() > 0
     b.profile(True)
     b.run(3, use_graph=False)
-    for name in ("sweep_x_gas", "ring_gas", "sources_corrector", "prologue"):
+    for name in ("sweep_x_gas", "ring_gas", "corrector_viscosity", "prologue"):
         us, cnt = b.kernel_time_us(name)
         assert cnt == 3 and us > 0.0, name
