@@ -153,6 +153,15 @@ def batched_leg(ndisk: int, steps: int, precision: int) -> dict:
     b.run(steps)
     tb = time.perf_counter() - t0
     sched = b.schedule()
+    # per-pass device time of the batch, from HIP events around each launch
+    # of a short uncaptured run after the timed region
+    b.profile(True)
+    b.run(10, use_graph=False)
+    passes = {}
+    for name in WORDS_PER_CELL:
+        us, cnt = b.kernel_time_us(name)
+        if cnt:
+            passes[name] = round(us, 2)
     b.close()
     tl = 0.0
     for p in ps:
@@ -167,8 +176,9 @@ def batched_leg(ndisk: int, steps: int, precision: int) -> dict:
     return {"disks": ndisk, "steps": steps, "cells": cells, "value": round(vb, 1), "unit": "cell-updates/s",
             "ms_per_step": round(1e3 * tb / steps, 4), "looped_value": round(vl, 1),
             "looped_ms_per_step": round(1e3 * tl / steps, 4), "speedup": round(vb / vl, 3),
-            "schedule": {"tile_x": sched[0], "ring_threads": sched[1], "ring_order": sched[2],
+            "schedule": {"tile_x": sched[0], "ring_threads": sched[1], "ring_order": sched[2], "pt_rows": sched[3],
                          "autotune_ms_per_step": round(tune_ms, 4)},
+            "pass_us": passes,
             "workload": f"{ndisk} 2D gas disks, nr in {{128,256,384,512}}, nphi in {{256,512,1024,1536}}, "
                         "q logU[1e-5,1e-3], alpha logU[1e-4,1e-2], h U[0.03,0.08], seed 1"}
 

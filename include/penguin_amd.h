@@ -174,12 +174,13 @@ int pg_batch_set_state(pg_batch* b, int disk, const double* gas, const double* d
 /* nsteps of every disk; t_io, dt_io: n entries each (time and previous step
    per disk, in and out); dts (n*nsteps, disk-major) may be NULL */
 int pg_batch_run(pg_batch* b, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph);
-/* launch schedule: x-sweep tile width (128, 256, 384, 512, 1024), threads
-   per azimuthal ring block (256, 512, 1024), ring order (0: longest rings
-   first, XCD-contiguous radii; 1: disk order, XCD-contiguous; 2: longest
-   first, plain radius order) */
-int pg_batch_set_schedule(pg_batch* b, int tile_x, int ring_b, int order);
-int pg_batch_get_schedule(pg_batch* b, int* tile_x, int* ring_b, int* order);
+/* launch schedule: x-sweep tile width (128, 256, 384, 512, 1024); threads
+   per azimuthal tile (256: 16 radii x 16 rows, 512: 16 x 32, 1024: 16 x 64);
+   azimuthal tile order (0: each XCD a contiguous band of tile rows of every
+   disk, 1: row by row, 2: radius strip by strip); rows of the pointwise and
+   stencil tiles (4, 8 or 16; 0 = chosen from the batch size) */
+int pg_batch_set_schedule(pg_batch* b, int tile_x, int ring_b, int order, int pt_rows);
+int pg_batch_get_schedule(pg_batch* b, int* tile_x, int* ring_b, int* order, int* pt_rows);
 /* time `steps` captured steps of each schedule candidate on the batch's
    own state (restored afterwards) and keep the fastest; *best_ms (may be
    NULL) receives its device time per step */

@@ -284,6 +284,7 @@ struct Group {
     size_t wl_cap = 0;
     int n_x = 0, n_ring = 0, n_pt = 0;
     int tile_x = 256, ring_b = RING_THREADS, order = 0;
+    int pt_rows = PT_YMAX;       // rows of the pointwise/stencil tiles
     int ny_max = 0;
     // profiling: event pairs recorded around launches without a host wait,
     // resolved after the stream drains
@@ -315,6 +316,7 @@ struct Group {
         L.ny_max = ny_max;
         L.wl_pt = d_wl + n_x + n_ring;
         L.n_pt = n_pt;
+        L.pt_rows = pt_rows;
         return L;
     }
     void set_par(int p) {
@@ -562,8 +564,9 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
         (rc = dev_alloc(B, &h->inc, h->nx * sizeof(int))) || (rc = dev_alloc(B, &h->rot, rot.size() * sizeof(double))) ||
         (rc = dev_alloc(B, &h->ctl, sizeof(StepCtl))))
         return cleanup(rc);
-    // one rate partial per 64-radius x 4-row tile of the pointwise kernels
-    h->rate_nblk = ((h->nx + 63) / 64) * ((h->ny + 3) / 4);
+    // one rate partial per tile of the pointwise kernels
+    // (sized for the shortest tiles; the schedule sets the count in use)
+    h->rate_nblk = ((h->nx + PT_X - 1) / PT_X) * ((h->ny + PT_TY - 1) / PT_TY);
     if ((rc = dev_alloc(B, &h->rate_part, sizeof(double) * h->rate_nblk))) return cleanup(rc);
     if (P.dust_flag) {
         if ((rc = dev_alloc(B, &h->dust[0], 3 * N * es)) || (rc = dev_alloc(B, &h->dust[1], 3 * N * es)))
@@ -599,7 +602,7 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
 // ------------------------------------------------------------ groups
 // Work lists of one launch schedule.
 //   x sweeps: every (tile, row) of every disk, disk by disk; pointwise and
-//   stencil kernels: every 64-radius x 4-row tile, tile rows in
+//   stencil kernels: every PT_X x pt_rows tile, tile rows in
 //   XCD-contiguous bands.
 //   azimuthal pass: tiles of TR radii x TY rows (tile = ty * ntr + tr).
 //   Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -609,7 +612,7 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
 //   run x (entry 8k+x of the list), radius strips innermost.  Order 1 lists
 //   the tiles row by row, order 2 strip by strip.  Entries with tile -1 pad
 //   the list.
-void build_worklists(const Group& g, int tile_x, int ring_b, int order, std::vector<int2>& wx,
+void build_worklists(const Group& g, int tile_x, int ring_b, int order, int pt_rows, std::vector<int2>& wx,
                      std::vector<int2>& wr, std::vector<int2>& wp) {
     wx.clear();
     wr.clear();
@@ -623,7 +626,7 @@ void build_worklists(const Group& g, int tile_x, int ring_b, int order, std::vec
         const DiskDev* d = g.disks[k];
         const int tiles = (d->nx + tile_x - 1) / tile_x;
         for (int b = 0; b < tiles * d->ny; ++b) wx.push_back(make_int2(k, b));
-        const int ntx = (d->nx + 63) / 64, nty = (d->ny + 3) / 4;   // 64-radius x 4-row tiles
+        const int ntx = (d->nx + PT_X - 1) / PT_X, nty = (d->ny + pt_rows - 1) / pt_rows;
         const int per = (nty + 7) / 8;
         for (int x = 0; x < 8; ++x)
             for (int ty = x * per; ty < std::min((x + 1) * per, nty); ++ty)
@@ -668,12 +671,26 @@ bool tile_ok(int t) { return t == 128 || t == 256 || t == 384 || t == 512 || t =
 bool ring_ok(int b) { return b == 256 || b == 512 || b == 1024; }
 
 // Install a launch schedule: rebuild and upload the work lists.
-int group_schedule(Group& g, int tile_x, int ring_b, int order) {
+int group_upload_args(Group& g);
+
+// Rows of the pointwise/stencil tiles when the caller leaves the choice:
+// tall tiles (a smaller stencil halo) when the set has enough of them to
+// give every CU several workgroups, short ones otherwise.
+int auto_pt_rows(const Group& g) {
+    long tall = 0;
+    for (const DiskDev* d : g.disks)
+        tall += (long)((d->nx + PT_X - 1) / PT_X) * ((d->ny + PT_YMAX - 1) / PT_YMAX);
+    return tall >= 1024 ? PT_YMAX : PT_TY;
+}
+
+int group_schedule(Group& g, int tile_x, int ring_b, int order, int pt_rows) {
     if (!tile_ok(tile_x)) return fail(PG_E_INVALID, "tile_x=%d: use 128, 256, 384, 512 or 1024", tile_x);
     if (!ring_ok(ring_b)) return fail(PG_E_INVALID, "ring_b=%d: use 256, 512 or 1024", ring_b);
     if (order < 0 || order > 2) return fail(PG_E_INVALID, "order=%d: use 0, 1 or 2", order);
+    if (pt_rows == 0) pt_rows = auto_pt_rows(g);
+    if (pt_rows != 4 && pt_rows != 8 && pt_rows != 16) return fail(PG_E_INVALID, "pt_rows=%d: use 4, 8 or 16", pt_rows);
     std::vector<int2> wx, wr, wp;
-    build_worklists(g, tile_x, ring_b, order, wx, wr, wp);
+    build_worklists(g, tile_x, ring_b, order, pt_rows, wx, wr, wp);
     const size_t need = wx.size() + wr.size() + wp.size();
     HIP_TRY(hipStreamSynchronize(g.stream));
     if (need > g.wl_cap) {
@@ -696,8 +713,11 @@ int group_schedule(Group& g, int tile_x, int ring_b, int order) {
     g.tile_x = tile_x;
     g.ring_b = ring_b;
     g.order = order;
+    g.pt_rows = pt_rows;
+    // one CFL-rate partial per pointwise tile
+    for (DiskDev* d : g.disks) d->rate_nblk = ((d->nx + PT_X - 1) / PT_X) * ((d->ny + pt_rows - 1) / pt_rows);
     g.drop_graph();
-    return PG_OK;
+    return group_upload_args(g);
 }
 
 // Upload the per-disk descriptors (after creation or a reallocation).
@@ -785,7 +805,7 @@ int group_init(Group& g, int n, const pg_params* params) {
     ringb = env_int("PG_RING_B", ringb);
     if (!tile_ok(tile)) tile = 256;
     if (!ring_ok(ringb)) ringb = RING_THREADS;
-    return group_schedule(g, tile, ringb, 0);
+    return group_schedule(g, tile, ringb, 0, env_int("PG_PT_ROWS", 0));
 }
 
 // ------------------------------------------------------------ launching
@@ -1132,41 +1152,44 @@ int group_autotune(Group& g, int steps, double* best_ms) {
     const bool prof = g.profile;
     g.profile = false;
     const int tiles[] = {128, 256, 512};
-    const int rings[] = {256, 512};
+    const int rings[] = {256, 512, 1024};
+    const int pts[] = {PT_TY, PT_YMAX};
     double best = 1e300;
-    int bt = g.tile_x, bb = g.ring_b, bo = g.order;
+    int bt = g.tile_x, bb = g.ring_b, bo = g.order, bp = g.pt_rows;
     std::vector<double> t(nd), dt(nd);
     for (int tx : tiles)
         for (int rb : rings)
-            for (int order = 0; order < 3; ++order) {
-                if ((rc = group_schedule(g, tx, rb, order))) {
-                    if (rc == PG_E_INVALID) continue;  // does not fit this batch
-                    g.profile = prof;
-                    return free_saved(), rc;
+            for (int order = 0; order < 3; ++order)
+                for (int pr : pts) {
+                    if ((rc = group_schedule(g, tx, rb, order, pr))) {
+                        if (rc == PG_E_INVALID) continue;  // does not fit this batch
+                        g.profile = prof;
+                        return free_saved(), rc;
+                    }
+                    float ms = 0.f;
+                    for (int k = 0; k < nd; ++k) {
+                        t[k] = saved_ctl[k].t;
+                        dt[k] = saved_ctl[k].dt > 0.0 ? saved_ctl[k].dt : g.disks[k]->P.end_time;
+                    }
+                    // first call captures the graph and warms up; the second is timed
+                    if ((rc = group_run(g, 1, t.data(), dt.data(), nullptr, 1, nullptr)) ||
+                        (rc = group_run(g, steps, t.data(), dt.data(), nullptr, 1, &ms)) || (rc = restore())) {
+                        g.profile = prof;
+                        return free_saved(), rc;
+                    }
+                    const double per = ms / steps;
+                    if (per < best) {
+                        best = per;
+                        bt = tx;
+                        bb = rb;
+                        bo = order;
+                        bp = pr;
+                    }
                 }
-                float ms = 0.f;
-                for (int k = 0; k < nd; ++k) {
-                    t[k] = saved_ctl[k].t;
-                    dt[k] = saved_ctl[k].dt > 0.0 ? saved_ctl[k].dt : g.disks[k]->P.end_time;
-                }
-                // first call captures the graph and warms up; the second is timed
-                if ((rc = group_run(g, 1, t.data(), dt.data(), nullptr, 1, nullptr)) ||
-                    (rc = group_run(g, steps, t.data(), dt.data(), nullptr, 1, &ms)) || (rc = restore())) {
-                    g.profile = prof;
-                    return free_saved(), rc;
-                }
-                const double per = ms / steps;
-                if (per < best) {
-                    best = per;
-                    bt = tx;
-                    bb = rb;
-                    bo = order;
-                }
-            }
     g.profile = prof;
     free_saved();
     g_err.clear();
-    if ((rc = group_schedule(g, bt, bb, bo))) return rc;
+    if ((rc = group_schedule(g, bt, bb, bo, bp))) return rc;
     if (best_ms) *best_ms = best;
     return PG_OK;
 }
@@ -1180,9 +1203,10 @@ int group_autotune(Group& g, int steps, double* best_ms) {
 // steps in reverse order.
 constexpr int SLOT_WORDS = 22;   // S0, S1, S2, SA, S3, T (3 fields each), V0, V1 (2 each)
 enum { SL_S0 = 0, SL_S1 = 3, SL_S2 = 6, SL_SA = 9, SL_S3 = 12, SL_T = 15, SL_V0 = 18, SL_V1 = 20 };
-// forward passes of one slot, each with its own disk descriptor (the
-// buffers it reads and writes)
-enum { AA_V0, AA_KICK, AA_X, AA_PA, AA_PB, AA_PRED, AA_V1, AA_CORR, AA_N };
+// forward passes of one slot (the fused kernels of the forward step, which
+// also store the intermediates the adjoint needs), each with its own disk
+// descriptor naming the buffers it reads and writes
+enum { AA_V0, AA_X, AA_P, AA_PRED, AA_CORR, AA_N };
 
 }  // namespace
 
@@ -1259,24 +1283,26 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
         real* sl = base + (size_t)m * SLOT_WORDS * N;
         real* nx = base + (size_t)std::min(m + 1, w->slots - 1) * SLOT_WORDS * N;
         const DiskArgs<real> t = d->args<real>();
-        auto set = [&](int k, real* g0, real* g1, real* vis) {
+        // parity 0 throughout: gas[0] is a pass's input, gas[1] its output
+        auto set = [&](int k, real* g0, real* g1, real* vis, real* vis2) -> DiskArgs<real>& {
             DiskArgs<real> x = t;
             x.gas[0] = g0;
             x.gas[1] = g1;
             x.dust[0] = x.dust[1] = nullptr;
             x.vis = vis;
+            x.vis2 = vis2;
             x.dts = nullptr;
             x.dts_cap = 0;
-            a[(size_t)m * AA_N + k] = x;
+            return a[(size_t)m * AA_N + k] = x;
         };
-        set(AA_V0, sl + SL_S0 * N, sl + SL_S0 * N, sl + SL_V0 * N);
-        set(AA_KICK, sl + SL_S1 * N, sl + SL_S1 * N, sl + SL_V0 * N);
-        set(AA_X, sl + SL_S1 * N, sl + SL_S2 * N, sl + SL_V0 * N);
-        set(AA_PA, sl + SL_S2 * N, sl + SL_SA * N, sl + SL_V0 * N);
-        set(AA_PB, sl + SL_SA * N, sl + SL_S3 * N, sl + SL_V0 * N);
-        set(AA_PRED, sl + SL_S3 * N, sl + SL_T * N, sl + SL_V0 * N);
-        set(AA_V1, sl + SL_T * N, sl + SL_T * N, sl + SL_V1 * N);
-        set(AA_CORR, nx + SL_S0 * N, nx + SL_S0 * N, sl + SL_V1 * N);
+        set(AA_V0, sl + SL_S0 * N, sl + SL_S0 * N, sl + SL_V0 * N, nullptr);
+        set(AA_X, sl + SL_S0 * N, sl + SL_S2 * N, sl + SL_V0 * N, nullptr).out_kick = sl + SL_S1 * N;
+        set(AA_P, sl + SL_S2 * N, sl + SL_S3 * N, nullptr, nullptr).out_swept = sl + SL_SA * N;
+        set(AA_PRED, sl + SL_S3 * N, nullptr, sl + SL_V0 * N, sl + SL_V1 * N).out_pred = sl + SL_T * N;
+        if (d->P.visc_flag)   // corrector + viscosity of the result: next slot's S0 and V0
+            set(AA_CORR, sl + SL_S3 * N, nx + SL_S0 * N, nx + SL_V0 * N, sl + SL_V1 * N);
+        else                  // corrector in place on a copy of S3
+            set(AA_CORR, nx + SL_S0 * N, nx + SL_S0 * N, nullptr, nullptr);
     }
     if (hipMemcpy(w->args, a.data(), a.size() * sizeof(a[0]), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(PG_E_HIP, "adjoint descriptor upload failed"));
@@ -1319,19 +1345,22 @@ int adj_recompute(pg_handle* h, int m, int n, bool write_next) {
     real* sl = (real*)w->slot + (size_t)m * SLOT_WORDS * N;
     real* nxt = (real*)w->slot + (size_t)(m + 1) * SLOT_WORDS * N;
     const bool visc = g.visc;
-    if (visc) HIPL(Launch<real>::viscosity(at(AA_V0), 0, 0, s));
-    HIPL(hipMemcpyAsync(sl + SL_S1 * N, sl + SL_S0 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
-    HIPL(Launch<real>::sources(at(AA_KICK), 0, 0, 0, 0, s));
-    HIPL(Launch<real>::sweep_x_gas(at(AA_X), 0, 0, s));
-    HIPL(Launch<real>::ring_gas(at(AA_PA), 0, 1, s));
-    HIPL(Launch<real>::ring_gas(at(AA_PB), 0, 2, s));
-    if (visc) {
-        HIPL(Launch<real>::sources(at(AA_PRED), 0, 1, 0, 0, s));
-        HIPL(Launch<real>::viscosity(at(AA_V1), 0, 0, s));
-    }
+    // the forward step's fused kernels (as group_run launches them), with
+    // every intermediate landing in the slot: the viscous forces of a
+    // segment's first state, then kick + x sweep, azimuthal sweep + shift +
+    // residual advection, predictor + its viscous forces, corrector + the
+    // viscous forces of the result (the next slot's S0 and V0)
+    if (visc && m == 0) HIPL(Launch<real>::viscosity(at(AA_V0), 0, 0, s));
+    HIPL(Launch<real>::sweep_x_gas(at(AA_X), 0, 1, s));
+    HIPL(Launch<real>::ring_gas(at(AA_P), 0, 3, s));
+    if (visc) HIPL(Launch<real>::predvis(at(AA_PRED), 0, s));
     if (write_next) {
-        HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
-        HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 0, s));
+        if (visc) {
+            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, s));
+        } else {
+            HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
+            HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 0, s));
+        }
     }
     return PG_OK;
 }
@@ -2002,17 +2031,18 @@ int pg_batch_run(pg_batch* b, int nsteps, double* t_io, double* dt_io, double* d
     return group_run(b->g, nsteps, t_io, dt_io, dts, use_graph, nullptr);
 }
 
-int pg_batch_set_schedule(pg_batch* b, int tile_x, int ring_b, int order) {
+int pg_batch_set_schedule(pg_batch* b, int tile_x, int ring_b, int order, int pt_rows) {
     CHECK_H(b);
-    return group_schedule(b->g, tile_x, ring_b, order);
+    return group_schedule(b->g, tile_x, ring_b, order, pt_rows);
 }
 
-int pg_batch_get_schedule(pg_batch* b, int* tile_x, int* ring_b, int* order) {
+int pg_batch_get_schedule(pg_batch* b, int* tile_x, int* ring_b, int* order, int* pt_rows) {
     CHECK_H(b);
-    if (!tile_x || !ring_b || !order) return fail(PG_E_INVALID, "null argument");
+    if (!tile_x || !ring_b || !order || !pt_rows) return fail(PG_E_INVALID, "null argument");
     *tile_x = b->g.tile_x;
     *ring_b = b->g.ring_b;
     *order = b->g.order;
+    *pt_rows = b->g.pt_rows;
     return PG_OK;
 }
 

@@ -26,6 +26,12 @@
 namespace pg {
 
 constexpr int NPAD = 3;
+// Tiles of the pointwise and stencil kernels: PT_X radii x pt_rows rows
+// (a launch-set parameter, a multiple of PT_TY up to PT_YMAX), 256 threads
+// of PT_X x PT_TY, each thread covering pt_rows / PT_TY rows.  Tall tiles
+// keep the one-cell halo of the stencil kernels small; short ones give a
+// small disk more workgroups.
+constexpr int PT_X = 64, PT_TY = 4, PT_YMAX = 16;
 constexpr int NGAS = 3;
 constexpr int NDUST = 3;
 constexpr int RING_THREADS = 512;   // default threads per azimuthal-pass block
@@ -111,6 +117,13 @@ struct DiskArgs {
     int rate_nblk;
     double* dts;         // step sizes recorded by the device loop (or null)
     int dts_cap;         // entries of dts
+    // intermediate states the adjoint's recompute keeps (null otherwise):
+    // the gas after the first half kick (x sweep), after the azimuthal
+    // sweep before the orbital shift (azimuthal pass), and the predicted
+    // state (predictor), 3*N each
+    real* out_kick;
+    real* out_swept;
+    real* out_pred;
 };
 
 // Buffers of one adjoint pass (pg_adjoint.hip): the forward inputs the
@@ -148,8 +161,9 @@ struct LaunchSet {
     int n_ring;
     int ring_b;          // ring chunk (threads per block)
     int ny_max;          // largest ring, sizes the LDS
-    const int2* wl_pt;   // pointwise kernels: local block = 256-radius segment + segments_per_row * row
+    const int2* wl_pt;   // pointwise and stencil kernels: local block = tile tx + ntx * ty
     int n_pt;
+    int pt_rows;         // rows of their tiles (PT_TY .. PT_YMAX)
 };
 
 }  // namespace pg

@@ -163,6 +163,13 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par,
         s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
     }
     __syncthreads();
+    if (d.out_kick) {   // the kicked state of the tile's own cells
+        PG_GLOBAL real* k1 = gmem(d.out_kick);
+        for (int s = t + NPAD; s < ns - NPAD; s += TX) {
+            const size_t o = row + x0 - NPAD + s;
+            k1[o] = s_r[s]; k1[N + o] = s_u[s]; k1[2 * N + o] = s_vp[s];
+        }
+    }
     for (int s = t + 2; s < ns - 2; s += TX) {
         s_f[s] = flatten5(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2],
                           s_u[s - 1], s_u[s], s_u[s + 1], real(1));
@@ -347,7 +354,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
 template <int B> struct PhiTile;
 template <> struct PhiTile<256> { static constexpr int TR = 16, TY = 16; };
 template <> struct PhiTile<512> { static constexpr int TR = 16, TY = 32; };
-template <> struct PhiTile<1024> { static constexpr int TR = 32, TY = 32; };
+template <> struct PhiTile<1024> { static constexpr int TR = 16, TY = 64; };
 
 // Azimuthal pass of one fluid over a tile, fused with the orbital shift and
 // the residual advection: sweepy_inplace (sweeps.cu:361-456) or
@@ -485,6 +492,15 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
     }
     __syncthreads();
     const int rows = min(TY, ny - y0);
+    if (!DUST && d.out_swept && live) {
+        // LDS row k holds swept row y0 - 2 NPAD - inc + k; rows k in
+        // [2 NPAD, 2 NPAD + rows) of every tile cover each row once
+        PG_GLOBAL real* sw = gmem(d.out_swept);
+        for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + rows) * TR; e += B) {
+            const size_t o = (size_t)wrap(y0 - inc + e / TR - 2 * NPAD, ny) * nx + i;
+            sw[o] = c0[e]; sw[N + o] = c2[e]; sw[2 * N + o] = c1[e];
+        }
+    }
     if (!orbadv) {
         for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
             const int y = e / TR - 2 * NPAD;
@@ -548,9 +564,11 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
 }
 
 // ----------------------------------------------------------------- viscosity
-// Pointwise and stencil kernels work on tiles of PT_X radii x PT_Y rows,
-// one cell per thread; the tile id of a work-list entry is ty * ntx + tx.
-constexpr int PT_X = 64, PT_Y = 4;
+// Pointwise and stencil kernels work on tiles of PT_X radii x TH rows (the
+// launch set's pt_rows as a template parameter); thread t covers radius
+// t % PT_X of rows t / PT_X + PT_TY * k.  The tile id of a work-list entry
+// is ty * ntx + tx.
+static_assert(PT_X * PT_TY == 256 && PT_YMAX % PT_TY == 0, "pointwise tile");
 __device__ __forceinline__ int pt_ntx(int nx) { return (nx + PT_X - 1) / PT_X; }
 
 // 9-point stencil of viscous_forces (viscosity.cu:34-197, ndim 2) at the cell
@@ -597,7 +615,7 @@ __device__ __forceinline__ void vis_cell(const DiskArgs<real>& d, int ip, const 
 
 // which = 0: viscosity of the current state; 1: of the other buffer (the
 // predictor, viscosity_tensor_evaluation2).  Result in d.vis.
-template <typename real>
+template <typename real, int TH>
 __global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, int which) {
     const int2 w = ls.wl_pt[blockIdx.x];
     if (w.y < 0) return;   // padding entry of the work list (whole block)
@@ -607,39 +625,43 @@ __global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, 
     const int ntx = pt_ntx(nx);
     const int ty = w.y / ntx;
     const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
-    const int j = ty * PT_Y + (int)(threadIdx.x / PT_X);
     const PG_GLOBAL real* __restrict__ st = gmem(d.gas[gsel(par) ^ which]);
-    if (i >= nx || j >= ny) return;
+    if (i >= nx) return;
     const int ip = i + NPAD;
     real nu[3];
     for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
-    real nr[9], u[9], v[9];
-    const int jm = wrap(j - 1, ny), jp = wrap(j + 1, ny);
-    const int rows[3] = {j, jm, jp};
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
+    for (int k = 0; k < TH / PT_TY; ++k) {
+        const int j = ty * TH + (int)(threadIdx.x / PT_X) + PT_TY * k;
+        if (j >= ny) break;
+        real nr[9], u[9], v[9];
+        const int jm = wrap(j - 1, ny), jp = wrap(j + 1, ny);
+        const int rows[3] = {j, jm, jp};
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            int n = 3 * b + a;
-            int ii = i - 1 + a;
-            real rr, uu, vv;
-            if (ii < 0 || ii >= nx) {
-                rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
-            } else {
-                size_t o = (size_t)rows[b] * nx + ii;
-                rr = st[o]; uu = st[N + o]; vv = st[2 * N + o];
+        for (int b = 0; b < 3; ++b) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                int n = 3 * b + a;
+                int ii = i - 1 + a;
+                real rr, uu, vv;
+                if (ii < 0 || ii >= nx) {
+                    rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
+                } else {
+                    size_t o = (size_t)rows[b] * nx + ii;
+                    rr = st[o]; uu = st[N + o]; vv = st[2 * N + o];
+                }
+                nr[n] = rr * nu[a];
+                u[n] = uu;
+                v[n] = vv;  // deviation vp
             }
-            nr[n] = rr * nu[a];
-            u[n] = uu;
-            v[n] = vv;  // deviation vp
         }
+        real fx, fy;
+        vis_cell(d, ip, nr, u, v, fx, fy);
+        size_t o = (size_t)j * nx + i;
+        PG_GLOBAL real* vis = gmem(d.vis);
+        vis[o] = fx;
+        vis[N + o] = fy;
     }
-    real fx, fy;
-    vis_cell(d, ip, nr, u, v, fx, fy);
-    size_t o = (size_t)j * nx + i;
-    PG_GLOBAL real* vis = gmem(d.vis);
-    vis[o] = fx;
-    vis[N + o] = fy;
 }
 
 // ------------------------------------------------------------- sources
@@ -689,7 +711,7 @@ __device__ __forceinline__ double reduce_partials(const double* part, int n) {
 // 1 predictor (planet at t+dt, gas written to gasT), 2 corrector (planet at
 // t+dt, in place) optionally followed by wave killing and the CFL rate,
 // 3 wave killing alone.
-template <typename real>
+template <typename real, int TH>
 __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, int mode, int do_kill, int do_rate) {
     const int2 w = ls.wl_pt[blockIdx.x];
     if (w.y < 0) return;   // padding entry of the work list (whole block)
@@ -699,12 +721,14 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
     const int ntx = pt_ntx(nx);
     const int ty = w.y / ntx;
     const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
-    const int j = ty * PT_Y + (int)(threadIdx.x / PT_X);
     PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
     PG_GLOBAL real* __restrict__ gasT = gmem(d.gas[gsel(par) ^ 1]);
     PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
     real rate = real(0);
-    if (i < nx && j < ny) {
+#pragma unroll
+    for (int k = 0; k < TH / PT_TY; ++k) {
+        const int j = ty * TH + (int)(threadIdx.x / PT_X) + PT_TY * k;
+        if (i >= nx || j >= ny) break;
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
         const real h = (real)(0.5 * d.ctl->dt);
@@ -736,46 +760,52 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
                 const real cs = TAB(T_C0, ip);
                 real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
                 if (d.has_dust) { sx = fmax(fabs(ndu), sx); sy = fmax(fabs(ndvp), sy); }
-                rate = fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip));
+                rate = fmax(rate, fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip)));
             }
         }
     }
     if (do_rate) block_max_store(d.rate_part + w.y, rate);
 }
 
-// Stencil tile of the fused kernels: the PT_X x PT_Y cells of the tile and
-// a one-cell halo, (PT_X+2) x (PT_Y+2) LDS slots; slot (lx, ly) is radius
+// Stencil tile of the fused kernels: the PT_X x TH cells of the tile and a
+// one-cell halo, (PT_X+2) x (TH+2) LDS slots; slot (lx, ly) is radius
 // i0-1+lx, row j0-1+ly (periodic).  Radial ghosts hold the analytic state.
-constexpr int ST_X = PT_X + 2, ST_Y = PT_Y + 2, ST_N = ST_X * ST_Y;
+constexpr int ST_X = PT_X + 2;
 
 // The 9-point viscosity of the tile's own cells from a staged state.
-template <typename real>
+template <typename real, int TH>
 __device__ __forceinline__ void vis_from_tile(const DiskArgs<real>& d, const real* r_, const real* u_,
                                               const real* v_, int i0, int j0, PG_GLOBAL real* out) {
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
-    const int lx = (int)(threadIdx.x % PT_X) + 1, ly = (int)(threadIdx.x / PT_X) + 1;
-    const int i = i0 + lx - 1, j = j0 + ly - 1;
-    if (i >= nx || j >= ny) return;
+    const int lx = (int)(threadIdx.x % PT_X) + 1;
+    const int i = i0 + lx - 1;
+    if (i >= nx) return;
     const int ip = i + NPAD;
     real nu[3];
     for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
-    const int rows[3] = {ly, ly - 1, ly + 1};
-    real nr[9], u[9], v[9];
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
+    for (int k = 0; k < TH / PT_TY; ++k) {
+        const int ly = (int)(threadIdx.x / PT_X) + PT_TY * k + 1;
+        const int j = j0 + ly - 1;
+        if (j >= ny) break;
+        const int rows[3] = {ly, ly - 1, ly + 1};
+        real nr[9], u[9], v[9];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int n = 3 * b + a, e = rows[b] * ST_X + lx - 1 + a;
-            nr[n] = r_[e] * nu[a];
-            u[n] = u_[e];
-            v[n] = v_[e];
-        }
-    real fx, fy;
-    vis_cell(d, ip, nr, u, v, fx, fy);
-    const size_t o = (size_t)j * nx + i;
-    out[o] = fx;
-    out[N + o] = fy;
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int n = 3 * b + a, e = rows[b] * ST_X + lx - 1 + a;
+                nr[n] = r_[e] * nu[a];
+                u[n] = u_[e];
+                v[n] = v_[e];
+            }
+        real fx, fy;
+        vis_cell(d, ip, nr, u, v, fx, fy);
+        const size_t o = (size_t)j * nx + i;
+        out[o] = fx;
+        out[N + o] = fy;
+    }
 }
 
 // Predictor + viscosity of the predicted state (apply_source_terms with
@@ -783,9 +813,9 @@ __device__ __forceinline__ void vis_from_tile(const DiskArgs<real>& d, const rea
 // the predicted state of the tile and its halo is formed in LDS from the
 // corrected-sweep state S and the viscous forces of the step start
 // (d.vis), and only its viscous forces (into d.vis2) leave the kernel.
-template <typename real>
+template <typename real, int TH>
 __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
-    __shared__ real Tr[ST_N], Tu[ST_N], Tv[ST_N];
+    __shared__ real Tr[ST_X * (TH + 2)], Tu[ST_X * (TH + 2)], Tv[ST_X * (TH + 2)];
     const int2 w = ls.wl_pt[blockIdx.x];
     if (w.y < 0) return;
     const DiskArgs<real>& d = ls.disks[w.x];
@@ -793,11 +823,12 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
     const size_t N = (size_t)nx * ny;
     const int ntx = pt_ntx(nx);
     const int ty = w.y / ntx;
-    const int i0 = (w.y - ty * ntx) * PT_X, j0 = ty * PT_Y;
+    constexpr int th = TH, sn = ST_X * (TH + 2);
+    const int i0 = (w.y - ty * ntx) * PT_X, j0 = ty * th;
     const PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
     const PG_GLOBAL real* __restrict__ vis = gmem(d.vis);
     const real h = (real)(0.5 * d.ctl->dt);
-    for (int e = threadIdx.x; e < ST_N; e += blockDim.x) {
+    for (int e = threadIdx.x; e < sn; e += PT_X * PT_TY) {
         const int lx = e % ST_X, ly = e / ST_X;
         const int i = i0 - 1 + lx, j = wrap(j0 - 1 + ly, ny);
         real r, u, v;
@@ -815,7 +846,18 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
         Tr[e] = r; Tu[e] = u; Tv[e] = v;
     }
     __syncthreads();
-    vis_from_tile(d, Tr, Tu, Tv, i0, j0, gmem(d.vis2));
+    if (d.out_pred) {   // the predicted state of the tile's own cells
+        PG_GLOBAL real* tp = gmem(d.out_pred);
+        const int lx = (int)(threadIdx.x % PT_X) + 1, i = i0 + lx - 1;
+        for (int ly = (int)(threadIdx.x / PT_X) + 1; ly <= th; ly += PT_TY) {
+            const int j = j0 + ly - 1;
+            if (i >= nx || j >= ny) break;
+            const size_t o = (size_t)j * nx + i;
+            const int e = ly * ST_X + lx;
+            tp[o] = Tr[e]; tp[N + o] = Tu[e]; tp[2 * N + o] = Tv[e];
+        }
+    }
+    vis_from_tile<real, TH>(d, Tr, Tu, Tv, i0, j0, gmem(d.vis2));
 }
 
 // Corrector (apply_source_terms_inplace with the viscous forces of the
@@ -827,9 +869,9 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
 // result goes to the other buffer of the pair, because neighbouring tiles
 // read this tile's cells as their halo input.  (The dust is corrected in
 // place: its halo results are not used.)
-template <typename real>
+template <typename real, int TH>
 __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, int do_kill) {
-    __shared__ real Sr[ST_N], Su[ST_N], Sv[ST_N];
+    __shared__ real Sr[ST_X * (TH + 2)], Su[ST_X * (TH + 2)], Sv[ST_X * (TH + 2)];
     const int2 w = ls.wl_pt[blockIdx.x];
     if (w.y < 0) return;
     const DiskArgs<real>& d = ls.disks[w.x];
@@ -837,14 +879,15 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
     const size_t N = (size_t)nx * ny;
     const int ntx = pt_ntx(nx);
     const int ty = w.y / ntx;
-    const int i0 = (w.y - ty * ntx) * PT_X, j0 = ty * PT_Y;
+    constexpr int th = TH, sn = ST_X * (TH + 2);
+    const int i0 = (w.y - ty * ntx) * PT_X, j0 = ty * th;
     const PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
     PG_GLOBAL real* __restrict__ gasN = gmem(d.gas[gsel(par) ^ 1]);
     PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
     const PG_GLOBAL real* __restrict__ vis2 = gmem(d.vis2);
     const real h = (real)(0.5 * d.ctl->dt);
     real rate = real(0);
-    for (int e = threadIdx.x; e < ST_N; e += blockDim.x) {
+    for (int e = threadIdx.x; e < sn; e += PT_X * PT_TY) {
         const int lx = e % ST_X, ly = e / ST_X;
         const int i = i0 - 1 + lx, jr = j0 - 1 + ly, j = wrap(jr, ny);
         real r, u, v;
@@ -873,7 +916,7 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
             }
             r = nr_; u = nu_; v = nvp;
             // the tile's own cells are written back, with their CFL rate
-            const bool own = lx >= 1 && lx <= PT_X && ly >= 1 && ly <= PT_Y && jr < ny;
+            const bool own = lx >= 1 && lx <= PT_X && ly >= 1 && ly <= th && jr < ny;
             if (own) {
                 gasN[o] = nr_;
                 gasN[N + o] = nu_;
@@ -889,11 +932,11 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
     }
     block_max_store(d.rate_part + w.y, rate);   // includes the barrier the tile needs
     __syncthreads();
-    vis_from_tile(d, Sr, Su, Sv, i0, j0, gmem(d.vis));
+    vis_from_tile<real, TH>(d, Sr, Su, Sv, i0, j0, gmem(d.vis));
 }
 
 // CFL rate of the current state (get_dt_lv1, timestep.cu:18-97).
-template <typename real>
+template <typename real, int TH>
 __global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
     const int2 w = ls.wl_pt[blockIdx.x];
     if (w.y < 0) return;   // padding entry of the work list (whole block)
@@ -903,17 +946,19 @@ __global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
     const int ntx = pt_ntx(nx);
     const int ty = w.y / ntx;
     const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
-    const int j = ty * PT_Y + (int)(threadIdx.x / PT_X);
     const PG_GLOBAL real* __restrict__ gas = gmem(d.gas[gsel(par)]);
     const PG_GLOBAL real* __restrict__ dust = gmem(d.dust[dsel(par)]);
     real rate = real(0);
-    if (i < nx && j < ny) {
+#pragma unroll
+    for (int k = 0; k < TH / PT_TY; ++k) {
+        const int j = ty * TH + (int)(threadIdx.x / PT_X) + PT_TY * k;
+        if (i >= nx || j >= ny) break;
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
         const real cs = TAB(T_C0, ip);
         real sx = fabs(gas[N + o]) + cs, sy = fabs(gas[2 * N + o]) + cs;
         if (d.has_dust) { sx = fmax(fabs(dust[N + o]), sx); sy = fmax(fabs(dust[2 * N + o]), sy); }
-        rate = fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip));
+        rate = fmax(rate, fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip)));
     }
     block_max_store(d.rate_part + w.y, rate);
 }
@@ -1055,6 +1100,18 @@ static hipError_t ring_any(const LaunchSet<real>& ls, int par, int flags, bool d
     }
 }
 
+// Launch a pointwise/stencil kernel with the tile height of the launch set
+// as a compile-time constant (4, 8 or 16 rows).
+#define PT_LAUNCH(K, ls, s, ...)                                                                       \
+    do {                                                                                               \
+        if ((ls).pt_rows == 4)                                                                         \
+            hipLaunchKernelGGL((K<real, 4>), dim3((ls).n_pt), dim3(256), 0, s, ls, __VA_ARGS__);       \
+        else if ((ls).pt_rows == 8)                                                                    \
+            hipLaunchKernelGGL((K<real, 8>), dim3((ls).n_pt), dim3(256), 0, s, ls, __VA_ARGS__);       \
+        else                                                                                           \
+            hipLaunchKernelGGL((K<real, 16>), dim3((ls).n_pt), dim3(256), 0, s, ls, __VA_ARGS__);      \
+    } while (0)
+
 template <typename real>
 struct Launch {
     static int phi_tile_radii(int B) { return B == 256 ? PhiTile<256>::TR : B == 1024 ? PhiTile<1024>::TR : PhiTile<512>::TR; }
@@ -1064,23 +1121,23 @@ struct Launch {
     static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, false, s); }
     static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, true, s); }
     static hipError_t viscosity(const LaunchSet<real>& ls, int par, int which, hipStream_t s) {
-        hipLaunchKernelGGL(k_viscosity<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par, which);
+        PT_LAUNCH(k_viscosity, ls, s, par, which);
         return hipGetLastError();
     }
     static hipError_t sources(const LaunchSet<real>& ls, int par, int mode, int kill, int rate, hipStream_t s) {
-        hipLaunchKernelGGL(k_sources<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par, mode, kill, rate);
+        PT_LAUNCH(k_sources, ls, s, par, mode, kill, rate);
         return hipGetLastError();
     }
     static hipError_t predvis(const LaunchSet<real>& ls, int par, hipStream_t s) {
-        hipLaunchKernelGGL(k_predvis<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par);
+        PT_LAUNCH(k_predvis, ls, s, par);
         return hipGetLastError();
     }
     static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s) {
-        hipLaunchKernelGGL(k_corrvis<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par, kill);
+        PT_LAUNCH(k_corrvis, ls, s, par, kill);
         return hipGetLastError();
     }
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s) {
-        hipLaunchKernelGGL(k_cfl<real>, dim3(ls.n_pt), dim3(256), 0, s, ls, par);
+        PT_LAUNCH(k_cfl, ls, s, par);
         return hipGetLastError();
     }
     static hipError_t rate_reduce(const LaunchSet<real>& ls, hipStream_t s) {

@@ -59,8 +59,8 @@ SIGNATURES = {
     "pg_batch_get_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "pg_batch_set_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "pg_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
-    "pg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
-    "pg_batch_get_schedule": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
+    "pg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pg_batch_get_schedule": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "pg_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "pg_batch_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pg_batch_profile_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int)]),
@@ -336,13 +336,14 @@ class Batch:
         self.t, self.dt = t, dt
         return dts[:, :nsteps]
 
-    def set_schedule(self, tile_x: int, ring_b: int, order: int = 0):
-        _check(load_library().pg_batch_set_schedule(self._h, tile_x, ring_b, order))
+    def set_schedule(self, tile_x: int, ring_b: int, order: int = 0, pt_rows: int = 0):
+        _check(load_library().pg_batch_set_schedule(self._h, tile_x, ring_b, order, pt_rows))
 
     def schedule(self):
-        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _check(load_library().pg_batch_get_schedule(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        """(x-sweep tile width, azimuthal-tile threads, tile order, pointwise tile rows)"""
+        v = [ctypes.c_int32() for _ in range(4)]
+        _check(load_library().pg_batch_get_schedule(self._h, *(ctypes.byref(x) for x in v)))
+        return tuple(x.value for x in v)
 
     def autotune(self, steps: int = 5) -> float:
         """Pick the fastest schedule on this batch; returns its ms per step."""

@@ -102,4 +102,4 @@ def test_batch_configuration_is_checked_before_any_device_work():
     assert L.pg_batch_destroy(None) == 0
     t = (ctypes.c_double * 1)()
     assert L.pg_batch_run(None, 1, t, t, None, 1) == -1
-    assert L.pg_batch_set_schedule(None, 256, 512, 0) == -1
+    assert L.pg_batch_set_schedule(None, 256, 512, 0, 0) == -1

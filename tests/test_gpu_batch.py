@@ -71,7 +71,7 @@ def test_batch_without_graph_equals_each_disk_alone():
     _assert_same_as_alone(_small_batch(n=4), 6, use_graph=False)
 
 
-@pytest.mark.parametrize("sched", [(128, 256, 0), (512, 512, 1), (256, 1024, 2), (384, 512, 0)])
+@pytest.mark.parametrize("sched", [(128, 256, 0, 16), (512, 512, 1, 4), (256, 1024, 2, 8), (384, 512, 0, 16)])
 def test_every_schedule_gives_the_same_bits(sched):
     ps = _small_batch()
     ref = Batch(ps)
@@ -91,8 +91,8 @@ def test_autotune_restores_state_and_keeps_bits():
     a.run(3)
     ms = a.autotune(steps=2)
     assert ms > 0.0
-    tile, ring, order = a.schedule()
-    print("autotuned schedule", tile, ring, order, f"{ms:.3f} ms/step", flush=True)
+    tile, ring, order, rows = a.schedule()
+    print("autotuned schedule", tile, ring, order, rows, f"{ms:.3f} ms/step", flush=True)
     a.run(7)
     b = Batch(ps)
     b.run(3)
