@@ -221,6 +221,13 @@ def main():
     achieved = bytes_launch / (kernels[dom]["avg_us"] * 1e-6) / 1e9
     # full-step roofline: 3 read+write passes of 6 fields (x, phi+OA, sources)
     step_bytes = 3 * 2 * 6 * word
+    # the boundary takes and returns host FP64 state: one upload + download
+    # of the whole disk (all fields, as pg_set_state / pg_get_state move it)
+    hg, hd = d.get_state()
+    h0 = time.perf_counter()
+    d.set_state(hg, hd)
+    d.get_state()
+    host_rt = time.perf_counter() - h0
     graph_value = None
     if a.graph:
         d.run(2, use_graph=True)
@@ -268,6 +275,7 @@ def main():
                           "roofline_cell_updates_per_s": PEAK_HBM / step_bytes,
                           "frac": round(value / (PEAK_HBM / step_bytes), 4)},
         "kernels": kernels,
+        "host_state_roundtrip_ms": round(1e3 * host_rt, 3),
         "cpu_baseline": cpu,
     }
     if graph_value is not None:
