@@ -55,15 +55,19 @@ template <typename real, int TX>
 __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
     const DiskArgs<real>& d = *A.d;
     constexpr int S = TX + 4 * NPAD;
+    constexpr int NT = (S + TX - 1) / TX;   // cells per thread
     __shared__ real r_[S], p_[S], u_[S], vp_[S], L_[S], fl_[S];
-    __shared__ real sl_[4][S], sr_[4][S];
-    __shared__ real Fr_[S], Fu_[S], FL_[S], pr_[S];
-    __shared__ real ar_[S], ap_[S], au_[S], aL_[S], avp_[S];
-    __shared__ real asl_[4][S], asr_[4][S];
-    __shared__ real cA_[S], cB_[S], cC_[S], cP_[S];
-    __shared__ real ld_[4][4][S];
-    __shared__ real lfl_[S];
-    __shared__ real lfp_[5][S];
+    // W: parabolas (8), fluxes (4) and update adjoints (4) of the forward and
+    // update phases, then the difference adjoints (16) and flattening
+    // adjoint (1) of the parabola reversal
+    __shared__ real W[17][S];
+    // per-interface adjoints sent to the left (CL) and right (CR) cell;
+    // CL[0..4] then hold the flattening adjoints of the pressures
+    __shared__ real CL[TI_N][S], CR[TI_N][S];
+    real (*sl_)[S] = W;
+    real (*sr_)[S] = W + 4;
+    real *Fr_ = W[8], *Fu_ = W[9], *FL_ = W[10], *pr_ = W[11];
+    real *cA_ = W[12], *cB_ = W[13], *cC_ = W[14], *cP_ = W[15];
     const int nx = d.nx;
     const size_t N = (size_t)nx * d.ny;
     const int t = threadIdx.x;
@@ -94,11 +98,11 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
     for (int s = t + 2; s < ns - 2; s += TX)
         fl_[s] = flatten5(p_[s - 2], p_[s - 1], p_[s], p_[s + 1], p_[s + 2], u_[s - 1], u_[s], u_[s + 1], real(1));
     __syncthreads();
+    const real* f4[4] = {r_, p_, u_, L_};
     for (int s = t + 2; s < ns - 2; s += TX) {
         const int ip = tip(s);
         const real w0 = TAB(T_WL0, ip), w1 = TAB(T_WL1, ip), w2 = TAB(T_WL2, ip);
         const real v0 = TAB(T_WR0, ip), v1 = TAB(T_WR1, ip), v2 = TAB(T_WR2, ip);
-        const real* f4[4] = {r_, p_, u_, L_};
         for (int f = 0; f < 4; ++f) {
             const real* a = f4[f];
             real l, r;
@@ -114,6 +118,7 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
         in[TI_USL] = sl_[2][s]; in[TI_USR] = sr_[2][s];
         in[TI_VSL] = sl_[3][s]; in[TI_V] = L_[s]; in[TI_VSR] = sr_[3][s];
     };
+    // forward fluxes of interfaces [3, ns-3] (interface k: between slots k-1 and k)
     for (int k = t + NPAD; k < ns - 2; k += TX) {
         const int ipl = tip(k - 1), ipr = tip(k);
         real inL[TI_N], inR[TI_N];
@@ -129,13 +134,13 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
         const real g = TAB(T_XA, ipr);
         Fr_[k] = Fr * g; Fu_[k] = Fu * g; FL_[k] = FL * g; pr_[k] = pres;
     }
-    // ---- reverse of the update (cells [3, ns-3)) ----
-    for (int s = t; s < S; s += TX) {
-        ap_[s] = real(0); aL_[s] = real(0);
-        for (int f = 0; f < 4; ++f) { asl_[f][s] = real(0); asr_[f][s] = real(0); }
-    }
     __syncthreads();
-    for (int s = t; s < S; s += TX) {
+    // ---- reverse of the update (cells [3, ns-3)); the cell adjoints stay
+    //      in registers (thread t owns slots t + q TX in every later phase)
+    real ar[NT], ap[NT], au[NT], aL[NT], avp[NT];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int s = t + q * TX;
         real lr = 0, lu = 0, lvp = 0, cA = 0, cB = 0, cC = 0, cP = 0;
         if (s >= NPAD && s < ns - NPAD && inside(s)) {
             const int g = x0 - 2 * NPAD + s;
@@ -166,17 +171,18 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
                 cA = lDr * f; cB = lDu * f; cC = lDL * f; cP = lDu * real(0.5) * (g0 + g1) * f;
             }
         }
-        ar_[s] = lr; au_[s] = lu; avp_[s] = lvp;
-        cA_[s] = cA; cB_[s] = cB; cC_[s] = cC; cP_[s] = cP;
+        ar[q] = lr; ap[q] = real(0); au[q] = lu; aL[q] = real(0); avp[q] = lvp;
+        if (s < S) { cA_[s] = cA; cB_[s] = cB; cC_[s] = cC; cP_[s] = cP; }
     }
     __syncthreads();
-    // ---- reverse of the interface fluxes: even interfaces, then odd ones,
-    //      so that the two cells each touches are never shared in a pass ----
-    for (int par = 0; par < 2; ++par) {
-        for (int k = NPAD + par + 2 * t; k < ns - 2; k += 2 * TX) {
-            const real lFs = cA_[k] - cA_[k - 1], lUs = cB_[k] - cB_[k - 1];
-            const real lLs = cC_[k] - cC_[k - 1], lpres = cP_[k] - cP_[k - 1];
-            if (lFs == real(0) && lUs == real(0) && lLs == real(0) && lpres == real(0)) continue;
+    // ---- reverse of every interface in one pass: each writes the adjoints
+    //      it sends to its two cells, which gather them below ----
+    for (int k = t + NPAD; k < ns - 2; k += TX) {
+        const real lFs = cA_[k] - cA_[k - 1], lUs = cB_[k] - cB_[k - 1];
+        const real lLs = cC_[k] - cC_[k - 1], lpres = cP_[k] - cP_[k - 1];
+        real lL[TI_N], lR[TI_N];
+        for (int m = 0; m < TI_N; ++m) { lL[m] = real(0); lR[m] = real(0); }
+        if (lFs != real(0) || lUs != real(0) || lLs != real(0) || lpres != real(0)) {
             const int ipl = tip(k - 1), ipr = tip(k);
             const real g = TAB(T_XA, ipr);
             real inL[TI_N], inR[TI_N];
@@ -190,45 +196,57 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
                         inR[9], inR[10], inR[11], S8[4], S8[5], S8[6], S8[7], cr);
             real lS8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lcl = 0, lcr = 0;
             hllc_iso_adj(cl, cr, S8, lFs * g, lUs * g, lLs * g, lpres, lS8, lcl, lcr);
-            real lL[TI_N], lR[TI_N];
-            for (int q = 0; q < TI_N; ++q) { lL[q] = real(0); lR[q] = real(0); }
             trace_left_adj(cL0, dt, idxL, inL, lS8[0], lS8[1], lS8[2], lS8[3], lcl, lL);
             trace_right_adj(cR0, dt, idxR, inR, lS8[4], lS8[5], lS8[6], lS8[7], lcr, lR);
-            const int m = k - 1;
-            ar_[m] += lL[TI_R]; ap_[m] += lL[TI_P]; au_[m] += lL[TI_U]; aL_[m] += lL[TI_V];
-            asl_[0][m] += lL[TI_RSL]; asr_[0][m] += lL[TI_RSR]; asl_[1][m] += lL[TI_PSL]; asr_[1][m] += lL[TI_PSR];
-            asl_[2][m] += lL[TI_USL]; asr_[2][m] += lL[TI_USR]; asl_[3][m] += lL[TI_VSL]; asr_[3][m] += lL[TI_VSR];
-            ar_[k] += lR[TI_R]; ap_[k] += lR[TI_P]; au_[k] += lR[TI_U]; aL_[k] += lR[TI_V];
-            asl_[0][k] += lR[TI_RSL]; asr_[0][k] += lR[TI_RSR]; asl_[1][k] += lR[TI_PSL]; asr_[1][k] += lR[TI_PSR];
-            asl_[2][k] += lR[TI_USL]; asr_[2][k] += lR[TI_USR]; asl_[3][k] += lR[TI_VSL]; asr_[3][k] += lR[TI_VSR];
         }
-        __syncthreads();
+        for (int m = 0; m < TI_N; ++m) { CL[m][k] = lL[m]; CR[m][k] = lR[m]; }
     }
-    // ---- reverse of the parabolas: adjoints of the neighbour differences ----
-    for (int s = t; s < S; s += TX) {
+    __syncthreads();
+    // ---- gather per cell (right cell of interface s, left cell of
+    //      interface s+1), then the reverse of its parabolas: adjoints of
+    //      the neighbour differences and of flattening (into W) ----
+    real (*ld_)[4][S] = reinterpret_cast<real (*)[4][S]>(&W[0][0]);
+    real* lfl_ = W[16];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int s = t + q * TX;
+        if (s >= S) break;
+        real gsum[TI_N];
+        for (int m = 0; m < TI_N; ++m) gsum[m] = real(0);
+        if (s >= NPAD && s <= ns - 3)
+            for (int m = 0; m < TI_N; ++m) gsum[m] += CR[m][s];
+        if (s + 1 >= NPAD && s + 1 <= ns - 3)
+            for (int m = 0; m < TI_N; ++m) gsum[m] += CL[m][s + 1];
+        ar[q] += gsum[TI_R]; ap[q] += gsum[TI_P]; au[q] += gsum[TI_U]; aL[q] += gsum[TI_V];
+        const real asl[4] = {gsum[TI_RSL], gsum[TI_PSL], gsum[TI_USL], gsum[TI_VSL]};
+        const real asr[4] = {gsum[TI_RSR], gsum[TI_PSR], gsum[TI_USR], gsum[TI_VSR]};
         real lflat = real(0);
         real ldf[4][4];
         for (int f = 0; f < 4; ++f)
-            for (int q = 0; q < 4; ++q) ldf[f][q] = real(0);
+            for (int m = 0; m < 4; ++m) ldf[f][m] = real(0);
         if (s >= 2 && s < ns - 2) {
             const int ip = tip(s);
             const real w0 = TAB(T_WL0, ip), w1 = TAB(T_WL1, ip), w2 = TAB(T_WL2, ip);
             const real v0 = TAB(T_WR0, ip), v1 = TAB(T_WR1, ip), v2 = TAB(T_WR2, ip);
-            const real* f4[4] = {r_, p_, u_, L_};
             for (int f = 0; f < 4; ++f) {
-                if (asl_[f][s] == real(0) && asr_[f][s] == real(0)) continue;
+                if (asl[f] == real(0) && asr[f] == real(0)) continue;
                 const real* a = f4[f];
                 slopes_adj(a[s - 2], a[s - 1], a[s], a[s + 1], a[s + 2], w0, w1, w2, v0, v1, v2, fl_[s],
-                           asl_[f][s], asr_[f][s], ldf[f], lflat);
+                           asl[f], asr[f], ldf[f], lflat);
             }
         }
+        // (W's parabolas and fluxes are dead: the interface pass is done)
         for (int f = 0; f < 4; ++f)
-            for (int q = 0; q < 4; ++q) ld_[f][q][s] = ldf[f][q];
+            for (int m = 0; m < 4; ++m) ld_[f][m][s] = ldf[f][m];
         lfl_[s] = lflat;
     }
     __syncthreads();
     // gather the differences into the cells; flattening adjoint per cell
-    for (int s = t; s < S; s += TX) {
+    real (*lfp_)[S] = CL;
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int s = t + q * TX;
+        if (s >= S) break;
         real add[4];
         for (int f = 0; f < 4; ++f) {
             real v = real(0);
@@ -239,17 +257,20 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
             if (s >= 2) v += ld_[f][3][s - 2];
             add[f] = v;
         }
-        ar_[s] += add[0]; ap_[s] += add[1]; au_[s] += add[2]; aL_[s] += add[3];
+        ar[q] += add[0]; ap[q] += add[1]; au[q] += add[2]; aL[q] += add[3];
         real lp5[5] = {0, 0, 0, 0, 0};
         if (s >= 2 && s < ns - 2 && lfl_[s] != real(0))
             flatten5_adj(p_[s - 2], p_[s - 1], p_[s], p_[s + 1], p_[s + 2], u_[s - 1], u_[s], u_[s + 1], lfl_[s], lp5);
-        for (int q = 0; q < 5; ++q) lfp_[q][s] = lp5[q];
+        for (int m = 0; m < 5; ++m) lfp_[m][s] = lp5[m];
     }
     __syncthreads();
     // ---- gather the flattening adjoints, chain p = cs2 rho and L = R (vbg + vp), store ----
     double lalpha = 0.0;
-    for (int s = t; s < ns; s += TX) {
-        real lp = ap_[s];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int s = t + q * TX;
+        if (s >= ns) break;
+        real lp = ap[q];
         if (s + 2 < S) lp += lfp_[0][s + 2];
         if (s + 1 < S) lp += lfp_[1][s + 1];
         lp += lfp_[2][s];
@@ -259,12 +280,12 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
         const int ip = tip(s);
         if (s >= 2 * NPAD && s < 2 * NPAD + nt) {
             const size_t o = row + g;
-            put(A.li, o, ar_[s] + TAB(T_CS2, ip) * lp, A.accumulate);
-            put(A.li, N + o, au_[s], A.accumulate);
-            put(A.li, 2 * N + o, avp_[s] + TAB(T_XC, ip) * aL_[s], A.accumulate);
+            put(A.li, o, ar[q] + TAB(T_CS2, ip) * lp, A.accumulate);
+            put(A.li, N + o, au[q], A.accumulate);
+            put(A.li, 2 * N + o, avp[q] + TAB(T_XC, ip) * aL[q], A.accumulate);
         } else if ((g < 0 && g >= -NPAD) || (g >= nx && g < nx + NPAD)) {
             // radial ghost: u = -1.5 nu(R)/R is proportional to alpha
-            lalpha += (double)au_[s] * (double)TAB(T_G_U, ip);
+            lalpha += (double)au[q] * (double)TAB(T_G_U, ip);
         }
     }
     if (A.alpha > 0.0) block_sum_add(A.lparam + A.nslot, lalpha / A.alpha);
@@ -279,15 +300,19 @@ template <typename real>
 __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k_adj_phi(AdjArgs<real> A) {
     constexpr int TR = AdjPhiTile<real>::TR, TY = AdjPhiTile<real>::TY, B = TR * TY;
     constexpr int NK = TY + 4 * NPAD, E = NK * TR;
+    constexpr int NT = (E + B - 1) / B;   // cells per thread
     __shared__ real c0[E], c1[E], c2[E], fl[E];
-    __shared__ real sl[3][E], sr[3][E];
-    __shared__ real F0[E], F1[E], F2[E], F3[E];
-    __shared__ real a0[E], a1[E], a2[E];
-    __shared__ real asl[3][E], asr[3][E];
-    __shared__ real cA[E], cB[E], cC[E];
-    __shared__ real ld[3][4][E];
-    __shared__ real lfl[E];
-    __shared__ real lfp[5][E];
+    // W: parabolas (6), fluxes (4) and update adjoints (3) of the forward and
+    // update phases, then the difference adjoints (12) and flattening adjoint
+    // (1) of the parabola reversal
+    __shared__ real W[13][E];
+    // per-interface adjoints sent to the left (CL) and right (CR) cell;
+    // CL[0..4] then hold the flattening adjoints of the pressures
+    __shared__ real CL[TC_N][E], CR[TC_N][E];
+    real (*sl)[E] = W;
+    real (*sr)[E] = W + 3;
+    real *F0 = W[6], *F1 = W[7], *F2 = W[8], *F3 = W[9];
+    real *cA = W[10], *cB = W[11], *cC = W[12];
     const DiskArgs<real>& d = *A.d;
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
@@ -304,6 +329,7 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
     const real cs2 = TAB(T_CS2, ip), cs0 = TAB(T_C0, ip), ic0 = TAB(T_IC0, ip);
     const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
     const int jbase = y0 - 2 * NPAD;
+    static_assert(B % TR == 0, "tile");
     for (int e = t; e < E; e += B) {
         real r = real(1), vp = real(0), u = real(0);
         if (live) {
@@ -330,6 +356,7 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
         in[TC_R] = c0[e]; in[TC_U] = c1[e]; in[TC_RSL] = sl[0][e]; in[TC_RSR] = sr[0][e];
         in[TC_USL] = sl[1][e]; in[TC_USR] = sr[1][e]; in[TC_VSL] = sl[2][e]; in[TC_V] = c2[e]; in[TC_VSR] = sr[2][e];
     };
+    // forward fluxes of interfaces [3, NK-3] (interface k: between rows k-1 and k)
     for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
         real inL[TC_N], inR[TC_N];
         state_in(e - TR, inL);
@@ -341,15 +368,16 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
         hllc_c(cs0, ic0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
         F0[e] = Fr; F1[e] = Fu; F2[e] = Fw; F3[e] = pres;
     }
-    for (int e = t; e < E; e += B) {
-        for (int f = 0; f < 3; ++f) { asl[f][e] = real(0); asr[f][e] = real(0); }
-    }
     __syncthreads();
-    // reverse of the update, cells [3, NK-3)
-    for (int e = t; e < E; e += B) {
+    // reverse of the update, rows [3, NK-3); the cell adjoints stay in
+    // registers (thread t owns cells t + q B in every later phase)
+    real a0[NT], a1[NT], a2[NT];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int e = t + q * B;
         const int k = e / TR;
         real l0 = 0, l1 = 0, l2 = 0, ca = 0, cb = 0, cc = 0;
-        if (live && k >= NPAD && k < NK - NPAD) {
+        if (e < E && live && k >= NPAD && k < NK - NPAD) {
             const size_t o = (size_t)wrap(jbase + k, ny) * nx + i;
             const real lrn = A.lo[o], lun = A.lo[N + o], lvn = A.lo[2 * N + o];
             const real Dr = (F0[e] - F0[e + TR]) * fup;
@@ -372,18 +400,18 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
                 ca = lrt * fup; cb = lvn * irn * fup; cc = lun * irn * fup;
             }
         }
-        a0[e] = l0; a1[e] = l1; a2[e] = l2;
-        cA[e] = ca; cB[e] = cb; cC[e] = cc;
+        a0[q] = l0; a1[q] = l1; a2[q] = l2;
+        if (e < E) { cA[e] = ca; cB[e] = cb; cC[e] = cc; }
     }
     __syncthreads();
-    // reverse of the interfaces, even rows then odd rows
-    for (int par = 0; par < 2; ++par) {
-        for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
-            const int k = e / TR;
-            if ((k & 1) != par) continue;
-            const real lFr = cA[e] - cA[e - TR], lFu = cB[e] - cB[e - TR], lFw = cC[e] - cC[e - TR];
-            const real lpres = lFu;   // the pressure difference enters Du like the momentum flux
-            if (lFr == real(0) && lFu == real(0) && lFw == real(0)) continue;
+    // reverse of every interface in one pass: each writes the adjoints it
+    // sends to its two cells, which gather them below
+    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
+        const real lFr = cA[e] - cA[e - TR], lFu = cB[e] - cB[e - TR], lFw = cC[e] - cC[e - TR];
+        const real lpres = lFu;   // the pressure difference enters Du like the momentum flux
+        real lL[TC_N], lR[TC_N];
+        for (int m = 0; m < TC_N; ++m) { lL[m] = real(0); lR[m] = real(0); }
+        if (lFr != real(0) || lFu != real(0) || lFw != real(0)) {
             real inL[TC_N], inR[TC_N];
             state_in(e - TR, inL);
             state_in(e, inR);
@@ -394,41 +422,53 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
                           Sc[3], Sc[4], Sc[5]);
             real lSc[6] = {0, 0, 0, 0, 0, 0};
             hllc_c_adj(cs0, ic0, cs2, Sc, lFr, lFu, lFw, lpres, lSc);
-            real lL[TC_N], lR[TC_N];
-            for (int q = 0; q < TC_N; ++q) { lL[q] = real(0); lR[q] = real(0); }
             trace_left_c_adj(cs0, ic0, dts, idy, inL, lSc[0], lSc[1], lSc[2], lL);
             trace_right_c_adj(cs0, ic0, dts, idy, inR, lSc[3], lSc[4], lSc[5], lR);
-            const int m = e - TR;
-            a0[m] += lL[TC_R]; a1[m] += lL[TC_U]; a2[m] += lL[TC_V];
-            asl[0][m] += lL[TC_RSL]; asr[0][m] += lL[TC_RSR]; asl[1][m] += lL[TC_USL]; asr[1][m] += lL[TC_USR];
-            asl[2][m] += lL[TC_VSL]; asr[2][m] += lL[TC_VSR];
-            a0[e] += lR[TC_R]; a1[e] += lR[TC_U]; a2[e] += lR[TC_V];
-            asl[0][e] += lR[TC_RSL]; asr[0][e] += lR[TC_RSR]; asl[1][e] += lR[TC_USL]; asr[1][e] += lR[TC_USR];
-            asl[2][e] += lR[TC_VSL]; asr[2][e] += lR[TC_VSR];
         }
-        __syncthreads();
+        for (int m = 0; m < TC_N; ++m) { CL[m][e] = lL[m]; CR[m][e] = lR[m]; }
     }
-    // reverse of the parabolas
-    for (int e = t; e < E; e += B) {
+    __syncthreads();
+    // gather per cell (right cell of interface k, left cell of interface
+    // k+1), then the reverse of its parabolas: adjoints of the neighbour
+    // differences and of the flattening coefficient (into W)
+    real (*ld)[4][E] = reinterpret_cast<real (*)[4][E]>(&W[0][0]);
+    real* lfl = W[12];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int e = t + q * B;
+        if (e >= E) break;
         const int k = e / TR;
+        real g[TC_N];
+        for (int m = 0; m < TC_N; ++m) g[m] = real(0);
+        if (k >= NPAD && k <= NK - 3)
+            for (int m = 0; m < TC_N; ++m) g[m] += CR[m][e];
+        if (k + 1 >= NPAD && k + 1 <= NK - 3)
+            for (int m = 0; m < TC_N; ++m) g[m] += CL[m][e + TR];
+        a0[q] += g[TC_R]; a1[q] += g[TC_U]; a2[q] += g[TC_V];
+        const real asl[3] = {g[TC_RSL], g[TC_USL], g[TC_VSL]}, asr[3] = {g[TC_RSR], g[TC_USR], g[TC_VSR]};
         real lflat = real(0);
         real ldf[3][4];
         for (int f = 0; f < 3; ++f)
-            for (int q = 0; q < 4; ++q) ldf[f][q] = real(0);
+            for (int m = 0; m < 4; ++m) ldf[f][m] = real(0);
         if (k >= 2 && k < NK - 2) {
             for (int f = 0; f < 3; ++f) {
-                if (asl[f][e] == real(0) && asr[f][e] == real(0)) continue;
+                if (asl[f] == real(0) && asr[f] == real(0)) continue;
                 const real* a = cf[f];
                 slopes_adj(a[e - 2 * TR], a[e - TR], a[e], a[e + TR], a[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4],
-                           wy[5], fl[e], asl[f][e], asr[f][e], ldf[f], lflat);
+                           wy[5], fl[e], asl[f], asr[f], ldf[f], lflat);
             }
         }
+        // (W's parabolas and fluxes are dead: the interface pass is done)
         for (int f = 0; f < 3; ++f)
-            for (int q = 0; q < 4; ++q) ld[f][q][e] = ldf[f][q];
+            for (int m = 0; m < 4; ++m) ld[f][m][e] = ldf[f][m];
         lfl[e] = lflat;
     }
     __syncthreads();
-    for (int e = t; e < E; e += B) {
+    real (*lfp)[E] = CL;
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int e = t + q * B;
+        if (e >= E) break;
         const int k = e / TR;
         real add[3];
         for (int f = 0; f < 3; ++f) {
@@ -440,24 +480,26 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
             if (k >= 2) v += ld[f][3][e - 2 * TR];
             add[f] = v;
         }
-        a0[e] += add[0]; a1[e] += add[1]; a2[e] += add[2];
+        a0[q] += add[0]; a1[q] += add[1]; a2[q] += add[2];
         real lp5[5] = {0, 0, 0, 0, 0};
         if (k >= 2 && k < NK - 2 && lfl[e] != real(0))
             flatten5_adj(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR], c1[e - TR], c1[e], c1[e + TR],
                          lfl[e], lp5);
-        for (int q = 0; q < 5; ++q) lfp[q][e] = lp5[q];
+        for (int m = 0; m < 5; ++m) lfp[m][e] = lp5[m];
     }
     __syncthreads();
     const int rows = min(TY, ny - y0);
-    for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int e = t + q * B;
         const int k = e / TR;
         const int y = k - 2 * NPAD;
-        if (!live || y >= rows) continue;
-        real lr = a0[e] + lfp[0][e + 2 * TR] + lfp[1][e + TR] + lfp[2][e] + lfp[3][e - TR] + lfp[4][e - 2 * TR];
+        if (e >= E || !live || y < 0 || y >= rows) continue;
+        real lr = a0[q] + lfp[0][e + 2 * TR] + lfp[1][e + TR] + lfp[2][e] + lfp[3][e - TR] + lfp[4][e - 2 * TR];
         const size_t o = (size_t)(y0 + y) * nx + i;
         put(A.li, o, lr, A.accumulate);
-        put(A.li, N + o, a2[e], A.accumulate);
-        put(A.li, 2 * N + o, a1[e], A.accumulate);
+        put(A.li, N + o, a2[q], A.accumulate);
+        put(A.li, 2 * N + o, a1[q], A.accumulate);
     }
 }
 
@@ -833,7 +875,7 @@ __global__ __launch_bounds__(256) void k_adj_src(AdjArgs<real> A) {
 // ------------------------------------------------------------- launchers
 template <typename real>
 struct AdjLaunch {
-    static constexpr int TXA = 128;
+    static constexpr int TXA = 256;
     static hipError_t x(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
         hipLaunchKernelGGL((k_adj_x<real, TXA>), dim3((nx + TXA - 1) / TXA, ny), dim3(TXA), 0, s, A);
         return hipGetLastError();
