@@ -664,166 +664,273 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
 }
 
 // ------------------------------------------------------ viscosity adjoint
-// Forward: k_viscosity, a 9-point stencil of (rho nu, u, vp).  The partial
-// kernel writes, per output cell, the adjoints it sends to each input of its
-// stencil (slot n: u of neighbour n, 9+n: vp, 18..22: rho nu of neighbours
-// 0,1,2,4,7; neighbour n = 3b + a at radius i-1+a, row j, j-1, j+1 for
-// b = 0, 1, 2); the gather kernel sums them per input cell.
+// Forward: k_viscosity / vis_from_tile, a 9-point stencil of (rho nu, u, vp).
+// Per output cell, the adjoints it sends to each input of its stencil:
+// lu[n], lv[n] for the u and vp of neighbour n, ln[n] for its rho nu
+// (nonzero for n = 0, 1, 2, 4, 7); neighbour n = 3b + a at radius i-1+a,
+// row j, j-1, j+1 for b = 0, 1, 2.  Returns X fx + Y fy (the forces are
+// linear in nu, hence in alpha).
 template <typename real>
-__global__ __launch_bounds__(256) void k_adj_vis_part(AdjArgs<real> A) {
-    const DiskArgs<real>& d = *A.d;
-    const int nx = d.nx, ny = d.ny;
-    const size_t N = (size_t)nx * ny;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int j = blockIdx.y;
-    double lal = 0.0;
-    if (i < nx) {
-        const int ip = i + NPAD;
-        real nu[3];
-        for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
-        real nr[9], u[9], v[9];
-        const int rows[3] = {j, wrap(j - 1, ny), wrap(j + 1, ny)};
-        for (int b = 0; b < 3; ++b)
-            for (int a = 0; a < 3; ++a) {
-                const int n = 3 * b + a, ii = i - 1 + a;
-                real rr, uu, vv;
-                if (ii < 0 || ii >= nx) {
-                    rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
-                } else {
-                    const size_t o = (size_t)rows[b] * nx + ii;
-                    rr = A.s[o]; uu = A.s[N + o]; vv = A.s[2 * N + o];
-                }
-                nr[n] = rr * nu[a]; u[n] = uu; v[n] = vv;
-            }
-        const real i20 = TAB(T_I20, ip), i21 = TAB(T_I21, ip), i10 = TAB(T_I10, ip);
-        const real ir = TAB(T_IR, ip), idy = d.idy, hdy = real(0.5) * idy;
-        const real dudx = (u[2] - u[0]) * i20;
-        const real d2udx2 = ((u[2] - u[1]) * i21 - (u[1] - u[0]) * i10) * (real(2) * i20);
-        const real dvdx = TAB(T_DVBG, ip) + (v[2] - v[0]) * i20;
-        const real d2vdx2 = TAB(T_D2VBG, ip) + ((v[2] - v[1]) * i21 - (v[1] - v[0]) * i10) * (real(2) * i20);
-        const real dndx = (nr[2] - nr[0]) * i20;
-        const real ky = hdy * ir, kyy = (idy * idy) * ir * ir, kxy = i20 * hdy * ir;
-        const real dvdy = (v[7] - v[4]) * ky;
-        const real d2vdy2 = (v[7] - real(2) * v[1] + v[4]) * kyy;
-        const real d2vdxy = (v[8] - v[6] - v[5] + v[3]) * kxy;
-        const real dudy = (u[7] - u[4]) * ky;
-        const real d2udy2 = (u[7] - real(2) * u[1] + u[4]) * kyy;
-        const real d2udxy = (u[8] - u[6] - u[5] + u[3]) * kxy;
-        const real dndy = (nr[7] - nr[4]) * ky;
-        const real v1 = TAB(T_VBG, ip) + v[1];
-        const real n1 = nr[1];
-        const size_t o = (size_t)j * nx + i;
-        const real X = A.lvis[o], Y = A.lvis[N + o];
-        // forward values (for the alpha adjoint: the forces are linear in nu)
-        {
-            real sxx = real(2) * n1 * dudx;
-            real fx = real(2) * dndx * dudx + real(2) * n1 * d2udx2 + sxx * ir;
-            real syy = real(2) * n1 * (dvdy + u[1] * ir);
-            real sxy = n1 * (dudy + dvdx - v1 * ir);
-            real fy = real(2) * dndy * dvdy + real(2) * n1 * d2vdy2 + real(2) * n1 * dudy * ir + real(2) * dndy * u[1] * ir;
-            fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 * ir - n1 * dvdy * ir - syy * ir;
-            fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 * ir - n1 * dvdx * ir + n1 * v1 * ir * ir +
-                  real(2) * sxy * ir;
-            lal = (double)X * (double)fx + (double)Y * (double)fy;
-        }
-        // adjoints of the derivative terms
-        const real l_dndx = X * (real(2) * dudx) + Y * ((dudy + dvdx) - v1 * ir);
-        const real l_dudx = X * (real(2) * dndx + real(2) * n1 * ir);
-        const real l_n1 = X * (real(2) * d2udx2 + real(2) * dudx * ir + (d2udy2 + d2vdxy) - dvdy * ir -
-                               real(2) * (dvdy + u[1] * ir) * ir) +
-                          Y * (real(2) * d2vdy2 + real(2) * dudy * ir + (d2udxy + d2vdx2) - dvdx * ir + v1 * ir * ir +
-                               real(2) * (dudy + dvdx - v1 * ir) * ir);
-        const real l_d2udx2 = X * real(2) * n1;
-        const real l_dndy = X * ((dudy + dvdx) - v1 * ir) + Y * (real(2) * dvdy + real(2) * u[1] * ir);
-        const real l_dudy = X * dndy + Y * (real(4) * n1 * ir + dndx);
-        const real l_dvdx = X * dndy + Y * (dndx + n1 * ir);
-        const real l_d2udy2 = X * n1, l_d2vdxy = X * n1;
-        const real l_v1 = X * (-dndy * ir) + Y * (-dndx * ir - n1 * ir * ir);
-        const real l_dvdy = X * (real(-3) * n1 * ir) + Y * (real(2) * dndy);
-        const real l_u1 = X * (real(-2) * n1 * ir * ir) + Y * (real(2) * dndy * ir);
-        const real l_d2vdy2 = Y * real(2) * n1, l_d2udxy = Y * n1, l_d2vdx2 = Y * n1;
-        real lu[9], lv[9], ln[9];
-        for (int n = 0; n < 9; ++n) { lu[n] = real(0); lv[n] = real(0); ln[n] = real(0); }
-        lu[2] += l_dudx * i20; lu[0] -= l_dudx * i20;
-        {
-            const real c = real(2) * i20 * l_d2udx2;
-            lu[2] += c * i21; lu[1] -= c * (i21 + i10); lu[0] += c * i10;
-        }
-        lv[2] += l_dvdx * i20; lv[0] -= l_dvdx * i20;
-        {
-            const real c = real(2) * i20 * l_d2vdx2;
-            lv[2] += c * i21; lv[1] -= c * (i21 + i10); lv[0] += c * i10;
-        }
-        ln[2] += l_dndx * i20; ln[0] -= l_dndx * i20;
-        lv[7] += l_dvdy * ky; lv[4] -= l_dvdy * ky;
-        lv[7] += l_d2vdy2 * kyy; lv[1] -= real(2) * l_d2vdy2 * kyy; lv[4] += l_d2vdy2 * kyy;
-        lv[8] += l_d2vdxy * kxy; lv[6] -= l_d2vdxy * kxy; lv[5] -= l_d2vdxy * kxy; lv[3] += l_d2vdxy * kxy;
-        lu[7] += l_dudy * ky; lu[4] -= l_dudy * ky;
-        lu[7] += l_d2udy2 * kyy; lu[1] -= real(2) * l_d2udy2 * kyy; lu[4] += l_d2udy2 * kyy;
-        lu[8] += l_d2udxy * kxy; lu[6] -= l_d2udxy * kxy; lu[5] -= l_d2udxy * kxy; lu[3] += l_d2udxy * kxy;
-        ln[7] += l_dndy * ky; ln[4] -= l_dndy * ky;
-        lv[1] += l_v1;
-        ln[1] += l_n1;
-        lu[1] += l_u1;
-        for (int n = 0; n < 9; ++n) {
-            A.part[(size_t)n * N + o] = lu[n];
-            A.part[(size_t)(9 + n) * N + o] = lv[n];
-        }
-        const int rn[5] = {0, 1, 2, 4, 7};
-        for (int k = 0; k < 5; ++k) A.part[(size_t)(18 + k) * N + o] = ln[rn[k]] * nu[rn[k] % 3];
+__device__ __forceinline__ double vis_adj_cell(const DiskArgs<real>& d, int ip, const real* nr, const real* u,
+                                               const real* v, real X, real Y, real* lu, real* lv, real* ln) {
+    const real i20 = TAB(T_I20, ip), i21 = TAB(T_I21, ip), i10 = TAB(T_I10, ip);
+    const real ir = TAB(T_IR, ip), idy = d.idy, hdy = real(0.5) * idy;
+    const real dudx = (u[2] - u[0]) * i20;
+    const real d2udx2 = ((u[2] - u[1]) * i21 - (u[1] - u[0]) * i10) * (real(2) * i20);
+    const real dvdx = TAB(T_DVBG, ip) + (v[2] - v[0]) * i20;
+    const real d2vdx2 = TAB(T_D2VBG, ip) + ((v[2] - v[1]) * i21 - (v[1] - v[0]) * i10) * (real(2) * i20);
+    const real dndx = (nr[2] - nr[0]) * i20;
+    const real ky = hdy * ir, kyy = (idy * idy) * ir * ir, kxy = i20 * hdy * ir;
+    const real dvdy = (v[7] - v[4]) * ky;
+    const real d2vdy2 = (v[7] - real(2) * v[1] + v[4]) * kyy;
+    const real d2vdxy = (v[8] - v[6] - v[5] + v[3]) * kxy;
+    const real dudy = (u[7] - u[4]) * ky;
+    const real d2udy2 = (u[7] - real(2) * u[1] + u[4]) * kyy;
+    const real d2udxy = (u[8] - u[6] - u[5] + u[3]) * kxy;
+    const real dndy = (nr[7] - nr[4]) * ky;
+    const real v1 = TAB(T_VBG, ip) + v[1];
+    const real n1 = nr[1];
+    double lal;
+    {   // forward values (for the alpha adjoint)
+        real sxx = real(2) * n1 * dudx;
+        real fx = real(2) * dndx * dudx + real(2) * n1 * d2udx2 + sxx * ir;
+        real syy = real(2) * n1 * (dvdy + u[1] * ir);
+        real sxy = n1 * (dudy + dvdx - v1 * ir);
+        real fy = real(2) * dndy * dvdy + real(2) * n1 * d2vdy2 + real(2) * n1 * dudy * ir + real(2) * dndy * u[1] * ir;
+        fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 * ir - n1 * dvdy * ir - syy * ir;
+        fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 * ir - n1 * dvdx * ir + n1 * v1 * ir * ir +
+              real(2) * sxy * ir;
+        lal = (double)X * (double)fx + (double)Y * (double)fy;
     }
-    if (A.alpha > 0.0) block_sum_add(A.lparam + A.nslot, lal / A.alpha);
+    // adjoints of the derivative terms
+    const real l_dndx = X * (real(2) * dudx) + Y * ((dudy + dvdx) - v1 * ir);
+    const real l_dudx = X * (real(2) * dndx + real(2) * n1 * ir);
+    const real l_n1 = X * (real(2) * d2udx2 + real(2) * dudx * ir + (d2udy2 + d2vdxy) - dvdy * ir -
+                           real(2) * (dvdy + u[1] * ir) * ir) +
+                      Y * (real(2) * d2vdy2 + real(2) * dudy * ir + (d2udxy + d2vdx2) - dvdx * ir + v1 * ir * ir +
+                           real(2) * (dudy + dvdx - v1 * ir) * ir);
+    const real l_d2udx2 = X * real(2) * n1;
+    const real l_dndy = X * ((dudy + dvdx) - v1 * ir) + Y * (real(2) * dvdy + real(2) * u[1] * ir);
+    const real l_dudy = X * dndy + Y * (real(4) * n1 * ir + dndx);
+    const real l_dvdx = X * dndy + Y * (dndx + n1 * ir);
+    const real l_d2udy2 = X * n1, l_d2vdxy = X * n1;
+    const real l_v1 = X * (-dndy * ir) + Y * (-dndx * ir - n1 * ir * ir);
+    const real l_dvdy = X * (real(-3) * n1 * ir) + Y * (real(2) * dndy);
+    const real l_u1 = X * (real(-2) * n1 * ir * ir) + Y * (real(2) * dndy * ir);
+    const real l_d2vdy2 = Y * real(2) * n1, l_d2udxy = Y * n1, l_d2vdx2 = Y * n1;
+    for (int n = 0; n < 9; ++n) { lu[n] = real(0); lv[n] = real(0); ln[n] = real(0); }
+    lu[2] += l_dudx * i20; lu[0] -= l_dudx * i20;
+    {
+        const real c = real(2) * i20 * l_d2udx2;
+        lu[2] += c * i21; lu[1] -= c * (i21 + i10); lu[0] += c * i10;
+    }
+    lv[2] += l_dvdx * i20; lv[0] -= l_dvdx * i20;
+    {
+        const real c = real(2) * i20 * l_d2vdx2;
+        lv[2] += c * i21; lv[1] -= c * (i21 + i10); lv[0] += c * i10;
+    }
+    ln[2] += l_dndx * i20; ln[0] -= l_dndx * i20;
+    lv[7] += l_dvdy * ky; lv[4] -= l_dvdy * ky;
+    lv[7] += l_d2vdy2 * kyy; lv[1] -= real(2) * l_d2vdy2 * kyy; lv[4] += l_d2vdy2 * kyy;
+    lv[8] += l_d2vdxy * kxy; lv[6] -= l_d2vdxy * kxy; lv[5] -= l_d2vdxy * kxy; lv[3] += l_d2vdxy * kxy;
+    lu[7] += l_dudy * ky; lu[4] -= l_dudy * ky;
+    lu[7] += l_d2udy2 * kyy; lu[1] -= real(2) * l_d2udy2 * kyy; lu[4] += l_d2udy2 * kyy;
+    lu[8] += l_d2udxy * kxy; lu[6] -= l_d2udxy * kxy; lu[5] -= l_d2udxy * kxy; lu[3] += l_d2udxy * kxy;
+    ln[7] += l_dndy * ky; ln[4] -= l_dndy * ky;
+    lv[1] += l_v1;
+    ln[1] += l_n1;
+    lu[1] += l_u1;
+    return lal;
 }
 
-// Sum of the partials each input cell received; columns -1 and nx are the
-// radial ghosts, whose u = -1.5 nu(R)/R makes them alpha-dependent.
-template <typename real>
-__global__ __launch_bounds__(256) void k_adj_vis_gather(AdjArgs<real> A) {
-    const DiskArgs<real>& d = *A.d;
-    const int nx = d.nx, ny = d.ny;
-    const size_t N = (size_t)nx * ny;
-    const int i = blockIdx.x * 256 + threadIdx.x - 1;   // -1 .. nx
-    const int j = blockIdx.y;
-    double lal = 0.0;
-    if (i <= nx) {
-        real lr = 0, lu = 0, lv = 0;
-        const int db[3] = {0, -1, 1};
-        for (int b = 0; b < 3; ++b)
-            for (int a = 0; a < 3; ++a) {
-                const int n = 3 * b + a;
-                const int io = i - (a - 1);
-                if (io < 0 || io >= nx) continue;
-                const int jo = wrap(j - db[b], ny);
-                const size_t o = (size_t)jo * nx + io;
-                lu += A.part[(size_t)n * N + o];
-                lv += A.part[(size_t)(9 + n) * N + o];
-                int slot = -1;
-                if (n == 0) slot = 18;
-                else if (n == 1) slot = 19;
-                else if (n == 2) slot = 20;
-                else if (n == 4) slot = 21;
-                else if (n == 7) slot = 22;
-                if (slot >= 0) lr += A.part[(size_t)slot * N + o];
-            }
-        if (i >= 0 && i < nx) {
-            const size_t o = (size_t)j * nx + i;
-            put(A.li, o, lr, A.accumulate & 1);
-            put(A.li, N + o, lu, A.accumulate & 1);
-            put(A.li, 2 * N + o, lv, A.accumulate & 1);
-        } else {
-            lal = (double)lu * (double)TAB(T_G_U, i + NPAD);
-        }
-    }
-    if (A.alpha > 0.0) block_sum_add(A.lparam + A.nslot, lal / A.alpha);
-}
-
-// ---------------------------------------------------- source-term adjoint
-// Forward: k_sources for gas without dust or wave killing, h = dt/2:
+// Source-term adjoint of one cell.  Forward (k_sources / cell_sources, gas
+// without dust or wave killing, h = dt/2):
 //   mode 0:  u' = u + (ebal + vp (2 vbg + vp)/R + px + Vr/rho) h,
 //            vp' = vp + (Vphi/rho + py) h
 //   mode 1/2: the same with vp in the centrifugal term replaced by the
 //            predicted W = vp + (Vphi/rho + py) h.
-// accumulate: bit 0 -> li +=, bit 1 -> lvis +=.
+// In: the state (rho, vp), the forces V, the output adjoints ln*; out: the
+// adjoints of rho, u, vp, V, and lq += dL/d(planet mass) * q.
+template <typename real>
+__device__ __forceinline__ void src_adj_cell(const DiskArgs<real>& d, int mode, int ip, int j, real h, real rho,
+                                             real cvp, real Vr, real Vp, real lnr, real lnu, real lnv, real& lrho,
+                                             real& lu, real& lcvp, real& lVr, real& lVp, double& lq) {
+    const PlanetK& pk = mode == 0 ? d.ctl->pk0 : d.ctl->pk1;
+    const real iR = TAB(T_IR, ip), vbg = TAB(T_VBG, ip);
+    real px = real(0), py = real(0);
+    if (d.n_planet > 0) planet_acc(pk, d.phitab + 4 * j, TAB(T_XC, ip), px, py);
+    const real irho = FM<real>::rcp(rho);
+    if (!d.visc) { Vr = real(0); Vp = real(0); }
+    const real vy = Vp * irho;
+    const real W = mode == 0 ? cvp : cvp + (vy + py) * h;
+    const real lW = lnu * h * (real(2) * vbg + real(2) * W) * iR;
+    const real lpx = lnu * h, lvx = lnu * h;
+    real lvy = lnv * h, lpy = lnv * h;
+    if (mode != 0) { lvy += lW * h; lpy += lW * h; }
+    lrho = lnr;
+    lu = lnu;
+    lcvp = lnv + lW;
+    lVr = real(0);
+    lVp = real(0);
+    if (d.visc) {
+        lVr = lvx * irho;
+        lVp = lvy * irho;
+        lrho -= (lvx * Vr + lvy * Vp) * irho * irho;
+    }
+    lq += (double)lpx * (double)px + (double)lpy * (double)py;
+}
+
+// Viscosity adjoint on tiles of AV_X radii x TH rows, fused with the
+// pointwise passes around it.  The outputs whose stencil meets the tile
+// (the tile and a one-cell halo) compute their partial adjoints into LDS
+// from the forward state staged with a two-cell halo; each tile cell then
+// gathers what its 3x3 neighbours sent.
+//   MODE 0: V = vis(S): lvis -> li (module entry point).
+//   MODE 1: end of the step: corrector S4 = C(S3, V1) (lo = adjoint of S4,
+//           s = S3, vis = V1), V1 = vis(T) (s2 = T), predictor
+//           T = P(S3, V0) (vis2 = V0): li = adjoint of S3, lvis = of V0.
+//   MODE 2: start of the step: kick S1 = K(S0, V0) (lo = adjoint of S1,
+//           s = S0, vis = V0), V0 = vis(S0) with lvis holding what the
+//           predictor sent to V0: li = adjoint of S0.
+constexpr int AV_X = 64;
+
+template <typename real, int MODE, int TH>
+__global__ __launch_bounds__(AV_X * TH) void k_adj_vis(AdjArgs<real> A) {
+    constexpr int B = AV_X * TH;
+    constexpr int H1X = AV_X + 2, H1 = H1X * (TH + 2);   // outputs: tile + 1
+    constexpr int H2X = AV_X + 4, H2 = H2X * (TH + 4);   // their state: tile + 2
+    __shared__ real sr[H2], su[H2], sv[H2];
+    __shared__ real P[23][H1];
+    const DiskArgs<real>& d = *A.d;
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * AV_X, j0 = blockIdx.y * TH;
+    const real h = (real)(0.5 * d.ctl->dt);
+    const real* vs = MODE == 1 ? A.s2 : A.s;   // the state the viscosity read
+    for (int e = t; e < H2; e += B) {
+        const int lx = e % H2X, ly = e / H2X;
+        const int i = i0 - 2 + lx, j = wrap(j0 - 2 + ly, ny);
+        real r, u, v;
+        if (i < 0 || i >= nx) {   // radial ghosts (the second halo column feeds no output)
+            const int ip = min(max(i, -NPAD), nx + NPAD - 1) + NPAD;
+            r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); v = TAB(T_G_V, ip);
+        } else {
+            const size_t o = (size_t)j * nx + i;
+            r = vs[o]; u = vs[N + o]; v = vs[2 * N + o];
+        }
+        sr[e] = r; su[e] = u; sv[e] = v;
+    }
+    __syncthreads();
+    double lal = 0.0, lq = 0.0;
+    for (int e = t; e < H1; e += B) {
+        const int lx = e % H1X, ly = e / H1X;
+        const int i = i0 - 1 + lx, jr = j0 - 1 + ly, j = wrap(jr, ny);
+        real lu[9], lv[9], ln[9];
+        for (int n = 0; n < 9; ++n) { lu[n] = real(0); lv[n] = real(0); ln[n] = real(0); }
+        real nu[3] = {real(0), real(0), real(0)};
+        if (i >= 0 && i < nx) {
+            const int ip = i + NPAD;
+            const size_t o = (size_t)j * nx + i;
+            real X, Y;
+            double lq_unused = 0.0;
+            if (MODE == 0) {
+                X = A.lvis[o]; Y = A.lvis[N + o];
+            } else {
+                // adjoint of the forces from the source pass that read them
+                real a0, a1, a2;
+                src_adj_cell(d, MODE == 1 ? 2 : 0, ip, j, h, A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o],
+                             A.lo[N + o], A.lo[2 * N + o], a0, a1, a2, X, Y, lq_unused);
+                if (MODE == 2) { X += A.lvis[o]; Y += A.lvis[N + o]; }
+            }
+            for (int n = 0; n < 3; ++n) nu[n] = TAB(T_NU, ip - 1 + n);
+            real nr[9], u[9], v[9];
+            const int rows[3] = {ly + 1, ly, ly + 2};   // H2 rows of j, j-1, j+1
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int n = 3 * b + a, s = rows[b] * H2X + lx + a;
+                    nr[n] = sr[s] * nu[a]; u[n] = su[s]; v[n] = sv[s];
+                }
+            const double l = vis_adj_cell(d, ip, nr, u, v, X, Y, lu, lv, ln);
+            if (lx >= 1 && lx <= AV_X && ly >= 1 && ly <= TH && jr < ny) lal += l;   // each output once
+        }
+        for (int n = 0; n < 9; ++n) { P[n][e] = lu[n]; P[9 + n][e] = lv[n]; }
+        P[18][e] = ln[0] * nu[0]; P[19][e] = ln[1] * nu[1]; P[20][e] = ln[2] * nu[2];
+        P[21][e] = ln[4] * nu[1]; P[22][e] = ln[7] * nu[1];
+    }
+    __syncthreads();
+    // what the outputs sent to the cell at H1 slot (cx, cy): output
+    // (cx + 1 - a, cy - db) holds it as its neighbour n = 3b + a
+    // (slots outside the H1 columns hold no output: skipped for the ghost
+    // columns, whose outer neighbours are not radii of the disk)
+    auto gather = [&](int cx, int cy, real& gr, real& gu, real& gv) {
+        gr = gu = gv = real(0);
+        const int db[3] = {0, -1, 1};
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ox = cx + 1 - a;
+                if (ox < 0 || ox >= H1X) continue;
+                const int n = 3 * b + a, e = (cy - db[b]) * H1X + ox;
+                gu += P[n][e];
+                gv += P[9 + n][e];
+                if (n == 0) gr += P[18][e];
+                else if (n == 1) gr += P[19][e];
+                else if (n == 2) gr += P[20][e];
+                else if (n == 4) gr += P[21][e];
+                else if (n == 7) gr += P[22][e];
+            }
+    };
+    const int cx = t % AV_X + 1, cy = t / AV_X + 1;
+    const int i = i0 + cx - 1, j = j0 + cy - 1;
+    if (j < ny) {
+        // radial ghost columns next to this cell: their u = -1.5 nu/R is
+        // proportional to alpha
+        if (i == 0) {
+            real gr, gu, gv;
+            gather(cx - 1, cy, gr, gu, gv);
+            lal += (double)gu * (double)TAB(T_G_U, -1 + NPAD);
+        }
+        if (i == nx - 1) {
+            real gr, gu, gv;
+            gather(cx + 1, cy, gr, gu, gv);
+            lal += (double)gu * (double)TAB(T_G_U, nx + NPAD);
+        }
+        if (i < nx) {
+            const int ip = i + NPAD;
+            const size_t o = (size_t)j * nx + i;
+            real gr, gu, gv;
+            gather(cx, cy, gr, gu, gv);
+            if (MODE == 0) {
+                put(A.li, o, gr, A.accumulate & 1);
+                put(A.li, N + o, gu, A.accumulate & 1);
+                put(A.li, 2 * N + o, gv, A.accumulate & 1);
+            } else if (MODE == 1) {
+                real cr, cu, cv, cX, cY, pr, pu, pv, pX, pY;
+                src_adj_cell(d, 2, ip, j, h, A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o], A.lo[N + o],
+                             A.lo[2 * N + o], cr, cu, cv, cX, cY, lq);
+                src_adj_cell(d, 1, ip, j, h, A.s[o], A.s[2 * N + o], A.vis2[o], A.vis2[N + o], gr, gu, gv, pr, pu, pv,
+                             pX, pY, lq);
+                A.li[o] = cr + pr; A.li[N + o] = cu + pu; A.li[2 * N + o] = cv + pv;
+                A.lvis[o] = pX; A.lvis[N + o] = pY;
+            } else {
+                real kr, ku, kv, kX, kY;
+                src_adj_cell(d, 0, ip, j, h, A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o], A.lo[N + o],
+                             A.lo[2 * N + o], kr, ku, kv, kX, kY, lq);
+                A.li[o] = kr + gr; A.li[N + o] = ku + gu; A.li[2 * N + o] = kv + gv;
+            }
+        }
+    }
+    if (MODE != 0 && A.q > 0.0) block_sum_add(A.lparam, lq / A.q);
+    if (A.alpha > 0.0) block_sum_add(A.lparam + A.nslot, lal / A.alpha);
+}
+
+// ---------------------------------------------------- source-term adjoint
+// k_sources without viscosity fused around it (module entry points, and
+// steps without viscosity).  accumulate: bit 0 -> li +=, bit 1 -> lvis +=.
 template <typename real>
 __global__ __launch_bounds__(256) void k_adj_src(AdjArgs<real> A) {
     const DiskArgs<real>& d = *A.d;
@@ -835,39 +942,19 @@ __global__ __launch_bounds__(256) void k_adj_src(AdjArgs<real> A) {
     if (i < nx) {
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
-        const int mode = A.mode;
         const real h = (real)(0.5 * d.ctl->dt);
-        const PlanetK& pk = mode == 0 ? d.ctl->pk0 : d.ctl->pk1;
-        const real iR = TAB(T_IR, ip), vbg = TAB(T_VBG, ip);
-        real px = real(0), py = real(0);
-        if (d.n_planet > 0) planet_acc(pk, d.phitab + 4 * j, TAB(T_XC, ip), px, py);
-        const real rho = A.s[o], cvp = A.s[2 * N + o];
-        const real irho = FM<real>::rcp(rho);
         real Vr = 0, Vp = 0;
         if (d.visc) { Vr = A.vis[o]; Vp = A.vis[N + o]; }
-        const real vy = Vp * irho;
-        const real W = mode == 0 ? cvp : cvp + (vy + py) * h;
-        const real lnr = A.lo[o], lnu = A.lo[N + o], lnv = A.lo[2 * N + o];
-        const real lW = lnu * h * (real(2) * vbg + real(2) * W) * iR;
-        const real lpx = lnu * h, lvx = lnu * h;
-        real lcvp = lnv + lW;
-        real lvy = lnv * h, lpy = lnv * h;
-        if (mode != 0) { lvy += lW * h; lpy += lW * h; }
-        real lrho = lnr;
-        real lVr = 0, lVp = 0;
-        if (d.visc) {
-            lVr = lvx * irho;
-            lVp = lvy * irho;
-            lrho -= (lvx * Vr + lvy * Vp) * irho * irho;
-        }
+        real lrho, lu, lcvp, lVr, lVp;
+        src_adj_cell(d, A.mode, ip, j, h, A.s[o], A.s[2 * N + o], Vr, Vp, A.lo[o], A.lo[N + o], A.lo[2 * N + o], lrho,
+                     lu, lcvp, lVr, lVp, lq);
         put(A.li, o, lrho, A.accumulate & 1);
-        put(A.li, N + o, lnu, A.accumulate & 1);
+        put(A.li, N + o, lu, A.accumulate & 1);
         put(A.li, 2 * N + o, lcvp, A.accumulate & 1);
         if (d.visc) {
             put(A.lvis, o, lVr, A.accumulate & 2);
             put(A.lvis, N + o, lVp, A.accumulate & 2);
         }
-        lq = (double)lpx * (double)px + (double)lpy * (double)py;
     }
     if (A.q > 0.0) block_sum_add(A.lparam, lq / A.q);
 }
@@ -890,11 +977,15 @@ struct AdjLaunch {
         hipLaunchKernelGGL(k_adj_adv<real>, dim3((nx + TR - 1) / TR, (ny + TY - 1) / TY), dim3(TR * TY), 0, s, A);
         return hipGetLastError();
     }
-    static hipError_t vis(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
-        hipLaunchKernelGGL(k_adj_vis_part<real>, dim3((nx + 255) / 256, ny), dim3(256), 0, s, A);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_adj_vis_gather<real>, dim3((nx + 2 + 255) / 256, ny), dim3(256), 0, s, A);
+    // viscosity tiles: 64 x 16 in FP32, 64 x 8 in FP64 (LDS)
+    static constexpr int THV = sizeof(real) == 8 ? 8 : 16;
+    // mode 0: viscosity alone; 1: corrector + viscosity + predictor;
+    // 2: kick + viscosity (AdjArgs field use: k_adj_vis)
+    static hipError_t vis(const AdjArgs<real>& A, int mode, int nx, int ny, hipStream_t s) {
+        const dim3 grid((nx + AV_X - 1) / AV_X, (ny + THV - 1) / THV), block(AV_X * THV);
+        if (mode == 1) hipLaunchKernelGGL((k_adj_vis<real, 1, THV>), grid, block, 0, s, A);
+        else if (mode == 2) hipLaunchKernelGGL((k_adj_vis<real, 2, THV>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((k_adj_vis<real, 0, THV>), grid, block, 0, s, A);
         return hipGetLastError();
     }
     static hipError_t src(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {

@@ -1216,8 +1216,7 @@ struct AdjWork {
     std::vector<double> ts, dts;   // start time and size of every forward step
     void* ckpts = nullptr;         // nseg states (3N)
     void* slot = nullptr;          // slots x SLOT_WORDS*N
-    void* lam = nullptr;           // lamA, lamB, lamT (3N each), lamV0, lamV1 (2N each)
-    void* part = nullptr;          // VIS_SLOTS*N
+    void* lam = nullptr;           // lamA, lamB (3N each), lamV0 (2N)
     double* lparam = nullptr;      // per-workgroup partial sums of dL/dq, dL/dalpha
     int nslot = 0;                 // slots per parameter
     double* td = nullptr;          // (t, dt) of every step, on the device
@@ -1235,12 +1234,13 @@ int adj_slots(int nx, int ny) {
     const int a = ((nx + 127) / 128) * ny;
     const int b = ((nx + 2 + 255) / 256) * ny;
     const int c = ((nx + 7) / 8) * ((ny + 15) / 16);
-    return std::max(a, std::max(b, c));
+    const int v = ((nx + 63) / 64) * ((ny + 7) / 8);   // viscosity tiles (the smaller, FP64)
+    return std::max(std::max(a, v), std::max(b, c));
 }
 
 void adj_free(AdjWork* w) {
     if (!w) return;
-    void* bufs[] = {w->ckpts, w->slot, w->lam, w->part, w->lparam, w->td, w->args};
+    void* bufs[] = {w->ckpts, w->slot, w->lam, w->lparam, w->td, w->args};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete w;
@@ -1269,8 +1269,7 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
     };
     if ((rc = dev_alloc(w->bytes, &w->ckpts, (size_t)w->nseg * 3 * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->slot, (size_t)w->slots * SLOT_WORDS * N * es)) ||
-        (rc = dev_alloc(w->bytes, &w->lam, (size_t)13 * N * es)) ||
-        (rc = dev_alloc(w->bytes, &w->part, (size_t)VIS_SLOTS * N * es)) ||
+        (rc = dev_alloc(w->bytes, &w->lam, (size_t)8 * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->lparam, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->td, 2 * (size_t)nsteps * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))))
@@ -1378,27 +1377,23 @@ int adj_step(pg_handle* h, int m, int n, real*& lamA, real*& lamB) {
     int rc = set_step(g, w, n);
     if (rc) return rc;
     real* sl = (real*)w->slot + (size_t)m * SLOT_WORDS * N;
-    real* lamT = (real*)w->lam + 6 * N;
-    real* lamV0 = (real*)w->lam + 9 * N;
-    real* lamV1 = (real*)w->lam + 11 * N;
+    real* lamV0 = (real*)w->lam + 6 * N;
     AdjArgs<real> A{};
     A.d = (const DiskArgs<real>*)g.d_args;
-    A.part = (real*)w->part;
     A.lparam = w->lparam;
     A.nslot = w->nslot;
     A.q = d->P.n_planet > 0 ? d->P.planet_mass : 0.0;
     A.alpha = d->P.visc_flag ? d->P.ss_alpha : 0.0;
     const bool visc = g.visc;
-    // corrector: S4 = C(S3, V1)
-    A.mode = 2; A.s = sl + SL_S3 * N; A.vis = sl + SL_V1 * N; A.lo = lamA; A.li = lamB; A.lvis = lamV1; A.accumulate = 0;
-    HIPL(AdjLaunch<real>::src(A, nx, ny, s));
     if (visc) {
-        // V1 = vis(T)
-        A.s = sl + SL_T * N; A.lvis = lamV1; A.li = lamT; A.accumulate = 0;
-        HIPL(AdjLaunch<real>::vis(A, nx, ny, s));
-        // T = P(S3, V0)
-        A.mode = 1; A.s = sl + SL_S3 * N; A.vis = sl + SL_V0 * N; A.lo = lamT; A.li = lamB; A.lvis = lamV0;
-        A.accumulate = 1;
+        // corrector S4 = C(S3, V1), V1 = vis(T), predictor T = P(S3, V0):
+        // the adjoint of S3 into lamB, of V0 into lamV0
+        A.s = sl + SL_S3 * N; A.vis = sl + SL_V1 * N; A.s2 = sl + SL_T * N; A.vis2 = sl + SL_V0 * N;
+        A.lo = lamA; A.li = lamB; A.lvis = lamV0; A.accumulate = 0;
+        HIPL(AdjLaunch<real>::vis(A, 1, nx, ny, s));
+    } else {
+        // corrector S4 = C(S3)
+        A.mode = 2; A.s = sl + SL_S3 * N; A.vis = nullptr; A.lo = lamA; A.li = lamB; A.accumulate = 0;
         HIPL(AdjLaunch<real>::src(A, nx, ny, s));
     }
     // S3 = shift + advection(SA)
@@ -1410,14 +1405,15 @@ int adj_step(pg_handle* h, int m, int n, real*& lamA, real*& lamB) {
     // S2 = x sweep(S1)
     A.s = sl + SL_S1 * N; A.lo = lamB; A.li = lamA; A.accumulate = 0;
     HIPL(AdjLaunch<real>::x(A, nx, ny, s));
-    // S1 = kick(S0, V0)
-    A.mode = 0; A.s = sl + SL_S0 * N; A.vis = sl + SL_V0 * N; A.lo = lamA; A.li = lamB; A.lvis = lamV0;
-    A.accumulate = visc ? 2 : 0;
-    HIPL(AdjLaunch<real>::src(A, nx, ny, s));
     if (visc) {
-        // V0 = vis(S0)
-        A.s = sl + SL_S0 * N; A.lvis = lamV0; A.li = lamB; A.accumulate = 1;
-        HIPL(AdjLaunch<real>::vis(A, nx, ny, s));
+        // kick S1 = K(S0, V0) and V0 = vis(S0); lamV0 holds what the
+        // predictor sent to V0
+        A.s = sl + SL_S0 * N; A.vis = sl + SL_V0 * N; A.lo = lamA; A.li = lamB; A.lvis = lamV0; A.accumulate = 0;
+        HIPL(AdjLaunch<real>::vis(A, 2, nx, ny, s));
+    } else {
+        // kick S1 = K(S0)
+        A.mode = 0; A.s = sl + SL_S0 * N; A.vis = nullptr; A.lo = lamA; A.li = lamB; A.accumulate = 0;
+        HIPL(AdjLaunch<real>::src(A, nx, ny, s));
     }
     std::swap(lamA, lamB);
     return PG_OK;
@@ -1484,7 +1480,7 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     real* lamB = (real*)w->lam + 3 * N;
     if ((rc = upload(g.prec, g.stream, l0, lamA))) return rc;
     HIPL(hipMemsetAsync(w->lparam, 0, 2 * (size_t)w->nslot * sizeof(double), g.stream));
-    HIPL(hipMemsetAsync((real*)w->lam + 9 * N, 0, 4 * N * sizeof(real), g.stream));
+    HIPL(hipMemsetAsync((real*)w->lam + 6 * N, 0, 2 * N * sizeof(real), g.stream));
     const size_t bytes = 3 * N * sizeof(real);
     for (int k = w->nseg - 1; k >= 0; --k) {
         const int n0 = k * w->ckpt, len = std::min(w->ckpt, w->nsteps - n0);
@@ -1915,17 +1911,16 @@ static int adjoint_pass(pg_handle* h, int pass, double t, double dt, const doubl
         if ((rc = push_ctl(g))) return rc;
     }
     size_t scratch = 0;
-    real *lo = nullptr, *li = nullptr, *lv = nullptr, *part = nullptr;
+    real *lo = nullptr, *li = nullptr, *lv = nullptr;
     double* lp = nullptr;
     auto cleanup = [&](int code) {
-        void* b[] = {lo, li, lv, part, lp};
+        void* b[] = {lo, li, lv, lp};
         for (void* x : b)
             if (x) (void)hipFree(x);
         return code;
     };
     if ((rc = dev_alloc(scratch, &lo, 3 * N * sizeof(real))) || (rc = dev_alloc(scratch, &li, 3 * N * sizeof(real))) ||
         (rc = dev_alloc(scratch, &lv, 2 * N * sizeof(real))) ||
-        (rc = dev_alloc(scratch, &part, (size_t)VIS_SLOTS * N * sizeof(real))) ||
         (rc = dev_alloc(scratch, &lp, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))))
         return cleanup(rc);
     const int nslot = adj_slots(d->nx, d->ny);
@@ -1940,7 +1935,6 @@ static int adjoint_pass(pg_handle* h, int pass, double t, double dt, const doubl
     A.lo = lo;
     A.li = li;
     A.lvis = lv;
-    A.part = part;
     A.lparam = lp;
     A.nslot = nslot;
     A.q = d->P.n_planet > 0 ? d->P.planet_mass : 0.0;
@@ -1951,7 +1945,7 @@ static int adjoint_pass(pg_handle* h, int pass, double t, double dt, const doubl
         case 0: e = AdjLaunch<real>::x(A, d->nx, d->ny, g.stream); break;
         case 1: e = AdjLaunch<real>::phi(A, d->nx, d->ny, g.stream); break;
         case 2: e = AdjLaunch<real>::adv(A, d->nx, d->ny, g.stream); break;
-        case 3: e = AdjLaunch<real>::vis(A, d->nx, d->ny, g.stream); break;
+        case 3: e = AdjLaunch<real>::vis(A, 0, d->nx, d->ny, g.stream); break;
         default: A.mode = pass - 4; e = AdjLaunch<real>::src(A, d->nx, d->ny, g.stream); break;
     }
     if (e != hipSuccess) return cleanup(fail(PG_E_HIP, "adjoint pass launch: %s", hipGetErrorString(e)));

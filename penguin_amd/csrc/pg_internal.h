@@ -134,10 +134,11 @@ struct AdjArgs {
     const DiskArgs<real>* d;   // the disk: tables, control block (dt, planets, shift)
     const real* s;             // forward input state of the pass (3N)
     const real* vis;           // viscous forces the pass read (2N)
+    const real* s2;            // fused viscosity adjoint: the state the viscosity read (3N)
+    const real* vis2;          // fused viscosity adjoint: the forces the predictor read (2N)
     const real* lo;            // adjoint of the pass output (3N)
     real* li;                  // adjoint of the pass input state (3N)
     real* lvis;                // adjoint of vis (2N)
-    real* part;                // viscosity: per-output partial adjoints (VIS_SLOTS*N)
     double* lparam;            // per-workgroup partial sums: [0, nslot) dL/d planet_mass,
                                // [nslot, 2 nslot) dL/d ss_alpha
     int nslot;                 // slots per parameter (>= the largest adjoint grid)
@@ -145,7 +146,6 @@ struct AdjArgs {
     int mode;                  // source-term mode, 0 kick / 1 predictor / 2 corrector
     double q, alpha;           // planet_mass, ss_alpha
 };
-constexpr int VIS_SLOTS = 23;
 
 // A set of disks launched together (one disk, or a batch).  Each kernel's
 // grid walks a work list of (disk, local block) pairs, so one launch covers

@@ -37,7 +37,7 @@ struct AdjLaunch {
     static hipError_t x(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
     static hipError_t phi(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
     static hipError_t adv(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
-    static hipError_t vis(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
+    static hipError_t vis(const AdjArgs<real>& A, int mode, int nx, int ny, hipStream_t s);
     static hipError_t src(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
 };
 
