@@ -1220,16 +1220,27 @@ struct AdjWork {
     double* lparam = nullptr;      // per-workgroup partial sums of dL/dq, dL/dalpha
     int nslot = 0;                 // slots per parameter
     double* td = nullptr;          // (t, dt) of every step, on the device
+    double* td_seg = nullptr;      // (t, dt) of the segment being swept back
     void* args = nullptr;          // slots x AA_N disk descriptors
+    void* stage = nullptr;         // pinned host staging of one adjoint state (3N reals)
     bool forward_done = false;
     size_t bytes = 0;
+    // captured backward sweeps of one segment (recompute + adjoint steps),
+    // by segment length and the buffer holding the incoming adjoint
+    struct SegGraph {
+        int len;
+        void* lam_in;
+        void* lam_out;
+        hipGraphExec_t exec;
+    };
+    std::vector<SegGraph> graphs;
 };
 
 namespace {
 
-// workgroups of the largest adjoint grid (x adjoint: 128-radius tiles x
-// rows; viscosity gather: 256-column segments of nx+2 columns x rows;
-// azimuthal adjoints: 8-radius x 16-row tiles at most)
+// workgroups of the largest adjoint grid (x adjoint: 256-radius tiles x
+// rows; source adjoints: 256-radius segments x rows; azimuthal adjoints:
+// 8-radius x 16-row tiles at most; viscosity: 64 x 8 tiles at most)
 int adj_slots(int nx, int ny) {
     const int a = ((nx + 127) / 128) * ny;
     const int b = ((nx + 2 + 255) / 256) * ny;
@@ -1240,9 +1251,12 @@ int adj_slots(int nx, int ny) {
 
 void adj_free(AdjWork* w) {
     if (!w) return;
-    void* bufs[] = {w->ckpts, w->slot, w->lam, w->lparam, w->td, w->args};
+    for (auto& sg : w->graphs)
+        if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+    void* bufs[] = {w->ckpts, w->slot, w->lam, w->lparam, w->td, w->td_seg, w->args};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (w->stage) (void)hipHostFree(w->stage);
     delete w;
 }
 
@@ -1272,8 +1286,13 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
         (rc = dev_alloc(w->bytes, &w->lam, (size_t)8 * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->lparam, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->td, 2 * (size_t)nsteps * sizeof(double))) ||
+        (rc = dev_alloc(w->bytes, &w->td_seg, 2 * (size_t)ckpt * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))))
         return cleanup(rc);
+    if (hipHostMalloc(&w->stage, 3 * N * es, hipHostMallocDefault) != hipSuccess) {
+        w->stage = nullptr;
+        return cleanup(fail(PG_E_NOMEM, "pinned staging of %zu bytes", 3 * N * es));
+    }
     // descriptors of the recompute passes: the disk's own tables and control
     // block, with the state and viscosity pointers of the slot
     std::vector<DiskArgs<real>> a((size_t)w->slots * AA_N);
@@ -1309,11 +1328,12 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
     return PG_OK;
 }
 
-// prologue of recorded step n: t, dt from the device copy, then the planets
-// and the orbital shift as in the forward step
-int set_step(Group& g, const AdjWork* w, int n) {
-    hipError_t e = g.prec == 8 ? Launch<double>::prologue_replay(g.ls<double>(), w->td + 2 * (size_t)n, g.stream)
-                               : Launch<float>::prologue_replay(g.ls<float>(), w->td + 2 * (size_t)n, g.stream);
+// prologue of step m of the segment being swept back: t, dt from the
+// segment's device copy, then the planets and the orbital shift as in the
+// forward step
+int set_step(Group& g, const AdjWork* w, int m) {
+    hipError_t e = g.prec == 8 ? Launch<double>::prologue_replay(g.ls<double>(), w->td_seg + 2 * (size_t)m, g.stream)
+                               : Launch<float>::prologue_replay(g.ls<float>(), w->td_seg + 2 * (size_t)m, g.stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
     return PG_OK;
 }
@@ -1326,12 +1346,12 @@ int set_step(Group& g, const AdjWork* w, int n) {
 
 // Recompute step n into slot m (and its end state into slot m+1's S0).
 template <typename real>
-int adj_recompute(pg_handle* h, int m, int n, bool write_next) {
+int adj_recompute(pg_handle* h, int m, bool write_next) {
     Group& g = h->g;
     AdjWork* w = h->adj;
     const size_t N = w->N;
     hipStream_t s = g.stream;
-    int rc = set_step(g, w, n);
+    int rc = set_step(g, w, m);
     if (rc) return rc;
     LaunchSet<real> L = g.ls<real>();
     const DiskArgs<real>* a = (const DiskArgs<real>*)w->args + (size_t)m * AA_N;
@@ -1367,14 +1387,14 @@ int adj_recompute(pg_handle* h, int m, int n, bool write_next) {
 // Adjoint passes of step n held in slot m; lam[0] holds the adjoint of the
 // step's end state on entry and of its start state on exit.
 template <typename real>
-int adj_step(pg_handle* h, int m, int n, real*& lamA, real*& lamB) {
+int adj_step(pg_handle* h, int m, real*& lamA, real*& lamB) {
     Group& g = h->g;
     AdjWork* w = h->adj;
     DiskDev* d = g.disks[0];
     const size_t N = w->N;
     const int nx = d->nx, ny = d->ny;
     hipStream_t s = g.stream;
-    int rc = set_step(g, w, n);
+    int rc = set_step(g, w, m);
     if (rc) return rc;
     real* sl = (real*)w->slot + (size_t)m * SLOT_WORDS * N;
     real* lamV0 = (real*)w->lam + 6 * N;
@@ -1432,6 +1452,10 @@ int adj_forward(pg_handle* h, int nsteps, int ckpt, double t0, double dt0, doubl
     if ((rc = adj_alloc<real>(h, nsteps, ckpt))) return rc;
     AdjWork* w = h->adj;
     w->forward_done = false;
+    // backward graphs hold the launch configuration of the previous run
+    for (auto& sg : w->graphs)
+        if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+    w->graphs.clear();
     const size_t bytes = 3 * w->N * sizeof(real);
     w->dts.assign(nsteps, 0.0);
     w->ts.assign(nsteps, 0.0);
@@ -1469,31 +1493,62 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     int rc;
     // adjoint of the final state in the device representation:
     // rho (p = cs2 rho), u, vp (v = vbg + vp)
-    std::vector<double> l0(3 * N);
-    for (size_t o = 0; o < N; ++o) {
-        const int ip = (int)(o % nx) + NPAD;
-        l0[o] = dL[o] + tabv(d, T_CS2, ip) * dL[N + o];
-        l0[N + o] = dL[2 * N + o];
-        l0[2 * N + o] = dL[3 * N + o];
-    }
+    // (through the workspace's pinned staging: no fresh host pages per call)
+    real* st = (real*)w->stage;
+    const double* cs2 = d->tabh.data() + (size_t)T_CS2 * d->tab_stride + NPAD;
+    for (size_t row = 0; row < N; row += nx)
+        for (int i = 0; i < nx; ++i) {
+            const size_t o = row + i;
+            st[o] = (real)(dL[o] + cs2[i] * dL[N + o]);
+            st[N + o] = (real)dL[2 * N + o];
+            st[2 * N + o] = (real)dL[3 * N + o];
+        }
     real* lamA = (real*)w->lam;
     real* lamB = (real*)w->lam + 3 * N;
-    if ((rc = upload(g.prec, g.stream, l0, lamA))) return rc;
+    HIPL(hipMemcpyAsync(lamA, st, 3 * N * sizeof(real), hipMemcpyHostToDevice, g.stream));
     HIPL(hipMemsetAsync(w->lparam, 0, 2 * (size_t)w->nslot * sizeof(double), g.stream));
     HIPL(hipMemsetAsync((real*)w->lam + 6 * N, 0, 2 * N * sizeof(real), g.stream));
     const size_t bytes = 3 * N * sizeof(real);
     for (int k = w->nseg - 1; k >= 0; --k) {
         const int n0 = k * w->ckpt, len = std::min(w->ckpt, w->nsteps - n0);
         HIPL(hipMemcpyAsync(w->slot, (char*)w->ckpts + (size_t)k * bytes, bytes, hipMemcpyDeviceToDevice, g.stream));
-        for (int m = 0; m < len; ++m)
-            if ((rc = adj_recompute<real>(h, m, n0 + m, m + 1 < len))) return rc;
-        for (int m = len - 1; m >= 0; --m)
-            if ((rc = adj_step<real>(h, m, n0 + m, lamA, lamB))) return rc;
+        HIPL(hipMemcpyAsync(w->td_seg, w->td + 2 * (size_t)n0, 2 * (size_t)len * sizeof(double),
+                            hipMemcpyDeviceToDevice, g.stream));
+        // the segment's sweep (recompute, then the adjoint steps in reverse)
+        // as one captured graph, replayed for every segment of the same
+        // length that starts from the same adjoint buffer
+        AdjWork::SegGraph* sg = nullptr;
+        for (auto& x : w->graphs)
+            if (x.len == len && x.lam_in == (void*)lamA) sg = &x;
+        if (!sg) {
+            real* la = lamA;
+            real* lb = lamB;
+            HIPL(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            rc = PG_OK;
+            for (int m = 0; m < len && !rc; ++m) rc = adj_recompute<real>(h, m, m + 1 < len);
+            for (int m = len - 1; m >= 0 && !rc; --m) rc = adj_step<real>(h, m, la, lb);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(g.stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ec != hipSuccess) return fail(PG_E_HIP, "adjoint capture: %s", hipGetErrorString(ec));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) return fail(PG_E_HIP, "adjoint graph: %s", hipGetErrorString(ei));
+            w->graphs.push_back({len, (void*)lamA, (void*)la, exec});
+            sg = &w->graphs.back();
+        }
+        HIPL(hipGraphLaunch(sg->exec, g.stream));
+        if (sg->lam_out != (void*)lamA) std::swap(lamA, lamB);
     }
-    std::vector<double> lam0;
-    if ((rc = download(g.prec, g.stream, lamA, lam0, 3 * N))) return rc;
+    HIPL(hipMemcpyAsync(st, lamA, 3 * N * sizeof(real), hipMemcpyDeviceToHost, g.stream));
     std::vector<double> slots(2 * (size_t)w->nslot);
-    HIPL(hipMemcpy(slots.data(), w->lparam, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPL(hipMemcpyAsync(slots.data(), w->lparam, slots.size() * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPL(hipStreamSynchronize(g.stream));
+    const real* lam0 = st;
     double lp[2] = {0.0, 0.0};
     for (int k = 0; k < w->nslot; ++k) {
         lp[0] += slots[k];
@@ -1502,8 +1557,10 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     // the initial radial velocity -1.5 nu(R)/R is proportional to alpha
     double dal = lp[1];
     if (d->P.visc_flag && d->P.ss_alpha > 0.0) {
+        const double* gu = d->tabh.data() + (size_t)T_G_U * d->tab_stride + NPAD;
         double acc = 0.0;
-        for (size_t o = 0; o < N; ++o) acc += lam0[N + o] * tabv(d, T_G_U, (int)(o % nx) + NPAD);
+        for (size_t row = 0; row < N; row += nx)
+            for (int i = 0; i < nx; ++i) acc += (double)lam0[N + row + i] * gu[i];
         dal += acc / d->P.ss_alpha;
     }
     out->dq = lp[0];
@@ -1511,14 +1568,13 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     if (out->dstate0) {
         double* s0 = out->dstate0;
         for (size_t o = 0; o < N; ++o) {
-            s0[o] = lam0[o];
+            s0[o] = (double)lam0[o];
             s0[N + o] = 0.0;
-            s0[2 * N + o] = lam0[N + o];
-            s0[3 * N + o] = lam0[2 * N + o];
+            s0[2 * N + o] = (double)lam0[N + o];
+            s0[3 * N + o] = (double)lam0[2 * N + o];
             s0[4 * N + o] = 0.0;
         }
     }
-    HIPL(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 

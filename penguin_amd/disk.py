@@ -232,7 +232,7 @@ class Disk:
         state) (5, ny, nx))."""
         dL = np.ascontiguousarray(dL_dgas, dtype=np.float64)
         assert dL.shape == (5, self.ny, self.nx)
-        d0 = np.zeros((5, self.ny, self.nx))
+        d0 = np.empty((5, self.ny, self.nx))   # every field is written
         g = Grads(0.0, 0.0, _ptr(d0))
         _check(load_library().pg_adjoint_backward(self._h, _ptr(dL), ctypes.byref(g)))
         return g.dq, g.dalpha, d0
