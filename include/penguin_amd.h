@@ -104,6 +104,25 @@ int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out);
    the snapshot of frame n is "binary_" + label + "_" + 5-digit n (main.cu:403). */
 int pg_snapshot_label(const pg_params* params, char* buf, size_t len);
 
+/* ---- diagnostics ------------------------------------------------------- */
+/* Start a time average of the gas over a window of `interval`: zero the
+   average (init_average, output.cu:316-329) and from now on add
+   state * dt/2 / interval before and after every device-loop step
+   (averaging, output.cu:299-314, as main.cu:344-350 calls it), in FP64. */
+int pg_average_begin(pg_handle* h, double interval);
+/* Stop adding to the average (it is kept). */
+int pg_average_end(pg_handle* h);
+/* The average so far in the pg_get_state gas layout (5 x xres*yres). */
+int pg_average_get(pg_handle* h, double* gas);
+/* Snapshot of the average in the write_check_point layout, as main.cu:409-414
+   writes "binary_" + label + "_ave_" + frame: averaged gas, current dust. */
+int pg_write_average_snapshot(pg_handle* h, const char* filename, double t);
+/* Torque of the planet on the gas at the current state: the sum over cells
+   of R gy rho dV with the planet at the end of the last step (global_sum
+   with cal_torque and output_gy, output.cu:336-457, output/forces.cu:41-58,
+   106-127), FP64.  main.cu:379-388 prints it divided by q^2/h^2 (2D). */
+int pg_torque(pg_handle* h, double* torque);
+
 /* ---- introspection ----------------------------------------------------- */
 /* HIP stream the handle launches on (as a void*), for external capture. */
 int pg_stream(pg_handle* h, void** stream);

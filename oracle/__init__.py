@@ -75,6 +75,10 @@ def lib():
             "orc_dust_flux": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                      ctypes.c_int, ctypes.c_double, _dp]),
             "orc_orbit_solution": (None, [pp, ctypes.c_double, _dp]),
+            "orc_average_begin": (None, [vp, ctypes.c_double]),
+            "orc_average_end": (None, [vp]),
+            "orc_average_get": (None, [vp, _dp]),
+            "orc_torque": (ctypes.c_double, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -195,6 +199,20 @@ class Oracle:
     def write_snapshot(self, filename: str, t: float):
         if lib().orc_write_snapshot(self._h, filename.encode(), t) != 0:
             raise OSError(filename)
+
+    def average_begin(self, interval: float):
+        lib().orc_average_begin(self._h, interval)
+
+    def average_end(self):
+        lib().orc_average_end(self._h)
+
+    def average(self) -> np.ndarray:
+        g = np.zeros((5, self.ny, self.nx))
+        lib().orc_average_get(self._h, ptr(g))
+        return g
+
+    def torque(self) -> float:
+        return lib().orc_torque(self._h)
 
 
 def num_threads() -> int:

@@ -631,6 +631,8 @@ struct orc_grid {
     std::vector<double> rot, res;
     std::vector<int> inc;
     Body planet;
+    std::vector<Cell> A;        // time average of the gas (Grid::A)
+    double ave_interval = 0.0;  // 0: not averaging
     size_t ind(int i, int j) const { return (size_t)i + (size_t)xarr * (size_t)j; }
     double xc(int i) const { return 0.5 * (xa[i + 1] + xa[i]); }
     double yc(int j) const { return 0.5 * (ya[j + 1] + ya[j]); }
@@ -1303,12 +1305,28 @@ void orc_solve(orc_grid* g, double t, double dt) {  // solve.cu:279-292
     if (g->P.kill_flag) killwave(*g, dt);
 }
 
+// add_grid (output.cu:250-273): A += C * fac over interior cells
+static void add_grid(orc_grid& g, double fac) {
+    for (int j = g.ypad; j < g.yarr - g.ypad; ++j)
+        for (int i = g.xpad; i < g.xarr - g.xpad; ++i) {
+            const Cell& c = g.C[g.ind(i, j)];
+            Cell& a = g.A[g.ind(i, j)];
+            a.r += c.r * fac;
+            a.p += c.p * fac;
+            a.u += c.u * fac;
+            a.v += c.v * fac;
+            a.w += c.w * fac;
+        }
+}
+
 void orc_run(orc_grid* g, int nsteps, double* t_io, double* dt_io, double* dts) {  // main.cu:320-359
     double t = *t_io, dt = *dt_io;
     for (int n = 0; n < nsteps; ++n) {
         dt = orc_global_dt(g, dt);
         if (t + dt > g->P.end_time) dt = g->P.end_time - t;
+        if (g->ave_interval > 0.0) add_grid(*g, 0.5 * dt / g->ave_interval);  // averaging, main.cu:344
         orc_solve(g, t, dt);
+        if (g->ave_interval > 0.0) add_grid(*g, 0.5 * dt / g->ave_interval);  // main.cu:350
         t += dt;
         if (dts) dts[n] = dt;
     }
@@ -1390,6 +1408,46 @@ int orc_write_snapshot(const orc_grid* g, const char* filename, double t) {
     return 0;
 }
 void orc_evolve_planet(orc_grid* g, double t) { orbit_solution(g->P, g->planet, t); }
+
+// init_average (output.cu:316-329) and the averaging switch of main.cu
+void orc_average_begin(orc_grid* g, double interval) {
+    g->A.assign(g->C.size(), Cell{0, 0, 0, 0, 0});
+    g->ave_interval = interval;
+}
+void orc_average_end(orc_grid* g) { g->ave_interval = 0.0; }
+// the average over interior cells, fields rho, p, u, v, w (x fastest)
+void orc_average_get(const orc_grid* g, double* gas) {
+    const size_t N = (size_t)g->xres * g->yres;
+    for (int j = 0; j < g->yres; ++j)
+        for (int i = 0; i < g->xres; ++i) {
+            const Cell& a = g->A[g->ind(i + g->xpad, j + g->ypad)];
+            const size_t o = (size_t)j * g->xres + i;
+            gas[o] = a.r;
+            gas[N + o] = a.p;
+            gas[2 * N + o] = a.u;
+            gas[3 * N + o] = a.v;
+            gas[4 * N + o] = a.w;
+        }
+}
+
+// global_sum (output.cu:336-457) for one planet in 2D cylindrical geometry:
+// the sum over interior cells of R gy rho dV (cal_torque), gy from
+// o_star_planet_grav_cyl at dt = 0 (output/forces.cu:41-58, 106-127), which
+// is the solver's star_planet_grav_cyl; summed in cell order
+double orc_torque(const orc_grid* g) {
+    double sum = 0.0;
+    for (int j = g->ypad; j < g->yarr - g->ypad; ++j)
+        for (int i = g->xpad; i < g->xarr - g->xpad; ++i) {
+            const double rad = 0.5 * (g->xa[i + 1] + g->xa[i]);
+            const double azi = 0.5 * (g->ya[j + 1] + g->ya[j]);
+            double dv = 0.5 * (g->xa[i + 1] * g->xa[i + 1] - g->xa[i] * g->xa[i]);
+            dv *= (g->ya[j + 1] - g->ya[j]);
+            double fx, fy, fz;
+            planet_force_cyl(rad, azi, 0.0, g->planet, fx, fy, fz);
+            sum += rad * fy * (g->C[g->ind(i, j)].r * dv);
+        }
+    return sum;
+}
 
 // ------------------------------------------------------------ unit level
 void orc_fourth_parameters(int geom, const double* x6, const double* a5, double* par3) {

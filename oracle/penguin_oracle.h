@@ -83,6 +83,13 @@ void orc_dust_flux(int geom, int n, const double* xa, const double* dv,
                    const double* w, int i, double dt, double* F4);
 /* analytic Kepler orbit (planet.cu:105-133): body10 = m,a,e,x,y,z,vx,vy,vz,rs */
 void orc_orbit_solution(const pg_params* P, double t, double* body10);
+/* time average of the gas (init_average/averaging, output.cu:250-329; applied
+   around each orc_run step as main.cu:344-350 does) */
+void orc_average_begin(orc_grid* g, double interval);
+void orc_average_end(orc_grid* g);
+void orc_average_get(const orc_grid* g, double* gas5);
+/* planet torque on the gas (global_sum, output.cu:336-457), 2D cylindrical */
+double orc_torque(const orc_grid* g);
 
 #ifdef __cplusplus
 }

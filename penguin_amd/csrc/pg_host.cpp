@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/penguin_amd.h"
+#include "pg_device.h"
 #include "pg_internal.h"
 #include "pg_launch.h"
 
@@ -202,6 +203,8 @@ struct DiskDev {
     void* dust[2] = {nullptr, nullptr};
     void* vis = nullptr;
     void* vis2 = nullptr;
+    double* ave = nullptr;        // time average of the gas (3N FP64), while averaging
+    double* diag = nullptr;       // per-tile partial sums of a diagnostic
     void* tab = nullptr;
     void* phitab = nullptr;
     void* res = nullptr;
@@ -263,6 +266,7 @@ struct DiskDev {
         d.rate_nblk = rate_nblk;
         d.dts = dts;
         d.dts_cap = dts_cap;
+        d.ave = ave;
         return d;
     }
 };
@@ -285,6 +289,7 @@ struct Group {
     int n_x = 0, n_ring = 0, n_pt = 0;
     int tile_x = 256, ring_b = RING_THREADS, order = 0;
     int pt_rows = PT_YMAX;       // rows of the pointwise/stencil tiles
+    bool ave = false;            // time-average the gas around each device-loop step
     int ny_max = 0;
     // profiling: event pairs recorded around launches without a host wait,
     // resolved after the stream drains
@@ -530,7 +535,7 @@ void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<do
 void disk_free(DiskDev* d) {
     if (!d) return;
     void* bufs[] = {d->gas[0], d->gas[1], d->dust[0], d->dust[1], d->vis, d->vis2, d->tab, d->phitab,
-                    d->res, d->inc, d->rot, d->dts, d->ctl, d->rate_part};
+                    d->res, d->inc, d->rot, d->dts, d->ctl, d->rate_part, d->ave, d->diag};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (d->ctl_host) (void)hipHostFree(d->ctl_host);
@@ -589,8 +594,13 @@ int disk_create(const pg_params& P, hipStream_t s, DiskDev** out) {
         return cleanup(fail(PG_E_HIP, "rot upload failed"));
     if (hipMemset(h->inc, 0, h->nx * sizeof(int)) != hipSuccess) return cleanup(fail(PG_E_HIP, "memset failed"));
     StepCtl& c = *h->ctl_host;
-    c.planet0 = h->planet_init;
-    c.planet1 = h->planet_init;
+    // the planet at t = 0 (init_planet, planet.cu:319-403), for diagnostics
+    // taken before the first step; every step's prologue recomputes it
+    Body b0 = h->planet_init;
+    if (P.n_planet > 0) orbit_solution(b0, 0.0, P.ramp_time, P.planet_mass, P.frame_omega);
+    c.planet0 = b0;
+    c.planet1 = b0;
+    c.pk0 = c.pk1 = planet_kernel_view(b0);
     c.end_time = P.end_time;
     if (hipMemcpyAsync(h->ctl, h->ctl_host, sizeof(StepCtl), hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
@@ -928,6 +938,16 @@ int pull_ctl(Group& g) {
     return PG_OK;
 }
 
+// the time average's share of the current state (before and after each
+// step, main.cu:344-350), when averaging is on
+int average_pass(Group& g) {
+    if (!g.ave) return PG_OK;
+    hipError_t e = g.prec == 8 ? Launch<double>::average(g.ls<double>(), g.par(), g.stream)
+                               : Launch<float>::average(g.ls<float>(), g.par(), g.stream);
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch average failed: %s", hipGetErrorString(e));
+    return PG_OK;
+}
+
 int launch_cfl(Group& g) {
     hipError_t e = g.prec == 8 ? Launch<double>::cfl(g.ls<double>(), g.par(), g.stream)
                                : Launch<float>::cfl(g.ls<float>(), g.par(), g.stream);
@@ -990,7 +1010,9 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
                 hipGraph_t gr;
                 HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
                 int rc1 = prologue(g, 1, 1);
+                if (!rc1) rc1 = average_pass(g);
                 int rc2 = rc1 ? rc1 : passes(g, true);
+                if (!rc2) rc2 = average_pass(g);
                 hipError_t ec = hipStreamEndCapture(g.stream, &gr);
                 if (rc2) return rc2;
                 if (ec != hipSuccess) return fail(PG_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
@@ -1013,7 +1035,9 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
             if ((rc = timed_begin(g))) return rc;
             if ((rc = prologue(g, 1, 1))) return rc;
             if ((rc = timed_end(g, K_PRO))) return rc;
+            if ((rc = average_pass(g))) return rc;
             if ((rc = passes(g, true))) return rc;
+            if ((rc = average_pass(g))) return rc;
         }
         if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
     }
@@ -1844,14 +1868,15 @@ int pg_snapshot_label(const pg_params* P, char* buf, size_t len) {
     return PG_OK;
 }
 
-int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
-    CHECK_H(h);
-    if (!filename) return fail(PG_E_INVALID, "null filename");
-    const DiskDev* d = h->g.disks[0];
+}  // extern "C"
+
+namespace {
+
+// write_check_point layout (output.cu:125-180): t, x edges, y edges, then
+// the interior fields rho, p, u, v (2D: no w) and, with dust, its four
+int write_snapshot_file(const DiskDev* d, const char* filename, double t, const std::vector<double>& gas,
+                        const std::vector<double>& dust) {
     const size_t N = d->N;
-    std::vector<double> gas(5 * N), dust(d->P.dust_flag ? 4 * N : 0);
-    int rc = disk_get_state(h->g, 0, gas.data(), d->P.dust_flag ? dust.data() : nullptr);
-    if (rc) return rc;
     std::ofstream f(filename, std::ios::out | std::ios::binary | std::ios::trunc);
     if (!f) return fail(PG_E_INVALID, "cannot open %s for writing", filename);
     std::vector<double> xa(d->nx + 1), ya(d->ny + 1);
@@ -1863,6 +1888,109 @@ int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
     f.write((const char*)gas.data(), sizeof(double) * 4 * N);  // rho, p, u, v (ndim 2: no w)
     if (d->P.dust_flag) f.write((const char*)dust.data(), sizeof(double) * 4 * N);  // w_d too (output.cu:171)
     if (!f) return fail(PG_E_INVALID, "write to %s failed", filename);
+    return PG_OK;
+}
+
+// the gas time average in the pg_get_state layout (rho, p, u, v, w)
+int average_get(pg_handle* h, double* gas) {
+    Group& g = h->g;
+    DiskDev* d = g.disks[0];
+    if (!d->ave) return fail(PG_E_STATE, "no time average: call pg_average_begin first");
+    const size_t N = d->N;
+    const int nx = d->nx;
+    std::vector<double> a(3 * N);
+    HIP_TRY(hipMemcpyAsync(a.data(), d->ave, 3 * N * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    int rc = pull_ctl(g);
+    if (rc) return rc;
+    const double W = d->ctl_host->ave_w;
+    for (size_t o = 0; o < N; ++o) {
+        const int ip = (int)(o % nx) + NPAD;
+        gas[o] = a[o];
+        gas[N + o] = tabv(d, T_CS2, ip) * a[o];
+        gas[2 * N + o] = a[N + o];
+        gas[3 * N + o] = tabv(d, T_VBG, ip) * W + a[2 * N + o];
+        gas[4 * N + o] = 0.0;
+    }
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
+    CHECK_H(h);
+    if (!filename) return fail(PG_E_INVALID, "null filename");
+    const DiskDev* d = h->g.disks[0];
+    const size_t N = d->N;
+    std::vector<double> gas(5 * N), dust(d->P.dust_flag ? 4 * N : 0);
+    int rc = disk_get_state(h->g, 0, gas.data(), d->P.dust_flag ? dust.data() : nullptr);
+    if (rc) return rc;
+    return write_snapshot_file(d, filename, t, gas, dust);
+}
+
+int pg_write_average_snapshot(pg_handle* h, const char* filename, double t) {
+    CHECK_H(h);
+    if (!filename) return fail(PG_E_INVALID, "null filename");
+    const DiskDev* d = h->g.disks[0];
+    const size_t N = d->N;
+    std::vector<double> gas(5 * N), dust(d->P.dust_flag ? 4 * N : 0);
+    int rc = average_get(h, gas.data());
+    if (rc) return rc;
+    // the dust is the current state, as cpy_grid_DevicetoHost(.., true) leaves it
+    if (d->P.dust_flag && (rc = disk_get_state(h->g, 0, nullptr, dust.data()))) return rc;
+    return write_snapshot_file(d, filename, t, gas, dust);
+}
+
+int pg_average_begin(pg_handle* h, double interval) {
+    CHECK_H(h);
+    if (!(interval > 0.0)) return fail(PG_E_INVALID, "interval must be > 0");
+    Group& g = h->g;
+    DiskDev* d = g.disks[0];
+    int rc;
+    if (!d->ave && (rc = dev_alloc(g.bytes, &d->ave, 3 * d->N * sizeof(double)))) return rc;
+    HIP_TRY(hipMemsetAsync(d->ave, 0, 3 * d->N * sizeof(double), g.stream));
+    if ((rc = pull_ctl(g))) return rc;
+    d->ctl_host->ave_interval = interval;
+    d->ctl_host->ave_w = 0.0;
+    if ((rc = push_ctl(g)) || (rc = group_upload_args(g))) return rc;
+    g.ave = true;
+    g.drop_graph();
+    return PG_OK;
+}
+
+int pg_average_end(pg_handle* h) {
+    CHECK_H(h);
+    h->g.ave = false;
+    h->g.drop_graph();
+    return PG_OK;
+}
+
+int pg_average_get(pg_handle* h, double* gas) {
+    CHECK_H(h);
+    if (!gas) return fail(PG_E_INVALID, "null argument");
+    return average_get(h, gas);
+}
+
+int pg_torque(pg_handle* h, double* torque) {
+    CHECK_H(h);
+    if (!torque) return fail(PG_E_INVALID, "null argument");
+    Group& g = h->g;
+    DiskDev* d = g.disks[0];
+    if (d->P.n_planet < 1) return fail(PG_E_STATE, "the disk has no planet");
+    int rc;
+    const int ntile = ((d->nx + PT_X - 1) / PT_X) * ((d->ny + PT_TY - 1) / PT_TY);   // the most tiles
+    if (!d->diag && (rc = dev_alloc(g.bytes, &d->diag, ntile * sizeof(double)))) return rc;
+    hipError_t e = g.prec == 8 ? Launch<double>::torque(g.ls<double>(), g.par(), d->diag, g.stream)
+                               : Launch<float>::torque(g.ls<float>(), g.par(), d->diag, g.stream);
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch torque failed: %s", hipGetErrorString(e));
+    const int n = d->rate_nblk;   // tiles of the current schedule
+    std::vector<double> part(n);
+    HIP_TRY(hipMemcpyAsync(part.data(), d->diag, n * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    double s = 0.0;
+    for (double x : part) s += x;
+    *torque = s;
     return PG_OK;
 }
 

@@ -18,6 +18,9 @@
 //             [0, nx + 2*NPAD)), precomputed in FP64 on the host.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 
@@ -46,6 +49,20 @@ struct PlanetK {
     double m, x, rs, m_over_x2, sphi, cphi, shphi, chphi;
 };
 
+// What the source kernel needs of a planet, precomputed once per step.
+__host__ __device__ inline PlanetK planet_kernel_view(const Body& b) {
+    PlanetK k;
+    k.m = b.m;
+    k.x = b.x;
+    k.rs = b.rs;
+    k.m_over_x2 = b.m / (b.x * b.x);
+    k.sphi = sin(b.y);
+    k.cphi = cos(b.y);
+    k.shphi = sin(0.5 * b.y);
+    k.chphi = cos(0.5 * b.y);
+    return k;
+}
+
 // Per-step control block, device resident, always FP64.
 struct StepCtl {
     double t;            // time at the start of the step
@@ -58,6 +75,8 @@ struct StepCtl {
     Body planet0;        // planet at t     (first half kick)
     Body planet1;        // planet at t+dt  (predictor and corrector)
     PlanetK pk0, pk1;    // the same, as the source kernel reads them
+    double ave_interval; // time-average window (the averaging weight is dt/2 over it)
+    double ave_w;        // sum of the averaging weights so far
 };
 
 // Per-radius tables (FP64 on the host; cast to the storage type on upload).
@@ -124,6 +143,7 @@ struct DiskArgs {
     real* out_kick;
     real* out_swept;
     real* out_pred;
+    double* ave;         // 3*N FP64 time average of the gas (rho, u, vp), or null
 };
 
 // Buffers of one adjoint pass (pg_adjoint.hip): the forward inputs the

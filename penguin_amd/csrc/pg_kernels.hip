@@ -963,26 +963,93 @@ __global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
     block_max_store(d.rate_part + w.y, rate);
 }
 
+// ---------------------------------------------------------- diagnostics
+// Time average of the gas: add_grid (output.cu:250-273) with the weight
+// dt/2 / interval that averaging (output.cu:299-314) applies before and
+// after each step (main.cu:344-350), accumulated in FP64.  The average holds
+// rho, u and vp; the sum of the weights (for p = cs2 rho and v = vbg + vp)
+// accumulates in the control block, added once per disk by its tile 0.
+template <typename real, int TH>
+__global__ __launch_bounds__(256) void k_average(LaunchSet<real> ls, int par) {
+    const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;   // padding entry of the work list (whole block)
+    const DiskArgs<real>& d = ls.disks[w.x];
+    if (!d.ave) return;
+    const int nx = d.nx, ny = d.ny;
+    const size_t N = (size_t)nx * ny;
+    const double fac = 0.5 * d.ctl->dt / d.ctl->ave_interval;
+    if (w.y == 0 && threadIdx.x == 0) d.ctl->ave_w += fac;
+    const int ntx = pt_ntx(nx);
+    const int ty = w.y / ntx;
+    const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
+    if (i >= nx) return;
+    const PG_GLOBAL real* __restrict__ st = gmem(d.gas[gsel(par)]);
+    double* __restrict__ ave = d.ave;
+#pragma unroll
+    for (int k = 0; k < TH / PT_TY; ++k) {
+        const int j = ty * TH + (int)(threadIdx.x / PT_X) + PT_TY * k;
+        if (j >= ny) break;
+        const size_t o = (size_t)j * nx + i;
+        for (int f = 0; f < 3; ++f) ave[f * N + o] += fac * (double)st[f * N + o];
+    }
+}
+
+// Block sum of a double into this block's slot of a partial array.
+__device__ __forceinline__ void block_sum_store(double* part, double x) {
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    __shared__ double wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) m += wsum[k];
+        part[0] = m;
+    }
+}
+
+// Torque of the planet on the gas: the sum over interior cells of
+// R gy rho dV (get_sum_lv1 + cal_torque, output.cu:336-402), with gy the
+// azimuthal planet acceleration of o_star_planet_grav_cyl at dt = 0
+// (output/forces.cu:41-58, 106-127; its indirect term included) for the
+// planet at the end of the last step, in FP64.  One partial per tile.
+template <typename real, int TH>
+__global__ __launch_bounds__(256) void k_torque(LaunchSet<real> ls, int par, double* part) {
+    const int2 w = ls.wl_pt[blockIdx.x];
+    if (w.y < 0) return;   // padding entry of the work list (whole block)
+    const DiskArgs<real>& d = ls.disks[w.x];
+    const int nx = d.nx, ny = d.ny;
+    const int ntx = pt_ntx(nx);
+    const int ty = w.y / ntx;
+    const int i = (w.y - ty * ntx) * PT_X + (int)(threadIdx.x % PT_X);
+    const PG_GLOBAL real* __restrict__ rho = gmem(d.gas[gsel(par)]);
+    const PlanetK pk = d.ctl->pk1;
+    double acc = 0.0;
+    if (i < nx) {
+        const double R = d.rot[nx + i];
+        const double dv = (double)TAB(T_XV, i + NPAD) * d.dy_d;
+#pragma unroll
+        for (int k = 0; k < TH / PT_TY; ++k) {
+            const int j = ty * TH + (int)(threadIdx.x / PT_X) + PT_TY * k;
+            if (j >= ny) break;
+            const double sj = (double)d.phitab[4 * j], cj = (double)d.phitab[4 * j + 1];
+            const double sn = sj * pk.cphi - cj * pk.sphi;   // sin(phi - phi_p)
+            const double cs = cj * pk.cphi + sj * pk.sphi;   // cos(phi - phi_p)
+            const double Rp = sqrt(R * R + pk.x * pk.x - 2.0 * pk.x * R * cs + pk.rs * pk.rs);
+            const double kk = 1.0 / (Rp * Rp * Rp);
+            const double gy = -pk.m * pk.x * sn * kk + sn * pk.m_over_x2;
+            acc += R * gy * ((double)rho[(size_t)j * nx + i] * dv);
+        }
+    }
+    block_sum_store(part + w.y, acc);
+}
+
 // Max of the partials into the control block (host-driven global_dt).
 template <typename real>
 __global__ __launch_bounds__(1024) void k_rate_reduce(LaunchSet<real> ls) {
     const DiskArgs<real>& d = ls.disks[blockIdx.x];
     double m = reduce_partials(d.rate_part, d.rate_nblk);
     if (threadIdx.x == 0) d.ctl->rate_bits = (unsigned long long)__double_as_longlong(m);
-}
-
-// What the source kernel needs of a planet, precomputed once per step.
-__host__ __device__ inline PlanetK planet_kernel_view(const Body& b) {
-    PlanetK k;
-    k.m = b.m;
-    k.x = b.x;
-    k.rs = b.rs;
-    k.m_over_x2 = b.m / (b.x * b.x);
-    k.sphi = sin(b.y);
-    k.cphi = cos(b.y);
-    k.shphi = sin(0.5 * b.y);
-    k.chphi = cos(0.5 * b.y);
-    return k;
 }
 
 // Step prologue, one block.  mode 0: t and dt were set by the host.
@@ -1134,6 +1201,14 @@ struct Launch {
     }
     static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s) {
         PT_LAUNCH(k_corrvis, ls, s, par, kill);
+        return hipGetLastError();
+    }
+    static hipError_t average(const LaunchSet<real>& ls, int par, hipStream_t s) {
+        PT_LAUNCH(k_average, ls, s, par);
+        return hipGetLastError();
+    }
+    static hipError_t torque(const LaunchSet<real>& ls, int par, double* part, hipStream_t s) {
+        PT_LAUNCH(k_torque, ls, s, par, part);
         return hipGetLastError();
     }
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s) {

@@ -24,6 +24,8 @@ struct Launch {
     // CFL rate + viscosity of the corrected state (d.vis2 -> d.vis)
     static hipError_t predvis(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s);
+    static hipError_t average(const LaunchSet<real>& ls, int par, hipStream_t s);
+    static hipError_t torque(const LaunchSet<real>& ls, int par, double* part, hipStream_t s);
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t rate_reduce(const LaunchSet<real>& ls, hipStream_t s);
     static hipError_t prologue(const LaunchSet<real>& ls, int mode, int do_orbadv, hipStream_t s);

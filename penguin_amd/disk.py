@@ -59,6 +59,11 @@ SIGNATURES = {
     "pg_batch_get_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "pg_batch_set_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "pg_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
+    "pg_average_begin": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "pg_average_end": (ctypes.c_int, [_vp]),
+    "pg_average_get": (ctypes.c_int, [_vp, _dp]),
+    "pg_write_average_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
+    "pg_torque": (ctypes.c_int, [_vp, _dp]),
     "pg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pg_batch_get_schedule": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "pg_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -248,6 +253,30 @@ class Disk:
         dp = np.zeros(2)
         _check(load_library().pg_adjoint_pass(self._h, k, t, dt, _ptr(lo), _ptr(li), _ptr(dp)))
         return li, dp
+
+    # ---- diagnostics (output.cu:250-457) --------------------------------
+    def average_begin(self, interval: float):
+        """Start a time average of the gas over `interval` (init_average +
+        averaging around every device-loop step, main.cu:344-350)."""
+        _check(load_library().pg_average_begin(self._h, float(interval)))
+
+    def average_end(self):
+        _check(load_library().pg_average_end(self._h))
+
+    def average(self) -> np.ndarray:
+        """The gas average so far, (5, ny, nx): rho, p, u, v, w."""
+        g = np.empty((5, self.ny, self.nx))
+        _check(load_library().pg_average_get(self._h, _ptr(g)))
+        return g
+
+    def write_average_snapshot(self, filename: str, t: float = None):
+        _check(load_library().pg_write_average_snapshot(self._h, filename.encode(), self.t if t is None else t))
+
+    def torque(self) -> float:
+        """Planet torque on the gas (global_sum, output.cu:336-457)."""
+        x = ctypes.c_double()
+        _check(load_library().pg_torque(self._h, ctypes.byref(x)))
+        return x.value
 
     # ---- snapshots (output.cu:125-248) ----------------------------------
     def write_snapshot(self, filename: str, t: float = None):
