@@ -16,18 +16,76 @@ from penguin_amd.params import Params
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+TLM_PATH = os.path.join(_HERE, "_build", "liboracle_tlm.so")
 _lib = None
+_tlm = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with g++/OpenMP (seconds)."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["_build/liboracle.so"],
-                       cwd=_HERE, check=True)
+    """Compile the oracle and its tangent-linear twin with g++/OpenMP
+    (seconds)."""
+    if force or not (os.path.exists(LIB_PATH) and os.path.exists(TLM_PATH)):
+        subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["all"], cwd=_HERE, check=True)
     return LIB_PATH
+
+
+def _bind(L):
+    vp = ctypes.c_void_p
+    pp = ctypes.POINTER(Params)
+    sig = {
+        "orc_create": (vp, [pp]),
+        "orc_destroy": (None, [vp]),
+        "orc_num_threads": (ctypes.c_int, []),
+        "orc_get_state": (None, [vp, _dp, _dp]),
+        "orc_set_state": (None, [vp, _dp, _dp]),
+        "orc_get_coords": (None, [vp, _dp, _dp, _dp, _dp]),
+        "orc_get_planet": (None, [vp, _dp]),
+        "orc_get_rot": (None, [vp, _dp]),
+        "orc_global_dt": (ctypes.c_double, [vp, ctypes.c_double]),
+        "orc_solve": (None, [vp, ctypes.c_double, ctypes.c_double]),
+        "orc_run": (None, [vp, ctypes.c_int, _dp, _dp, _dp]),
+        "orc_viscosity": (None, [vp, ctypes.c_int, _dp]),
+        "orc_sources": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+        "orc_sweepx": (None, [vp, ctypes.c_double]),
+        "orc_sweepy": (None, [vp, ctypes.c_double, ctypes.c_double]),
+        "orc_sweepx_dust": (None, [vp, ctypes.c_double]),
+        "orc_sweepy_dust": (None, [vp, ctypes.c_double, ctypes.c_double]),
+        "orc_orbital_advection": (None, [vp, ctypes.c_double]),
+        "orc_get_orbadv": (None, [vp, _ip, _dp]),
+        "orc_killwave": (None, [vp, ctypes.c_double]),
+        "orc_write_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_double]),
+        "orc_evolve_planet": (None, [vp, ctypes.c_double]),
+        "orc_fourth_parameters": (None, [ctypes.c_int, _dp, _dp, _dp]),
+        "orc_ppm_adjust": (None, [ctypes.c_double, _dp]),
+        "orc_ppm_aveL": (ctypes.c_double, [ctypes.c_double, _dp]),
+        "orc_ppm_aveR": (ctypes.c_double, [ctypes.c_double, _dp]),
+        "orc_flatten": (ctypes.c_double, [ctypes.c_double, _dp, _dp, _dp]),
+        "orc_set_state_row": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                     _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                     ctypes.c_double, _dp]),
+        "orc_riemann_hllc": (None, [ctypes.c_double, ctypes.c_int, _dp, _dp]),
+        "orc_riemann_row": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                   _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]),
+        "orc_dust_flux": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 ctypes.c_int, ctypes.c_double, _dp]),
+        "orc_orbit_solution": (None, [pp, ctypes.c_double, _dp]),
+        "orc_average_begin": (None, [vp, ctypes.c_double]),
+        "orc_average_end": (None, [vp]),
+        "orc_average_get": (None, [vp, _dp]),
+        "orc_torque": (ctypes.c_double, [vp]),
+        "orc_set_tangent": (None, [vp, _dp, _dp]),
+        "orc_get_tangent": (None, [vp, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        if not hasattr(L, name):
+            continue   # the tangent build leaves out the unit-level entries, the FP64 one the tangent ones
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def lib():
@@ -35,57 +93,18 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        vp = ctypes.c_void_p
-        pp = ctypes.POINTER(Params)
-        sig = {
-            "orc_create": (vp, [pp]),
-            "orc_destroy": (None, [vp]),
-            "orc_num_threads": (ctypes.c_int, []),
-            "orc_get_state": (None, [vp, _dp, _dp]),
-            "orc_set_state": (None, [vp, _dp, _dp]),
-            "orc_get_coords": (None, [vp, _dp, _dp, _dp, _dp]),
-            "orc_get_planet": (None, [vp, _dp]),
-            "orc_get_rot": (None, [vp, _dp]),
-            "orc_global_dt": (ctypes.c_double, [vp, ctypes.c_double]),
-            "orc_solve": (None, [vp, ctypes.c_double, ctypes.c_double]),
-            "orc_run": (None, [vp, ctypes.c_int, _dp, _dp, _dp]),
-            "orc_viscosity": (None, [vp, ctypes.c_int, _dp]),
-            "orc_sources": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
-            "orc_sweepx": (None, [vp, ctypes.c_double]),
-            "orc_sweepy": (None, [vp, ctypes.c_double, ctypes.c_double]),
-            "orc_sweepx_dust": (None, [vp, ctypes.c_double]),
-            "orc_sweepy_dust": (None, [vp, ctypes.c_double, ctypes.c_double]),
-            "orc_orbital_advection": (None, [vp, ctypes.c_double]),
-            "orc_get_orbadv": (None, [vp, _ip, _dp]),
-            "orc_killwave": (None, [vp, ctypes.c_double]),
-            "orc_write_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_double]),
-            "orc_evolve_planet": (None, [vp, ctypes.c_double]),
-            "orc_fourth_parameters": (None, [ctypes.c_int, _dp, _dp, _dp]),
-            "orc_ppm_adjust": (None, [ctypes.c_double, _dp]),
-            "orc_ppm_aveL": (ctypes.c_double, [ctypes.c_double, _dp]),
-            "orc_ppm_aveR": (ctypes.c_double, [ctypes.c_double, _dp]),
-            "orc_flatten": (ctypes.c_double, [ctypes.c_double, _dp, _dp, _dp]),
-            "orc_set_state_row": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
-                                         _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
-                                         ctypes.c_double, _dp]),
-            "orc_riemann_hllc": (None, [ctypes.c_double, ctypes.c_int, _dp, _dp]),
-            "orc_riemann_row": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
-                                       _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]),
-            "orc_dust_flux": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
-                                     ctypes.c_int, ctypes.c_double, _dp]),
-            "orc_orbit_solution": (None, [pp, ctypes.c_double, _dp]),
-            "orc_average_begin": (None, [vp, ctypes.c_double]),
-            "orc_average_end": (None, [vp]),
-            "orc_average_get": (None, [vp, _dp]),
-            "orc_torque": (ctypes.c_double, [vp]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def tlm_lib():
+    """The tangent-linear build: the same oracle over forward-mode duals."""
+    global _tlm
+    if _tlm is None:
+        if not os.path.exists(TLM_PATH):
+            build()
+        _tlm = _bind(ctypes.CDLL(TLM_PATH))
+    return _tlm
 
 
 def ptr(a: np.ndarray):
@@ -100,16 +119,17 @@ def arr(x) -> np.ndarray:
 class Oracle:
     """One disk advanced by the FP64 host oracle."""
 
-    def __init__(self, p: Params):
+    def __init__(self, p: Params, _L=None):
+        self._L = lib() if _L is None else _L
         self.p = p.copy()
-        self._h = lib().orc_create(ctypes.byref(self.p))
+        self._h = self._L.orc_create(ctypes.byref(self.p))
         self.nx, self.ny = int(p.xres), int(p.yres)
         self.t = 0.0
         self.dt = float(p.end_time)   # main.cu:144: the first cap is end_time
 
     def close(self):
         if self._h:
-            lib().orc_destroy(self._h)
+            self._L.orc_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -122,98 +142,120 @@ class Oracle:
     def get_state(self):
         gas = np.zeros((5, self.ny, self.nx))
         dust = np.zeros((4, self.ny, self.nx))
-        lib().orc_get_state(self._h, ptr(gas), ptr(dust))
+        self._L.orc_get_state(self._h, ptr(gas), ptr(dust))
         return gas, dust
 
     def set_state(self, gas=None, dust=None):
         g = None if gas is None else arr(gas)
         d = None if dust is None else arr(dust)
-        lib().orc_set_state(self._h, ptr(g) if g is not None else None,
+        self._L.orc_set_state(self._h, ptr(g) if g is not None else None,
                             ptr(d) if d is not None else None)
 
     def coords(self):
         xa = np.zeros(self.nx + 7); xv = np.zeros(self.nx + 6)
         ya = np.zeros(self.ny + 7); yv = np.zeros(self.ny + 6)
-        lib().orc_get_coords(self._h, ptr(xa), ptr(xv), ptr(ya), ptr(yv))
+        self._L.orc_get_coords(self._h, ptr(xa), ptr(xv), ptr(ya), ptr(yv))
         return xa, xv, ya, yv
 
     def planet(self):
         b = np.zeros(10)
-        lib().orc_get_planet(self._h, ptr(b))
+        self._L.orc_get_planet(self._h, ptr(b))
         return b
 
     def rot(self):
         r = np.zeros(self.nx + 6)
-        lib().orc_get_rot(self._h, ptr(r))
+        self._L.orc_get_rot(self._h, ptr(r))
         return r
 
     def global_dt(self, dt_prev: float) -> float:
-        return lib().orc_global_dt(self._h, dt_prev)
+        return self._L.orc_global_dt(self._h, dt_prev)
 
     def solve(self, t: float, dt: float):
-        lib().orc_solve(self._h, t, dt)
+        self._L.orc_solve(self._h, t, dt)
 
     def run(self, nsteps: int) -> np.ndarray:
         dts = np.zeros(nsteps)
         t = ctypes.c_double(self.t)
         dt = ctypes.c_double(self.dt)
-        lib().orc_run(self._h, nsteps, ctypes.byref(t), ctypes.byref(dt), ptr(dts))
+        self._L.orc_run(self._h, nsteps, ctypes.byref(t), ctypes.byref(dt), ptr(dts))
         self.t, self.dt = t.value, dt.value
         return dts
 
     def viscosity(self, on_T: bool = False) -> np.ndarray:
         v = np.zeros((2, self.ny, self.nx))
-        lib().orc_viscosity(self._h, int(on_T), ptr(v))
+        self._L.orc_viscosity(self._h, int(on_T), ptr(v))
         return v
 
     def sources(self, mdt: float, dt: float, inplace: bool = True):
-        lib().orc_sources(self._h, mdt, dt, int(inplace))
+        self._L.orc_sources(self._h, mdt, dt, int(inplace))
 
     def sweepx(self, dt):
-        lib().orc_sweepx(self._h, dt)
+        self._L.orc_sweepx(self._h, dt)
 
     def sweepy(self, t, dt):
-        lib().orc_sweepy(self._h, t, dt)
+        self._L.orc_sweepy(self._h, t, dt)
 
     def sweepx_dust(self, dt):
-        lib().orc_sweepx_dust(self._h, dt)
+        self._L.orc_sweepx_dust(self._h, dt)
 
     def sweepy_dust(self, t, dt):
-        lib().orc_sweepy_dust(self._h, t, dt)
+        self._L.orc_sweepy_dust(self._h, t, dt)
 
     def orbital_advection(self, dt):
-        lib().orc_orbital_advection(self._h, dt)
+        self._L.orc_orbital_advection(self._h, dt)
 
     def orbadv(self):
         inc = np.zeros(self.nx, dtype=np.int32)
         res = np.zeros(self.nx)
-        lib().orc_get_orbadv(self._h, inc.ctypes.data_as(_ip), ptr(res))
+        self._L.orc_get_orbadv(self._h, inc.ctypes.data_as(_ip), ptr(res))
         return inc, res
 
     def killwave(self, dt):
-        lib().orc_killwave(self._h, dt)
+        self._L.orc_killwave(self._h, dt)
 
     def evolve_planet(self, t):
-        lib().orc_evolve_planet(self._h, t)
+        self._L.orc_evolve_planet(self._h, t)
 
     def write_snapshot(self, filename: str, t: float):
-        if lib().orc_write_snapshot(self._h, filename.encode(), t) != 0:
+        if self._L.orc_write_snapshot(self._h, filename.encode(), t) != 0:
             raise OSError(filename)
 
     def average_begin(self, interval: float):
-        lib().orc_average_begin(self._h, interval)
+        self._L.orc_average_begin(self._h, interval)
 
     def average_end(self):
-        lib().orc_average_end(self._h)
+        self._L.orc_average_end(self._h)
 
     def average(self) -> np.ndarray:
         g = np.zeros((5, self.ny, self.nx))
-        lib().orc_average_get(self._h, ptr(g))
+        self._L.orc_average_get(self._h, ptr(g))
         return g
 
     def torque(self) -> float:
-        return lib().orc_torque(self._h)
+        return self._L.orc_torque(self._h)
 
+
+
+class TangentOracle(Oracle):
+    """The oracle compiled over forward-mode dual numbers: its FP64
+    tangent-linear model.  set_tangent seeds the derivative parts of the
+    state with a direction d; solve(t, dt) with the recorded (frozen) step
+    sizes then carries J d, read back by get_tangent."""
+
+    def __init__(self, p: Params):
+        super().__init__(p, tlm_lib())
+
+    def set_tangent(self, gas=None, dust=None):
+        g = None if gas is None else arr(gas)
+        d = None if dust is None else arr(dust)
+        self._L.orc_set_tangent(self._h, ptr(g) if g is not None else None,
+                                ptr(d) if d is not None else None)
+
+    def get_tangent(self):
+        gas = np.zeros((5, self.ny, self.nx))
+        dust = np.zeros((4, self.ny, self.nx))
+        self._L.orc_get_tangent(self._h, ptr(gas), ptr(dust))
+        return gas, dust
 
 def num_threads() -> int:
     return lib().orc_num_threads()

@@ -38,6 +38,7 @@
 //   * killwave's __sinf (killwave.cu:11) is sinf on a float argument.
 
 #include "penguin_oracle.h"
+#include "oracle_real.h"
 
 #include <algorithm>
 #include <cmath>
@@ -52,58 +53,58 @@
 
 namespace {
 
-const double kPi = 3.1415926535897932384626434;
-const double kHalfPi = 0.5 * kPi;
-const double kTwoPi = 2.0 * kPi;
-const double kSmallR = 1.0e-10;
-const double kSmallP = 1.0e-10;
+const real_t kPi = 3.1415926535897932384626434;
+const real_t kHalfPi = 0.5 * kPi;
+const real_t kTwoPi = 2.0 * kPi;
+const real_t kSmallR = 1.0e-10;
+const real_t kSmallP = 1.0e-10;
 const int kPad = 3;  // npad for recon_flag > 4 (parameters.h:49-53)
 
-struct Cell { double r, p, u, v, w; };
-struct Dust { double r, u, v, w; };
-struct Body { double m, a, e, x, y, z, vx, vy, vz, rs; };
+struct Cell { real_t r, p, u, v, w; };
+struct Dust { real_t r, u, v, w; };
+struct Body { real_t m, a, e, x, y, z, vx, vy, vz, rs; };
 
 // ---------------------------------------------------------------- profiles
 // init.cu:7-50, 72-349 (init_flag 2, geomx 1, ndim 2), 351-454.
 
-double cs2_at(const pg_params& P, double x) {
-    double coeff = P.sc_h * P.sc_h * (1.0 / P.planet_radius);
-    return coeff * std::pow(x / P.planet_radius, -P.p_beta);
+real_t cs2_at(const pg_params& P, real_t x) {
+    real_t coeff = P.sc_h * P.sc_h * (1.0 / P.planet_radius);
+    return coeff * mpow(x / P.planet_radius, -P.p_beta);
 }
-double h_at(const pg_params& P, double x) { return std::sqrt(cs2_at(P, x) * std::pow(x, 3)); }
-double nu_at(const pg_params& P, double x) {
+real_t h_at(const pg_params& P, real_t x) { return msqrt(cs2_at(P, x) * mpow(x, 3)); }
+real_t nu_at(const pg_params& P, real_t x) {
     if (!P.visc_flag) return 0.0;
-    double rp = P.planet_radius;
-    double coeff = P.ss_alpha * std::sqrt(cs2_at(P, rp)) * h_at(P, rp);
-    return coeff * std::pow(x / rp, P.p_alpha);
+    real_t rp = P.planet_radius;
+    real_t coeff = P.ss_alpha * msqrt(cs2_at(P, rp)) * h_at(P, rp);
+    return coeff * mpow(x / rp, P.p_alpha);
 }
-double rho_init(const pg_params& P, double x) { return std::pow(x / P.planet_radius, -P.p_alpha); }
-double p_init(const pg_params& P, double x) { return cs2_at(P, x) * rho_init(P, x); }
-double u_init(const pg_params& P, double x) { return -1.5 * nu_at(P, x) / x; }
-double v_init(const pg_params& P, double x) {
-    double rho = rho_init(P, x);
-    double dr = 1.1514178e-7;
-    double dPdr = (p_init(P, x + dr) - p_init(P, x - dr)) / (2.0 * dr);
-    return std::sqrt(1.0 / x + (x / rho) * dPdr);
+real_t rho_init(const pg_params& P, real_t x) { return mpow(x / P.planet_radius, -P.p_alpha); }
+real_t p_init(const pg_params& P, real_t x) { return cs2_at(P, x) * rho_init(P, x); }
+real_t u_init(const pg_params& P, real_t x) { return -1.5 * nu_at(P, x) / x; }
+real_t v_init(const pg_params& P, real_t x) {
+    real_t rho = rho_init(P, x);
+    real_t dr = 1.1514178e-7;
+    real_t dPdr = (p_init(P, x + dr) - p_init(P, x - dr)) / (2.0 * dr);
+    return msqrt(1.0 / x + (x / rho) * dPdr);
 }
-Cell cell_init(const pg_params& P, double x) {
+Cell cell_init(const pg_params& P, real_t x) {
     return Cell{rho_init(P, x), p_init(P, x), u_init(P, x), v_init(P, x), 0.0};
 }
-Dust dust_init(const pg_params& P, double x) {
-    double St = P.Stokes;
-    double f_gas = v_init(P, x) / std::sqrt(1.0 / x);
-    double tmp = St * St * f_gas;
-    double L = ((1.0 + tmp) / tmp) *
-               (1.0 - std::sqrt(1.0 - 2.0 * tmp * (1.0 - f_gas) / (1.0 + tmp) / (1.0 + tmp)));
-    double u0 = St * (L * L - 2.0 * L);
-    double u = (1.0 - std::sqrt(1.0 - 2.0 * St * u0)) / St;
+Dust dust_init(const pg_params& P, real_t x) {
+    real_t St = P.Stokes;
+    real_t f_gas = v_init(P, x) / msqrt(1.0 / x);
+    real_t tmp = St * St * f_gas;
+    real_t L = ((1.0 + tmp) / tmp) *
+               (1.0 - msqrt(1.0 - 2.0 * tmp * (1.0 - f_gas) / (1.0 + tmp) / (1.0 + tmp)));
+    real_t u0 = St * (L * L - 2.0 * L);
+    real_t u = (1.0 - msqrt(1.0 - 2.0 * St * u0)) / St;
     L = (1.0 - f_gas + St * u / 2.0) / (1.0 + St * u / 2.0);
     u0 = St * (L * L - 2.0 * L);
-    u = (1.0 - std::sqrt(1.0 - 2.0 * St * u0)) / St;
+    u = (1.0 - msqrt(1.0 - 2.0 * St * u0)) / St;
     Dust d;
-    d.r = P.D_G_ratio * std::pow(x / P.planet_radius, -P.p_alpha);
-    d.u = u * std::sqrt(1.0 / x);
-    d.v = std::sqrt(1.0 / x);
+    d.r = P.D_G_ratio * mpow(x / P.planet_radius, -P.p_alpha);
+    d.u = u * msqrt(1.0 / x);
+    d.v = msqrt(1.0 / x);
     d.w = 0.0;
     return d;
 }
@@ -111,28 +112,28 @@ Dust dust_init(const pg_params& P, double x) {
 // ---------------------------------------------------------------- geometry
 // geom.cu:3-33, init.cu:472-500
 
-double dv_dr(int geom, double ra, double dr) {
-    if (geom == 5) return dr < 1.0e-4 ? std::sin(ra + 0.5 * dr) : (std::cos(ra) - std::cos(ra + dr)) / dr;
+real_t dv_dr(int geom, real_t ra, real_t dr) {
+    if (geom == 5) return dr < 1.0e-4 ? msin(ra + 0.5 * dr) : (mcos(ra) - mcos(ra + dr)) / dr;
     if (geom == 2) return ra * ra + ra * dr + dr * dr / 3.0;
     if (geom == 1) return ra + dr / 2.0;
     return 1.0;
 }
-double cell_volume(int geom, double ra, double dr) { return dr * dv_dr(geom, ra, dr); }
+real_t cell_volume(int geom, real_t ra, real_t dr) { return dr * dv_dr(geom, ra, dr); }
 
-void build_axis(std::vector<double>& a, std::vector<double>& v, double amin, double amax,
+void build_axis(std::vector<real_t>& a, std::vector<real_t>& v, real_t amin, real_t amax,
                 int res, int pad, int geom, int grid) {
     a.assign(res + 2 * pad + 1, 0.0);
     v.assign(res + 2 * pad, 0.0);
     if (grid == 1) {
-        double t = std::log(amax / amin);
-        for (int i = 0; i <= res; ++i) a[pad + i] = amin * std::exp(t * (double)i / (double)res);
+        real_t t = mlog(amax / amin);
+        for (int i = 0; i <= res; ++i) a[pad + i] = amin * mexp(t * (real_t)i / (real_t)res);
     } else {
-        for (int i = 0; i <= res; ++i) a[pad + i] = amin + (amax - amin) * (double)i / (double)res;
+        for (int i = 0; i <= res; ++i) a[pad + i] = amin + (amax - amin) * (real_t)i / (real_t)res;
     }
     for (int i = 0; i < pad; ++i) {
         if (grid == 1) {
-            a[i] = amin * std::exp(std::log(amax / amin) * (double)(i - pad) / (double)res);
-            a[res + pad + 1 + i] = amin * std::exp(std::log(amax / amin) * (double)(res + 1 + i) / (double)res);
+            a[i] = amin * mexp(mlog(amax / amin) * (real_t)(i - pad) / (real_t)res);
+            a[res + pad + 1 + i] = amin * mexp(mlog(amax / amin) * (real_t)(res + 1 + i) / (real_t)res);
         } else {
             a[i] = a[pad] - (a[2 * pad - i] - a[pad]);
             a[res + pad + 1 + i] = a[res + pad] + (a[res + pad] - a[res + pad - 1 - i]);
@@ -142,24 +143,24 @@ void build_axis(std::vector<double>& a, std::vector<double>& v, double amin, dou
 }
 
 // ---------------------------------------------------------- reconstruction
-inline double lim01(double a) { return std::fmin(std::fmax(a, 0.0), 1.0); }  // solve.cu:14-17
-inline double exp_lim(double x) { return std::fmax(1.0 + x, 1.0e-6); }       // solve.cu:29-34
+inline real_t lim01(real_t a) { return mfmin(mfmax(a, 0.0), 1.0); }  // solve.cu:14-17
+inline real_t exp_lim(real_t x) { return mfmax(1.0 + x, 1.0e-6); }       // solve.cu:29-34
 
 // Cell means of x, x^2, x^3 under the geometric weight (1, x, x^2 for
 // geom 0/3/4, 1, 2), recon.cu:242-301.
-inline void cell_moments(int geom, double xl, double xr, double m[3]) {
+inline void cell_moments(int geom, real_t xl, real_t xr, real_t m[3]) {
     if (geom == 2) {
-        double d = xr * xr * xr - xl * xl * xl;
+        real_t d = xr * xr * xr - xl * xl * xl;
         m[0] = (xr * xr * xr * xr - xl * xl * xl * xl) / d / (4.0 / 3.0);
         m[1] = (xr * xr * xr * xr * xr - xl * xl * xl * xl * xl) / d / (5.0 / 3.0);
         m[2] = (xr * xr * xr * xr * xr * xr - xl * xl * xl * xl * xl * xl) / d / 2.0;
     } else if (geom == 1) {
-        double d = xr * xr - xl * xl;
+        real_t d = xr * xr - xl * xl;
         m[0] = (xr * xr * xr - xl * xl * xl) / d / 1.5;
         m[1] = (xr * xr * xr * xr - xl * xl * xl * xl) / d / 2.0;
         m[2] = (xr * xr * xr * xr * xr - xl * xl * xl * xl * xl) / d / 2.5;
     } else {
-        double d = xr - xl;
+        real_t d = xr - xl;
         m[0] = (xr * xr - xl * xl) / d / 2.0;
         m[1] = (xr * xr * xr - xl * xl * xl) / d / 3.0;
         m[2] = (xr * xr * xr * xr - xl * xl * xl * xl) / d / 4.0;
@@ -168,62 +169,62 @@ inline void cell_moments(int geom, double xl, double xr, double m[3]) {
 
 // Coefficients c1..c3 of the cubic whose cell means over four consecutive
 // cells (moments M[0..3]) differ by d[0..2] (recon.cu:305-328).
-inline void cubic_from_means(const double M[4][3], const double d[3], double c[3]) {
-    double D0[3], D1[3], D2[3];
+inline void cubic_from_means(const real_t M[4][3], const real_t d[3], real_t c[3]) {
+    real_t D0[3], D1[3], D2[3];
     for (int k = 0; k < 3; ++k) {
         D0[k] = M[1][k] - M[0][k];
         D1[k] = M[2][k] - M[1][k];
         D2[k] = M[3][k] - M[2][k];
     }
     // D?2 -> index 0, D?3 -> index 1, D?4 -> index 2
-    double den_a = D2[1] * D0[0] - D0[1] * D2[0];
-    double den_b = D1[1] * D0[0] - D0[1] * D1[0];
-    double c3 = (D0[0] * d[2] - D2[0] * d[0]) / den_a - (d[1] * D0[0] - d[0] * D1[0]) / den_b;
-    double t = (D0[0] * D2[2] - D2[0] * D0[2]) / den_a - (D1[2] * D0[0] - D0[2] * D1[0]) / den_b;
+    real_t den_a = D2[1] * D0[0] - D0[1] * D2[0];
+    real_t den_b = D1[1] * D0[0] - D0[1] * D1[0];
+    real_t c3 = (D0[0] * d[2] - D2[0] * d[0]) / den_a - (d[1] * D0[0] - d[0] * D1[0]) / den_b;
+    real_t t = (D0[0] * D2[2] - D2[0] * D0[2]) / den_a - (D1[2] * D0[0] - D0[2] * D1[0]) / den_b;
     c3 /= t;
     t = (d[1] * D0[0] - d[0] * D1[0]) / den_b;
-    double c2 = t - c3 * (D1[2] * D0[0] - D0[2] * D1[0]) / den_b;
+    real_t c2 = t - c3 * (D1[2] * D0[0] - D0[2] * D1[0]) / den_b;
     t = d[0] / D0[0];
-    double c1 = t - c2 * D0[1] / D0[0] - c3 * D0[2] / D0[0];
+    real_t c1 = t - c2 * D0[1] / D0[0] - c3 * D0[2] / D0[0];
     c[0] = c1; c[1] = c2; c[2] = c3;
 }
 
 // get_4th_parameters (recon.cu:216-378): par = {sL, <a>, sR} of cell i, with
 // x[i-2..i+3] the edges and a[i-2..i+2] the cell means.
-void fourth_parameters(int i, int geom, const double* x, const double* a, double par[3]) {
-    double M[5][3];
+void fourth_parameters(int i, int geom, const real_t* x, const real_t* a, real_t par[3]) {
+    real_t M[5][3];
     for (int c = 0; c < 5; ++c) cell_moments(geom, x[i - 2 + c], x[i - 1 + c], M[c]);
-    double a0 = a[i - 2], a1 = a[i - 1], a2 = a[i], a3 = a[i + 1], a4 = a[i + 2];
-    double x2 = x[i], x3 = x[i + 1];
-    double c[3];
+    real_t a0 = a[i - 2], a1 = a[i - 1], a2 = a[i], a3 = a[i + 1], a4 = a[i + 2];
+    real_t x2 = x[i], x3 = x[i + 1];
+    real_t c[3];
     {
-        double Mw[4][3];
+        real_t Mw[4][3];
         for (int m = 0; m < 4; ++m) for (int k = 0; k < 3; ++k) Mw[m][k] = M[m][k];
-        double d[3] = {a1 - a0, a2 - a1, a3 - a2};
+        real_t d[3] = {a1 - a0, a2 - a1, a3 - a2};
         cubic_from_means(Mw, d, c);
     }
-    double sL = c[0] * (M[2][0] - x2) + c[1] * (M[2][1] - x2 * x2) + c[2] * (M[2][2] - x2 * x2 * x2);
+    real_t sL = c[0] * (M[2][0] - x2) + c[1] * (M[2][1] - x2 * x2) + c[2] * (M[2][2] - x2 * x2 * x2);
     {
-        double Mw[4][3];
+        real_t Mw[4][3];
         for (int m = 0; m < 4; ++m) for (int k = 0; k < 3; ++k) Mw[m][k] = M[m + 1][k];
-        double d[3] = {a2 - a1, a3 - a2, a4 - a3};
+        real_t d[3] = {a2 - a1, a3 - a2, a4 - a3};
         cubic_from_means(Mw, d, c);
     }
-    double sR = c[0] * (x3 - M[2][0]) + c[1] * (x3 * x3 - M[2][1]) + c[2] * (x3 * x3 * x3 - M[2][2]);
-    sL = std::copysign(std::fmin(std::fabs(sL), std::fabs(a2 - a1)), a2 - a1);
-    sR = std::copysign(std::fmin(std::fabs(sR), std::fabs(a3 - a2)), a3 - a2);
+    real_t sR = c[0] * (x3 - M[2][0]) + c[1] * (x3 * x3 - M[2][1]) + c[2] * (x3 * x3 * x3 - M[2][2]);
+    sL = mcopysign(mfmin(mfabs(sL), mfabs(a2 - a1)), a2 - a1);
+    sR = mcopysign(mfmin(mfabs(sR), mfabs(a3 - a2)), a3 - a2);
     if (sR * sL <= 0.0) { par[0] = 0.0; par[1] = a2; par[2] = 0.0; }
     else                { par[0] = sL;  par[1] = a2; par[2] = sR; }
 }
 
 // adjust_PPM_parameters (PPM.cu:64-91)
-void ppm_adjust(double flat, double par[3]) {
-    double sL = par[0], sR = par[2];
+void ppm_adjust(real_t flat, real_t par[3]) {
+    real_t sL = par[0], sR = par[2];
     sR *= (1.0 - flat);
     sL *= (1.0 - flat);
-    double t3 = sR + sL;
-    double t2 = t3 * t3;
-    double t1 = t3 * 3.0 * (sL - sR);
+    real_t t3 = sR + sL;
+    real_t t2 = t3 * t3;
+    real_t t1 = t3 * 3.0 * (sL - sR);
     if (t2 < t1) sL = 2.0 * sR;
     else if (t2 < -t1) sR = 2.0 * sL;
     par[0] = sL;
@@ -231,44 +232,44 @@ void ppm_adjust(double flat, double par[3]) {
 }
 
 // get_PPM_aveR / aveL (PPM.cu:95-113)
-inline double ppm_aveR(double x, const double* par) {
-    double a6 = 3.0 * (par[0] - par[2]);
-    double aR = par[1] + par[2];
-    double S = par[2] + par[0];
+inline real_t ppm_aveR(real_t x, const real_t* par) {
+    real_t a6 = 3.0 * (par[0] - par[2]);
+    real_t aR = par[1] + par[2];
+    real_t S = par[2] + par[0];
     x = 1.0 - x;
     return aR - (x / 2.0) * (S - (1.0 - x / 1.5) * a6);
 }
-inline double ppm_aveL(double x, const double* par) {
-    double a6 = 3.0 * (par[0] - par[2]);
-    double aL = par[1] - par[0];
-    double S = par[2] + par[0];
+inline real_t ppm_aveL(real_t x, const real_t* par) {
+    real_t a6 = 3.0 * (par[0] - par[2]);
+    real_t aL = par[1] - par[0];
+    real_t S = par[2] + par[0];
     return aL + (x / 2.0) * (S + (1.0 - x / 1.5) * a6);
 }
 
 // recon_flag 6: 4th-order parameters, then flattening + PPM monotonicity
-inline void recon_par(int i, int geom, const double* x, const double* a, const double* flat, double par[3]) {
+inline void recon_par(int i, int geom, const real_t* x, const real_t* a, const real_t* flat, real_t par[3]) {
     fourth_parameters(i, geom, x, a, par);
     ppm_adjust(flat[i], par);
 }
 
 // flatten (PPM.cu:7-62) for cell i of a row of n cells
-double flatten_cell(int i, int n, double gam, const double* r, const double* p, const double* u) {
-    double f = 0.0;
+real_t flatten_cell(int i, int n, real_t gam, const real_t* r, const real_t* p, const real_t* u) {
+    real_t f = 0.0;
     if (i >= 2 && i < n - 2) {
-        double pb = p[i - 1], p0 = p[i], pf = p[i + 1];
-        double cs = std::sqrt(gam * p0 / r[i]);
-        double dp1f = pf - p0;
-        double dp2f = p[i + 2] - pb;
-        double dp1b = p0 - pb;
-        double dp2b = pf - p[i - 2];
-        double Mf = (u[i] - u[i + 1]) / cs;
-        double Mb = (u[i - 1] - u[i]) / cs;
+        real_t pb = p[i - 1], p0 = p[i], pf = p[i + 1];
+        real_t cs = msqrt(gam * p0 / r[i]);
+        real_t dp1f = pf - p0;
+        real_t dp2f = p[i + 2] - pb;
+        real_t dp1b = p0 - pb;
+        real_t dp2b = pf - p[i - 2];
+        real_t Mf = (u[i] - u[i + 1]) / cs;
+        real_t Mb = (u[i - 1] - u[i]) / cs;
         if (Mf > 0.0 || Mb > 0.0) {
             if (dp2f != 0.0 && dp2b != 0.0) {
-                double ff = 10.0 * std::fmax(0.0, std::fabs(dp1f / dp2f - 0.5) - 0.25);
-                double fb = 10.0 * std::fmax(0.0, std::fabs(dp1b / dp2b - 0.5) - 0.25);
-                f = std::fmax(ff, fb);
-                f = std::fmin(0.5, f * std::fmax(std::fabs(dp1f), std::fabs(dp1b)) / p0);
+                real_t ff = 10.0 * mfmax(0.0, mfabs(dp1f / dp2f - 0.5) - 0.25);
+                real_t fb = 10.0 * mfmax(0.0, mfabs(dp1b / dp2b - 0.5) - 0.25);
+                f = mfmax(ff, fb);
+                f = mfmin(0.5, f * mfmax(mfabs(dp1f), mfabs(dp1b)) / p0);
             }
         }
     }
@@ -276,34 +277,34 @@ double flatten_cell(int i, int n, double gam, const double* r, const double* p, 
 }
 
 // ------------------------------------------------------------ Riemann step
-struct State { double rl, pl, ul, vl, wl, rr, pr, ur, vr, wr; };
+struct State { real_t rl, pl, ul, vl, wl, rr, pr, ur, vr, wr; };
 
 // set_state (states.cu:133-267) for interface i (between cells i-1 and i)
-void trace_state(int i, int geom, double gam, const double* xa, const double* r, const double* p,
-                 const double* u, const double* v, const double* w, const double* flat,
-                 double dt, double us, State& S) {
-    double rp[3], pp[3], up[3], vp[3], wp[3];
-    double hdt = 0.5 * dt;
+void trace_state(int i, int geom, real_t gam, const real_t* xa, const real_t* r, const real_t* p,
+                 const real_t* u, const real_t* v, const real_t* w, const real_t* flat,
+                 real_t dt, real_t us, State& S) {
+    real_t rp[3], pp[3], up[3], vp[3], wp[3];
+    real_t hdt = 0.5 * dt;
     // right state: cell i, swept region adjacent to its left edge
     recon_par(i, geom, xa, r, flat, rp);
     recon_par(i, geom, xa, p, flat, pp);
     recon_par(i, geom, xa, u, flat, up);
-    double xl = xa[i], xr = xa[i + 1];
-    double cs = std::sqrt(gam * p[i] / r[i]);
-    double tmp = xl - std::fmin(u[i], 0.0) * dt + cs * dt;
-    double q = lim01((tmp - xl) / (xr - xl));
-    double rmin = r[i] * 1.0e-10, pmin = p[i] * 1.0e-10;
-    double ur = ppm_aveL(q, up);
-    double dis = std::fmin(ur, 0.0) * dt;
+    real_t xl = xa[i], xr = xa[i + 1];
+    real_t cs = msqrt(gam * p[i] / r[i]);
+    real_t tmp = xl - mfmin(u[i], 0.0) * dt + cs * dt;
+    real_t q = lim01((tmp - xl) / (xr - xl));
+    real_t rmin = r[i] * 1.0e-10, pmin = p[i] * 1.0e-10;
+    real_t ur = ppm_aveL(q, up);
+    real_t dis = mfmin(ur, 0.0) * dt;
     tmp = xl - dis;
     q = lim01((tmp - xl) / (xr - xl));
-    double r0 = ppm_aveL(q, rp), p0 = ppm_aveL(q, pp), u0 = ppm_aveL(q, up);
-    r0 = std::fmax(r0, rmin);
-    p0 = std::fmax(p0, pmin);
-    cs = std::sqrt(gam * p0 / r0);
+    real_t r0 = ppm_aveL(q, rp), p0 = ppm_aveL(q, pp), u0 = ppm_aveL(q, up);
+    r0 = mfmax(r0, rmin);
+    p0 = mfmax(p0, pmin);
+    cs = msqrt(gam * p0 / r0);
     tmp = xl - dis + cs * dt;
     q = lim01((tmp - xl) / (xr - xl));
-    double u_ = ppm_aveL(q, up), p_ = ppm_aveL(q, pp);
+    real_t u_ = ppm_aveL(q, up), p_ = ppm_aveL(q, pp);
     u_ = 2.0 * (u_ - u0) / (cs * dt);
     p_ = 2.0 * (p_ - p0) / (cs * dt);
     u_ *= -hdt;
@@ -321,21 +322,21 @@ void trace_state(int i, int geom, double gam, const double* xa, const double* r,
     recon_par(i - 1, geom, xa, u, flat, up);
     xl = xa[i - 1];
     xr = xa[i];
-    cs = std::sqrt(gam * p[i - 1] / r[i - 1]);
-    tmp = xr - std::fmax(u[i - 1], 0.0) * dt - cs * dt;
+    cs = msqrt(gam * p[i - 1] / r[i - 1]);
+    tmp = xr - mfmax(u[i - 1], 0.0) * dt - cs * dt;
     q = lim01((tmp - xl) / (xr - xl));
     rmin = r[i - 1] * 1.0e-10;
     pmin = p[i - 1] * 1.0e-10;
-    double ul = ppm_aveR(q, up);
-    dis = std::fmax(ul, 0.0) * dt;
+    real_t ul = ppm_aveR(q, up);
+    dis = mfmax(ul, 0.0) * dt;
     tmp = xr - dis;
     q = lim01((tmp - xl) / (xr - xl));
     r0 = ppm_aveR(q, rp);
     p0 = ppm_aveR(q, pp);
     u0 = ppm_aveR(q, up);
-    r0 = std::fmax(r0, rmin);
-    p0 = std::fmax(p0, pmin);
-    cs = std::sqrt(gam * p0 / r0);
+    r0 = mfmax(r0, rmin);
+    p0 = mfmax(p0, pmin);
+    cs = msqrt(gam * p0 / r0);
     tmp = xr - dis - cs * dt;
     q = lim01((tmp - xl) / (xr - xl));
     u_ = ppm_aveR(q, up);
@@ -354,52 +355,52 @@ void trace_state(int i, int geom, double gam, const double* xa, const double* r,
 }
 
 // get_pm_simple, get_lr_speeds_Einfeldt, get_sm (wave_speeds.cu:24-35,288-315)
-void wave_speeds(const State& S, double gam, double us, double& pm, double& sl, double& sm, double& sr) {
-    double cl = std::sqrt(gam * S.pl / S.rl);
-    double cr = std::sqrt(gam * S.pr / S.rr);
-    double c_ = 0.5 * (cr + cl);
-    double p_ = 0.5 * (S.pr + S.pl);
+void wave_speeds(const State& S, real_t gam, real_t us, real_t& pm, real_t& sl, real_t& sm, real_t& sr) {
+    real_t cl = msqrt(gam * S.pl / S.rl);
+    real_t cr = msqrt(gam * S.pr / S.rr);
+    real_t c_ = 0.5 * (cr + cl);
+    real_t p_ = 0.5 * (S.pr + S.pl);
     pm = p_ * exp_lim(0.5 * gam * (S.ul - S.ur) / c_);
-    double srl = std::sqrt(S.rl), srr = std::sqrt(S.rr);
-    double ubar = (srl * S.ul + srr * S.ur) / (srl + srr);
-    double n2 = 0.5 * srl * srr / ((srl + srr) * (srl + srr));
-    double cbar = std::sqrt((cl * cl * srl + cr * cr * srr) / (srl + srr) + n2 * (S.ur - S.ul) * (S.ur - S.ul));
+    real_t srl = msqrt(S.rl), srr = msqrt(S.rr);
+    real_t ubar = (srl * S.ul + srr * S.ur) / (srl + srr);
+    real_t n2 = 0.5 * srl * srr / ((srl + srr) * (srl + srr));
+    real_t cbar = msqrt((cl * cl * srl + cr * cr * srr) / (srl + srr) + n2 * (S.ur - S.ul) * (S.ur - S.ul));
     sl = ubar - cbar;
     sr = ubar + cbar;
     sm = (S.rl * S.ul * cl + S.rr * S.ur * cr) / (S.rl * cl + S.rr * cr);
     sm += (S.pl - S.pr) / (S.rl + S.rr) / (0.5 * (cl + cr));
-    sm += us * (std::fmax(S.ul, 0.0) + std::fmax(-S.ur, 0.0)) / (cl + cr);
+    sm += us * (mfmax(S.ul, 0.0) + mfmax(-S.ur, 0.0)) / (cl + cr);
 }
 
 // upwind selection (HLLC.cu:2-8)
-inline double pick(double FL, double F_L, double F_R, double FR, double sl, double sm, double sr) {
-    if (!std::signbit(sl)) return FL;
-    if (std::signbit(sr)) return FR;
-    if (!std::signbit(sm)) return F_L;
+inline real_t pick(real_t FL, real_t F_L, real_t F_R, real_t FR, real_t sl, real_t sm, real_t sr) {
+    if (!msignbit(sl)) return FL;
+    if (msignbit(sr)) return FR;
+    if (!msignbit(sm)) return F_L;
     return F_R;
 }
 
-double energy(int eos, double gam, double r, double p, double u, double v, double w) {  // EOS.cu:4-18
+real_t energy(int eos, real_t gam, real_t r, real_t p, real_t u, real_t v, real_t w) {  // EOS.cu:4-18
     (void)u; (void)v; (void)w;
     if (eos == 2) return p / ((gam - 1.0) * r);  // internal_e_flag 1
-    if (eos == 1) return p / std::pow(r, gam);
+    if (eos == 1) return p / mpow(r, gam);
     return p / r;
 }
 
 // HLLC_fluxes (HLLC.cu:10-77), internal_e_flag 1 for EOS_flag 2
-void hllc(const State& S, int eos, double gam, double pm, double sl, double sm, double sr,
-          Cell& F, double& pres, double& uprs) {
-    double fl = S.ul * S.rl;
-    double f_l = sm * S.rl * (sl - S.ul) / (sl - sm);
-    double f_r = sm * S.rr * (sr - S.ur) / (sr - sm);
-    double fr = S.ur * S.rr;
+void hllc(const State& S, int eos, real_t gam, real_t pm, real_t sl, real_t sm, real_t sr,
+          Cell& F, real_t& pres, real_t& uprs) {
+    real_t fl = S.ul * S.rl;
+    real_t f_l = sm * S.rl * (sl - S.ul) / (sl - sm);
+    real_t f_r = sm * S.rr * (sr - S.ur) / (sr - sm);
+    real_t fr = S.ur * S.rr;
     F.r = pick(fl, f_l, f_r, fr, sl, sm, sr);
     F.u = pick(S.ul, S.ul, S.ur, S.ur, sl, sm, sr) * F.r;
     F.v = pick(S.vl, S.vl, S.vr, S.vr, sl, sm, sr) * F.r;
     F.w = pick(S.wl, S.wl, S.wr, S.wr, sl, sm, sr) * F.r;
     if (eos > 0) {
-        double el = energy(eos, gam, S.rl, S.pl, S.ul, 0.0, 0.0);
-        double er = energy(eos, gam, S.rr, S.pr, S.ur, 0.0, 0.0);
+        real_t el = energy(eos, gam, S.rl, S.pl, S.ul, 0.0, 0.0);
+        real_t er = energy(eos, gam, S.rr, S.pr, S.ur, 0.0, 0.0);
         F.p = pick(el, el, er, er, sl, sm, sr) * F.r;
     } else {
         F.p = 0.0;
@@ -413,15 +414,15 @@ void hllc(const State& S, int eos, double gam, double pm, double sl, double sm, 
     }
 }
 
-inline double geom_factor(int geom, double x) {  // eval_gfac (riemann.cu:5-11)
+inline real_t geom_factor(int geom, real_t x) {  // eval_gfac (riemann.cu:5-11)
     if (geom == 1) return x;
     if (geom == 2) return x * x;
-    if (geom == 5) return std::sin(x);
+    if (geom == 5) return msin(x);
     return 1.0;
 }
 
 struct RowScratch {
-    std::vector<double> r, p, u, v, w, flat, pres, uprs, gfac;
+    std::vector<real_t> r, p, u, v, w, flat, pres, uprs, gfac;
     std::vector<Cell> F, Del;
     std::vector<Dust> FD, DelD;
     void resize(int n) {
@@ -433,22 +434,22 @@ struct RowScratch {
 
 // riemann() + net_source() over a whole row of n cells (riemann.cu:40-131).
 // Del[i] is valid for cells i in [kPad, n-kPad).
-void riemann_row(int geom, int eos, double gam, int n, const double* xa, RowScratch& s,
-                 double dt_state) {
+void riemann_row(int geom, int eos, real_t gam, int n, const real_t* xa, RowScratch& s,
+                 real_t dt_state) {
     for (int i = 0; i < n; ++i) s.flat[i] = flatten_cell(i, n, gam, s.r.data(), s.p.data(), s.u.data());
     for (int i = kPad; i < n + 1 - kPad; ++i) {
         State S;
         trace_state(i, geom, gam, xa, s.r.data(), s.p.data(), s.u.data(), s.v.data(), s.w.data(),
                     s.flat.data(), dt_state, 0.0, S);
-        double pm, sl, sm, sr;
+        real_t pm, sl, sm, sr;
         wave_speeds(S, gam, 0.0, pm, sl, sm, sr);
         hllc(S, eos, gam, pm, sl, sm, sr, s.F[i], s.pres[i], s.uprs[i]);
         s.gfac[i] = geom_factor(geom, xa[i]);
     }
     for (int i = kPad; i < n - kPad; ++i) {
-        double g0 = s.gfac[i], g1 = s.gfac[i + 1];
-        double du = 0.5 * (g0 + g1) * (s.pres[i] - s.pres[i + 1]);
-        double de = 0.0;
+        real_t g0 = s.gfac[i], g1 = s.gfac[i + 1];
+        real_t du = 0.5 * (g0 + g1) * (s.pres[i] - s.pres[i + 1]);
+        real_t de = 0.0;
         if (eos == 2) de = 0.5 * (s.pres[i] + s.pres[i + 1]) * (s.uprs[i] * g0 - s.uprs[i + 1] * g1);
         Cell& D = s.Del[i];
         D.r = s.F[i].r * g0 - s.F[i + 1].r * g1;
@@ -461,21 +462,21 @@ void riemann_row(int geom, int eos, double gam, int n, const double* xa, RowScra
 
 // ------------------------------------------------------------------ dust
 // dust_flux (dust.cu:6-105), flattening coefficient 0
-Dust dust_interface(int i, int geom, const double* xa, const double* r, const double* u,
-                    const double* v, const double* w, const double* flat, double dt, double us) {
+Dust dust_interface(int i, int geom, const real_t* xa, const real_t* r, const real_t* u,
+                    const real_t* v, const real_t* w, const real_t* flat, real_t dt, real_t us) {
     State S;
-    double rp[3], upar[3];
-    double xl = xa[i], xr = xa[i + 1];
+    real_t rp[3], upar[3];
+    real_t xl = xa[i], xr = xa[i + 1];
     recon_par(i, geom, xa, u, flat, upar);
-    double tmp = xl - std::fmin(upar[1] - upar[0], 0.0) * dt;
-    double q = lim01((tmp - xl) / (xr - xl));
-    double ur = ppm_aveL(q, upar);
-    double u_ = 0.5 * dt * (ur - u[i]) / (0.5 * (xr - tmp));
-    tmp = xl - std::fmin(ur, 0.0) * dt;
+    real_t tmp = xl - mfmin(upar[1] - upar[0], 0.0) * dt;
+    real_t q = lim01((tmp - xl) / (xr - xl));
+    real_t ur = ppm_aveL(q, upar);
+    real_t u_ = 0.5 * dt * (ur - u[i]) / (0.5 * (xr - tmp));
+    tmp = xl - mfmin(ur, 0.0) * dt;
     q = lim01((tmp - xl) / (xr - xl));
     recon_par(i, geom, xa, r, flat, rp);
-    double r0 = ppm_aveL(q, rp);
-    double u0 = ppm_aveL(q, upar);
+    real_t r0 = ppm_aveL(q, rp);
+    real_t u0 = ppm_aveL(q, upar);
     S.rr = r0 * exp_lim(u_);
     S.ur = u0 + us;
     recon_par(i, geom, xa, v, flat, rp);
@@ -486,11 +487,11 @@ Dust dust_interface(int i, int geom, const double* xa, const double* r, const do
     xl = xa[i - 1];
     xr = xa[i];
     recon_par(i - 1, geom, xa, u, flat, upar);
-    tmp = xr - std::fmax(upar[1] + upar[2], 0.0) * dt;
+    tmp = xr - mfmax(upar[1] + upar[2], 0.0) * dt;
     q = lim01((tmp - xl) / (xr - xl));
-    double ul = ppm_aveR(q, upar);
+    real_t ul = ppm_aveR(q, upar);
     u_ = 0.5 * dt * (u[i - 1] - ul) / (0.5 * (tmp - xl));
-    tmp = xr - std::fmax(ul, 0.0) * dt;
+    tmp = xr - mfmax(ul, 0.0) * dt;
     q = lim01((tmp - xl) / (xr - xl));
     recon_par(i - 1, geom, xa, r, flat, rp);
     r0 = ppm_aveR(q, rp);
@@ -502,8 +503,8 @@ Dust dust_interface(int i, int geom, const double* xa, const double* r, const do
     S.vl = ppm_aveR(q, rp);
     S.wl = ppm_aveR(q, upar);
 
-    S.rl = std::fmin(r[i - 1] * dt / (xa[i] - xa[i - 1]), std::fmax(S.ul * S.rl, 0.0));
-    S.rr = std::fmax(r[i] * dt / (xa[i] - xa[i + 1]), std::fmin(S.ur * S.rr, 0.0));
+    S.rl = mfmin(r[i - 1] * dt / (xa[i] - xa[i - 1]), mfmax(S.ul * S.rl, 0.0));
+    S.rr = mfmax(r[i] * dt / (xa[i] - xa[i + 1]), mfmin(S.ur * S.rr, 0.0));
     Dust F;
     F.r = S.rl + S.rr;
     F.u = S.rl * S.ul + S.rr * S.ur;
@@ -513,7 +514,7 @@ Dust dust_interface(int i, int geom, const double* xa, const double* r, const do
 }
 
 // dust_flow (dust.cu:107-147) over a row; rows hold r,u,v,w in s.r,s.u,s.v,s.w
-void dust_row(int geom, int n, const double* xa, RowScratch& s, double dt_flux) {
+void dust_row(int geom, int n, const real_t* xa, RowScratch& s, real_t dt_flux) {
     std::fill(s.flat.begin(), s.flat.begin() + n, 0.0);
     for (int i = kPad; i < n + 1 - kPad; ++i) {
         s.FD[i] = dust_interface(i, geom, xa, s.r.data(), s.u.data(), s.v.data(), s.w.data(),
@@ -521,7 +522,7 @@ void dust_row(int geom, int n, const double* xa, RowScratch& s, double dt_flux) 
         s.gfac[i] = geom_factor(geom, xa[i]);
     }
     for (int i = kPad; i < n - kPad; ++i) {
-        double g0 = s.gfac[i], g1 = s.gfac[i + 1];
+        real_t g0 = s.gfac[i], g1 = s.gfac[i + 1];
         Dust& D = s.DelD[i];
         D.r = s.FD[i].r * g0 - s.FD[i + 1].r * g1;
         D.u = s.FD[i].u * g0 - s.FD[i + 1].u * g1;
@@ -531,10 +532,10 @@ void dust_row(int geom, int n, const double* xa, RowScratch& s, double dt_flux) 
 }
 
 // ---------------------------------------------------------- cell updates
-Cell update_cell(const pg_params& P, double x, Cell Q, const Cell& D) {  // sweeps.cu:218-261
-    double r_min = std::fmax(Q.r * 1.0e-10, kSmallR);
-    double p_min = std::fmax(Q.p * 1.0e-10, kSmallP);
-    double uo = Q.u, vo = Q.v, wo = Q.w;
+Cell update_cell(const pg_params& P, real_t x, Cell Q, const Cell& D) {  // sweeps.cu:218-261
+    real_t r_min = mfmax(Q.r * 1.0e-10, kSmallR);
+    real_t p_min = mfmax(Q.p * 1.0e-10, kSmallP);
+    real_t uo = Q.u, vo = Q.v, wo = Q.w;
     Q.p = energy(P.EOS_flag, P.gam, Q.r, Q.p, Q.u, Q.v, Q.w);
     Q.p *= Q.r;
     Q.u *= Q.r; Q.v *= Q.r; Q.w *= Q.r;
@@ -543,14 +544,14 @@ Cell update_cell(const pg_params& P, double x, Cell Q, const Cell& D) {  // swee
     if (Q.r < r_min) {
         Q.r = r_min; Q.p = p_min; Q.u = uo; Q.v = vo; Q.w = wo;
     } else if (P.EOS_flag == 2) {
-        Q.p = std::fmax(Q.p * (P.gam - 1.0), p_min);
+        Q.p = mfmax(Q.p * (P.gam - 1.0), p_min);
     } else if (P.EOS_flag == 0) {
         Q.p = cs2_at(P, x) * Q.r;
     }
     return Q;
 }
 
-Cell update_cell_advect(const pg_params& P, double x, Cell Q, const Cell& D) {  // advection.cu:88-105
+Cell update_cell_advect(const pg_params& P, real_t x, Cell Q, const Cell& D) {  // advection.cu:88-105
     Q.u *= Q.r; Q.v *= Q.r; Q.w *= Q.r;
     Q.r += D.r; Q.p += D.p; Q.u += D.u; Q.v += D.v; Q.w += D.w;
     Q.u /= Q.r; Q.v /= Q.r; Q.w /= Q.r;
@@ -559,8 +560,8 @@ Cell update_cell_advect(const pg_params& P, double x, Cell Q, const Cell& D) {  
 }
 
 Dust update_dust(const pg_params& P, Dust Q, const Dust& D) {  // dust.cu:149-176
-    double r_min = std::fmax(Q.r * 1.0e-10, P.D_G_ratio * kSmallR);
-    double uo = Q.u, vo = Q.v, wo = Q.w;
+    real_t r_min = mfmax(Q.r * 1.0e-10, P.D_G_ratio * kSmallR);
+    real_t uo = Q.u, vo = Q.v, wo = Q.w;
     Q.u *= Q.r; Q.v *= Q.r; Q.w *= Q.r;
     Q.r += D.r; Q.u += D.u; Q.v += D.v; Q.w += D.w;
     Q.u /= Q.r; Q.v /= Q.r; Q.w /= Q.r;
@@ -569,39 +570,39 @@ Dust update_dust(const pg_params& P, Dust Q, const Dust& D) {  // dust.cu:149-17
 }
 
 // ----------------------------------------------------------------- planet
-double ramp(double t, double tau, double val) { return t < tau ? val * std::sin(kHalfPi * t / tau) : val; }
+real_t ramp(real_t t, real_t tau, real_t val) { return t < tau ? val * msin(kHalfPi * t / tau) : val; }
 
-double ecc_anomaly(double n, double e, double t) {  // planet.cu:60-103
-    double l = std::fmod(n * t, kTwoPi);
-    double D = -e * std::sin(l);
+real_t ecc_anomaly(real_t n, real_t e, real_t t) {  // planet.cu:60-103
+    real_t l = mfmod(n * t, kTwoPi);
+    real_t D = -e * msin(l);
     if (e == 0.0 || l == 0.0 || l == kPi || l == kTwoPi) return l;
-    double tor = 1.0e-12, step;
+    real_t tor = 1.0e-12, step;
     int count = 0;
     do {
-        double es = std::sin(l - D), ec = std::cos(l - D);
+        real_t es = msin(l - D), ec = mcos(l - D);
         es *= e; ec *= e;
-        double x = D + es, dx = 1.0 - ec, d2x = -es;
-        double eps = 2.0 * x * d2x / dx / dx;
-        if (std::fabs(eps) < 1.0) step = -(x / dx) * (1.0 + eps / 4.0 + eps * eps / 8.0 + 5.0 * eps * eps * eps / 64.0);
-        else step = std::copysign(std::fmin(std::fabs(x / dx), kHalfPi / 2.0), -x);
+        real_t x = D + es, dx = 1.0 - ec, d2x = -es;
+        real_t eps = 2.0 * x * d2x / dx / dx;
+        if (mfabs(eps) < 1.0) step = -(x / dx) * (1.0 + eps / 4.0 + eps * eps / 8.0 + 5.0 * eps * eps * eps / 64.0);
+        else step = mcopysign(mfmin(mfabs(x / dx), kHalfPi / 2.0), -x);
         D += step;
         count++;
-    } while (std::fabs(step) > tor || count < 10);
+    } while (mfabs(step) > tor || count < 10);
     return l - D;
 }
 
-void orbit_solution(const pg_params& P, Body& b, double t) {  // planet.cu:105-133
-    double a = b.a, e = b.e;
-    double n = std::sqrt(1.0 / a / a / a);
-    double efac = std::sqrt(1.0 - e * e);
-    double B = e / (1.0 + efac);
-    double u = ecc_anomaly(n, e, t);
-    double sinu = std::sin(u), cosu = std::cos(u);
-    double f = u + 2.0 * std::atan(B * sinu / (1.0 - B * cosu));
-    double esinf = std::sin(f) * e, ecosf = std::cos(f) * e;
+void orbit_solution(const pg_params& P, Body& b, real_t t) {  // planet.cu:105-133
+    real_t a = b.a, e = b.e;
+    real_t n = msqrt(1.0 / a / a / a);
+    real_t efac = msqrt(1.0 - e * e);
+    real_t B = e / (1.0 + efac);
+    real_t u = ecc_anomaly(n, e, t);
+    real_t sinu = msin(u), cosu = mcos(u);
+    real_t f = u + 2.0 * matan(B * sinu / (1.0 - B * cosu));
+    real_t esinf = msin(f) * e, ecosf = mcos(f) * e;
     b.m = ramp(t, P.ramp_time, P.planet_mass);
     b.x = a * efac * efac / (1.0 + ecosf);
-    b.y = f - std::fmod(P.frame_omega * t, kTwoPi) + kPi;
+    b.y = f - mfmod(P.frame_omega * t, kTwoPi) + kPi;
     b.z = kHalfPi;
     b.vx = n * a * esinf / efac;
     b.vy = n * (1.0 + ecosf) * (1.0 + ecosf) / efac / efac / efac;
@@ -609,10 +610,10 @@ void orbit_solution(const pg_params& P, Body& b, double t) {  // planet.cu:105-1
 }
 
 // star_planet_grav_cyl (forces.cu:41-58), evaluated with dt = 0
-void planet_force_cyl(double rad, double azi, double z, const Body& b, double& fx, double& fy, double& fz) {
-    double s = std::sin(azi - (b.y + 0.0 * b.vy)), c = std::cos(azi - (b.y + 0.0 * b.vy));
-    double Rp = std::sqrt(rad * rad + b.x * b.x - 2.0 * b.x * rad * c + z * z + b.rs * b.rs);
-    double k = 1.0 / Rp / Rp / Rp;
+void planet_force_cyl(real_t rad, real_t azi, real_t z, const Body& b, real_t& fx, real_t& fy, real_t& fz) {
+    real_t s = msin(azi - (b.y + 0.0 * b.vy)), c = mcos(azi - (b.y + 0.0 * b.vy));
+    real_t Rp = msqrt(rad * rad + b.x * b.x - 2.0 * b.x * rad * c + z * z + b.rs * b.rs);
+    real_t k = 1.0 / Rp / Rp / Rp;
     fx = -b.m * (rad - b.x * c) * k - b.m * c / b.x / b.x;
     fy = -b.m * b.x * s * k + b.m * s / b.x / b.x;
     fz = -b.m * z * k;
@@ -624,18 +625,18 @@ void planet_force_cyl(double rad, double azi, double z, const Body& b, double& f
 struct orc_grid {
     pg_params P;
     int xres, yres, xpad, ypad, xarr, yarr;
-    std::vector<double> xa, xv, ya, yv, za, zv;
+    std::vector<real_t> xa, xv, ya, yv, za, zv;
     std::vector<Cell> C, T;
     std::vector<Dust> CD, TD;
-    std::vector<double> vis;
-    std::vector<double> rot, res;
+    std::vector<real_t> vis;
+    std::vector<real_t> rot, res;
     std::vector<int> inc;
     Body planet;
     std::vector<Cell> A;        // time average of the gas (Grid::A)
-    double ave_interval = 0.0;  // 0: not averaging
+    real_t ave_interval = 0.0;  // 0: not averaging
     size_t ind(int i, int j) const { return (size_t)i + (size_t)xarr * (size_t)j; }
-    double xc(int i) const { return 0.5 * (xa[i + 1] + xa[i]); }
-    double yc(int j) const { return 0.5 * (ya[j + 1] + ya[j]); }
+    real_t xc(int i) const { return 0.5 * (xa[i + 1] + xa[i]); }
+    real_t yc(int j) const { return 0.5 * (ya[j + 1] + ya[j]); }
 };
 
 namespace {
@@ -647,9 +648,9 @@ uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 // uniform in [0,1) keyed by (seed, field, interior i, interior j)
-double keyed_uniform(uint64_t seed, int field, int i, int j) {
+real_t keyed_uniform(uint64_t seed, int field, int i, int j) {
     uint64_t key = splitmix64(seed) ^ splitmix64(((uint64_t)field << 48) ^ ((uint64_t)j << 24) ^ (uint64_t)i);
-    return (double)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+    return (real_t)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
 }
 
 // boundx / boundx_dust / boundx2 for one device with types 0 or 1
@@ -689,7 +690,7 @@ void fill_y_ghosts(orc_grid& g, std::vector<T>& X) {
 }
 
 // ------------------------------------------------------------------ sweeps
-void sweep_x_gas(orc_grid& g, double dt) {  // sweeps.cu:263-358
+void sweep_x_gas(orc_grid& g, real_t dt) {  // sweeps.cu:263-358
     const pg_params& P = g.P;
     const int n = g.xarr;
 #pragma omp parallel
@@ -700,15 +701,15 @@ void sweep_x_gas(orc_grid& g, double dt) {  // sweeps.cu:263-358
         for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
             for (int i = 0; i < n; ++i) {
                 const Cell& c = g.C[g.ind(i, j)];
-                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
                 s.r[i] = c.r; s.p[i] = c.p; s.u[i] = c.u; s.v[i] = c.v * rad; s.w[i] = c.w;
             }
             riemann_row(P.geomx, P.EOS_flag, P.gam, n, g.xa.data(), s, dt / 1.0);
             for (int i = g.xpad; i < n - g.xpad; ++i) {
                 Cell D = s.Del[i];
-                double f = dt / g.xv[i];
+                real_t f = dt / g.xv[i];
                 D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
-                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
                 D.v /= rad;
                 g.T[g.ind(i, j)] = update_cell(P, g.xc(i), g.C[g.ind(i, j)], D);
             }
@@ -717,7 +718,7 @@ void sweep_x_gas(orc_grid& g, double dt) {  // sweeps.cu:263-358
     std::swap(g.C, g.T);
 }
 
-void sweep_y_gas(orc_grid& g, double dt) {  // sweeps.cu:361-456
+void sweep_y_gas(orc_grid& g, real_t dt) {  // sweeps.cu:361-456
     const pg_params& P = g.P;
     const int n = g.yarr;
 #pragma omp parallel
@@ -726,8 +727,8 @@ void sweep_y_gas(orc_grid& g, double dt) {  // sweeps.cu:361-456
         s.resize(n);
 #pragma omp for schedule(static)
         for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            double R = g.xc(i);
-            double rot = g.rot[i];
+            real_t R = g.xc(i);
+            real_t rot = g.rot[i];
             for (int j = 0; j < n; ++j) {
                 const Cell& c = g.C[g.ind(i, j)];
                 s.r[j] = c.r; s.p[j] = c.p;
@@ -738,10 +739,10 @@ void sweep_y_gas(orc_grid& g, double dt) {  // sweeps.cu:361-456
             riemann_row(P.geomy, P.EOS_flag, P.gam, n, g.ya.data(), s, dt / R);
             for (int j = g.ypad; j < n - g.ypad; ++j) {
                 Cell D = s.Del[j];
-                double f = dt / g.yv[j] / R;
+                real_t f = dt / g.yv[j] / R;
                 D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
                 D.u += (rot + R * P.frame_omega) * D.r;
-                double t = D.v;
+                real_t t = D.v;
                 D.v = D.u; D.u = D.w; D.w = t;
                 g.T[g.ind(i, j)] = update_cell(P, g.xc(i), g.C[g.ind(i, j)], D);
             }
@@ -750,7 +751,7 @@ void sweep_y_gas(orc_grid& g, double dt) {  // sweeps.cu:361-456
     std::swap(g.C, g.T);
 }
 
-void sweep_x_dust(orc_grid& g, double dt) {  // dust.cu:178-257
+void sweep_x_dust(orc_grid& g, real_t dt) {  // dust.cu:178-257
     const pg_params& P = g.P;
     const int n = g.xarr;
 #pragma omp parallel
@@ -761,15 +762,15 @@ void sweep_x_dust(orc_grid& g, double dt) {  // dust.cu:178-257
         for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
             for (int i = 0; i < n; ++i) {
                 const Dust& c = g.CD[g.ind(i, j)];
-                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
                 s.r[i] = c.r; s.u[i] = c.u; s.v[i] = c.v * rad; s.w[i] = c.w;
             }
             dust_row(P.geomx, n, g.xa.data(), s, dt);
             for (int i = g.xpad; i < n - g.xpad; ++i) {
                 Dust D = s.DelD[i];
-                double f = dt / g.xv[i];
+                real_t f = dt / g.xv[i];
                 D.r *= f; D.u *= f; D.v *= f; D.w *= f;
-                double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+                real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
                 D.v /= rad;
                 g.TD[g.ind(i, j)] = update_dust(P, g.CD[g.ind(i, j)], D);
             }
@@ -778,7 +779,7 @@ void sweep_x_dust(orc_grid& g, double dt) {  // dust.cu:178-257
     std::swap(g.CD, g.TD);
 }
 
-void sweep_y_dust(orc_grid& g, double dt) {  // dust.cu:260-353
+void sweep_y_dust(orc_grid& g, real_t dt) {  // dust.cu:260-353
     const pg_params& P = g.P;
     const int n = g.yarr;
 #pragma omp parallel
@@ -787,8 +788,8 @@ void sweep_y_dust(orc_grid& g, double dt) {  // dust.cu:260-353
         s.resize(n);
 #pragma omp for schedule(static)
         for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            double R = g.xc(i);
-            double rot = g.rot[i];
+            real_t R = g.xc(i);
+            real_t rot = g.rot[i];
             for (int j = 0; j < n; ++j) {
                 const Dust& c = g.CD[g.ind(i, j)];
                 s.r[j] = c.r;
@@ -799,12 +800,12 @@ void sweep_y_dust(orc_grid& g, double dt) {  // dust.cu:260-353
             dust_row(P.geomy, n, g.ya.data(), s, dt / R);
             for (int j = g.ypad; j < n - g.ypad; ++j) {
                 Dust D = s.DelD[j];
-                double f = dt / g.yv[j];
+                real_t f = dt / g.yv[j];
                 D.r *= f; D.u *= f; D.v *= f; D.w *= f;
-                double f2 = 1.0 / R;
+                real_t f2 = 1.0 / R;
                 D.r *= f2; D.u *= f2; D.v *= f2; D.w *= f2;
                 D.u += (rot + R * P.frame_omega) * D.r;
-                double t = D.v;
+                real_t t = D.v;
                 D.v = D.u; D.u = D.w; D.w = t;
                 g.TD[g.ind(i, j)] = update_dust(P, g.CD[g.ind(i, j)], D);
             }
@@ -814,14 +815,14 @@ void sweep_y_dust(orc_grid& g, double dt) {  // dust.cu:260-353
 }
 
 // --------------------------------------------------------- orbital advection
-void set_orbital_advection(orc_grid& g, double dt) {  // orbit.cu:74-99
+void set_orbital_advection(orc_grid& g, real_t dt) {  // orbit.cu:74-99
     const pg_params& P = g.P;
-    double dy0 = (P.ymax - P.ymin) / (double)P.yres;
+    real_t dy0 = (P.ymax - P.ymin) / (real_t)P.yres;
     for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-        double dy = dy0 * g.xc(i);
-        double rot = g.rot[i];
-        int shift = (int)std::floor(rot * dt / dy + 0.5);
-        double speed = shift * dy / dt;
+        real_t dy = dy0 * g.xc(i);
+        real_t rot = g.rot[i];
+        int shift = (int)val(mfloor(rot * dt / dy + 0.5));
+        real_t speed = shift * dy / dt;
         g.res[i] = rot - speed;
         g.inc[i] = shift;
     }
@@ -844,25 +845,25 @@ void shift_orbital_advection(orc_grid& g) {  // orbit.cu:17-49, 144-157
 }
 
 // advection() row phases (advection.cu:1-86 gas, 175-252 dust), flat = 0
-void advect_row(int geom, int n, const double* xa, RowScratch& s, double speed, double dt_eff,
+void advect_row(int geom, int n, const real_t* xa, RowScratch& s, real_t speed, real_t dt_eff,
                 bool gas, std::vector<Cell>& flux) {
     std::fill(s.flat.begin(), s.flat.begin() + n, 0.0);
-    double par[3];
+    real_t par[3];
     for (int i = kPad; i < n + 1 - kPad; ++i) {
         Cell D{0.0, 0.0, 0.0, 0.0, 0.0};
         if (speed > 0.0) {
-            double xl = xa[i - 1], xr = xa[i];
-            double x0 = xr - speed * dt_eff;
-            double q = lim01((x0 - xl) / (xr - xl));
+            real_t xl = xa[i - 1], xr = xa[i];
+            real_t x0 = xr - speed * dt_eff;
+            real_t q = lim01((x0 - xl) / (xr - xl));
             recon_par(i - 1, geom, xa, s.r.data(), s.flat.data(), par); D.r = ppm_aveR(q, par);
             if (gas) { recon_par(i - 1, geom, xa, s.p.data(), s.flat.data(), par); D.p = ppm_aveR(q, par); }
             recon_par(i - 1, geom, xa, s.u.data(), s.flat.data(), par); D.u = ppm_aveR(q, par) * D.r;
             recon_par(i - 1, geom, xa, s.v.data(), s.flat.data(), par); D.v = ppm_aveR(q, par) * D.r;
             recon_par(i - 1, geom, xa, s.w.data(), s.flat.data(), par); D.w = ppm_aveR(q, par) * D.r;
         } else {
-            double xl = xa[i], xr = xa[i + 1];
-            double x0 = xl - speed * dt_eff;
-            double q = lim01((x0 - xl) / (xr - xl));
+            real_t xl = xa[i], xr = xa[i + 1];
+            real_t x0 = xl - speed * dt_eff;
+            real_t q = lim01((x0 - xl) / (xr - xl));
             recon_par(i, geom, xa, s.r.data(), s.flat.data(), par); D.r = ppm_aveL(q, par);
             if (gas) { recon_par(i, geom, xa, s.p.data(), s.flat.data(), par); D.p = ppm_aveL(q, par); }
             recon_par(i, geom, xa, s.u.data(), s.flat.data(), par); D.u = ppm_aveL(q, par) * D.r;
@@ -874,7 +875,7 @@ void advect_row(int geom, int n, const double* xa, RowScratch& s, double speed, 
         s.gfac[i] = geom_factor(geom, xa[i]);
     }
     for (int i = kPad; i < n - kPad; ++i) {
-        double g0 = s.gfac[i], g1 = s.gfac[i + 1];
+        real_t g0 = s.gfac[i], g1 = s.gfac[i + 1];
         Cell& D = s.Del[i];
         D.r = flux[i].r * g0 - flux[i + 1].r * g1;
         D.p = flux[i].p * g0 - flux[i + 1].p * g1;
@@ -884,7 +885,7 @@ void advect_row(int geom, int n, const double* xa, RowScratch& s, double speed, 
     }
 }
 
-void advect_y_gas(orc_grid& g, double dt) {  // advection.cu:107-173, 321-338
+void advect_y_gas(orc_grid& g, real_t dt) {  // advection.cu:107-173, 321-338
     const pg_params& P = g.P;
     fill_y_ghosts(g, g.C);
     const int n = g.yarr;
@@ -895,7 +896,7 @@ void advect_y_gas(orc_grid& g, double dt) {  // advection.cu:107-173, 321-338
         std::vector<Cell> flux(n + 1);
 #pragma omp for schedule(static)
         for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            double R = g.xc(i);
+            real_t R = g.xc(i);
             for (int j = 0; j < n; ++j) {
                 const Cell& c = g.C[g.ind(i, j)];
                 s.r[j] = c.r; s.p[j] = c.p; s.u[j] = c.v; s.v[j] = c.w; s.w[j] = c.u;
@@ -903,11 +904,11 @@ void advect_y_gas(orc_grid& g, double dt) {  // advection.cu:107-173, 321-338
             advect_row(P.geomy, n, g.ya.data(), s, g.res[i], dt / R, true, flux);
             for (int j = g.ypad; j < n - g.ypad; ++j) {
                 Cell D = s.Del[j];
-                double f = dt / g.yv[j];
+                real_t f = dt / g.yv[j];
                 D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
-                double f2 = 1.0 / R;
+                real_t f2 = 1.0 / R;
                 D.r *= f2; D.p *= f2; D.u *= f2; D.v *= f2; D.w *= f2;
-                double t = D.v;
+                real_t t = D.v;
                 D.v = D.u; D.u = D.w; D.w = t;
                 g.T[g.ind(i, j)] = update_cell_advect(P, g.xc(i), g.C[g.ind(i, j)], D);
             }
@@ -916,7 +917,7 @@ void advect_y_gas(orc_grid& g, double dt) {  // advection.cu:107-173, 321-338
     std::swap(g.C, g.T);
 }
 
-void advect_y_dust(orc_grid& g, double dt) {  // advection.cu:254-319, 340-357
+void advect_y_dust(orc_grid& g, real_t dt) {  // advection.cu:254-319, 340-357
     const pg_params& P = g.P;
     fill_y_ghosts(g, g.CD);
     const int n = g.yarr;
@@ -927,7 +928,7 @@ void advect_y_dust(orc_grid& g, double dt) {  // advection.cu:254-319, 340-357
         std::vector<Cell> flux(n + 1);
 #pragma omp for schedule(static)
         for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            double R = g.xc(i);
+            real_t R = g.xc(i);
             for (int j = 0; j < n; ++j) {
                 const Dust& c = g.CD[g.ind(i, j)];
                 s.r[j] = c.r; s.u[j] = c.v; s.v[j] = c.w; s.w[j] = c.u;
@@ -935,11 +936,11 @@ void advect_y_dust(orc_grid& g, double dt) {  // advection.cu:254-319, 340-357
             advect_row(P.geomy, n, g.ya.data(), s, g.res[i], dt / R, false, flux);
             for (int j = g.ypad; j < n - g.ypad; ++j) {
                 Cell D = s.Del[j];
-                double f = dt / g.yv[j];
+                real_t f = dt / g.yv[j];
                 D.r *= f; D.u *= f; D.v *= f; D.w *= f;
-                double f2 = 1.0 / R;
+                real_t f2 = 1.0 / R;
                 D.r *= f2; D.u *= f2; D.v *= f2; D.w *= f2;
-                double t = D.v;
+                real_t t = D.v;
                 D.v = D.u; D.u = D.w; D.w = t;
                 Dust DD{D.r, D.u, D.v, D.w};
                 g.TD[g.ind(i, j)] = update_dust(P, g.CD[g.ind(i, j)], DD);
@@ -955,9 +956,9 @@ void viscous_forces(orc_grid& g, const std::vector<Cell>& X) {  // viscosity.cu:
 #pragma omp parallel for schedule(static)
     for (int j = 1; j < g.yarr - 1; ++j) {
         for (int i = 1; i < g.xarr - 1; ++i) {
-            double r[3], t[3], u[9], v[9], nr[9];
+            real_t r[3], t[3], u[9], v[9], nr[9];
             for (int n = 0; n < 3; ++n) { r[n] = g.xc(i + n - 1); t[n] = g.yc(j + n - 1); }
-            double nu0 = nu_at(P, r[0]), nu1 = nu_at(P, r[1]), nu2 = nu_at(P, r[2]);
+            real_t nu0 = nu_at(P, r[0]), nu1 = nu_at(P, r[1]), nu2 = nu_at(P, r[2]);
             for (int n = 0; n < 9; ++n) {
                 size_t id;
                 if (n < 3) id = g.ind(i + n - 1, j);
@@ -968,27 +969,27 @@ void viscous_forces(orc_grid& g, const std::vector<Cell>& X) {  // viscosity.cu:
                 else if (n == 2 || n == 5 || n == 8) nr[n] *= nu2;
                 else nr[n] *= nu1;
             }
-            auto d1 = [&](const double* a, double& dx, double& dxx) {
+            auto d1 = [&](const real_t* a, real_t& dx, real_t& dxx) {
                 dx = (a[2] - a[0]) / (r[2] - r[0]);
                 dxx = ((a[2] - a[1]) / (r[2] - r[1]) - (a[1] - a[0]) / (r[1] - r[0])) / (0.5 * (r[2] - r[0]));
             };
-            auto d2 = [&](const double* a, double& dy, double& dyy, double& dxy) {
+            auto d2 = [&](const real_t* a, real_t& dy, real_t& dyy, real_t& dxy) {
                 dy = (a[7] - a[4]) / (t[2] - t[0]) / r[1];
                 dyy = ((a[7] - a[1]) / (t[2] - t[1]) - (a[1] - a[4]) / (t[1] - t[0])) / (0.5 * (t[2] - t[0])) / r[1] / r[1];
                 dxy = (a[8] - a[6] - a[5] + a[3]) / (r[2] - r[0]) / (t[2] - t[0]) / r[1];
             };
-            double dudx, d2udx2, dvdx, d2vdx2, dvdy, d2vdy2, d2vdxy, dudy, d2udy2, d2udxy;
+            real_t dudx, d2udx2, dvdx, d2vdx2, dvdy, d2vdy2, d2vdxy, dudy, d2udy2, d2udxy;
             d1(u, dudx, d2udx2);
-            double dndx = (nr[2] - nr[0]) / (r[2] - r[0]);
+            real_t dndx = (nr[2] - nr[0]) / (r[2] - r[0]);
             d1(v, dvdx, d2vdx2);
             d2(v, dvdy, d2vdy2, d2vdxy);
             d2(u, dudy, d2udy2, d2udxy);
-            double dndy = (nr[7] - nr[4]) / (t[2] - t[0]) / r[1];
-            double sxx = 2.0 * nr[1] * dudx;
-            double fx = 2.0 * dndx * dudx + 2.0 * nr[1] * d2udx2 + sxx / r[1];
-            double syy = 2.0 * nr[1] * (dvdy + u[1] / r[1]);
-            double sxy = nr[1] * (dudy + dvdx - v[1] / r[1]);
-            double fy = 2.0 * dndy * dvdy + 2.0 * nr[1] * d2vdy2 + 2.0 * nr[1] * dudy / r[1] + 2.0 * dndy * u[1] / r[1];
+            real_t dndy = (nr[7] - nr[4]) / (t[2] - t[0]) / r[1];
+            real_t sxx = 2.0 * nr[1] * dudx;
+            real_t fx = 2.0 * dndx * dudx + 2.0 * nr[1] * d2udx2 + sxx / r[1];
+            real_t syy = 2.0 * nr[1] * (dvdy + u[1] / r[1]);
+            real_t sxy = nr[1] * (dudy + dvdx - v[1] / r[1]);
+            real_t fy = 2.0 * dndy * dvdy + 2.0 * nr[1] * d2vdy2 + 2.0 * nr[1] * dudy / r[1] + 2.0 * dndy * u[1] / r[1];
             fx += dndy * (dudy + dvdx) + nr[1] * (d2udy2 + d2vdxy) - dndy * v[1] / r[1] - nr[1] * dvdy / r[1] - syy / r[1];
             fy += dndx * (dudy + dvdx) + nr[1] * (d2udxy + d2vdx2) - dndx * v[1] / r[1] - nr[1] * dvdx / r[1] +
                   nr[1] * v[1] / r[1] / r[1] + 2.0 * sxy / r[1];
@@ -1009,7 +1010,7 @@ void viscosity_eval(orc_grid& g, bool on_T) {  // viscosity.cu:489-549
 // ----------------------------------------------------------- source terms
 // apply_source_terms (sweeps.cu:597-711), ndim 2, geomx 1, cool_flag 0
 void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
-                  std::vector<Cell>& out, std::vector<Dust>& out_d, double mdt, double dt) {
+                  std::vector<Cell>& out, std::vector<Dust>& out_d, real_t mdt, real_t dt) {
     const pg_params& P = g.P;
     const bool dust = P.dust_flag != 0;
     const bool visc = P.visc_flag != 0;
@@ -1022,31 +1023,31 @@ void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
             Cell C = in[id], T = C;
             Dust Q{0, 0, 0, 0}, D{0, 0, 0, 0};
             if (dust) { Q = in_d[id]; D = Q; }
-            double xc = g.xc(i);
-            double yc = g.yc(j) + g.rot[i] * 0.0;
-            double zc = 0.0;
+            real_t xc = g.xc(i);
+            real_t yc = g.yc(j) + g.rot[i] * 0.0;
+            real_t zc = 0.0;
             // get_gx / get_gy / get_gz (forces.cu:182-327), twobd_flag 0
-            double dis = std::sqrt(xc * xc + zc * zc);
-            double gx = -xc / dis / dis / dis;
-            double gy = 0.0;
-            double gz = -zc / dis / dis / dis;
+            real_t dis = msqrt(xc * xc + zc * zc);
+            real_t gx = -xc / dis / dis / dis;
+            real_t gy = 0.0;
+            real_t gz = -zc / dis / dis / dis;
             for (int m = 0; m < np; ++m) {
-                double fx, fy, fz;
+                real_t fx, fy, fz;
                 planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
                 gx += fx;
             }
             for (int m = 0; m < np; ++m) {
-                double fx, fy, fz;
+                real_t fx, fy, fz;
                 planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
                 gy += fy;
             }
             for (int m = 0; m < np; ++m) {
-                double fx, fy, fz;
+                real_t fx, fy, fz;
                 planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
                 gz += fz;
             }
-            double fx = 0.0, fy = 0.0, fz = 0.0;
-            double fxd = 0.0, fyd = 0.0, fzd = 0.0, ts = 0.0;
+            real_t fx = 0.0, fy = 0.0, fz = 0.0;
+            real_t fxd = 0.0, fyd = 0.0, fzd = 0.0, ts = 0.0;
             auto gas_forces = [&]() {  // get_g_forces (sweeps.cu:555-576)
                 fy = 0.0;
                 if (visc) fy += g.vis[2 * id + 1] / T.r;
@@ -1058,7 +1059,7 @@ void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
                 fxd = (D.v * D.v / xc) * ts + T.u - Q.u;
             };
             if (dust) {
-                ts = P.Stokes / (std::sqrt(P.gam * T.p / T.r) / P.sc_h) / T.r;  // dust.cu:1-4
+                ts = P.Stokes / (msqrt(P.gam * T.p / T.r) / P.sc_h) / T.r;  // dust.cu:1-4
                 dust_forces();
             }
             gas_forces();
@@ -1067,11 +1068,11 @@ void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
                 T.v = C.v + fy * mdt + gy * mdt;
                 T.w = C.w + fz * mdt + gz * mdt;
                 if (dust) {
-                    double dtau = 1.0 - std::exp(-mdt / ts);
+                    real_t dtau = 1.0 - mexp(-mdt / ts);
                     D.u = Q.u + fxd * dtau + gx * ts * dtau;
                     D.v = Q.v + fyd * dtau + gy * ts * dtau;
                     D.w = Q.w + fzd * dtau + gz * ts * dtau;
-                    ts = P.Stokes / (std::sqrt(P.gam * T.p / T.r) / P.sc_h) / T.r;
+                    ts = P.Stokes / (msqrt(P.gam * T.p / T.r) / P.sc_h) / T.r;
                     dust_forces();
                 }
                 gas_forces();
@@ -1084,7 +1085,7 @@ void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
             o.p = in[id].p;
             out[id] = o;
             if (dust) {
-                double dtau = 1.0 - std::exp(-dt / ts);
+                real_t dtau = 1.0 - mexp(-dt / ts);
                 Dust od;
                 od.u = Q.u + fxd * dtau + gx * ts * dtau;
                 od.v = Q.v + fyd * dtau + gy * ts * dtau;
@@ -1097,22 +1098,22 @@ void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
 }
 
 // ----------------------------------------------------------------- killwave
-void killwave(orc_grid& g, double dt) {  // killwave.cu:8-117
+void killwave(orc_grid& g, real_t dt) {  // killwave.cu:8-117
     const pg_params& P = g.P;
-    double d_in = h_at(P, P.xmin) * P.kill_width;
-    double d_out = h_at(P, P.xmax) * P.kill_width;
-    double inner = P.xmin + d_in, outer = P.xmax - d_out;
+    real_t d_in = h_at(P, P.xmin) * P.kill_width;
+    real_t d_out = h_at(P, P.xmax) * P.kill_width;
+    real_t inner = P.xmin + d_in, outer = P.xmax - d_out;
 #pragma omp parallel for schedule(static)
     for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
         for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            double rad = g.xc(i);
+            real_t rad = g.xc(i);
             Cell& c = g.C[g.ind(i, j)];
             for (int side = 0; side < 2; ++side) {
                 bool in_zone = side == 0 ? rad < inner : rad > outer;
                 if (!in_zone) continue;
-                double f = side == 0 ? (inner - rad) / d_in : (rad - outer) / d_out;
-                double tau = std::pow(rad, 1.5);
-                double s = (double)sinf((float)(kHalfPi * f));
+                real_t f = side == 0 ? (inner - rad) / d_in : (rad - outer) / d_out;
+                real_t tau = mpow(rad, 1.5);
+                real_t s = (real_t)sinf((float)val(kHalfPi * f));
                 f = s * s;
                 f *= dt / tau;
                 Cell I = cell_init(P, rad);
@@ -1127,34 +1128,34 @@ void killwave(orc_grid& g, double dt) {  // killwave.cu:8-117
 }
 
 // ------------------------------------------------------------------- CFL
-double cfl_rate(const orc_grid& g) {  // get_dt_lv1 (timestep.cu:18-97), ndim 2, geomy 3
+real_t cfl_rate(const orc_grid& g) {  // get_dt_lv1 (timestep.cu:18-97), ndim 2, geomy 3
     const pg_params& P = g.P;
-    double best = 0.0;
+    double best = 0.0;   // (the step size is frozen in the tangent-linear build)
 #pragma omp parallel for reduction(max : best) schedule(static)
     for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
         for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
             size_t id = g.ind(i, j);
             const Cell& c = g.C[id];
-            double rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
-            double cs = std::sqrt(P.gam * c.p / c.r);
-            double spd = std::fabs(c.u) + cs;
-            if (P.dust_flag) spd = std::fmax(std::fabs(g.CD[id].u), spd);
-            double wid = std::fabs(g.xa[i + 1] - g.xa[i]);
-            double tmp = spd / wid;
-            spd = std::fabs(c.v - rad * P.frame_omega - g.rot[i]) + cs;
-            if (P.dust_flag) spd = std::fmax(std::fabs(g.CD[id].v - rad * P.frame_omega - g.rot[i]), spd);
-            wid = std::fabs(g.ya[j + 1] - g.ya[j]) * rad;
-            tmp = std::fmax(spd / wid, tmp);
-            best = std::fmax(best, tmp);
+            real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+            real_t cs = msqrt(P.gam * c.p / c.r);
+            real_t spd = mfabs(c.u) + cs;
+            if (P.dust_flag) spd = mfmax(mfabs(g.CD[id].u), spd);
+            real_t wid = mfabs(g.xa[i + 1] - g.xa[i]);
+            real_t tmp = spd / wid;
+            spd = mfabs(c.v - rad * P.frame_omega - g.rot[i]) + cs;
+            if (P.dust_flag) spd = mfmax(mfabs(g.CD[id].v - rad * P.frame_omega - g.rot[i]), spd);
+            wid = mfabs(g.ya[j + 1] - g.ya[j]) * rad;
+            tmp = mfmax(spd / wid, tmp);
+            best = std::fmax(best, val(tmp));
         }
     }
     return best;
 }
 
 // -------------------------------------------------------------------- DS
-void ds_step(orc_grid& g, double time, double dt) {  // solve.cu:212-277
+void ds_step(orc_grid& g, real_t time, real_t dt) {  // solve.cu:212-277
     const pg_params& P = g.P;
-    double hdt = 0.5 * dt;
+    real_t hdt = 0.5 * dt;
     if (P.visc_flag) viscosity_eval(g, false);
     fill_x_ghosts(g, g.C);
     if (P.dust_flag) fill_x_ghosts_dust(g, g.CD);
@@ -1214,13 +1215,13 @@ orc_grid* orc_create(const pg_params* Pin) {
     for (int j = 0; j < P.yres; ++j)
         for (int i = 0; i < P.xres; ++i) {
             Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
-            double x = g->xc(i + g->xpad);
+            real_t x = g->xc(i + g->xpad);
             if (P.noise_rho != 0.0) {
                 c.r *= 1.0 + P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, j) - 1.0);
                 if (P.EOS_flag == 0) c.p = cs2_at(P, x) * c.r;
             }
             if (P.noise_vr != 0.0)
-                c.u += P.noise_vr * std::sqrt(P.gam * cs2_at(P, x)) * (2.0 * keyed_uniform(P.seed, 1, i, j) - 1.0);
+                c.u += P.noise_vr * msqrt(P.gam * cs2_at(P, x)) * (2.0 * keyed_uniform(P.seed, 1, i, j) - 1.0);
         }
     g->T = g->C;
     g->TD = g->CD;
@@ -1235,7 +1236,7 @@ orc_grid* orc_create(const pg_params* Pin) {
     g->res.assign(g->xarr, 0.0);
     g->inc.assign(g->xarr, 0);
     for (int i = 0; i < g->xarr; ++i) {
-        double rad = g->xc(i);
+        real_t rad = g->xc(i);
         g->rot[i] = P.OrbAdv_flag ? v_init(P, rad) - rad * P.frame_omega : 0.0;
     }
     return g;
@@ -1258,11 +1259,12 @@ void orc_get_state(const orc_grid* g, double* gas, double* dust) {
             size_t o = (size_t)i + (size_t)g->xres * j;
             const Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
             if (gas) {
-                gas[o] = c.r; gas[N + o] = c.p; gas[2 * N + o] = c.u; gas[3 * N + o] = c.v; gas[4 * N + o] = c.w;
+                gas[o] = val(c.r); gas[N + o] = val(c.p); gas[2 * N + o] = val(c.u); gas[3 * N + o] = val(c.v);
+                gas[4 * N + o] = val(c.w);
             }
             if (dust) {
                 const Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
-                dust[o] = d.r; dust[N + o] = d.u; dust[2 * N + o] = d.v; dust[3 * N + o] = d.w;
+                dust[o] = val(d.r); dust[N + o] = val(d.u); dust[2 * N + o] = val(d.v); dust[3 * N + o] = val(d.w);
             }
         }
 }
@@ -1283,19 +1285,27 @@ void orc_set_state(orc_grid* g, const double* gas, const double* dust) {
         }
 }
 
-void orc_get_coords(const orc_grid* g, double* xa, double* xv, double* ya, double* yv) {
-    if (xa) std::memcpy(xa, g->xa.data(), g->xa.size() * sizeof(double));
-    if (xv) std::memcpy(xv, g->xv.data(), g->xv.size() * sizeof(double));
-    if (ya) std::memcpy(ya, g->ya.data(), g->ya.size() * sizeof(double));
-    if (yv) std::memcpy(yv, g->yv.data(), g->yv.size() * sizeof(double));
+static void copy_out(double* dst, const std::vector<real_t>& src) {
+    for (size_t k = 0; k < src.size(); ++k) dst[k] = val(src[k]);
 }
 
-void orc_get_planet(const orc_grid* g, double* b) { std::memcpy(b, &g->planet, sizeof(Body)); }
+void orc_get_coords(const orc_grid* g, double* xa, double* xv, double* ya, double* yv) {
+    if (xa) copy_out(xa, g->xa);
+    if (xv) copy_out(xv, g->xv);
+    if (ya) copy_out(ya, g->ya);
+    if (yv) copy_out(yv, g->yv);
+}
 
-void orc_get_rot(const orc_grid* g, double* rot) { std::memcpy(rot, g->rot.data(), g->rot.size() * sizeof(double)); }
+void orc_get_planet(const orc_grid* g, double* b) {
+    const Body& p = g->planet;
+    const real_t f[10] = {p.m, p.a, p.e, p.x, p.y, p.z, p.vx, p.vy, p.vz, p.rs};
+    for (int k = 0; k < 10; ++k) b[k] = val(f[k]);
+}
+
+void orc_get_rot(const orc_grid* g, double* rot) { copy_out(rot, g->rot); }
 
 double orc_global_dt(orc_grid* g, double dt_prev) {  // timestep.cu:99-140
-    double dt = g->P.CFL / cfl_rate(*g);
+    double dt = g->P.CFL / val(cfl_rate(*g));
     double cap = dt_prev * 1.1;
     return dt < cap ? dt : cap;
 }
@@ -1324,9 +1334,9 @@ void orc_run(orc_grid* g, int nsteps, double* t_io, double* dt_io, double* dts) 
     for (int n = 0; n < nsteps; ++n) {
         dt = orc_global_dt(g, dt);
         if (t + dt > g->P.end_time) dt = g->P.end_time - t;
-        if (g->ave_interval > 0.0) add_grid(*g, 0.5 * dt / g->ave_interval);  // averaging, main.cu:344
+        if (g->ave_interval > 0.0) add_grid(*g, 0.5 * dt / val(g->ave_interval));  // averaging, main.cu:344
         orc_solve(g, t, dt);
-        if (g->ave_interval > 0.0) add_grid(*g, 0.5 * dt / g->ave_interval);  // main.cu:350
+        if (g->ave_interval > 0.0) add_grid(*g, 0.5 * dt / val(g->ave_interval));  // main.cu:350
         t += dt;
         if (dts) dts[n] = dt;
     }
@@ -1341,8 +1351,8 @@ void orc_viscosity(orc_grid* g, int on_T, double* vis) {
     for (int j = 0; j < g->yres; ++j)
         for (int i = 0; i < g->xres; ++i) {
             size_t id = g->ind(i + g->xpad, j + g->ypad), o = (size_t)i + (size_t)g->xres * j;
-            vis[o] = g->vis[2 * id];
-            vis[N + o] = g->vis[2 * id + 1];
+            vis[o] = val(g->vis[2 * id]);
+            vis[N + o] = val(g->vis[2 * id + 1]);
         }
 }
 
@@ -1366,7 +1376,7 @@ void orc_orbital_advection(orc_grid* g, double dt) {
 void orc_get_orbadv(const orc_grid* g, int* inc, double* res) {
     for (int i = 0; i < g->xres; ++i) {
         if (inc) inc[i] = g->inc[i + g->xpad];
-        if (res) res[i] = g->res[i + g->xpad];
+        if (res) res[i] = val(g->res[i + g->xpad]);
     }
 }
 
@@ -1379,8 +1389,11 @@ int orc_write_snapshot(const orc_grid* g, const char* filename, double t) {
     FILE* f = std::fopen(filename, "wb");
     if (!f) return -1;
     std::fwrite(&t, sizeof(double), 1, f);
-    std::fwrite(&g->xa[g->xpad], sizeof(double), g->xres + 1, f);
-    std::fwrite(&g->ya[g->ypad], sizeof(double), g->yres + 1, f);
+    std::vector<double> ax(g->xres + 1), ay(g->yres + 1);
+    for (int i = 0; i <= g->xres; ++i) ax[i] = val(g->xa[g->xpad + i]);
+    for (int j = 0; j <= g->yres; ++j) ay[j] = val(g->ya[g->ypad + j]);
+    std::fwrite(ax.data(), sizeof(double), ax.size(), f);
+    std::fwrite(ay.data(), sizeof(double), ay.size(), f);
     std::vector<double> tmp((size_t)g->xres * g->yres);
     int nf = g->P.dust_flag ? 8 : 4;
     for (int field = 0; field < nf; ++field) {
@@ -1389,7 +1402,7 @@ int orc_write_snapshot(const orc_grid* g, const char* filename, double t) {
                 size_t id = g->ind(i + g->xpad, j + g->ypad);
                 const Cell& c = g->C[id];
                 const Dust& d = g->CD[id];
-                double v = 0.0;
+                real_t v = 0.0;
                 switch (field) {
                     case 0: v = c.r; break;
                     case 1: v = c.p; break;
@@ -1400,7 +1413,7 @@ int orc_write_snapshot(const orc_grid* g, const char* filename, double t) {
                     case 6: v = d.v; break;
                     default: v = d.w; break;
                 }
-                tmp[(size_t)i + (size_t)g->xres * j] = v;
+                tmp[(size_t)i + (size_t)g->xres * j] = val(v);
             }
         std::fwrite(tmp.data(), sizeof(double), tmp.size(), f);
     }
@@ -1422,11 +1435,11 @@ void orc_average_get(const orc_grid* g, double* gas) {
         for (int i = 0; i < g->xres; ++i) {
             const Cell& a = g->A[g->ind(i + g->xpad, j + g->ypad)];
             const size_t o = (size_t)j * g->xres + i;
-            gas[o] = a.r;
-            gas[N + o] = a.p;
-            gas[2 * N + o] = a.u;
-            gas[3 * N + o] = a.v;
-            gas[4 * N + o] = a.w;
+            gas[o] = val(a.r);
+            gas[N + o] = val(a.p);
+            gas[2 * N + o] = val(a.u);
+            gas[3 * N + o] = val(a.v);
+            gas[4 * N + o] = val(a.w);
         }
 }
 
@@ -1435,21 +1448,59 @@ void orc_average_get(const orc_grid* g, double* gas) {
 // o_star_planet_grav_cyl at dt = 0 (output/forces.cu:41-58, 106-127), which
 // is the solver's star_planet_grav_cyl; summed in cell order
 double orc_torque(const orc_grid* g) {
-    double sum = 0.0;
+    real_t sum = 0.0;
     for (int j = g->ypad; j < g->yarr - g->ypad; ++j)
         for (int i = g->xpad; i < g->xarr - g->xpad; ++i) {
-            const double rad = 0.5 * (g->xa[i + 1] + g->xa[i]);
-            const double azi = 0.5 * (g->ya[j + 1] + g->ya[j]);
-            double dv = 0.5 * (g->xa[i + 1] * g->xa[i + 1] - g->xa[i] * g->xa[i]);
+            const real_t rad = 0.5 * (g->xa[i + 1] + g->xa[i]);
+            const real_t azi = 0.5 * (g->ya[j + 1] + g->ya[j]);
+            real_t dv = 0.5 * (g->xa[i + 1] * g->xa[i + 1] - g->xa[i] * g->xa[i]);
             dv *= (g->ya[j + 1] - g->ya[j]);
-            double fx, fy, fz;
+            real_t fx, fy, fz;
             planet_force_cyl(rad, azi, 0.0, g->planet, fx, fy, fz);
             sum += rad * fy * (g->C[g->ind(i, j)].r * dv);
         }
-    return sum;
+    return val(sum);
 }
 
+#ifdef ORC_TANGENT
+// tangent-linear build: the derivative parts of the interior state (fields
+// as orc_get_state); orc_set_state leaves them zero
+void orc_set_tangent(orc_grid* g, const double* gas, const double* dust) {
+    size_t N = (size_t)g->xres * g->yres;
+    for (int j = 0; j < g->yres; ++j)
+        for (int i = 0; i < g->xres; ++i) {
+            size_t o = (size_t)i + (size_t)g->xres * j;
+            if (gas) {
+                Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+                c.r.d = gas[o]; c.p.d = gas[N + o]; c.u.d = gas[2 * N + o]; c.v.d = gas[3 * N + o];
+                c.w.d = gas[4 * N + o];
+            }
+            if (dust) {
+                Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                d.r.d = dust[o]; d.u.d = dust[N + o]; d.v.d = dust[2 * N + o]; d.w.d = dust[3 * N + o];
+            }
+        }
+}
+void orc_get_tangent(const orc_grid* g, double* gas, double* dust) {
+    size_t N = (size_t)g->xres * g->yres;
+    for (int j = 0; j < g->yres; ++j)
+        for (int i = 0; i < g->xres; ++i) {
+            size_t o = (size_t)i + (size_t)g->xres * j;
+            const Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+            if (gas) {
+                gas[o] = c.r.d; gas[N + o] = c.p.d; gas[2 * N + o] = c.u.d; gas[3 * N + o] = c.v.d;
+                gas[4 * N + o] = c.w.d;
+            }
+            if (dust) {
+                const Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                dust[o] = d.r.d; dust[N + o] = d.u.d; dust[2 * N + o] = d.v.d; dust[3 * N + o] = d.w.d;
+            }
+        }
+}
+#endif
+
 // ------------------------------------------------------------ unit level
+#ifndef ORC_TANGENT
 void orc_fourth_parameters(int geom, const double* x6, const double* a5, double* par3) {
     fourth_parameters(2, geom, x6, a5, par3);
 }
@@ -1507,5 +1558,7 @@ void orc_orbit_solution(const pg_params* P, double t, double* b10) {
     orbit_solution(*P, b, t);
     std::memcpy(b10, &b, sizeof(Body));
 }
+
+#endif  // ORC_TANGENT
 
 }  // extern "C"

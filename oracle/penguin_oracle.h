@@ -90,6 +90,10 @@ void orc_average_end(orc_grid* g);
 void orc_average_get(const orc_grid* g, double* gas5);
 /* planet torque on the gas (global_sum, output.cu:336-457), 2D cylindrical */
 double orc_torque(const orc_grid* g);
+/* tangent-linear build only (liboracle_tlm.so): the derivative parts of the
+   interior state, in the orc_get_state field order */
+void orc_set_tangent(orc_grid* g, const double* gas, const double* dust);
+void orc_get_tangent(const orc_grid* g, double* gas, double* dust);
 
 #ifdef __cplusplus
 }
