@@ -51,19 +51,25 @@ __device__ __forceinline__ void put(real* li, size_t o, real v, int acc) {
 // x0+TX+5 (LDS slot s <-> cell x0-6+s).  Cells beyond the radial ghost
 // zone take the outermost ghost's values: they feed nothing the outputs
 // depend on, and finite values keep their zero adjoints zero.
+// Block of the x-sweep adjoint: one thread per LDS slot of a TX-cell tile
+// (TX + 12 slots, rounded up to whole waves).
+template <int TX> struct AdjXBlock { static constexpr int B = (TX + 4 * NPAD + 63) / 64 * 64; };
+
 template <typename real, int TX>
-__global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
+__global__ __launch_bounds__(AdjXBlock<TX>::B) void k_adj_x(AdjArgs<real> A) {
     const DiskArgs<real>& d = *A.d;
     constexpr int S = TX + 4 * NPAD;
-    constexpr int NT = (S + TX - 1) / TX;   // cells per thread
+    constexpr int BT = AdjXBlock<TX>::B;    // one thread per slot: every phase is one trip
+    constexpr int NT = (S + BT - 1) / BT;   // slots per thread (1)
     __shared__ real r_[S], p_[S], u_[S], vp_[S], L_[S], fl_[S];
     // W: parabolas (8), fluxes (4) and update adjoints (4) of the forward and
     // update phases, then the difference adjoints (16) and flattening
     // adjoint (1) of the parabola reversal
     __shared__ real W[17][S];
-    // per-interface adjoints sent to the left (CL) and right (CR) cell;
-    // CL[0..4] then hold the flattening adjoints of the pressures
-    __shared__ real CL[TI_N][S], CR[TI_N][S];
+    // per-interface adjoints sent to the left cell (those sent to the right
+    // cell stay in the registers of the thread that owns it); CL[0..4] then
+    // hold the flattening adjoints of the pressures
+    __shared__ real CL[TI_N][S];
     real (*sl_)[S] = W;
     real (*sr_)[S] = W + 4;
     real *Fr_ = W[8], *Fu_ = W[9], *FL_ = W[10], *pr_ = W[11];
@@ -82,7 +88,7 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
     auto inside = [&](int s) { const int g = x0 - 2 * NPAD + s; return g >= 0 && g < nx; };
 
     // ---- forward recompute (k_sweep_x_gas) ----
-    for (int s = t; s < ns; s += TX) {
+    for (int s = t; s < ns; s += BT) {
         const int g = x0 - 2 * NPAD + s;
         const int ip = tip(s);
         real r, u, vp;
@@ -95,11 +101,11 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
         L_[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
     }
     __syncthreads();
-    for (int s = t + 2; s < ns - 2; s += TX)
+    for (int s = t + 2; s < ns - 2; s += BT)
         fl_[s] = flatten5(p_[s - 2], p_[s - 1], p_[s], p_[s + 1], p_[s + 2], u_[s - 1], u_[s], u_[s + 1], real(1));
     __syncthreads();
     const real* f4[4] = {r_, p_, u_, L_};
-    for (int s = t + 2; s < ns - 2; s += TX) {
+    for (int s = t + 2; s < ns - 2; s += BT) {
         const int ip = tip(s);
         const real w0 = TAB(T_WL0, ip), w1 = TAB(T_WL1, ip), w2 = TAB(T_WL2, ip);
         const real v0 = TAB(T_WR0, ip), v1 = TAB(T_WR1, ip), v2 = TAB(T_WR2, ip);
@@ -119,7 +125,7 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
         in[TI_VSL] = sl_[3][s]; in[TI_V] = L_[s]; in[TI_VSR] = sr_[3][s];
     };
     // forward fluxes of interfaces [3, ns-3] (interface k: between slots k-1 and k)
-    for (int k = t + NPAD; k < ns - 2; k += TX) {
+    for (int k = t + NPAD; k < ns - 2; k += BT) {
         const int ipl = tip(k - 1), ipr = tip(k);
         real inL[TI_N], inR[TI_N];
         state_in(k - 1, inL);
@@ -136,11 +142,11 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
     }
     __syncthreads();
     // ---- reverse of the update (cells [3, ns-3)); the cell adjoints stay
-    //      in registers (thread t owns slots t + q TX in every later phase)
+    //      in registers (thread t owns slots t + q BT in every later phase)
     real ar[NT], ap[NT], au[NT], aL[NT], avp[NT];
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
-        const int s = t + q * TX;
+        const int s = t + q * BT;
         real lr = 0, lu = 0, lvp = 0, cA = 0, cB = 0, cC = 0, cP = 0;
         if (s >= NPAD && s < ns - NPAD && inside(s)) {
             const int g = x0 - 2 * NPAD + s;
@@ -175,13 +181,19 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
         if (s < S) { cA_[s] = cA; cB_[s] = cB; cC_[s] = cC; cP_[s] = cP; }
     }
     __syncthreads();
-    // ---- reverse of every interface in one pass: each writes the adjoints
-    //      it sends to its two cells, which gather them below ----
-    for (int k = t + NPAD; k < ns - 2; k += TX) {
-        const real lFs = cA_[k] - cA_[k - 1], lUs = cB_[k] - cB_[k - 1];
-        const real lLs = cC_[k] - cC_[k - 1], lpres = cP_[k] - cP_[k - 1];
-        real lL[TI_N], lR[TI_N];
-        for (int m = 0; m < TI_N; ++m) { lL[m] = real(0); lR[m] = real(0); }
+    // ---- reverse of every interface in one pass: interface k (slots k-1 |
+    //      k) is reversed by the thread owning slot k, which keeps what it
+    //      sends to that slot and writes what it sends to slot k-1 into CL ----
+    real lR[NT][TI_N];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int k = t + q * BT;
+        real lL[TI_N];
+        for (int m = 0; m < TI_N; ++m) { lL[m] = real(0); lR[q][m] = real(0); }
+        if (k >= S) break;
+        const bool iface = k >= NPAD && k < ns - 2;
+        const real lFs = iface ? cA_[k] - cA_[k - 1] : real(0), lUs = iface ? cB_[k] - cB_[k - 1] : real(0);
+        const real lLs = iface ? cC_[k] - cC_[k - 1] : real(0), lpres = iface ? cP_[k] - cP_[k - 1] : real(0);
         if (lFs != real(0) || lUs != real(0) || lLs != real(0) || lpres != real(0)) {
             const int ipl = tip(k - 1), ipr = tip(k);
             const real g = TAB(T_XA, ipr);
@@ -197,9 +209,9 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
             real lS8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lcl = 0, lcr = 0;
             hllc_iso_adj(cl, cr, S8, lFs * g, lUs * g, lLs * g, lpres, lS8, lcl, lcr);
             trace_left_adj(cL0, dt, idxL, inL, lS8[0], lS8[1], lS8[2], lS8[3], lcl, lL);
-            trace_right_adj(cR0, dt, idxR, inR, lS8[4], lS8[5], lS8[6], lS8[7], lcr, lR);
+            trace_right_adj(cR0, dt, idxR, inR, lS8[4], lS8[5], lS8[6], lS8[7], lcr, lR[q]);
         }
-        for (int m = 0; m < TI_N; ++m) { CL[m][k] = lL[m]; CR[m][k] = lR[m]; }
+        for (int m = 0; m < TI_N; ++m) CL[m][k] = lL[m];
     }
     __syncthreads();
     // ---- gather per cell (right cell of interface s, left cell of
@@ -209,12 +221,10 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
     real* lfl_ = W[16];
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
-        const int s = t + q * TX;
+        const int s = t + q * BT;
         if (s >= S) break;
         real gsum[TI_N];
-        for (int m = 0; m < TI_N; ++m) gsum[m] = real(0);
-        if (s >= NPAD && s <= ns - 3)
-            for (int m = 0; m < TI_N; ++m) gsum[m] += CR[m][s];
+        for (int m = 0; m < TI_N; ++m) gsum[m] = lR[q][m];   // zero unless interface s exists
         if (s + 1 >= NPAD && s + 1 <= ns - 3)
             for (int m = 0; m < TI_N; ++m) gsum[m] += CL[m][s + 1];
         ar[q] += gsum[TI_R]; ap[q] += gsum[TI_P]; au[q] += gsum[TI_U]; aL[q] += gsum[TI_V];
@@ -245,7 +255,7 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
     real (*lfp_)[S] = CL;
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
-        const int s = t + q * TX;
+        const int s = t + q * BT;
         if (s >= S) break;
         real add[4];
         for (int f = 0; f < 4; ++f) {
@@ -268,7 +278,7 @@ __global__ __launch_bounds__(TX) void k_adj_x(AdjArgs<real> A) {
     double lalpha = 0.0;
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
-        const int s = t + q * TX;
+        const int s = t + q * BT;
         if (s >= ns) break;
         real lp = ap[q];
         if (s + 2 < S) lp += lfp_[0][s + 2];
@@ -306,9 +316,10 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
     // update phases, then the difference adjoints (12) and flattening adjoint
     // (1) of the parabola reversal
     __shared__ real W[13][E];
-    // per-interface adjoints sent to the left (CL) and right (CR) cell;
-    // CL[0..4] then hold the flattening adjoints of the pressures
-    __shared__ real CL[TC_N][E], CR[TC_N][E];
+    // per-interface adjoints sent to the left cell (those sent to the right
+    // cell stay in the registers of the thread that owns it); CL[0..4] then
+    // hold the flattening adjoints of the pressures
+    __shared__ real CL[TC_N][E];
     real (*sl)[E] = W;
     real (*sr)[E] = W + 3;
     real *F0 = W[6], *F1 = W[7], *F2 = W[8], *F3 = W[9];
@@ -404,13 +415,20 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
         if (e < E) { cA[e] = ca; cB[e] = cb; cC[e] = cc; }
     }
     __syncthreads();
-    // reverse of every interface in one pass: each writes the adjoints it
-    // sends to its two cells, which gather them below
-    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
-        const real lFr = cA[e] - cA[e - TR], lFu = cB[e] - cB[e - TR], lFw = cC[e] - cC[e - TR];
+    // reverse of every interface in one pass: interface e (rows k-1 | k)
+    // is reversed by the thread owning cell e, which keeps what it sends to
+    // that cell and writes what it sends to cell e - TR into CL
+    real lR[NT][TC_N];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const int e = t + q * B;
+        real lL[TC_N];
+        for (int m = 0; m < TC_N; ++m) { lL[m] = real(0); lR[q][m] = real(0); }
+        if (e >= E) break;
+        const bool iface = e >= NPAD * TR && e < (NK - 2) * TR;
+        const real lFr = iface ? cA[e] - cA[e - TR] : real(0), lFu = iface ? cB[e] - cB[e - TR] : real(0);
+        const real lFw = iface ? cC[e] - cC[e - TR] : real(0);
         const real lpres = lFu;   // the pressure difference enters Du like the momentum flux
-        real lL[TC_N], lR[TC_N];
-        for (int m = 0; m < TC_N; ++m) { lL[m] = real(0); lR[m] = real(0); }
         if (lFr != real(0) || lFu != real(0) || lFw != real(0)) {
             real inL[TC_N], inR[TC_N];
             state_in(e - TR, inL);
@@ -423,9 +441,9 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
             real lSc[6] = {0, 0, 0, 0, 0, 0};
             hllc_c_adj(cs0, ic0, cs2, Sc, lFr, lFu, lFw, lpres, lSc);
             trace_left_c_adj(cs0, ic0, dts, idy, inL, lSc[0], lSc[1], lSc[2], lL);
-            trace_right_c_adj(cs0, ic0, dts, idy, inR, lSc[3], lSc[4], lSc[5], lR);
+            trace_right_c_adj(cs0, ic0, dts, idy, inR, lSc[3], lSc[4], lSc[5], lR[q]);
         }
-        for (int m = 0; m < TC_N; ++m) { CL[m][e] = lL[m]; CR[m][e] = lR[m]; }
+        for (int m = 0; m < TC_N; ++m) CL[m][e] = lL[m];
     }
     __syncthreads();
     // gather per cell (right cell of interface k, left cell of interface
@@ -439,9 +457,7 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
         if (e >= E) break;
         const int k = e / TR;
         real g[TC_N];
-        for (int m = 0; m < TC_N; ++m) g[m] = real(0);
-        if (k >= NPAD && k <= NK - 3)
-            for (int m = 0; m < TC_N; ++m) g[m] += CR[m][e];
+        for (int m = 0; m < TC_N; ++m) g[m] = lR[q][m];   // zero unless interface e exists
         if (k + 1 >= NPAD && k + 1 <= NK - 3)
             for (int m = 0; m < TC_N; ++m) g[m] += CL[m][e + TR];
         a0[q] += g[TC_R]; a1[q] += g[TC_U]; a2[q] += g[TC_V];
@@ -964,7 +980,7 @@ template <typename real>
 struct AdjLaunch {
     static constexpr int TXA = 256;
     static hipError_t x(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
-        hipLaunchKernelGGL((k_adj_x<real, TXA>), dim3((nx + TXA - 1) / TXA, ny), dim3(TXA), 0, s, A);
+        hipLaunchKernelGGL((k_adj_x<real, TXA>), dim3((nx + TXA - 1) / TXA, ny), dim3(AdjXBlock<TXA>::B), 0, s, A);
         return hipGetLastError();
     }
     static hipError_t phi(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
