@@ -527,13 +527,15 @@ template <typename real>
 __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k_adj_adv(AdjArgs<real> A) {
     constexpr int TR = AdjPhiTile<real>::TR, TY = AdjPhiTile<real>::TY, B = TR * TY;
     constexpr int NK = TY + 4 * NPAD, E = NK * TR;
+    constexpr int NT = (E + B - 1) / B;   // cells per thread
     __shared__ real c0[E], c1[E], c2[E];
-    __shared__ real sl[3][E], sr[3][E];
-    __shared__ real F0[E], F1[E], F2[E];
-    __shared__ real a0[E], a1[E], a2[E];
-    __shared__ real cA[E], cB[E], cC[E];
-    __shared__ real lsl[3][E], lsr[3][E];
-    __shared__ real ld[3][4][E];
+    // W: parabolas (6), fluxes (3) and update adjoints (3), then the
+    // adjoints of the neighbour differences (12)
+    __shared__ real W[12][E];
+    real (*sl)[E] = W;
+    real (*sr)[E] = W + 3;
+    real *F0 = W[6], *F1 = W[7], *F2 = W[8];
+    real *cA = W[9], *cB = W[10], *cC = W[11];
     const DiskArgs<real>& d = *A.d;
     const int nx = d.nx, ny = d.ny;
     const size_t N = (size_t)nx * ny;
@@ -552,6 +554,7 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
     const real speed = d.res[ic];
     const real q = lim01(fabs(speed) * dts * idy);
     const int jbase = y0 - 2 * NPAD;
+    static_assert(B % TR == 0, "tile");
     for (int e = t; e < E; e += B) {
         real r = real(1), vp = real(0), u = real(0);
         if (live) {
@@ -588,11 +591,15 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
         F0[e] = Dr * speed; F1[e] = Dv * speed; F2[e] = Du * speed;
     }
     __syncthreads();
-    // reverse of the update (update_Cell_advect, no floor), cells [3, NK-3)
-    for (int e = t; e < E; e += B) {
+    // reverse of the update (update_Cell_advect, no floor), cells [3, NK-3);
+    // the cell adjoints stay in registers (thread t owns cells t + q B)
+    real a0[NT], a1[NT], a2[NT];
+#pragma unroll
+    for (int qq = 0; qq < NT; ++qq) {
+        const int e = t + qq * B;
         const int k = e / TR;
         real l0 = 0, l1 = 0, l2 = 0, ca = 0, cb = 0, cc = 0;
-        if (live && k >= NPAD && k < NK - NPAD) {
+        if (e < E && live && k >= NPAD && k < NK - NPAD) {
             const size_t o = (size_t)wrap(jbase + k + inc, ny) * nx + i;
             const real lrn = A.lo[o], lun = A.lo[N + o], lvn = A.lo[2 * N + o];
             const real Dr = (F0[e] - F0[e + TR]) * fup;
@@ -608,15 +615,20 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
             l2 = lun * r * irn;
             ca = lrt * fup; cb = lvn * irn * fup; cc = lun * irn * fup;
         }
-        a0[e] = l0; a1[e] = l1; a2[e] = l2;
-        cA[e] = ca; cB[e] = cb; cC[e] = cc;
-        for (int f = 0; f < 3; ++f) { lsl[f][e] = real(0); lsr[f][e] = real(0); }
+        a0[qq] = l0; a1[qq] = l1; a2[qq] = l2;
+        if (e < E) { cA[e] = ca; cB[e] = cb; cC[e] = cc; }
     }
     __syncthreads();
-    // reverse of the fluxes: each cell is the upwind cell of exactly one
-    // interface, so the accumulation has no conflicts
-    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
-        const int m = e + up;
+    // reverse of the fluxes: cell m is the upwind cell of exactly one
+    // interface (m + TR when speed > 0, m otherwise), reversed by m's own
+    // thread; the parabola adjoints of m stay in its registers
+    real lsl[NT][3], lsr[NT][3];
+#pragma unroll
+    for (int qq = 0; qq < NT; ++qq) {
+        const int m = t + qq * B;
+        const int e = m - up;   // the interface m feeds
+        for (int f = 0; f < 3; ++f) { lsl[qq][f] = real(0); lsr[qq][f] = real(0); }
+        if (m >= E || e < NPAD * TR || e >= (NK - 2) * TR) continue;
         const real lF0 = (cA[e] - cA[e - TR]) * speed;
         const real lF1 = (cB[e] - cB[e - TR]) * speed;
         const real lF2 = (cC[e] - cC[e - TR]) * speed;
@@ -631,51 +643,54 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
             A2 = aveL(q, sl[2][m], c2[m], sr[2][m]);
         }
         // F0 = A0 s, F1 = A1 A0 s, F2 = A2 A0 s
-        const real lA0 = lF0 + lF1 * A1 + lF2 * A2;
-        const real lA1 = lF1 * A0, lA2 = lF2 * A0;
-        real lq = 0, ls[3][2] = {{0, 0}, {0, 0}, {0, 0}}, la[3] = {0, 0, 0};
-        const real lA[3] = {lA0, lA1, lA2};
+        const real lA[3] = {lF0 + lF1 * A1 + lF2 * A2, lF1 * A0, lF2 * A0};
+        real lq = 0, la[3] = {0, 0, 0};
         for (int f = 0; f < 3; ++f) {
             const real* a = cf[f];
-            if (speed > real(0)) aveR_adj(q, sl[f][m], a[m], sr[f][m], lA[f], lq, ls[f][0], la[f], ls[f][1]);
-            else aveL_adj(q, sl[f][m], a[m], sr[f][m], lA[f], lq, ls[f][0], la[f], ls[f][1]);
+            if (speed > real(0)) aveR_adj(q, sl[f][m], a[m], sr[f][m], lA[f], lq, lsl[qq][f], la[f], lsr[qq][f]);
+            else aveL_adj(q, sl[f][m], a[m], sr[f][m], lA[f], lq, lsl[qq][f], la[f], lsr[qq][f]);
         }
-        a0[m] += la[0]; a1[m] += la[1]; a2[m] += la[2];
-        for (int f = 0; f < 3; ++f) { lsl[f][m] = ls[f][0]; lsr[f][m] = ls[f][1]; }
+        a0[qq] += la[0]; a1[qq] += la[1]; a2[qq] += la[2];
     }
-    __syncthreads();
-    for (int e = t; e < E; e += B) {
+    __syncthreads();   // W's parabolas, fluxes and update adjoints are dead from here
+    real (*ld)[4][E] = reinterpret_cast<real (*)[4][E]>(&W[0][0]);
+#pragma unroll
+    for (int qq = 0; qq < NT; ++qq) {
+        const int e = t + qq * B;
+        if (e >= E) break;
         const int k = e / TR;
         real ldf[3][4];
         for (int f = 0; f < 3; ++f)
-            for (int qq = 0; qq < 4; ++qq) ldf[f][qq] = real(0);
+            for (int m = 0; m < 4; ++m) ldf[f][m] = real(0);
         if (k >= 2 && k < NK - 2) {
             real lflat = 0;
             for (int f = 0; f < 3; ++f) {
-                if (lsl[f][e] == real(0) && lsr[f][e] == real(0)) continue;
+                if (lsl[qq][f] == real(0) && lsr[qq][f] == real(0)) continue;
                 const real* a = cf[f];
                 slopes_adj(a[e - 2 * TR], a[e - TR], a[e], a[e + TR], a[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4],
-                           wy[5], real(0), lsl[f][e], lsr[f][e], ldf[f], lflat);
+                           wy[5], real(0), lsl[qq][f], lsr[qq][f], ldf[f], lflat);
             }
         }
         for (int f = 0; f < 3; ++f)
-            for (int qq = 0; qq < 4; ++qq) ld[f][qq][e] = ldf[f][qq];
+            for (int m = 0; m < 4; ++m) ld[f][m][e] = ldf[f][m];
     }
     __syncthreads();
     const int rows = min(TY, ny - y0);
-    for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
+#pragma unroll
+    for (int qq = 0; qq < NT; ++qq) {
+        const int e = t + qq * B;
         const int k = e / TR;
         const int y = k - 2 * NPAD;
-        if (!live || y >= rows) continue;
+        if (e >= E || !live || y < 0 || y >= rows) continue;
         real add[3];
         for (int f = 0; f < 3; ++f) {
             add[f] = -ld[f][0][e + 2 * TR] + ld[f][0][e + TR] - ld[f][1][e + TR] + ld[f][1][e] - ld[f][2][e] +
                      ld[f][2][e - TR] - ld[f][3][e - TR] + ld[f][3][e - 2 * TR];
         }
         const size_t o = (size_t)(y0 + y) * nx + i;
-        put(A.li, o, a0[e] + add[0], A.accumulate);
-        put(A.li, N + o, a2[e] + add[2], A.accumulate);
-        put(A.li, 2 * N + o, a1[e] + add[1], A.accumulate);
+        put(A.li, o, a0[qq] + add[0], A.accumulate);
+        put(A.li, N + o, a2[qq] + add[2], A.accumulate);
+        put(A.li, 2 * N + o, a1[qq] + add[1], A.accumulate);
     }
 }
 
