@@ -1008,8 +1008,8 @@ struct AdjLaunch {
         hipLaunchKernelGGL(k_adj_adv<real>, dim3((nx + TR - 1) / TR, (ny + TY - 1) / TY), dim3(TR * TY), 0, s, A);
         return hipGetLastError();
     }
-    // viscosity tiles: 64 x 16 in FP32, 64 x 8 in FP64 (LDS)
-    static constexpr int THV = sizeof(real) == 8 ? 8 : 16;
+    // viscosity tiles: 64 x 8 (two blocks per CU in FP32)
+    static constexpr int THV = 8;
     // mode 0: viscosity alone; 1: corrector + viscosity + predictor;
     // 2: kick + viscosity (AdjArgs field use: k_adj_vis)
     static hipError_t vis(const AdjArgs<real>& A, int mode, int nx, int ny, hipStream_t s) {
