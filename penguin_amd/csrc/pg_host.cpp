@@ -1247,6 +1247,12 @@ struct AdjWork {
     double* td_seg = nullptr;      // (t, dt) of the segment being swept back
     void* args = nullptr;          // slots x AA_N disk descriptors
     void* stage = nullptr;         // pinned host staging of one adjoint state (3N reals)
+    // per-slot control block and orbital shift (the step's dt, planets, inc,
+    // res), written by the slot's prologue replay in the recompute and read
+    // by the recompute and the adjoint passes of that step
+    StepCtl* sctl = nullptr;
+    int* sinc = nullptr;
+    void* sres = nullptr;
     bool forward_done = false;
     size_t bytes = 0;
     // captured backward sweeps of one segment (recompute + adjoint steps),
@@ -1277,7 +1283,7 @@ void adj_free(AdjWork* w) {
     if (!w) return;
     for (auto& sg : w->graphs)
         if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
-    void* bufs[] = {w->ckpts, w->slot, w->lam, w->lparam, w->td, w->td_seg, w->args};
+    void* bufs[] = {w->ckpts, w->slot, w->lam, w->lparam, w->td, w->td_seg, w->args, w->sctl, w->sinc, w->sres};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (w->stage) (void)hipHostFree(w->stage);
@@ -1311,14 +1317,17 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
         (rc = dev_alloc(w->bytes, &w->lparam, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->td, 2 * (size_t)nsteps * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->td_seg, 2 * (size_t)ckpt * sizeof(double))) ||
-        (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))))
+        (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))) ||
+        (rc = dev_alloc(w->bytes, &w->sctl, (size_t)w->slots * sizeof(StepCtl))) ||
+        (rc = dev_alloc(w->bytes, &w->sinc, (size_t)w->slots * d->nx * sizeof(int))) ||
+        (rc = dev_alloc(w->bytes, &w->sres, (size_t)w->slots * d->nx * es)))
         return cleanup(rc);
     if (hipHostMalloc(&w->stage, 3 * N * es, hipHostMallocDefault) != hipSuccess) {
         w->stage = nullptr;
         return cleanup(fail(PG_E_NOMEM, "pinned staging of %zu bytes", 3 * N * es));
     }
-    // descriptors of the recompute passes: the disk's own tables and control
-    // block, with the state and viscosity pointers of the slot
+    // descriptors of the recompute passes: the disk's own tables, with the
+    // state, viscosity, control-block and shift pointers of the slot
     std::vector<DiskArgs<real>> a((size_t)w->slots * AA_N);
     real* base = (real*)w->slot;
     for (int m = 0; m < w->slots; ++m) {
@@ -1335,6 +1344,9 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
             x.vis2 = vis2;
             x.dts = nullptr;
             x.dts_cap = 0;
+            x.ctl = w->sctl + m;
+            x.inc = w->sinc + (size_t)m * d->nx;
+            x.res = (real*)w->sres + (size_t)m * d->nx;
             return a[(size_t)m * AA_N + k] = x;
         };
         set(AA_V0, sl + SL_S0 * N, sl + SL_S0 * N, sl + SL_V0 * N, nullptr);
@@ -1352,12 +1364,15 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
     return PG_OK;
 }
 
-// prologue of step m of the segment being swept back: t, dt from the
-// segment's device copy, then the planets and the orbital shift as in the
-// forward step
+// prologue of step m of the segment being swept back, into slot m's
+// control block and shift: t, dt from the segment's device copy, then the
+// planets and the orbital shift as in the forward step
+template <typename real>
 int set_step(Group& g, const AdjWork* w, int m) {
-    hipError_t e = g.prec == 8 ? Launch<double>::prologue_replay(g.ls<double>(), w->td_seg + 2 * (size_t)m, g.stream)
-                               : Launch<float>::prologue_replay(g.ls<float>(), w->td_seg + 2 * (size_t)m, g.stream);
+    LaunchSet<real> L = g.ls<real>();
+    L.disks = (const DiskArgs<real>*)w->args + (size_t)m * AA_N + AA_X;
+    L.ndisk = 1;
+    hipError_t e = Launch<real>::prologue_replay(L, w->td_seg + 2 * (size_t)m, g.stream);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
     return PG_OK;
 }
@@ -1375,7 +1390,7 @@ int adj_recompute(pg_handle* h, int m, bool write_next) {
     AdjWork* w = h->adj;
     const size_t N = w->N;
     hipStream_t s = g.stream;
-    int rc = set_step(g, w, m);
+    int rc = set_step<real>(g, w, m);
     if (rc) return rc;
     LaunchSet<real> L = g.ls<real>();
     const DiskArgs<real>* a = (const DiskArgs<real>*)w->args + (size_t)m * AA_N;
@@ -1418,12 +1433,12 @@ int adj_step(pg_handle* h, int m, real*& lamA, real*& lamB) {
     const size_t N = w->N;
     const int nx = d->nx, ny = d->ny;
     hipStream_t s = g.stream;
-    int rc = set_step(g, w, m);
-    if (rc) return rc;
     real* sl = (real*)w->slot + (size_t)m * SLOT_WORDS * N;
     real* lamV0 = (real*)w->lam + 6 * N;
     AdjArgs<real> A{};
-    A.d = (const DiskArgs<real>*)g.d_args;
+    // the slot's descriptor: its control block and shift hold step m's dt,
+    // planets, inc and res from the recompute
+    A.d = (const DiskArgs<real>*)w->args + (size_t)m * AA_N + AA_X;
     A.lparam = w->lparam;
     A.nslot = w->nslot;
     A.q = d->P.n_planet > 0 ? d->P.planet_mass : 0.0;
@@ -1503,6 +1518,11 @@ int adj_forward(pg_handle* h, int nsteps, int ckpt, double t0, double dt0, doubl
     }
     HIPL(hipMemcpy(w->td, td.data(), td.size() * sizeof(double), hipMemcpyHostToDevice));
     if (dts_out) std::memcpy(dts_out, w->dts.data(), sizeof(double) * nsteps);
+    // the slots' control blocks start as the disk's (planet elements,
+    // softening); each slot's replay rewrites the step-dependent fields
+    for (int m = 0; m < w->slots; ++m)
+        HIPL(hipMemcpyAsync(w->sctl + m, g.disks[0]->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, g.stream));
+    HIPL(hipStreamSynchronize(g.stream));
     w->forward_done = true;
     return PG_OK;
 }
