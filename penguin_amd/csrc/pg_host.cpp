@@ -24,6 +24,7 @@
 #include <fstream>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/penguin_amd.h"
@@ -36,6 +37,24 @@ using namespace pg;
 namespace {
 
 thread_local std::string g_err;
+
+// f(lo, hi) over [0, n) in contiguous chunks on up to 8 host threads (the
+// conversions between the caller's FP64 fields and device staging run at
+// host memory bandwidth, which one thread does not reach)
+template <typename F>
+void host_parallel(size_t n, F f) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nt = std::min<size_t>(std::max(1u, std::min(hw, 8u)), std::max<size_t>(1, n / 65536));
+    if (nt <= 1) return f((size_t)0, n);
+    std::vector<std::thread> th;
+    const size_t chunk = (n + nt - 1) / nt;
+    for (size_t k = 1; k < nt; ++k) {
+        const size_t lo = std::min(n, k * chunk), hi = std::min(n, lo + chunk);
+        th.emplace_back([=, &f] { f(lo, hi); });
+    }
+    f((size_t)0, std::min(n, chunk));
+    for (auto& t : th) t.join();
+}
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -1540,13 +1559,15 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     // (through the workspace's pinned staging: no fresh host pages per call)
     real* st = (real*)w->stage;
     const double* cs2 = d->tabh.data() + (size_t)T_CS2 * d->tab_stride + NPAD;
-    for (size_t row = 0; row < N; row += nx)
-        for (int i = 0; i < nx; ++i) {
-            const size_t o = row + i;
-            st[o] = (real)(dL[o] + cs2[i] * dL[N + o]);
-            st[N + o] = (real)dL[2 * N + o];
-            st[2 * N + o] = (real)dL[3 * N + o];
-        }
+    host_parallel(N / nx, [&](size_t j0, size_t j1) {
+        for (size_t row = j0 * nx; row < j1 * nx; row += nx)
+            for (int i = 0; i < nx; ++i) {
+                const size_t o = row + i;
+                st[o] = (real)(dL[o] + cs2[i] * dL[N + o]);
+                st[N + o] = (real)dL[2 * N + o];
+                st[2 * N + o] = (real)dL[3 * N + o];
+            }
+    });
     real* lamA = (real*)w->lam;
     real* lamB = (real*)w->lam + 3 * N;
     HIPL(hipMemcpyAsync(lamA, st, 3 * N * sizeof(real), hipMemcpyHostToDevice, g.stream));
@@ -1611,13 +1632,15 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     out->dalpha = dal;
     if (out->dstate0) {
         double* s0 = out->dstate0;
-        for (size_t o = 0; o < N; ++o) {
-            s0[o] = (double)lam0[o];
-            s0[N + o] = 0.0;
-            s0[2 * N + o] = (double)lam0[N + o];
-            s0[3 * N + o] = (double)lam0[2 * N + o];
-            s0[4 * N + o] = 0.0;
-        }
+        host_parallel(N, [&](size_t o0, size_t o1) {
+            for (size_t o = o0; o < o1; ++o) {
+                s0[o] = (double)lam0[o];
+                s0[N + o] = 0.0;
+                s0[2 * N + o] = (double)lam0[N + o];
+                s0[3 * N + o] = (double)lam0[2 * N + o];
+                s0[4 * N + o] = 0.0;
+            }
+        });
     }
     return PG_OK;
 }

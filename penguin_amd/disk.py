@@ -232,12 +232,17 @@ class Disk:
         self.dt = float(dts[-1])
         return dts
 
-    def adjoint_backward(self, dL_dgas: np.ndarray):
+    def adjoint_backward(self, dL_dgas: np.ndarray, out: np.ndarray = None):
         """dL/d(final gas state) (5, ny, nx) -> (dL/dq, dL/dalpha, dL/d(start
-        state) (5, ny, nx))."""
+        state) (5, ny, nx)).  `out`, a C-contiguous float64 (5, ny, nx) array,
+        receives the last (every field is written), so repeated calls need no
+        fresh host pages."""
         dL = np.ascontiguousarray(dL_dgas, dtype=np.float64)
         assert dL.shape == (5, self.ny, self.nx)
-        d0 = np.empty((5, self.ny, self.nx))   # every field is written
+        if out is None:
+            out = np.empty((5, self.ny, self.nx))
+        assert out.shape == (5, self.ny, self.nx) and out.dtype == np.float64 and out.flags.c_contiguous
+        d0 = out
         g = Grads(0.0, 0.0, _ptr(d0))
         _check(load_library().pg_adjoint_backward(self._h, _ptr(dL), ctypes.byref(g)))
         return g.dq, g.dalpha, d0

@@ -49,8 +49,10 @@ def test_adjoint_dot_product_against_tangent_linear(prec, tol):
     dts = dev.adjoint_forward(nsteps, ckpt)
     dL = np.zeros((5,) + g.shape[1:])
     dL[0], dL[2], dL[3] = lam
-    _, _, d0 = dev.adjoint_backward(dL)
+    buf = np.full((5,) + g.shape[1:], np.nan)     # a caller's buffer: every field must be written
+    _, _, d0 = dev.adjoint_backward(dL, out=buf)
     dev.close()
+    assert d0 is buf and np.all(np.isfinite(buf)) and np.all(buf[1] == 0.0) and np.all(buf[4] == 0.0)
     adj = float(np.sum(d0[0] * d[0]) + np.sum(d0[2] * d[2]) + np.sum(d0[3] * d[3]))
 
     tl = oracle.TangentOracle(p.replace(precision=8))
