@@ -23,13 +23,14 @@ extern "C" {
 
 typedef struct pg_params {
     /* grid (parameters.h:47, 67-69, 194-207) */
-    int32_t ndim;            /* 2 (cylindrical r,phi); 3 reserved          */
+    int32_t ndim;            /* 2 (cylindrical r, phi) or 3 (spherical r, phi, theta) */
     int32_t xres, yres, zres;
-    int32_t geomx, geomy, geomz;   /* 1 = cylindrical radius, 3 = azimuth */
+    int32_t geomx, geomy, geomz;   /* 1 cylindrical radius, 3 azimuth (2D);
+                                      2 spherical radius, 4 azimuth, 5 polar angle (3D) */
     int32_t gridx, gridy, gridz;   /* 0 = linear spacing, 1 = log spacing */
     /* numerics (parameters.h:43, 115-116) */
-    int32_t recon_flag;      /* 6 = PPM4 (4th-order slopes + PPM)          */
-    int32_t EOS_flag;        /* 0 = locally isothermal                     */
+    int32_t recon_flag;      /* 6 = PPM4 (4th-order slopes + PPM), 5 = PEM4 */
+    int32_t EOS_flag;        /* 0 = locally isothermal, 2 = adiabatic (internal_e_flag 1) */
     /* feature switches (parameters.h:4-13): 0 = off, 1 = on               */
     int32_t visc_flag;
     int32_t OrbAdv_flag;
@@ -41,7 +42,8 @@ typedef struct pg_params {
     int32_t bound_lft, bound_rgh, bound_bak, bound_frn;
     /* storage precision of the device state: 4 (FP32) or 8 (FP64) */
     int32_t precision;
-    int32_t reserved0;
+    /* polar-angle boundary types (ndim 3), same codes */
+    int32_t bound_bom, bound_top;
     /* domain (parameters.h:194-199) */
     double xmin, xmax, ymin, ymax, zmin, zmax;
     /* hydro and disk (parameters.h:103, 119-127, 155-171) */

@@ -42,6 +42,7 @@ def _bind(L):
         "orc_get_state": (None, [vp, _dp, _dp]),
         "orc_set_state": (None, [vp, _dp, _dp]),
         "orc_get_coords": (None, [vp, _dp, _dp, _dp, _dp]),
+        "orc_get_zcoords": (None, [vp, _dp, _dp]),
         "orc_get_planet": (None, [vp, _dp]),
         "orc_get_rot": (None, [vp, _dp]),
         "orc_global_dt": (ctypes.c_double, [vp, ctypes.c_double]),
@@ -50,6 +51,7 @@ def _bind(L):
         "orc_viscosity": (None, [vp, ctypes.c_int, _dp]),
         "orc_sources": (None, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
         "orc_sweepx": (None, [vp, ctypes.c_double]),
+        "orc_sweepz": (None, [vp, ctypes.c_double]),
         "orc_sweepy": (None, [vp, ctypes.c_double, ctypes.c_double]),
         "orc_sweepx_dust": (None, [vp, ctypes.c_double]),
         "orc_sweepy_dust": (None, [vp, ctypes.c_double, ctypes.c_double]),
@@ -62,6 +64,8 @@ def _bind(L):
         "orc_ppm_adjust": (None, [ctypes.c_double, _dp]),
         "orc_ppm_aveL": (ctypes.c_double, [ctypes.c_double, _dp]),
         "orc_ppm_aveR": (ctypes.c_double, [ctypes.c_double, _dp]),
+        "orc_pem_aveL": (ctypes.c_double, [ctypes.c_double, _dp]),
+        "orc_pem_aveR": (ctypes.c_double, [ctypes.c_double, _dp]),
         "orc_flatten": (ctypes.c_double, [ctypes.c_double, _dp, _dp, _dp]),
         "orc_set_state_row": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
                                      _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
@@ -123,7 +127,10 @@ class Oracle:
         self._L = lib() if _L is None else _L
         self.p = p.copy()
         self._h = self._L.orc_create(ctypes.byref(self.p))
-        self.nx, self.ny = int(p.xres), int(p.yres)
+        self.nx, self.ny, self.nz = int(p.xres), int(p.yres), int(p.zres)
+        # interior arrays: (ny, nx) in 2D, (nz, ny, nx) in 3D, x fastest
+        self.shape = (self.nz, self.ny, self.nx) if int(p.ndim) == 3 else (self.ny, self.nx)
+        self.zarr = self.nz + 6 if int(p.ndim) == 3 else 1
         self.t = 0.0
         self.dt = float(p.end_time)   # main.cu:144: the first cap is end_time
 
@@ -138,10 +145,10 @@ class Oracle:
         except Exception:
             pass
 
-    # state: gas (5, ny, nx): rho, p, u, v, w; dust (4, ny, nx): rho, u, v, w
+    # state: gas (5,) + shape: rho, p, u, v, w; dust (4,) + shape: rho, u, v, w
     def get_state(self):
-        gas = np.zeros((5, self.ny, self.nx))
-        dust = np.zeros((4, self.ny, self.nx))
+        gas = np.zeros((5,) + self.shape)
+        dust = np.zeros((4,) + self.shape)
         self._L.orc_get_state(self._h, ptr(gas), ptr(dust))
         return gas, dust
 
@@ -157,15 +164,22 @@ class Oracle:
         self._L.orc_get_coords(self._h, ptr(xa), ptr(xv), ptr(ya), ptr(yv))
         return xa, xv, ya, yv
 
+    def zcoords(self):
+        """polar-angle edges and volumes with ghosts (3D)"""
+        za = np.zeros(self.zarr + 1); zv = np.zeros(self.zarr)
+        self._L.orc_get_zcoords(self._h, ptr(za), ptr(zv))
+        return za, zv
+
     def planet(self):
         b = np.zeros(10)
         self._L.orc_get_planet(self._h, ptr(b))
         return b
 
     def rot(self):
-        r = np.zeros(self.nx + 6)
+        """orbital-advection speed per radius (per radius and polar layer in 3D), ghosts included"""
+        r = np.zeros((self.zarr, self.nx + 6))
         self._L.orc_get_rot(self._h, ptr(r))
-        return r
+        return r[0] if self.zarr == 1 else r
 
     def global_dt(self, dt_prev: float) -> float:
         return self._L.orc_global_dt(self._h, dt_prev)
@@ -182,7 +196,7 @@ class Oracle:
         return dts
 
     def viscosity(self, on_T: bool = False) -> np.ndarray:
-        v = np.zeros((2, self.ny, self.nx))
+        v = np.zeros((2,) + self.shape)
         self._L.orc_viscosity(self._h, int(on_T), ptr(v))
         return v
 
@@ -191,6 +205,9 @@ class Oracle:
 
     def sweepx(self, dt):
         self._L.orc_sweepx(self._h, dt)
+
+    def sweepz(self, dt):
+        self._L.orc_sweepz(self._h, dt)
 
     def sweepy(self, t, dt):
         self._L.orc_sweepy(self._h, t, dt)
@@ -205,10 +222,11 @@ class Oracle:
         self._L.orc_orbital_advection(self._h, dt)
 
     def orbadv(self):
-        inc = np.zeros(self.nx, dtype=np.int32)
-        res = np.zeros(self.nx)
+        n = self.shape[0] if len(self.shape) == 3 else 1
+        inc = np.zeros((n, self.nx), dtype=np.int32)
+        res = np.zeros((n, self.nx))
         self._L.orc_get_orbadv(self._h, inc.ctypes.data_as(_ip), ptr(res))
-        return inc, res
+        return (inc[0], res[0]) if n == 1 and len(self.shape) == 2 else (inc, res)
 
     def killwave(self, dt):
         self._L.orc_killwave(self._h, dt)
@@ -227,7 +245,7 @@ class Oracle:
         self._L.orc_average_end(self._h)
 
     def average(self) -> np.ndarray:
-        g = np.zeros((5, self.ny, self.nx))
+        g = np.zeros((5,) + self.shape)
         self._L.orc_average_get(self._h, ptr(g))
         return g
 
@@ -252,8 +270,8 @@ class TangentOracle(Oracle):
                                 ptr(d) if d is not None else None)
 
     def get_tangent(self):
-        gas = np.zeros((5, self.ny, self.nx))
-        dust = np.zeros((4, self.ny, self.nx))
+        gas = np.zeros((5,) + self.shape)
+        dust = np.zeros((4,) + self.shape)
         self._L.orc_get_tangent(self._h, ptr(gas), ptr(dust))
         return gas, dust
 
@@ -280,6 +298,14 @@ def ppm_aveL(q, par3):
 
 def ppm_aveR(q, par3):
     return lib().orc_ppm_aveR(q, ptr(arr(par3)))
+
+
+def pem_aveL(q, par3):
+    return lib().orc_pem_aveL(q, ptr(arr(par3)))
+
+
+def pem_aveR(q, par3):
+    return lib().orc_pem_aveR(q, ptr(arr(par3)))
 
 
 def flatten(gam, r5, p5, u5):

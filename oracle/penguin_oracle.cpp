@@ -30,12 +30,20 @@
 //
 // Scope: ndim = 2 (geomx = 1 cylindrical radius, geomy = 3 azimuth),
 // init_flag 2 disk, recon_flag 6 (PPM4), EOS_flag 0 (locally isothermal),
-// twobd_flag 0, cool_flag 0.  Resolutions of the reference's ambiguities:
+// twobd_flag 0, cool_flag 0, gas and dust; and ndim = 3 (geomx 2 spherical
+// radius, geomy 4 azimuth, geomz 5 polar angle), init_flag 3 disk,
+// recon_flag 5 (PEM4) or 6, EOS_flag 2 with internal_e_flag 1 (adiabatic,
+// internal energy), gas only, no wave killing: the polar sweep
+// (sweeps.cu:458-553), polar ghost fills (boundary.cu:270-321), spherical
+// forces with the spline-softened planet (forces.cu:14-39, 110-327) and the
+// per-(radius, polar layer) orbital advection (orbit.cu:51-99).
+// Resolutions of the reference's ambiguities:
 //   * dust sweeps and dust advection read the flattening coefficient from
 //     uninitialised shared memory under recon_flag 6 (dust.cu:116-122,
 //     advection.cu:186-192 vs PPM.cu:69-78); here it is 0, the convention the
 //     gas advection uses (advection.cu:12-18);
-//   * killwave's __sinf (killwave.cu:11) is sinf on a float argument.
+//   * killwave's __sinf (killwave.cu:11) is sinf on a float argument, PEM's
+//     __logf (PEM.cu:13,29) logf on a float argument.
 
 #include "penguin_oracle.h"
 #include "oracle_real.h"
@@ -89,6 +97,36 @@ real_t v_init(const pg_params& P, real_t x) {
 }
 Cell cell_init(const pg_params& P, real_t x) {
     return Cell{rho_init(P, x), p_init(P, x), u_init(P, x), v_init(P, x), 0.0};
+}
+
+// init_flag 3 (ndim 3, geomx 2): x is the spherical radius, z the polar
+// angle, x sin z the cylindrical radius (init.cu:7-70, 72-349)
+real_t cs2_3(const pg_params& P, real_t x, real_t z) {
+    real_t coeff = P.sc_h * P.sc_h * (1.0 / P.planet_radius);
+    return coeff * mpow(x * msin(z) / P.planet_radius, -P.p_beta);
+}
+real_t h_3(const pg_params& P, real_t x, real_t z) { return msqrt(cs2_3(P, x, z) * mpow(x * msin(z), 3)); }
+real_t nu_3(const pg_params& P, real_t x, real_t z) {
+    if (!P.visc_flag) return 0.0;
+    real_t rp = P.planet_radius;
+    real_t coeff = P.ss_alpha * msqrt(cs2_3(P, rp, z)) * h_3(P, rp, z);
+    return coeff * mpow(x * msin(z) / rp, P.p_alpha);
+}
+real_t vertical_3(const pg_params& P, real_t x, real_t z) {  // get_vertical, geomx 2
+    return mexp((1.0 / x - 1.0 / (x * msin(z))) / cs2_3(P, x, z));
+}
+real_t rho_3(const pg_params& P, real_t x, real_t z) {
+    return mpow(x * msin(z) / P.planet_radius, -P.p_alpha) * vertical_3(P, x, z);
+}
+real_t p_3(const pg_params& P, real_t x, real_t z) { return cs2_3(P, x, z) * rho_3(P, x, z); }
+real_t v_3(const pg_params& P, real_t x, real_t z) {
+    real_t rho = rho_3(P, x, z);
+    real_t dr = 1.1514178e-7;
+    real_t dPdr = (p_3(P, x + dr, z) - p_3(P, x - dr, z)) / (2.0 * dr);
+    return msqrt(1.0 / x + (x / rho) * dPdr);
+}
+Cell cell_init3(const pg_params& P, real_t x, real_t z) {
+    return Cell{rho_3(P, x, z), p_3(P, x, z), -1.5 * nu_3(P, x, z) / (x * msin(z)), v_3(P, x, z), 0.0};
 }
 Dust dust_init(const pg_params& P, real_t x) {
     real_t St = P.Stokes;
@@ -252,6 +290,54 @@ inline void recon_par(int i, int geom, const real_t* x, const real_t* a, const r
     ppm_adjust(flat[i], par);
 }
 
+// get_PEM_0 / get_PEM_1 (PEM.cu:3-48): power-law sub-cell profiles; the
+// logarithm is the single-precision one of a float argument (__logf,
+// PEM.cu:13,29).  get_PEM_1's x >= 1 assignment is overwritten by the
+// branch that follows it (PEM.cu:23-27) and so has no effect here either.
+inline real_t pem0(real_t x, real_t S, real_t aB, real_t eta) {
+    real_t v = x == 0.0 ? real_t(0.0) : mexp(eta * mlogf(x));
+    return aB + S * v;
+}
+inline real_t pem1(real_t x, real_t S, real_t aB, real_t eta) {
+    real_t v;
+    if (eta * x > 0.001) {
+        v = (1.0 - x) * (mexp(eta * mlogf(1.0 - x)) - 1.0) / (eta * x) + 1.0;
+    } else {
+        real_t t = (eta + 1.0) * x / 2.0;
+        v = t;
+        t *= -(eta - 1.0) * x / 3.0;
+        v += t;
+        t *= -(eta - 2.0) * x / 4.0;
+        v += t;
+        t *= -(eta - 3.0) * x / 5.0;
+        v += t;
+    }
+    return aB + S * v;
+}
+// get_PEM_aveL / aveR (PEM.cu:51-93): the parabola while the slope ratio is
+// at most 2, the power law beyond
+inline real_t pem_aveL(real_t x, const real_t* par) {
+    real_t sL = par[0], aM = par[1], sR = par[2];
+    if (sL * sR == 0.0) return aM;
+    if (sR / sL >= 1.0) return sR / sL <= 2.0 ? ppm_aveL(x, par) : pem0(x, sL, aM - sL, sR / sL);
+    return sL / sR <= 2.0 ? ppm_aveL(x, par) : pem1(x, sL, aM - sL, sL / sR);
+}
+inline real_t pem_aveR(real_t x, const real_t* par) {
+    real_t sL = par[0], aM = par[1], sR = par[2];
+    if (sR * sL == 0.0) return aM;
+    if (sR / sL > 1.0) return sR / sL <= 2.0 ? ppm_aveR(x, par) : pem1(1.0 - x, -sR, aM + sR, sR / sL);
+    return sL / sR <= 2.0 ? ppm_aveR(x, par) : pem0(1.0 - x, -sR, aM + sR, sL / sR);
+}
+// get_CON/PRM_parameters and _aveL/_aveR dispatch (recon.cu:383-501) for
+// recon_flag 6 (PPM4: 4th-order slopes, flattening, PPM) and 5 (PEM4:
+// 4th-order slopes, PEM averages, no flattening)
+inline void con_par(int recon, int i, int geom, const real_t* x, const real_t* a, const real_t* flat, real_t par[3]) {
+    fourth_parameters(i, geom, x, a, par);
+    if (recon == 6) ppm_adjust(flat[i], par);
+}
+inline real_t con_aveL(int recon, real_t q, const real_t* par) { return recon == 5 ? pem_aveL(q, par) : ppm_aveL(q, par); }
+inline real_t con_aveR(int recon, real_t q, const real_t* par) { return recon == 5 ? pem_aveR(q, par) : ppm_aveR(q, par); }
+
 // flatten (PPM.cu:7-62) for cell i of a row of n cells
 real_t flatten_cell(int i, int n, real_t gam, const real_t* r, const real_t* p, const real_t* u) {
     real_t f = 0.0;
@@ -280,31 +366,31 @@ real_t flatten_cell(int i, int n, real_t gam, const real_t* r, const real_t* p, 
 struct State { real_t rl, pl, ul, vl, wl, rr, pr, ur, vr, wr; };
 
 // set_state (states.cu:133-267) for interface i (between cells i-1 and i)
-void trace_state(int i, int geom, real_t gam, const real_t* xa, const real_t* r, const real_t* p,
+void trace_state(int i, int geom, int recon, real_t gam, const real_t* xa, const real_t* r, const real_t* p,
                  const real_t* u, const real_t* v, const real_t* w, const real_t* flat,
                  real_t dt, real_t us, State& S) {
     real_t rp[3], pp[3], up[3], vp[3], wp[3];
     real_t hdt = 0.5 * dt;
     // right state: cell i, swept region adjacent to its left edge
-    recon_par(i, geom, xa, r, flat, rp);
-    recon_par(i, geom, xa, p, flat, pp);
-    recon_par(i, geom, xa, u, flat, up);
+    con_par(recon, i, geom, xa, r, flat, rp);
+    con_par(recon, i, geom, xa, p, flat, pp);
+    con_par(recon, i, geom, xa, u, flat, up);
     real_t xl = xa[i], xr = xa[i + 1];
     real_t cs = msqrt(gam * p[i] / r[i]);
     real_t tmp = xl - mfmin(u[i], 0.0) * dt + cs * dt;
     real_t q = lim01((tmp - xl) / (xr - xl));
     real_t rmin = r[i] * 1.0e-10, pmin = p[i] * 1.0e-10;
-    real_t ur = ppm_aveL(q, up);
+    real_t ur = con_aveL(recon, q, up);
     real_t dis = mfmin(ur, 0.0) * dt;
     tmp = xl - dis;
     q = lim01((tmp - xl) / (xr - xl));
-    real_t r0 = ppm_aveL(q, rp), p0 = ppm_aveL(q, pp), u0 = ppm_aveL(q, up);
+    real_t r0 = con_aveL(recon, q, rp), p0 = con_aveL(recon, q, pp), u0 = con_aveL(recon, q, up);
     r0 = mfmax(r0, rmin);
     p0 = mfmax(p0, pmin);
     cs = msqrt(gam * p0 / r0);
     tmp = xl - dis + cs * dt;
     q = lim01((tmp - xl) / (xr - xl));
-    real_t u_ = ppm_aveL(q, up), p_ = ppm_aveL(q, pp);
+    real_t u_ = con_aveL(recon, q, up), p_ = con_aveL(recon, q, pp);
     u_ = 2.0 * (u_ - u0) / (cs * dt);
     p_ = 2.0 * (p_ - p0) / (cs * dt);
     u_ *= -hdt;
@@ -312,14 +398,14 @@ void trace_state(int i, int geom, real_t gam, const real_t* xa, const real_t* r,
     S.rr = r0 * exp_lim(u_);
     S.pr = p0 * exp_lim(gam * u_);
     S.ur = u0 + p_ / r0 + us;
-    recon_par(i, geom, xa, v, flat, vp);
-    recon_par(i, geom, xa, w, flat, wp);
-    S.vr = ppm_aveL(q, vp);
-    S.wr = ppm_aveL(q, wp);
+    con_par(recon, i, geom, xa, v, flat, vp);
+    con_par(recon, i, geom, xa, w, flat, wp);
+    S.vr = con_aveL(recon, q, vp);
+    S.wr = con_aveL(recon, q, wp);
     // left state: cell i-1, swept region adjacent to its right edge
-    recon_par(i - 1, geom, xa, r, flat, rp);
-    recon_par(i - 1, geom, xa, p, flat, pp);
-    recon_par(i - 1, geom, xa, u, flat, up);
+    con_par(recon, i - 1, geom, xa, r, flat, rp);
+    con_par(recon, i - 1, geom, xa, p, flat, pp);
+    con_par(recon, i - 1, geom, xa, u, flat, up);
     xl = xa[i - 1];
     xr = xa[i];
     cs = msqrt(gam * p[i - 1] / r[i - 1]);
@@ -327,20 +413,20 @@ void trace_state(int i, int geom, real_t gam, const real_t* xa, const real_t* r,
     q = lim01((tmp - xl) / (xr - xl));
     rmin = r[i - 1] * 1.0e-10;
     pmin = p[i - 1] * 1.0e-10;
-    real_t ul = ppm_aveR(q, up);
+    real_t ul = con_aveR(recon, q, up);
     dis = mfmax(ul, 0.0) * dt;
     tmp = xr - dis;
     q = lim01((tmp - xl) / (xr - xl));
-    r0 = ppm_aveR(q, rp);
-    p0 = ppm_aveR(q, pp);
-    u0 = ppm_aveR(q, up);
+    r0 = con_aveR(recon, q, rp);
+    p0 = con_aveR(recon, q, pp);
+    u0 = con_aveR(recon, q, up);
     r0 = mfmax(r0, rmin);
     p0 = mfmax(p0, pmin);
     cs = msqrt(gam * p0 / r0);
     tmp = xr - dis - cs * dt;
     q = lim01((tmp - xl) / (xr - xl));
-    u_ = ppm_aveR(q, up);
-    p_ = ppm_aveR(q, pp);
+    u_ = con_aveR(recon, q, up);
+    p_ = con_aveR(recon, q, pp);
     u_ = 2.0 * (u0 - u_) / (cs * dt);
     p_ = 2.0 * (p0 - p_) / (cs * dt);
     u_ *= -hdt;
@@ -348,10 +434,10 @@ void trace_state(int i, int geom, real_t gam, const real_t* xa, const real_t* r,
     S.rl = r0 * exp_lim(u_);
     S.pl = p0 * exp_lim(gam * u_);
     S.ul = u0 + p_ / r0 + us;
-    recon_par(i - 1, geom, xa, v, flat, vp);
-    recon_par(i - 1, geom, xa, w, flat, wp);
-    S.vl = ppm_aveR(q, vp);
-    S.wl = ppm_aveR(q, wp);
+    con_par(recon, i - 1, geom, xa, v, flat, vp);
+    con_par(recon, i - 1, geom, xa, w, flat, wp);
+    S.vl = con_aveR(recon, q, vp);
+    S.wl = con_aveR(recon, q, wp);
 }
 
 // get_pm_simple, get_lr_speeds_Einfeldt, get_sm (wave_speeds.cu:24-35,288-315)
@@ -435,11 +521,12 @@ struct RowScratch {
 // riemann() + net_source() over a whole row of n cells (riemann.cu:40-131).
 // Del[i] is valid for cells i in [kPad, n-kPad).
 void riemann_row(int geom, int eos, real_t gam, int n, const real_t* xa, RowScratch& s,
-                 real_t dt_state) {
-    for (int i = 0; i < n; ++i) s.flat[i] = flatten_cell(i, n, gam, s.r.data(), s.p.data(), s.u.data());
+                 real_t dt_state, int recon = 6) {
+    if (recon == 6)   // flattening only under recon_flag 6 (riemann.cu:99-102)
+        for (int i = 0; i < n; ++i) s.flat[i] = flatten_cell(i, n, gam, s.r.data(), s.p.data(), s.u.data());
     for (int i = kPad; i < n + 1 - kPad; ++i) {
         State S;
-        trace_state(i, geom, gam, xa, s.r.data(), s.p.data(), s.u.data(), s.v.data(), s.w.data(),
+        trace_state(i, geom, recon, gam, xa, s.r.data(), s.p.data(), s.u.data(), s.v.data(), s.w.data(),
                     s.flat.data(), dt_state, 0.0, S);
         real_t pm, sl, sm, sr;
         wave_speeds(S, gam, 0.0, pm, sl, sm, sr);
@@ -619,24 +706,48 @@ void planet_force_cyl(real_t rad, real_t azi, real_t z, const Body& b, real_t& f
     fz = -b.m * z * k;
 }
 
+// star_planet_grav (forces.cu:14-39), evaluated with dt = 0: the
+// spline-softened planet in spherical coordinates (rad, azimuth, polar
+// angle) plus the indirect term
+void planet_force_sph(real_t rad, real_t azi, real_t pol, const Body& b, real_t& fx, real_t& fy, real_t& fz) {
+    real_t s = msin(azi - (b.y + 0.0 * b.vy)), c = mcos(azi - (b.y + 0.0 * b.vy));
+    real_t sp = msin(pol), cp = mcos(pol);
+    real_t Rp = msqrt(rad * rad + b.x * b.x - 2.0 * b.x * rad * c * sp);
+    real_t k = mfmax(4.0 - 3.0 * Rp / b.rs, 1.0) / mfmax(b.rs * b.rs * b.rs, Rp * Rp * Rp);
+    fx = -b.m * (rad - b.x * c * sp) * k - b.m * sp * c / b.x / b.x;
+    fy = -b.m * b.x * s * k + b.m * sp * s / b.x / b.x;
+    fz = b.m * b.x * c * cp * k - b.m * cp * c / (sp * b.x * b.x);
+}
+
 }  // namespace
 
 // ================================================================== grid
 struct orc_grid {
     pg_params P;
     int xres, yres, xpad, ypad, xarr, yarr;
+    int zres = 1, zpad = 0, zarr = 1;   // ndim 3: the polar-angle axis
     std::vector<real_t> xa, xv, ya, yv, za, zv;
     std::vector<Cell> C, T;
     std::vector<Dust> CD, TD;
     std::vector<real_t> vis;
-    std::vector<real_t> rot, res;
+    std::vector<real_t> rot, res;   // per (i, k): ik(i, k)
     std::vector<int> inc;
     Body planet;
     std::vector<Cell> A;        // time average of the gas (Grid::A)
     real_t ave_interval = 0.0;  // 0: not averaging
-    size_t ind(int i, int j) const { return (size_t)i + (size_t)xarr * (size_t)j; }
+    size_t ind(int i, int j, int k = 0) const { return (size_t)i + (size_t)xarr * ((size_t)j + (size_t)yarr * k); }
+    size_t ik(int i, int k) const { return (size_t)i + (size_t)xarr * k; }
     real_t xc(int i) const { return 0.5 * (xa[i + 1] + xa[i]); }
     real_t yc(int j) const { return 0.5 * (ya[j + 1] + ya[j]); }
+    real_t zc(int k) const { return 0.5 * (za[k + 1] + za[k]); }
+    bool sph() const { return P.ndim == 3; }
+    // cylindrical radius of cell (i, k) (sweeps.cu:288-291, 375-381)
+    real_t rcyl(int i, int k) const { return sph() ? xc(i) * msin(zc(k)) : xc(i); }
+    // the analytic state at the centre of cell (i, k) (init_C, init.cu:456-468)
+    Cell init_at(int i, int k) const { return sph() ? cell_init3(P, xc(i), zc(k)) : cell_init(P, xc(i)); }
+    // interior k range (a single layer in 2D)
+    int k0() const { return zpad; }
+    int k1() const { return zpad + zres; }
 };
 
 namespace {
@@ -657,13 +768,14 @@ real_t keyed_uniform(uint64_t seed, int field, int i, int j) {
 // (boundary.cu:13-113, 462-560)
 void fill_x_ghosts(orc_grid& g, std::vector<Cell>& X) {
     const pg_params& P = g.P;
+    for (int k = 0; k < g.zarr; ++k)
     for (int j = 0; j < g.yarr; ++j) {
         for (int n = 0; n < g.xpad; ++n) {
-            size_t l = g.ind(n, j), r = g.ind(g.xres + g.xpad + n, j);
-            if (P.bound_lft == 0) X[l] = cell_init(P, g.xc(n));
-            else if (P.bound_lft == 1) X[l] = X[g.ind(g.xpad, j)];
-            if (P.bound_rgh == 0) X[r] = cell_init(P, g.xc(g.xres + g.xpad + n));
-            else if (P.bound_rgh == 1) X[r] = X[g.ind(g.xres + g.xpad - 1, j)];
+            size_t l = g.ind(n, j, k), r = g.ind(g.xres + g.xpad + n, j, k);
+            if (P.bound_lft == 0) X[l] = g.init_at(n, k);
+            else if (P.bound_lft == 1) X[l] = X[g.ind(g.xpad, j, k)];
+            if (P.bound_rgh == 0) X[r] = g.init_at(g.xres + g.xpad + n, k);
+            else if (P.bound_rgh == 1) X[r] = X[g.ind(g.xres + g.xpad - 1, j, k)];
         }
     }
 }
@@ -682,36 +794,71 @@ void fill_x_ghosts_dust(orc_grid& g, std::vector<Dust>& X) {
 // boundy / boundy_dust, periodic (boundary.cu:141-268), all i
 template <class T>
 void fill_y_ghosts(orc_grid& g, std::vector<T>& X) {
+    for (int k = 0; k < g.zarr; ++k)
     for (int i = 0; i < g.xarr; ++i)
         for (int n = 0; n < g.ypad; ++n) {
-            X[g.ind(i, n)] = X[g.ind(i, g.yres + n)];
-            X[g.ind(i, g.yres + g.ypad + n)] = X[g.ind(i, g.ypad + n)];
+            X[g.ind(i, n, k)] = X[g.ind(i, g.yres + n, k)];
+            X[g.ind(i, g.yres + g.ypad + n, k)] = X[g.ind(i, g.ypad + n, k)];
         }
+}
+// boundz (boundary.cu:270-321), all i and j: 3 periodic, 2 reflecting (w
+// negated), 1 zero gradient, 0 the analytic state
+void fill_z_ghosts(orc_grid& g, std::vector<Cell>& X) {
+    const pg_params& P = g.P;
+    if (g.zpad == 0) return;
+    for (int j = 0; j < g.yarr; ++j)
+        for (int i = 0; i < g.xarr; ++i)
+            for (int n = 0; n < g.zpad; ++n) {
+                Cell T;
+                if (P.bound_bom == 3) T = X[g.ind(i, j, g.zres + n)];
+                else if (P.bound_bom == 2) { T = X[g.ind(i, j, 2 * g.zpad - 1 - n)]; T.w *= -1.0; }
+                else if (P.bound_bom == 1) T = X[g.ind(i, j, g.zpad)];
+                else T = g.init_at(i, n);
+                X[g.ind(i, j, n)] = T;
+                if (P.bound_top == 3) T = X[g.ind(i, j, g.zpad + n)];
+                else if (P.bound_top == 2) { T = X[g.ind(i, j, g.zres + g.zpad - 1 - n)]; T.w *= -1.0; }
+                else if (P.bound_top == 1) T = X[g.ind(i, j, g.zres + g.zpad - 1)];
+                else T = g.init_at(i, g.zres + g.zpad + n);
+                X[g.ind(i, j, g.zres + g.zpad + n)] = T;
+            }
 }
 
 // ------------------------------------------------------------------ sweeps
 void sweep_x_gas(orc_grid& g, real_t dt) {  // sweeps.cu:263-358
     const pg_params& P = g.P;
     const int n = g.xarr;
+    const int nrow = g.yres * g.zres;
 #pragma omp parallel
     {
         RowScratch s;
         s.resize(n);
 #pragma omp for schedule(static)
-        for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
+        for (int row = 0; row < nrow; ++row) {
+            const int j = g.ypad + row % g.yres, k = g.zpad + row / g.yres;
             for (int i = 0; i < n; ++i) {
-                const Cell& c = g.C[g.ind(i, j)];
+                const Cell& c = g.C[g.ind(i, j, k)];
                 real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
-                s.r[i] = c.r; s.p[i] = c.p; s.u[i] = c.u; s.v[i] = c.v * rad; s.w[i] = c.w;
+                s.r[i] = c.r; s.p[i] = c.p; s.u[i] = c.u;
+                if (g.sph()) {   // angular momenta: v R_cyl, w r
+                    s.v[i] = c.v * (rad * msin(g.zc(k)));
+                    s.w[i] = c.w * rad;
+                } else {
+                    s.v[i] = c.v * rad; s.w[i] = c.w;
+                }
             }
-            riemann_row(P.geomx, P.EOS_flag, P.gam, n, g.xa.data(), s, dt / 1.0);
+            riemann_row(P.geomx, P.EOS_flag, P.gam, n, g.xa.data(), s, dt / 1.0, P.recon_flag);
             for (int i = g.xpad; i < n - g.xpad; ++i) {
                 Cell D = s.Del[i];
                 real_t f = dt / g.xv[i];
                 D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
                 real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
-                D.v /= rad;
-                g.T[g.ind(i, j)] = update_cell(P, g.xc(i), g.C[g.ind(i, j)], D);
+                if (g.sph()) {
+                    D.v /= rad * msin(g.zc(k));
+                    D.w /= rad;
+                } else {
+                    D.v /= rad;
+                }
+                g.T[g.ind(i, j, k)] = update_cell(P, g.xc(i), g.C[g.ind(i, j, k)], D);
             }
         }
     }
@@ -721,22 +868,24 @@ void sweep_x_gas(orc_grid& g, real_t dt) {  // sweeps.cu:263-358
 void sweep_y_gas(orc_grid& g, real_t dt) {  // sweeps.cu:361-456
     const pg_params& P = g.P;
     const int n = g.yarr;
+    const int nrow = g.xres * g.zres;
 #pragma omp parallel
     {
         RowScratch s;
         s.resize(n);
 #pragma omp for schedule(static)
-        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            real_t R = g.xc(i);
-            real_t rot = g.rot[i];
+        for (int row = 0; row < nrow; ++row) {
+            const int i = g.xpad + row % g.xres, k = g.zpad + row / g.xres;
+            real_t R = g.rcyl(i, k);
+            real_t rot = g.rot[g.ik(i, k)];
             for (int j = 0; j < n; ++j) {
-                const Cell& c = g.C[g.ind(i, j)];
+                const Cell& c = g.C[g.ind(i, j, k)];
                 s.r[j] = c.r; s.p[j] = c.p;
                 s.u[j] = c.v - rot;
                 s.u[j] -= R * P.frame_omega;
                 s.v[j] = c.w; s.w[j] = c.u;
             }
-            riemann_row(P.geomy, P.EOS_flag, P.gam, n, g.ya.data(), s, dt / R);
+            riemann_row(P.geomy, P.EOS_flag, P.gam, n, g.ya.data(), s, dt / R, P.recon_flag);
             for (int j = g.ypad; j < n - g.ypad; ++j) {
                 Cell D = s.Del[j];
                 real_t f = dt / g.yv[j] / R;
@@ -744,7 +893,42 @@ void sweep_y_gas(orc_grid& g, real_t dt) {  // sweeps.cu:361-456
                 D.u += (rot + R * P.frame_omega) * D.r;
                 real_t t = D.v;
                 D.v = D.u; D.u = D.w; D.w = t;
-                g.T[g.ind(i, j)] = update_cell(P, g.xc(i), g.C[g.ind(i, j)], D);
+                g.T[g.ind(i, j, k)] = update_cell(P, g.xc(i), g.C[g.ind(i, j, k)], D);
+            }
+        }
+    }
+    std::swap(g.C, g.T);
+}
+
+void sweep_z_gas(orc_grid& g, real_t dt) {  // sweepz_inplace (sweeps.cu:458-553), geomz 5
+    const pg_params& P = g.P;
+    const int n = g.zarr;
+    const int nrow = g.xres * g.yres;
+#pragma omp parallel
+    {
+        RowScratch s;
+        s.resize(n);
+#pragma omp for schedule(static)
+        for (int row = 0; row < nrow; ++row) {
+            const int i = g.xpad + row % g.xres, j = g.ypad + row / g.xres;
+            const real_t rad = g.xc(i);
+            for (int k = 0; k < n; ++k) {
+                const Cell& c = g.C[g.ind(i, j, k)];
+                // normal w, transverse u and the angular momentum v R_cyl
+                s.r[k] = c.r; s.p[k] = c.p; s.u[k] = c.w; s.v[k] = c.u;
+                s.w[k] = c.v * (rad * msin(0.5 * (g.za[k + 1] + g.za[k])));
+            }
+            riemann_row(P.geomz, P.EOS_flag, P.gam, n, g.za.data(), s, dt / rad, P.recon_flag);
+            for (int k = g.zpad; k < n - g.zpad; ++k) {
+                Cell D = s.Del[k];
+                real_t f = dt / g.zv[k];
+                D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
+                f = 1.0 / rad;
+                D.r *= f; D.p *= f; D.u *= f; D.v *= f; D.w *= f;
+                D.w /= rad * msin(0.5 * (g.za[k + 1] + g.za[k]));
+                real_t t = D.w;
+                D.w = D.u; D.u = D.v; D.v = t;
+                g.T[g.ind(i, j, k)] = update_cell(P, g.xc(i), g.C[g.ind(i, j, k)], D);
             }
         }
     }
@@ -818,26 +1002,29 @@ void sweep_y_dust(orc_grid& g, real_t dt) {  // dust.cu:260-353
 void set_orbital_advection(orc_grid& g, real_t dt) {  // orbit.cu:74-99
     const pg_params& P = g.P;
     real_t dy0 = (P.ymax - P.ymin) / (real_t)P.yres;
+    for (int k = g.k0(); k < g.k1(); ++k)
     for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-        real_t dy = dy0 * g.xc(i);
-        real_t rot = g.rot[i];
+        real_t dy = dy0 * g.rcyl(i, k);
+        real_t rot = g.rot[g.ik(i, k)];
         int shift = (int)val(mfloor(rot * dt / dy + 0.5));
         real_t speed = shift * dy / dt;
-        g.res[i] = rot - speed;
-        g.inc[i] = shift;
+        g.res[g.ik(i, k)] = rot - speed;
+        g.inc[g.ik(i, k)] = shift;
     }
 }
 
 void shift_orbital_advection(orc_grid& g) {  // orbit.cu:17-49, 144-157
     const int jmax = g.yres;
+    const int nrow = g.xres * g.zres;
 #pragma omp parallel for schedule(static)
-    for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-        int inc = g.inc[i];
+    for (int row = 0; row < nrow; ++row) {
+        const int i = g.xpad + row % g.xres, k = g.zpad + row / g.xres;
+        int inc = g.inc[g.ik(i, k)];
         for (int j = 0; j < jmax; ++j) {
             int jj = (j + inc) % jmax;
             if (jj < 0) jj += jmax;
-            g.T[g.ind(i, jj + g.ypad)] = g.C[g.ind(i, j + g.ypad)];
-            if (g.P.dust_flag) g.TD[g.ind(i, jj + g.ypad)] = g.CD[g.ind(i, j + g.ypad)];
+            g.T[g.ind(i, jj + g.ypad, k)] = g.C[g.ind(i, j + g.ypad, k)];
+            if (g.P.dust_flag) g.TD[g.ind(i, jj + g.ypad, k)] = g.CD[g.ind(i, j + g.ypad, k)];
         }
     }
     std::swap(g.C, g.T);
@@ -846,7 +1033,7 @@ void shift_orbital_advection(orc_grid& g) {  // orbit.cu:17-49, 144-157
 
 // advection() row phases (advection.cu:1-86 gas, 175-252 dust), flat = 0
 void advect_row(int geom, int n, const real_t* xa, RowScratch& s, real_t speed, real_t dt_eff,
-                bool gas, std::vector<Cell>& flux) {
+                bool gas, std::vector<Cell>& flux, int recon = 6) {
     std::fill(s.flat.begin(), s.flat.begin() + n, 0.0);
     real_t par[3];
     for (int i = kPad; i < n + 1 - kPad; ++i) {
@@ -855,20 +1042,20 @@ void advect_row(int geom, int n, const real_t* xa, RowScratch& s, real_t speed, 
             real_t xl = xa[i - 1], xr = xa[i];
             real_t x0 = xr - speed * dt_eff;
             real_t q = lim01((x0 - xl) / (xr - xl));
-            recon_par(i - 1, geom, xa, s.r.data(), s.flat.data(), par); D.r = ppm_aveR(q, par);
-            if (gas) { recon_par(i - 1, geom, xa, s.p.data(), s.flat.data(), par); D.p = ppm_aveR(q, par); }
-            recon_par(i - 1, geom, xa, s.u.data(), s.flat.data(), par); D.u = ppm_aveR(q, par) * D.r;
-            recon_par(i - 1, geom, xa, s.v.data(), s.flat.data(), par); D.v = ppm_aveR(q, par) * D.r;
-            recon_par(i - 1, geom, xa, s.w.data(), s.flat.data(), par); D.w = ppm_aveR(q, par) * D.r;
+            con_par(recon, i - 1, geom, xa, s.r.data(), s.flat.data(), par); D.r = con_aveR(recon, q, par);
+            if (gas) { con_par(recon, i - 1, geom, xa, s.p.data(), s.flat.data(), par); D.p = con_aveR(recon, q, par); }
+            con_par(recon, i - 1, geom, xa, s.u.data(), s.flat.data(), par); D.u = con_aveR(recon, q, par) * D.r;
+            con_par(recon, i - 1, geom, xa, s.v.data(), s.flat.data(), par); D.v = con_aveR(recon, q, par) * D.r;
+            con_par(recon, i - 1, geom, xa, s.w.data(), s.flat.data(), par); D.w = con_aveR(recon, q, par) * D.r;
         } else {
             real_t xl = xa[i], xr = xa[i + 1];
             real_t x0 = xl - speed * dt_eff;
             real_t q = lim01((x0 - xl) / (xr - xl));
-            recon_par(i, geom, xa, s.r.data(), s.flat.data(), par); D.r = ppm_aveL(q, par);
-            if (gas) { recon_par(i, geom, xa, s.p.data(), s.flat.data(), par); D.p = ppm_aveL(q, par); }
-            recon_par(i, geom, xa, s.u.data(), s.flat.data(), par); D.u = ppm_aveL(q, par) * D.r;
-            recon_par(i, geom, xa, s.v.data(), s.flat.data(), par); D.v = ppm_aveL(q, par) * D.r;
-            recon_par(i, geom, xa, s.w.data(), s.flat.data(), par); D.w = ppm_aveL(q, par) * D.r;
+            con_par(recon, i, geom, xa, s.r.data(), s.flat.data(), par); D.r = con_aveL(recon, q, par);
+            if (gas) { con_par(recon, i, geom, xa, s.p.data(), s.flat.data(), par); D.p = con_aveL(recon, q, par); }
+            con_par(recon, i, geom, xa, s.u.data(), s.flat.data(), par); D.u = con_aveL(recon, q, par) * D.r;
+            con_par(recon, i, geom, xa, s.v.data(), s.flat.data(), par); D.v = con_aveL(recon, q, par) * D.r;
+            con_par(recon, i, geom, xa, s.w.data(), s.flat.data(), par); D.w = con_aveL(recon, q, par) * D.r;
         }
         D.r *= speed; D.p *= speed; D.u *= speed; D.v *= speed; D.w *= speed;
         flux[i] = D;
@@ -889,19 +1076,21 @@ void advect_y_gas(orc_grid& g, real_t dt) {  // advection.cu:107-173, 321-338
     const pg_params& P = g.P;
     fill_y_ghosts(g, g.C);
     const int n = g.yarr;
+    const int nrow = g.xres * g.zres;
 #pragma omp parallel
     {
         RowScratch s;
         s.resize(n);
         std::vector<Cell> flux(n + 1);
 #pragma omp for schedule(static)
-        for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            real_t R = g.xc(i);
+        for (int row = 0; row < nrow; ++row) {
+            const int i = g.xpad + row % g.xres, k = g.zpad + row / g.xres;
+            real_t R = g.rcyl(i, k);
             for (int j = 0; j < n; ++j) {
-                const Cell& c = g.C[g.ind(i, j)];
+                const Cell& c = g.C[g.ind(i, j, k)];
                 s.r[j] = c.r; s.p[j] = c.p; s.u[j] = c.v; s.v[j] = c.w; s.w[j] = c.u;
             }
-            advect_row(P.geomy, n, g.ya.data(), s, g.res[i], dt / R, true, flux);
+            advect_row(P.geomy, n, g.ya.data(), s, g.res[g.ik(i, k)], dt / R, true, flux, P.recon_flag);
             for (int j = g.ypad; j < n - g.ypad; ++j) {
                 Cell D = s.Del[j];
                 real_t f = dt / g.yv[j];
@@ -910,7 +1099,7 @@ void advect_y_gas(orc_grid& g, real_t dt) {  // advection.cu:107-173, 321-338
                 D.r *= f2; D.p *= f2; D.u *= f2; D.v *= f2; D.w *= f2;
                 real_t t = D.v;
                 D.v = D.u; D.u = D.w; D.w = t;
-                g.T[g.ind(i, j)] = update_cell_advect(P, g.xc(i), g.C[g.ind(i, j)], D);
+                g.T[g.ind(i, j, k)] = update_cell_advect(P, g.xc(i), g.C[g.ind(i, j, k)], D);
             }
         }
     }
@@ -951,19 +1140,31 @@ void advect_y_dust(orc_grid& g, real_t dt) {  // advection.cu:254-319, 340-357
 }
 
 // ---------------------------------------------------------------- viscosity
-void viscous_forces(orc_grid& g, const std::vector<Cell>& X) {  // viscosity.cu:34-197 (ndim 2)
+// ndim 3 evaluates the same (r, phi) stress divergence on every polar layer
+// with the spherical radius and nu at the layer's angle; its z component is
+// zero and the z derivatives it computes are unused (viscosity.cu:169-181)
+void viscous_forces(orc_grid& g, const std::vector<Cell>& X) {  // viscosity.cu:34-197
     const pg_params& P = g.P;
+    const int kb = g.sph() ? 1 : 0, ke = g.sph() ? g.zarr - 1 : 1;
+    const int nrow = (g.yarr - 2) * (ke - kb);
 #pragma omp parallel for schedule(static)
-    for (int j = 1; j < g.yarr - 1; ++j) {
+    for (int row = 0; row < nrow; ++row) {
+        const int j = 1 + row % (g.yarr - 2), k = kb + row / (g.yarr - 2);
         for (int i = 1; i < g.xarr - 1; ++i) {
             real_t r[3], t[3], u[9], v[9], nr[9];
             for (int n = 0; n < 3; ++n) { r[n] = g.xc(i + n - 1); t[n] = g.yc(j + n - 1); }
-            real_t nu0 = nu_at(P, r[0]), nu1 = nu_at(P, r[1]), nu2 = nu_at(P, r[2]);
+            real_t nu0, nu1, nu2;
+            if (g.sph()) {
+                const real_t z = g.zc(k);
+                nu0 = nu_3(P, r[0], z); nu1 = nu_3(P, r[1], z); nu2 = nu_3(P, r[2], z);
+            } else {
+                nu0 = nu_at(P, r[0]); nu1 = nu_at(P, r[1]); nu2 = nu_at(P, r[2]);
+            }
             for (int n = 0; n < 9; ++n) {
                 size_t id;
-                if (n < 3) id = g.ind(i + n - 1, j);
-                else if (n < 6) id = g.ind(i + n - 4, j - 1);
-                else id = g.ind(i + n - 7, j + 1);
+                if (n < 3) id = g.ind(i + n - 1, j, k);
+                else if (n < 6) id = g.ind(i + n - 4, j - 1, k);
+                else id = g.ind(i + n - 7, j + 1, k);
                 nr[n] = X[id].r; u[n] = X[id].u; v[n] = X[id].v;
                 if (n == 0 || n == 3 || n == 6) nr[n] *= nu0;
                 else if (n == 2 || n == 5 || n == 8) nr[n] *= nu2;
@@ -993,7 +1194,7 @@ void viscous_forces(orc_grid& g, const std::vector<Cell>& X) {  // viscosity.cu:
             fx += dndy * (dudy + dvdx) + nr[1] * (d2udy2 + d2vdxy) - dndy * v[1] / r[1] - nr[1] * dvdy / r[1] - syy / r[1];
             fy += dndx * (dudy + dvdx) + nr[1] * (d2udxy + d2vdx2) - dndx * v[1] / r[1] - nr[1] * dvdx / r[1] +
                   nr[1] * v[1] / r[1] / r[1] + 2.0 * sxy / r[1];
-            size_t id = g.ind(i, j);
+            size_t id = g.ind(i, j, k);
             g.vis[2 * id + 0] = fx;
             g.vis[2 * id + 1] = fy;
         }
@@ -1002,6 +1203,7 @@ void viscous_forces(orc_grid& g, const std::vector<Cell>& X) {  // viscosity.cu:
 
 void viscosity_eval(orc_grid& g, bool on_T) {  // viscosity.cu:489-549
     std::vector<Cell>& X = on_T ? g.T : g.C;
+    fill_z_ghosts(g, X);
     fill_y_ghosts(g, X);
     fill_x_ghosts(g, X);
     viscous_forces(g, X);
@@ -1016,42 +1218,65 @@ void source_terms(orc_grid& g, std::vector<Cell>& in, std::vector<Dust>& in_d,
     const bool visc = P.visc_flag != 0;
     const Body b = g.planet;
     const int np = P.n_planet;
+    const bool sph = g.sph();
+    // ghost +- 1 in every direction (sweeps.cu:622-624)
+    const int kb = sph ? g.zpad - 1 : 0, ke = sph ? g.zarr - g.zpad + 1 : 1;
+    const int jb = g.ypad - 1, nj = g.yarr - 2 * g.ypad + 2;
+    const int nrow = nj * (ke - kb);
 #pragma omp parallel for schedule(static)
-    for (int j = g.ypad - 1; j < g.yarr - g.ypad + 1; ++j) {
+    for (int row = 0; row < nrow; ++row) {
+        const int j = jb + row % nj, k = kb + row / nj;
         for (int i = g.xpad - 1; i < g.xarr - g.xpad + 1; ++i) {
-            size_t id = g.ind(i, j);
+            size_t id = g.ind(i, j, k);
             Cell C = in[id], T = C;
             Dust Q{0, 0, 0, 0}, D{0, 0, 0, 0};
             if (dust) { Q = in_d[id]; D = Q; }
             real_t xc = g.xc(i);
-            real_t yc = g.yc(j) + g.rot[i] * 0.0;
-            real_t zc = 0.0;
+            real_t yc = g.yc(j) + g.rot[g.ik(i, k)] * 0.0;
+            real_t zc = sph ? g.zc(k) : real_t(0.0);
             // get_gx / get_gy / get_gz (forces.cu:182-327), twobd_flag 0
-            real_t dis = msqrt(xc * xc + zc * zc);
-            real_t gx = -xc / dis / dis / dis;
-            real_t gy = 0.0;
-            real_t gz = -zc / dis / dis / dis;
-            for (int m = 0; m < np; ++m) {
-                real_t fx, fy, fz;
-                planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
-                gx += fx;
-            }
-            for (int m = 0; m < np; ++m) {
-                real_t fx, fy, fz;
-                planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
-                gy += fy;
-            }
-            for (int m = 0; m < np; ++m) {
-                real_t fx, fy, fz;
-                planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
-                gz += fz;
+            real_t gx, gy, gz;
+            if (sph) {
+                gx = -1.0 / xc / xc;
+                gy = 0.0;
+                gz = 0.0;
+                for (int m = 0; m < np; ++m) {
+                    real_t fx, fy, fz;
+                    planet_force_sph(xc, yc, zc, b, fx, fy, fz);
+                    gx += fx; gy += fy; gz += fz;
+                }
+                if (zc < P.zmin) gz *= -1.0;
+            } else {
+                real_t dis = msqrt(xc * xc + zc * zc);
+                gx = -xc / dis / dis / dis;
+                gy = 0.0;
+                gz = -zc / dis / dis / dis;
+                for (int m = 0; m < np; ++m) {
+                    real_t fx, fy, fz;
+                    planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
+                    gx += fx;
+                }
+                for (int m = 0; m < np; ++m) {
+                    real_t fx, fy, fz;
+                    planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
+                    gy += fy;
+                }
+                for (int m = 0; m < np; ++m) {
+                    real_t fx, fy, fz;
+                    planet_force_cyl(xc, yc, zc, b, fx, fy, fz);
+                    gz += fz;
+                }
             }
             real_t fx = 0.0, fy = 0.0, fz = 0.0;
             real_t fxd = 0.0, fyd = 0.0, fzd = 0.0, ts = 0.0;
-            auto gas_forces = [&]() {  // get_g_forces (sweeps.cu:555-576)
+            auto gas_forces = [&]() {  // get_g_forces (sweeps.cu:555-576), get_fx/fy/fz (forces.cu:110-180)
+                if (sph) {   // the viscous z component is zero
+                    fz = T.v * T.v / xc / mtan(zc);
+                    if (zc < P.zmin) fz *= -1.0;
+                }
                 fy = 0.0;
                 if (visc) fy += g.vis[2 * id + 1] / T.r;
-                fx = T.v * T.v / xc;
+                fx = sph ? T.v * T.v / xc + T.w * T.w / xc : T.v * T.v / xc;
                 if (visc) fx += g.vis[2 * id + 0] / T.r;
             };
             auto dust_forces = [&]() {  // get_d_forces (sweeps.cu:578-594)
@@ -1131,21 +1356,30 @@ void killwave(orc_grid& g, real_t dt) {  // killwave.cu:8-117
 real_t cfl_rate(const orc_grid& g) {  // get_dt_lv1 (timestep.cu:18-97), ndim 2, geomy 3
     const pg_params& P = g.P;
     double best = 0.0;   // (the step size is frozen in the tangent-linear build)
+    const int nrow = g.yres * g.zres;
 #pragma omp parallel for reduction(max : best) schedule(static)
-    for (int j = g.ypad; j < g.ypad + g.yres; ++j) {
+    for (int row = 0; row < nrow; ++row) {
+        const int j = g.ypad + row % g.yres, k = g.zpad + row / g.yres;
         for (int i = g.xpad; i < g.xpad + g.xres; ++i) {
-            size_t id = g.ind(i, j);
+            size_t id = g.ind(i, j, k);
             const Cell& c = g.C[id];
             real_t rad = 0.5 * (g.xa[i + 1] + g.xa[i]);
+            real_t rcyl = g.sph() ? rad * msin(0.5 * (g.za[k + 1] + g.za[k])) : rad;
+            real_t rot = g.rot[g.ik(i, k)];
             real_t cs = msqrt(P.gam * c.p / c.r);
             real_t spd = mfabs(c.u) + cs;
             if (P.dust_flag) spd = mfmax(mfabs(g.CD[id].u), spd);
             real_t wid = mfabs(g.xa[i + 1] - g.xa[i]);
             real_t tmp = spd / wid;
-            spd = mfabs(c.v - rad * P.frame_omega - g.rot[i]) + cs;
-            if (P.dust_flag) spd = mfmax(mfabs(g.CD[id].v - rad * P.frame_omega - g.rot[i]), spd);
-            wid = mfabs(g.ya[j + 1] - g.ya[j]) * rad;
+            spd = mfabs(c.v - rcyl * P.frame_omega - rot) + cs;
+            if (P.dust_flag) spd = mfmax(mfabs(g.CD[id].v - rcyl * P.frame_omega - rot), spd);
+            wid = mfabs(g.ya[j + 1] - g.ya[j]) * rcyl;
             tmp = mfmax(spd / wid, tmp);
+            if (g.sph()) {   // polar term (timestep.cu:80-88)
+                spd = mfabs(c.w) + cs;
+                wid = mfabs(g.za[k + 1] - g.za[k]) * rad;
+                tmp = mfmax(spd / wid, tmp);
+            }
             best = std::fmax(best, val(tmp));
         }
     }
@@ -1164,6 +1398,10 @@ void ds_step(orc_grid& g, real_t time, real_t dt) {  // solve.cu:212-277
     sweep_x_gas(g, dt);
     fill_y_ghosts(g, g.C);
     sweep_y_gas(g, dt);
+    if (P.ndim > 2) {
+        fill_z_ghosts(g, g.C);
+        sweep_z_gas(g, dt);
+    }
     if (P.dust_flag) {
         fill_x_ghosts_dust(g, g.CD);
         sweep_x_dust(g, dt);
@@ -1201,43 +1439,57 @@ orc_grid* orc_create(const pg_params* Pin) {
     g->yarr = g->yres + 2 * g->ypad;
     build_axis(g->xa, g->xv, P.xmin, P.xmax, P.xres, g->xpad, P.geomx, P.gridx);
     build_axis(g->ya, g->yv, P.ymin, P.ymax, P.yres, g->ypad, P.geomy, P.gridy);
-    build_axis(g->za, g->zv, P.zmin, P.zmax, 1, 0, P.geomz, P.gridz);
-    size_t N = (size_t)g->xarr * g->yarr;
+    if (P.ndim == 3) {
+        g->zres = P.zres; g->zpad = kPad; g->zarr = g->zres + 2 * g->zpad;
+        build_axis(g->za, g->zv, P.zmin, P.zmax, P.zres, g->zpad, P.geomz, P.gridz);
+    } else {
+        build_axis(g->za, g->zv, P.zmin, P.zmax, 1, 0, P.geomz, P.gridz);
+    }
+    size_t N = (size_t)g->xarr * g->yarr * g->zarr;
     g->C.resize(N); g->T.resize(N); g->CD.resize(N); g->TD.resize(N);
     g->vis.assign(2 * N, 0.0);
     // fill_grid (init.cu:502-519): cell-centre values everywhere
+    for (int k = 0; k < g->zarr; ++k)
     for (int j = 0; j < g->yarr; ++j)
         for (int i = 0; i < g->xarr; ++i) {
-            g->C[g->ind(i, j)] = cell_init(P, g->xc(i));
-            g->CD[g->ind(i, j)] = P.dust_flag ? dust_init(P, g->xc(i)) : Dust{0, 0, 0, 0};
+            g->C[g->ind(i, j, k)] = g->init_at(i, k);
+            g->CD[g->ind(i, j, k)] = P.dust_flag ? dust_init(P, g->xc(i)) : Dust{0, 0, 0, 0};
         }
-    // seeded perturbation of the interior (not in the reference)
+    // seeded perturbation of the interior (not in the reference); 3D keys
+    // the azimuthal index with j + yres k
+    for (int k = 0; k < g->zres; ++k)
     for (int j = 0; j < P.yres; ++j)
         for (int i = 0; i < P.xres; ++i) {
-            Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+            Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
             real_t x = g->xc(i + g->xpad);
+            const int jk = j + P.yres * k;
+            const real_t cs2 = g->sph() ? cs2_3(P, x, g->zc(k + g->zpad)) : cs2_at(P, x);
             if (P.noise_rho != 0.0) {
-                c.r *= 1.0 + P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, j) - 1.0);
-                if (P.EOS_flag == 0) c.p = cs2_at(P, x) * c.r;
+                c.r *= 1.0 + P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, jk) - 1.0);
+                if (P.EOS_flag == 0) c.p = cs2 * c.r;
             }
             if (P.noise_vr != 0.0)
-                c.u += P.noise_vr * msqrt(P.gam * cs2_at(P, x)) * (2.0 * keyed_uniform(P.seed, 1, i, j) - 1.0);
+                c.u += P.noise_vr * msqrt(P.gam * cs2) * (2.0 * keyed_uniform(P.seed, 1, i, jk) - 1.0);
         }
     g->T = g->C;
     g->TD = g->CD;
-    // init_planet (planet.cu:319-403), n_planet 1, 2D softening (planet.cu:308-317)
+    // init_planet (planet.cu:319-403), n_planet 1; softening (planet.cu:308-317):
+    // rs_fac h0 a in 2D, rs_fac (q/3)^(1/3) a in 3D
     g->planet = Body{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     g->planet.a = P.planet_radius;
     g->planet.e = P.planet_ecc;
     orbit_solution(P, g->planet, 0.0);
-    g->planet.rs = P.rs_fac * P.sc_h * P.planet_radius;
-    // init_orbital_advection (orbit.cu:51-72)
-    g->rot.assign(g->xarr, 0.0);
-    g->res.assign(g->xarr, 0.0);
-    g->inc.assign(g->xarr, 0);
+    g->planet.rs = P.ndim == 3 ? P.rs_fac * mpow(P.planet_mass / 3.0, 1.0 / 3.0) * P.planet_radius
+                               : P.rs_fac * P.sc_h * P.planet_radius;
+    // init_orbital_advection (orbit.cu:51-72), per (i, k)
+    g->rot.assign((size_t)g->xarr * g->zarr, 0.0);
+    g->res.assign((size_t)g->xarr * g->zarr, 0.0);
+    g->inc.assign((size_t)g->xarr * g->zarr, 0);
+    for (int k = 0; k < g->zarr; ++k)
     for (int i = 0; i < g->xarr; ++i) {
         real_t rad = g->xc(i);
-        g->rot[i] = P.OrbAdv_flag ? v_init(P, rad) - rad * P.frame_omega : 0.0;
+        real_t v = g->sph() ? v_3(P, rad, g->zc(k)) : v_init(P, rad);
+        g->rot[g->ik(i, k)] = P.OrbAdv_flag ? v - g->rcyl(i, k) * P.frame_omega : 0.0;
     }
     return g;
 }
@@ -1253,33 +1505,35 @@ int orc_num_threads(void) {
 }
 
 void orc_get_state(const orc_grid* g, double* gas, double* dust) {
-    size_t N = (size_t)g->xres * g->yres;
+    size_t N = (size_t)g->xres * g->yres * g->zres;
+    for (int k = 0; k < g->zres; ++k)
     for (int j = 0; j < g->yres; ++j)
         for (int i = 0; i < g->xres; ++i) {
-            size_t o = (size_t)i + (size_t)g->xres * j;
-            const Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+            size_t o = (size_t)i + (size_t)g->xres * (j + (size_t)g->yres * k);
+            const Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
             if (gas) {
                 gas[o] = val(c.r); gas[N + o] = val(c.p); gas[2 * N + o] = val(c.u); gas[3 * N + o] = val(c.v);
                 gas[4 * N + o] = val(c.w);
             }
             if (dust) {
-                const Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                const Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
                 dust[o] = val(d.r); dust[N + o] = val(d.u); dust[2 * N + o] = val(d.v); dust[3 * N + o] = val(d.w);
             }
         }
 }
 
 void orc_set_state(orc_grid* g, const double* gas, const double* dust) {
-    size_t N = (size_t)g->xres * g->yres;
+    size_t N = (size_t)g->xres * g->yres * g->zres;
+    for (int k = 0; k < g->zres; ++k)
     for (int j = 0; j < g->yres; ++j)
         for (int i = 0; i < g->xres; ++i) {
-            size_t o = (size_t)i + (size_t)g->xres * j;
+            size_t o = (size_t)i + (size_t)g->xres * (j + (size_t)g->yres * k);
             if (gas) {
-                Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+                Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
                 c.r = gas[o]; c.p = gas[N + o]; c.u = gas[2 * N + o]; c.v = gas[3 * N + o]; c.w = gas[4 * N + o];
             }
             if (dust) {
-                Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
                 d.r = dust[o]; d.u = dust[N + o]; d.v = dust[2 * N + o]; d.w = dust[3 * N + o];
             }
         }
@@ -1294,6 +1548,10 @@ void orc_get_coords(const orc_grid* g, double* xa, double* xv, double* ya, doubl
     if (xv) copy_out(xv, g->xv);
     if (ya) copy_out(ya, g->ya);
     if (yv) copy_out(yv, g->yv);
+}
+void orc_get_zcoords(const orc_grid* g, double* za, double* zv) {
+    if (za) copy_out(za, g->za);
+    if (zv) copy_out(zv, g->zv);
 }
 
 void orc_get_planet(const orc_grid* g, double* b) {
@@ -1317,10 +1575,11 @@ void orc_solve(orc_grid* g, double t, double dt) {  // solve.cu:279-292
 
 // add_grid (output.cu:250-273): A += C * fac over interior cells
 static void add_grid(orc_grid& g, double fac) {
+    for (int k = g.k0(); k < g.k1(); ++k)
     for (int j = g.ypad; j < g.yarr - g.ypad; ++j)
         for (int i = g.xpad; i < g.xarr - g.xpad; ++i) {
-            const Cell& c = g.C[g.ind(i, j)];
-            Cell& a = g.A[g.ind(i, j)];
+            const Cell& c = g.C[g.ind(i, j, k)];
+            Cell& a = g.A[g.ind(i, j, k)];
             a.r += c.r * fac;
             a.p += c.p * fac;
             a.u += c.u * fac;
@@ -1347,10 +1606,12 @@ void orc_run(orc_grid* g, int nsteps, double* t_io, double* dt_io, double* dts) 
 void orc_viscosity(orc_grid* g, int on_T, double* vis) {
     viscosity_eval(*g, on_T != 0);
     if (!vis) return;
-    size_t N = (size_t)g->xres * g->yres;
+    size_t N = (size_t)g->xres * g->yres * g->zres;
+    for (int k = 0; k < g->zres; ++k)
     for (int j = 0; j < g->yres; ++j)
         for (int i = 0; i < g->xres; ++i) {
-            size_t id = g->ind(i + g->xpad, j + g->ypad), o = (size_t)i + (size_t)g->xres * j;
+            size_t id = g->ind(i + g->xpad, j + g->ypad, k + g->zpad);
+            size_t o = (size_t)i + (size_t)g->xres * (j + (size_t)g->yres * k);
             vis[o] = val(g->vis[2 * id]);
             vis[N + o] = val(g->vis[2 * id + 1]);
         }
@@ -1362,6 +1623,7 @@ void orc_sources(orc_grid* g, double mdt, double dt, int inplace) {
 }
 
 void orc_sweepx(orc_grid* g, double dt) { fill_x_ghosts(*g, g->C); sweep_x_gas(*g, dt); }
+void orc_sweepz(orc_grid* g, double dt) { fill_z_ghosts(*g, g->C); sweep_z_gas(*g, dt); }
 void orc_sweepy(orc_grid* g, double t, double dt) { (void)t; fill_y_ghosts(*g, g->C); sweep_y_gas(*g, dt); }
 void orc_sweepx_dust(orc_grid* g, double dt) { fill_x_ghosts_dust(*g, g->CD); sweep_x_dust(*g, dt); }
 void orc_sweepy_dust(orc_grid* g, double t, double dt) { (void)t; fill_y_ghosts(*g, g->CD); sweep_y_dust(*g, dt); }
@@ -1374,9 +1636,11 @@ void orc_orbital_advection(orc_grid* g, double dt) {
 }
 
 void orc_get_orbadv(const orc_grid* g, int* inc, double* res) {
+    for (int k = 0; k < g->zres; ++k)
     for (int i = 0; i < g->xres; ++i) {
-        if (inc) inc[i] = g->inc[i + g->xpad];
-        if (res) res[i] = val(g->res[i + g->xpad]);
+        const size_t o = (size_t)i + (size_t)g->xres * k;
+        if (inc) inc[o] = g->inc[g->ik(i + g->xpad, k + g->zpad)];
+        if (res) res[o] = val(g->res[g->ik(i + g->xpad, k + g->zpad)]);
     }
 }
 
@@ -1389,31 +1653,38 @@ int orc_write_snapshot(const orc_grid* g, const char* filename, double t) {
     FILE* f = std::fopen(filename, "wb");
     if (!f) return -1;
     std::fwrite(&t, sizeof(double), 1, f);
-    std::vector<double> ax(g->xres + 1), ay(g->yres + 1);
+    std::vector<double> ax(g->xres + 1), ay(g->yres + 1), az(g->zres + 1);
     for (int i = 0; i <= g->xres; ++i) ax[i] = val(g->xa[g->xpad + i]);
     for (int j = 0; j <= g->yres; ++j) ay[j] = val(g->ya[g->ypad + j]);
+    for (int k = 0; k <= g->zres; ++k) az[k] = val(g->za[g->zpad + k]);
     std::fwrite(ax.data(), sizeof(double), ax.size(), f);
     std::fwrite(ay.data(), sizeof(double), ay.size(), f);
-    std::vector<double> tmp((size_t)g->xres * g->yres);
-    int nf = g->P.dust_flag ? 8 : 4;
+    if (g->sph()) std::fwrite(az.data(), sizeof(double), az.size(), f);
+    std::vector<double> tmp((size_t)g->xres * g->yres * g->zres);
+    // gas rho, p, u, v (+ w in 3D), then dust rho, u, v, w
+    const int ng = g->sph() ? 5 : 4;
+    int nf = ng + (g->P.dust_flag ? 4 : 0);
     for (int field = 0; field < nf; ++field) {
+        for (int k = 0; k < g->zres; ++k)
         for (int j = 0; j < g->yres; ++j)
             for (int i = 0; i < g->xres; ++i) {
-                size_t id = g->ind(i + g->xpad, j + g->ypad);
+                size_t id = g->ind(i + g->xpad, j + g->ypad, k + g->zpad);
                 const Cell& c = g->C[id];
                 const Dust& d = g->CD[id];
                 real_t v = 0.0;
-                switch (field) {
+                const int fk = field < ng ? field : 5 + (field - ng);
+                switch (fk) {
                     case 0: v = c.r; break;
                     case 1: v = c.p; break;
                     case 2: v = c.u; break;
                     case 3: v = c.v; break;
-                    case 4: v = d.r; break;
-                    case 5: v = d.u; break;
-                    case 6: v = d.v; break;
+                    case 4: v = c.w; break;
+                    case 5: v = d.r; break;
+                    case 6: v = d.u; break;
+                    case 7: v = d.v; break;
                     default: v = d.w; break;
                 }
-                tmp[(size_t)i + (size_t)g->xres * j] = val(v);
+                tmp[(size_t)i + (size_t)g->xres * (j + (size_t)g->yres * k)] = val(v);
             }
         std::fwrite(tmp.data(), sizeof(double), tmp.size(), f);
     }
@@ -1430,11 +1701,12 @@ void orc_average_begin(orc_grid* g, double interval) {
 void orc_average_end(orc_grid* g) { g->ave_interval = 0.0; }
 // the average over interior cells, fields rho, p, u, v, w (x fastest)
 void orc_average_get(const orc_grid* g, double* gas) {
-    const size_t N = (size_t)g->xres * g->yres;
+    const size_t N = (size_t)g->xres * g->yres * g->zres;
+    for (int k = 0; k < g->zres; ++k)
     for (int j = 0; j < g->yres; ++j)
         for (int i = 0; i < g->xres; ++i) {
-            const Cell& a = g->A[g->ind(i + g->xpad, j + g->ypad)];
-            const size_t o = (size_t)j * g->xres + i;
+            const Cell& a = g->A[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
+            const size_t o = (size_t)i + (size_t)g->xres * (j + (size_t)g->yres * k);
             gas[o] = val(a.r);
             gas[N + o] = val(a.p);
             gas[2 * N + o] = val(a.u);
@@ -1466,33 +1738,35 @@ double orc_torque(const orc_grid* g) {
 // tangent-linear build: the derivative parts of the interior state (fields
 // as orc_get_state); orc_set_state leaves them zero
 void orc_set_tangent(orc_grid* g, const double* gas, const double* dust) {
-    size_t N = (size_t)g->xres * g->yres;
+    size_t N = (size_t)g->xres * g->yres * g->zres;
+    for (int k = 0; k < g->zres; ++k)
     for (int j = 0; j < g->yres; ++j)
         for (int i = 0; i < g->xres; ++i) {
-            size_t o = (size_t)i + (size_t)g->xres * j;
+            size_t o = (size_t)i + (size_t)g->xres * (j + (size_t)g->yres * k);
             if (gas) {
-                Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+                Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
                 c.r.d = gas[o]; c.p.d = gas[N + o]; c.u.d = gas[2 * N + o]; c.v.d = gas[3 * N + o];
                 c.w.d = gas[4 * N + o];
             }
             if (dust) {
-                Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
                 d.r.d = dust[o]; d.u.d = dust[N + o]; d.v.d = dust[2 * N + o]; d.w.d = dust[3 * N + o];
             }
         }
 }
 void orc_get_tangent(const orc_grid* g, double* gas, double* dust) {
-    size_t N = (size_t)g->xres * g->yres;
+    size_t N = (size_t)g->xres * g->yres * g->zres;
+    for (int k = 0; k < g->zres; ++k)
     for (int j = 0; j < g->yres; ++j)
         for (int i = 0; i < g->xres; ++i) {
-            size_t o = (size_t)i + (size_t)g->xres * j;
-            const Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad)];
+            size_t o = (size_t)i + (size_t)g->xres * (j + (size_t)g->yres * k);
+            const Cell& c = g->C[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
             if (gas) {
                 gas[o] = c.r.d; gas[N + o] = c.p.d; gas[2 * N + o] = c.u.d; gas[3 * N + o] = c.v.d;
                 gas[4 * N + o] = c.w.d;
             }
             if (dust) {
-                const Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad)];
+                const Dust& d = g->CD[g->ind(i + g->xpad, j + g->ypad, k + g->zpad)];
                 dust[o] = d.r.d; dust[N + o] = d.u.d; dust[2 * N + o] = d.v.d; dust[3 * N + o] = d.w.d;
             }
         }
@@ -1507,6 +1781,8 @@ void orc_fourth_parameters(int geom, const double* x6, const double* a5, double*
 void orc_ppm_adjust(double flat, double* par3) { ppm_adjust(flat, par3); }
 double orc_ppm_aveL(double q, const double* par3) { return ppm_aveL(q, par3); }
 double orc_ppm_aveR(double q, const double* par3) { return ppm_aveR(q, par3); }
+double orc_pem_aveL(double q, const double* par3) { return pem_aveL(q, par3); }
+double orc_pem_aveR(double q, const double* par3) { return pem_aveR(q, par3); }
 double orc_flatten(double gam, const double* r5, const double* p5, const double* u5) {
     return flatten_cell(2, 5, gam, r5, p5, u5);
 }
@@ -1515,7 +1791,7 @@ void orc_set_state_row(int geom, double gam, int n, const double* xa, const doub
                        const double* w, const double* flat, int i, double dt, double* S10) {
     (void)n; (void)dv;
     State S;
-    trace_state(i, geom, gam, xa, r, p, u, v, w, flat, dt, 0.0, S);
+    trace_state(i, geom, 6, gam, xa, r, p, u, v, w, flat, dt, 0.0, S);
     std::memcpy(S10, &S, sizeof(State));
 }
 void orc_riemann_hllc(double gam, int eos, const double* S10, double* out) {

@@ -34,6 +34,7 @@ void orc_get_state(const orc_grid* g, double* gas5, double* dust4);
 void orc_set_state(orc_grid* g, const double* gas5, const double* dust4);
 /* padded coordinate arrays: xa[xarr+1], ya[yarr+1], xv[xarr], yv[yarr] */
 void orc_get_coords(const orc_grid* g, double* xa, double* xv, double* ya, double* yv);
+void orc_get_zcoords(const orc_grid* g, double* za, double* zv);
 void orc_get_planet(const orc_grid* g, double* body10);
 void orc_get_rot(const orc_grid* g, double* rot_xarr);
 
@@ -48,6 +49,7 @@ void   orc_run(orc_grid* g, int nsteps, double* t_io, double* dt_io, double* dts
 void orc_viscosity(orc_grid* g, int on_T, double* vis_interior2);
 void orc_sources(orc_grid* g, double mdt, double dt, int inplace);
 void orc_sweepx(orc_grid* g, double dt);
+void orc_sweepz(orc_grid* g, double dt);
 void orc_sweepy(orc_grid* g, double t, double dt);
 void orc_sweepx_dust(orc_grid* g, double dt);
 void orc_sweepy_dust(orc_grid* g, double t, double dt);
@@ -62,6 +64,9 @@ void orc_fourth_parameters(int geom, const double* x6, const double* a5, double*
 void orc_ppm_adjust(double flat, double* par3);
 double orc_ppm_aveL(double q, const double* par3);
 double orc_ppm_aveR(double q, const double* par3);
+/* PEM sub-cell means (PEM.cu:51-93) */
+double orc_pem_aveL(double q, const double* par3);
+double orc_pem_aveR(double q, const double* par3);
 /* flattening of cell 2 of a 5-cell window (PPM.cu:7-62) */
 double orc_flatten(double gam, const double* r5, const double* p5, const double* u5);
 /* traced interface state at interface i of a row (states.cu:133-267) with the

@@ -33,7 +33,7 @@ class Params(ctypes.Structure):
         ("bound_lft", ctypes.c_int32), ("bound_rgh", ctypes.c_int32),
         ("bound_bak", ctypes.c_int32), ("bound_frn", ctypes.c_int32),
         ("precision", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("bound_bom", ctypes.c_int32), ("bound_top", ctypes.c_int32),
         ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
         ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
         ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
@@ -128,6 +128,38 @@ def disk2d(nr: int, nphi: int, *, precision: int = 4, **kw) -> Params:
     return p
 
 
+def disk3d(nr: int, nphi: int, ntheta: int, *, precision: int = 4, **kw) -> Params:
+    """3D adiabatic disk in spherical coordinates (init_flag 3): r in
+    [0.4, 2.5] log-spaced, phi in [0, 2pi) uniform, theta uniform in
+    [pi/2 - 3h, pi/2 + 3h] with reflecting polar boundaries; PEM4 + HLLC,
+    EOS_flag 2 (gamma 1.4, internal energy), temperature h/r = 0.05 without
+    flaring (p_beta = 1); volume density slope p_alpha = 1 - p_beta/2 + 3/2
+    = 2, i.e. Sigma ~ r^-1 (the reference's 3D formula, parameters.h:155-159);
+    alpha = 1e-4; one planet at r = 1 on a circular orbit with spline
+    softening rs_fac (q/3)^(1/3), rs_fac = 0.5; orbital advection; CFL 0.3."""
+    p = disk2d(nr, nphi, precision=precision)
+    p.ndim = 3
+    p.zres = ntheta
+    p.geomx, p.geomy, p.geomz = 2, 4, 5
+    p.gridx, p.gridy, p.gridz = 1, 0, 0
+    p.recon_flag = 5
+    p.EOS_flag = 2
+    p.gam = 1.4
+    p.bound_bom, p.bound_top = 2, 2
+    p.zmin, p.zmax = HPI - 3.0 * 0.05, HPI + 3.0 * 0.05
+    p.p_alpha = 1.0 - 0.5 * 1.0 + 1.5
+    p.ss_alpha = 1.0e-4
+    p.rs_fac = 0.5
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def config_c3(precision: int = 4, **kw) -> Params:
+    """3D adiabatic disk, 384 x 1536 x 64 (r, phi, theta), q = 1e-4."""
+    return disk3d(384, 1536, 64, precision=precision, **kw)
+
+
 def config_c1(precision: int = 4, **kw) -> Params:
     """Plain method, small: 256 x 512 gas disk, q = 1e-4, alpha = 1e-3,
     100 steps, 1e-4 seeded noise on Sigma."""
@@ -175,5 +207,5 @@ def interior_cell_updates(p: Params, steps: int) -> float:
     return float(p.xres) * float(p.yres) * float(p.zres) * steps
 
 
-__all__ = ["Params", "disk2d", "config_c1", "config_c2", "config_c5", "batch_configs",
+__all__ = ["Params", "disk2d", "disk3d", "config_c1", "config_c2", "config_c3", "config_c5", "batch_configs",
            "soft_length", "interior_cell_updates", "TWOPI", "HPI", "math"]
