@@ -45,14 +45,19 @@ const char* pg_version(void);
 
 /* Allocate a disk and fill it with the initial state: grid (init.cu:472-500),
    profiles (init.cu:502-532), planet (planet.cu:319-403) and orbital-advection
-   rotation (orbit.cu:111-120); replaces main.cu:157-301. */
+   rotation (orbit.cu:111-120); replaces main.cu:157-301.  params->ndim = 3
+   creates the spherical adiabatic disk (init_flag 3: geomx/y/z = 2/4/5,
+   EOS_flag 2, recon_flag 5 or 6, polar boundaries 2); its state arrays are
+   x fastest over (zres, yres, xres).  The dust, wave-killing, averaging,
+   torque and adjoint entry points are two-dimensional and return
+   PG_E_INVALID for it. */
 int pg_create(const pg_params* params, pg_handle** out);
 /* Free everything the handle owns (main.cu:433-518). */
 int pg_destroy(pg_handle* h);
 
 /* Copy the interior state to/from host FP64 buffers (main.cu:61-104).
-   gas: 5*xres*yres doubles (rho,p,u,v,w); dust: 4*xres*yres (rho,u,v,w) or
-   NULL.  On set, p is ignored under EOS_flag 0 (it is c_s^2(R)*rho). */
+   gas: 5*xres*yres*zres doubles (rho,p,u,v,w); dust: 4*xres*yres (rho,u,v,w)
+   or NULL.  On set, p is ignored under EOS_flag 0 (it is c_s^2(R)*rho). */
 int pg_get_state(pg_handle* h, double* gas, double* dust);
 int pg_set_state(pg_handle* h, const double* gas, const double* dust);
 
@@ -82,9 +87,12 @@ int pg_sweep_x(pg_handle* h, double dt);
 int pg_sweep_y(pg_handle* h, double dt);
 int pg_sweep_x_dust(pg_handle* h, double dt);
 int pg_sweep_y_dust(pg_handle* h, double dt);
+/* polar-angle sweep of a 3D disk (sweepz_inplace, sweeps.cu:458-553, after
+   the reflecting boundz of boundary.cu:270-321) */
+int pg_sweep_z(pg_handle* h, double dt);
 /* set_OrbAdv + shift_OrbAdv + advecty(_dust) (orbit.cu:122-157,
-   advection.cu:321-357); inc/res (xres each, may be NULL) receive the
-   integer shift and residual speed per radius */
+   advection.cu:321-357); inc/res (xres each, xres*zres in 3D, may be NULL)
+   receive the integer shift and residual speed per radius (and polar layer) */
 int pg_orbital_advection(pg_handle* h, double dt, int32_t* inc, double* res);
 /* killwave (killwave.cu:105-117) */
 int pg_killwave(pg_handle* h, double dt);
@@ -95,7 +103,8 @@ int pg_planet(pg_handle* h, double t, double* body10);
 /* ---- snapshots (the reference's checkpoint files) ------------------------ */
 /* Write the current state in the layout of write_check_point
    (output.cu:125-180): FP64 t, xa[xres+1], ya[yres+1], rho, p, u, v
-   (xres*yres each, x fastest), then with dust rho_d, u_d, v_d, w_d. */
+   (xres*yres each, x fastest), then with dust rho_d, u_d, v_d, w_d; 3D:
+   t, xa, ya, za[zres+1], rho, p, u, v, w. */
 int pg_write_snapshot(pg_handle* h, const char* filename, double t);
 /* Restart from such a file (load_grid, output.cu:182-248); unlike the
    reference this also restores the dust fields.  *t_out receives the time. */

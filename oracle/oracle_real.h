@@ -60,7 +60,7 @@ inline Dual mfmax(const Dual& a, const Dual& b) { return a.v >= b.v ? a : b; }  
 inline Dual mfmin(const Dual& a, const Dual& b) { return a.v <= b.v ? a : b; }
 inline Dual mexp(const Dual& x) { const double e = std::exp(x.v); return Dual(e, e * x.d); }
 inline Dual mlog(const Dual& x) { return Dual(std::log(x.v), x.d / x.v); }
-inline Dual mlogf(const Dual& x) { return Dual((double)logf((float)x.v), x.d / x.v); }
+inline Dual mlogf(const Dual& x) { return Dual((double)(float)std::log((double)(float)x.v), x.d / x.v); }
 inline Dual msin(const Dual& x) { return Dual(std::sin(x.v), std::cos(x.v) * x.d); }
 inline Dual mcos(const Dual& x) { return Dual(std::cos(x.v), -std::sin(x.v) * x.d); }
 inline Dual mtan(const Dual& x) { const double c = std::cos(x.v); return Dual(std::tan(x.v), x.d / (c * c)); }
@@ -101,7 +101,9 @@ inline double mfmax(double a, double b) { return std::fmax(a, b); }
 inline double mfmin(double a, double b) { return std::fmin(a, b); }
 inline double mexp(double x) { return std::exp(x); }
 inline double mlog(double x) { return std::log(x); }
-inline double mlogf(double x) { return (double)logf((float)x); }   // single-precision log of a float argument
+// the single-precision logarithm of a float argument, correctly rounded
+// (the double logarithm of the float, rounded to float)
+inline double mlogf(double x) { return (double)(float)std::log((double)(float)x); }
 inline double msin(double x) { return std::sin(x); }
 inline double mcos(double x) { return std::cos(x); }
 inline double mtan(double x) { return std::tan(x); }

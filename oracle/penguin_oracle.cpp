@@ -43,7 +43,8 @@
 //     advection.cu:186-192 vs PPM.cu:69-78); here it is 0, the convention the
 //     gas advection uses (advection.cu:12-18);
 //   * killwave's __sinf (killwave.cu:11) is sinf on a float argument, PEM's
-//     __logf (PEM.cu:13,29) logf on a float argument.
+//     __logf (PEM.cu:13,29) the correctly rounded float logarithm of a float
+//     argument (CUDA's approximation is not reproduced).
 
 #include "penguin_oracle.h"
 #include "oracle_real.h"
@@ -292,7 +293,7 @@ inline void recon_par(int i, int geom, const real_t* x, const real_t* a, const r
 
 // get_PEM_0 / get_PEM_1 (PEM.cu:3-48): power-law sub-cell profiles; the
 // logarithm is the single-precision one of a float argument (__logf,
-// PEM.cu:13,29).  get_PEM_1's x >= 1 assignment is overwritten by the
+// PEM.cu:13,29), here correctly rounded.  get_PEM_1's x >= 1 assignment is overwritten by the
 // branch that follows it (PEM.cu:23-27) and so has no effect here either.
 inline real_t pem0(real_t x, real_t S, real_t aB, real_t eta) {
     real_t v = x == 0.0 ? real_t(0.0) : mexp(eta * mlogf(x));

@@ -31,6 +31,7 @@
 #include "pg_device.h"
 #include "pg_internal.h"
 #include "pg_launch.h"
+#include "pg_sph.h"
 
 using namespace pg;
 
@@ -130,9 +131,12 @@ void make_axis(std::vector<double>& a, std::vector<double>& v, double amin, doub
             a[res + pad + 1 + i] = a[res + pad] + (a[res + pad] - a[res + pad - 1 - i]);
         }
     }
-    for (int i = 0; i < res + 2 * pad; ++i) {
+    for (int i = 0; i < res + 2 * pad; ++i) {   // get_volume (geom.cu:3-20)
         double ra = a[i], dr = a[i + 1] - a[i];
-        double dvdr = geom == 2 ? ra * ra + ra * dr + dr * dr / 3.0 : geom == 1 ? ra + dr / 2.0 : 1.0;
+        double dvdr = geom == 5   ? (dr < 1.0e-4 ? std::sin(ra + 0.5 * dr) : (std::cos(ra) - std::cos(ra + dr)) / dr)
+                      : geom == 2 ? ra * ra + ra * dr + dr * dr / 3.0
+                      : geom == 1 ? ra + dr / 2.0
+                                  : 1.0;
         v[i] = dr * dvdr;
     }
 }
@@ -359,6 +363,7 @@ struct AdjWork;
 struct pg_handle {
     Group g;
     AdjWork* adj = nullptr;   // adjoint workspace, allocated by the first adjoint call
+    pg::Sph* sph = nullptr;   // ndim 3: the spherical disk (pg_sph.hip) instead of g
 };
 struct pg_batch {
     Group g;
@@ -1647,8 +1652,30 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
 
 #define CHECK_H(h) \
     if (!(h)) return fail(PG_E_INVALID, "null handle")
+// entry points of the two-dimensional path only
+#define CHECK_2D(h) \
+    if ((h)->sph) return fail(PG_E_INVALID, "%s: not available for 3D disks", __func__)
 
 }  // namespace
+
+namespace pg {
+int report_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+void host_make_axis(double* a, double* v, double amin, double amax, int res, int pad, int geom, int grid) {
+    std::vector<double> va, vv;
+    make_axis(va, vv, amin, amax, res, pad, geom, grid);
+    std::copy(va.begin(), va.end(), a);
+    std::copy(vv.begin(), vv.end(), v);
+}
+void host_slope_weights(int geom, const double* x6, double wl[3], double wr[3]) { slope_weights(geom, x6, wl, wr); }
+}  // namespace pg
 
 // =================================================================== C ABI
 extern "C" {
@@ -1660,6 +1687,15 @@ int pg_create(const pg_params* params, pg_handle** out) {
     g_err.clear();
     if (!params || !out) return fail(PG_E_INVALID, "null argument");
     *out = nullptr;
+    if (params->ndim == 3) {   // the spherical disk
+        pg::Sph* s = nullptr;
+        const int rc = pg::sph_create(*params, &s);
+        if (rc) return rc;
+        pg_handle* h = new pg_handle();
+        h->sph = s;
+        *out = h;
+        return PG_OK;
+    }
     pg_handle* h = new pg_handle();
     int rc = group_init(h->g, 1, params);
     if (rc) {
@@ -1673,6 +1709,11 @@ int pg_create(const pg_params* params, pg_handle** out) {
 
 int pg_destroy(pg_handle* h) {
     if (!h) return PG_OK;
+    if (h->sph) {
+        pg::sph_destroy(h->sph);
+        delete h;
+        return PG_OK;
+    }
     if (h->g.stream) (void)hipStreamSynchronize(h->g.stream);
     adj_free(h->adj);
     group_release(h->g);
@@ -1682,17 +1723,20 @@ int pg_destroy(pg_handle* h) {
 
 int pg_get_state(pg_handle* h, double* gas, double* dust) {
     CHECK_H(h);
+    if (h->sph) return pg::sph_get_state(h->sph, gas);
     return disk_get_state(h->g, 0, gas, dust);
 }
 
 int pg_set_state(pg_handle* h, const double* gas, const double* dust) {
     CHECK_H(h);
+    if (h->sph) return pg::sph_set_state(h->sph, gas);
     return disk_set_state(h->g, 0, gas, dust);
 }
 
 int pg_global_dt(pg_handle* h, double dt_prev, double* dt_out) {
     CHECK_H(h);
     if (!dt_out) return fail(PG_E_INVALID, "null dt_out");
+    if (h->sph) return pg::sph_global_dt(h->sph, dt_prev, dt_out);
     Group& g = h->g;
     int rc;
     if ((rc = cfl_rate(g))) return rc;
@@ -1710,6 +1754,7 @@ int pg_global_dt(pg_handle* h, double dt_prev, double* dt_out) {
 int pg_solve(pg_handle* h, double t, double dt) {
     CHECK_H(h);
     if (!(dt > 0.0)) return fail(PG_E_INVALID, "dt must be positive");
+    if (h->sph) return pg::sph_solve(h->sph, t, dt);
     Group& g = h->g;
     int rc;
     if ((rc = pull_ctl(g))) return rc;
@@ -1727,11 +1772,16 @@ int pg_solve(pg_handle* h, double t, double dt) {
 
 int pg_run(pg_handle* h, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph) {
     CHECK_H(h);
+    if (h->sph) {
+        if (!t_io || !dt_io || nsteps < 0) return fail(PG_E_INVALID, "pg_run: null t/dt or negative steps");
+        return pg::sph_run(h->sph, nsteps, t_io, dt_io, dts);
+    }
     return group_run(h->g, nsteps, t_io, dt_io, dts, use_graph, nullptr);
 }
 
 int pg_viscosity_eval(pg_handle* h, int on_T, double* vis_out) {
     CHECK_H(h);
+    if (h->sph) return pg::sph_viscosity(h->sph, on_T, vis_out);
     Group& g = h->g;
     hipError_t e = g.prec == 8 ? Launch<double>::viscosity(g.ls<double>(), g.par(), on_T ? 1 : 0, g.stream)
                                : Launch<float>::viscosity(g.ls<float>(), g.par(), on_T ? 1 : 0, g.stream);
@@ -1749,6 +1799,10 @@ int pg_viscosity_eval(pg_handle* h, int on_T, double* vis_out) {
 
 int pg_planet(pg_handle* h, double t, double* body10) {
     CHECK_H(h);
+    if (h->sph) {
+        if (!body10) return PG_OK;
+        return pg::sph_planet(h->sph, t, body10);
+    }
     Group& g = h->g;
     int rc;
     if ((rc = pull_ctl(g))) return rc;
@@ -1775,6 +1829,7 @@ int pg_sources(pg_handle* h, double t_planet, double mdt, double dt, int inplace
             return fail(PG_E_INVALID, "mdt must be 0 or equal to dt (the two uses in DS)");
         mode = inplace ? 2 : 1;
     }
+    if (h->sph) return pg::sph_sources(h->sph, t_planet, mdt, dt, inplace);
     // the kernel takes its half step from the control block: dt_block = 2*dt
     int rc;
     if ((rc = pull_ctl(g))) return rc;
@@ -1833,11 +1888,34 @@ static int sweep_module(pg_handle* h, double dt, int which) {
     return PG_OK;
 }
 
-int pg_sweep_x(pg_handle* h, double dt) { return sweep_module(h, dt, 0); }
-int pg_sweep_x_dust(pg_handle* h, double dt) { return sweep_module(h, dt, 1); }
+int pg_sweep_x(pg_handle* h, double dt) {
+    CHECK_H(h);
+    if (h->sph) return pg::sph_sweep(h->sph, 0, dt);
+    return sweep_module(h, dt, 0);
+}
+int pg_sweep_x_dust(pg_handle* h, double dt) {
+    CHECK_H(h);
+    CHECK_2D(h);
+    return sweep_module(h, dt, 1);
+}
 // azimuthal sweep alone: the ring kernels with the orbital stages switched off
-int pg_sweep_y(pg_handle* h, double dt) { return sweep_module(h, dt, 2); }
-int pg_sweep_y_dust(pg_handle* h, double dt) { return sweep_module(h, dt, 3); }
+int pg_sweep_y(pg_handle* h, double dt) {
+    CHECK_H(h);
+    if (h->sph) return pg::sph_sweep(h->sph, 1, dt);
+    return sweep_module(h, dt, 2);
+}
+int pg_sweep_y_dust(pg_handle* h, double dt) {
+    CHECK_H(h);
+    CHECK_2D(h);
+    return sweep_module(h, dt, 3);
+}
+// polar-angle sweep (sweepz_inplace, sweeps.cu:458-553), 3D disks only
+int pg_sweep_z(pg_handle* h, double dt) {
+    CHECK_H(h);
+    if (!h->sph) return fail(PG_E_INVALID, "pg_sweep_z: the disk is two-dimensional");
+    if (!(dt > 0.0)) return fail(PG_E_INVALID, "dt must be positive");
+    return pg::sph_sweep(h->sph, 2, dt);
+}
 
 // set_OrbAdv + shift_OrbAdv + advecty(_dust) alone: the ring kernels with
 // their sweep stage switched off (each cell is carried unchanged to its
@@ -1845,6 +1923,7 @@ int pg_sweep_y_dust(pg_handle* h, double dt) { return sweep_module(h, dt, 3); }
 int pg_orbital_advection(pg_handle* h, double dt, int32_t* inc, double* res) {
     CHECK_H(h);
     if (!(dt > 0.0)) return fail(PG_E_INVALID, "dt must be positive");
+    if (h->sph) return pg::sph_orbital_advection(h->sph, dt, inc, res);
     Group& g = h->g;
     int rc = set_dt_and_prologue(g, dt, 1);
     if (rc) return rc;
@@ -1874,6 +1953,7 @@ int pg_orbital_advection(pg_handle* h, double dt, int32_t* inc, double* res) {
 
 int pg_killwave(pg_handle* h, double dt) {
     CHECK_H(h);
+    CHECK_2D(h);
     Group& g = h->g;
     if (!g.kill) return PG_OK;
     int rc = set_dt_and_prologue(g, dt, 0);
@@ -1964,6 +2044,7 @@ extern "C" {
 int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
     CHECK_H(h);
     if (!filename) return fail(PG_E_INVALID, "null filename");
+    if (h->sph) return pg::sph_write_snapshot(h->sph, filename, t);
     const DiskDev* d = h->g.disks[0];
     const size_t N = d->N;
     std::vector<double> gas(5 * N), dust(d->P.dust_flag ? 4 * N : 0);
@@ -1974,6 +2055,7 @@ int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
 
 int pg_write_average_snapshot(pg_handle* h, const char* filename, double t) {
     CHECK_H(h);
+    CHECK_2D(h);
     if (!filename) return fail(PG_E_INVALID, "null filename");
     const DiskDev* d = h->g.disks[0];
     const size_t N = d->N;
@@ -1987,6 +2069,7 @@ int pg_write_average_snapshot(pg_handle* h, const char* filename, double t) {
 
 int pg_average_begin(pg_handle* h, double interval) {
     CHECK_H(h);
+    CHECK_2D(h);
     if (!(interval > 0.0)) return fail(PG_E_INVALID, "interval must be > 0");
     Group& g = h->g;
     DiskDev* d = g.disks[0];
@@ -2004,6 +2087,7 @@ int pg_average_begin(pg_handle* h, double interval) {
 
 int pg_average_end(pg_handle* h) {
     CHECK_H(h);
+    CHECK_2D(h);
     h->g.ave = false;
     h->g.drop_graph();
     return PG_OK;
@@ -2011,12 +2095,14 @@ int pg_average_end(pg_handle* h) {
 
 int pg_average_get(pg_handle* h, double* gas) {
     CHECK_H(h);
+    CHECK_2D(h);
     if (!gas) return fail(PG_E_INVALID, "null argument");
     return average_get(h, gas);
 }
 
 int pg_torque(pg_handle* h, double* torque) {
     CHECK_H(h);
+    CHECK_2D(h);
     if (!torque) return fail(PG_E_INVALID, "null argument");
     Group& g = h->g;
     DiskDev* d = g.disks[0];
@@ -2040,6 +2126,7 @@ int pg_torque(pg_handle* h, double* torque) {
 int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out) {
     CHECK_H(h);
     if (!filename) return fail(PG_E_INVALID, "null filename");
+    if (h->sph) return pg::sph_read_snapshot(h->sph, filename, t_out);
     const DiskDev* d = h->g.disks[0];
     const size_t N = d->N;
     const size_t nfield = 4 + (d->P.dust_flag ? 4 : 0);
@@ -2073,12 +2160,13 @@ int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out) {
 int pg_stream(pg_handle* h, void** stream) {
     CHECK_H(h);
     if (!stream) return fail(PG_E_INVALID, "null argument");
-    *stream = (void*)h->g.stream;
+    *stream = h->sph ? pg::sph_stream(h->sph) : (void*)h->g.stream;
     return PG_OK;
 }
 
 int pg_profile_enable(pg_handle* h, int on) {
     CHECK_H(h);
+    if (h->sph) return pg::sph_profile_enable(h->sph, on);
     h->g.profile = on != 0;
     h->g.timers.clear();
     h->g.ev_used = 0;
@@ -2087,12 +2175,14 @@ int pg_profile_enable(pg_handle* h, int on) {
 
 int pg_profile_get(pg_handle* h, const char* kernel, double* avg_us, int* count) {
     CHECK_H(h);
+    if (h->sph) return pg::sph_profile_get(h->sph, kernel, avg_us, count);
     return group_profile_get(h->g, kernel, avg_us, count);
 }
 
 int pg_device_bytes(pg_handle* h, size_t* bytes) {
     CHECK_H(h);
     if (!bytes) return fail(PG_E_INVALID, "null argument");
+    if (h->sph) return pg::sph_device_bytes(h->sph, bytes);
     *bytes = h->g.bytes;
     return PG_OK;
 }
@@ -2100,6 +2190,7 @@ int pg_device_bytes(pg_handle* h, size_t* bytes) {
 // ---- adjoint ---------------------------------------------------------------
 int pg_adjoint_forward(pg_handle* h, int nsteps, int ckpt_every, double t0, double dt0, double* dts) {
     CHECK_H(h);
+    CHECK_2D(h);
     if (nsteps < 1 || ckpt_every < 1) return fail(PG_E_INVALID, "nsteps and ckpt_every must be >= 1");
     int rc = adj_check_config(h->g.disks[0]->P);
     if (rc) return rc;
@@ -2109,6 +2200,7 @@ int pg_adjoint_forward(pg_handle* h, int nsteps, int ckpt_every, double t0, doub
 
 int pg_adjoint_backward(pg_handle* h, const double* dL_dstate, pg_grads* grads) {
     CHECK_H(h);
+    CHECK_2D(h);
     if (!dL_dstate || !grads) return fail(PG_E_INVALID, "null argument");
     if (!h->adj || !h->adj->forward_done) return fail(PG_E_STATE, "pg_adjoint_forward has not run");
     return h->g.prec == 8 ? adj_backward<double>(h, dL_dstate, grads) : adj_backward<float>(h, dL_dstate, grads);
@@ -2197,6 +2289,7 @@ extern "C" {
 int pg_adjoint_pass(pg_handle* h, int pass, double t, double dt, const double* lam_out, double* lam_in,
                     double* dparam) {
     CHECK_H(h);
+    CHECK_2D(h);
     if (pass < 0 || pass > 6) return fail(PG_E_INVALID, "pass %d: use 0..6", pass);
     if (!lam_out || !lam_in) return fail(PG_E_INVALID, "null argument");
     int rc = adj_check_config(h->g.disks[0]->P);

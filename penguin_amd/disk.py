@@ -38,6 +38,7 @@ SIGNATURES = {
     "pg_sources": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "pg_sweep_x": (ctypes.c_int, [_vp, ctypes.c_double]),
     "pg_sweep_y": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "pg_sweep_z": (ctypes.c_int, [_vp, ctypes.c_double]),
     "pg_sweep_x_dust": (ctypes.c_int, [_vp, ctypes.c_double]),
     "pg_sweep_y_dust": (ctypes.c_int, [_vp, ctypes.c_double]),
     "pg_orbital_advection": (ctypes.c_int, [_vp, ctypes.c_double, _ip, _dp]),
@@ -123,7 +124,9 @@ class Disk:
         h = _vp()
         _check(L.pg_create(ctypes.byref(self.p), ctypes.byref(h)))
         self._h = h
-        self.nx, self.ny = int(p.xres), int(p.yres)
+        self.nx, self.ny, self.nz = int(p.xres), int(p.yres), int(p.zres)
+        # interior arrays, x fastest: (ny, nx) in 2D, (nz, ny, nx) for a 3D disk
+        self.shape = (self.nz, self.ny, self.nx) if int(p.ndim) == 3 else (self.ny, self.nx)
         self.t = 0.0
         self.dt = float(p.end_time)   # main.cu:144: the first cap is end_time
 
@@ -147,8 +150,8 @@ class Disk:
     # ---- state -----------------------------------------------------------
     def get_state(self):
         """gas (5, ny, nx): rho, p, u, v, w; dust (4, ny, nx) or None."""
-        gas = np.zeros((5, self.ny, self.nx))
-        dust = np.zeros((4, self.ny, self.nx)) if self.p.dust_flag else None
+        gas = np.zeros((5,) + self.shape)
+        dust = np.zeros((4,) + self.shape) if self.p.dust_flag else None
         _check(load_library().pg_get_state(self._h, _ptr(gas), _ptr(dust) if dust is not None else None))
         return gas, dust
 
@@ -189,7 +192,7 @@ class Disk:
 
     # ---- module-level entry points --------------------------------------
     def viscosity(self, on_T: bool = False) -> np.ndarray:
-        v = np.zeros((2, self.ny, self.nx))
+        v = np.zeros((2,) + self.shape)
         _check(load_library().pg_viscosity_eval(self._h, int(on_T), _ptr(v)))
         return v
 
@@ -208,11 +211,17 @@ class Disk:
     def sweep_y_dust(self, dt: float):
         _check(load_library().pg_sweep_y_dust(self._h, dt))
 
+    def sweep_z(self, dt: float):
+        """polar-angle sweep of a 3D disk"""
+        _check(load_library().pg_sweep_z(self._h, dt))
+
     def orbital_advection(self, dt: float):
-        inc = np.zeros(self.nx, dtype=np.int32)
-        res = np.zeros(self.nx)
+        """integer shift and residual speed per radius ((nz, nx) in 3D)"""
+        n = self.nz if len(self.shape) == 3 else 1
+        inc = np.zeros((n, self.nx), dtype=np.int32)
+        res = np.zeros((n, self.nx))
         _check(load_library().pg_orbital_advection(self._h, dt, inc.ctypes.data_as(_ip), _ptr(res)))
-        return inc, res
+        return (inc, res) if n > 1 or len(self.shape) == 3 else (inc[0], res[0])
 
     def killwave(self, dt: float):
         _check(load_library().pg_killwave(self._h, dt))

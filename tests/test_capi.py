@@ -69,6 +69,15 @@ def test_invalid_configuration_is_rejected():
     with pytest.raises(PenguinError):
         from penguin_amd import Disk
         Disk(config_c2(xres=64, yres=64).replace(ndim=3))
+    # 3D: what the spherical path does not cover is refused before any device work
+    from penguin_amd import disk3d
+    for bad in (dict(dust_flag=1), dict(kill_flag=1), dict(recon_flag=4), dict(EOS_flag=0), dict(gam=1.0),
+                dict(OrbAdv_flag=0), dict(bound_bom=3), dict(bound_top=1), dict(zres=200), dict(geomz=3),
+                dict(gridz=1), dict(zmin=-0.1)):
+        p = disk3d(32, 32, 8).replace(**bad)
+        rc = L.pg_create(ctypes.byref(p), ctypes.byref(h))
+        assert rc == -1, bad
+        assert "3D" in L.pg_last_error().decode() or "polar" in L.pg_last_error().decode(), bad
 
 
 def test_null_handle_is_an_error_not_a_crash():
