@@ -66,6 +66,7 @@ def parse():
     ap.add_argument("--batch-disks", type=int, default=64, help="disks of the batched leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the adjoint leg (0 skips it)")
+    ap.add_argument("--sph-steps", type=int, default=20, help="steps of the 3D disk leg (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -109,6 +110,64 @@ def adjoint_leg(steps: int, precision: int) -> dict:
             "forward_s": round(t_fwd, 4), "forward_with_checkpoints_s": round(t_afwd, 4),
             "backward_s": round(t_bwd, 4), "cost_ratio": round(t_bwd / t_fwd, 3),
             "loss": loss, "dL_dq": dq, "dL_dalpha": dal, "dL_dsigma0_norm": float(np.linalg.norm(d0[0]))}
+
+
+# words (reals) each 3D kernel reads and writes per interior cell: the sweeps
+# and the orbital advection read and write the five fields; the viscosity
+# reads rho, u, vp and writes two components; the sources read five fields
+# and two viscous components and write three velocities (in place) or five
+SPH_WORDS = {"sweep_x_gas": 10, "sweep_y_gas": 10, "sweep_z_gas": 10, "orbital_advection": 10,
+             "viscosity": 5, "sources": 10, "cfl": 5}
+
+
+def sph_leg(steps: int, precision: int, cpu: bool) -> dict:
+    """The 3D adiabatic disk (384 x 1536 x 64, PEM4, viscosity, planet,
+    orbital advection): `steps` of the host-driven loop (CFL dt read back
+    every step, as the reference's global_dt does), then a short profiled run
+    for the per-kernel times."""
+    import numpy as np
+    from penguin_amd import Disk, config_c3
+
+    p = config_c3(precision=precision)
+    cells = p.xres * p.yres * p.zres
+    d = Disk(p)
+    d.run(2)
+    t0 = time.perf_counter()
+    dts = d.run(steps)
+    el = time.perf_counter() - t0
+    d.profile(True)
+    d.run(3)
+    kernels = {}
+    for name, words in SPH_WORDS.items():
+        avg_us, cnt = d.kernel_time_us(name)
+        if cnt:
+            kernels[name] = {"avg_us": round(avg_us, 2), "launches_per_step": cnt // 3,
+                             "GBps": round(words * precision * cells / (avg_us * 1e-6) / 1e9, 1)}
+    d.profile(False)
+    d.close()
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches_per_step"])
+    out = {"workload": "3D adiabatic disk 384x1536x64 (r, phi, theta): PEM4 + HLLC with the internal-energy "
+                       "equation, polar sweep with reflecting walls, orbital advection, viscosity, planet "
+                       "(q 1e-4, spline softening), host CFL dt",
+           "value": round(cells * steps / el, 1), "unit": "cell-updates/s", "steps": steps,
+           "ms_per_step": round(1e3 * el / steps, 3), "mean_dt": float(np.mean(dts)),
+           "kernels": kernels,
+           "roofline": {"bound": "hbm", "kernel": dom,
+                        "achieved": kernels[dom]["GBps"], "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                        "frac": round(kernels[dom]["GBps"] * 1e9 / PEAK_HBM, 4),
+                        "bytes_per_launch": SPH_WORDS[dom] * precision * cells}}
+    if cpu:
+        import oracle
+        oracle.build()
+        q = config_c3(precision=8, xres=96, yres=384, zres=64)
+        o = oracle.Oracle(q)
+        c0 = time.perf_counter()
+        o.run(2)
+        cdt = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": round(96 * 384 * 64 * 2 / cdt, 1), "unit": "cell-updates/s",
+                               "cores": oracle.num_threads(), "kind": "port",
+                               "sample": f"2 steps of a 96x384x64 3D disk (FP64 oracle, {cdt:.1f} s)"}
+    return out
 
 
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "c2_gas_dust_1024x3072_hbm_counters.md")
@@ -238,6 +297,7 @@ def main():
 
     batched = batched_leg(a.batch_disks, a.batch_steps, a.precision) if a.batch_disks > 0 else None
     adjoint = adjoint_leg(a.adjoint_steps, a.precision) if a.adjoint_steps > 0 else None
+    sph = sph_leg(a.sph_steps, a.precision, not a.no_cpu) if a.sph_steps > 0 else None
 
     cpu = None
     if not a.no_cpu and a.cpu_steps > 0:
@@ -285,6 +345,8 @@ def main():
         out["batched"] = batched
     if adjoint is not None:
         out["adjoint"] = adjoint
+    if sph is not None:
+        out["sph"] = sph
     print(json.dumps(out))
 
 
