@@ -49,11 +49,12 @@ namespace pg {
 namespace {
 
 constexpr int NF = 5;            // rho, p, u, vp, w
-constexpr int SPH_BT = 256;      // threads per sweep block
+constexpr int SPH_BT = 256;      // threads per block of the pointwise kernels and the radial sweep
+constexpr int SPH_BT2 = 512;     // threads per block of the strided sweeps (about one element each)
 constexpr int SPH_TX = 256;      // radial segment per block
 constexpr int SPH_R = 8;         // rows per strided-sweep block
 constexpr int SPH_TY = 64;       // azimuthal segment per block
-constexpr int SPH_ZMAX = 100;    // polar cells: the whole column is one tile (FP64: 149 KB of LDS)
+constexpr int SPH_ZMAX = 100;    // polar cells: the whole column is one tile (FP64: 102 KB of LDS)
 
 template <typename real>
 struct SphK {
@@ -109,9 +110,15 @@ template <> __device__ __forceinline__ double flog<double>(double x) { return (d
 __device__ __forceinline__ void sincos_r(float x, float* s, float* c) { sincosf(x, s, c); }
 __device__ __forceinline__ void sincos_r(double x, double* s, double* c) { sincos(x, s, c); }
 
+// Parameters of a cell's sub-cell profile: slopes, mean, and (PEM4) which
+// form each of get_CON_aveL / aveR takes (PEM.cu:51-93), decided once per
+// cell and field since it depends on the slopes only: k = 0 the mean,
+// 1 the parabola, 2 get_PEM_0, 3 get_PEM_1, with exponents eL, eR.
 template <typename real>
 struct Par {
     real sL, a, sR;
+    real eL, eR;
+    int kL, kR;
 };
 
 // get_PEM_0 / get_PEM_1 (PEM.cu:3-48)
@@ -138,25 +145,34 @@ __device__ __forceinline__ real pem1(real x, real S, real aB, real eta) {
     return aB + S * v;
 }
 
+// the PEM4 branch choices of get_PEM_aveL / aveR (PEM.cu:51-93)
+template <typename real>
+__device__ __forceinline__ void pem_select(Par<real>& p) {
+    if (p.sL * p.sR == real(0)) {
+        p.kL = p.kR = 0;
+        p.eL = p.eR = real(0);
+        return;
+    }
+    const real rr = FM<real>::div(p.sR, p.sL), rl = FM<real>::div(p.sL, p.sR);
+    p.kL = rr >= real(1) ? (rr <= real(2) ? 1 : 2) : (rl <= real(2) ? 1 : 3);
+    p.eL = rr >= real(1) ? rr : rl;
+    p.kR = rr > real(1) ? (rr <= real(2) ? 1 : 3) : (rl <= real(2) ? 1 : 2);
+    p.eR = rr > real(1) ? rr : rl;
+}
+
 // get_CON_aveL (recon.cu:424-441): mean over the first fraction q
 template <int RECON, typename real>
 __device__ __forceinline__ real avgL(real q, const Par<real>& p) {
-    if (RECON == 6) return aveL(q, p.sL, p.a, p.sR);
-    if (p.sL * p.sR == real(0)) return p.a;
-    const real rr = FM<real>::div(p.sR, p.sL);
-    if (rr >= real(1)) return rr <= real(2) ? aveL(q, p.sL, p.a, p.sR) : pem0(q, p.sL, p.a - p.sL, rr);
-    const real rl = FM<real>::div(p.sL, p.sR);
-    return rl <= real(2) ? aveL(q, p.sL, p.a, p.sR) : pem1(q, p.sL, p.a - p.sL, rl);
+    if (RECON == 6 || p.kL == 1) return aveL(q, p.sL, p.a, p.sR);
+    if (p.kL == 0) return p.a;
+    return p.kL == 2 ? pem0(q, p.sL, p.a - p.sL, p.eL) : pem1(q, p.sL, p.a - p.sL, p.eL);
 }
 // get_CON_aveR (recon.cu:405-422): mean over the cell from q to its right edge
 template <int RECON, typename real>
 __device__ __forceinline__ real avgR(real q, const Par<real>& p) {
-    if (RECON == 6) return aveR(real(1) - q, p.sL, p.a, p.sR);
-    if (p.sR * p.sL == real(0)) return p.a;
-    const real rr = FM<real>::div(p.sR, p.sL);
-    if (rr > real(1)) return rr <= real(2) ? aveR(real(1) - q, p.sL, p.a, p.sR) : pem1(real(1) - q, -p.sR, p.a + p.sR, rr);
-    const real rl = FM<real>::div(p.sL, p.sR);
-    return rl <= real(2) ? aveR(real(1) - q, p.sL, p.a, p.sR) : pem0(real(1) - q, -p.sR, p.a + p.sR, rl);
+    if (RECON == 6 || p.kR == 1) return aveR(real(1) - q, p.sL, p.a, p.sR);
+    if (p.kR == 0) return p.a;
+    return p.kR == 3 ? pem1(real(1) - q, -p.sR, p.a + p.sR, p.eR) : pem0(real(1) - q, -p.sR, p.a + p.sR, p.eR);
 }
 
 // 4th-order limited slopes (recon.cu:216-378); under RECON 6 also the
@@ -167,6 +183,8 @@ __device__ __forceinline__ Par<real> params(real am2, real am1, real a0, real ap
     p.a = a0;
     if (RECON == 6) {
         slopes(am2, am1, a0, ap1, ap2, w[0], w[1], w[2], w[3], w[4], w[5], flat, p.sL, p.sR);
+        p.kL = p.kR = 1;
+        p.eL = p.eR = real(0);
         return p;
     }
     real d0 = am1 - am2, d1 = a0 - am1, d2 = ap1 - a0, d3 = ap2 - ap1;
@@ -180,6 +198,7 @@ __device__ __forceinline__ Par<real> params(real am2, real am1, real a0, real ap
     }
     p.sL = l;
     p.sR = r;
+    pem_select(p);
     return p;
 }
 
@@ -297,6 +316,15 @@ __device__ __forceinline__ void update_cell(real* c, const real* D, real gam) {
     c[0] = rn;
 }
 
+// Workgroups are dealt to the 8 XCDs round robin (block b runs on XCD b % 8),
+// each XCD with its own L2.  Renumbering so that XCD x takes a contiguous
+// range of tiles keeps tiles that share cache lines (neighbouring radii of
+// the strided sweeps) in one L2: a bijection of [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, n = b >> 3;
+    return x * q + min(x, r) + n;
+}
+
 __device__ __forceinline__ int wrapi(int j, int n) {
     j %= n;
     return j < 0 ? j + n : j;
@@ -306,43 +334,46 @@ __device__ __forceinline__ int wrapi(int j, int n) {
 // DIR 0: radius (rows (j, k), segments of SPH_TX cells); 1: azimuth (R radii
 // x SPH_TY cells per block); 2: polar angle (R radii x the whole column).
 template <typename real, int DIR, int RECON>
-__global__ __launch_bounds__(SPH_BT) void k3_sweep(SphK<real> a, real dt) {
+__global__ __launch_bounds__(DIR == 0 ? SPH_BT : SPH_BT2) void k3_sweep(SphK<real> a, real dt) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     real* sm = reinterpret_cast<real*>(sm_raw);
     constexpr int R = DIR == 0 ? 1 : SPH_R;
+    constexpr int BT = DIR == 0 ? SPH_BT : SPH_BT2;
     const int L = DIR == 0 ? SPH_TX : DIR == 1 ? SPH_TY : a.nz;
     const int S = L + 2 * NPAD;
-    const int RS = R * S;
+    const int RS = R * S;         // at most 2 BT (checked by the host)
     real* raw = sm;              // [5][RS]
     real* TLs = raw + NF * RS;   // state left of each cell's right edge
     real* TRs = TLs + NF * RS;   // state right of each cell's left edge
-    real* FX = TRs + NF * RS;    // per interface: flux[5], p*, u*
+    // per interface, once the traced states are consumed: flux[5] in TLs,
+    // p* and u* in TRs
     const int t = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const size_t N = a.N;
 
     // tile of this block
     int i0 = 0, j0 = 0, k0 = 0;   // first interior cell of the segment / first row
+    const int bt = xcd_tile(blockIdx.x, gridDim.x);
     if (DIR == 0) {
         const int ntx = (nx + SPH_TX - 1) / SPH_TX;
-        const int row = blockIdx.x / ntx;
-        i0 = (blockIdx.x - row * ntx) * SPH_TX;
+        const int row = bt / ntx;
+        i0 = (bt - row * ntx) * SPH_TX;
         j0 = row % ny;
         k0 = row / ny;
     } else if (DIR == 1) {
         const int nty = (ny + SPH_TY - 1) / SPH_TY, nig = (nx + R - 1) / R;
-        const int ty = blockIdx.x % nty, rest = blockIdx.x / nty;
-        i0 = (rest % nig) * R;
-        k0 = rest / nig;
-        j0 = ty * SPH_TY;
+        const int rest = bt / nig;
+        i0 = (bt - rest * nig) * R;
+        j0 = (rest % nty) * SPH_TY;
+        k0 = rest / nty;
     } else {
         const int nig = (nx + R - 1) / R;
-        i0 = (blockIdx.x % nig) * R;
-        j0 = blockIdx.x / nig;
+        i0 = (bt % nig) * R;
+        j0 = bt / nig;
     }
 
     // ---- load (with the sweep's field rotation and angular momenta)
-    for (int e = t; e < RS; e += SPH_BT) {
+    for (int e = t; e < RS; e += BT) {
         const int s = e / R, r = e - (e / R) * R;
         real v[NF] = {real(1), real(1), real(0), real(0), real(0)};
         if (DIR == 0) {
@@ -393,7 +424,7 @@ __global__ __launch_bounds__(SPH_BT) void k3_sweep(SphK<real> a, real dt) {
     __syncthreads();
 
     // ---- per cell: parameters and the two traced states
-    for (int e = t; e < RS; e += SPH_BT) {
+    for (int e = t; e < RS; e += BT) {
         const int s = e / R, r = e - (e / R) * R;
         if (s < 2 || s > S - 3) continue;
         real idx, dts;
@@ -443,25 +474,33 @@ __global__ __launch_bounds__(SPH_BT) void k3_sweep(SphK<real> a, real dt) {
     }
     __syncthreads();
 
-    // ---- per interface s (between cells s-1 and s): HLLC
-    for (int e = t; e < RS; e += SPH_BT) {
-        const int s = e / R;
-        if (s < NPAD || s > S - NPAD) continue;
-        if (DIR == 0 && i0 + s > nx + 2 * NPAD - 3) continue;
-        real Ls[NF], Rs[NF], F[NF], pres, uprs;
+    // ---- per interface s (between cells s-1 and s): HLLC; every thread
+    // reads its traced states before any flux overwrites them
+    real Fr[2][NF + 2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int e = t + m * BT, s = e / R;
+        if (e >= RS || s < NPAD || s > S - NPAD || (DIR == 0 && i0 + s > nx + 2 * NPAD - 3)) continue;
+        real Ls[NF], Rs[NF];
         for (int f = 0; f < NF; ++f) {
             Ls[f] = TLs[f * RS + e - R];
             Rs[f] = TRs[f * RS + e];
         }
-        hllc_adi(Ls, Rs, a.gam, F, pres, uprs);
-        for (int f = 0; f < NF; ++f) FX[f * RS + e] = F[f];
-        FX[5 * RS + e] = pres;
-        FX[6 * RS + e] = uprs;
+        hllc_adi(Ls, Rs, a.gam, Fr[m], Fr[m][NF], Fr[m][NF + 1]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int e = t + m * BT, s = e / R;
+        if (e >= RS || s < NPAD || s > S - NPAD || (DIR == 0 && i0 + s > nx + 2 * NPAD - 3)) continue;
+        for (int f = 0; f < NF; ++f) TLs[f * RS + e] = Fr[m][f];
+        TRs[e] = Fr[m][NF];
+        TRs[RS + e] = Fr[m][NF + 1];
     }
     __syncthreads();
 
     // ---- per cell: flux and source differences (net_source, riemann.cu:40-87), update
-    for (int e = t; e < RS; e += SPH_BT) {
+    for (int e = t; e < RS; e += BT) {
         const int s = e / R, r = e - (e / R) * R;
         if (s < NPAD || s >= S - NPAD) continue;
         int i, j, k;
@@ -484,11 +523,11 @@ __global__ __launch_bounds__(SPH_BT) void k3_sweep(SphK<real> a, real dt) {
             g1 = a.sza[s + 1];
             fac = FM<real>::div(dt, a.zv[s] * a.xc[i + NPAD]);
         }
-        const real* F0 = FX + e;
-        const real* F1 = FX + e + R;
-        const real p0s = F0[5 * RS], p1s = F1[5 * RS];
+        const real* F0 = TLs + e;
+        const real* F1 = TLs + e + R;
+        const real p0s = TRs[e], p1s = TRs[e + R];
         const real du = real(0.5) * (g0 + g1) * (p0s - p1s);
-        const real de = real(0.5) * (p0s + p1s) * (F0[6 * RS] * g0 - F1[6 * RS] * g1);
+        const real de = real(0.5) * (p0s + p1s) * (TRs[RS + e] * g0 - TRs[RS + e + R] * g1);
         real D[NF];
         D[0] = (F0[0] * g0 - F1[0] * g1) * fac;
         D[1] = (F0[RS] * g0 - F1[RS] * g1 + de) * fac;
@@ -527,7 +566,7 @@ __global__ __launch_bounds__(SPH_BT) void k3_sweep(SphK<real> a, real dt) {
 // (advection.cu:1-173) in one pass: each row (i, k) is read shifted by its
 // integer increment and its residual is transported conservatively.
 template <typename real, int RECON>
-__global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int* inc_out, double* res_out) {
+__global__ __launch_bounds__(SPH_BT2) void k3_orbadv(SphK<real> a, double dt, int* inc_out, double* res_out) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     real* sm = reinterpret_cast<real*>(sm_raw);
     constexpr int R = SPH_R;
@@ -540,8 +579,9 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
     const int nx = a.nx, ny = a.ny;
     const size_t N = a.N;
     const int nty = (ny + SPH_TY - 1) / SPH_TY, nig = (nx + R - 1) / R;
-    const int ty = blockIdx.x % nty, rest = blockIdx.x / nty;
-    const int i0 = (rest % nig) * R, k0 = rest / nig, j0 = ty * SPH_TY;
+    const int bt = xcd_tile(blockIdx.x, gridDim.x);
+    const int rest = bt / nig, ty = rest % nty;
+    const int i0 = (bt - rest * nig) * R, k0 = rest / nty, j0 = ty * SPH_TY;
     if (t < R) {
         const int i = min(i0 + t, nx - 1);
         const size_t ik = (size_t)i + (size_t)nx * k0;
@@ -557,7 +597,7 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
         }
     }
     __syncthreads();
-    for (int e = t; e < RS; e += SPH_BT) {
+    for (int e = t; e < RS; e += SPH_BT2) {
         const int s = e / R, r = e - (e / R) * R;
         const int i = i0 + r;
         real v[NF] = {real(1), real(1), real(0), real(0), real(0)};
@@ -571,7 +611,7 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
     }
     __syncthreads();
     // fluxes at interfaces s in [3, S-3]: upwind sub-cell means times the speed
-    for (int e = t; e < RS; e += SPH_BT) {
+    for (int e = t; e < RS; e += SPH_BT2) {
         const int s = e / R, r = e - (e / R) * R;
         if (s < NPAD || s > S - NPAD) continue;
         const int i = min(i0 + r, nx - 1);
@@ -592,7 +632,7 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
         for (int f = 2; f < NF; ++f) FX[f * RS + e] = D[f] * D[0] * speed;
     }
     __syncthreads();
-    for (int e = t; e < RS; e += SPH_BT) {
+    for (int e = t; e < RS; e += SPH_BT2) {
         const int s = e / R, r = e - (e / R) * R;
         const int i = i0 + r, j = j0 + s - NPAD;
         if (s < NPAD || s >= S - NPAD || i >= nx || j >= ny) continue;
@@ -892,7 +932,7 @@ template <typename real>
 size_t sweep_lds(int dir, int nz) {
     const int R = dir == 0 ? 1 : SPH_R;
     const int L = dir == 0 ? SPH_TX : dir == 1 ? SPH_TY : nz;
-    return (size_t)(3 * NF + 7) * R * (L + 2 * NPAD) * sizeof(real);
+    return (size_t)(3 * NF) * R * (L + 2 * NPAD) * sizeof(real);
 }
 
 // launch timing (profiling): events around one launch, synchronous
@@ -941,12 +981,13 @@ int launch_sweep(Sph* s, int dir, int in, int out, double dt) {
     const bool pem = s->P.recon_flag == 5;
 #define SPH_SWEEP(D)                                                                                   \
     do {                                                                                               \
+        const int bt = D == 0 ? SPH_BT : SPH_BT2;                                                      \
         if (pem) {                                                                                     \
             SPH_TRY(allow_lds(k3_sweep<real, D, 5>, lds));                                             \
-            hipLaunchKernelGGL((k3_sweep<real, D, 5>), dim3(nb), dim3(SPH_BT), lds, s->stream, a, (real)dt); \
+            hipLaunchKernelGGL((k3_sweep<real, D, 5>), dim3(nb), dim3(bt), lds, s->stream, a, (real)dt); \
         } else {                                                                                       \
             SPH_TRY(allow_lds(k3_sweep<real, D, 6>, lds));                                             \
-            hipLaunchKernelGGL((k3_sweep<real, D, 6>), dim3(nb), dim3(SPH_BT), lds, s->stream, a, (real)dt); \
+            hipLaunchKernelGGL((k3_sweep<real, D, 6>), dim3(nb), dim3(bt), lds, s->stream, a, (real)dt); \
         }                                                                                              \
     } while (0)
     if (dir == 0) SPH_SWEEP(0);
@@ -966,9 +1007,9 @@ int launch_orbadv(Sph* s, int in, int out, double dt, int* inc_d, double* res_d)
     SPH_TRY(allow_lds(k3_orbadv<real, 6>, lds));
     Timed tm(s, "orbital_advection");
     if (s->P.recon_flag == 5)
-        hipLaunchKernelGGL((k3_orbadv<real, 5>), dim3(nb), dim3(SPH_BT), lds, s->stream, a, dt, inc_d, res_d);
+        hipLaunchKernelGGL((k3_orbadv<real, 5>), dim3(nb), dim3(SPH_BT2), lds, s->stream, a, dt, inc_d, res_d);
     else
-        hipLaunchKernelGGL((k3_orbadv<real, 6>), dim3(nb), dim3(SPH_BT), lds, s->stream, a, dt, inc_d, res_d);
+        hipLaunchKernelGGL((k3_orbadv<real, 6>), dim3(nb), dim3(SPH_BT2), lds, s->stream, a, dt, inc_d, res_d);
     SPH_TRY(hipGetLastError());
     return PG_OK;
 }
