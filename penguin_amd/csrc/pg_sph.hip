@@ -566,7 +566,7 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BT : SPH_BT2) void k3_sweep(SphK<rea
 // (advection.cu:1-173) in one pass: each row (i, k) is read shifted by its
 // integer increment and its residual is transported conservatively.
 template <typename real, int RECON>
-__global__ __launch_bounds__(SPH_BT2) void k3_orbadv(SphK<real> a, double dt, int* inc_out, double* res_out) {
+__global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int* inc_out, double* res_out) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     real* sm = reinterpret_cast<real*>(sm_raw);
     constexpr int R = SPH_R;
@@ -597,7 +597,7 @@ __global__ __launch_bounds__(SPH_BT2) void k3_orbadv(SphK<real> a, double dt, in
         }
     }
     __syncthreads();
-    for (int e = t; e < RS; e += SPH_BT2) {
+    for (int e = t; e < RS; e += SPH_BT) {
         const int s = e / R, r = e - (e / R) * R;
         const int i = i0 + r;
         real v[NF] = {real(1), real(1), real(0), real(0), real(0)};
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(SPH_BT2) void k3_orbadv(SphK<real> a, double dt, in
     }
     __syncthreads();
     // fluxes at interfaces s in [3, S-3]: upwind sub-cell means times the speed
-    for (int e = t; e < RS; e += SPH_BT2) {
+    for (int e = t; e < RS; e += SPH_BT) {
         const int s = e / R, r = e - (e / R) * R;
         if (s < NPAD || s > S - NPAD) continue;
         const int i = min(i0 + r, nx - 1);
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(SPH_BT2) void k3_orbadv(SphK<real> a, double dt, in
         for (int f = 2; f < NF; ++f) FX[f * RS + e] = D[f] * D[0] * speed;
     }
     __syncthreads();
-    for (int e = t; e < RS; e += SPH_BT2) {
+    for (int e = t; e < RS; e += SPH_BT) {
         const int s = e / R, r = e - (e / R) * R;
         const int i = i0 + r, j = j0 + s - NPAD;
         if (s < NPAD || s >= S - NPAD || i >= nx || j >= ny) continue;
@@ -1007,9 +1007,9 @@ int launch_orbadv(Sph* s, int in, int out, double dt, int* inc_d, double* res_d)
     SPH_TRY(allow_lds(k3_orbadv<real, 6>, lds));
     Timed tm(s, "orbital_advection");
     if (s->P.recon_flag == 5)
-        hipLaunchKernelGGL((k3_orbadv<real, 5>), dim3(nb), dim3(SPH_BT2), lds, s->stream, a, dt, inc_d, res_d);
+        hipLaunchKernelGGL((k3_orbadv<real, 5>), dim3(nb), dim3(SPH_BT), lds, s->stream, a, dt, inc_d, res_d);
     else
-        hipLaunchKernelGGL((k3_orbadv<real, 6>), dim3(nb), dim3(SPH_BT2), lds, s->stream, a, dt, inc_d, res_d);
+        hipLaunchKernelGGL((k3_orbadv<real, 6>), dim3(nb), dim3(SPH_BT), lds, s->stream, a, dt, inc_d, res_d);
     SPH_TRY(hipGetLastError());
     return PG_OK;
 }
