@@ -186,6 +186,11 @@ def test_snapshot_layout_3d(tmp_path):
     assert np.array_equal(raw[off:off + nz + 1], za[3:nz + 4]); off += nz + 1
     g = d.get_state()[0]
     assert np.array_equal(raw[off:].reshape(5, nz, ny, nx), g)
+    from penguin_amd.snapshot import read_snapshot
+    snap = read_snapshot(f, nx, ny, nz=nz)
+    assert snap["t"] == 1.25 and np.array_equal(snap["za"], za[3:nz + 4])
+    for k, name in enumerate(("rho", "p", "u", "v", "w")):
+        assert np.array_equal(snap[name], g[k])
     d2 = Disk(p)
     assert d2.read_snapshot(f) == 1.25
     assert np.array_equal(d2.get_state()[0], g)
