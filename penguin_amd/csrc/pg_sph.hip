@@ -654,35 +654,45 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
 // ------------------------------------------------------------- viscosity
 // viscous_forces (viscosity.cu:34-197) on every polar layer: the (r, phi)
 // stress divergence with the spherical radius, nu at the layer's angle; the
-// polar component is zero.  One thread per interior cell.
+// polar component is zero.  Tiles of VX radii x VY rows of one layer with a
+// one-cell halo staged in LDS (rho, u, full v); 256 threads, each computing
+// VY / 4 cells of one column.
+constexpr int VX = 64, VY = 16;
+
 template <typename real>
 __global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.N) return;
+    constexpr int HX = VX + 2, HY = VY + 2;
+    __shared__ real sr[HY * HX], su[HY * HX], sv[HY * HX];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int i = (int)(c % nx), jk = (int)(c / nx), j = jk % ny, k = jk / ny;
     const size_t N = a.N;
     const int nxe = nx + 2 * NPAD;
-    real r[3], nu[3], u[9], v[9], nr[9];
-    for (int n = 0; n < 3; ++n) {
-        r[n] = a.xc[i + NPAD - 1 + n];
-        nu[n] = a.nu[(size_t)k * nxe + i + NPAD - 1 + n];
-    }
-    for (int n = 0; n < 9; ++n) {
-        const int di = n % 3 - 1, dj = n < 3 ? 0 : (n < 6 ? -1 : 1);
-        const int ii = i + di, jj = wrapi(j + dj, ny), ip = ii + NPAD;
-        real rho, uu, vp;
+    const int ntx = (nx + VX - 1) / VX, nty = (ny + VY - 1) / VY;
+    const int b = blockIdx.x, k = b / (ntx * nty), rem = b - k * ntx * nty;
+    const int i0 = (rem % ntx) * VX, j0 = (rem / ntx) * VY;
+    const int t = threadIdx.x;
+    for (int e = t; e < HX * HY; e += 256) {
+        const int lx = e % HX, ly = e / HX;
+        const int ii = i0 - 1 + lx, jj = wrapi(j0 - 1 + ly, ny), ip = ii + NPAD;
+        real rho = real(1), uu = real(0), vp = real(0);
         if (ii >= 0 && ii < nx) {
             const size_t cc = (size_t)ii + (size_t)nx * ((size_t)jj + (size_t)ny * k);
             rho = a.in[cc]; uu = a.in[2 * N + cc]; vp = a.in[3 * N + cc];
-        } else {
+        } else if (ii >= -1 && ii <= nx) {
             const int g = ii < 0 ? ip : ip - nx;
             const real* xg = a.xg + (size_t)g * nz + k;
             rho = xg[0]; uu = xg[12 * nz]; vp = xg[18 * nz];
         }
-        nr[n] = rho * nu[n % 3];
-        u[n] = uu;
-        v[n] = vp + a.vbg[(size_t)k * nxe + ip];
+        sr[e] = rho;
+        su[e] = uu;
+        sv[e] = ip >= 0 && ip < nxe ? vp + a.vbg[(size_t)k * nxe + ip] : vp;
+    }
+    __syncthreads();
+    const int tx = t % VX, i = i0 + tx;
+    if (i >= nx) return;
+    real r[3], nu[3];
+    for (int n = 0; n < 3; ++n) {
+        r[n] = a.xc[i + NPAD - 1 + n];
+        nu[n] = a.nu[(size_t)k * nxe + i + NPAD - 1 + n];
     }
     const real dy = a.dy;
     const real ir1 = FM<real>::rcp(r[1]);
@@ -696,24 +706,37 @@ __global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a) {
         dyy = FM<real>::div(FM<real>::div(f[7] - f[1], dy) - FM<real>::div(f[1] - f[4], dy), dy) * ir1 * ir1;
         dxy = FM<real>::div(f[8] - f[6] - f[5] + f[3], (r[2] - r[0]) * (real(2) * dy)) * ir1;
     };
-    real dudx, d2udx2, dvdx, d2vdx2, dvdy, d2vdy2, d2vdxy, dudy, d2udy2, d2udxy;
-    d1(u, dudx, d2udx2);
-    const real dndx = FM<real>::div(nr[2] - nr[0], r[2] - r[0]);
-    d1(v, dvdx, d2vdx2);
-    d2(v, dvdy, d2vdy2, d2vdxy);
-    d2(u, dudy, d2udy2, d2udxy);
-    const real dndy = FM<real>::div(nr[7] - nr[4], real(2) * dy) * ir1;
-    const real sxx = real(2) * nr[1] * dudx;
-    real fx = real(2) * dndx * dudx + real(2) * nr[1] * d2udx2 + sxx * ir1;
-    const real syy = real(2) * nr[1] * (dvdy + u[1] * ir1);
-    const real sxy = nr[1] * (dudy + dvdx - v[1] * ir1);
-    real fy = real(2) * dndy * dvdy + real(2) * nr[1] * d2vdy2 + real(2) * nr[1] * dudy * ir1 +
-              real(2) * dndy * u[1] * ir1;
-    fx += dndy * (dudy + dvdx) + nr[1] * (d2udy2 + d2vdxy) - dndy * v[1] * ir1 - nr[1] * dvdy * ir1 - syy * ir1;
-    fy += dndx * (dudy + dvdx) + nr[1] * (d2udxy + d2vdx2) - dndx * v[1] * ir1 - nr[1] * dvdx * ir1 +
-          nr[1] * v[1] * ir1 * ir1 + real(2) * sxy * ir1;
-    a.vis_out[c] = fx;
-    a.vis_out[N + c] = fy;
+    for (int ty = t / VX; ty < VY; ty += 256 / VX) {
+        const int j = j0 + ty;
+        if (j >= ny) break;
+        real u[9], v[9], nr[9];
+        for (int n = 0; n < 9; ++n) {
+            const int di = n % 3 - 1, dj = n < 3 ? 0 : (n < 6 ? -1 : 1);
+            const int e = (ty + 1 + dj) * HX + tx + 1 + di;
+            nr[n] = sr[e] * nu[n % 3];
+            u[n] = su[e];
+            v[n] = sv[e];
+        }
+        real dudx, d2udx2, dvdx, d2vdx2, dvdy, d2vdy2, d2vdxy, dudy, d2udy2, d2udxy;
+        d1(u, dudx, d2udx2);
+        const real dndx = FM<real>::div(nr[2] - nr[0], r[2] - r[0]);
+        d1(v, dvdx, d2vdx2);
+        d2(v, dvdy, d2vdy2, d2vdxy);
+        d2(u, dudy, d2udy2, d2udxy);
+        const real dndy = FM<real>::div(nr[7] - nr[4], real(2) * dy) * ir1;
+        const real sxx = real(2) * nr[1] * dudx;
+        real fx = real(2) * dndx * dudx + real(2) * nr[1] * d2udx2 + sxx * ir1;
+        const real syy = real(2) * nr[1] * (dvdy + u[1] * ir1);
+        const real sxy = nr[1] * (dudy + dvdx - v[1] * ir1);
+        real fy = real(2) * dndy * dvdy + real(2) * nr[1] * d2vdy2 + real(2) * nr[1] * dudy * ir1 +
+                  real(2) * dndy * u[1] * ir1;
+        fx += dndy * (dudy + dvdx) + nr[1] * (d2udy2 + d2vdxy) - dndy * v[1] * ir1 - nr[1] * dvdy * ir1 - syy * ir1;
+        fy += dndx * (dudy + dvdx) + nr[1] * (d2udxy + d2vdx2) - dndx * v[1] * ir1 - nr[1] * dvdx * ir1 +
+              nr[1] * v[1] * ir1 * ir1 + real(2) * sxy * ir1;
+        const size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+        a.vis_out[c] = fx;
+        a.vis_out[N + c] = fy;
+    }
 }
 
 // ---------------------------------------------------------- source terms
@@ -750,41 +773,55 @@ __device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<re
     gz += m * px * c * cp * k - FM<real>::div(cp * c, sp) * ind;
 }
 
+// four cells per thread (a block covers 1024 consecutive cells), their loads
+// issued together
+constexpr int SRC_CPT = 4;
+
 template <typename real>
 __global__ __launch_bounds__(256) void k3_sources(SphK<real> a, PlanetS<real> pl, int planet, real mdt, real dt) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.N) return;
     const int nx = a.nx, ny = a.ny;
-    const int i = (int)(c % nx), jk = (int)(c / nx), j = jk % ny, k = jk / ny;
     const size_t N = a.N;
-    const int ip = i + NPAD, kp = k + NPAD;
-    const real rad = a.xc[ip];
-    const real phi = a.y0 + (real(j) + real(0.5)) * a.dy;
-    real gx, gy, gz;
-    sph_forces(a, pl, planet != 0, rad, phi, kp, gx, gy, gz);
-    const real vb = a.vbg[(size_t)k * (nx + 2 * NPAD) + ip];
-    const real rho = a.in[c], u = a.in[2 * N + c], vp = a.in[3 * N + c], w = a.in[4 * N + c];
-    const real vis0 = a.vis ? FM<real>::div(a.vis[c], rho) : real(0);
-    const real vis1 = a.vis ? FM<real>::div(a.vis[N + c], rho) : real(0);
-    const real ir = FM<real>::rcp(rad), cot = a.cotz[kp];
-    // get_g_forces (sweeps.cu:555-576): fictitious forces of the state T
-    auto forces = [&](real Tv, real Tw, real& fx, real& fy, real& fz) {
-        fz = Tv * Tv * ir * cot;
-        fy = vis1;
-        fx = Tv * Tv * ir + Tw * Tw * ir + vis0;
-    };
-    real fx, fy, fz;
-    forces(vp + vb, w, fx, fy, fz);
-    if (mdt != real(0)) {
-        const real Tv = vp + vb + fy * mdt + gy * mdt, Tw = w + fz * mdt + gz * mdt;
-        forces(Tv, Tw, fx, fy, fz);
+    const size_t c0 = (size_t)blockIdx.x * (256 * SRC_CPT) + threadIdx.x;
+    real rho[SRC_CPT], u[SRC_CPT], vp[SRC_CPT], w[SRC_CPT], v0[SRC_CPT], v1[SRC_CPT];
+#pragma unroll
+    for (int m = 0; m < SRC_CPT; ++m) {
+        const size_t c = min(c0 + (size_t)m * 256, N - 1);
+        rho[m] = a.in[c]; u[m] = a.in[2 * N + c]; vp[m] = a.in[3 * N + c]; w[m] = a.in[4 * N + c];
+        v0[m] = a.vis ? a.vis[c] : real(0);
+        v1[m] = a.vis ? a.vis[N + c] : real(0);
     }
-    a.out[2 * N + c] = u + fx * dt + gx * dt;
-    a.out[3 * N + c] = vp + fy * dt + gy * dt;
-    a.out[4 * N + c] = w + fz * dt + gz * dt;
-    if (a.out != a.in) {
-        a.out[c] = rho;
-        a.out[N + c] = a.in[N + c];
+#pragma unroll
+    for (int m = 0; m < SRC_CPT; ++m) {
+        const size_t c = c0 + (size_t)m * 256;
+        if (c >= N) break;
+        const int i = (int)(c % nx), jk = (int)(c / nx), j = jk % ny, k = jk / ny;
+        const int ip = i + NPAD, kp = k + NPAD;
+        const real rad = a.xc[ip];
+        const real phi = a.y0 + (real(j) + real(0.5)) * a.dy;
+        real gx, gy, gz;
+        sph_forces(a, pl, planet != 0, rad, phi, kp, gx, gy, gz);
+        const real vb = a.vbg[(size_t)k * (nx + 2 * NPAD) + ip];
+        const real vis0 = FM<real>::div(v0[m], rho[m]), vis1 = FM<real>::div(v1[m], rho[m]);
+        const real ir = FM<real>::rcp(rad), cot = a.cotz[kp];
+        // get_g_forces (sweeps.cu:555-576): fictitious forces of the state T
+        auto forces = [&](real Tv, real Tw, real& fx, real& fy, real& fz) {
+            fz = Tv * Tv * ir * cot;
+            fy = vis1;
+            fx = Tv * Tv * ir + Tw * Tw * ir + vis0;
+        };
+        real fx, fy, fz;
+        forces(vp[m] + vb, w[m], fx, fy, fz);
+        if (mdt != real(0)) {
+            const real Tv = vp[m] + vb + fy * mdt + gy * mdt, Tw = w[m] + fz * mdt + gz * mdt;
+            forces(Tv, Tw, fx, fy, fz);
+        }
+        a.out[2 * N + c] = u[m] + fx * dt + gx * dt;
+        a.out[3 * N + c] = vp[m] + fy * dt + gy * dt;
+        a.out[4 * N + c] = w[m] + fz * dt + gz * dt;
+        if (a.out != a.in) {
+            a.out[c] = rho[m];
+            a.out[N + c] = a.in[N + c];
+        }
     }
 }
 
@@ -1019,7 +1056,8 @@ int launch_vis(Sph* s, int in) {
     SphK<real> a = kargs<real>(s, in, -1);
     a.vis_out = (real*)s->vis;
     Timed tm(s, "viscosity");
-    hipLaunchKernelGGL(k3_viscosity<real>, dim3((unsigned)((s->N + 255) / 256)), dim3(256), 0, s->stream, a);
+    const unsigned nb = (unsigned)(((s->nx + VX - 1) / VX) * ((s->ny + VY - 1) / VY) * s->nz);
+    hipLaunchKernelGGL(k3_viscosity<real>, dim3(nb), dim3(256), 0, s->stream, a);
     SPH_TRY(hipGetLastError());
     return PG_OK;
 }
@@ -1033,7 +1071,8 @@ int launch_sources(Sph* s, int in, int out, const Body& b, double mdt, double dt
     pl.shy = (real)std::sin(0.5 * b.y);
     pl.chy = (real)std::cos(0.5 * b.y);
     Timed tm(s, "sources");
-    hipLaunchKernelGGL(k3_sources<real>, dim3((unsigned)((s->N + 255) / 256)), dim3(256), 0, s->stream, a, pl,
+    hipLaunchKernelGGL(k3_sources<real>, dim3((unsigned)((s->N + 256 * SRC_CPT - 1) / (256 * SRC_CPT))), dim3(256), 0,
+                       s->stream, a, pl,
                        s->P.n_planet > 0 ? 1 : 0, (real)mdt, (real)dt);
     SPH_TRY(hipGetLastError());
     return PG_OK;
