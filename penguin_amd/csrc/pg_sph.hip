@@ -773,13 +773,40 @@ __device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<re
     gz += m * px * c * cp * k - FM<real>::div(cp * c, sp) * ind;
 }
 
+// CFL rate of one cell (get_dt_lv1, timestep.cu:18-97): signal speed over
+// cell width in each direction (the azimuthal frame velocity is vp)
+template <typename real>
+__device__ __forceinline__ real cell_rate(const SphK<real>& a, int ip, int kp, real rho, real p, real u, real vp, real w) {
+    const real cs = FM<real>::sqrt(FM<real>::div(a.gam * p, rho));
+    const real rad = a.xc[ip], rcyl = rad * a.sinz[kp];
+    real t = FM<real>::div(fabs(u) + cs, a.xa[ip + 1] - a.xa[ip]);
+    t = fmax(FM<real>::div(fabs(vp) + cs, a.dy * rcyl), t);
+    return fmax(FM<real>::div(fabs(w) + cs, (a.za[kp + 1] - a.za[kp]) * rad), t);
+}
+
+// maximum of non-negative values through their bit patterns (a NaN rate
+// compares above every number and is caught by the host)
+__device__ __forceinline__ void atomic_max_nonneg(float* p, float v) {
+    atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+constexpr int CFL_BLOCKS = 1024;
+
 // four cells per thread (a block covers 1024 consecutive cells), their loads
 // issued together
 constexpr int SRC_CPT = 4;
 
+// with `rate` set (the corrector) the CFL rate of the new state is reduced
+// per block into a.rate[block % CFL_BLOCKS] (zeroed beforehand): the next
+// step's dt then needs no pass of its own
 template <typename real>
-__global__ __launch_bounds__(256) void k3_sources(SphK<real> a, PlanetS<real> pl, int planet, real mdt, real dt) {
+__global__ __launch_bounds__(256) void k3_sources(SphK<real> a, PlanetS<real> pl, int planet, real mdt, real dt,
+                                                  int rate) {
     const int nx = a.nx, ny = a.ny;
+    real rmax = real(0);
     const size_t N = a.N;
     const size_t c0 = (size_t)blockIdx.x * (256 * SRC_CPT) + threadIdx.x;
     real rho[SRC_CPT], u[SRC_CPT], vp[SRC_CPT], w[SRC_CPT], v0[SRC_CPT], v1[SRC_CPT];
@@ -815,13 +842,25 @@ __global__ __launch_bounds__(256) void k3_sources(SphK<real> a, PlanetS<real> pl
             const real Tv = vp[m] + vb + fy * mdt + gy * mdt, Tw = w[m] + fz * mdt + gz * mdt;
             forces(Tv, Tw, fx, fy, fz);
         }
-        a.out[2 * N + c] = u[m] + fx * dt + gx * dt;
-        a.out[3 * N + c] = vp[m] + fy * dt + gy * dt;
-        a.out[4 * N + c] = w[m] + fz * dt + gz * dt;
+        const real un = u[m] + fx * dt + gx * dt, vn = vp[m] + fy * dt + gy * dt, wn = w[m] + fz * dt + gz * dt;
+        a.out[2 * N + c] = un;
+        a.out[3 * N + c] = vn;
+        a.out[4 * N + c] = wn;
         if (a.out != a.in) {
             a.out[c] = rho[m];
             a.out[N + c] = a.in[N + c];
         }
+        if (rate) rmax = fmax(rmax, cell_rate(a, ip, kp, rho[m], a.in[N + c], un, vn, wn));
+    }
+    if (rate) {
+        __shared__ real red[256];
+        red[threadIdx.x] = rmax;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomic_max_nonneg(a.rate + blockIdx.x % CFL_BLOCKS, red[0]);
     }
 }
 
@@ -836,14 +875,8 @@ __global__ __launch_bounds__(256) void k3_cfl(SphK<real> a) {
     const int nx = a.nx, ny = a.ny;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(c % nx), k = (int)(c / nx / ny);
-        const int ip = i + NPAD, kp = k + NPAD;
-        const real rho = a.in[c], p = a.in[N + c];
-        const real cs = FM<real>::sqrt(FM<real>::div(a.gam * p, rho));
-        const real rad = a.xc[ip], rcyl = rad * a.sinz[kp];
-        real t = FM<real>::div(fabs(a.in[2 * N + c]) + cs, a.xa[ip + 1] - a.xa[ip]);
-        t = fmax(FM<real>::div(fabs(a.in[3 * N + c]) + cs, a.dy * rcyl), t);
-        t = fmax(FM<real>::div(fabs(a.in[4 * N + c]) + cs, (a.za[kp + 1] - a.za[kp]) * rad), t);
-        m = fmax(m, t);
+        m = fmax(m, cell_rate(a, i + NPAD, k + NPAD, a.in[c], a.in[N + c], a.in[2 * N + c], a.in[3 * N + c],
+                              a.in[4 * N + c]));
     }
     red[threadIdx.x] = m;
     __syncthreads();
@@ -853,8 +886,6 @@ __global__ __launch_bounds__(256) void k3_cfl(SphK<real> a) {
     }
     if (threadIdx.x == 0) a.rate[blockIdx.x] = red[0];
 }
-
-constexpr int CFL_BLOCKS = 1024;
 
 }  // namespace
 
@@ -930,6 +961,7 @@ struct Sph {
     Body planet{};              // planet at the time of the last evolve
     double yw[6] = {0};
     size_t bytes = 0;
+    bool rate_valid = false;    // rate partials hold the current state's rate (written by the corrector)
     bool profile = false;
     std::map<std::string, std::pair<double, int>> timers;
 };
@@ -1063,25 +1095,25 @@ int launch_vis(Sph* s, int in) {
 }
 
 template <typename real>
-int launch_sources(Sph* s, int in, int out, const Body& b, double mdt, double dt) {
+int launch_sources(Sph* s, int in, int out, const Body& b, double mdt, double dt, bool rate = false) {
     SphK<real> a = kargs<real>(s, in, out);
     a.vis = s->P.visc_flag ? (const real*)s->vis : nullptr;
     PlanetS<real> pl;
     pl.m = (real)b.m; pl.x = (real)b.x; pl.y = (real)b.y; pl.rs = (real)b.rs;
     pl.shy = (real)std::sin(0.5 * b.y);
     pl.chy = (real)std::cos(0.5 * b.y);
+    if (rate) SPH_TRY(hipMemsetAsync(s->rate, 0, CFL_BLOCKS * sizeof(real), s->stream));
     Timed tm(s, "sources");
     hipLaunchKernelGGL(k3_sources<real>, dim3((unsigned)((s->N + 256 * SRC_CPT - 1) / (256 * SRC_CPT))), dim3(256), 0,
-                       s->stream, a, pl,
-                       s->P.n_planet > 0 ? 1 : 0, (real)mdt, (real)dt);
+                       s->stream, a, pl, s->P.n_planet > 0 ? 1 : 0, (real)mdt, (real)dt, rate ? 1 : 0);
     SPH_TRY(hipGetLastError());
     return PG_OK;
 }
 
 template <typename real>
 int rate_of(Sph* s, int in, double* rate) {
-    SphK<real> a = kargs<real>(s, in, -1);
-    {
+    if (!s->rate_valid) {
+        SphK<real> a = kargs<real>(s, in, -1);
         Timed tm(s, "cfl");
         hipLaunchKernelGGL(k3_cfl<real>, dim3(CFL_BLOCKS), dim3(256), 0, s->stream, a);
         SPH_TRY(hipGetLastError());
@@ -1116,7 +1148,9 @@ int ds_step(Sph* s, double t, double dt) {
         if ((rc = launch_sources<real>(s, 0, 2, s->planet, hdt, hdt))) return rc;
         if ((rc = launch_vis<real>(s, 2))) return rc;
     }
-    return launch_sources<real>(s, 0, 0, s->planet, hdt, hdt);
+    if ((rc = launch_sources<real>(s, 0, 0, s->planet, hdt, hdt, true))) return rc;
+    s->rate_valid = true;
+    return PG_OK;
 }
 
 template <typename real>
@@ -1210,6 +1244,7 @@ int upload_tables(Sph* s) {
 // host FP64 state (5, nz, ny, nx) <-> device buffer 0
 template <typename real>
 int put_state(Sph* s, const double* gas) {
+    s->rate_valid = false;
     const size_t N = s->N;
     const int nx = s->nx, ny = s->ny, nxe = nx + 2 * NPAD;
     std::vector<real> h((size_t)NF * N);
@@ -1412,6 +1447,7 @@ int sph_run(Sph* s, int nsteps, double* t_io, double* dt_io, double* dts) {   //
 
 int sph_sweep(Sph* s, int dir, double dt) {
     if (dir < 0 || dir > 2) return report_error(PG_E_INVALID, "sweep direction %d", dir);
+    s->rate_valid = false;
     int rc = s->es == 8 ? launch_sweep<double>(s, dir, 0, 1, dt) : launch_sweep<float>(s, dir, 0, 1, dt);
     if (rc) return rc;
     SPH_TRY(hipMemcpyAsync(s->st[0], s->st[1], (size_t)NF * s->N * s->es, hipMemcpyDeviceToDevice, s->stream));
@@ -1420,6 +1456,7 @@ int sph_sweep(Sph* s, int dir, double dt) {
 }
 
 int sph_orbital_advection(Sph* s, double dt, int* inc, double* res) {
+    s->rate_valid = false;
     const size_t nik = (size_t)s->nx * s->nz;
     int* inc_d = nullptr;
     double* res_d = nullptr;
@@ -1443,6 +1480,7 @@ int sph_orbital_advection(Sph* s, double dt, int* inc, double* res) {
 }
 
 int sph_sources(Sph* s, double t_planet, double mdt, double dt, int inplace) {
+    s->rate_valid = false;
     Body b = s->planet;
     if (s->P.n_planet > 0) orbit_solution(b, t_planet, s->P.ramp_time, s->P.planet_mass, s->P.frame_omega);
     const int out = inplace ? 0 : 2;
