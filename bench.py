@@ -114,10 +114,12 @@ def adjoint_leg(steps: int, precision: int) -> dict:
 
 # words (reals) each 3D kernel reads and writes per interior cell: the sweeps
 # and the orbital advection read and write the five fields; the viscosity
-# reads rho, u, vp and writes two components; the sources read five fields
-# and two viscous components and write three velocities (in place) or five
+# reads rho, u, vp and writes two components; the fused predictor-viscosity
+# reads rho, u, vp, w and two viscous components and writes two; the
+# sources read five fields and two viscous components and write three
+# velocities (in place) or five
 SPH_WORDS = {"sweep_x_gas": 10, "sweep_y_gas": 10, "sweep_z_gas": 10, "orbital_advection": 10,
-             "viscosity": 5, "sources": 10, "cfl": 5}
+             "viscosity": 5, "predictor_viscosity": 8, "sources": 10, "cfl": 5}
 
 
 def sph_leg(steps: int, precision: int, cpu: bool) -> dict:

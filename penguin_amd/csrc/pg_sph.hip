@@ -651,6 +651,144 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
     }
 }
 
+// ---------------------------------------------------------- source terms
+// apply_source_terms (sweeps.cu:597-711) with the spherical forces
+// (forces.cu:110-327): mode 0 the half kick (in -> out, forces at the state
+// itself), modes 1/2 predictor and corrector (forces re-evaluated at the
+// midpoint state, in -> out).
+template <typename real>
+__device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<real>& pl, bool planet, real rad, real phi,
+                                           int kp, real& gx, real& gy, real& gz) {
+    gx = -FM<real>::rcp(rad * rad);
+    gy = real(0);
+    gz = real(0);
+    if (!planet) return;
+    // star_planet_grav (forces.cu:14-39) at dt = 0, spline softening; the
+    // squared distance r^2 + a^2 - 2 a r cos(dphi) sin(theta) written as
+    // (r - a)^2 + 2 a r (2 sin^2(dphi/2) + cos(dphi) cos^2(theta) / (1 + sin(theta)))
+    const real sp = a.sinz[kp];
+    const real cp = sp * a.cotz[kp];
+    real sh, ch, s, c;
+    sincos_r(real(0.5) * phi, &sh, &ch);
+    const real hs = sh * pl.chy - ch * pl.shy;   // sin((phi - phi_p) / 2)
+    const real hc = ch * pl.chy + sh * pl.shy;   // cos((phi - phi_p) / 2)
+    s = real(2) * hs * hc;
+    c = hc * hc - hs * hs;
+    const real px = pl.x, m = pl.m, rs = pl.rs;
+    const real Rp2 = (rad - px) * (rad - px) +
+                     real(2) * px * rad * (real(2) * hs * hs + c * FM<real>::div(cp * cp, real(1) + sp));
+    const real Rp = FM<real>::sqrt(Rp2);
+    const real k = FM<real>::div(fmax(real(4) - real(3) * FM<real>::div(Rp, rs), real(1)), fmax(rs * rs * rs, Rp2 * Rp));
+    const real ind = FM<real>::div(m, px * px);
+    gx += -m * (rad - px * c * sp) * k - sp * c * ind;
+    gy += -m * px * s * k + sp * s * ind;
+    gz += m * px * c * cp * k - FM<real>::div(cp * c, sp) * ind;
+}
+
+// CFL rate of one cell (get_dt_lv1, timestep.cu:18-97): signal speed over
+// cell width in each direction (the azimuthal frame velocity is vp)
+template <typename real>
+__device__ __forceinline__ real cell_rate(const SphK<real>& a, int ip, int kp, real rho, real p, real u, real vp, real w) {
+    const real cs = FM<real>::sqrt(FM<real>::div(a.gam * p, rho));
+    const real rad = a.xc[ip], rcyl = rad * a.sinz[kp];
+    real t = FM<real>::div(fabs(u) + cs, a.xa[ip + 1] - a.xa[ip]);
+    t = fmax(FM<real>::div(fabs(vp) + cs, a.dy * rcyl), t);
+    return fmax(FM<real>::div(fabs(w) + cs, (a.za[kp + 1] - a.za[kp]) * rad), t);
+}
+
+// maximum of non-negative values through their bit patterns (a NaN rate
+// compares above every number and is caught by the host)
+__device__ __forceinline__ void atomic_max_nonneg(float* p, float v) {
+    atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+constexpr int CFL_BLOCKS = 1024;
+
+// the velocities after the source terms of one cell (sweeps.cu:597-711):
+// forces at the state itself (mdt = 0) or re-evaluated at the midpoint state
+// C + (f(C) + g) mdt, then C + (f + g) dt
+template <typename real>
+__device__ __forceinline__ void source_cell(const SphK<real>& a, const PlanetS<real>& pl, bool planet, int ip, int kp,
+                                            int j, real rho, real u, real vp, real w, real visr, real visp, real mdt,
+                                            real dt, real& un, real& vn, real& wn) {
+    const real rad = a.xc[ip];
+    const real phi = a.y0 + (real(j) + real(0.5)) * a.dy;
+    real gx, gy, gz;
+    sph_forces(a, pl, planet, rad, phi, kp, gx, gy, gz);
+    const real vb = a.vbg[(size_t)(kp - NPAD) * (a.nx + 2 * NPAD) + ip];
+    const real vis0 = FM<real>::div(visr, rho), vis1 = FM<real>::div(visp, rho);
+    const real ir = FM<real>::rcp(rad), cot = a.cotz[kp];
+    // get_g_forces (sweeps.cu:555-576): fictitious forces of the state T
+    auto forces = [&](real Tv, real Tw, real& fx, real& fy, real& fz) {
+        fz = Tv * Tv * ir * cot;
+        fy = vis1;
+        fx = Tv * Tv * ir + Tw * Tw * ir + vis0;
+    };
+    real fx, fy, fz;
+    forces(vp + vb, w, fx, fy, fz);
+    if (mdt != real(0)) {
+        const real Tv = vp + vb + fy * mdt + gy * mdt, Tw = w + fz * mdt + gz * mdt;
+        forces(Tv, Tw, fx, fy, fz);
+    }
+    un = u + fx * dt + gx * dt;
+    vn = vp + fy * dt + gy * dt;
+    wn = w + fz * dt + gz * dt;
+}
+
+// four cells per thread (a block covers 1024 consecutive cells), their loads
+// issued together
+constexpr int SRC_CPT = 4;
+
+// with `rate` set (the corrector) the CFL rate of the new state is reduced
+// per block into a.rate[block % CFL_BLOCKS] (zeroed beforehand): the next
+// step's dt then needs no pass of its own
+template <typename real>
+__global__ __launch_bounds__(256) void k3_sources(SphK<real> a, PlanetS<real> pl, int planet, real mdt, real dt,
+                                                  int rate) {
+    const int nx = a.nx, ny = a.ny;
+    real rmax = real(0);
+    const size_t N = a.N;
+    const size_t c0 = (size_t)blockIdx.x * (256 * SRC_CPT) + threadIdx.x;
+    real rho[SRC_CPT], u[SRC_CPT], vp[SRC_CPT], w[SRC_CPT], v0[SRC_CPT], v1[SRC_CPT];
+#pragma unroll
+    for (int m = 0; m < SRC_CPT; ++m) {
+        const size_t c = min(c0 + (size_t)m * 256, N - 1);
+        rho[m] = a.in[c]; u[m] = a.in[2 * N + c]; vp[m] = a.in[3 * N + c]; w[m] = a.in[4 * N + c];
+        v0[m] = a.vis ? a.vis[c] : real(0);
+        v1[m] = a.vis ? a.vis[N + c] : real(0);
+    }
+#pragma unroll
+    for (int m = 0; m < SRC_CPT; ++m) {
+        const size_t c = c0 + (size_t)m * 256;
+        if (c >= N) break;
+        const int i = (int)(c % nx), jk = (int)(c / nx), j = jk % ny, k = jk / ny;
+        const int ip = i + NPAD, kp = k + NPAD;
+        real un, vn, wn;
+        source_cell(a, pl, planet != 0, ip, kp, j, rho[m], u[m], vp[m], w[m], v0[m], v1[m], mdt, dt, un, vn, wn);
+        a.out[2 * N + c] = un;
+        a.out[3 * N + c] = vn;
+        a.out[4 * N + c] = wn;
+        if (a.out != a.in) {
+            a.out[c] = rho[m];
+            a.out[N + c] = a.in[N + c];
+        }
+        if (rate) rmax = fmax(rmax, cell_rate(a, ip, kp, rho[m], a.in[N + c], un, vn, wn));
+    }
+    if (rate) {
+        __shared__ real red[256];
+        red[threadIdx.x] = rmax;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomic_max_nonneg(a.rate + blockIdx.x % CFL_BLOCKS, red[0]);
+    }
+}
+
 // ------------------------------------------------------------- viscosity
 // viscous_forces (viscosity.cu:34-197) on every polar layer: the (r, phi)
 // stress divergence with the spherical radius, nu at the layer's angle; the
@@ -659,8 +797,12 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
 // VY / 4 cells of one column.
 constexpr int VX = 64, VY = 16;
 
-template <typename real>
-__global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a) {
+// PRED: the predictor (apply_source_terms with mdt = dt = dt/2 into T,
+// sweeps.cu:597-711) is applied to the staged interior cells first, so the
+// kernel yields the viscosity of the predicted state without storing it
+// (radial ghosts stay analytic, as boundx2 makes them, boundary.cu:30-33)
+template <typename real, bool PRED>
+__global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a, PlanetS<real> pl, int planet, real hdt) {
     constexpr int HX = VX + 2, HY = VY + 2;
     __shared__ real sr[HY * HX], su[HY * HX], sv[HY * HX];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -677,6 +819,11 @@ __global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a) {
         if (ii >= 0 && ii < nx) {
             const size_t cc = (size_t)ii + (size_t)nx * ((size_t)jj + (size_t)ny * k);
             rho = a.in[cc]; uu = a.in[2 * N + cc]; vp = a.in[3 * N + cc];
+            if (PRED) {
+                real wn;
+                source_cell(a, pl, planet != 0, ip, k + NPAD, jj, rho, uu, vp, a.in[4 * N + cc], a.vis[cc],
+                            a.vis[N + cc], hdt, hdt, uu, vp, wn);
+            }
         } else if (ii >= -1 && ii <= nx) {
             const int g = ii < 0 ? ip : ip - nx;
             const real* xg = a.xg + (size_t)g * nz + k;
@@ -736,131 +883,6 @@ __global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a) {
         const size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
         a.vis_out[c] = fx;
         a.vis_out[N + c] = fy;
-    }
-}
-
-// ---------------------------------------------------------- source terms
-// apply_source_terms (sweeps.cu:597-711) with the spherical forces
-// (forces.cu:110-327): mode 0 the half kick (in -> out, forces at the state
-// itself), modes 1/2 predictor and corrector (forces re-evaluated at the
-// midpoint state, in -> out).
-template <typename real>
-__device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<real>& pl, bool planet, real rad, real phi,
-                                           int kp, real& gx, real& gy, real& gz) {
-    gx = -FM<real>::rcp(rad * rad);
-    gy = real(0);
-    gz = real(0);
-    if (!planet) return;
-    // star_planet_grav (forces.cu:14-39) at dt = 0, spline softening; the
-    // squared distance r^2 + a^2 - 2 a r cos(dphi) sin(theta) written as
-    // (r - a)^2 + 2 a r (2 sin^2(dphi/2) + cos(dphi) cos^2(theta) / (1 + sin(theta)))
-    const real sp = a.sinz[kp];
-    const real cp = sp * a.cotz[kp];
-    real sh, ch, s, c;
-    sincos_r(real(0.5) * phi, &sh, &ch);
-    const real hs = sh * pl.chy - ch * pl.shy;   // sin((phi - phi_p) / 2)
-    const real hc = ch * pl.chy + sh * pl.shy;   // cos((phi - phi_p) / 2)
-    s = real(2) * hs * hc;
-    c = hc * hc - hs * hs;
-    const real px = pl.x, m = pl.m, rs = pl.rs;
-    const real Rp2 = (rad - px) * (rad - px) +
-                     real(2) * px * rad * (real(2) * hs * hs + c * FM<real>::div(cp * cp, real(1) + sp));
-    const real Rp = FM<real>::sqrt(Rp2);
-    const real k = FM<real>::div(fmax(real(4) - real(3) * FM<real>::div(Rp, rs), real(1)), fmax(rs * rs * rs, Rp2 * Rp));
-    const real ind = FM<real>::div(m, px * px);
-    gx += -m * (rad - px * c * sp) * k - sp * c * ind;
-    gy += -m * px * s * k + sp * s * ind;
-    gz += m * px * c * cp * k - FM<real>::div(cp * c, sp) * ind;
-}
-
-// CFL rate of one cell (get_dt_lv1, timestep.cu:18-97): signal speed over
-// cell width in each direction (the azimuthal frame velocity is vp)
-template <typename real>
-__device__ __forceinline__ real cell_rate(const SphK<real>& a, int ip, int kp, real rho, real p, real u, real vp, real w) {
-    const real cs = FM<real>::sqrt(FM<real>::div(a.gam * p, rho));
-    const real rad = a.xc[ip], rcyl = rad * a.sinz[kp];
-    real t = FM<real>::div(fabs(u) + cs, a.xa[ip + 1] - a.xa[ip]);
-    t = fmax(FM<real>::div(fabs(vp) + cs, a.dy * rcyl), t);
-    return fmax(FM<real>::div(fabs(w) + cs, (a.za[kp + 1] - a.za[kp]) * rad), t);
-}
-
-// maximum of non-negative values through their bit patterns (a NaN rate
-// compares above every number and is caught by the host)
-__device__ __forceinline__ void atomic_max_nonneg(float* p, float v) {
-    atomicMax(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
-}
-__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
-    atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
-}
-
-constexpr int CFL_BLOCKS = 1024;
-
-// four cells per thread (a block covers 1024 consecutive cells), their loads
-// issued together
-constexpr int SRC_CPT = 4;
-
-// with `rate` set (the corrector) the CFL rate of the new state is reduced
-// per block into a.rate[block % CFL_BLOCKS] (zeroed beforehand): the next
-// step's dt then needs no pass of its own
-template <typename real>
-__global__ __launch_bounds__(256) void k3_sources(SphK<real> a, PlanetS<real> pl, int planet, real mdt, real dt,
-                                                  int rate) {
-    const int nx = a.nx, ny = a.ny;
-    real rmax = real(0);
-    const size_t N = a.N;
-    const size_t c0 = (size_t)blockIdx.x * (256 * SRC_CPT) + threadIdx.x;
-    real rho[SRC_CPT], u[SRC_CPT], vp[SRC_CPT], w[SRC_CPT], v0[SRC_CPT], v1[SRC_CPT];
-#pragma unroll
-    for (int m = 0; m < SRC_CPT; ++m) {
-        const size_t c = min(c0 + (size_t)m * 256, N - 1);
-        rho[m] = a.in[c]; u[m] = a.in[2 * N + c]; vp[m] = a.in[3 * N + c]; w[m] = a.in[4 * N + c];
-        v0[m] = a.vis ? a.vis[c] : real(0);
-        v1[m] = a.vis ? a.vis[N + c] : real(0);
-    }
-#pragma unroll
-    for (int m = 0; m < SRC_CPT; ++m) {
-        const size_t c = c0 + (size_t)m * 256;
-        if (c >= N) break;
-        const int i = (int)(c % nx), jk = (int)(c / nx), j = jk % ny, k = jk / ny;
-        const int ip = i + NPAD, kp = k + NPAD;
-        const real rad = a.xc[ip];
-        const real phi = a.y0 + (real(j) + real(0.5)) * a.dy;
-        real gx, gy, gz;
-        sph_forces(a, pl, planet != 0, rad, phi, kp, gx, gy, gz);
-        const real vb = a.vbg[(size_t)k * (nx + 2 * NPAD) + ip];
-        const real vis0 = FM<real>::div(v0[m], rho[m]), vis1 = FM<real>::div(v1[m], rho[m]);
-        const real ir = FM<real>::rcp(rad), cot = a.cotz[kp];
-        // get_g_forces (sweeps.cu:555-576): fictitious forces of the state T
-        auto forces = [&](real Tv, real Tw, real& fx, real& fy, real& fz) {
-            fz = Tv * Tv * ir * cot;
-            fy = vis1;
-            fx = Tv * Tv * ir + Tw * Tw * ir + vis0;
-        };
-        real fx, fy, fz;
-        forces(vp[m] + vb, w[m], fx, fy, fz);
-        if (mdt != real(0)) {
-            const real Tv = vp[m] + vb + fy * mdt + gy * mdt, Tw = w[m] + fz * mdt + gz * mdt;
-            forces(Tv, Tw, fx, fy, fz);
-        }
-        const real un = u[m] + fx * dt + gx * dt, vn = vp[m] + fy * dt + gy * dt, wn = w[m] + fz * dt + gz * dt;
-        a.out[2 * N + c] = un;
-        a.out[3 * N + c] = vn;
-        a.out[4 * N + c] = wn;
-        if (a.out != a.in) {
-            a.out[c] = rho[m];
-            a.out[N + c] = a.in[N + c];
-        }
-        if (rate) rmax = fmax(rmax, cell_rate(a, ip, kp, rho[m], a.in[N + c], un, vn, wn));
-    }
-    if (rate) {
-        __shared__ real red[256];
-        red[threadIdx.x] = rmax;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomic_max_nonneg(a.rate + blockIdx.x % CFL_BLOCKS, red[0]);
     }
 }
 
@@ -952,6 +974,7 @@ struct Sph {
     hipStream_t stream = nullptr;
     void* st[3] = {nullptr, nullptr, nullptr};   // state buffers A, B, predicted
     void* vis = nullptr;
+    void* vis2 = nullptr;       // the fused predictor writes here, then the two swap
     void* tab = nullptr;        // all real tables, one allocation
     double* dtab = nullptr;     // FP64 tables: rot, rcyl
     void* rate = nullptr;
@@ -1084,13 +1107,40 @@ int launch_orbadv(Sph* s, int in, int out, double dt, int* inc_d, double* res_d)
 }
 
 template <typename real>
+PlanetS<real> planet_view(const Body& b) {
+    PlanetS<real> pl;
+    pl.m = (real)b.m; pl.x = (real)b.x; pl.y = (real)b.y; pl.rs = (real)b.rs;
+    pl.shy = (real)std::sin(0.5 * b.y);
+    pl.chy = (real)std::cos(0.5 * b.y);
+    return pl;
+}
+
+template <typename real>
 int launch_vis(Sph* s, int in) {
     SphK<real> a = kargs<real>(s, in, -1);
     a.vis_out = (real*)s->vis;
     Timed tm(s, "viscosity");
     const unsigned nb = (unsigned)(((s->nx + VX - 1) / VX) * ((s->ny + VY - 1) / VY) * s->nz);
-    hipLaunchKernelGGL(k3_viscosity<real>, dim3(nb), dim3(256), 0, s->stream, a);
+    hipLaunchKernelGGL((k3_viscosity<real, false>), dim3(nb), dim3(256), 0, s->stream, a, PlanetS<real>{}, 0, real(0));
     SPH_TRY(hipGetLastError());
+    return PG_OK;
+}
+
+// the predictor and the viscosity of the predicted state in one pass: vis
+// (the start-of-step viscosity) is read, the predicted viscosity written to
+// the second viscosity buffer, which the caller then swaps in
+template <typename real>
+int launch_predvis(Sph* s, int in, const Body& b, double hdt) {
+    SphK<real> a = kargs<real>(s, in, -1);
+    a.vis = (const real*)s->vis;
+    a.vis_out = (real*)s->vis2;
+    PlanetS<real> pl = planet_view<real>(b);
+    Timed tm(s, "predictor_viscosity");
+    const unsigned nb = (unsigned)(((s->nx + VX - 1) / VX) * ((s->ny + VY - 1) / VY) * s->nz);
+    hipLaunchKernelGGL((k3_viscosity<real, true>), dim3(nb), dim3(256), 0, s->stream, a, pl,
+                       s->P.n_planet > 0 ? 1 : 0, (real)hdt);
+    SPH_TRY(hipGetLastError());
+    std::swap(s->vis, s->vis2);
     return PG_OK;
 }
 
@@ -1098,10 +1148,7 @@ template <typename real>
 int launch_sources(Sph* s, int in, int out, const Body& b, double mdt, double dt, bool rate = false) {
     SphK<real> a = kargs<real>(s, in, out);
     a.vis = s->P.visc_flag ? (const real*)s->vis : nullptr;
-    PlanetS<real> pl;
-    pl.m = (real)b.m; pl.x = (real)b.x; pl.y = (real)b.y; pl.rs = (real)b.rs;
-    pl.shy = (real)std::sin(0.5 * b.y);
-    pl.chy = (real)std::cos(0.5 * b.y);
+    PlanetS<real> pl = planet_view<real>(b);
     if (rate) SPH_TRY(hipMemsetAsync(s->rate, 0, CFL_BLOCKS * sizeof(real), s->stream));
     Timed tm(s, "sources");
     hipLaunchKernelGGL(k3_sources<real>, dim3((unsigned)((s->N + 256 * SRC_CPT - 1) / (256 * SRC_CPT))), dim3(256), 0,
@@ -1144,10 +1191,7 @@ int ds_step(Sph* s, double t, double dt) {
     if ((rc = launch_sweep<real>(s, 2, 0, 1, dt))) return rc;
     if ((rc = launch_orbadv<real>(s, 1, 0, dt, nullptr, nullptr))) return rc;
     evolve_planet(s, t + dt);
-    if (visc) {
-        if ((rc = launch_sources<real>(s, 0, 2, s->planet, hdt, hdt))) return rc;
-        if ((rc = launch_vis<real>(s, 2))) return rc;
-    }
+    if (visc && (rc = launch_predvis<real>(s, 0, s->planet, hdt))) return rc;
     if ((rc = launch_sources<real>(s, 0, 0, s->planet, hdt, hdt, true))) return rc;
     s->rate_valid = true;
     return PG_OK;
@@ -1325,7 +1369,8 @@ int create_t(Sph* s) {
         s->bytes += (size_t)NF * N * sizeof(real);
     }
     SPH_TRY(hipMalloc(&s->vis, 2 * N * sizeof(real)));
-    s->bytes += 2 * N * sizeof(real);
+    SPH_TRY(hipMalloc(&s->vis2, 2 * N * sizeof(real)));
+    s->bytes += 4 * N * sizeof(real);
     SPH_TRY(hipMemset(s->vis, 0, 2 * N * sizeof(real)));
     SPH_TRY(hipMalloc(&s->rate, CFL_BLOCKS * sizeof(real)));
     SPH_TRY(hipHostMalloc(&s->rate_h, CFL_BLOCKS * sizeof(real), hipHostMallocDefault));
@@ -1394,6 +1439,7 @@ void sph_destroy(Sph* s) {
     for (void* p : s->st)
         if (p) (void)hipFree(p);
     if (s->vis) (void)hipFree(s->vis);
+    if (s->vis2) (void)hipFree(s->vis2);
     if (s->tab) (void)hipFree(s->tab);
     if (s->dtab) (void)hipFree(s->dtab);
     if (s->rate) (void)hipFree(s->rate);
