@@ -809,7 +809,7 @@ __global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a, PlanetS<real> 
     const size_t N = a.N;
     const int nxe = nx + 2 * NPAD;
     const int ntx = (nx + VX - 1) / VX, nty = (ny + VY - 1) / VY;
-    const int b = blockIdx.x, k = b / (ntx * nty), rem = b - k * ntx * nty;
+    const int b = xcd_tile(blockIdx.x, gridDim.x), k = b / (ntx * nty), rem = b - k * ntx * nty;
     const int i0 = (rem % ntx) * VX, j0 = (rem / ntx) * VY;
     const int t = threadIdx.x;
     for (int e = t; e < HX * HY; e += 256) {
