@@ -12,16 +12,17 @@ time loop does it.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line.  `value` is interior cell updates per second over the
-K timed steps (state resident in HBM before timing starts).  `roofline`
-describes the dominant kernel: algorithmic bytes per launch over its average
-HIP-event duration in the timed region, against 8 TB/s; its `traffic` is the
+K timed steps (state resident in HBM before timing starts; the device loop
+replays its captured step).  `roofline` describes the dominant kernel:
+algorithmic bytes per launch over its average duration from HIP events
+around every launch of further uncaptured steps, against 8 TB/s; its `traffic` is the
 HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
 passes (profiles/).  `cpu_baseline` times the FP64 host oracle (OpenMP) on a
 bounded sample of the same workload.  `batched` times 64 uneven disks
 advanced together (one launch per pass, captured step, autotuned schedule)
 against looping the single-disk path over the same disks.  `adjoint` times
 the backward sweep of the gap-profile loss on the 512x1024 disk against the
-plain forward run.
+plain forward run.  `sph` times the 3D adiabatic disk (384x1536x64).
 """
 from __future__ import annotations
 
@@ -62,7 +63,10 @@ def parse():
     ap.add_argument("--nphi", type=int, default=3072)
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--graph", action="store_true", help="also time the hipGraph-replayed loop")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the uncaptured loop (one launch per pass) instead of the replayed hipGraph step")
+    ap.add_argument("--kernel-steps", type=int, default=50,
+                    help="uncaptured steps with HIP events around every launch, for the per-kernel durations")
     ap.add_argument("--batch-disks", type=int, default=64, help="disks of the batched leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the adjoint leg (0 skips it)")
@@ -261,15 +265,23 @@ def main():
     word = a.precision
     cells = p.xres * p.yres
     d = Disk(p)
-    d.run(a.warmup, use_graph=False)
+    graph = not a.no_graph
+    d.run(a.warmup, use_graph=graph)
 
-    # timed region: K device-loop steps, HIP events around every launch
-    d.profile(True)
+    # timed region: K steps of the device loop (dt on the device), replaying
+    # the captured step; nothing else runs on the stream
     t0 = time.perf_counter()
-    dts = d.run(a.steps, use_graph=False)
+    dts = d.run(a.steps, use_graph=graph)
     t1 = time.perf_counter()
     elapsed = t1 - t0
     value = cells * a.steps / elapsed
+    # per-kernel durations: HIP events on the launch stream around every
+    # launch of further uncaptured steps (the events cost ~40 us per step
+    # in launch gaps, so they stay out of the throughput region)
+    d.profile(True)
+    k0 = time.perf_counter()
+    d.run(a.kernel_steps, use_graph=False)
+    kel = time.perf_counter() - k0
     kernels = {}
     for name, words in WORDS_PER_CELL.items():
         avg_us, cnt = d.kernel_time_us(name)
@@ -290,12 +302,6 @@ def main():
     d.set_state(hg, hd)
     d.get_state()
     host_rt = time.perf_counter() - h0
-    graph_value = None
-    if a.graph:
-        d.run(2, use_graph=True)
-        g0 = time.perf_counter()
-        d.run(a.steps, use_graph=True)
-        graph_value = cells * a.steps / (time.perf_counter() - g0)
 
     batched = batched_leg(a.batch_disks, a.batch_steps, a.precision) if a.batch_disks > 0 else None
     adjoint = adjoint_leg(a.adjoint_steps, a.precision) if a.adjoint_steps > 0 else None
@@ -322,6 +328,9 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": round(1e3 * elapsed / a.steps, 4),
+        "timing": ("captured step replayed (hipGraph)" if graph else "uncaptured launches") +
+                  f"; kernel durations from HIP events over {a.kernel_steps} further uncaptured steps "
+                  f"({1e3 * kel / a.kernel_steps:.4f} ms per step with the events)",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -341,8 +350,6 @@ def main():
         "host_state_roundtrip_ms": round(1e3 * host_rt, 3),
         "cpu_baseline": cpu,
     }
-    if graph_value is not None:
-        out["graph_value"] = round(graph_value, 1)
     if batched is not None:
         out["batched"] = batched
     if adjoint is not None:
