@@ -121,30 +121,6 @@ struct Par {
     int kL, kR;
 };
 
-// get_PEM_0 / get_PEM_1 (PEM.cu:3-48)
-template <typename real>
-__device__ __forceinline__ real pem0(real x, real S, real aB, real eta) {
-    real v = x == real(0) ? real(0) : FM<real>::exp(eta * flog(x));
-    return aB + S * v;
-}
-template <typename real>
-__device__ __forceinline__ real pem1(real x, real S, real aB, real eta) {
-    real v;
-    if (eta * x > real(0.001)) {
-        v = FM<real>::div((real(1) - x) * (FM<real>::exp(eta * flog(real(1) - x)) - real(1)), eta * x) + real(1);
-    } else {
-        real t = (eta + real(1)) * x / real(2);
-        v = t;
-        t *= -(eta - real(1)) * x / real(3);
-        v += t;
-        t *= -(eta - real(2)) * x / real(4);
-        v += t;
-        t *= -(eta - real(3)) * x / real(5);
-        v += t;
-    }
-    return aB + S * v;
-}
-
 // the PEM4 branch choices of get_PEM_aveL / aveR (PEM.cu:51-93)
 template <typename real>
 __device__ __forceinline__ void pem_select(Par<real>& p) {
@@ -160,19 +136,47 @@ __device__ __forceinline__ void pem_select(Par<real>& p) {
     p.eR = rr > real(1) ? rr : rl;
 }
 
+// get_PEM_0 / get_PEM_1 (PEM.cu:3-48) through one logarithm and one
+// exponential: with y = x (PEM_0) or 1 - x (PEM_1), E = y^eta; PEM_0 is E,
+// PEM_1 is (1 - x)(E - 1)/(eta x) + 1 or its series below eta x = 0.001 (so
+// lanes of a wave that take different forms share the transcendental work).
+// get_PEM_1's x >= 1 assignment is overwritten by the branch after it
+// (PEM.cu:23-27) and has no counterpart.
+template <typename real>
+__device__ __forceinline__ real pem01(bool zero_form, real x, real S, real aB, real eta) {
+    const real y = zero_form ? x : real(1) - x;
+    const real E = y == real(0) ? real(0) : FM<real>::exp(eta * flog(y));
+    real v;
+    if (zero_form) {
+        v = E;
+    } else if (eta * x > real(0.001)) {
+        v = FM<real>::div(y * (E - real(1)), eta * x) + real(1);
+    } else {
+        real t = (eta + real(1)) * x / real(2);
+        v = t;
+        t *= -(eta - real(1)) * x / real(3);
+        v += t;
+        t *= -(eta - real(2)) * x / real(4);
+        v += t;
+        t *= -(eta - real(3)) * x / real(5);
+        v += t;
+    }
+    return aB + S * v;
+}
+
 // get_CON_aveL (recon.cu:424-441): mean over the first fraction q
 template <int RECON, typename real>
 __device__ __forceinline__ real avgL(real q, const Par<real>& p) {
     if (RECON == 6 || p.kL == 1) return aveL(q, p.sL, p.a, p.sR);
     if (p.kL == 0) return p.a;
-    return p.kL == 2 ? pem0(q, p.sL, p.a - p.sL, p.eL) : pem1(q, p.sL, p.a - p.sL, p.eL);
+    return pem01(p.kL == 2, q, p.sL, p.a - p.sL, p.eL);
 }
 // get_CON_aveR (recon.cu:405-422): mean over the cell from q to its right edge
 template <int RECON, typename real>
 __device__ __forceinline__ real avgR(real q, const Par<real>& p) {
     if (RECON == 6 || p.kR == 1) return aveR(real(1) - q, p.sL, p.a, p.sR);
     if (p.kR == 0) return p.a;
-    return p.kR == 3 ? pem1(real(1) - q, -p.sR, p.a + p.sR, p.eR) : pem0(real(1) - q, -p.sR, p.a + p.sR, p.eR);
+    return pem01(p.kR == 2, real(1) - q, -p.sR, p.a + p.sR, p.eR);
 }
 
 // 4th-order limited slopes (recon.cu:216-378); under RECON 6 also the
