@@ -113,6 +113,11 @@ int pg_read_snapshot(pg_handle* h, const char* filename, double* t_out);
    the snapshot of frame n is "binary_" + label + "_" + 5-digit n (main.cu:403). */
 int pg_snapshot_label(const pg_params* params, char* buf, size_t len);
 
+/* ---- EOS (get_energy, EOS.h:4 / EOS.cu:4-18) ------------------------------ */
+/* Specific energy of a cell under params->EOS_flag: 0 p/rho, 1 p/rho^gam,
+   2 p/((gam-1) rho) (internal_e_flag 1, the only form the library runs). */
+int pg_get_energy(const pg_params* params, double rho, double p, double u, double v, double w, double* e);
+
 /* ---- diagnostics ------------------------------------------------------- */
 /* Start a time average of the gas over a window of `interval`: zero the
    average (init_average, output.cu:316-329) and from now on add

@@ -1965,6 +1965,23 @@ int pg_killwave(pg_handle* h, double dt) {
     return PG_OK;
 }
 
+int pg_get_energy(const pg_params* P, double rho, double p, double u, double v, double w, double* e) {
+    if (!P || !e) return fail(PG_E_INVALID, "null argument");
+    if (!(rho > 0.0)) return fail(PG_E_INVALID, "rho must be positive");
+    (void)u; (void)v; (void)w;   // kinetic terms only under internal_e_flag 0
+    if (P->EOS_flag == 2) {
+        if (!(P->gam > 1.0)) return fail(PG_E_INVALID, "EOS_flag 2 needs gam > 1");
+        *e = p / ((P->gam - 1.0) * rho);
+    } else if (P->EOS_flag == 1) {
+        *e = p / std::pow(rho, P->gam);
+    } else if (P->EOS_flag == 0) {
+        *e = p / rho;
+    } else {
+        return fail(PG_E_INVALID, "EOS_flag=%d", P->EOS_flag);
+    }
+    return PG_OK;
+}
+
 int pg_snapshot_label(const pg_params* P, char* buf, size_t len) {
     if (!P || !buf || len == 0) return fail(PG_E_INVALID, "null argument");
     // create_label (output.cu:14-93): each number passes through an int,

@@ -51,6 +51,8 @@ SIGNATURES = {
     "pg_write_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
     "pg_read_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
     "pg_snapshot_label": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_size_t]),
+    "pg_get_energy": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, _dp]),
     "pg_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]),
     "pg_adjoint_backward": (ctypes.c_int, [_vp, _dp, _vp]),
     "pg_adjoint_pass": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),

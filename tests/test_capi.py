@@ -112,3 +112,24 @@ def test_batch_configuration_is_checked_before_any_device_work():
     t = (ctypes.c_double * 1)()
     assert L.pg_batch_run(None, 1, t, t, None, 1) == -1
     assert L.pg_batch_set_schedule(None, 256, 512, 0, 0) == -1
+
+
+def test_eos_energy_matches_the_oracle_hllc_energy_flux():
+    """pg_get_energy (EOS.cu:4-18) against the energy the oracle's HLLC carries
+    (F_p / F_rho of a uniform state is the upwinded specific energy)."""
+    import numpy as np
+    import oracle
+    from penguin_amd import disk3d
+    L = load_library()
+    e = ctypes.c_double()
+    p3 = disk3d(16, 16, 8, gam=1.4)
+    for r, p, u in ((1.0, 0.7, 0.3), (0.2, 0.05, -0.1), (3.0, 2.0, 1.5)):
+        assert L.pg_get_energy(ctypes.byref(p3), r, p, u, 0.1, -0.2, ctypes.byref(e)) == 0
+        S = np.array([r, p, u, 0.1, -0.2] * 2, dtype=float)
+        out = np.zeros(10)
+        oracle.lib().orc_riemann_hllc(1.4, 2, oracle.ptr(S), oracle.ptr(out))
+        assert abs(e.value - out[5] / out[4]) <= 1e-14 * e.value
+        assert abs(e.value - p / (0.4 * r)) <= 1e-15 * e.value
+    p2 = config_c2(xres=64, yres=64)                     # EOS_flag 0: p / rho
+    assert L.pg_get_energy(ctypes.byref(p2), 2.0, 3.0, 0.0, 0.0, 0.0, ctypes.byref(e)) == 0 and e.value == 1.5
+    assert L.pg_get_energy(ctypes.byref(p3), -1.0, 1.0, 0.0, 0.0, 0.0, ctypes.byref(e)) == -1
