@@ -80,7 +80,9 @@ def adjoint_leg(steps: int, precision: int) -> dict:
     """Gap-profile loss on the 512x1024 gas disk (q 1e-4, alpha 1e-3):
     forward `steps` with checkpoints every 20, backward through the adjoint;
     cost = backward wall time (recompute included) / plain forward wall time
-    (pg_run with its captured step) over the same steps."""
+    (pg_run with its captured step) over the same steps.  The tape_* keys:
+    the same with every step's intermediates kept in HBM (ckpt_every 0, no
+    recompute)."""
     import numpy as np
     from penguin_amd import Disk, config_c5
     from penguin_amd.adjoint import gap_profile
@@ -109,10 +111,30 @@ def adjoint_leg(steps: int, precision: int) -> dict:
     dq, dal, d0 = d.adjoint_backward(dL, out=d0)
     t_bwd = time.perf_counter() - t0
     d.close()
+    # the tape (ckpt_every 0): the forward keeps every step's intermediates in
+    # HBM and the backward recomputes nothing; timed on a second call, after
+    # the first allocated the tape
+    d = Disk(p)
+    d.adjoint_forward(steps, 0)
+    d.adjoint_backward(dL, out=d0)
+    d.close()
+    d = Disk(p)
+    dev0 = d.device_bytes()
+    t0 = time.perf_counter()
+    d.adjoint_forward(steps, 0)
+    t_tfwd = time.perf_counter() - t0
+    tape_bytes = d.device_bytes() - dev0
+    t0 = time.perf_counter()
+    tq, tal, d1 = d.adjoint_backward(dL, out=np.zeros((5, d.ny, d.nx)))
+    t_tbwd = time.perf_counter() - t0
+    d.close()
     return {"workload": f"512x1024 gas disk, q 1e-4, alpha 1e-3, {steps} steps, checkpoints every 20, "
                         "loss = L2 of <Sigma>_phi(r) to a Gaussian-gap target",
             "forward_s": round(t_fwd, 4), "forward_with_checkpoints_s": round(t_afwd, 4),
             "backward_s": round(t_bwd, 4), "cost_ratio": round(t_bwd / t_fwd, 3),
+            "tape_forward_s": round(t_tfwd, 4), "tape_backward_s": round(t_tbwd, 4),
+            "tape_cost_ratio": round(t_tbwd / t_fwd, 3), "tape_bytes": int(tape_bytes),
+            "tape_dL_dq_rel_diff": abs(tq - dq) / max(abs(dq), 1e-300),
             "loss": loss, "dL_dq": dq, "dL_dalpha": dal, "dL_dsigma0_norm": float(np.linalg.norm(d0[0]))}
 
 

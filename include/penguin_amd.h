@@ -165,12 +165,15 @@ typedef struct pg_grads {
 } pg_grads;
 /* Run nsteps of the device loop from the current state (time t0, previous
    step dt0, as pg_run), keeping the state every ckpt_every steps and the
-   step sizes (dts, nsteps doubles, may be NULL). */
+   step sizes (dts, nsteps doubles, may be NULL).  ckpt_every = 0 keeps every
+   step's intermediates instead (the tape: 22 reals per cell and step of
+   device memory, e.g. 46 MB per step for a 512x1024 FP32 disk), and the
+   backward sweep then recomputes nothing; not with the time average on. */
 int pg_adjoint_forward(pg_handle* h, int nsteps, int ckpt_every, double t0, double dt0, double* dts);
 /* Given dL/d(final state) in pg_get_state layout (5*xres*yres; the p entry
    is folded in as c_s^2 dL/dp on rho), propagate back through every step of
    the last pg_adjoint_forward, recomputing each segment from its
-   checkpoint; fills *grads.  The handle's state is left at the forward's
+   checkpoint (or reading the tape); fills *grads.  The handle's state is left at the forward's
    final state. */
 int pg_adjoint_backward(pg_handle* h, const double* dL_dstate, pg_grads* grads);
 /* Adjoint of one module call at the current state: the transpose of

@@ -2,7 +2,8 @@
 
 ``DiskEvolve`` is a ``torch.autograd.Function``: forward runs ``nsteps`` of
 the device loop from an initial surface density with a given planet mass and
-viscosity alpha (checkpointing every ``ckpt_every`` steps) and returns the
+viscosity alpha (checkpointing every ``ckpt_every`` steps; 0 keeps every
+step's intermediates in device memory and recomputes nothing) and returns the
 final surface density; backward runs the adjoint of every step
 (pg_adjoint_backward) and returns dL/dSigma0, dL/dq, dL/dalpha.  The step
 sizes of the forward pass are frozen in the backward pass.

@@ -1248,7 +1248,10 @@ int group_autotune(Group& g, int steps, double* best_ms) {
 // `ckpt` steps and the step sizes; the backward sweep, segment by segment
 // from the last, recomputes the segment once from its checkpoint, storing
 // each step's intermediates in a slot, then runs the adjoint passes of its
-// steps in reverse order.
+// steps in reverse order.  With ckpt 0 (the tape) every step has a slot of
+// its own, written by the forward run itself, and the backward sweep runs the
+// adjoint passes alone: no recompute, at SLOT_WORDS reals per cell and step
+// of device memory (a 512 x 1024 FP32 disk: 46 MB per step).
 constexpr int SLOT_WORDS = 22;   // S0, S1, S2, SA, S3, T (3 fields each), V0, V1 (2 each)
 enum { SL_S0 = 0, SL_S1 = 3, SL_S2 = 6, SL_SA = 9, SL_S3 = 12, SL_T = 15, SL_V0 = 18, SL_V1 = 20 };
 // forward passes of one slot (the fused kernels of the forward step, which
@@ -1324,8 +1327,8 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
     AdjWork* w = new AdjWork();
     w->nsteps = nsteps;
     w->ckpt = ckpt;
-    w->nseg = (nsteps + ckpt - 1) / ckpt;
-    w->slots = std::min(ckpt, nsteps) + 1;
+    w->nseg = ckpt > 0 ? (nsteps + ckpt - 1) / ckpt : 0;
+    w->slots = (ckpt > 0 ? std::min(ckpt, nsteps) : nsteps) + 1;
     w->prec = g.prec;
     w->N = d->N;
     w->nslot = adj_slots(d->nx, d->ny);
@@ -1335,12 +1338,12 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
         adj_free(w);
         return code;
     };
-    if ((rc = dev_alloc(w->bytes, &w->ckpts, (size_t)w->nseg * 3 * N * es)) ||
+    if ((w->nseg > 0 && (rc = dev_alloc(w->bytes, &w->ckpts, (size_t)w->nseg * 3 * N * es))) ||
         (rc = dev_alloc(w->bytes, &w->slot, (size_t)w->slots * SLOT_WORDS * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->lam, (size_t)8 * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->lparam, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->td, 2 * (size_t)nsteps * sizeof(double))) ||
-        (rc = dev_alloc(w->bytes, &w->td_seg, 2 * (size_t)ckpt * sizeof(double))) ||
+        (rc = dev_alloc(w->bytes, &w->td_seg, 2 * (size_t)(ckpt > 0 ? ckpt : 1) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))) ||
         (rc = dev_alloc(w->bytes, &w->sctl, (size_t)w->slots * sizeof(StepCtl))) ||
         (rc = dev_alloc(w->bytes, &w->sinc, (size_t)w->slots * d->nx * sizeof(int))) ||
@@ -1508,10 +1511,80 @@ int adj_check_config(const pg_params& P) {
     return PG_OK;
 }
 
+// The tape forward: the device loop's step (prologue mode 1: the CFL step
+// from the previous corrector's rate, capped at 1.1 dt_prev and clipped to
+// end_time) with the recompute's kernels and descriptors, so that step n's
+// intermediates, control block and orbital shift stay in slot n.  Each
+// slot's control block starts as a copy of the previous one, as the disk's
+// own block carries over from step to step in group_run; the arithmetic is
+// the device loop's, so the states and step sizes equal pg_run's.
+template <typename real>
+int adj_tape_forward(pg_handle* h, int nsteps, double t0, double dt0) {
+    Group& g = h->g;
+    AdjWork* w = h->adj;
+    DiskDev* d = g.disks[0];
+    const size_t N = w->N;
+    hipStream_t s = g.stream;
+    int rc;
+    if ((rc = pull_ctl(g))) return rc;
+    StepCtl& c = *d->ctl_host;
+    c.t = t0;
+    c.dt = dt0;
+    c.dt_prev = dt0;
+    c.nstep = 0;
+    c.mode = 1;
+    c.end_time = d->P.end_time;
+    c.rate_bits = 0ull;
+    if ((rc = push_ctl(g))) return rc;
+    if ((rc = launch_cfl(g))) return rc;   // rate of the start state
+    real* base = (real*)w->slot;
+    HIPL(hipMemcpyAsync(base + SL_S0 * N, d->gas[g.gpar], 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
+    HIPL(hipMemcpyAsync(w->sctl, d->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, s));
+    const LaunchSet<real> L = g.ls<real>();
+    const bool visc = g.visc;
+    for (int m = 0; m < nsteps; ++m) {
+        const DiskArgs<real>* a = (const DiskArgs<real>*)w->args + (size_t)m * AA_N;
+        auto at = [&](int k) {
+            LaunchSet<real> x = L;
+            x.disks = a + k;
+            x.ndisk = 1;
+            return x;
+        };
+        real* sl = base + (size_t)m * SLOT_WORDS * N;
+        real* nxt = base + (size_t)(m + 1) * SLOT_WORDS * N;
+        if (m > 0) HIPL(hipMemcpyAsync(w->sctl + m, w->sctl + m - 1, sizeof(StepCtl), hipMemcpyDeviceToDevice, s));
+        HIPL(Launch<real>::prologue(at(AA_X), 1, 1, s));
+        if (visc && m == 0) HIPL(Launch<real>::viscosity(at(AA_V0), 0, 0, s));
+        HIPL(Launch<real>::sweep_x_gas(at(AA_X), 0, 1, s));
+        HIPL(Launch<real>::ring_gas(at(AA_P), 0, 3, s));
+        if (visc) {
+            HIPL(Launch<real>::predvis(at(AA_PRED), 0, s));
+            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, s));
+        } else {
+            HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
+            HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 1, s));
+        }
+    }
+    // the disk ends at the last step's result, with the last step's control
+    // block (t and dt of that step, as the device loop leaves them)
+    HIPL(hipMemcpyAsync(d->gas[g.gpar], base + (size_t)nsteps * SLOT_WORDS * N + SL_S0 * N, 3 * N * sizeof(real),
+                        hipMemcpyDeviceToDevice, s));
+    HIPL(hipMemcpyAsync(d->ctl, w->sctl + nsteps - 1, sizeof(StepCtl), hipMemcpyDeviceToDevice, s));
+    if ((rc = pull_ctl(g))) return rc;
+    std::vector<StepCtl> cs(nsteps);
+    HIPL(hipMemcpy(cs.data(), w->sctl, sizeof(StepCtl) * nsteps, hipMemcpyDeviceToHost));
+    for (int n = 0; n < nsteps; ++n) {
+        w->ts[n] = cs[n].t;
+        w->dts[n] = cs[n].dt;
+    }
+    return PG_OK;
+}
+
 template <typename real>
 int adj_forward(pg_handle* h, int nsteps, int ckpt, double t0, double dt0, double* dts_out) {
     Group& g = h->g;
     int rc;
+    if (ckpt == 0 && g.ave) return fail(PG_E_STATE, "the tape forward runs without the time average");
     if ((rc = adj_alloc<real>(h, nsteps, ckpt))) return rc;
     AdjWork* w = h->adj;
     w->forward_done = false;
@@ -1529,11 +1602,15 @@ int adj_forward(pg_handle* h, int nsteps, int ckpt, double t0, double dt0, doubl
                             hipMemcpyDeviceToDevice, g.stream));
         if ((rc = group_run(g, len, &t, &dt, w->dts.data() + n0, 1, nullptr))) return rc;
     }
-    // start times, summed in the order the device loop sums them
-    double tt = t0;
-    for (int n = 0; n < nsteps; ++n) {
-        w->ts[n] = tt;
-        tt += w->dts[n];
+    if (ckpt == 0) {
+        if ((rc = adj_tape_forward<real>(h, nsteps, t0, dt0))) return rc;
+    } else {
+        // start times, summed in the order the device loop sums them
+        double tt = t0;
+        for (int n = 0; n < nsteps; ++n) {
+            w->ts[n] = tt;
+            tt += w->dts[n];
+        }
     }
     std::vector<double> td(2 * (size_t)nsteps);
     for (int n = 0; n < nsteps; ++n) {
@@ -1544,8 +1621,10 @@ int adj_forward(pg_handle* h, int nsteps, int ckpt, double t0, double dt0, doubl
     if (dts_out) std::memcpy(dts_out, w->dts.data(), sizeof(double) * nsteps);
     // the slots' control blocks start as the disk's (planet elements,
     // softening); each slot's replay rewrites the step-dependent fields
-    for (int m = 0; m < w->slots; ++m)
-        HIPL(hipMemcpyAsync(w->sctl + m, g.disks[0]->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, g.stream));
+    // (the tape's were written by its own forward steps)
+    if (ckpt > 0)
+        for (int m = 0; m < w->slots; ++m)
+            HIPL(hipMemcpyAsync(w->sctl + m, g.disks[0]->ctl, sizeof(StepCtl), hipMemcpyDeviceToDevice, g.stream));
     HIPL(hipStreamSynchronize(g.stream));
     w->forward_done = true;
     return PG_OK;
@@ -1579,6 +1658,13 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
     HIPL(hipMemsetAsync(w->lparam, 0, 2 * (size_t)w->nslot * sizeof(double), g.stream));
     HIPL(hipMemsetAsync((real*)w->lam + 6 * N, 0, 2 * N * sizeof(real), g.stream));
     const size_t bytes = 3 * N * sizeof(real);
+    if (w->ckpt == 0) {
+        // the tape: the adjoint steps alone, last to first (launched
+        // directly: the host issues a step's five launches well within the
+        // step's device time)
+        for (int m = w->nsteps - 1; m >= 0; --m)
+            if ((rc = adj_step<real>(h, m, lamA, lamB))) return rc;
+    }
     for (int k = w->nseg - 1; k >= 0; --k) {
         const int n0 = k * w->ckpt, len = std::min(w->ckpt, w->nsteps - n0);
         HIPL(hipMemcpyAsync(w->slot, (char*)w->ckpts + (size_t)k * bytes, bytes, hipMemcpyDeviceToDevice, g.stream));
@@ -2200,7 +2286,7 @@ int pg_device_bytes(pg_handle* h, size_t* bytes) {
     CHECK_H(h);
     if (!bytes) return fail(PG_E_INVALID, "null argument");
     if (h->sph) return pg::sph_device_bytes(h->sph, bytes);
-    *bytes = h->g.bytes;
+    *bytes = h->g.bytes + (h->adj ? h->adj->bytes : 0);   // with the adjoint's workspace
     return PG_OK;
 }
 
@@ -2208,7 +2294,7 @@ int pg_device_bytes(pg_handle* h, size_t* bytes) {
 int pg_adjoint_forward(pg_handle* h, int nsteps, int ckpt_every, double t0, double dt0, double* dts) {
     CHECK_H(h);
     CHECK_2D(h);
-    if (nsteps < 1 || ckpt_every < 1) return fail(PG_E_INVALID, "nsteps and ckpt_every must be >= 1");
+    if (nsteps < 1 || ckpt_every < 0) return fail(PG_E_INVALID, "nsteps must be >= 1 and ckpt_every >= 0");
     int rc = adj_check_config(h->g.disks[0]->P);
     if (rc) return rc;
     return h->g.prec == 8 ? adj_forward<double>(h, nsteps, ckpt_every, t0, dt0, dts)

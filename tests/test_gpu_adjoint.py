@@ -158,3 +158,60 @@ def test_autograd_wrapper():
     assert abs(float(q.grad) - dq) <= 1e-12 * abs(dq)
     assert abs(float(al.grad) - dal) <= 1e-12 * abs(dal)
     assert np.allclose(sig.grad.numpy(), d0[0], rtol=0, atol=1e-14 * np.abs(d0[0]).max())
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("visc", [1, 0])
+def test_adjoint_tape_equals_checkpoints(prec, visc):
+    """ckpt_every 0 (the forward keeps every step's intermediates, the
+    backward recomputes nothing): the device loop's step sizes and final
+    state bit for bit (the same kernels on the same inputs, the step chosen
+    on the device), and the checkpointed sweep's gradients (whose segments
+    restart the viscous forces and the CFL rate from their checkpoints: equal
+    to rounding)."""
+    p = _params(prec, visc_flag=visc)
+    w = _weights(p)
+    d = Disk(p)
+    _smooth_state(d, p)
+    dts_run = d.run(7, use_graph=True)
+    g_run, _ = d.get_state()
+    d.close()
+    L0, dts0, dq0, dal0, d00, _ = _adjoint(p, 7, 3, w)
+    d = Disk(p)
+    _smooth_state(d, p)
+    dts1 = d.adjoint_forward(7, 0)
+    g1, _ = d.get_state()
+    assert np.array_equal(dts1, dts_run)
+    assert np.array_equal(g1, g_run)
+    dL = np.zeros((5, d.ny, d.nx))
+    dL[0] = w
+    dq1, dal1, d01 = d.adjoint_backward(dL)
+    g2, _ = d.get_state()
+    assert np.array_equal(g2, g_run), "the backward sweep must leave the forward's final state"
+    dq2, dal2, d02 = d.adjoint_backward(dL)   # a second sweep over the same tape
+    d.run(1, use_graph=True)                  # the device loop continues from the tape's end
+    d.close()
+    assert dq2 == dq1 and dal2 == dal1 and np.array_equal(d02, d01)
+    tol = 1e-10 if prec == 8 else 1e-4
+    print("tape == checkpoints bitwise:", dq1 == dq0 and dal1 == dal0 and np.array_equal(d01, d00), flush=True)
+    assert np.allclose(dts1, dts0, rtol=tol, atol=0)
+    if p.n_planet:
+        assert abs(dq1 - dq0) <= tol * abs(dq0)
+    if p.visc_flag:
+        assert abs(dal1 - dal0) <= tol * abs(dal0)
+    assert np.abs(d01 - d00).max() <= tol * np.abs(d00).max()
+
+
+def test_adjoint_tape_fd_fp64():
+    _fd_checks(_params(8), 7, 0, 1e-5)
+
+
+def test_adjoint_tape_refuses_the_time_average():
+    from penguin_amd.disk import PenguinError
+    p = _params(8)
+    d = Disk(p)
+    d.average_begin(1.0)
+    with pytest.raises(PenguinError):
+        d.adjoint_forward(3, 0)
+    d.average_end()
+    d.close()
