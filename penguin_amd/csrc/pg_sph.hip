@@ -1187,6 +1187,7 @@ template <typename real>
 int ds_step(Sph* s, double t, double dt) {
     const double hdt = 0.5 * dt;
     int rc;
+    s->rate_valid = false;   // until the corrector has written the new state's rate
     const bool visc = s->P.visc_flag != 0;
     if (visc && (rc = launch_vis<real>(s, 0))) return rc;
     if ((rc = launch_sources<real>(s, 0, 0, s->planet, 0.0, hdt))) return rc;
