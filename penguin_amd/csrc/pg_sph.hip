@@ -51,7 +51,7 @@ namespace {
 constexpr int NF = 5;            // rho, p, u, vp, w
 constexpr int SPH_BT = 256;      // threads per block of the pointwise kernels and the radial sweep
 constexpr int SPH_BT2 = 512;     // threads per block of the strided sweeps (about one element each)
-constexpr int SPH_TX = 256;      // radial segment per block
+constexpr int SPH_BTX = 512;     // most threads per radial-sweep block (one per LDS slot)
 constexpr int SPH_R = 8;         // rows per strided-sweep block
 constexpr int SPH_TY = 64;       // azimuthal segment per block
 constexpr int SPH_ZMAX = 100;    // polar cells: the whole column is one tile (FP64: 102 KB of LDS)
@@ -92,7 +92,18 @@ struct SphK {
     real dy, idy, y0;
     double dy_d;
     real gam;
+    int tx;   // radial segment of the radial sweep (sph_tx)
 };
+
+// Radial segment of the radial sweep: the row cut into the fewest equal
+// parts whose slots (segment + 2 NPAD) fit one block of SPH_BTX threads, so
+// every thread holds one slot and no block runs half empty (a 384-radius row
+// is one 390-slot block of 448 threads).
+inline int sph_tx(int nx) {
+    const int parts = (nx + SPH_BTX - 2 * NPAD - 1) / (SPH_BTX - 2 * NPAD);
+    return (nx + parts - 1) / parts;
+}
+inline int sph_bx(int nx) { return (sph_tx(nx) + 2 * NPAD + 63) / 64 * 64; }
 
 template <typename real>
 struct PlanetS {   // planet for the spherical force: mass, radius, angle, softening
@@ -335,15 +346,16 @@ __device__ __forceinline__ int wrapi(int j, int n) {
 }
 
 // ----------------------------------------------------------------- sweeps
-// DIR 0: radius (rows (j, k), segments of SPH_TX cells); 1: azimuth (R radii
-// x SPH_TY cells per block); 2: polar angle (R radii x the whole column).
+// DIR 0: radius (rows (j, k), segments of a.tx cells, one thread per slot);
+// 1: azimuth (R radii x SPH_TY cells per block); 2: polar angle (R radii x
+// the whole column).
 template <typename real, int DIR, int RECON>
-__global__ __launch_bounds__(DIR == 0 ? SPH_BT : SPH_BT2) void k3_sweep(SphK<real> a, real dt) {
+__global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<real> a, real dt) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     real* sm = reinterpret_cast<real*>(sm_raw);
     constexpr int R = DIR == 0 ? 1 : SPH_R;
-    constexpr int BT = DIR == 0 ? SPH_BT : SPH_BT2;
-    const int L = DIR == 0 ? SPH_TX : DIR == 1 ? SPH_TY : a.nz;
+    const int BT = DIR == 0 ? (int)blockDim.x : SPH_BT2;
+    const int L = DIR == 0 ? a.tx : DIR == 1 ? SPH_TY : a.nz;
     const int S = L + 2 * NPAD;
     const int RS = R * S;         // at most 2 BT (checked by the host)
     real* raw = sm;              // [5][RS]
@@ -359,9 +371,9 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BT : SPH_BT2) void k3_sweep(SphK<rea
     int i0 = 0, j0 = 0, k0 = 0;   // first interior cell of the segment / first row
     const int bt = xcd_tile(blockIdx.x, gridDim.x);
     if (DIR == 0) {
-        const int ntx = (nx + SPH_TX - 1) / SPH_TX;
+        const int ntx = (nx + a.tx - 1) / a.tx;
         const int row = bt / ntx;
-        i0 = (bt - row * ntx) * SPH_TX;
+        i0 = (bt - row * ntx) * a.tx;
         j0 = row % ny;
         k0 = row / ny;
     } else if (DIR == 1) {
@@ -1021,13 +1033,14 @@ SphK<real> kargs(Sph* s, int in, int out) {
     a.y0 = (real)s->P.ymin;
     a.dy_d = dy;
     a.gam = (real)s->P.gam;
+    a.tx = sph_tx(s->nx);
     return a;
 }
 
 template <typename real>
-size_t sweep_lds(int dir, int nz) {
+size_t sweep_lds(int dir, int nx, int nz) {
     const int R = dir == 0 ? 1 : SPH_R;
-    const int L = dir == 0 ? SPH_TX : dir == 1 ? SPH_TY : nz;
+    const int L = dir == 0 ? sph_tx(nx) : dir == 1 ? SPH_TY : nz;
     return (size_t)(3 * NF) * R * (L + 2 * NPAD) * sizeof(real);
 }
 
@@ -1068,16 +1081,16 @@ hipError_t allow_lds(K kernel, size_t bytes) {
 template <typename real>
 int launch_sweep(Sph* s, int dir, int in, int out, double dt) {
     SphK<real> a = kargs<real>(s, in, out);
-    const size_t lds = sweep_lds<real>(dir, s->nz);
+    const size_t lds = sweep_lds<real>(dir, s->nx, s->nz);
     unsigned nb;
-    if (dir == 0) nb = (unsigned)(((s->nx + SPH_TX - 1) / SPH_TX) * s->ny * s->nz);
+    if (dir == 0) nb = (unsigned)(((s->nx + a.tx - 1) / a.tx) * s->ny * s->nz);
     else if (dir == 1) nb = (unsigned)(((s->ny + SPH_TY - 1) / SPH_TY) * ((s->nx + SPH_R - 1) / SPH_R) * s->nz);
     else nb = (unsigned)(((s->nx + SPH_R - 1) / SPH_R) * s->ny);
     Timed tm(s, kSphKernels[dir]);
     const bool pem = s->P.recon_flag == 5;
 #define SPH_SWEEP(D)                                                                                   \
     do {                                                                                               \
-        const int bt = D == 0 ? SPH_BT : SPH_BT2;                                                      \
+        const int bt = D == 0 ? sph_bx(s->nx) : SPH_BT2;                                               \
         if (pem) {                                                                                     \
             SPH_TRY(allow_lds(k3_sweep<real, D, 5>, lds));                                             \
             hipLaunchKernelGGL((k3_sweep<real, D, 5>), dim3(nb), dim3(bt), lds, s->stream, a, (real)dt); \
