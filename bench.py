@@ -139,12 +139,14 @@ def adjoint_leg(steps: int, precision: int) -> dict:
 
 
 # words (reals) each 3D kernel reads and writes per interior cell: the sweeps
-# and the orbital advection read and write the five fields; the viscosity
+# and the orbital advection read and write the five fields (the radial sweep,
+# which applies the first half kick as it loads, also reads the two viscous
+# force components); the viscosity
 # reads rho, u, vp and writes two components; the fused predictor-viscosity
 # reads rho, u, vp, w and two viscous components and writes two; the
 # sources read five fields and two viscous components and write three
 # velocities (in place) or five
-SPH_WORDS = {"sweep_x_gas": 10, "sweep_y_gas": 10, "sweep_z_gas": 10, "orbital_advection": 10,
+SPH_WORDS = {"sweep_x_gas": 12, "sweep_y_gas": 10, "sweep_z_gas": 10, "orbital_advection": 10,
              "viscosity": 5, "predictor_viscosity": 8, "sources": 10, "cfl": 5}
 
 

@@ -345,12 +345,22 @@ __device__ __forceinline__ int wrapi(int j, int n) {
     return j < 0 ? j + n : j;
 }
 
+template <typename real>
+__device__ __forceinline__ void source_cell(const SphK<real>& a, const PlanetS<real>& pl, bool planet, int ip, int kp,
+                                            int j, real rho, real u, real vp, real w, real visr, real visp, real mdt,
+                                            real dt, real& un, real& vn, real& wn);
+
 // ----------------------------------------------------------------- sweeps
 // DIR 0: radius (rows (j, k), segments of a.tx cells, one thread per slot);
 // 1: azimuth (R radii x SPH_TY cells per block); 2: polar angle (R radii x
-// the whole column).
+// the whole column).  DIR 0 with hkick != 0 first applies the step's first
+// half kick (the source terms at t with mdt = 0 over hkick = dt/2, planet pl)
+// to every interior cell it loads, in place of a source-term pass; the block
+// has a thread per slot, so each thread keeps its own cell's kicked state in
+// registers for the update.
 template <typename real, int DIR, int RECON>
-__global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<real> a, real dt) {
+__global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<real> a, real dt, PlanetS<real> pl,
+                                                                         int planet, real hkick) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     real* sm = reinterpret_cast<real*>(sm_raw);
     constexpr int R = DIR == 0 ? 1 : SPH_R;
@@ -389,6 +399,7 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<re
     }
 
     // ---- load (with the sweep's field rotation and angular momenta)
+    real own[NF] = {};   // DIR 0: this thread's cell as loaded (and kicked)
     for (int e = t; e < RS; e += BT) {
         const int s = e / R, r = e - (e / R) * R;
         real v[NF] = {real(1), real(1), real(0), real(0), real(0)};
@@ -401,6 +412,16 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<re
                     const size_t c = (size_t)i + (size_t)nx * ((size_t)j0 + (size_t)ny * k);
                     v[0] = a.in[c]; v[1] = a.in[N + c]; v[2] = a.in[2 * N + c];
                     vp = a.in[3 * N + c]; v[4] = a.in[4 * N + c];
+                    if (hkick != real(0)) {
+                        const real visr = a.vis ? a.vis[c] : real(0), visp = a.vis ? a.vis[N + c] : real(0);
+                        real un, vn, wn;
+                        source_cell(a, pl, planet != 0, ip, k + NPAD, j0, v[0], v[2], vp, v[4], visr, visp, real(0),
+                                    hkick, un, vn, wn);
+                        v[2] = un;
+                        vp = vn;
+                        v[4] = wn;
+                    }
+                    own[0] = v[0]; own[1] = v[1]; own[2] = v[2]; own[3] = vp; own[4] = v[4];
                 } else {
                     const int g = i < 0 ? ip : ip - nx;
                     const real* xg = a.xg + (size_t)g * nz + k;
@@ -571,7 +592,7 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<re
             U[3] = FM<real>::div(D[4], rcyl) - vb * D[0];
         }
         real cell[NF];
-        for (int f = 0; f < NF; ++f) cell[f] = a.in[f * N + c];
+        for (int f = 0; f < NF; ++f) cell[f] = DIR == 0 ? own[f] : a.in[f * N + c];
         update_cell(cell, U, a.gam);
         for (int f = 0; f < NF; ++f) a.out[f * N + c] = cell[f];
     }
@@ -1079,8 +1100,21 @@ hipError_t allow_lds(K kernel, size_t bytes) {
 }
 
 template <typename real>
-int launch_sweep(Sph* s, int dir, int in, int out, double dt) {
+PlanetS<real> planet_view(const Body& b);
+
+// kick (radial sweep only): the first half kick with the planet at t over
+// hkick = dt/2 applied to the cells as they are loaded
+template <typename real>
+int launch_sweep(Sph* s, int dir, int in, int out, double dt, const Body* kick = nullptr, double hkick = 0.0) {
     SphK<real> a = kargs<real>(s, in, out);
+    PlanetS<real> pl{};
+    if (kick) {
+        if (dir != 0) return report_error(PG_E_INVALID, "the fused kick belongs to the radial sweep");
+        a.vis = s->P.visc_flag ? (const real*)s->vis : nullptr;
+        pl = planet_view<real>(*kick);
+    }
+    const int planet = kick && s->P.n_planet > 0 ? 1 : 0;
+    const real hk = kick ? (real)hkick : real(0);
     const size_t lds = sweep_lds<real>(dir, s->nx, s->nz);
     unsigned nb;
     if (dir == 0) nb = (unsigned)(((s->nx + a.tx - 1) / a.tx) * s->ny * s->nz);
@@ -1093,10 +1127,12 @@ int launch_sweep(Sph* s, int dir, int in, int out, double dt) {
         const int bt = D == 0 ? sph_bx(s->nx) : SPH_BT2;                                               \
         if (pem) {                                                                                     \
             SPH_TRY(allow_lds(k3_sweep<real, D, 5>, lds));                                             \
-            hipLaunchKernelGGL((k3_sweep<real, D, 5>), dim3(nb), dim3(bt), lds, s->stream, a, (real)dt); \
+            hipLaunchKernelGGL((k3_sweep<real, D, 5>), dim3(nb), dim3(bt), lds, s->stream, a, (real)dt, pl, \
+                               planet, hk);                                                            \
         } else {                                                                                       \
             SPH_TRY(allow_lds(k3_sweep<real, D, 6>, lds));                                             \
-            hipLaunchKernelGGL((k3_sweep<real, D, 6>), dim3(nb), dim3(bt), lds, s->stream, a, (real)dt); \
+            hipLaunchKernelGGL((k3_sweep<real, D, 6>), dim3(nb), dim3(bt), lds, s->stream, a, (real)dt, pl, \
+                               planet, hk);                                                            \
         }                                                                                              \
     } while (0)
     if (dir == 0) SPH_SWEEP(0);
@@ -1203,8 +1239,8 @@ int ds_step(Sph* s, double t, double dt) {
     s->rate_valid = false;   // until the corrector has written the new state's rate
     const bool visc = s->P.visc_flag != 0;
     if (visc && (rc = launch_vis<real>(s, 0))) return rc;
-    if ((rc = launch_sources<real>(s, 0, 0, s->planet, 0.0, hdt))) return rc;
-    if ((rc = launch_sweep<real>(s, 0, 0, 1, dt))) return rc;
+    // first half kick fused into the radial sweep's load
+    if ((rc = launch_sweep<real>(s, 0, 0, 1, dt, &s->planet, hdt))) return rc;
     if ((rc = launch_sweep<real>(s, 1, 1, 0, dt))) return rc;
     if ((rc = launch_sweep<real>(s, 2, 0, 1, dt))) return rc;
     if ((rc = launch_orbadv<real>(s, 1, 0, dt, nullptr, nullptr))) return rc;
