@@ -84,12 +84,14 @@ struct SphK {
     const real* nu;
     // radial ghosts: [f][g][k], g = 0..2 (ip = 0..2), 3..5 (ip = nx+3..nx+5)
     const real* xg;
+    // per azimuthal row j: sin and cos of phi_j / 2 (phi_j the row centre)
+    const real* hphi;
     // per interior (i, k), FP64: orbital-advection speed and R_cyl
     const double* rot;
     const double* rcyl;
     real* rate;
     real yw[6];
-    real dy, idy, y0;
+    real dy, idy;
     double dy_d;
     real gam;
     int tx;   // radial segment of the radial sweep (sph_tx)
@@ -117,9 +119,6 @@ struct PlanetS {   // planet for the spherical force: mass, radius, angle, softe
 template <typename real> __device__ __forceinline__ real flog(real x);
 template <> __device__ __forceinline__ float flog<float>(float x) { return __logf(x); }
 template <> __device__ __forceinline__ double flog<double>(double x) { return (double)(float)log((double)(float)x); }
-
-__device__ __forceinline__ void sincos_r(float x, float* s, float* c) { sincosf(x, s, c); }
-__device__ __forceinline__ void sincos_r(double x, double* s, double* c) { sincos(x, s, c); }
 
 // Parameters of a cell's sub-cell profile: slopes, mean, and (PEM4) which
 // form each of get_CON_aveL / aveR takes (PEM.cu:51-93), decided once per
@@ -694,7 +693,7 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
 // itself), modes 1/2 predictor and corrector (forces re-evaluated at the
 // midpoint state, in -> out).
 template <typename real>
-__device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<real>& pl, bool planet, real rad, real phi,
+__device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<real>& pl, bool planet, real rad, int j,
                                            int kp, real& gx, real& gy, real& gz) {
     gx = -FM<real>::rcp(rad * rad);
     gy = real(0);
@@ -705,8 +704,8 @@ __device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<re
     // (r - a)^2 + 2 a r (2 sin^2(dphi/2) + cos(dphi) cos^2(theta) / (1 + sin(theta)))
     const real sp = a.sinz[kp];
     const real cp = sp * a.cotz[kp];
-    real sh, ch, s, c;
-    sincos_r(real(0.5) * phi, &sh, &ch);
+    real s, c;
+    const real sh = a.hphi[2 * j], ch = a.hphi[2 * j + 1];   // sin, cos of phi / 2
     const real hs = sh * pl.chy - ch * pl.shy;   // sin((phi - phi_p) / 2)
     const real hc = ch * pl.chy + sh * pl.shy;   // cos((phi - phi_p) / 2)
     s = real(2) * hs * hc;
@@ -752,9 +751,8 @@ __device__ __forceinline__ void source_cell(const SphK<real>& a, const PlanetS<r
                                             int j, real rho, real u, real vp, real w, real visr, real visp, real mdt,
                                             real dt, real& un, real& vn, real& wn) {
     const real rad = a.xc[ip];
-    const real phi = a.y0 + (real(j) + real(0.5)) * a.dy;
     real gx, gy, gz;
-    sph_forces(a, pl, planet, rad, phi, kp, gx, gy, gz);
+    sph_forces(a, pl, planet, rad, j, kp, gx, gy, gz);
     const real vb = a.vbg[(size_t)(kp - NPAD) * (a.nx + 2 * NPAD) + ip];
     const real vis0 = FM<real>::div(visr, rho), vis1 = FM<real>::div(visp, rho);
     const real ir = FM<real>::rcp(rad), cot = a.cotz[kp];
@@ -1028,7 +1026,7 @@ struct Sph {
 
 namespace {
 
-enum { TB_XA, TB_XV, TB_XC, TB_XW, TB_ZA, TB_ZV, TB_ZC, TB_ZW, TB_SZA, TB_SINZ, TB_COTZ, TB_VBG, TB_NU, TB_XG, TB_N };
+enum { TB_XA, TB_XV, TB_XC, TB_XW, TB_ZA, TB_ZV, TB_ZC, TB_ZW, TB_SZA, TB_SINZ, TB_COTZ, TB_VBG, TB_NU, TB_XG, TB_HPHI, TB_N };
 
 template <typename real>
 SphK<real> kargs(Sph* s, int in, int out) {
@@ -1044,6 +1042,7 @@ SphK<real> kargs(Sph* s, int in, int out) {
     a.zw = T + s->tab_off[TB_ZW]; a.sza = T + s->tab_off[TB_SZA]; a.sinz = T + s->tab_off[TB_SINZ];
     a.cotz = T + s->tab_off[TB_COTZ];
     a.vbg = T + s->tab_off[TB_VBG]; a.nu = T + s->tab_off[TB_NU]; a.xg = T + s->tab_off[TB_XG];
+    a.hphi = T + s->tab_off[TB_HPHI];
     a.rot = s->dtab;
     a.rcyl = s->dtab + (size_t)s->nx * s->nz;
     a.rate = (real*)s->rate;
@@ -1051,7 +1050,6 @@ SphK<real> kargs(Sph* s, int in, int out) {
     const double dy = (s->P.ymax - s->P.ymin) / (double)s->ny;
     a.dy = (real)dy;
     a.idy = (real)(1.0 / dy);
-    a.y0 = (real)s->P.ymin;
     a.dy_d = dy;
     a.gam = (real)s->P.gam;
     a.tx = sph_tx(s->nx);
@@ -1313,13 +1311,20 @@ int upload_tables(Sph* s) {
             for (int q = 0; q < NF; ++q) xg[((size_t)q * 6 + g) * nz + k] = f[q];
         }
     }
+    std::vector<double> hphi(2 * (size_t)ny);
+    const double dyh = (P.ymax - P.ymin) / ny;
+    for (int j = 0; j < ny; ++j) {
+        const double phi = P.ymin + (j + 0.5) * dyh;
+        hphi[2 * (size_t)j] = std::sin(0.5 * phi);
+        hphi[2 * (size_t)j + 1] = std::cos(0.5 * phi);
+    }
     // one real allocation for the tables
     std::vector<const std::vector<double>*> parts(TB_N);
     std::vector<double> xa = s->xa, xv = s->xv, za = s->za, zv = s->zv;
     parts[TB_XA] = &xa; parts[TB_XV] = &xv; parts[TB_XC] = &xc; parts[TB_XW] = &xw;
     parts[TB_ZA] = &za; parts[TB_ZV] = &zv; parts[TB_ZC] = &zc; parts[TB_ZW] = &zw;
     parts[TB_SZA] = &sza; parts[TB_SINZ] = &sinz; parts[TB_COTZ] = &cotz;
-    parts[TB_VBG] = &s->vbg; parts[TB_NU] = &nu; parts[TB_XG] = &xg;
+    parts[TB_VBG] = &s->vbg; parts[TB_NU] = &nu; parts[TB_XG] = &xg; parts[TB_HPHI] = &hphi;
     size_t tot = 0;
     for (int q = 0; q < TB_N; ++q) {
         s->tab_off[q] = tot;
