@@ -1566,9 +1566,13 @@ int adj_tape_forward(pg_handle* h, int nsteps, double t0, double dt0) {
         }
     }
     // the disk ends at the last step's result, with the last step's control
-    // block (t and dt of that step, as the device loop leaves them)
+    // block (t and dt of that step, as the device loop leaves them) and the
+    // result's viscous forces (as the device loop's corrector leaves them)
     HIPL(hipMemcpyAsync(d->gas[g.gpar], base + (size_t)nsteps * SLOT_WORDS * N + SL_S0 * N, 3 * N * sizeof(real),
                         hipMemcpyDeviceToDevice, s));
+    if (visc)
+        HIPL(hipMemcpyAsync(d->vis, base + (size_t)nsteps * SLOT_WORDS * N + SL_V0 * N, 2 * N * sizeof(real),
+                            hipMemcpyDeviceToDevice, s));
     HIPL(hipMemcpyAsync(d->ctl, w->sctl + nsteps - 1, sizeof(StepCtl), hipMemcpyDeviceToDevice, s));
     if ((rc = pull_ctl(g))) return rc;
     std::vector<StepCtl> cs(nsteps);
