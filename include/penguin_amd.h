@@ -210,7 +210,8 @@ int pg_batch_set_state(pg_batch* b, int disk, const double* gas, const double* d
 /* nsteps of every disk; t_io, dt_io: n entries each (time and previous step
    per disk, in and out); dts (n*nsteps, disk-major) may be NULL */
 int pg_batch_run(pg_batch* b, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph);
-/* launch schedule: x-sweep tile width (128, 256, 384, 512, 1024); threads
+/* launch schedule: x-sweep tile width (128, 256, 384, 512, 1024, or 122,
+   250, 506, whose tiles with their 6 halo cells fill 2, 4, 8 waves); threads
    per azimuthal tile (256: 16 radii x 16 rows, 512: 16 x 32, 1024: 16 x 64);
    azimuthal tile order (0: each XCD a contiguous band of tile rows of every
    disk, 1: row by row, 2: radius strip by strip); rows of the pointwise and
@@ -218,8 +219,9 @@ int pg_batch_run(pg_batch* b, int nsteps, double* t_io, double* dt_io, double* d
 int pg_batch_set_schedule(pg_batch* b, int tile_x, int ring_b, int order, int pt_rows);
 int pg_batch_get_schedule(pg_batch* b, int* tile_x, int* ring_b, int* order, int* pt_rows);
 /* time `steps` captured steps of each schedule candidate on the batch's
-   own state (restored afterwards) and keep the fastest; *best_ms (may be
-   NULL) receives its device time per step */
+   own state (restored afterwards), time the four fastest again over
+   4*steps and keep the fastest of those; *best_ms (may be NULL) receives
+   its device time per step */
 int pg_batch_autotune(pg_batch* b, int steps, double* best_ms);
 int pg_batch_profile_enable(pg_batch* b, int on);
 int pg_batch_profile_get(pg_batch* b, const char* kernel, double* avg_us, int* count);

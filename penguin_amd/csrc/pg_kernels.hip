@@ -99,6 +99,16 @@ __device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, 
     }
 }
 
+// Threads of an x-sweep block for a tile of TX cells: TX itself for the
+// tiles that are whole waves (the first NPAD*2 threads then take a second
+// slot in each phase), or the TX + 2 NPAD slots rounded up to whole waves
+// for the tiles sized to fill them (122 + 6 = two waves, 250 + 6 = four,
+// 506 + 6 = eight: one slot per thread, no phase run twice by one wave).
+template <int TX>
+struct XBlock {
+    static constexpr int BT = TX % 64 == 0 ? TX : (TX + 2 * NPAD + 63) / 64 * 64;
+};
+
 // ------------------------------------------------------------ x sweep, gas
 // One block = TX interior cells of one azimuthal row (the whole row when
 // nr <= 1024) plus 3 ghost/halo cells each side, staged in LDS; thread t
@@ -109,7 +119,7 @@ __device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, 
 // the viscous forces of the step start) to every interior cell as it is
 // loaded, in place of a separate source-term pass.
 template <typename real, int TX>
-__global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par, int kick) {
+__global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> ls, int par, int kick) {
     const int2 w = ls.wl_x[blockIdx.x];
     const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
@@ -139,7 +149,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par,
     const PG_GLOBAL real* __restrict__ in = gmem(d.gas[gsel(par)]);
     const size_t row = (size_t)j * nx;
 
-    for (int s = t; s < ns; s += TX) {
+    for (int s = t; s < ns; s += XBlock<TX>::BT) {
         const int i = x0 - NPAD + s;
         const int ip = i + NPAD;
         real r, u, vp;
@@ -165,17 +175,17 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par,
     __syncthreads();
     if (d.out_kick) {   // the kicked state of the tile's own cells
         PG_GLOBAL real* k1 = gmem(d.out_kick);
-        for (int s = t + NPAD; s < ns - NPAD; s += TX) {
+        for (int s = t + NPAD; s < ns - NPAD; s += XBlock<TX>::BT) {
             const size_t o = row + x0 - NPAD + s;
             k1[o] = s_r[s]; k1[N + o] = s_u[s]; k1[2 * N + o] = s_vp[s];
         }
     }
-    for (int s = t + 2; s < ns - 2; s += TX) {
+    for (int s = t + 2; s < ns - 2; s += XBlock<TX>::BT) {
         s_f[s] = flatten5(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2],
                           s_u[s - 1], s_u[s], s_u[s + 1], real(1));
     }
     __syncthreads();
-    for (int s = t + 2; s < ns - 2; s += TX) {
+    for (int s = t + 2; s < ns - 2; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
         const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
         const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
@@ -191,7 +201,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par,
         s_sl[3 * S + s] = l; s_sr[3 * S + s] = r;
     }
     __syncthreads();
-    for (int k = t + NPAD; k < ns - 2; k += TX) {
+    for (int k = t + NPAD; k < ns - 2; k += XBlock<TX>::BT) {
         const int ipk = x0 + k;
         real rl, pl, ul, vl, rr, pr, ur, vr, cl, cr;
         trace_left(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_p[k - 1], s_u[k - 1],
@@ -211,7 +221,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par,
     }
     __syncthreads();
     PG_GLOBAL real* __restrict__ out = gmem(d.gas[gsel(par) ^ 1]);
-    for (int s = t + NPAD; s < ns - NPAD; s += TX) {
+    for (int s = t + NPAD; s < ns - NPAD; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
         const int i = ip - NPAD;
         const real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
@@ -242,7 +252,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_gas(LaunchSet<real> ls, int par,
 // loaded; the gas state before its own kick and sweep is the buffer
 // opposite the current one (the gas x sweep has just run).
 template <typename real, int TX>
-__global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par, int kick) {
+__global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_dust(LaunchSet<real> ls, int par, int kick) {
     const int2 w = ls.wl_x[blockIdx.x];
     const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
@@ -269,7 +279,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
     const PG_GLOBAL real* __restrict__ in = gmem(d.dust[dsel(par)]);
     const size_t row = (size_t)j * nx;
 
-    for (int s = t; s < ns; s += TX) {
+    for (int s = t; s < ns; s += XBlock<TX>::BT) {
         const int i = x0 - NPAD + s;
         const int ip = i + NPAD;
         real r, u, vp;
@@ -293,7 +303,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
         s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
     }
     __syncthreads();
-    for (int s = t + 2; s < ns - 2; s += TX) {
+    for (int s = t + 2; s < ns - 2; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
         const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
         const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
@@ -306,7 +316,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
         s_sl[2 * S + s] = l; s_sr[2 * S + s] = r;
     }
     __syncthreads();
-    for (int k = t + NPAD; k < ns - 2; k += TX) {
+    for (int k = t + NPAD; k < ns - 2; k += XBlock<TX>::BT) {
         const int ipk = x0 + k;
         real Fr, Fu, FL;
         dust_iface(dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_sl[k - 1], s_sr[k - 1],
@@ -322,7 +332,7 @@ __global__ __launch_bounds__(TX) void k_sweep_x_dust(LaunchSet<real> ls, int par
     __syncthreads();
     PG_GLOBAL real* __restrict__ out = gmem(d.dust[dsel(par) ^ 1]);
     const real rfloor = d.D_G_ratio * real(1.0e-10);
-    for (int s = t + NPAD; s < ns - NPAD; s += TX) {
+    for (int s = t + NPAD; s < ns - NPAD; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
         const int i = ip - NPAD;
         const real f = dt * TAB(T_IXV, ip);
@@ -1128,12 +1138,12 @@ static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, bool dust, 
         size_t lds = 13 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_dust<real, TX>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par, kick);
+        hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
     } else {
         size_t lds = 18 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_gas<real, TX>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(TX), lds, s, ls, par, kick);
+        hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
     }
     return hipGetLastError();
 }
@@ -1141,6 +1151,9 @@ static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, bool dust, 
 template <typename real>
 static hipError_t sweep_x_any(const LaunchSet<real>& ls, int par, bool dust, int kick, hipStream_t s) {
     switch (ls.tile_x) {
+        case 122: return launch_sweep_x<real, 122>(ls, par, dust, kick, s);
+        case 250: return launch_sweep_x<real, 250>(ls, par, dust, kick, s);
+        case 506: return launch_sweep_x<real, 506>(ls, par, dust, kick, s);
         case 128: return launch_sweep_x<real, 128>(ls, par, dust, kick, s);
         case 384: return launch_sweep_x<real, 384>(ls, par, dust, kick, s);
         case 512: return launch_sweep_x<real, 512>(ls, par, dust, kick, s);

@@ -71,7 +71,8 @@ def test_batch_without_graph_equals_each_disk_alone():
     _assert_same_as_alone(_small_batch(n=4), 6, use_graph=False)
 
 
-@pytest.mark.parametrize("sched", [(128, 256, 0, 16), (512, 512, 1, 4), (256, 1024, 2, 8), (384, 512, 0, 16)])
+@pytest.mark.parametrize("sched", [(128, 256, 0, 16), (512, 512, 1, 4), (256, 1024, 2, 8), (384, 512, 0, 16),
+                                   (122, 1024, 2, 16), (250, 512, 0, 8), (506, 256, 1, 4)])
 def test_every_schedule_gives_the_same_bits(sched):
     ps = _small_batch()
     ref = Batch(ps)
@@ -83,6 +84,23 @@ def test_every_schedule_gives_the_same_bits(sched):
     for k in range(len(ps)):
         assert np.array_equal(b.get_state(k)[0], ref.get_state(k)[0]), (sched, k)
         assert b.t[k] == ref.t[k]
+
+
+@pytest.mark.parametrize("tile", [122, 250, 506])
+def test_wave_filling_x_tiles_gas_dust(tile):
+    # x tiles whose 6 halo cells fill whole waves (one LDS slot per thread),
+    # gas and dust sweeps, against the default schedule
+    ps = _dust_batch()
+    ref = Batch(ps)
+    ref.run(6)
+    b = Batch(ps)
+    tx, rb, order, rows = b.schedule()
+    b.set_schedule(tile, rb, order, rows)
+    b.run(6)
+    for k in range(len(ps)):
+        g0, d0 = ref.get_state(k)
+        g1, d1 = b.get_state(k)
+        assert np.array_equal(g1, g0) and np.array_equal(d1, d0), (tile, k)
 
 
 def test_autotune_restores_state_and_keeps_bits():
