@@ -345,7 +345,9 @@ def main():
                          f"(FP64 oracle, OpenMP over rows, {cdt:.1f} s)"}
 
     out = {
-        "metric": "cell-updates/s on 1 GPU",
+        # the headline metric as a whole: `value` is the cell-updates/s, the
+        # roofline object the %HBM, the batched and adjoint objects the rest
+        "metric": "cell-updates/s on 1 GPU + %HBM roofline; batched vs looped; adjoint/forward cost",
         "value": round(value, 1),
         "unit": "cell-updates/s",
         "n_gpus": 1,
