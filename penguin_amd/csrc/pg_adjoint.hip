@@ -993,7 +993,9 @@ __global__ __launch_bounds__(256) void k_adj_src(AdjArgs<real> A) {
 // ------------------------------------------------------------- launchers
 template <typename real>
 struct AdjLaunch {
-    static constexpr int TXA = 256;
+    // 244-cell tiles: with the 12 halo slots, 256 slots fill four waves (a
+    // 256-cell tile's 268 slots took a fifth, mostly idle, wave)
+    static constexpr int TXA = 244;
     static hipError_t x(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
         hipLaunchKernelGGL((k_adj_x<real, TXA>), dim3((nx + TXA - 1) / TXA, ny), dim3(AdjXBlock<TXA>::B), 0, s, A);
         return hipGetLastError();
