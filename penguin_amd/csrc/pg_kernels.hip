@@ -1062,20 +1062,43 @@ __global__ __launch_bounds__(1024) void k_rate_reduce(LaunchSet<real> ls) {
     if (threadIdx.x == 0) d.ctl->rate_bits = (unsigned long long)__double_as_longlong(m);
 }
 
-// Step prologue, one block.  mode 0: t and dt were set by the host.
-// mode 1 (device loop): finish the previous step (t += dt), take
+// Step prologue, one block of 1024 threads.  mode 0: t and dt were set by
+// the host.  mode 1 (device loop): finish the previous step (t += dt), take
 // dt = min(CFL/rate, 1.1 dt_prev) clipped to end_time, record it.  Then the
 // planet at t and t+dt and the orbital shift of every radius.
 // mode 2 (replay of a recorded step, one disk): t = td[0], dt = td[1].
+// The loads that do not depend on the new step size (the control block, the
+// planets, each thread's first radius of the shift) are issued before the
+// rate reduction, and the two planets (lanes 0 and 1 of wave 0, one
+// evaluation for both) need no barrier before the shifts, which do not read
+// them.
 template <typename real>
 __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode, int do_orbadv, const double* td) {
     const DiskArgs<real>& d = ls.disks[blockIdx.x];
     double* dts = d.dts;
     StepCtl* ctl = d.ctl;
     __shared__ double s_dt, s_t;
+    const int tid = threadIdx.x;
+    const bool oa = d.orbadv && do_orbadv;
+    double rt0 = 0.0, ry0 = 0.0;
+    if (oa && tid < d.nx) {
+        rt0 = d.rot[tid];
+        ry0 = d.rot[d.nx + tid];
+    }
+    double t0 = 0.0, dt0 = 0.0, tend = 0.0, cfl = 0.0;
+    int n0 = 0;
+    Body b{};
+    if (tid == 0) {
+        t0 = ctl->t;
+        dt0 = ctl->dt;
+        n0 = ctl->nstep;
+        tend = ctl->end_time;
+        cfl = d.CFL;
+    }
+    if (tid < 2) b = tid == 0 ? ctl->planet0 : ctl->planet1;
     double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
-    if (threadIdx.x == 0) {
-        double t = ctl->t, dt = ctl->dt;
+    if (tid == 0) {
+        double t = t0, dt = dt0;
         if (mode == 2) {
             t = td[0];
             dt = td[1];
@@ -1083,39 +1106,36 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
             ctl->dt = dt;
         }
         if (mode == 1) {
-            if (ctl->nstep > 0) t += dt;
+            if (n0 > 0) t += dt;
             ctl->rate_bits = (unsigned long long)__double_as_longlong(rate);
-            double dtn = d.CFL / rate;
+            double dtn = cfl / rate;
             double cap = dt * 1.1;
             dtn = dtn < cap ? dtn : cap;
-            if (t + dtn > ctl->end_time) dtn = ctl->end_time - t;
+            if (t + dtn > tend) dtn = tend - t;
             dt = dtn;
             ctl->t = t;
             ctl->dt = dt;
             ctl->dt_prev = dt;
-            if (dts && ctl->nstep < d.dts_cap) dts[ctl->nstep] = dt;
-            ctl->nstep += 1;
+            if (dts && n0 < d.dts_cap) dts[n0] = dt;
+            ctl->nstep = n0 + 1;
         }
         s_t = t;
         s_dt = dt;
     }
     __syncthreads();
     // the planet at t and at t + dt, on two lanes
-    if (threadIdx.x < 2) {
-        const int k = threadIdx.x;
-        Body b = k == 0 ? ctl->planet0 : ctl->planet1;
-        orbit_solution(b, k == 0 ? s_t : s_t + s_dt, d.ramp_time, d.planet_mass, (double)d.frame_omega);
+    if (tid < 2) {
+        orbit_solution(b, tid == 0 ? s_t : s_t + s_dt, d.ramp_time, d.planet_mass, (double)d.frame_omega);
         const PlanetK v = planet_kernel_view(b);
-        if (k == 0) { ctl->planet0 = b; ctl->pk0 = v; }
+        if (tid == 0) { ctl->planet0 = b; ctl->pk0 = v; }
         else { ctl->planet1 = b; ctl->pk1 = v; }
     }
-    __syncthreads();
-    if (d.orbadv && do_orbadv) {
+    if (oa) {
         const double dt = s_dt;
-        for (int i = threadIdx.x; i < d.nx; i += blockDim.x) {
+        for (int i = tid; i < d.nx; i += blockDim.x) {
             // set_orbital_advection (orbit.cu:74-99), FP64
-            double dy = d.dy_d * d.rot[d.nx + i];
-            double rt = d.rot[i];
+            double dy = d.dy_d * (i == tid ? ry0 : d.rot[d.nx + i]);
+            double rt = i == tid ? rt0 : d.rot[i];
             int shift = (int)floor(rt * dt / dy + 0.5);
             double speed = shift * dy / dt;
             d.inc[i] = shift;
