@@ -215,3 +215,24 @@ def test_adjoint_tape_refuses_the_time_average():
         d.adjoint_forward(3, 0)
     d.average_end()
     d.close()
+
+
+def test_adjoint_error_paths():
+    """Refused without touching the state: dust or wave killing, a negative
+    checkpoint interval, a backward sweep before any forward run."""
+    from penguin_amd import config_c2
+    from penguin_amd.disk import PenguinError
+    d = Disk(config_c2(precision=4, xres=64, yres=96))   # gas + dust, wave killing
+    with pytest.raises(PenguinError):
+        d.adjoint_forward(2, 1)
+    d.close()
+    p = _params(8)
+    d = Disk(p)
+    with pytest.raises(PenguinError):
+        d.adjoint_backward(np.zeros((5, d.ny, d.nx)))
+    with pytest.raises(PenguinError):
+        d.adjoint_forward(2, -1)
+    g0, _ = d.get_state()
+    dts = d.adjoint_forward(2, 0)   # still usable after the refusals
+    assert dts.shape == (2,) and np.all(dts > 0)
+    d.close()
