@@ -154,7 +154,7 @@ def test_autograd_wrapper():
     out = evolve(sig, q, al, p, 5, 2)
     loss = torch.sum(torch.from_numpy(w) * out)
     loss.backward()
-    assert abs(float(loss) - L) <= 1e-12 * abs(L)
+    assert abs(float(loss.detach()) - L) <= 1e-12 * abs(L)
     assert abs(float(q.grad) - dq) <= 1e-12 * abs(dq)
     assert abs(float(al.grad) - dal) <= 1e-12 * abs(dal)
     assert np.allclose(sig.grad.numpy(), d0[0], rtol=0, atol=1e-14 * np.abs(d0[0]).max())
