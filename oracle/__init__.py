@@ -76,6 +76,8 @@ def _bind(L):
         "orc_dust_flux": (None, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                  ctypes.c_int, ctypes.c_double, _dp]),
         "orc_orbit_solution": (None, [pp, ctypes.c_double, _dp]),
+        "orc_riemann_hllc_e": (None, [ctypes.c_double, ctypes.c_int, _dp, _dp]),
+        "orc_planet_force": (None, [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
         "orc_average_begin": (None, [vp, ctypes.c_double]),
         "orc_average_end": (None, [vp]),
         "orc_average_get": (None, [vp, _dp]),
@@ -325,6 +327,21 @@ def riemann_hllc(gam, S10, eos=0):
     out = np.zeros(10)
     lib().orc_riemann_hllc(gam, eos, ptr(arr(S10)), ptr(out))
     return out
+
+
+def riemann_hllc_e(gam, S10, eos=2):
+    """riemann_hllc plus the interface velocity uprs (HLLC.cu:58-68)."""
+    out = np.zeros(11)
+    lib().orc_riemann_hllc_e(gam, eos, ptr(arr(S10)), ptr(out))
+    return out
+
+
+def planet_force(body10, rad, azi, pol, sph=True):
+    """Planet acceleration (star_planet_grav, or star_planet_grav_cyl with
+    sph False and pol = z) at one point, dt = 0."""
+    f = np.zeros(3)
+    lib().orc_planet_force(int(sph), ptr(arr(body10)), rad, azi, pol, ptr(f))
+    return f
 
 
 def riemann_row(geom, gam, xa, r, p, u, v, w, dt):

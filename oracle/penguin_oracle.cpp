@@ -7,7 +7,7 @@
 // What it restates, and from where (reference = fakesuji/PEnGUIn):
 //   * grid and profiles        src/geom/geom.cu:3-33, src/init/init.cu:7-532
 //   * reconstruction           src/solver/recon/recon.cu:216-501, PPM.cu:7-113
-//   * traced states + Riemann  src/solver/riemann/states.cu:133-267,
+//   * traced states + Riemann  src/solver/riemann/states.cu:1-135,
 //                              wave_speeds.cu:24-35,288-333, HLLC.cu:2-77,
 //                              riemann.cu:5-131
 //   * sweeps + update          src/solver/sweeps.cu:218-456
@@ -366,7 +366,7 @@ real_t flatten_cell(int i, int n, real_t gam, const real_t* r, const real_t* p, 
 // ------------------------------------------------------------ Riemann step
 struct State { real_t rl, pl, ul, vl, wl, rr, pr, ur, vr, wr; };
 
-// set_state (states.cu:133-267) for interface i (between cells i-1 and i)
+// set_state (states.cu:1-135) for interface i (between cells i-1 and i)
 void trace_state(int i, int geom, int recon, real_t gam, const real_t* xa, const real_t* r, const real_t* p,
                  const real_t* u, const real_t* v, const real_t* w, const real_t* flat,
                  real_t dt, real_t us, State& S) {
@@ -1834,6 +1834,26 @@ void orc_orbit_solution(const pg_params* P, double t, double* b10) {
     Body b{0, P->planet_radius, P->planet_ecc, 0, 0, 0, 0, 0, 0, P->rs_fac * P->sc_h * P->planet_radius};
     orbit_solution(*P, b, t);
     std::memcpy(b10, &b, sizeof(Body));
+}
+void orc_riemann_hllc_e(double gam, int eos, const double* S10, double* out) {
+    State S;
+    std::memcpy(&S, S10, sizeof(State));
+    double pm, sl, sm, sr, pres, uprs;
+    wave_speeds(S, gam, 0.0, pm, sl, sm, sr);
+    Cell F;
+    hllc(S, eos, gam, pm, sl, sm, sr, F, pres, uprs);
+    double o[11] = {pm, sl, sm, sr, F.r, F.p, F.u, F.v, F.w, pres, uprs};
+    std::memcpy(out, o, sizeof(o));
+}
+void orc_planet_force(int sph, const double* b10, double rad, double azi, double pol, double* f3) {
+    Body b;
+    std::memcpy(&b, b10, sizeof(Body));
+    double fx, fy, fz;
+    if (sph) planet_force_sph(rad, azi, pol, b, fx, fy, fz);
+    else planet_force_cyl(rad, azi, pol, b, fx, fy, fz);
+    f3[0] = fx;
+    f3[1] = fy;
+    f3[2] = fz;
 }
 
 #endif  // ORC_TANGENT

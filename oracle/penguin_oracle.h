@@ -69,7 +69,7 @@ double orc_pem_aveL(double q, const double* par3);
 double orc_pem_aveR(double q, const double* par3);
 /* flattening of cell 2 of a 5-cell window (PPM.cu:7-62) */
 double orc_flatten(double gam, const double* r5, const double* p5, const double* u5);
-/* traced interface state at interface i of a row (states.cu:133-267) with the
+/* traced interface state at interface i of a row (states.cu:1-135) with the
    flattening coefficients supplied; S10 = rl,pl,ul,vl,wl,rr,pr,ur,vr,wr */
 void orc_set_state_row(int geom, double gam, int n, const double* xa, const double* dv,
                        const double* r, const double* p, const double* u,
@@ -88,6 +88,14 @@ void orc_dust_flux(int geom, int n, const double* xa, const double* dv,
                    const double* w, int i, double dt, double* F4);
 /* analytic Kepler orbit (planet.cu:105-133): body10 = m,a,e,x,y,z,vx,vy,vz,rs */
 void orc_orbit_solution(const pg_params* P, double t, double* body10);
+/* orc_riemann_hllc plus the interface velocity uprs of the internal-energy
+   equation (HLLC.cu:58-68): out11 = the ten values above, uprs */
+void orc_riemann_hllc_e(double gam, int eos, const double* S10, double* out11);
+/* planet acceleration at (rad, azi, pol) for body10 with dt = 0 (the star's
+   own gravity excluded): sph 1 star_planet_grav (forces.cu:14-39, spline
+   softening, spherical components r, phi, theta), sph 0
+   star_planet_grav_cyl (forces.cu:41-58, pol = z) */
+void orc_planet_force(int sph, const double* body10, double rad, double azi, double pol, double* f3);
 /* time average of the gas (init_average/averaging, output.cu:250-329; applied
    around each orc_run step as main.cu:344-350 does) */
 void orc_average_begin(orc_grid* g, double interval);

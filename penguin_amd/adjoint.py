@@ -56,12 +56,22 @@ class DiskEvolve(torch.autograd.Function):
         d.set_state(gas)
         d.adjoint_forward(nsteps, ckpt_every)
         out, _ = d.get_state()
-        ctx.disk = d
+        if any(ctx.needs_input_grad[:3]):
+            # the handle holds the checkpoints (or, with ckpt_every 0, every
+            # step's intermediates) until backward runs or the graph that
+            # owns ctx is freed, whichever comes first
+            ctx.disk = d
+        else:
+            d.close()
+            ctx.disk = None
         return torch.from_numpy(out[0].copy()).to(sigma0.dtype)
 
     @staticmethod
     def backward(ctx, g_sigma: torch.Tensor):
         d = ctx.disk
+        if d is None:
+            raise RuntimeError("DiskEvolve.backward: the forward pass kept no adjoint workspace "
+                               "(no input required a gradient), or backward already ran")
         dL = np.zeros((5, d.ny, d.nx))
         dL[0] = g_sigma.detach().cpu().double().numpy()
         dq, dal, d0 = d.adjoint_backward(dL)

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(AdjXBlock<TX>::B) void k_adj_x(AdjArgs<real> A) {
         if (g < 0 || g >= nx) {
             r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); vp = TAB(T_G_V, ip);
         } else {
-            r = A.s[row + g]; u = A.s[N + row + g]; vp = A.s[2 * N + row + g];
+            r = TAB(T_G_R, ip) + A.s[row + g]; u = A.s[N + row + g]; vp = A.s[2 * N + row + g];
         }
         r_[s] = r; p_[s] = TAB(T_CS2, ip) * r; u_[s] = u; vp_[s] = vp;
         L_[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
         real r = real(1), vp = real(0), u = real(0);
         if (live) {
             const size_t o = (size_t)wrap(jbase + e / TR, ny) * nx + i;
-            r = A.s[o]; vp = A.s[2 * N + o]; u = A.s[N + o];
+            r = TAB(T_G_R, ip) + A.s[o]; vp = A.s[2 * N + o]; u = A.s[N + o];
         }
         c0[e] = r; c1[e] = vp; c2[e] = u;
     }
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(AdjPhiTile<real>::TR * AdjPhiTile<real>::TY) void k
         real r = real(1), vp = real(0), u = real(0);
         if (live) {
             const size_t o = (size_t)wrap(jbase + e / TR, ny) * nx + i;
-            r = A.s[o]; vp = A.s[2 * N + o]; u = A.s[N + o];
+            r = TAB(T_G_R, ip) + A.s[o]; vp = A.s[2 * N + o]; u = A.s[N + o];
         }
         c0[e] = r; c1[e] = vp; c2[e] = u;
     }
@@ -848,7 +848,7 @@ __global__ __launch_bounds__(AV_X * TH) void k_adj_vis(AdjArgs<real> A) {
             r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); v = TAB(T_G_V, ip);
         } else {
             const size_t o = (size_t)j * nx + i;
-            r = vs[o]; u = vs[N + o]; v = vs[2 * N + o];
+            r = TAB(T_G_R, i + NPAD) + vs[o]; u = vs[N + o]; v = vs[2 * N + o];
         }
         sr[e] = r; su[e] = u; sv[e] = v;
     }
@@ -870,7 +870,7 @@ __global__ __launch_bounds__(AV_X * TH) void k_adj_vis(AdjArgs<real> A) {
             } else {
                 // adjoint of the forces from the source pass that read them
                 real a0, a1, a2;
-                src_adj_cell(d, MODE == 1 ? 2 : 0, ip, j, h, A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o],
+                src_adj_cell(d, MODE == 1 ? 2 : 0, ip, j, h, TAB(T_G_R, ip) + A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o],
                              A.lo[N + o], A.lo[2 * N + o], a0, a1, a2, X, Y, lq_unused);
                 if (MODE == 2) { X += A.lvis[o]; Y += A.lvis[N + o]; }
             }
@@ -941,15 +941,15 @@ __global__ __launch_bounds__(AV_X * TH) void k_adj_vis(AdjArgs<real> A) {
                 put(A.li, 2 * N + o, gv, A.accumulate & 1);
             } else if (MODE == 1) {
                 real cr, cu, cv, cX, cY, pr, pu, pv, pX, pY;
-                src_adj_cell(d, 2, ip, j, h, A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o], A.lo[N + o],
+                src_adj_cell(d, 2, ip, j, h, TAB(T_G_R, ip) + A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o], A.lo[N + o],
                              A.lo[2 * N + o], cr, cu, cv, cX, cY, lq);
-                src_adj_cell(d, 1, ip, j, h, A.s[o], A.s[2 * N + o], A.vis2[o], A.vis2[N + o], gr, gu, gv, pr, pu, pv,
+                src_adj_cell(d, 1, ip, j, h, TAB(T_G_R, ip) + A.s[o], A.s[2 * N + o], A.vis2[o], A.vis2[N + o], gr, gu, gv, pr, pu, pv,
                              pX, pY, lq);
                 A.li[o] = cr + pr; A.li[N + o] = cu + pu; A.li[2 * N + o] = cv + pv;
                 A.lvis[o] = pX; A.lvis[N + o] = pY;
             } else {
                 real kr, ku, kv, kX, kY;
-                src_adj_cell(d, 0, ip, j, h, A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o], A.lo[N + o],
+                src_adj_cell(d, 0, ip, j, h, TAB(T_G_R, ip) + A.s[o], A.s[2 * N + o], A.vis[o], A.vis[N + o], A.lo[o], A.lo[N + o],
                              A.lo[2 * N + o], kr, ku, kv, kX, kY, lq);
                 A.li[o] = kr + gr; A.li[N + o] = ku + gu; A.li[2 * N + o] = kv + gv;
             }
@@ -977,7 +977,7 @@ __global__ __launch_bounds__(256) void k_adj_src(AdjArgs<real> A) {
         real Vr = 0, Vp = 0;
         if (d.visc) { Vr = A.vis[o]; Vp = A.vis[N + o]; }
         real lrho, lu, lcvp, lVr, lVp;
-        src_adj_cell(d, A.mode, ip, j, h, A.s[o], A.s[2 * N + o], Vr, Vp, A.lo[o], A.lo[N + o], A.lo[2 * N + o], lrho,
+        src_adj_cell(d, A.mode, ip, j, h, TAB(T_G_R, ip) + A.s[o], A.s[2 * N + o], Vr, Vp, A.lo[o], A.lo[N + o], A.lo[2 * N + o], lrho,
                      lu, lcvp, lVr, lVp, lq);
         put(A.li, o, lrho, A.accumulate & 1);
         put(A.li, N + o, lu, A.accumulate & 1);

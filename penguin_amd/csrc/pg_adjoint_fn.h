@@ -149,7 +149,7 @@ PG_HD void flatten5_adj(real pm2, real pm1, real p0, real pp1, real pp2, real um
 // vr, cout.
 enum { TI_R, TI_P, TI_U, TI_RSL, TI_RSR, TI_PSL, TI_PSR, TI_USL, TI_USR, TI_VSL, TI_V, TI_VSR, TI_N };
 
-// trace_right (states.cu:151-206)
+// trace_right (states.cu:17-75)
 template <typename real>
 PG_HD void trace_right_adj(real c, real dt, real idx, const real* in, real lrr, real lpr, real lur, real lvr,
                            real lcout, real* lin) {
@@ -214,7 +214,7 @@ PG_HD void trace_right_adj(real c, real dt, real idx, const real* in, real lrr, 
     if (u < real(0)) lin[TI_U] -= lz1 * dt * idx;
 }
 
-// trace_left (states.cu:208-264)
+// trace_left (states.cu:76-132)
 template <typename real>
 PG_HD void trace_left_adj(real c, real dt, real idx, const real* in, real lrl, real lpl, real lul, real lvl,
                           real lcout, real* lin) {

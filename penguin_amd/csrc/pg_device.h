@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <limits>
 
 #include "pg_internal.h"
 
@@ -76,11 +77,13 @@ PG_HD real exp_lim(real x) {  // solve.cu:29-34
 // get_4th_parameters applies to the four neighbouring differences,
 // recon.cu:216-357), limiter (recon.cu:361-376), flattening and the PPM
 // monotonicity correction (PPM.cu:64-91).
+// slopes_d takes the four differences of neighbouring cell means around the
+// cell (d1 = a0 - am1 across its left edge, d2 = ap1 - a0 across its right
+// edge); slopes forms them from the five means.
 template <typename real>
-PG_HD void slopes(real am2, real am1, real a0, real ap1, real ap2,
-                                       real wl0, real wl1, real wl2, real wr0, real wr1, real wr2,
-                                       real flat, real& sL, real& sR) {
-    real d0 = am1 - am2, d1 = a0 - am1, d2 = ap1 - a0, d3 = ap2 - ap1;
+PG_HD void slopes_d(real d0, real d1, real d2, real d3,
+                    real wl0, real wl1, real wl2, real wr0, real wr1, real wr2,
+                    real flat, real& sL, real& sR) {
     real l = wl0 * d0 + wl1 * d1 + wl2 * d2;
     real r = wr0 * d1 + wr1 * d2 + wr2 * d3;
     l = copysign(fmin(fabs(l), fabs(d1)), d1);
@@ -100,6 +103,12 @@ PG_HD void slopes(real am2, real am1, real a0, real ap1, real ap2,
     sL = l;
     sR = r;
 }
+template <typename real>
+PG_HD void slopes(real am2, real am1, real a0, real ap1, real ap2,
+                                       real wl0, real wl1, real wl2, real wr0, real wr1, real wr2,
+                                       real flat, real& sL, real& sR) {
+    slopes_d(am1 - am2, a0 - am1, ap1 - a0, ap2 - ap1, wl0, wl1, wl2, wr0, wr1, wr2, flat, sL, sR);
+}
 
 // Mean of the cell parabola over its first fraction q (get_PPM_aveL,
 // PPM.cu:106-113) and over its last fraction x (get_PPM_aveR with x = 1-q,
@@ -118,13 +127,13 @@ PG_HD real aveR(real x, real sL, real a, real sR) {
 
 // Flattening coefficient of the centre cell of a 5-cell window
 // (flatten, PPM.cu:7-62).  cs is the cell's sound speed.
+// flatten_d takes the pressure differences dp1f = p[i+1] - p[i],
+// dp2f = p[i+2] - p[i-1], dp1b = p[i] - p[i-1], dp2b = p[i+1] - p[i-2]
+// and the cell's pressure p0.
 template <typename real>
-PG_HD real flatten5(real pm2, real pm1, real p0, real pp1, real pp2,
-                                         real um1, real u0, real up1, real cs) {
+PG_HD real flatten_d(real dp1f, real dp2f, real dp1b, real dp2b, real p0, real um1, real u0, real up1) {
     real f = real(0);
-    real dp1f = pp1 - p0, dp2f = pp2 - pm1, dp1b = p0 - pm1, dp2b = pp1 - pm2;
     // Mf > 0 || Mb > 0 with Mf = (u0-up1)/cs, Mb = (um1-u0)/cs and cs > 0
-    (void)cs;
     if (u0 - up1 > real(0) || um1 - u0 > real(0)) {
         if (dp2f != real(0) && dp2b != real(0)) {
             real ff = real(10) * fmax(real(0), fabs(FM<real>::div(dp1f, dp2f) - real(0.5)) - real(0.25));
@@ -135,9 +144,15 @@ PG_HD real flatten5(real pm2, real pm1, real p0, real pp1, real pp2,
     }
     return f;
 }
+template <typename real>
+PG_HD real flatten5(real pm2, real pm1, real p0, real pp1, real pp2,
+                                         real um1, real u0, real up1, real cs) {
+    (void)cs;
+    return flatten_d(pp1 - p0, pp2 - pm1, p0 - pm1, pp1 - pm2, p0, um1, u0, up1);
+}
 
 // Characteristic-traced half-step state on the right of an interface from
-// the cell right of it (set_state, states.cu:151-206), locally isothermal
+// the cell right of it (set_state, states.cu:17-75), locally isothermal
 // (EOS_flag 0 fixes gam = 1, parameters.h:118-119, so exp_lim(gam*x) =
 // exp_lim(x) and the sound speed of the traced state is sqrt(p0/r0)).
 // c: the cell's sound speed; idx = 1/dx.  Returns the state and its sound
@@ -169,7 +184,7 @@ PG_HD void trace_right(real c, real dt, real idx, real r, real p, real u,
     cout = cs;
 }
 
-// ... and on the left from the cell left of it (states.cu:208-264).
+// ... and on the left from the cell left of it (states.cu:76-132).
 template <typename real>
 PG_HD void trace_left(real c, real dt, real idx, real r, real p, real u,
                                            real rsL, real rsR, real psL, real psR, real usL, real usR,
@@ -305,6 +320,198 @@ PG_HD void hllc_c(real c, real ic, real c2, real rl, real ul, real vl, real rr, 
     real sl = ubar - cbar, sr = ubar + cbar;
     real sm = FM<real>::div(rl * (ul + c) + rr * (ur - c), rl + rr);
     hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, pres);
+}
+
+// ------------------------------------------------- deviation form (gas)
+// The gas density is held as its deviation from the time-independent radial
+// background rho_b(R) (the unperturbed initial disk) and the pressure as
+// p_b(R) + c_s^2(R) times that deviation, p_b = c_s^2 rho_b.  Differences of
+// neighbouring values (the slopes, the traced states' averages, the
+// interface pressures whose differences are the pressure force) are then
+// formed from deviations, whose perturbation part keeps the full precision
+// of the storage type, plus background differences tabulated in FP64.  In
+// real arithmetic each function equals its counterpart above with
+// value = background + deviation; only the rounding differs.
+//
+// exp_lim(x) - 1 without forming 1 + x (exp_lim, solve.cu:29-34)
+template <typename real>
+PG_HD real exp_lim_m1(real x) {
+    return fmax(x, real(1.0e-6) - real(1));
+}
+
+// The stored deviation of a cell set to the density floor rmin over the
+// background rb: rmin - rb, but never so close to -rb that rb + the
+// deviation rounds to zero in the storage precision (FP32 resolves a
+// deviation to ~6e-8 rb, so there a floor below ~5e-7 rb is held at
+// ~5e-7 rb; FP64 keeps the reference's floor).
+template <typename real>
+PG_HD real floor_dev(real rmin, real rb) {
+    return fmax(rmin - rb, -rb * (real(1) - real(4) * std::numeric_limits<real>::epsilon()));
+}
+
+// trace_right with the cell's background rb, pb and the deviations rd, pd
+// of its means; the slopes are of the full fields.  Returns the traced
+// density rr itself (positive: it is r0 >= the floor times exp_lim > 0) and
+// the traced state's deviations rrd, prd from the same background.
+template <typename real>
+PG_HD void trace_right_d(real c, real dt, real idx, real rb, real pb, real rd, real pd, real u,
+                         real rsL, real rsR, real psL, real psR, real usL, real usR,
+                         real vsL, real v, real vsR,
+                         real& rr, real& rrd, real& prd, real& ur, real& vr, real& cout) {
+    real q = lim01((c - fmin(u, real(0))) * dt * idx);
+    real ua = aveL(q, usL, u, usR);
+    real dis = fmin(ua, real(0)) * dt;
+    q = lim01(-dis * idx);
+    real r0d = aveL(q, rsL, rd, rsR), r0 = rb + r0d;
+    real p0d = aveL(q, psL, pd, psR), p0 = pb + p0d;
+    const real rmin = (rb + rd) * real(1.0e-10), pmin = (pb + pd) * real(1.0e-10);
+    if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
+    if (p0 < pmin) { p0 = pmin; p0d = pmin - pb; }
+    real u0 = aveL(q, usL, u, usR);
+    real ir0 = FM<real>::rcp(r0);
+    real w = p0 * ir0;
+    real ics = FM<real>::rsqrt(w);
+    real cs = w * ics;
+    q = lim01((cs * dt - dis) * idx);
+    real u_ = aveL(q, usL, u, usR);
+    real p_d = aveL(q, psL, pd, psR);
+    real xe = exp_lim_m1((u0 - u_) * ics);
+    rr = fma(r0, xe, r0);
+    rrd = fma(r0, xe, r0d);
+    prd = fma(p0, xe, p0d);
+    ur = u0 + (p0d - p_d) * ics * ir0;
+    vr = aveL(q, vsL, v, vsR);
+    cout = cs;
+}
+
+template <typename real>
+PG_HD void trace_left_d(real c, real dt, real idx, real rb, real pb, real rd, real pd, real u,
+                        real rsL, real rsR, real psL, real psR, real usL, real usR,
+                        real vsL, real v, real vsR,
+                        real& rl, real& rld, real& pld, real& ul, real& vl, real& cout) {
+    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
+    real ua = aveR(x, usL, u, usR);
+    real dis = fmax(ua, real(0)) * dt;
+    x = lim01(dis * idx);
+    real r0d = aveR(x, rsL, rd, rsR), r0 = rb + r0d;
+    real p0d = aveR(x, psL, pd, psR), p0 = pb + p0d;
+    const real rmin = (rb + rd) * real(1.0e-10), pmin = (pb + pd) * real(1.0e-10);
+    if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
+    if (p0 < pmin) { p0 = pmin; p0d = pmin - pb; }
+    real u0 = aveR(x, usL, u, usR);
+    real ir0 = FM<real>::rcp(r0);
+    real w = p0 * ir0;
+    real ics = FM<real>::rsqrt(w);
+    real cs = w * ics;
+    x = lim01((dis + cs * dt) * idx);
+    real u_ = aveR(x, usL, u, usR);
+    real p_d = aveR(x, psL, pd, psR);
+    real xe = exp_lim_m1((u_ - u0) * ics);
+    rl = fma(r0, xe, r0);
+    rld = fma(r0, xe, r0d);
+    pld = fma(p0, xe, p0d);
+    ul = u0 + (p_d - p0d) * ics * ir0;
+    vl = aveR(x, vsL, v, vsR);
+    cout = cs;
+}
+
+// hllc_iso for the traced states of the cells left (pressure background pbl)
+// and right of an interface: densities rl, rr, pressures as deviations pld,
+// prd from their cells' backgrounds.  tbp is the right background pressure
+// minus pbl.  The interface pressure is returned as its deviation presd
+// from the interface reference pbl + tbp/2 (the mean of the two
+// backgrounds).
+template <typename real>
+PG_HD void hllc_iso_d(real cl, real cr, real pbl, real tbp, real rl, real pld, real ul, real vl,
+                      real rr, real prd, real ur, real vr,
+                      real& Fr, real& us, real& vs, real& presd) {
+    const real h = real(0.5) * tbp;
+    const real pref = pbl + h;
+    const real pl = pld - h, pr = prd + h;    // deviations from pref
+    real csum = cl + cr;
+    real ics = FM<real>::rcp(csum);
+    const real xm = exp_lim_m1((ul - ur) * ics);
+    const real pm = fma(real(0.5) * (pr + pl), real(1) + xm, pref * xm);
+    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
+    real iss = FM<real>::rcp(srl + srr);
+    real ubar = (srl * ul + srr * ur) * iss;
+    real n2 = real(0.5) * srl * srr * iss * iss;
+    real du = ur - ul;
+    real cbar = FM<real>::sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
+    real sl = ubar - cbar, sr = ubar + cbar;
+    const real den = rl * cl + rr * cr, rs = rl + rr;
+    const real X = FM<real>::rcp(den * rs);
+    real sm = (rl * ul * cl + rr * ur * cr) * rs * X + real(2) * (pl - pr) * ics * den * X;
+    // the star pressure (s pm - sm p)/(s - sm) is affine in (pm, p) with
+    // weights summing to one, so it carries the common reference unchanged
+    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, presd);
+}
+
+// trace_right_c / trace_left_c with the ring's background density rb and
+// the deviation rd of the cell's mean; returns the traced density (positive)
+// and its deviation.
+template <typename real>
+PG_HD void trace_right_cd(real c, real ic, real dt, real idx, real rb, real rd, real u,
+                          real rsL, real rsR, real usL, real usR, real vsL, real v, real vsR,
+                          real& rr, real& rrd, real& ur, real& vr) {
+    real q = lim01((c - fmin(u, real(0))) * dt * idx);
+    real ua = aveL(q, usL, u, usR);
+    real dis = fmin(ua, real(0)) * dt;
+    q = lim01(-dis * idx);
+    real r0d = aveL(q, rsL, rd, rsR), r0 = rb + r0d;
+    const real rmin = (rb + rd) * real(1.0e-10);
+    if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
+    real u0 = aveL(q, usL, u, usR);
+    q = lim01((c * dt - dis) * idx);
+    real u_ = aveL(q, usL, u, usR);
+    real rad = aveL(q, rsL, rd, rsR);
+    const real xe = exp_lim_m1((u0 - u_) * ic);
+    rr = fma(r0, xe, r0);
+    rrd = fma(r0, xe, r0d);
+    ur = u0 + c * FM<real>::div(r0d - rad, r0);
+    vr = aveL(q, vsL, v, vsR);
+}
+
+template <typename real>
+PG_HD void trace_left_cd(real c, real ic, real dt, real idx, real rb, real rd, real u,
+                         real rsL, real rsR, real usL, real usR, real vsL, real v, real vsR,
+                         real& rl, real& rld, real& ul, real& vl) {
+    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
+    real ua = aveR(x, usL, u, usR);
+    real dis = fmax(ua, real(0)) * dt;
+    x = lim01(dis * idx);
+    real r0d = aveR(x, rsL, rd, rsR), r0 = rb + r0d;
+    const real rmin = (rb + rd) * real(1.0e-10);
+    if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
+    real u0 = aveR(x, usL, u, usR);
+    x = lim01((dis + c * dt) * idx);
+    real u_ = aveR(x, usL, u, usR);
+    real rad = aveR(x, rsL, rd, rsR);
+    const real xe = exp_lim_m1((u_ - u0) * ic);
+    rl = fma(r0, xe, r0);
+    rld = fma(r0, xe, r0d);
+    ul = u0 + c * FM<real>::div(rad - r0d, r0);
+    vl = aveR(x, vsL, v, vsR);
+}
+
+// hllc_c with the ring's background rb and the traced densities rl, rr
+// and their deviations; the interface pressure is returned as its
+// deviation from c^2 rb.
+template <typename real>
+PG_HD void hllc_cd(real c, real ic, real c2, real rb, real rl, real rld, real ul, real vl, real rr, real rrd,
+                   real ur, real vr, real& Fr, real& us, real& vs, real& presd) {
+    const real pl = c2 * rld, pr = c2 * rrd;
+    const real xm = exp_lim_m1(real(0.5) * (ul - ur) * ic);
+    const real pm = c2 * fma(real(0.5) * (rld + rrd), real(1) + xm, rb * xm);
+    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
+    real iss = FM<real>::rcp(srl + srr);
+    real ubar = (srl * ul + srr * ur) * iss;
+    real n2 = real(0.5) * srl * srr * iss * iss;
+    real du = ur - ul;
+    real cbar = FM<real>::sqrt(c2 + n2 * du * du);
+    real sl = ubar - cbar, sr = ubar + cbar;
+    real sm = FM<real>::div(rl * (ul + c) + rr * (ur - c), rl + rr);
+    hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, presd);
 }
 
 // Pressureless dust flux through an interface (dust_flux, dust.cu:6-105)

@@ -242,6 +242,7 @@ struct DiskDev {
     size_t bytes = 0;
     std::vector<double> tabh;  // per-radius tables, field-major
     std::vector<double> phih;  // 4*ny azimuthal sine/cosine table
+    std::vector<double> rho_bg;  // nx: the background density the stored gas density deviates from
     int tab_stride = 0;
     double dy = 0.0;
     double wly[3]{}, wry[3]{};
@@ -327,6 +328,9 @@ struct Group {
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     int graph_next[4] = {0, 0, 0, 0};
     size_t bytes = 0;
+    // pinned host staging for results read back after a run (the step sizes)
+    double* stage = nullptr;
+    size_t stage_bytes = 0;
 
     int par() const { return gpar | (dpar << 1); }
     int ndisk() const { return (int)disks.size(); }
@@ -475,6 +479,14 @@ void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<do
         T(T_TS, ip) = P.Stokes / (T(T_C0, ip) / P.sc_h);
         T(T_ITS, ip) = 1.0 / T(T_TS, ip);
     }
+    // the background the stored density deviates from: its differences
+    // across each edge and the x sweep's pressure force on it, in FP64
+    for (int ip = 1; ip < np; ++ip) {
+        T(T_TBR, ip) = T(T_G_R, ip) - T(T_G_R, ip - 1);
+        T(T_TBP, ip) = T(T_CS2, ip) * T(T_G_R, ip) - T(T_CS2, ip - 1) * T(T_G_R, ip - 1);
+    }
+    for (int ip = 1; ip < np - 1; ++ip)
+        T(T_PSRC, ip) = -0.25 * (T(T_XA, ip) + T(T_XA, ip + 1)) * (T(T_TBP, ip) + T(T_TBP, ip + 1));
     for (int ip = 1; ip < np - 1; ++ip) {
         double xm = T(T_XC, ip - 1), x0 = T(T_XC, ip), xp = T(T_XC, ip + 1);
         T(T_I20, ip) = 1.0 / (xp - xm);
@@ -519,13 +531,16 @@ void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<do
             T(T_KILL, ip) = k;
         }
     }
-    // orbital advection background rotation (orbit.cu:51-72) + R, FP64
-    rot.assign(2 * (size_t)nx, 0.0);
+    // orbital advection background rotation (orbit.cu:51-72), R and the
+    // background density, FP64
+    rot.assign(3 * (size_t)nx, 0.0);
     for (int i = 0; i < nx; ++i) {
         double xc = T(T_XC, i + NPAD);
         rot[i] = P.OrbAdv_flag ? pr.v(xc) - xc * P.frame_omega : 0.0;
         rot[nx + i] = xc;
+        rot[2 * nx + i] = T(T_G_R, i + NPAD);
     }
+    h->rho_bg.assign(rot.begin() + 2 * nx, rot.end());
     // initial state (fill_grid, init.cu:502-519) + seeded perturbation
     const size_t N = (size_t)nx * ny;
     gas.assign(3 * N, 0.0);
@@ -535,11 +550,12 @@ void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<do
             size_t o = (size_t)j * nx + i;
             int ip = i + NPAD;
             double xc = T(T_XC, ip);
-            double rho = T(T_G_R, ip), u = T(T_G_U, ip);
-            if (P.noise_rho != 0.0) rho *= 1.0 + P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, j) - 1.0);
+            // stored: rho - rho_b, here the noise's share of rho_b (1 + eps)
+            double drho = 0.0, u = T(T_G_U, ip);
+            if (P.noise_rho != 0.0) drho = T(T_G_R, ip) * (P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, j) - 1.0));
             if (P.noise_vr != 0.0)
                 u += P.noise_vr * std::sqrt(P.gam * pr.cs2(xc)) * (2.0 * keyed_uniform(P.seed, 1, i, j) - 1.0);
-            gas[o] = rho;
+            gas[o] = drho;
             gas[N + o] = u;
             gas[2 * N + o] = 0.0;
             if (P.dust_flag) {
@@ -783,6 +799,9 @@ void group_release(Group& g) {
     g.d_wl = nullptr;
     for (hipEvent_t e : g.ev_pool) (void)hipEventDestroy(e);
     g.ev_pool.clear();
+    if (g.stage) (void)hipHostFree(g.stage);
+    g.stage = nullptr;
+    g.stage_bytes = 0;
     if (g.ev_a) (void)hipEventDestroy(g.ev_a);
     if (g.ev_b) (void)hipEventDestroy(g.ev_b);
     if (g.stream) (void)hipStreamDestroy(g.stream);
@@ -993,6 +1012,22 @@ int cfl_rate(Group& g) {
     return PG_OK;
 }
 
+// Pinned staging of at least `bytes` (grown on demand, after the stream has
+// drained, so no copy in flight still targets the old block).
+int group_stage(Group& g, size_t bytes) {
+    if (g.stage_bytes >= bytes) return PG_OK;
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (g.stage) (void)hipHostFree(g.stage);
+    g.stage = nullptr;
+    g.stage_bytes = 0;
+    if (hipHostMalloc((void**)&g.stage, bytes, hipHostMallocDefault) != hipSuccess) {
+        g.stage = nullptr;
+        return fail(PG_E_NOMEM, "hipHostMalloc(%zu) failed", bytes);
+    }
+    g.stage_bytes = bytes;
+    return PG_OK;
+}
+
 // nsteps of the reference loop (main.cu:320-359) for every disk of the
 // group, each with its own dt.  t_io/dt_io hold one entry per disk; dts
 // (ndisk*nsteps, disk-major, may be null) receives the step sizes;
@@ -1023,6 +1058,8 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
         c.mode = 1;
         c.end_time = g.disks[k]->P.end_time;
         c.rate_bits = 0ull;
+        c.err = 0;
+        c.err_step = 0;
     }
     if ((rc = push_ctl(g))) return rc;
     // rate of the current state for the first step (later steps get it from
@@ -1073,11 +1110,29 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
     if ((rc = pull_ctl(g))) return rc;
     if (loop_ms) HIP_TRY(hipEventElapsedTime(loop_ms, g.ev_a, g.ev_b));
     if ((rc = timers_resolve(g))) return rc;
+    if (dts) {
+        // the step sizes, staged through pinned memory on the group's stream
+        const size_t n = (size_t)g.ndisk() * nsteps;
+        if ((rc = group_stage(g, n * sizeof(double)))) return rc;
+        for (int k = 0; k < g.ndisk(); ++k)
+            HIP_TRY(hipMemcpyAsync(g.stage + (size_t)k * nsteps, g.disks[k]->dts, sizeof(double) * nsteps,
+                                   hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        std::memcpy(dts, g.stage, n * sizeof(double));
+    }
+    int bad = -1;
     for (int k = 0; k < g.ndisk(); ++k) {
         const DiskDev* d = g.disks[k];
-        if (dts) HIP_TRY(hipMemcpy(dts + (size_t)k * nsteps, d->dts, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
         t_io[k] = d->ctl_host->t + d->ctl_host->dt;
         dt_io[k] = d->ctl_host->dt;
+        if (d->ctl_host->err && bad < 0) bad = k;
+    }
+    if (bad >= 0) {
+        const StepCtl& c = *g.disks[bad]->ctl_host;
+        double rate = 0.0;
+        std::memcpy(&rate, &c.rate_bits, sizeof(double));
+        return fail(PG_E_STATE, "disk %d: the CFL rate before step %d is %g (the state is not finite); "
+                    "the loop stopped there at t = %.17g", bad, c.err_step, rate, c.t);
     }
     return PG_OK;
 }
@@ -1093,7 +1148,7 @@ int disk_get_state(Group& g, int k, double* gas, double* dust) {
         if ((rc = download(h->prec, g.stream, h->gas[g.gpar], gv, 3 * N))) return rc;
         for (size_t o = 0; o < N; ++o) {
             int ip = (int)(o % nx) + NPAD;
-            double rho = gv[o];
+            double rho = h->rho_bg[o % nx] + gv[o];
             gas[o] = rho;
             gas[N + o] = tabv(h, T_CS2, ip) * rho;
             gas[2 * N + o] = gv[N + o];
@@ -1126,7 +1181,7 @@ int disk_set_state(Group& g, int k, const double* gas, const double* dust) {
         std::vector<double> v(3 * N);
         for (size_t o = 0; o < N; ++o) {
             int ip = (int)(o % nx) + NPAD;
-            v[o] = gas[o];
+            v[o] = gas[o] - h->rho_bg[o % nx];
             v[N + o] = gas[2 * N + o];
             v[2 * N + o] = gas[3 * N + o] - tabv(h, T_VBG, ip);
         }
@@ -2164,8 +2219,8 @@ int average_get(pg_handle* h, double* gas) {
     const double W = d->ctl_host->ave_w;
     for (size_t o = 0; o < N; ++o) {
         const int ip = (int)(o % nx) + NPAD;
-        gas[o] = a[o];
-        gas[N + o] = tabv(d, T_CS2, ip) * a[o];
+        gas[o] = d->rho_bg[o % nx] * W + a[o];
+        gas[N + o] = tabv(d, T_CS2, ip) * gas[o];
         gas[2 * N + o] = a[N + o];
         gas[3 * N + o] = tabv(d, T_VBG, ip) * W + a[2 * N + o];
         gas[4 * N + o] = 0.0;

@@ -4,8 +4,10 @@
 // Device data layout (one disk):
 //   state     SoA, interior cells only, radius fastest: field f of cell (i,j)
 //             at base[f*N + i + nx*j], N = nx*ny.
-//             gas  : rho, u (radial velocity), vp (azimuthal velocity minus
-//                    the background rotation vbg(R) = initial v_phi(R))
+//             gas  : rho - rho_b(R) (the density's deviation from the
+//                    unperturbed initial disk rho_b), u (radial velocity),
+//                    vp (azimuthal velocity minus the background rotation
+//                    vbg(R) = initial v_phi(R))
 //             dust : rho_d, u_d, vp_d (same background subtracted)
 //             Pressure is not stored: under the locally isothermal EOS the
 //             reference resets p = c_s^2(R) rho after every update
@@ -77,6 +79,8 @@ struct StepCtl {
     PlanetK pk0, pk1;    // the same, as the source kernel reads them
     double ave_interval; // time-average window (the averaging weight is dt/2 over it)
     double ave_w;        // sum of the averaging weights so far
+    int err;             // device loop: 1 once a step found a CFL rate that is not finite and positive
+    int err_step;        // the step at which it did
 };
 
 // Per-radius tables (FP64 on the host; cast to the storage type on upload).
@@ -105,6 +109,10 @@ enum TabField {
     T_IC0,        // 1/c_s
     T_TS,         // St * h0 / c_s: dust stopping time times rho (dust.cu:1-4)
     T_ITS,        // 1/T_TS
+    T_TBR,        // rho_b[ip] - rho_b[ip-1]: background density difference across edge ip
+    T_TBP,        // p_b[ip] - p_b[ip-1], p_b = c_s^2 rho_b
+    T_PSRC,       // background pressure force of the x sweep without dt:
+                  // -(xa[ip] + xa[ip+1]) (T_TBP[ip] + T_TBP[ip+1]) / 4
     T_NFIELDS
 };
 
@@ -126,8 +134,9 @@ struct DiskArgs {
     real gam, sc_h, Stokes, D_G_ratio, frame_omega;
     double CFL;
     double dy_d;         // dy in FP64 for the orbital shift
-    const double* rot;   // 2*nx FP64: [0,nx) vbg(R) - R*frame_omega
-                         // (orbit.cu:65), [nx,2nx) R of the interior radii
+    const double* rot;   // 3*nx FP64: [0,nx) vbg(R) - R*frame_omega
+                         // (orbit.cu:65), [nx,2nx) R of the interior radii,
+                         // [2nx,3nx) the background density rho_b(R)
     int* inc;            // nx, integer azimuthal shift of the step
     real* res;           // nx, residual advection speed of the step
     StepCtl* ctl;

@@ -37,11 +37,13 @@
 namespace pg {
 
 // ------------------------------------------------------- source terms
-// 1 - exp(-x) (the drag factor dtau, sweeps.cu:673,702)
+// 1 - exp(-x) (the drag factor dtau, sweeps.cu:673,702), as -expm1(-x):
+// 1 - exp(-x) formed by a subtraction loses the leading digits when x is
+// small (a long stopping time or a short step)
 template <typename real>
 __device__ __forceinline__ real one_minus_exp_neg(real x);
 template <>
-__device__ __forceinline__ float one_minus_exp_neg(float x) { return 1.0f - __expf(-x); }
+__device__ __forceinline__ float one_minus_exp_neg(float x) { return -expm1f(-x); }
 template <>
 __device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x); }
 
@@ -113,7 +115,14 @@ struct XBlock {
 // One block = TX interior cells of one azimuthal row (the whole row when
 // nr <= 1024) plus 3 ghost/halo cells each side, staged in LDS; thread t
 // owns slot t (interior cell t-3 of the tile).  Phases (barrier between):
-// load -> flattening -> parabolas -> interface fluxes -> conservative update.
+// load -> flattening and parabolas -> interface fluxes -> conservative
+// update (a cell's flattening coefficient is used by its own parabolas
+// only, PPM.cu:64-78, so the two share a phase).
+// Density and pressure are held as deviations from the radial background
+// (rho_b, p_b = c_s^2 rho_b); their differences across an edge are the
+// deviations' difference plus the tabulated background difference, and
+// the interface pressures are deviations from the mean of the backgrounds
+// on either side, whose own difference is the tabulated T_PSRC term.
 // TX is a compile-time constant so every LDS array offset is an immediate.
 // kick != 0: apply the first half kick (cell_sources mode 0: planet at t,
 // the viscous forces of the step start) to every interior cell as it is
@@ -125,18 +134,17 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> 
     constexpr int S = TX + 2 * NPAD;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     real* sm = reinterpret_cast<real*>(smem_raw);
-    real* s_r = sm;
-    real* s_p = sm + S;
+    real* s_r = sm;             // rho - rho_b
+    real* s_p = sm + S;         // p - p_b = c_s^2 (rho - rho_b)
     real* s_u = sm + 2 * S;
     real* s_L = sm + 3 * S;
     real* s_vp = sm + 4 * S;
-    real* s_f = sm + 5 * S;
-    real* s_sl = sm + 6 * S;    // 4 fields: r, p, u, L
-    real* s_sr = sm + 10 * S;
-    real* s_Fr = sm + 14 * S;
-    real* s_Fu = sm + 15 * S;
-    real* s_FL = sm + 16 * S;
-    real* s_pr = sm + 17 * S;
+    real* s_sl = sm + 5 * S;    // 4 fields: r, p, u, L
+    real* s_sr = sm + 9 * S;
+    real* s_Fr = sm + 13 * S;
+    real* s_Fu = sm + 14 * S;
+    real* s_FL = sm + 15 * S;
+    real* s_pr = sm + 16 * S;   // interface pressure - its background reference
     const int nx = d.nx;
     const size_t N = (size_t)nx * d.ny;
     const int t = threadIdx.x;
@@ -152,22 +160,22 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> 
     for (int s = t; s < ns; s += XBlock<TX>::BT) {
         const int i = x0 - NPAD + s;
         const int ip = i + NPAD;
-        real r, u, vp;
+        real rd, u, vp;
         if (i < 0 || i >= nx) {
-            r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); vp = TAB(T_G_V, ip);
+            rd = real(0); u = TAB(T_G_U, ip); vp = TAB(T_G_V, ip);   // the background itself
         } else {
-            r = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
+            rd = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
             if (kick) {
                 real Vr = real(0), Vp = real(0), nu, nvp, unused0, unused1;
                 if (d.visc) { Vr = gmem(d.vis)[row + i]; Vp = gmem(d.vis)[N + row + i]; }
-                cell_sources<real, true, false>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), r, u, vp, Vr, Vp, real(0),
-                                                real(0), nu, nvp, unused0, unused1);
+                cell_sources<real, true, false>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), TAB(T_G_R, ip) + rd, u, vp,
+                                                Vr, Vp, real(0), real(0), nu, nvp, unused0, unused1);
                 u = nu;
                 vp = nvp;
             }
         }
-        s_r[s] = r;
-        s_p[s] = TAB(T_CS2, ip) * r;
+        s_r[s] = rd;
+        s_p[s] = TAB(T_CS2, ip) * rd;
         s_u[s] = u;
         s_vp[s] = vp;
         s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
@@ -181,19 +189,22 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> 
         }
     }
     for (int s = t + 2; s < ns - 2; s += XBlock<TX>::BT) {
-        s_f[s] = flatten5(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2],
-                          s_u[s - 1], s_u[s], s_u[s + 1], real(1));
-    }
-    __syncthreads();
-    for (int s = t + 2; s < ns - 2; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
         const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
         const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
-        const real fl = s_f[s];
+        // differences across the edges ip-1 .. ip+2 (edge e between cells e-1, e)
+        const real p0 = s_p[s - 2], p1 = s_p[s - 1], p2 = s_p[s], p3 = s_p[s + 1], p4 = s_p[s + 2];
+        const real dp0 = TAB(T_TBP, ip - 1) + (p1 - p0), dp1 = TAB(T_TBP, ip) + (p2 - p1);
+        const real dp2 = TAB(T_TBP, ip + 1) + (p3 - p2), dp3 = TAB(T_TBP, ip + 2) + (p4 - p3);
+        const real r0 = s_r[s - 2], r1 = s_r[s - 1], r2 = s_r[s], r3 = s_r[s + 1], r4 = s_r[s + 2];
+        const real dr0 = TAB(T_TBR, ip - 1) + (r1 - r0), dr1 = TAB(T_TBR, ip) + (r2 - r1);
+        const real dr2 = TAB(T_TBR, ip + 1) + (r3 - r2), dr3 = TAB(T_TBR, ip + 2) + (r4 - r3);
+        const real pc = TAB(T_CS2, ip) * TAB(T_G_R, ip) + p2;
+        const real fl = flatten_d(dp2, dp1 + dp2 + dp3, dp1, dp0 + dp1 + dp2, pc, s_u[s - 1], s_u[s], s_u[s + 1]);
         real l, r;
-        slopes(s_r[s - 2], s_r[s - 1], s_r[s], s_r[s + 1], s_r[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+        slopes_d(dr0, dr1, dr2, dr3, wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
         s_sl[s] = l; s_sr[s] = r;
-        slopes(s_p[s - 2], s_p[s - 1], s_p[s], s_p[s + 1], s_p[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
+        slopes_d(dp0, dp1, dp2, dp3, wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
         s_sl[S + s] = l; s_sr[S + s] = r;
         slopes(s_u[s - 2], s_u[s - 1], s_u[s], s_u[s + 1], s_u[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
         s_sl[2 * S + s] = l; s_sr[2 * S + s] = r;
@@ -203,21 +214,23 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> 
     __syncthreads();
     for (int k = t + NPAD; k < ns - 2; k += XBlock<TX>::BT) {
         const int ipk = x0 + k;
-        real rl, pl, ul, vl, rr, pr, ur, vr, cl, cr;
-        trace_left(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_p[k - 1], s_u[k - 1],
-                   s_sl[k - 1], s_sr[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
-                   s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
-                   s_sl[3 * S + k - 1], s_L[k - 1], s_sr[3 * S + k - 1], rl, pl, ul, vl, cl);
-        trace_right(TAB(T_C0, ipk), dt, TAB(T_IDX, ipk), s_r[k], s_p[k], s_u[k],
-                    s_sl[k], s_sr[k], s_sl[S + k], s_sr[S + k], s_sl[2 * S + k], s_sr[2 * S + k],
-                    s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, pr, ur, vr, cr);
-        real Fr, Fu, FL, pres;
-        hllc_iso(cl, cr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, Fu, FL, pres);
+        const real rbl = TAB(T_G_R, ipk - 1), rbr = TAB(T_G_R, ipk);
+        const real pbl = TAB(T_CS2, ipk - 1) * rbl, pbr = TAB(T_CS2, ipk) * rbr;
+        real rl, rld, pld, ul, vl, rr, rrd, prd, ur, vr, cl, cr;
+        trace_left_d(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), rbl, pbl, s_r[k - 1], s_p[k - 1], s_u[k - 1],
+                     s_sl[k - 1], s_sr[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
+                     s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
+                     s_sl[3 * S + k - 1], s_L[k - 1], s_sr[3 * S + k - 1], rl, rld, pld, ul, vl, cl);
+        trace_right_d(TAB(T_C0, ipk), dt, TAB(T_IDX, ipk), rbr, pbr, s_r[k], s_p[k], s_u[k],
+                      s_sl[k], s_sr[k], s_sl[S + k], s_sr[S + k], s_sl[2 * S + k], s_sr[2 * S + k],
+                      s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, rrd, prd, ur, vr, cr);
+        real Fr, Fu, FL, presd;
+        hllc_iso_d(cl, cr, pbl, TAB(T_TBP, ipk), rl, pld, ul, vl, rr, prd, ur, vr, Fr, Fu, FL, presd);
         const real g = TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
         s_Fr[k] = Fr * g;
         s_Fu[k] = Fu * g;
         s_FL[k] = FL * g;
-        s_pr[k] = pres;
+        s_pr[k] = presd;
     }
     __syncthreads();
     PG_GLOBAL real* __restrict__ out = gmem(d.gas[gsel(par) ^ 1]);
@@ -227,20 +240,25 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> 
         const real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
         const real f = dt * TAB(T_IXV, ip);
         const real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
-        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1])) * f;
+        // net_source (riemann.cu:59-62): the deviations' part plus the
+        // background's, T_PSRC = (g0 + g1)/2 (pref[ip] - pref[ip+1])
+        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1]) + TAB(T_PSRC, ip)) * f;
         const real DL = (s_FL[s] - s_FL[s + 1]) * f;
-        const real r = s_r[s], u = s_u[s], vp = s_vp[s];
-        real rn = r + Dr;
+        const real rb = TAB(T_G_R, ip);
+        const real rd = s_r[s], u = s_u[s], vp = s_vp[s];
+        const real r = rb + rd;
+        const real rn = r + Dr;
+        real rdn = rd + Dr;
         const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
         real un = u, vn = vp;
-        if (rn < rmin) {
-            rn = rmin;
+        if (rn < rmin) {   // update_Cell's floor (sweeps.cu:238-246): old velocities kept
+            rdn = floor_dev(rmin, rb);
         } else {
             const real irn = FM<real>::rcp(rn);
             un = (r * u + Du) * irn;
             vn = (r * vp + (DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr)) * irn;
         }
-        out[row + i] = rn;
+        out[row + i] = rdn;
         out[N + row + i] = un;
         out[2 * N + row + i] = vn;
     }
@@ -289,7 +307,7 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_dust(LaunchSet<real>
             r = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
             if (kick) {
                 const PG_GLOBAL real* __restrict__ g0 = gmem(d.gas[gsel(par) ^ 1]);
-                const real rho = g0[row + i], cu = g0[N + row + i], cvp = g0[2 * N + row + i];
+                const real rho = TAB(T_G_R, ip) + g0[row + i], cu = g0[N + row + i], cvp = g0[2 * N + row + i];
                 real ndu, ndvp, unused0, unused1;
                 cell_sources<real, false, true>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), rho, cu, cvp, real(0), real(0),
                                                 u, vp, unused0, unused1, ndu, ndvp);
@@ -379,12 +397,17 @@ template <> struct PhiTile<1024> { static constexpr int TR = 16, TY = 64; };
 // rows [3, TY+9); stage B advects rows [6, TY+6) and writes them to rows
 // y0..y0+TY-1, the same rows for every radius, so the stores are contiguous.
 // flags: bit 0 = azimuthal sweep, bit 1 = orbital shift + residual advection.
+// The gas density is held as its deviation from the ring's background
+// rho_b(R), constant along the strip: differences and slopes are those of
+// the deviations, the interface pressures deviations from c_s^2 rho_b, and
+// the residual advection's mass flux the deviation's (the background's
+// uniform flux cancels in every difference).
 template <typename real, bool DUST, int B>
 __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flags) {
     constexpr int TR = PhiTile<B>::TR, TY = PhiTile<B>::TY;
     constexpr int NK = TY + 4 * NPAD;          // LDS rows per radius
     constexpr int E = NK * TR;                 // LDS cells per array
-    __shared__ real c0[E], c1[E], c2[E], fl[E];
+    __shared__ real c0[E], c1[E], c2[E];
     __shared__ real sl0[E], sr0[E], sl1[E], sr1[E], sl2[E], sr2[E];
     __shared__ real F0[E], F1[E], F2[E], F3[E];
     const int2 w = ls.wl_ring[blockIdx.x];
@@ -415,6 +438,7 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
     PG_GLOBAL real* __restrict__ out = gmem(DUST ? d.dust[dsel(par) ^ 1] : d.gas[gsel(par) ^ 1]);
     const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
     const real rfloor = d.D_G_ratio * real(1.0e-10);
+    const real rbg = DUST ? real(0) : TAB(T_G_R, ip);   // what the stored density is relative to
     // B is a multiple of TR, so lane t keeps radius ri in every loop below
     static_assert(B % TR == 0, "tile");
 
@@ -424,7 +448,7 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
     constexpr int KSTEP = B / TR;              // rows advanced per loop trip
     int jj = jb;
     for (int e = t; e < E; e += B) {
-        real r = real(1), vp = real(0), u = real(0);
+        real r = DUST ? real(1) : real(0), vp = real(0), u = real(0);
         if (live) {
             const size_t o = (size_t)jj * nx + i;
             r = in[o];
@@ -437,20 +461,15 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
     }
     __syncthreads();
     // ---- stage A: azimuthal sweep of LDS rows [3, TY+9) ----
-    if (!DUST) {
-        const real cs0 = TAB(T_C0, ip);
-        for (int e = t + 2 * TR; e < (NK - 2) * TR; e += B) {
-            // p = c_s^2 rho with c_s^2 fixed along the strip: flatten's
-            // pressure ratios are the density ratios
-            fl[e] = flatten5(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR],
-                             c1[e - TR], c1[e], c1[e + TR], cs0);
-        }
-        __syncthreads();
-    }
     for (int e = t + 2 * TR; e < (NK - 2) * TR; e += B) {
-        const real f = DUST ? real(0) : fl[e];
+        const real a0 = c0[e - 2 * TR], a1 = c0[e - TR], a2 = c0[e], a3 = c0[e + TR], a4 = c0[e + 2 * TR];
+        const real d0 = a1 - a0, d1 = a2 - a1, d2 = a3 - a2, d3 = a4 - a3;
+        // p = c_s^2 rho with c_s^2 fixed along the strip: flatten's
+        // pressure ratios are the density ratios (the dust is not flattened)
+        const real f = DUST ? real(0)
+                            : flatten_d(d2, d1 + d2 + d3, d1, d0 + d1 + d2, rbg + a2, c1[e - TR], c1[e], c1[e + TR]);
         real l, r;
-        slopes(c0[e - 2 * TR], c0[e - TR], c0[e], c0[e + TR], c0[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], f, l, r);
+        slopes_d(d0, d1, d2, d3, wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], f, l, r);
         sl0[e] = l; sr0[e] = r;
         slopes(c1[e - 2 * TR], c1[e - TR], c1[e], c1[e + TR], c1[e + 2 * TR], wy[0], wy[1], wy[2], wy[3], wy[4], wy[5], f, l, r);
         sl1[e] = l; sr1[e] = r;
@@ -467,15 +486,15 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
             F0[e] = Fr; F1[e] = Fu; F2[e] = Fw;
         } else {
             const real cs2 = TAB(T_CS2, ip), cs0 = TAB(T_C0, ip), ic0 = TAB(T_IC0, ip);
-            real rl, ul, vl, rr, ur, vr;
+            real rl, rld, ul, vl, rr, rrd, ur, vr;
             // the pressure parabola is c_s^2 times the density parabola
-            trace_left_c(cs0, ic0, dts, idy, c0[m], c1[m], sl0[m], sr0[m], sl1[m], sr1[m], sl2[m], c2[m], sr2[m],
-                         rl, ul, vl);
-            trace_right_c(cs0, ic0, dts, idy, c0[e], c1[e], sl0[e], sr0[e], sl1[e], sr1[e], sl2[e], c2[e], sr2[e],
-                          rr, ur, vr);
-            real Fr, Fu, Fw, pres;
-            hllc_c(cs0, ic0, cs2, rl, ul, vl, rr, ur, vr, Fr, Fu, Fw, pres);
-            F0[e] = Fr; F1[e] = Fu; F2[e] = Fw; F3[e] = pres;
+            trace_left_cd(cs0, ic0, dts, idy, rbg, c0[m], c1[m], sl0[m], sr0[m], sl1[m], sr1[m], sl2[m], c2[m],
+                          sr2[m], rl, rld, ul, vl);
+            trace_right_cd(cs0, ic0, dts, idy, rbg, c0[e], c1[e], sl0[e], sr0[e], sl1[e], sr1[e], sl2[e], c2[e],
+                           sr2[e], rr, rrd, ur, vr);
+            real Fr, Fu, Fw, presd;
+            hllc_cd(cs0, ic0, cs2, rbg, rl, rld, ul, vl, rr, rrd, ur, vr, Fr, Fu, Fw, presd);
+            F0[e] = Fr; F1[e] = Fu; F2[e] = Fw; F3[e] = presd;
         }
     }
     __syncthreads();
@@ -484,21 +503,22 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
         const real Du = DUST ? (F1[e] - F1[e + TR]) * fup
                              : (F1[e] - F1[e + TR] + (F3[e] - F3[e + TR])) * fup;
         const real Dw = (F2[e] - F2[e + TR]) * fup;
-        const real r = c0[e], vp = c1[e], u = c2[e];
-        real rn = r + Dr;
+        const real rd = c0[e], vp = c1[e], u = c2[e];
+        const real r = rbg + rd, rn = r + Dr;   // the density itself
+        real rdn = rd + Dr;                     // and what is stored
         const real rmin = fmax(r * real(1.0e-10), DUST ? rfloor : real(1.0e-10));
         real vn = vp, un = u;
         if (!sweep_on) {
-            rn = r;
+            rdn = rd;
         } else if (rn < rmin) {
-            rn = rmin;
+            rdn = floor_dev(rmin, rbg);
         } else {
             // gas: (rho v + Du + (rot + R Omega_f) Dr) / rho' - vbg == (rho vp + Du) / rho'
             const real irn = FM<real>::rcp(rn);
             vn = (r * vp + Du) * irn;
             un = (r * u + Dw) * irn;
         }
-        c0[e] = rn; c1[e] = vn; c2[e] = un;   // each thread rewrites only its own cells
+        c0[e] = rdn; c1[e] = vn; c2[e] = un;   // each thread rewrites only its own cells
     }
     __syncthreads();
     const int rows = min(TY, ny - y0);
@@ -534,16 +554,18 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
     }
     __syncthreads();
     for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY + 1) * TR; e += B) {
-        real Dr, Dv, Du;
+        real Dr, Dv, Du;   // Dr: upwind mean of the stored density (gas: its deviation)
         if (speed > real(0)) {
             const int m = e - TR;
             Dr = aveR(q, sl0[m], c0[m], sr0[m]);
-            Dv = aveR(q, sl1[m], c1[m], sr1[m]) * Dr;
-            Du = aveR(q, sl2[m], c2[m], sr2[m]) * Dr;
+            const real ra = rbg + Dr;
+            Dv = aveR(q, sl1[m], c1[m], sr1[m]) * ra;
+            Du = aveR(q, sl2[m], c2[m], sr2[m]) * ra;
         } else {
             Dr = aveL(q, sl0[e], c0[e], sr0[e]);
-            Dv = aveL(q, sl1[e], c1[e], sr1[e]) * Dr;
-            Du = aveL(q, sl2[e], c2[e], sr2[e]) * Dr;
+            const real ra = rbg + Dr;
+            Dv = aveL(q, sl1[e], c1[e], sr1[e]) * ra;
+            Du = aveL(q, sl2[e], c2[e], sr2[e]) * ra;
         }
         F0[e] = Dr * speed; F1[e] = Dv * speed; F2[e] = Du * speed;
     }
@@ -554,15 +576,15 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
         const real Dr = (F0[e] - F0[e + TR]) * fup;
         const real Dv = (F1[e] - F1[e + TR]) * fup;
         const real Du = (F2[e] - F2[e + TR]) * fup;
-        const real r = c0[e];
-        real rn = r + Dr;
+        const real rd = c0[e], r = rbg + rd;
+        real rn = rd + Dr;   // stored form
         real vn = c1[e], un = c2[e];
         // update_Dust floors (advection.cu:315); update_Cell_advect has none
         // (advection.cu:88-105)
         if (DUST && rn < fmax(r * real(1.0e-10), rfloor)) {
             rn = fmax(r * real(1.0e-10), rfloor);
         } else {
-            const real irn = FM<real>::rcp(rn);
+            const real irn = FM<real>::rcp(r + Dr);
             vn = (r * c1[e] + Dv) * irn;
             un = (r * c2[e] + Du) * irn;
         }
@@ -658,7 +680,7 @@ __global__ __launch_bounds__(256) void k_viscosity(LaunchSet<real> ls, int par, 
                     rr = TAB(T_G_R, ii + NPAD); uu = TAB(T_G_U, ii + NPAD); vv = TAB(T_G_V, ii + NPAD);
                 } else {
                     size_t o = (size_t)rows[b] * nx + ii;
-                    rr = st[o]; uu = st[N + o]; vv = st[2 * N + o];
+                    rr = TAB(T_G_R, ii + NPAD) + st[o]; uu = st[N + o]; vv = st[2 * N + o];
                 }
                 nr[n] = rr * nu[a];
                 u[n] = uu;
@@ -717,6 +739,20 @@ __device__ __forceinline__ double reduce_partials(const double* part, int n) {
     return m;
 }
 
+// CFL rate of one cell (get_dt_lv1, timestep.cu:18-97): the largest signal
+// speed over the cell width in each direction, the dust's included.  A
+// velocity that is not finite gives +inf, so that a broken state is not lost
+// in the maxima (fmax drops NaN) and the step that reads the rate stops.
+template <typename real>
+__device__ __forceinline__ real cell_rate(const DiskArgs<real>& d, int ip, real u, real vp, real du, real dvp) {
+    const real cs = TAB(T_C0, ip);
+    real sx = fabs(u) + cs, sy = fabs(vp) + cs;
+    if (d.has_dust) { sx = fmax(fabs(du), sx); sy = fmax(fabs(dvp), sy); }
+    const real r = fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip));
+    const real all = fabs(u) + fabs(vp) + (d.has_dust ? fabs(du) + fabs(dvp) : real(0));
+    return all < real(INFINITY) ? r : real(INFINITY);
+}
+
 // Pointwise source terms.  mode: 0 first half kick (planet at t, in place),
 // 1 predictor (planet at t+dt, gas written to gasT), 2 corrector (planet at
 // t+dt, in place) optionally followed by wave killing and the CFL rate,
@@ -742,7 +778,8 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
         const real h = (real)(0.5 * d.ctl->dt);
-        const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
+        const real rd = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
+        const real rho = TAB(T_G_R, ip) + rd;
         real Vr = real(0), Vp = real(0), Qu = real(0), Qvp = real(0);
         if (d.visc) { Vr = gmem(d.vis)[o]; Vp = gmem(d.vis)[N + o]; }
         if (d.has_dust) { Qu = dust[N + o]; Qvp = dust[2 * N + o]; }
@@ -753,25 +790,20 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
             else
                 cell_sources<real, true, false>(d, mode, ip, j, h, rho, cu, cvp, Vr, Vp, Qu, Qvp, nu_, nvp, ndu, ndvp);
         }
-        real nr_ = rho;
+        real nr_ = rd;   // stored form: the deviation from rho_b
         if (mode == 1) {
             gasT[o] = nr_; gasT[N + o] = nu_; gasT[2 * N + o] = nvp;
         } else {
-            if (do_kill) {   // killwave on the corrected state (solve.cu:287-289)
+            if (do_kill) {   // killwave on the corrected state (solve.cu:287-289), toward rho_b
                 const real kk = TAB(T_KILL, ip) * (real)d.ctl->dt;
-                nr_ += kk * (TAB(T_G_R, ip) - nr_);
+                nr_ += kk * (real(0) - nr_);
                 nu_ += kk * (TAB(T_G_U, ip) - nu_);
                 nvp += kk * (real(0) - nvp);
             }
             if (do_kill) gas[o] = nr_;   // rho changes only by wave killing
             gas[N + o] = nu_; gas[2 * N + o] = nvp;
             if (d.has_dust) { dust[N + o] = ndu; dust[2 * N + o] = ndvp; }
-            if (do_rate) {
-                const real cs = TAB(T_C0, ip);
-                real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
-                if (d.has_dust) { sx = fmax(fabs(ndu), sx); sy = fmax(fabs(ndvp), sy); }
-                rate = fmax(rate, fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip)));
-            }
+            if (do_rate) rate = fmax(rate, cell_rate(d, ip, nu_, nvp, ndu, ndvp));
         }
     }
     if (do_rate) block_max_store(d.rate_part + w.y, rate);
@@ -847,7 +879,7 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
             r = TAB(T_G_R, ip); u = TAB(T_G_U, ip); v = TAB(T_G_V, ip);
         } else {
             const size_t o = (size_t)j * nx + i;
-            const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
+            const real rho = TAB(T_G_R, i + NPAD) + gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
             real nu_, nvp, unused0, unused1;
             cell_sources<real, true, false>(d, 1, i + NPAD, j, h, rho, cu, cvp, vis[o], vis[N + o], real(0), real(0),
                                             nu_, nvp, unused0, unused1);
@@ -856,7 +888,7 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
         Tr[e] = r; Tu[e] = u; Tv[e] = v;
     }
     __syncthreads();
-    if (d.out_pred) {   // the predicted state of the tile's own cells
+    if (d.out_pred) {   // the predicted state of the tile's own cells (its density is the input's)
         PG_GLOBAL real* tp = gmem(d.out_pred);
         const int lx = (int)(threadIdx.x % PT_X) + 1, i = i0 + lx - 1;
         for (int ly = (int)(threadIdx.x / PT_X) + 1; ly <= th; ly += PT_TY) {
@@ -864,7 +896,7 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
             if (i >= nx || j >= ny) break;
             const size_t o = (size_t)j * nx + i;
             const int e = ly * ST_X + lx;
-            tp[o] = Tr[e]; tp[N + o] = Tu[e]; tp[2 * N + o] = Tv[e];
+            tp[o] = gas[o]; tp[N + o] = Tu[e]; tp[2 * N + o] = Tv[e];
         }
     }
     vis_from_tile<real, TH>(d, Tr, Tu, Tv, i0, j0, gmem(d.vis2));
@@ -907,7 +939,9 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
         } else {
             const int ip = i + NPAD;
             const size_t o = (size_t)j * nx + i;
-            const real rho = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
+            const real rb = TAB(T_G_R, ip);
+            const real rd = gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
+            const real rho = rb + rd;
             real Qu = real(0), Qvp = real(0);
             if (d.has_dust) { Qu = dust[N + o]; Qvp = dust[2 * N + o]; }
             real nu_ = cu, nvp = cvp, ndu = Qu, ndvp = Qvp;
@@ -917,25 +951,22 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
             else
                 cell_sources<real, true, false>(d, 2, ip, j, h, rho, cu, cvp, vis2[o], vis2[N + o], Qu, Qvp, nu_, nvp,
                                                 ndu, ndvp);
-            real nr_ = rho;
+            real nrd = rd;   // the deviation from rho_b, which wave killing relaxes to 0
             if (do_kill) {
                 const real kk = TAB(T_KILL, ip) * (real)d.ctl->dt;
-                nr_ += kk * (TAB(T_G_R, ip) - nr_);
+                nrd += kk * (real(0) - nrd);
                 nu_ += kk * (TAB(T_G_U, ip) - nu_);
                 nvp += kk * (real(0) - nvp);
             }
-            r = nr_; u = nu_; v = nvp;
+            r = rb + nrd; u = nu_; v = nvp;
             // the tile's own cells are written back, with their CFL rate
             const bool own = lx >= 1 && lx <= PT_X && ly >= 1 && ly <= th && jr < ny;
             if (own) {
-                gasN[o] = nr_;
+                gasN[o] = nrd;
                 gasN[N + o] = nu_;
                 gasN[2 * N + o] = nvp;
                 if (d.has_dust) { dust[N + o] = ndu; dust[2 * N + o] = ndvp; }
-                const real cs = TAB(T_C0, ip);
-                real sx = fabs(nu_) + cs, sy = fabs(nvp) + cs;
-                if (d.has_dust) { sx = fmax(fabs(ndu), sx); sy = fmax(fabs(ndvp), sy); }
-                rate = fmax(rate, fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip)));
+                rate = fmax(rate, cell_rate(d, ip, nu_, nvp, ndu, ndvp));
             }
         }
         Sr[e] = r; Su[e] = u; Sv[e] = v;
@@ -965,10 +996,8 @@ __global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
         if (i >= nx || j >= ny) break;
         const int ip = i + NPAD;
         const size_t o = (size_t)j * nx + i;
-        const real cs = TAB(T_C0, ip);
-        real sx = fabs(gas[N + o]) + cs, sy = fabs(gas[2 * N + o]) + cs;
-        if (d.has_dust) { sx = fmax(fabs(dust[N + o]), sx); sy = fmax(fabs(dust[2 * N + o]), sy); }
-        rate = fmax(rate, fmax(sx * TAB(T_IDX, ip), sy * d.idy * TAB(T_IR, ip)));
+        const real du = d.has_dust ? dust[N + o] : real(0), dvp = d.has_dust ? dust[2 * N + o] : real(0);
+        rate = fmax(rate, cell_rate(d, ip, gas[N + o], gas[2 * N + o], du, dvp));
     }
     block_max_store(d.rate_part + w.y, rate);
 }
@@ -977,8 +1006,9 @@ __global__ __launch_bounds__(256) void k_cfl(LaunchSet<real> ls, int par) {
 // Time average of the gas: add_grid (output.cu:250-273) with the weight
 // dt/2 / interval that averaging (output.cu:299-314) applies before and
 // after each step (main.cu:344-350), accumulated in FP64.  The average holds
-// rho, u and vp; the sum of the weights (for p = cs2 rho and v = vbg + vp)
-// accumulates in the control block, added once per disk by its tile 0.
+// the stored fields rho - rho_b, u and vp; the sum of the weights (for
+// rho = rho_b + the deviation, p = cs2 rho and v = vbg + vp) accumulates in
+// the control block, added once per disk by its tile 0.
 template <typename real, int TH>
 __global__ __launch_bounds__(256) void k_average(LaunchSet<real> ls, int par) {
     const int2 w = ls.wl_pt[blockIdx.x];
@@ -1037,6 +1067,7 @@ __global__ __launch_bounds__(256) void k_torque(LaunchSet<real> ls, int par, dou
     double acc = 0.0;
     if (i < nx) {
         const double R = d.rot[nx + i];
+        const double rb = d.rot[2 * nx + i];   // FP64 background density
         const double dv = (double)TAB(T_XV, i + NPAD) * d.dy_d;
 #pragma unroll
         for (int k = 0; k < TH / PT_TY; ++k) {
@@ -1048,7 +1079,7 @@ __global__ __launch_bounds__(256) void k_torque(LaunchSet<real> ls, int par, dou
             const double Rp = sqrt(R * R + pk.x * pk.x - 2.0 * pk.x * R * cs + pk.rs * pk.rs);
             const double kk = 1.0 / (Rp * Rp * Rp);
             const double gy = -pk.m * pk.x * sn * kk + sn * pk.m_over_x2;
-            acc += R * gy * ((double)rho[(size_t)j * nx + i] * dv);
+            acc += R * gy * ((rb + (double)rho[(size_t)j * nx + i]) * dv);
         }
     }
     block_sum_store(part + w.y, acc);
@@ -1066,6 +1097,12 @@ __global__ __launch_bounds__(1024) void k_rate_reduce(LaunchSet<real> ls) {
 // the host.  mode 1 (device loop): finish the previous step (t += dt), take
 // dt = min(CFL/rate, 1.1 dt_prev) clipped to end_time, record it.  Then the
 // planet at t and t+dt and the orbital shift of every radius.
+// The device loop stops where the reference loop does (main.cu:323-327):
+// once t reaches end_time dt is 0, and every pass of a step of size 0
+// leaves the state as it is (no shift, no flux, no kick).  A CFL rate that
+// is not finite and positive (a state with NaN or infinite velocities)
+// sets ctl->err with the step's index and also makes dt 0, so the loop
+// freezes there and the host reports it.
 // mode 2 (replay of a recorded step, one disk): t = td[0], dt = td[1].
 // The loads that do not depend on the new step size (the control block, the
 // planets, each thread's first radius of the shift) are issued before the
@@ -1086,7 +1123,7 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
         ry0 = d.rot[d.nx + tid];
     }
     double t0 = 0.0, dt0 = 0.0, tend = 0.0, cfl = 0.0;
-    int n0 = 0;
+    int n0 = 0, err0 = 0;
     Body b{};
     if (tid == 0) {
         t0 = ctl->t;
@@ -1094,6 +1131,7 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
         n0 = ctl->nstep;
         tend = ctl->end_time;
         cfl = d.CFL;
+        err0 = ctl->err;
     }
     if (tid < 2) b = tid == 0 ? ctl->planet0 : ctl->planet1;
     double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
@@ -1111,7 +1149,14 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
             double dtn = cfl / rate;
             double cap = dt * 1.1;
             dtn = dtn < cap ? dtn : cap;
-            if (t + dtn > tend) dtn = tend - t;
+            if (t + dtn > tend) dtn = tend > t ? tend - t : 0.0;
+            if (err0 || !(rate > 0.0 && rate < INFINITY)) {
+                if (!err0) {
+                    ctl->err = 1;
+                    ctl->err_step = n0;
+                }
+                dtn = 0.0;
+            }
             dt = dtn;
             ctl->t = t;
             ctl->dt = dt;
@@ -1136,10 +1181,14 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
             // set_orbital_advection (orbit.cu:74-99), FP64
             double dy = d.dy_d * (i == tid ? ry0 : d.rot[d.nx + i]);
             double rt = i == tid ? rt0 : d.rot[i];
-            int shift = (int)floor(rt * dt / dy + 0.5);
-            double speed = shift * dy / dt;
+            int shift = 0;
+            double res = 0.0;   // a step of size 0 moves nothing
+            if (dt > 0.0) {
+                shift = (int)floor(rt * dt / dy + 0.5);
+                res = rt - shift * dy / dt;
+            }
             d.inc[i] = shift;
-            d.res[i] = (real)(rt - speed);
+            d.res[i] = (real)res;
         }
     }
 }
@@ -1160,7 +1209,7 @@ static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, bool dust, 
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
     } else {
-        size_t lds = 18 * S * sizeof(real);
+        size_t lds = 17 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_gas<real, TX>, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);

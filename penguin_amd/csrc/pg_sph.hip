@@ -1533,10 +1533,18 @@ int sph_solve(Sph* s, double t, double dt) {
     return PG_OK;
 }
 
+// The loop stops at end_time as the reference's does (main.cu:323-327): the
+// steps left after t reaches it take dt = 0 and change nothing, as in the
+// 2D device loop.
 int sph_run(Sph* s, int nsteps, double* t_io, double* dt_io, double* dts) {   // main.cu:320-359
     double t = *t_io, dt = *dt_io;
     for (int n = 0; n < nsteps; ++n) {
         int rc;
+        if (!(t < s->P.end_time)) {
+            dt = 0.0;
+            if (dts) dts[n] = 0.0;
+            continue;
+        }
         if ((rc = sph_global_dt(s, dt, &dt))) return rc;
         if (t + dt > s->P.end_time) dt = s->P.end_time - t;
         rc = s->es == 8 ? ds_step<double>(s, t, dt) : ds_step<float>(s, t, dt);

@@ -2,9 +2,12 @@
 
 The headline gate: after 100 steps of the 256x512 disk and of the full
 1024x3072 gas+dust disk, the GPU state matches the oracle within 1e-5
-relative L1 on density and momentum.  Both sides take the oracle's dt sequence (the reference loop,
-main.cu:320-359, with dt from global_dt), so only the step arithmetic
-differs; the GPU's own dt is compared separately.
+relative L1 on density and on each momentum component, radial and
+azimuthal, gas and dust.  The lockstep tests give both sides the oracle's dt
+sequence (the reference loop, main.cu:320-359, with dt from global_dt), so
+only the step arithmetic differs; the full-size test runs the timed path
+itself (pg_run, the captured step, dt chosen on the device) and gives the
+oracle the GPU's step sizes.
 """
 import numpy as np
 import pytest
@@ -14,6 +17,9 @@ from penguin_amd import Disk, config_c1, config_c2
 from tests.helpers import parity_report
 
 pytestmark = pytest.mark.gpu
+
+GAS = ("rho", "rho_u", "rho_v", "momentum")
+DUST = ("rho_d", "rho_d_u", "rho_d_v")
 
 
 def _lockstep(p, nsteps):
@@ -40,22 +46,21 @@ def _lockstep(p, nsteps):
 def test_small_20_steps(prec):
     p = config_c2(precision=prec, xres=96, yres=160)
     rep = _lockstep(p, 20)
-    print("small gas+dust 20 steps", prec, {k: f"{v:.2e}" for k, v in rep.items()})
+    print("small gas+dust 20 steps", prec, {k: f"{v:.2e}" for k, v in rep.items()}, flush=True)
     tol = 1e-11 if prec == 8 else 1e-5
-    for k in ("rho", "rho_v", "momentum", "rho_d", "rho_d_v"):
-        assert rep[k] <= tol, (k, rep[k])
+    for k in GAS + DUST:
+        assert rep[k] <= (tol if k not in ("rho_u", "rho_d_u") or prec == 4 else 1e-9), (k, rep[k])
 
 
 @pytest.mark.parametrize("prec", [8, 4])
 def test_c1_100_steps(prec):
     p = config_c1(precision=prec)
     rep = _lockstep(p, 100)
-    print("C1 100 steps", prec, {k: f"{v:.2e}" for k, v in rep.items()})
+    print("C1 100 steps", prec, {k: f"{v:.2e}" for k, v in rep.items()}, flush=True)
     tol = 1e-10 if prec == 8 else 1e-5
     for k in ("rho", "rho_v", "momentum"):
         assert rep[k] <= tol, (k, rep[k])
-    if prec == 8:
-        assert rep["rho_u"] <= 1e-8
+    assert rep["rho_u"] <= (1e-8 if prec == 8 else 1e-5), rep["rho_u"]
 
 
 @pytest.mark.parametrize("prec", [8, 4])
@@ -81,44 +86,39 @@ def test_device_loop_matches_host_loop(prec):
 
 def test_c2_full_size_100_steps():
     """The headline gate at full size: 100 steps of the 1024x3072 gas+dust
-    disk (planet, viscosity, orbital advection, wave killing, dust drag) in
-    FP32 within 1e-5 relative L1 of the FP64 oracle on density and momenta,
-    gas and dust, with the FP64 GPU error stated beside it.  Both GPU runs
-    take the oracle's dt sequence; the GPU's own first dt is checked against
-    the oracle's."""
+    disk (planet, viscosity, orbital advection, wave killing, dust drag),
+    run in FP32 through the benchmarked path (pg_run replaying its captured
+    step, the CFL step chosen on the device), within 1e-5 relative L1 of the
+    FP64 oracle given the same step sizes: density and each momentum
+    component, gas and dust.  The FP64 GPU error is stated beside it (the
+    host-driven pg_solve loop with the same step sizes), and the GPU's
+    first step size is checked against the oracle's own."""
     p = config_c2(precision=4)
+    d = Disk(p)
+    dt0 = d.global_dt(p.end_time)
+    dts = d.run(100, use_graph=True)
+    g32, d32 = d.get_state()
+    d.close()
     o = oracle.Oracle(p)
-    t, dt_prev, dts = 0.0, p.end_time, []
-    for n in range(100):
-        dt = o.global_dt(dt_prev)
-        o.solve(t, dt)
-        dts.append(dt)
-        t += dt
-        dt_prev = dt
+    assert abs(dt0 / o.global_dt(p.end_time) - 1.0) <= 1e-6
+    t = 0.0
+    for dt in dts:
+        o.solve(t, float(dt))
+        t += float(dt)
     g_ref, d_ref = o.get_state()
     del o
-    reps = {}
-    for prec in (4, 8):
-        d = Disk(config_c2(precision=prec))
-        assert abs(d.global_dt(p.end_time) / dts[0] - 1.0) <= (1e-6 if prec == 4 else 1e-12)
-        t = 0.0
-        for dt in dts:
-            d.solve(t, dt)
-            t += dt
-        g, dd = d.get_state()
-        d.close()
-        reps[prec] = parity_report(g, g_ref, dd, d_ref)
-        print(f"C2 1024x3072 100 steps, precision {prec}:", {k: f"{v:.2e}" for k, v in reps[prec].items()},
-              flush=True)
-    # the gate: density and the momentum vector, gas and dust
-    for k in ("rho", "rho_v", "momentum", "rho_d", "rho_d_v"):
+    d = Disk(config_c2(precision=8))
+    t = 0.0
+    for dt in dts:
+        d.solve(t, float(dt))
+        t += float(dt)
+    g64, d64 = d.get_state()
+    d.close()
+    reps = {4: parity_report(g32, g_ref, d32, d_ref), 8: parity_report(g64, g_ref, d64, d_ref)}
+    for prec, rep in reps.items():
+        print(f"C2 1024x3072 100 steps, precision {prec}:", {k: f"{v:.2e}" for k, v in rep.items()}, flush=True)
+    for k in GAS + DUST:
         assert reps[4][k] <= 1e-5, (k, reps[4][k])
+    for k in ("rho", "rho_v", "momentum", "rho_d", "rho_d_v"):
         assert reps[8][k] <= 1e-8, (k, reps[8][k])
-    # the radial momenta alone are bounded, not gated: rho*u (viscous drift,
-    # the seeded noise, the planet's wake) is orders of magnitude below rho*v,
-    # so the same absolute rounding is far larger relative to it.  Measured:
-    # FP32 5e-4, FP64 2e-6, while the momentum vector's relative L1 exceeds
-    # rho*v's by a twentieth: the radial absolute error is ~1/20 of the
-    # azimuthal one.
-    assert reps[4]["rho_u"] <= 2e-3 and reps[4]["rho_d_u"] <= 5e-4
     assert reps[8]["rho_u"] <= 2e-5 and reps[8]["rho_d_u"] <= 5e-6
