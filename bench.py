@@ -34,12 +34,17 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
+# the CPU baseline's OpenMP threads stay on neighbouring cores (read when the
+# oracle library loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
 
 # algorithmic HBM bytes per interior cell per launch, in units of the storage
 # word: (fields read + fields written) that the kernel must move at minimum
 WORDS_PER_CELL = {
+    "sweep_x": 17,               # gas and dust x sweeps in one launch (8 + 9 below)
+    "ring": 12,                  # gas and dust azimuthal passes in one launch (6 + 6)
     "sweep_x_gas": 8,            # rho,u,vp in and out; + the viscous forces of the fused first half kick
     "ring_gas": 6,               # azimuthal sweep + orbital shift + residual advection
     "sweep_x_dust": 9,           # dust in and out; + gas rho,u,vp for the fused kick's drag
@@ -61,7 +66,9 @@ def parse():
     ap.add_argument("--precision", type=int, default=4, choices=(4, 8))
     ap.add_argument("--nr", type=int, default=1024)
     ap.add_argument("--nphi", type=int, default=3072)
-    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps of the CPU baseline sample")
+    ap.add_argument("--cpu-c1-steps", type=int, default=100, help="oracle steps of the 256x512 disk (CPU baseline)")
+    ap.add_argument("--fp64-steps", type=int, default=20, help="steps of the FP64 leg (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true",
                     help="time the uncaptured loop (one launch per pass) instead of the replayed hipGraph step")
@@ -224,6 +231,91 @@ def copy_probe(mib: int):
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
 
 
+def kernel_times(d, cells: int, word: int, steps: int) -> dict:
+    """Per-kernel durations from HIP events around every launch of `steps`
+    uncaptured steps of disk d."""
+    d.profile(True)
+    d.run(steps, use_graph=False)
+    kernels = {}
+    for name, words in WORDS_PER_CELL.items():
+        avg_us, cnt = d.kernel_time_us(name)
+        if cnt:
+            kernels[name] = {"avg_us": round(avg_us, 2), "launches": cnt,
+                             "GBps": round(words * word * cells / (avg_us * 1e-6) / 1e9, 1) if words else None}
+    d.profile(False)
+    return kernels
+
+
+def dominant(kernels: dict) -> str:
+    return max((k for k in kernels if WORDS_PER_CELL[k]), key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+
+
+def fp64_leg(nr: int, nphi: int, steps: int) -> dict:
+    """The same disk at the reference's own precision (FP64 storage and
+    arithmetic): the timed device loop and its dominant kernel."""
+    from penguin_amd import Disk, config_c2
+
+    p = config_c2(precision=8, xres=nr, yres=nphi)
+    cells = nr * nphi
+    d = Disk(p)
+    d.run(3, use_graph=True)
+    t0 = time.perf_counter()
+    d.run(steps, use_graph=True)
+    el = time.perf_counter() - t0
+    kernels = kernel_times(d, cells, 8, 10)
+    d.close()
+    dom = dominant(kernels)
+    achieved = WORDS_PER_CELL[dom] * 8 * cells / (kernels[dom]["avg_us"] * 1e-6) / 1e9
+    return {"steps": steps, "ms_per_step": round(1e3 * el / steps, 4), "value": round(cells * steps / el, 1),
+            "unit": "cell-updates/s", "dtype": "fp64",
+            "step_roofline_frac": round(cells * steps / el / (PEAK_HBM / (3 * 2 * 6 * 8)), 4),
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9,
+                         "unit": "GB/s", "frac": round(achieved * 1e9 / PEAK_HBM, 4),
+                         "bytes_per_launch": WORDS_PER_CELL[dom] * 8 * cells},
+            "kernels": kernels}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(nr: int, nphi: int, steps: int, c1_steps: int) -> dict:
+    """The FP64 oracle (OpenMP over rows) on this host's cores: `steps` of
+    the benchmarked disk from its initial state (the per-step rate
+    extrapolates to any run length: every step does the same work), and
+    c1_steps of the 256x512 gas disk in full."""
+    import oracle
+    from penguin_amd import config_c1, config_c2
+    oracle.build()
+    o = oracle.Oracle(config_c2(precision=8, xres=nr, yres=nphi))
+    c0 = time.perf_counter()
+    o.run(steps)
+    cdt = time.perf_counter() - c0
+    o.close()
+    out = {"value": round(nr * nphi * steps / cdt, 1), "unit": "cell-updates/s",
+           "cores": oracle.num_threads(), "kind": "port", "cpu_model": cpu_model(),
+           "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+           "sample": f"{steps} steps of the {nr}x{nphi} gas+dust disk from its initial state (FP64 oracle, "
+                     f"OpenMP over rows, {cdt:.1f} s, {1e3 * cdt / steps:.1f} ms per step; a run of K steps "
+                     f"takes K times that)"}
+    if c1_steps > 0:
+        q = config_c1(precision=8)
+        o = oracle.Oracle(q)
+        c0 = time.perf_counter()
+        o.run(c1_steps)
+        c1t = time.perf_counter() - c0
+        o.close()
+        out["c1"] = {"value": round(q.xres * q.yres * c1_steps / c1t, 1), "unit": "cell-updates/s",
+                     "sample": f"{c1_steps} steps of the 256x512 gas disk (FP64 oracle, {c1t:.2f} s)"}
+    return out
+
+
 def batched_leg(ndisk: int, steps: int, precision: int) -> dict:
     """The batched workload (ndisk uneven 2D gas disks, own CFL dt each,
     captured step, autotuned schedule) against looping the single-disk path
@@ -298,19 +390,10 @@ def main():
     # per-kernel durations: HIP events on the launch stream around every
     # launch of further uncaptured steps (the events cost ~40 us per step
     # in launch gaps, so they stay out of the throughput region)
-    d.profile(True)
     k0 = time.perf_counter()
-    d.run(a.kernel_steps, use_graph=False)
+    kernels = kernel_times(d, cells, word, a.kernel_steps)
     kel = time.perf_counter() - k0
-    kernels = {}
-    for name, words in WORDS_PER_CELL.items():
-        avg_us, cnt = d.kernel_time_us(name)
-        if cnt:
-            kernels[name] = {"avg_us": round(avg_us, 2), "launches": cnt,
-                             "GBps": round(words * word * cells / (avg_us * 1e-6) / 1e9, 1) if words else None}
-    d.profile(False)
-    dom = max((k for k in kernels if WORDS_PER_CELL[k]),
-              key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    dom = dominant(kernels)
     bytes_launch = WORDS_PER_CELL[dom] * word * cells
     achieved = bytes_launch / (kernels[dom]["avg_us"] * 1e-6) / 1e9
     # full-step roofline: 3 read+write passes of 6 fields (x, phi+OA, sources)
@@ -327,18 +410,8 @@ def main():
     adjoint = adjoint_leg(a.adjoint_steps, a.precision) if a.adjoint_steps > 0 else None
     sph = sph_leg(a.sph_steps, a.precision, not a.no_cpu) if a.sph_steps > 0 else None
 
-    cpu = None
-    if not a.no_cpu and a.cpu_steps > 0:
-        import oracle
-        oracle.build()
-        o = oracle.Oracle(config_c2(precision=8, xres=a.nr, yres=a.nphi))
-        c0 = time.perf_counter()
-        o.run(a.cpu_steps)
-        cdt = time.perf_counter() - c0
-        cpu = {"value": round(cells * a.cpu_steps / cdt, 1), "unit": "cell-updates/s",
-               "cores": oracle.num_threads(), "kind": "port",
-               "sample": f"{a.cpu_steps} steps of the {a.nr}x{a.nphi} gas+dust disk from its initial state "
-                         f"(FP64 oracle, OpenMP over rows, {cdt:.1f} s)"}
+    fp64 = fp64_leg(a.nr, a.nphi, a.fp64_steps) if a.fp64_steps > 0 and a.precision == 4 else None
+    cpu = cpu_baseline(a.nr, a.nphi, a.cpu_steps, a.cpu_c1_steps) if not a.no_cpu and a.cpu_steps > 0 else None
 
     out = {
         # the headline metric as a whole: `value` is the cell-updates/s, the
@@ -372,6 +445,8 @@ def main():
         "host_state_roundtrip_ms": round(1e3 * host_rt, 3),
         "cpu_baseline": cpu,
     }
+    if fp64 is not None:
+        out["fp64"] = fp64
     if batched is not None:
         out["batched"] = batched
     if adjoint is not None:

@@ -868,8 +868,8 @@ int group_init(Group& g, int n, const pg_params* params) {
 
 // ------------------------------------------------------------ launching
 const char* kname[] = {"prologue", "viscosity", "sources_kick", "sweep_x_gas", "ring_gas", "sweep_x_dust",
-                       "ring_dust", "predictor_viscosity", "corrector_viscosity", "cfl"};
-enum KId { K_PRO, K_VIS, K_KICK, K_SXG, K_RG, K_SXD, K_RD, K_PRED, K_CORR, K_CFL };
+                       "ring_dust", "predictor_viscosity", "corrector_viscosity", "cfl", "sweep_x", "ring"};
+enum KId { K_PRO, K_VIS, K_KICK, K_SXG, K_RG, K_SXD, K_RD, K_PRED, K_CORR, K_CFL, K_SX, K_RING };
 
 int timed_begin(Group& g) {
     if (!g.profile) return PG_OK;
@@ -922,31 +922,34 @@ int timers_resolve(Group& g) {
 // With viscosity, the forces of the step start (d.vis) come from the
 // previous step's corrector kernel; a step whose start state did not come
 // from one (vis_ready false) computes them first.
+// fuse: the corrector also runs the next step's prologue (the device loop
+// without time averaging; the loop launches the first prologue itself).
 template <typename real>
-int step_passes(Group& g, bool vis_ready) {
+int step_passes(Group& g, bool vis_ready, bool fuse) {
     const LaunchSet<real> L = g.ls<real>();
     hipStream_t s = g.stream;
     if (g.visc && !vis_ready) LAUNCH(g, K_VIS, Launch<real>::viscosity(L, g.par(), 0, s));
-    // first half kick fused into the x sweeps; the dust x sweep runs before
-    // the gas azimuthal pass (gas and dust sweeps do not interact) so that
-    // it still finds the gas state the kick's drag term needs
-    LAUNCH(g, K_SXG, Launch<real>::sweep_x_gas(L, g.par(), 1, s));
-    g.gpar ^= 1;
+    // first half kick fused into the x sweeps; gas and dust sweep in one
+    // launch (the dust's drag reads the gas before its sweep, the buffer the
+    // gas sweep reads), and their azimuthal passes in one launch
     if (g.dust) {
-        LAUNCH(g, K_SXD, Launch<real>::sweep_x_dust(L, g.par(), 1, s));
+        LAUNCH(g, K_SX, Launch<real>::sweep_x_both(L, g.par(), 1, s));
+        g.gpar ^= 1;
         g.dpar ^= 1;
-    }
-    LAUNCH(g, K_RG, Launch<real>::ring_gas(L, g.par(), 3, s));
-    g.gpar ^= 1;
-    if (g.dust) {
-        LAUNCH(g, K_RD, Launch<real>::ring_dust(L, g.par(), 3, s));
+        LAUNCH(g, K_RING, Launch<real>::ring_both(L, g.par(), 3, s));
+        g.gpar ^= 1;
         g.dpar ^= 1;
+    } else {
+        LAUNCH(g, K_SXG, Launch<real>::sweep_x_gas(L, g.par(), 1, s));
+        g.gpar ^= 1;
+        LAUNCH(g, K_RG, Launch<real>::ring_gas(L, g.par(), 3, s));
+        g.gpar ^= 1;
     }
     if (g.visc) {
         // predictor + viscosity of the predicted state; corrector + wave
         // killing + CFL rate + viscosity of the corrected state
         LAUNCH(g, K_PRED, Launch<real>::predvis(L, g.par(), s));
-        LAUNCH(g, K_CORR, Launch<real>::corrvis(L, g.par(), g.kill ? 1 : 0, s));
+        LAUNCH(g, K_CORR, Launch<real>::corrvis(L, g.par(), g.kill ? 1 : 0, fuse ? 1 : 0, s));
         g.gpar ^= 1;   // the corrected gas is in the other buffer
     } else {
         LAUNCH(g, K_CORR, Launch<real>::sources(L, g.par(), 2, g.kill ? 1 : 0, 1, s));
@@ -954,8 +957,8 @@ int step_passes(Group& g, bool vis_ready) {
     return PG_OK;
 }
 
-int passes(Group& g, bool vis_ready) {
-    return g.prec == 8 ? step_passes<double>(g, vis_ready) : step_passes<float>(g, vis_ready);
+int passes(Group& g, bool vis_ready, bool fuse = false) {
+    return g.prec == 8 ? step_passes<double>(g, vis_ready, fuse) : step_passes<float>(g, vis_ready, fuse);
 }
 
 // viscous forces of the current state into d.vis (start of a run)
@@ -1060,14 +1063,27 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
         c.rate_bits = 0ull;
         c.err = 0;
         c.err_step = 0;
+        c.arrive = 0u;
+        c.nstep_end = nsteps;
     }
     if ((rc = push_ctl(g))) return rc;
     // rate of the current state for the first step (later steps get it from
     // the corrector), and its viscous forces
     if ((rc = launch_cfl(g))) return rc;
     if ((rc = vis_start(g))) return rc;
+    // with viscosity and no time average the corrector runs each next
+    // step's prologue, and only the first one is launched here
+#ifndef PG_FUSE_PROLOGUE
+#define PG_FUSE_PROLOGUE 1
+#endif
+    const bool fuse = PG_FUSE_PROLOGUE && g.visc && !g.ave;
+    if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
+    if (fuse) {
+        if ((rc = timed_begin(g))) return rc;
+        if ((rc = prologue(g, 1, 1))) return rc;
+        if ((rc = timed_end(g, K_PRO))) return rc;
+    }
     if (use_graph && !g.profile) {
-        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
         for (int n = 0; n < nsteps; ++n) {
             const int p = g.par();
             if (!g.graph[p]) {
@@ -1075,9 +1091,9 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
                 // parity bookkeeping advances as if it had run
                 hipGraph_t gr;
                 HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-                int rc1 = prologue(g, 1, 1);
+                int rc1 = fuse ? PG_OK : prologue(g, 1, 1);
                 if (!rc1) rc1 = average_pass(g);
-                int rc2 = rc1 ? rc1 : passes(g, true);
+                int rc2 = rc1 ? rc1 : passes(g, true, fuse);
                 if (!rc2) rc2 = average_pass(g);
                 hipError_t ec = hipStreamEndCapture(g.stream, &gr);
                 if (rc2) return rc2;
@@ -1094,19 +1110,19 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
             }
             HIP_TRY(hipGraphLaunch(g.graph[p], g.stream));
         }
-        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
     } else {
-        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
         for (int n = 0; n < nsteps; ++n) {
-            if ((rc = timed_begin(g))) return rc;
-            if ((rc = prologue(g, 1, 1))) return rc;
-            if ((rc = timed_end(g, K_PRO))) return rc;
+            if (!fuse) {
+                if ((rc = timed_begin(g))) return rc;
+                if ((rc = prologue(g, 1, 1))) return rc;
+                if ((rc = timed_end(g, K_PRO))) return rc;
+            }
             if ((rc = average_pass(g))) return rc;
-            if ((rc = passes(g, true))) return rc;
+            if ((rc = passes(g, true, fuse))) return rc;
             if ((rc = average_pass(g))) return rc;
         }
-        if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
     }
+    if (loop_ms) HIP_TRY(hipEventRecord(g.ev_b, g.stream));
     if ((rc = pull_ctl(g))) return rc;
     if (loop_ms) HIP_TRY(hipEventElapsedTime(loop_ms, g.ev_a, g.ev_b));
     if ((rc = timers_resolve(g))) return rc;
@@ -1525,7 +1541,7 @@ int adj_recompute(pg_handle* h, int m, bool write_next) {
     if (visc) HIPL(Launch<real>::predvis(at(AA_PRED), 0, s));
     if (write_next) {
         if (visc) {
-            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, s));
+            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, 0, s));
         } else {
             HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
             HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 0, s));
@@ -1643,7 +1659,7 @@ int adj_tape_forward(pg_handle* h, int nsteps, double t0, double dt0) {
         HIPL(Launch<real>::ring_gas(at(AA_P), 0, 3, s));
         if (visc) {
             HIPL(Launch<real>::predvis(at(AA_PRED), 0, s));
-            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, s));
+            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, 0, s));
         } else {
             HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
             HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 1, s));

@@ -128,12 +128,9 @@ struct XBlock {
 // the viscous forces of the step start) to every interior cell as it is
 // loaded, in place of a separate source-term pass.
 template <typename real, int TX>
-__global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> ls, int par, int kick) {
-    const int2 w = ls.wl_x[blockIdx.x];
+__device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2 w, int par, int kick, real* sm) {
     const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    real* sm = reinterpret_cast<real*>(smem_raw);
     real* s_r = sm;             // rho - rho_b
     real* s_p = sm + S;         // p - p_b = c_s^2 (rho - rho_b)
     real* s_u = sm + 2 * S;
@@ -267,15 +264,13 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> 
 // ----------------------------------------------------------- x sweep, dust
 // kick != 0: apply the first half kick (drag toward the gas and the
 // planet's pull, cell_sources mode 0) to every interior cell as it is
-// loaded; the gas state before its own kick and sweep is the buffer
-// opposite the current one (the gas x sweep has just run).
+// loaded; the drag reads the gas state before its own kick and sweep, the
+// gas buffer gpre (the gas x sweep reads it and writes the other one).
 template <typename real, int TX>
-__global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_dust(LaunchSet<real> ls, int par, int kick) {
-    const int2 w = ls.wl_x[blockIdx.x];
+__device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int2 w, int par, int gpre, int kick,
+                                                  real* sm) {
     const DiskArgs<real>& d = ls.disks[w.x];
     constexpr int S = TX + 2 * NPAD;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    real* sm = reinterpret_cast<real*>(smem_raw);
     real* s_r = sm;
     real* s_u = sm + S;
     real* s_L = sm + 2 * S;
@@ -306,7 +301,7 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_dust(LaunchSet<real>
         } else {
             r = in[row + i]; u = in[N + row + i]; vp = in[2 * N + row + i];
             if (kick) {
-                const PG_GLOBAL real* __restrict__ g0 = gmem(d.gas[gsel(par) ^ 1]);
+                const PG_GLOBAL real* __restrict__ g0 = gmem(d.gas[gpre]);
                 const real rho = TAB(T_G_R, ip) + g0[row + i], cu = g0[N + row + i], cvp = g0[2 * N + row + i];
                 real ndu, ndvp, unused0, unused1;
                 cell_sources<real, false, true>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), rho, cu, cvp, real(0), real(0),
@@ -374,14 +369,41 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_dust(LaunchSet<real>
     }
 }
 
+// The x sweeps as kernels: one fluid per launch, or both in one launch
+// (blocks [0, n_x) the gas rows, [n_x, 2 n_x) the dust rows), which the
+// gas and dust sweeps allow because they write different buffers and the
+// dust's drag reads the gas buffer the gas sweep only reads.
+template <typename real, int TX>
+__global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_gas(LaunchSet<real> ls, int par, int kick) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    sweep_x_gas_tile<real, TX>(ls, ls.wl_x[blockIdx.x], par, kick, reinterpret_cast<real*>(smem_raw));
+}
+// par: the parity after the gas sweep (the pre-sweep gas is the other buffer)
+template <typename real, int TX>
+__global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_dust(LaunchSet<real> ls, int par, int kick) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    sweep_x_dust_tile<real, TX>(ls, ls.wl_x[blockIdx.x], par, gsel(par) ^ 1, kick, reinterpret_cast<real*>(smem_raw));
+}
+template <typename real, int TX>
+__global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_both(LaunchSet<real> ls, int par, int kick) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    real* sm = reinterpret_cast<real*>(smem_raw);
+    const int b = blockIdx.x;
+    if (b < ls.n_x) sweep_x_gas_tile<real, TX>(ls, ls.wl_x[b], par, kick, sm);
+    else sweep_x_dust_tile<real, TX>(ls, ls.wl_x[b - ls.n_x], par, gsel(par), kick, sm);
+}
+
 
 // Tile geometry of the azimuthal pass for a thread count: TR radii x TY
 // output rows.  The rows of one radius are independent of the other radii,
 // so a tile is TR independent azimuthal strips side by side; TR consecutive
 // radii make each row of the tile one contiguous segment in HBM.
+#ifndef PG_PHI512_TR
+#define PG_PHI512_TR 16
+#endif
 template <int B> struct PhiTile;
 template <> struct PhiTile<256> { static constexpr int TR = 16, TY = 16; };
-template <> struct PhiTile<512> { static constexpr int TR = 16, TY = 32; };
+template <> struct PhiTile<512> { static constexpr int TR = PG_PHI512_TR, TY = 512 / PG_PHI512_TR; };
 template <> struct PhiTile<1024> { static constexpr int TR = 16, TY = 64; };
 
 // Azimuthal pass of one fluid over a tile, fused with the orbital shift and
@@ -402,15 +424,30 @@ template <> struct PhiTile<1024> { static constexpr int TR = 16, TY = 64; };
 // the deviations, the interface pressures deviations from c_s^2 rho_b, and
 // the residual advection's mass flux the deviation's (the background's
 // uniform flux cancels in every difference).
+// LDS of one tile: 13 arrays of E cells (state 3, slopes 6, fluxes 4).
+template <int B> struct PhiLds {
+    static constexpr int NK = PhiTile<B>::TY + 4 * NPAD;   // LDS rows per radius
+    static constexpr int E = NK * PhiTile<B>::TR;          // LDS cells per array
+    static constexpr int WORDS = 13 * E;
+};
 template <typename real, bool DUST, int B>
-__global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flags) {
+__device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int par, int flags, real* lds) {
     constexpr int TR = PhiTile<B>::TR, TY = PhiTile<B>::TY;
-    constexpr int NK = TY + 4 * NPAD;          // LDS rows per radius
-    constexpr int E = NK * TR;                 // LDS cells per array
-    __shared__ real c0[E], c1[E], c2[E];
-    __shared__ real sl0[E], sr0[E], sl1[E], sr1[E], sl2[E], sr2[E];
-    __shared__ real F0[E], F1[E], F2[E], F3[E];
-    const int2 w = ls.wl_ring[blockIdx.x];
+    constexpr int NK = PhiLds<B>::NK;
+    constexpr int E = PhiLds<B>::E;
+    real* const c0 = lds;
+    real* const c1 = lds + E;
+    real* const c2 = lds + 2 * E;
+    real* const sl0 = lds + 3 * E;
+    real* const sr0 = lds + 4 * E;
+    real* const sl1 = lds + 5 * E;
+    real* const sr1 = lds + 6 * E;
+    real* const sl2 = lds + 7 * E;
+    real* const sr2 = lds + 8 * E;
+    real* const F0 = lds + 9 * E;
+    real* const F1 = lds + 10 * E;
+    real* const F2 = lds + 11 * E;
+    real* const F3 = lds + 12 * E;
     if (w.y < 0) return;   // padding entry of the work list (whole block)
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
@@ -595,6 +632,22 @@ __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flag
     }
 }
 
+// The azimuthal pass of one fluid, or of both in one launch (blocks
+// [0, n_ring) the gas tiles, [n_ring, 2 n_ring) the dust tiles; the two
+// fluids read and write different buffers).
+template <typename real, bool DUST, int B>
+__global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flags) {
+    __shared__ real lds[PhiLds<B>::WORDS];
+    phi_tile<real, DUST, B>(ls, ls.wl_ring[blockIdx.x], par, flags, lds);
+}
+template <typename real, int B>
+__global__ __launch_bounds__(B) void k_phi_both(LaunchSet<real> ls, int par, int flags) {
+    __shared__ real lds[PhiLds<B>::WORDS];
+    const int b = blockIdx.x;
+    if (b < ls.n_ring) phi_tile<real, false, B>(ls, ls.wl_ring[b], par, flags, lds);
+    else phi_tile<real, true, B>(ls, ls.wl_ring[b - ls.n_ring], par, flags, lds);
+}
+
 // ----------------------------------------------------------------- viscosity
 // Pointwise and stencil kernels work on tiles of PT_X radii x TH rows (the
 // launch set's pt_rows as a template parameter); thread t covers radius
@@ -716,17 +769,29 @@ __device__ __forceinline__ void block_max_store(double* part, real v) {
     }
 }
 
+// A partial written by another block of the running kernel: loaded at
+// device scope, past this compute unit's caches.
+template <bool COHERENT>
+__device__ __forceinline__ double load_partial(const double* p) {
+    if (!COHERENT) return *p;
+    const unsigned long long b =
+        __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)b);
+}
+
 // Max over the partial array by one block; result in thread 0.  Eight
 // independent loads per iteration keep the memory pipeline full.
+// COHERENT: the partials were written by other blocks of the same launch.
+template <bool COHERENT = false>
 __device__ __forceinline__ double reduce_partials(const double* part, int n) {
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int stride = blockDim.x * 8;
     int k0 = threadIdx.x;
     for (; k0 + 7 * (int)blockDim.x < n; k0 += stride) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = fmax(a[u], part[k0 + u * blockDim.x]);
+        for (int u = 0; u < 8; ++u) a[u] = fmax(a[u], load_partial<COHERENT>(part + k0 + u * blockDim.x));
     }
-    for (int k = k0; k < n; k += blockDim.x) a[0] = fmax(a[0], part[k]);
+    for (int k = k0; k < n; k += blockDim.x) a[0] = fmax(a[0], load_partial<COHERENT>(part + k));
     double x = fmax(fmax(fmax(a[0], a[1]), fmax(a[2], a[3])), fmax(fmax(a[4], a[5]), fmax(a[6], a[7])));
     for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
     __shared__ double wmax[16];
@@ -911,8 +976,17 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
 // result goes to the other buffer of the pair, because neighbouring tiles
 // read this tile's cells as their halo input.  (The dust is corrected in
 // place: its halo results are not used.)
+template <typename real, bool FUSED>
+__device__ __forceinline__ void prologue_disk(const DiskArgs<real>& d, int mode, int do_orbadv, const double* td);
+
+// fuse != 0: the next step's prologue runs in the last block of each disk to
+// finish (counted on the disk's control block), in place of a launch of its
+// own: every block stores its rate partial, makes its writes visible at
+// device scope and counts itself; the block that counts last has seen every
+// other block's count, so after an acquiring fence it reduces their
+// partials, and it sets the count back to 0 for the next launch.
 template <typename real, int TH>
-__global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, int do_kill) {
+__global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, int do_kill, int fuse) {
     __shared__ real Sr[ST_X * (TH + 2)], Su[ST_X * (TH + 2)], Sv[ST_X * (TH + 2)];
     const int2 w = ls.wl_pt[blockIdx.x];
     if (w.y < 0) return;
@@ -974,6 +1048,21 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
     block_max_store(d.rate_part + w.y, rate);   // includes the barrier the tile needs
     __syncthreads();
     vis_from_tile<real, TH>(d, Sr, Su, Sv, i0, j0, gmem(d.vis));
+    if (fuse) {
+        __shared__ int s_last;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();   // release: this block's partial before its count
+            const unsigned prev = atomicAdd(&d.ctl->arrive, 1u);
+            s_last = prev + 1u == (unsigned)d.rate_nblk;
+        }
+        __syncthreads();
+        if (s_last) {
+            __threadfence();   // acquire: the other blocks' partials after their counts
+            prologue_disk<real, true>(d, 1, 1, nullptr);
+            if (threadIdx.x == 0) d.ctl->arrive = 0u;
+        }
+    }
 }
 
 // CFL rate of the current state (get_dt_lv1, timestep.cu:18-97).
@@ -1109,12 +1198,18 @@ __global__ __launch_bounds__(1024) void k_rate_reduce(LaunchSet<real> ls) {
 // rate reduction, and the two planets (lanes 0 and 1 of wave 0, one
 // evaluation for both) need no barrier before the shifts, which do not read
 // them.
-template <typename real>
-__global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode, int do_orbadv, const double* td) {
-    const DiskArgs<real>& d = ls.disks[blockIdx.x];
+// The prologue of one disk, by the calling block (any size).  FUSED: the
+// call from the corrector's last block, which does nothing once the run's
+// last step has been taken (ctl->nstep == ctl->nstep_end), so that the
+// control block ends a run as the separate prologue leaves it; the rate
+// partials it reduces were written by the other blocks of the same launch
+// and are read through the device-coherent path.
+template <typename real, bool FUSED>
+__device__ __forceinline__ void prologue_disk(const DiskArgs<real>& d, int mode, int do_orbadv, const double* td) {
     double* dts = d.dts;
     StepCtl* ctl = d.ctl;
     __shared__ double s_dt, s_t;
+    __shared__ int s_skip;
     const int tid = threadIdx.x;
     const bool oa = d.orbadv && do_orbadv;
     double rt0 = 0.0, ry0 = 0.0;
@@ -1132,9 +1227,14 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
         tend = ctl->end_time;
         cfl = d.CFL;
         err0 = ctl->err;
+        if (FUSED) s_skip = n0 >= ctl->nstep_end;
+    }
+    if (FUSED) {
+        __syncthreads();
+        if (s_skip) return;
     }
     if (tid < 2) b = tid == 0 ? ctl->planet0 : ctl->planet1;
-    double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
+    double rate = mode == 1 ? reduce_partials<FUSED>(d.rate_part, d.rate_nblk) : 0.0;
     if (tid == 0) {
         double t = t0, dt = dt0;
         if (mode == 2) {
@@ -1193,6 +1293,11 @@ __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode,
     }
 }
 
+template <typename real>
+__global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode, int do_orbadv, const double* td) {
+    prologue_disk<real, false>(ls.disks[blockIdx.x], mode, do_orbadv, td);
+}
+
 // ------------------------------------------------------------- launchers
 template <typename Kern>
 static hipError_t set_lds(Kern k, size_t lds) {
@@ -1200,52 +1305,60 @@ static hipError_t set_lds(Kern k, size_t lds) {
     return hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// which: 0 the gas, 1 the dust, 2 both in one launch
 template <typename real, int TX>
-static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, bool dust, int kick, hipStream_t s) {
+static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, int which, int kick, hipStream_t s) {
     const size_t S = TX + 2 * NPAD;
-    if (dust) {
+    if (which == 1) {
         size_t lds = 13 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_dust<real, TX>, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
-    } else {
+    } else if (which == 0) {
         size_t lds = 17 * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_gas<real, TX>, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
+    } else {
+        size_t lds = 17 * S * sizeof(real);
+        hipError_t e = set_lds(k_sweep_x_both<real, TX>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_sweep_x_both<real, TX>), dim3(2 * ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
     }
     return hipGetLastError();
 }
 
 template <typename real>
-static hipError_t sweep_x_any(const LaunchSet<real>& ls, int par, bool dust, int kick, hipStream_t s) {
+static hipError_t sweep_x_any(const LaunchSet<real>& ls, int par, int which, int kick, hipStream_t s) {
     switch (ls.tile_x) {
-        case 122: return launch_sweep_x<real, 122>(ls, par, dust, kick, s);
-        case 250: return launch_sweep_x<real, 250>(ls, par, dust, kick, s);
-        case 506: return launch_sweep_x<real, 506>(ls, par, dust, kick, s);
-        case 128: return launch_sweep_x<real, 128>(ls, par, dust, kick, s);
-        case 384: return launch_sweep_x<real, 384>(ls, par, dust, kick, s);
-        case 512: return launch_sweep_x<real, 512>(ls, par, dust, kick, s);
-        case 1024: return launch_sweep_x<real, 1024>(ls, par, dust, kick, s);
-        default: return launch_sweep_x<real, 256>(ls, par, dust, kick, s);
+        case 122: return launch_sweep_x<real, 122>(ls, par, which, kick, s);
+        case 250: return launch_sweep_x<real, 250>(ls, par, which, kick, s);
+        case 506: return launch_sweep_x<real, 506>(ls, par, which, kick, s);
+        case 128: return launch_sweep_x<real, 128>(ls, par, which, kick, s);
+        case 384: return launch_sweep_x<real, 384>(ls, par, which, kick, s);
+        case 512: return launch_sweep_x<real, 512>(ls, par, which, kick, s);
+        case 1024: return launch_sweep_x<real, 1024>(ls, par, which, kick, s);
+        default: return launch_sweep_x<real, 256>(ls, par, which, kick, s);
     }
 }
 
 template <typename real, int B>
-static hipError_t launch_ring(const LaunchSet<real>& ls, int par, int flags, bool dust, hipStream_t s) {
-    if (dust)
+static hipError_t launch_ring(const LaunchSet<real>& ls, int par, int flags, int which, hipStream_t s) {
+    if (which == 1)
         hipLaunchKernelGGL((k_phi<real, true, B>), dim3(ls.n_ring), dim3(B), 0, s, ls, par, flags);
-    else
+    else if (which == 0)
         hipLaunchKernelGGL((k_phi<real, false, B>), dim3(ls.n_ring), dim3(B), 0, s, ls, par, flags);
+    else
+        hipLaunchKernelGGL((k_phi_both<real, B>), dim3(2 * ls.n_ring), dim3(B), 0, s, ls, par, flags);
     return hipGetLastError();
 }
 
 template <typename real>
-static hipError_t ring_any(const LaunchSet<real>& ls, int par, int flags, bool dust, hipStream_t s) {
+static hipError_t ring_any(const LaunchSet<real>& ls, int par, int flags, int which, hipStream_t s) {
     switch (ls.ring_b) {
-        case 256: return launch_ring<real, 256>(ls, par, flags, dust, s);
-        case 1024: return launch_ring<real, 1024>(ls, par, flags, dust, s);
-        default: return launch_ring<real, 512>(ls, par, flags, dust, s);
+        case 256: return launch_ring<real, 256>(ls, par, flags, which, s);
+        case 1024: return launch_ring<real, 1024>(ls, par, flags, which, s);
+        default: return launch_ring<real, 512>(ls, par, flags, which, s);
     }
 }
 
@@ -1265,10 +1378,12 @@ template <typename real>
 struct Launch {
     static int phi_tile_radii(int B) { return B == 256 ? PhiTile<256>::TR : B == 1024 ? PhiTile<1024>::TR : PhiTile<512>::TR; }
     static int phi_tile_rows(int B) { return B == 256 ? PhiTile<256>::TY : B == 1024 ? PhiTile<1024>::TY : PhiTile<512>::TY; }
-    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, false, kick, s); }
-    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, true, kick, s); }
-    static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, false, s); }
-    static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, true, s); }
+    static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, 0, kick, s); }
+    static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, 1, kick, s); }
+    static hipError_t sweep_x_both(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, 2, kick, s); }
+    static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, 0, s); }
+    static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, 1, s); }
+    static hipError_t ring_both(const LaunchSet<real>& ls, int par, int flags, hipStream_t s) { return ring_any(ls, par, flags, 2, s); }
     static hipError_t viscosity(const LaunchSet<real>& ls, int par, int which, hipStream_t s) {
         PT_LAUNCH(k_viscosity, ls, s, par, which);
         return hipGetLastError();
@@ -1281,8 +1396,8 @@ struct Launch {
         PT_LAUNCH(k_predvis, ls, s, par);
         return hipGetLastError();
     }
-    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s) {
-        PT_LAUNCH(k_corrvis, ls, s, par, kill);
+    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, int fuse, hipStream_t s) {
+        PT_LAUNCH(k_corrvis, ls, s, par, kill, fuse);
         return hipGetLastError();
     }
     static hipError_t average(const LaunchSet<real>& ls, int par, hipStream_t s) {

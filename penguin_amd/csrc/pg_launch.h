@@ -18,12 +18,17 @@ struct Launch {
     static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, int kick, hipStream_t s);
     static hipError_t ring_gas(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
     static hipError_t ring_dust(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
+    // gas and dust in one launch: the dust x sweep's drag reads the gas
+    // buffer that is current at `par` (the one the gas sweep reads)
+    static hipError_t sweep_x_both(const LaunchSet<real>& ls, int par, int kick, hipStream_t s);
+    static hipError_t ring_both(const LaunchSet<real>& ls, int par, int flags, hipStream_t s);
     static hipError_t viscosity(const LaunchSet<real>& ls, int par, int which, hipStream_t s);
     static hipError_t sources(const LaunchSet<real>& ls, int par, int mode, int kill, int rate, hipStream_t s);
     // fused: predictor + viscosity of T (d.vis -> d.vis2); corrector + kill +
     // CFL rate + viscosity of the corrected state (d.vis2 -> d.vis)
     static hipError_t predvis(const LaunchSet<real>& ls, int par, hipStream_t s);
-    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s);
+    // fuse != 0: the next step's prologue (mode 1) in each disk's last block
+    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, int fuse, hipStream_t s);
     static hipError_t average(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t torque(const LaunchSet<real>& ls, int par, double* part, hipStream_t s);
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s);
