@@ -58,6 +58,11 @@ WORDS_PER_CELL = {
 }
 
 
+def note(msg: str) -> None:
+    """One progress line on stderr as each leg of the bench ends."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -406,11 +411,21 @@ def main():
     d.get_state()
     host_rt = time.perf_counter() - h0
 
+    note(f"single disk: {1e3 * elapsed / a.steps:.4f} ms per step, {value:.4g} cell-updates/s")
+
     batched = batched_leg(a.batch_disks, a.batch_steps, a.precision) if a.batch_disks > 0 else None
+    if batched:
+        note(f"batched: speedup {batched['speedup']}")
     adjoint = adjoint_leg(a.adjoint_steps, a.precision) if a.adjoint_steps > 0 else None
+    if adjoint:
+        note(f"adjoint: cost ratio {adjoint['cost_ratio']}")
     sph = sph_leg(a.sph_steps, a.precision, not a.no_cpu) if a.sph_steps > 0 else None
+    if sph:
+        note(f"3D disk: {sph['ms_per_step']} ms per step")
 
     fp64 = fp64_leg(a.nr, a.nphi, a.fp64_steps) if a.fp64_steps > 0 and a.precision == 4 else None
+    if fp64:
+        note(f"fp64: {fp64['ms_per_step']} ms per step")
     cpu = cpu_baseline(a.nr, a.nphi, a.cpu_steps, a.cpu_c1_steps) if not a.no_cpu and a.cpu_steps > 0 else None
 
     out = {

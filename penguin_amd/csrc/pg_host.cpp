@@ -550,9 +550,14 @@ void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<do
             size_t o = (size_t)j * nx + i;
             int ip = i + NPAD;
             double xc = T(T_XC, ip);
-            // stored: rho - rho_b, here the noise's share of rho_b (1 + eps)
+            // stored: rho - rho_b, for the perturbed rho_b (1 + eps); the
+            // difference is exact in FP64 (the two are within a factor 2),
+            // so rho_b + the stored value gives back rho_b (1 + eps) itself
             double drho = 0.0, u = T(T_G_U, ip);
-            if (P.noise_rho != 0.0) drho = T(T_G_R, ip) * (P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, j) - 1.0));
+            if (P.noise_rho != 0.0) {
+                const double rb = T(T_G_R, ip);
+                drho = rb * (1.0 + P.noise_rho * (2.0 * keyed_uniform(P.seed, 0, i, j) - 1.0)) - rb;
+            }
             if (P.noise_vr != 0.0)
                 u += P.noise_vr * std::sqrt(P.gam * pr.cs2(xc)) * (2.0 * keyed_uniform(P.seed, 1, i, j) - 1.0);
             gas[o] = drho;
@@ -922,10 +927,8 @@ int timers_resolve(Group& g) {
 // With viscosity, the forces of the step start (d.vis) come from the
 // previous step's corrector kernel; a step whose start state did not come
 // from one (vis_ready false) computes them first.
-// fuse: the corrector also runs the next step's prologue (the device loop
-// without time averaging; the loop launches the first prologue itself).
 template <typename real>
-int step_passes(Group& g, bool vis_ready, bool fuse) {
+int step_passes(Group& g, bool vis_ready) {
     const LaunchSet<real> L = g.ls<real>();
     hipStream_t s = g.stream;
     if (g.visc && !vis_ready) LAUNCH(g, K_VIS, Launch<real>::viscosity(L, g.par(), 0, s));
@@ -949,7 +952,7 @@ int step_passes(Group& g, bool vis_ready, bool fuse) {
         // predictor + viscosity of the predicted state; corrector + wave
         // killing + CFL rate + viscosity of the corrected state
         LAUNCH(g, K_PRED, Launch<real>::predvis(L, g.par(), s));
-        LAUNCH(g, K_CORR, Launch<real>::corrvis(L, g.par(), g.kill ? 1 : 0, fuse ? 1 : 0, s));
+        LAUNCH(g, K_CORR, Launch<real>::corrvis(L, g.par(), g.kill ? 1 : 0, s));
         g.gpar ^= 1;   // the corrected gas is in the other buffer
     } else {
         LAUNCH(g, K_CORR, Launch<real>::sources(L, g.par(), 2, g.kill ? 1 : 0, 1, s));
@@ -957,8 +960,8 @@ int step_passes(Group& g, bool vis_ready, bool fuse) {
     return PG_OK;
 }
 
-int passes(Group& g, bool vis_ready, bool fuse = false) {
-    return g.prec == 8 ? step_passes<double>(g, vis_ready, fuse) : step_passes<float>(g, vis_ready, fuse);
+int passes(Group& g, bool vis_ready) {
+    return g.prec == 8 ? step_passes<double>(g, vis_ready) : step_passes<float>(g, vis_ready);
 }
 
 // viscous forces of the current state into d.vis (start of a run)
@@ -1063,26 +1066,13 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
         c.rate_bits = 0ull;
         c.err = 0;
         c.err_step = 0;
-        c.arrive = 0u;
-        c.nstep_end = nsteps;
     }
     if ((rc = push_ctl(g))) return rc;
     // rate of the current state for the first step (later steps get it from
     // the corrector), and its viscous forces
     if ((rc = launch_cfl(g))) return rc;
     if ((rc = vis_start(g))) return rc;
-    // with viscosity and no time average the corrector runs each next
-    // step's prologue, and only the first one is launched here
-#ifndef PG_FUSE_PROLOGUE
-#define PG_FUSE_PROLOGUE 1
-#endif
-    const bool fuse = PG_FUSE_PROLOGUE && g.visc && !g.ave;
     if (loop_ms) HIP_TRY(hipEventRecord(g.ev_a, g.stream));
-    if (fuse) {
-        if ((rc = timed_begin(g))) return rc;
-        if ((rc = prologue(g, 1, 1))) return rc;
-        if ((rc = timed_end(g, K_PRO))) return rc;
-    }
     if (use_graph && !g.profile) {
         for (int n = 0; n < nsteps; ++n) {
             const int p = g.par();
@@ -1091,9 +1081,9 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
                 // parity bookkeeping advances as if it had run
                 hipGraph_t gr;
                 HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-                int rc1 = fuse ? PG_OK : prologue(g, 1, 1);
+                int rc1 = prologue(g, 1, 1);
                 if (!rc1) rc1 = average_pass(g);
-                int rc2 = rc1 ? rc1 : passes(g, true, fuse);
+                int rc2 = rc1 ? rc1 : passes(g, true);
                 if (!rc2) rc2 = average_pass(g);
                 hipError_t ec = hipStreamEndCapture(g.stream, &gr);
                 if (rc2) return rc2;
@@ -1112,13 +1102,11 @@ int group_run(Group& g, int nsteps, double* t_io, double* dt_io, double* dts, in
         }
     } else {
         for (int n = 0; n < nsteps; ++n) {
-            if (!fuse) {
-                if ((rc = timed_begin(g))) return rc;
-                if ((rc = prologue(g, 1, 1))) return rc;
-                if ((rc = timed_end(g, K_PRO))) return rc;
-            }
+            if ((rc = timed_begin(g))) return rc;
+            if ((rc = prologue(g, 1, 1))) return rc;
+            if ((rc = timed_end(g, K_PRO))) return rc;
             if ((rc = average_pass(g))) return rc;
-            if ((rc = passes(g, true, fuse))) return rc;
+            if ((rc = passes(g, true))) return rc;
             if ((rc = average_pass(g))) return rc;
         }
     }
@@ -1362,7 +1350,10 @@ enum { AA_V0, AA_X, AA_P, AA_PRED, AA_CORR, AA_N };
 }  // namespace
 
 struct AdjWork {
-    int nsteps = 0, ckpt = 0, nseg = 0, slots = 0, prec = 4;
+    // slots: sps slots per set, nset sets.  A checkpointed sweep of several
+    // segments keeps two sets: the recompute of segment k-1 fills one on
+    // its own stream while the adjoint steps of segment k read the other.
+    int nsteps = 0, ckpt = 0, nseg = 0, slots = 0, sps = 0, nset = 1, prec = 4;
     size_t N = 0;
     std::vector<double> ts, dts;   // start time and size of every forward step
     void* ckpts = nullptr;         // nseg states (3N)
@@ -1371,7 +1362,7 @@ struct AdjWork {
     double* lparam = nullptr;      // per-workgroup partial sums of dL/dq, dL/dalpha
     int nslot = 0;                 // slots per parameter
     double* td = nullptr;          // (t, dt) of every step, on the device
-    double* td_seg = nullptr;      // (t, dt) of the segment being swept back
+    double* td_seg = nullptr;      // (t, dt) of the segment held in each set
     void* args = nullptr;          // slots x AA_N disk descriptors
     void* stage = nullptr;         // pinned host staging of one adjoint state (3N reals)
     // per-slot control block and orbital shift (the step's dt, planets, inc,
@@ -1382,10 +1373,16 @@ struct AdjWork {
     void* sres = nullptr;
     bool forward_done = false;
     size_t bytes = 0;
-    // captured backward sweeps of one segment (recompute + adjoint steps),
-    // by segment length and the buffer holding the incoming adjoint
+    // two sets: the recompute's stream (default priority, as every stream
+    // here) and the events that order the sets' reuse between the streams
+    hipStream_t rs = nullptr;
+    hipEvent_t ev_r[2] = {nullptr, nullptr};   // recompute into set k done
+    hipEvent_t ev_a[2] = {nullptr, nullptr};   // adjoint steps reading set k done
+    // captured graphs of one segment, by kind (0: recompute into a set,
+    // 1: adjoint steps reading it), set, length and, for the adjoint steps,
+    // the buffer holding the incoming adjoint
     struct SegGraph {
-        int len;
+        int kind, set, len;
         void* lam_in;
         void* lam_out;
         hipGraphExec_t exec;
@@ -1408,12 +1405,18 @@ int adj_slots(int nx, int ny) {
 
 void adj_free(AdjWork* w) {
     if (!w) return;
+    if (w->rs) (void)hipStreamSynchronize(w->rs);
     for (auto& sg : w->graphs)
         if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
     void* bufs[] = {w->ckpts, w->slot, w->lam, w->lparam, w->td, w->td_seg, w->args, w->sctl, w->sinc, w->sres};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (w->stage) (void)hipHostFree(w->stage);
+    for (int k = 0; k < 2; ++k) {
+        if (w->ev_r[k]) (void)hipEventDestroy(w->ev_r[k]);
+        if (w->ev_a[k]) (void)hipEventDestroy(w->ev_a[k]);
+    }
+    if (w->rs) (void)hipStreamDestroy(w->rs);
     delete w;
 }
 
@@ -1428,7 +1431,9 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
     w->nsteps = nsteps;
     w->ckpt = ckpt;
     w->nseg = ckpt > 0 ? (nsteps + ckpt - 1) / ckpt : 0;
-    w->slots = (ckpt > 0 ? std::min(ckpt, nsteps) : nsteps) + 1;
+    w->sps = (ckpt > 0 ? std::min(ckpt, nsteps) : nsteps) + 1;
+    w->nset = ckpt > 0 && w->nseg > 1 ? 2 : 1;
+    w->slots = w->sps * w->nset;
     w->prec = g.prec;
     w->N = d->N;
     w->nslot = adj_slots(d->nx, d->ny);
@@ -1443,7 +1448,7 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
         (rc = dev_alloc(w->bytes, &w->lam, (size_t)8 * N * es)) ||
         (rc = dev_alloc(w->bytes, &w->lparam, 2 * (size_t)adj_slots(d->nx, d->ny) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->td, 2 * (size_t)nsteps * sizeof(double))) ||
-        (rc = dev_alloc(w->bytes, &w->td_seg, 2 * (size_t)(ckpt > 0 ? ckpt : 1) * sizeof(double))) ||
+        (rc = dev_alloc(w->bytes, &w->td_seg, 2 * (size_t)w->nset * (ckpt > 0 ? ckpt : 1) * sizeof(double))) ||
         (rc = dev_alloc(w->bytes, &w->args, (size_t)w->slots * AA_N * sizeof(DiskArgs<real>))) ||
         (rc = dev_alloc(w->bytes, &w->sctl, (size_t)w->slots * sizeof(StepCtl))) ||
         (rc = dev_alloc(w->bytes, &w->sinc, (size_t)w->slots * d->nx * sizeof(int))) ||
@@ -1453,13 +1458,25 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
         w->stage = nullptr;
         return cleanup(fail(PG_E_NOMEM, "pinned staging of %zu bytes", 3 * N * es));
     }
+    if (w->nset == 2) {
+        if (hipStreamCreateWithFlags(&w->rs, hipStreamNonBlocking) != hipSuccess) {
+            w->rs = nullptr;
+            return cleanup(fail(PG_E_HIP, "hipStreamCreate failed"));
+        }
+        for (int k = 0; k < 2; ++k)
+            if (hipEventCreateWithFlags(&w->ev_r[k], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&w->ev_a[k], hipEventDisableTiming) != hipSuccess)
+                return cleanup(fail(PG_E_HIP, "hipEventCreate failed"));
+    }
     // descriptors of the recompute passes: the disk's own tables, with the
     // state, viscosity, control-block and shift pointers of the slot
     std::vector<DiskArgs<real>> a((size_t)w->slots * AA_N);
     real* base = (real*)w->slot;
     for (int m = 0; m < w->slots; ++m) {
         real* sl = base + (size_t)m * SLOT_WORDS * N;
-        real* nx = base + (size_t)std::min(m + 1, w->slots - 1) * SLOT_WORDS * N;
+        // the next slot of the same set (a set's last slot only receives)
+        const int first = m / w->sps * w->sps;
+        real* nx = base + (size_t)(first + std::min(m - first + 1, w->sps - 1)) * SLOT_WORDS * N;
         const DiskArgs<real> t = d->args<real>();
         // parity 0 throughout: gas[0] is a pass's input, gas[1] its output
         auto set = [&](int k, real* g0, real* g1, real* vis, real* vis2) -> DiskArgs<real>& {
@@ -1491,15 +1508,16 @@ int adj_alloc(pg_handle* h, int nsteps, int ckpt) {
     return PG_OK;
 }
 
-// prologue of step m of the segment being swept back, into slot m's
-// control block and shift: t, dt from the segment's device copy, then the
-// planets and the orbital shift as in the forward step
+// prologues of the len steps of the segment held in `set`, into its slots'
+// control blocks and shifts: t, dt from the set's device copy of the
+// segment's step times, then the planets and the orbital shift as in the
+// forward step; one workgroup per step (they depend on the times only)
 template <typename real>
-int set_step(Group& g, const AdjWork* w, int m) {
+int set_steps(Group& g, const AdjWork* w, int set, int len, hipStream_t s) {
     LaunchSet<real> L = g.ls<real>();
-    L.disks = (const DiskArgs<real>*)w->args + (size_t)m * AA_N + AA_X;
+    L.disks = (const DiskArgs<real>*)w->args + (size_t)set * w->sps * AA_N + AA_X;
     L.ndisk = 1;
-    hipError_t e = Launch<real>::prologue_replay(L, w->td_seg + 2 * (size_t)m, g.stream);
+    hipError_t e = Launch<real>::prologue_replay_many(L, AA_N, len, w->td_seg + 2 * (size_t)set * w->ckpt, s);
     if (e != hipSuccess) return fail(PG_E_HIP, "launch prologue failed: %s", hipGetErrorString(e));
     return PG_OK;
 }
@@ -1510,15 +1528,14 @@ int set_step(Group& g, const AdjWork* w, int m) {
         if (e_ != hipSuccess) return fail(PG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));         \
     } while (0)
 
-// Recompute step n into slot m (and its end state into slot m+1's S0).
+// Recompute a step into slot m (and its end state into the next slot's
+// S0); first: the segment's first step (its start state's viscous forces
+// are computed here too).  The slot's prologue has run (set_steps).
 template <typename real>
-int adj_recompute(pg_handle* h, int m, bool write_next) {
+int adj_recompute(pg_handle* h, int m, bool first, bool write_next, hipStream_t s) {
     Group& g = h->g;
     AdjWork* w = h->adj;
     const size_t N = w->N;
-    hipStream_t s = g.stream;
-    int rc = set_step<real>(g, w, m);
-    if (rc) return rc;
     LaunchSet<real> L = g.ls<real>();
     const DiskArgs<real>* a = (const DiskArgs<real>*)w->args + (size_t)m * AA_N;
     auto at = [&](int k) {
@@ -1535,13 +1552,13 @@ int adj_recompute(pg_handle* h, int m, bool write_next) {
     // segment's first state, then kick + x sweep, azimuthal sweep + shift +
     // residual advection, predictor + its viscous forces, corrector + the
     // viscous forces of the result (the next slot's S0 and V0)
-    if (visc && m == 0) HIPL(Launch<real>::viscosity(at(AA_V0), 0, 0, s));
+    if (visc && first) HIPL(Launch<real>::viscosity(at(AA_V0), 0, 0, s));
     HIPL(Launch<real>::sweep_x_gas(at(AA_X), 0, 1, s));
     HIPL(Launch<real>::ring_gas(at(AA_P), 0, 3, s));
     if (visc) HIPL(Launch<real>::predvis(at(AA_PRED), 0, s));
     if (write_next) {
         if (visc) {
-            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, 0, s));
+            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, s));
         } else {
             HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
             HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 0, s));
@@ -1550,16 +1567,16 @@ int adj_recompute(pg_handle* h, int m, bool write_next) {
     return PG_OK;
 }
 
-// Adjoint passes of step n held in slot m; lam[0] holds the adjoint of the
-// step's end state on entry and of its start state on exit.
+// Adjoint passes of the step held in slot m; lamA holds the adjoint of the
+// step's end state on entry, and after the swap of the two the adjoint of
+// its start state.
 template <typename real>
-int adj_step(pg_handle* h, int m, real*& lamA, real*& lamB) {
+int adj_step(pg_handle* h, int m, real*& lamA, real*& lamB, hipStream_t s) {
     Group& g = h->g;
     AdjWork* w = h->adj;
     DiskDev* d = g.disks[0];
     const size_t N = w->N;
     const int nx = d->nx, ny = d->ny;
-    hipStream_t s = g.stream;
     real* sl = (real*)w->slot + (size_t)m * SLOT_WORDS * N;
     real* lamV0 = (real*)w->lam + 6 * N;
     AdjArgs<real> A{};
@@ -1603,6 +1620,26 @@ int adj_step(pg_handle* h, int m, real*& lamA, real*& lamB) {
     }
     std::swap(lamA, lamB);
     return PG_OK;
+}
+
+// The recompute of the segment of len steps held in `set`: every step's
+// prologue, then the steps in order.
+template <typename real>
+int adj_recompute_segment(pg_handle* h, int set, int len, hipStream_t s) {
+    AdjWork* w = h->adj;
+    int rc = set_steps<real>(h->g, w, set, len, s);
+    const int m0 = set * w->sps;
+    for (int m = 0; m < len && !rc; ++m) rc = adj_recompute<real>(h, m0 + m, m == 0, m + 1 < len, s);
+    return rc;
+}
+
+// The adjoint steps of the segment held in `set`, last to first.
+template <typename real>
+int adj_steps_segment(pg_handle* h, int set, int len, real*& lamA, real*& lamB, hipStream_t s) {
+    const int m0 = set * h->adj->sps;
+    int rc = PG_OK;
+    for (int m = len - 1; m >= 0 && !rc; --m) rc = adj_step<real>(h, m0 + m, lamA, lamB, s);
+    return rc;
 }
 
 int adj_check_config(const pg_params& P) {
@@ -1659,7 +1696,7 @@ int adj_tape_forward(pg_handle* h, int nsteps, double t0, double dt0) {
         HIPL(Launch<real>::ring_gas(at(AA_P), 0, 3, s));
         if (visc) {
             HIPL(Launch<real>::predvis(at(AA_PRED), 0, s));
-            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, 0, s));
+            HIPL(Launch<real>::corrvis(at(AA_CORR), 0, 0, s));
         } else {
             HIPL(hipMemcpyAsync(nxt + SL_S0 * N, sl + SL_S3 * N, 3 * N * sizeof(real), hipMemcpyDeviceToDevice, s));
             HIPL(Launch<real>::sources(at(AA_CORR), 0, 2, 0, 1, s));
@@ -1693,6 +1730,7 @@ int adj_forward(pg_handle* h, int nsteps, int ckpt, double t0, double dt0, doubl
     AdjWork* w = h->adj;
     w->forward_done = false;
     // backward graphs hold the launch configuration of the previous run
+    if (w->rs) HIPL(hipStreamSynchronize(w->rs));
     for (auto& sg : w->graphs)
         if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
     w->graphs.clear();
@@ -1767,42 +1805,70 @@ int adj_backward(pg_handle* h, const double* dL, pg_grads* out) {
         // directly: the host issues a step's five launches well within the
         // step's device time)
         for (int m = w->nsteps - 1; m >= 0; --m)
-            if ((rc = adj_step<real>(h, m, lamA, lamB))) return rc;
+            if ((rc = adj_step<real>(h, m, lamA, lamB, g.stream))) return rc;
     }
-    for (int k = w->nseg - 1; k >= 0; --k) {
-        const int n0 = k * w->ckpt, len = std::min(w->ckpt, w->nsteps - n0);
-        HIPL(hipMemcpyAsync(w->slot, (char*)w->ckpts + (size_t)k * bytes, bytes, hipMemcpyDeviceToDevice, g.stream));
-        HIPL(hipMemcpyAsync(w->td_seg, w->td + 2 * (size_t)n0, 2 * (size_t)len * sizeof(double),
-                            hipMemcpyDeviceToDevice, g.stream));
-        // the segment's sweep (recompute, then the adjoint steps in reverse)
-        // as one captured graph, replayed for every segment of the same
-        // length that starts from the same adjoint buffer
+    // Checkpointed: per segment, last to first, the recompute into a set of
+    // slots, then the adjoint steps reading it.  With two sets the
+    // recompute of segment k-1 runs on the recompute stream while the
+    // adjoint steps of segment k run on the disk's stream: the adjoint
+    // steps of k wait for the recompute of k (ev_r), and the recompute of
+    // k-1 waits for the adjoint steps of k+1, the last reader of its set
+    // (ev_a).  Each is a captured graph, replayed for every segment of the
+    // same length (and, for the adjoint steps, the same incoming buffer).
+    hipStream_t rs = w->nset == 2 ? w->rs : g.stream;
+    auto launch = [&](int kind, int set, int len, real*& la, real*& lb, hipStream_t st) -> int {
         AdjWork::SegGraph* sg = nullptr;
         for (auto& x : w->graphs)
-            if (x.len == len && x.lam_in == (void*)lamA) sg = &x;
+            if (x.kind == kind && x.set == set && x.len == len && (kind == 0 || x.lam_in == (void*)la)) sg = &x;
         if (!sg) {
-            real* la = lamA;
-            real* lb = lamB;
-            HIPL(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-            rc = PG_OK;
-            for (int m = 0; m < len && !rc; ++m) rc = adj_recompute<real>(h, m, m + 1 < len);
-            for (int m = len - 1; m >= 0 && !rc; --m) rc = adj_step<real>(h, m, la, lb);
+            real* a = la;
+            real* b = lb;
+            HIPL(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int r = kind == 0 ? adj_recompute_segment<real>(h, set, len, st)
+                                    : adj_steps_segment<real>(h, set, len, a, b, st);
             hipGraph_t graph = nullptr;
-            const hipError_t ec = hipStreamEndCapture(g.stream, &graph);
-            if (rc) {
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (r) {
                 if (graph) (void)hipGraphDestroy(graph);
-                return rc;
+                return r;
             }
             if (ec != hipSuccess) return fail(PG_E_HIP, "adjoint capture: %s", hipGetErrorString(ec));
             hipGraphExec_t exec = nullptr;
             const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (ei != hipSuccess) return fail(PG_E_HIP, "adjoint graph: %s", hipGetErrorString(ei));
-            w->graphs.push_back({len, (void*)lamA, (void*)la, exec});
+            w->graphs.push_back({kind, set, len, (void*)la, (void*)a, exec});
             sg = &w->graphs.back();
         }
-        HIPL(hipGraphLaunch(sg->exec, g.stream));
-        if (sg->lam_out != (void*)lamA) std::swap(lamA, lamB);
+        HIPL(hipGraphLaunch(sg->exec, st));
+        if (kind == 1 && sg->lam_out != (void*)la) std::swap(la, lb);
+        return PG_OK;
+    };
+    auto recompute = [&](int k) -> int {
+        const int set = w->nset == 2 ? (k & 1) : 0;
+        const int n0 = k * w->ckpt, len = std::min(w->ckpt, w->nsteps - n0);
+        real* s0 = (real*)w->slot + (size_t)set * w->sps * SLOT_WORDS * N + SL_S0 * N;
+        HIPL(hipMemcpyAsync(s0, (char*)w->ckpts + (size_t)k * bytes, bytes, hipMemcpyDeviceToDevice, rs));
+        HIPL(hipMemcpyAsync(w->td_seg + 2 * (size_t)set * w->ckpt, w->td + 2 * (size_t)n0,
+                            2 * (size_t)len * sizeof(double), hipMemcpyDeviceToDevice, rs));
+        real* none_a = nullptr;
+        real* none_b = nullptr;
+        int r = launch(0, set, len, none_a, none_b, rs);
+        if (r) return r;
+        if (w->nset == 2) HIPL(hipEventRecord(w->ev_r[set], rs));
+        return PG_OK;
+    };
+    if (w->nseg > 0 && (rc = recompute(w->nseg - 1))) return rc;
+    for (int k = w->nseg - 1; k >= 0; --k) {
+        const int set = w->nset == 2 ? (k & 1) : 0;
+        const int len = std::min(w->ckpt, w->nsteps - k * w->ckpt);
+        if (k > 0) {
+            if (k + 1 < w->nseg) HIPL(hipStreamWaitEvent(rs, w->ev_a[(k - 1) & 1], 0));
+            if ((rc = recompute(k - 1))) return rc;
+        }
+        if (w->nset == 2) HIPL(hipStreamWaitEvent(g.stream, w->ev_r[set], 0));
+        if ((rc = launch(1, set, len, lamA, lamB, g.stream))) return rc;
+        if (w->nset == 2) HIPL(hipEventRecord(w->ev_a[set], g.stream));
     }
     HIPL(hipMemcpyAsync(st, lamA, 3 * N * sizeof(real), hipMemcpyDeviceToHost, g.stream));
     std::vector<double> slots(2 * (size_t)w->nslot);

@@ -81,8 +81,6 @@ struct StepCtl {
     double ave_w;        // sum of the averaging weights so far
     int err;             // device loop: 1 once a step found a CFL rate that is not finite and positive
     int err_step;        // the step at which it did
-    unsigned int arrive; // corrector blocks of this disk finished in the running launch (0 between launches)
-    int nstep_end;       // device loop: the steps of the run (the corrector's prologue stops there)
 };
 
 // Per-radius tables (FP64 on the host; cast to the storage type on upload).

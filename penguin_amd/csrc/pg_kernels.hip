@@ -248,12 +248,16 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
         real rdn = rd + Dr;
         const real rmin = fmax(r * real(1.0e-10), real(1.0e-10));
         real un = u, vn = vp;
-        if (rn < rmin) {   // update_Cell's floor (sweeps.cu:238-246): old velocities kept
+        if (dt == real(0)) {
+            rdn = rd;   // a step of size 0 (the loop past end_time) leaves the cell as it is
+        } else if (rn < rmin) {   // update_Cell's floor (sweeps.cu:238-246): old velocities kept
             rdn = floor_dev(rmin, rb);
         } else {
             const real irn = FM<real>::rcp(rn);
-            un = (r * u + Du) * irn;
-            vn = (r * vp + (DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr)) * irn;
+            // (r u + Du) / (r + Dr) as u + (Du - u Dr) / (r + Dr): the
+            // rounding falls on the change, not on the velocity itself
+            un = u + (Du - u * Dr) * irn;
+            vn = vp + ((DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr) - vp * Dr) * irn;
         }
         out[row + i] = rdn;
         out[N + row + i] = un;
@@ -356,12 +360,16 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
         real rn = r + Dr;
         const real rmin = fmax(r * real(1.0e-10), rfloor);
         real un = u, vn = vp;
-        if (rn < rmin) {
+        if (dt == real(0)) {
+            rn = r;   // a step of size 0 leaves the cell as it is
+        } else if (rn < rmin) {
             rn = rmin;
         } else {
             const real irn = FM<real>::rcp(rn);
-            un = (r * u + Du) * irn;
-            vn = (r * vp + (DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr)) * irn;
+            // (r u + Du) / (r + Dr) as u + (Du - u Dr) / (r + Dr): the
+            // rounding falls on the change, not on the velocity itself
+            un = u + (Du - u * Dr) * irn;
+            vn = vp + ((DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr) - vp * Dr) * irn;
         }
         out[row + i] = rn;
         out[N + row + i] = un;
@@ -545,15 +553,15 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
         real rdn = rd + Dr;                     // and what is stored
         const real rmin = fmax(r * real(1.0e-10), DUST ? rfloor : real(1.0e-10));
         real vn = vp, un = u;
-        if (!sweep_on) {
+        if (!sweep_on || dt == real(0)) {   // no sweep, or a step of size 0
             rdn = rd;
         } else if (rn < rmin) {
             rdn = floor_dev(rmin, rbg);
         } else {
             // gas: (rho v + Du + (rot + R Omega_f) Dr) / rho' - vbg == (rho vp + Du) / rho'
             const real irn = FM<real>::rcp(rn);
-            vn = (r * vp + Du) * irn;
-            un = (r * u + Dw) * irn;
+            vn = vp + (Du - vp * Dr) * irn;   // (r vp + Du) / rn, in increment form
+            un = u + (Dw - u * Dr) * irn;
         }
         c0[e] = rdn; c1[e] = vn; c2[e] = un;   // each thread rewrites only its own cells
     }
@@ -618,12 +626,14 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
         real vn = c1[e], un = c2[e];
         // update_Dust floors (advection.cu:315); update_Cell_advect has none
         // (advection.cu:88-105)
-        if (DUST && rn < fmax(r * real(1.0e-10), rfloor)) {
+        if (dt == real(0)) {
+            rn = rd;   // a step of size 0 moves nothing
+        } else if (DUST && rn < fmax(r * real(1.0e-10), rfloor)) {
             rn = fmax(r * real(1.0e-10), rfloor);
         } else {
             const real irn = FM<real>::rcp(r + Dr);
-            vn = (r * c1[e] + Dv) * irn;
-            un = (r * c2[e] + Du) * irn;
+            vn = c1[e] + (Dv - c1[e] * Dr) * irn;   // (r v + Dv) / (r + Dr), in increment form
+            un = c2[e] + (Du - c2[e] * Dr) * irn;
         }
         const size_t o = (size_t)(y0 + y) * nx + i;
         out[o] = rn;
@@ -769,29 +779,17 @@ __device__ __forceinline__ void block_max_store(double* part, real v) {
     }
 }
 
-// A partial written by another block of the running kernel: loaded at
-// device scope, past this compute unit's caches.
-template <bool COHERENT>
-__device__ __forceinline__ double load_partial(const double* p) {
-    if (!COHERENT) return *p;
-    const unsigned long long b =
-        __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)b);
-}
-
 // Max over the partial array by one block; result in thread 0.  Eight
 // independent loads per iteration keep the memory pipeline full.
-// COHERENT: the partials were written by other blocks of the same launch.
-template <bool COHERENT = false>
 __device__ __forceinline__ double reduce_partials(const double* part, int n) {
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int stride = blockDim.x * 8;
     int k0 = threadIdx.x;
     for (; k0 + 7 * (int)blockDim.x < n; k0 += stride) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = fmax(a[u], load_partial<COHERENT>(part + k0 + u * blockDim.x));
+        for (int u = 0; u < 8; ++u) a[u] = fmax(a[u], part[k0 + u * blockDim.x]);
     }
-    for (int k = k0; k < n; k += blockDim.x) a[0] = fmax(a[0], load_partial<COHERENT>(part + k));
+    for (int k = k0; k < n; k += blockDim.x) a[0] = fmax(a[0], part[k]);
     double x = fmax(fmax(fmax(a[0], a[1]), fmax(a[2], a[3])), fmax(fmax(a[4], a[5]), fmax(a[6], a[7])));
     for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
     __shared__ double wmax[16];
@@ -976,17 +974,8 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
 // result goes to the other buffer of the pair, because neighbouring tiles
 // read this tile's cells as their halo input.  (The dust is corrected in
 // place: its halo results are not used.)
-template <typename real, bool FUSED>
-__device__ __forceinline__ void prologue_disk(const DiskArgs<real>& d, int mode, int do_orbadv, const double* td);
-
-// fuse != 0: the next step's prologue runs in the last block of each disk to
-// finish (counted on the disk's control block), in place of a launch of its
-// own: every block stores its rate partial, makes its writes visible at
-// device scope and counts itself; the block that counts last has seen every
-// other block's count, so after an acquiring fence it reduces their
-// partials, and it sets the count back to 0 for the next launch.
 template <typename real, int TH>
-__global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, int do_kill, int fuse) {
+__global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, int do_kill) {
     __shared__ real Sr[ST_X * (TH + 2)], Su[ST_X * (TH + 2)], Sv[ST_X * (TH + 2)];
     const int2 w = ls.wl_pt[blockIdx.x];
     if (w.y < 0) return;
@@ -1048,21 +1037,6 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
     block_max_store(d.rate_part + w.y, rate);   // includes the barrier the tile needs
     __syncthreads();
     vis_from_tile<real, TH>(d, Sr, Su, Sv, i0, j0, gmem(d.vis));
-    if (fuse) {
-        __shared__ int s_last;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            __threadfence();   // release: this block's partial before its count
-            const unsigned prev = atomicAdd(&d.ctl->arrive, 1u);
-            s_last = prev + 1u == (unsigned)d.rate_nblk;
-        }
-        __syncthreads();
-        if (s_last) {
-            __threadfence();   // acquire: the other blocks' partials after their counts
-            prologue_disk<real, true>(d, 1, 1, nullptr);
-            if (threadIdx.x == 0) d.ctl->arrive = 0u;
-        }
-    }
 }
 
 // CFL rate of the current state (get_dt_lv1, timestep.cu:18-97).
@@ -1198,18 +1172,12 @@ __global__ __launch_bounds__(1024) void k_rate_reduce(LaunchSet<real> ls) {
 // rate reduction, and the two planets (lanes 0 and 1 of wave 0, one
 // evaluation for both) need no barrier before the shifts, which do not read
 // them.
-// The prologue of one disk, by the calling block (any size).  FUSED: the
-// call from the corrector's last block, which does nothing once the run's
-// last step has been taken (ctl->nstep == ctl->nstep_end), so that the
-// control block ends a run as the separate prologue leaves it; the rate
-// partials it reduces were written by the other blocks of the same launch
-// and are read through the device-coherent path.
-template <typename real, bool FUSED>
+// The prologue of one disk, by the calling block (any size).
+template <typename real>
 __device__ __forceinline__ void prologue_disk(const DiskArgs<real>& d, int mode, int do_orbadv, const double* td) {
     double* dts = d.dts;
     StepCtl* ctl = d.ctl;
     __shared__ double s_dt, s_t;
-    __shared__ int s_skip;
     const int tid = threadIdx.x;
     const bool oa = d.orbadv && do_orbadv;
     double rt0 = 0.0, ry0 = 0.0;
@@ -1227,14 +1195,9 @@ __device__ __forceinline__ void prologue_disk(const DiskArgs<real>& d, int mode,
         tend = ctl->end_time;
         cfl = d.CFL;
         err0 = ctl->err;
-        if (FUSED) s_skip = n0 >= ctl->nstep_end;
-    }
-    if (FUSED) {
-        __syncthreads();
-        if (s_skip) return;
     }
     if (tid < 2) b = tid == 0 ? ctl->planet0 : ctl->planet1;
-    double rate = mode == 1 ? reduce_partials<FUSED>(d.rate_part, d.rate_nblk) : 0.0;
+    double rate = mode == 1 ? reduce_partials(d.rate_part, d.rate_nblk) : 0.0;
     if (tid == 0) {
         double t = t0, dt = dt0;
         if (mode == 2) {
@@ -1295,7 +1258,15 @@ __device__ __forceinline__ void prologue_disk(const DiskArgs<real>& d, int mode,
 
 template <typename real>
 __global__ __launch_bounds__(1024) void k_prologue(LaunchSet<real> ls, int mode, int do_orbadv, const double* td) {
-    prologue_disk<real, false>(ls.disks[blockIdx.x], mode, do_orbadv, td);
+    prologue_disk<real>(ls.disks[blockIdx.x], mode, do_orbadv, td);
+}
+
+// Replays of n recorded steps at once: block b sets the control block and
+// orbital shift of descriptor b * stride from t = td[2b], dt = td[2b + 1]
+// (the replays depend on the recorded times only, not on any state).
+template <typename real>
+__global__ __launch_bounds__(1024) void k_prologue_replay(LaunchSet<real> ls, int stride, const double* td) {
+    prologue_disk<real>(ls.disks[(size_t)blockIdx.x * stride], 2, 1, td + 2 * (size_t)blockIdx.x);
 }
 
 // ------------------------------------------------------------- launchers
@@ -1396,8 +1367,8 @@ struct Launch {
         PT_LAUNCH(k_predvis, ls, s, par);
         return hipGetLastError();
     }
-    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, int fuse, hipStream_t s) {
-        PT_LAUNCH(k_corrvis, ls, s, par, kill, fuse);
+    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s) {
+        PT_LAUNCH(k_corrvis, ls, s, par, kill);
         return hipGetLastError();
     }
     static hipError_t average(const LaunchSet<real>& ls, int par, hipStream_t s) {
@@ -1422,6 +1393,11 @@ struct Launch {
     }
     static hipError_t prologue_replay(const LaunchSet<real>& ls, const double* td, hipStream_t s) {
         hipLaunchKernelGGL(k_prologue<real>, dim3(1), dim3(1024), 0, s, ls, 2, 1, td);
+        return hipGetLastError();
+    }
+    static hipError_t prologue_replay_many(const LaunchSet<real>& ls, int stride, int n, const double* td,
+                                           hipStream_t s) {
+        hipLaunchKernelGGL(k_prologue_replay<real>, dim3(n), dim3(1024), 0, s, ls, stride, td);
         return hipGetLastError();
     }
 };

@@ -27,8 +27,7 @@ struct Launch {
     // fused: predictor + viscosity of T (d.vis -> d.vis2); corrector + kill +
     // CFL rate + viscosity of the corrected state (d.vis2 -> d.vis)
     static hipError_t predvis(const LaunchSet<real>& ls, int par, hipStream_t s);
-    // fuse != 0: the next step's prologue (mode 1) in each disk's last block
-    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, int fuse, hipStream_t s);
+    static hipError_t corrvis(const LaunchSet<real>& ls, int par, int kill, hipStream_t s);
     static hipError_t average(const LaunchSet<real>& ls, int par, hipStream_t s);
     static hipError_t torque(const LaunchSet<real>& ls, int par, double* part, hipStream_t s);
     static hipError_t cfl(const LaunchSet<real>& ls, int par, hipStream_t s);
@@ -36,6 +35,9 @@ struct Launch {
     static hipError_t prologue(const LaunchSet<real>& ls, int mode, int do_orbadv, hipStream_t s);
     // replay of a recorded step: t = td[0], dt = td[1] (one disk)
     static hipError_t prologue_replay(const LaunchSet<real>& ls, const double* td, hipStream_t s);
+    // n replays: descriptor b * stride of ls.disks from t = td[2b], dt = td[2b+1]
+    static hipError_t prologue_replay_many(const LaunchSet<real>& ls, int stride, int n, const double* td,
+                                           hipStream_t s);
 };
 
 // Adjoint passes (pg_adjoint.hip), one disk, grid from nx, ny.

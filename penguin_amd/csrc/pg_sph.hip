@@ -322,10 +322,12 @@ __device__ __forceinline__ void update_cell(real* c, const real* D, real gam) {
         c[1] = pmin;
         return;   // velocities kept
     }
+    // (rho u + D) / (rho + D0) as u + (D - u D0) / (rho + D0): the rounding
+    // falls on the change, not on the velocity itself
     const real ir = FM<real>::rcp(rn);
-    c[2] = (c[2] * c[0] + D[2]) * ir;
-    c[3] = (c[3] * c[0] + D[3]) * ir;
-    c[4] = (c[4] * c[0] + D[4]) * ir;
+    c[2] += (D[2] - c[2] * D[0]) * ir;
+    c[3] += (D[3] - c[3] * D[0]) * ir;
+    c[4] += (D[4] - c[4] * D[0]) * ir;
     c[1] = fmax((E + D[1]) * (gam - real(1)), pmin);
     c[0] = rn;
 }
@@ -681,9 +683,9 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
         const size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k0);
         a.out[c] = rn;
         a.out[N + c] = q[RS] + D[1];
-        a.out[3 * N + c] = (q[2 * RS] * rho + D[2]) * ir;
-        a.out[4 * N + c] = (q[3 * RS] * rho + D[3]) * ir;
-        a.out[2 * N + c] = (q[4 * RS] * rho + D[4]) * ir;
+        a.out[3 * N + c] = q[2 * RS] + (D[2] - q[2 * RS] * D[0]) * ir;   // (rho u + D) / rn, in increment form
+        a.out[4 * N + c] = q[3 * RS] + (D[3] - q[3 * RS] * D[0]) * ir;
+        a.out[2 * N + c] = q[4 * RS] + (D[4] - q[4 * RS] * D[0]) * ir;
     }
 }
 

@@ -22,8 +22,8 @@ pytestmark = pytest.mark.gpu
 T_PLANET = 2.3   # planet mass well into its ramp
 
 
-def _params(**kw):
-    return config_c5(precision=8, xres=64, yres=96).replace(noise_rho=0.0, p_alpha=0.5, **kw)
+def _params(size=(64, 96), **kw):
+    return config_c5(precision=8, xres=size[0], yres=size[1]).replace(noise_rho=0.0, p_alpha=0.5, **kw)
 
 
 def _grid(p):
@@ -86,9 +86,12 @@ def _perturbed(g, dirn, e):
     return h
 
 
+# 520 radii: the x adjoint's rows span three 244-cell tiles (the last one
+# partial) and the azimuthal adjoint's 8-radius strips 65 tiles
+@pytest.mark.parametrize("size", [(64, 96), (520, 48)])
 @pytest.mark.parametrize("name", ["sweep_x", "sweep_y", "orbital_advection", "viscosity", "kick", "corrector"])
-def test_pass_adjoint_matches_finite_differences(name):
-    p = _params()
+def test_pass_adjoint_matches_finite_differences(name, size):
+    p = _params(size)
     g = _state(p)
     dirn = _direction(p, g)
     d = Disk(p)
@@ -127,4 +130,6 @@ def test_pass_adjoint_matches_finite_differences(name):
               np.sum(lam * _apply(p.replace(ss_alpha=a - ea), g, name, dt))) / (2 * ea)
         rela = abs(dpar[1] - fa) / max(abs(fa), 1e-300)
         print(f"{name}: dalpha adjoint {dpar[1]:.10e} fd {fa:.10e} rel {rela:.2e}", flush=True)
-        assert rela <= 1e-6
+        # the central difference's own error in alpha (its step is 1e-4 alpha)
+        # reaches ~1e-6 on the wide disk
+        assert rela <= (1e-6 if size == (64, 96) else 3e-6)
