@@ -510,7 +510,10 @@ PG_HD void hllc_cd(real c, real ic, real c2, real rb, real rl, real rld, real ul
     real du = ur - ul;
     real cbar = FM<real>::sqrt(c2 + n2 * du * du);
     real sl = ubar - cbar, sr = ubar + cbar;
-    real sm = FM<real>::div(rl * (ul + c) + rr * (ur - c), rl + rr);
+    // sm = (rl (ul + c) + rr (ur - c)) / (rl + rr) with c (rl - rr) formed
+    // from the deviations: the terms of order c rho cancel to order u rho,
+    // and in FP32 their rounding would be ~1e-3 of the contact speed
+    real sm = FM<real>::div(rl * ul + rr * ur + c * (rld - rrd), rl + rr);
     hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, presd);
 }
 

@@ -51,14 +51,20 @@ __device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x
 // dust does not act back on the gas).  mode 0: the first half kick with the
 // planet at t; 1/2: predictor / corrector with the planet at t+dt, the
 // velocity-dependent forces evaluated at the state predicted over h.
-// h = dt/2.  Gas inputs rho, cu, cvp (vp = deviation from vbg), viscous
-// forces Vr, Vp; dust Qu, Qvp.  GAS / DUST select the outputs computed.
+// h64 = dt/2 (FP64).  Gas inputs rho, cu, cvp (vp = deviation from vbg),
+// viscous forces Vr, Vp; dust Qu, Qvp.  GAS / DUST select the outputs
+// computed.  The gas velocity's share of the background force (centrifugal
+// minus gravity at vbg) is added in FP64 from an FP64 table: each step the
+// x sweep's background pressure force takes it back (the two balance in
+// the unperturbed disk), and in FP32 their roundings, the same every step,
+// would add up in the radial velocity, which is ~1e-3 of either.
 // Used by k_sources and by the x sweeps, which apply the first half kick to
 // the cells they load.
 template <typename real, bool GAS, bool DUST>
-__device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, int ip, int j, real h, real rho,
+__device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, int ip, int j, double h64, real rho,
                                              real cu, real cvp, real Vr, real Vp, real Qu, real Qvp, real& nu,
                                              real& nvp, real& ndu, real& ndvp) {
+    const real h = (real)h64;
     const real mdt = mode == 0 ? real(0) : h;
     const PlanetK& pk = mode == 0 ? d.ctl->pk0 : d.ctl->pk1;
     const real R = TAB(T_XC, ip);
@@ -91,7 +97,9 @@ __device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, 
         }
     }
     if (GAS) {
-        nu = cu + axg(Tvp) * h;
+        // cu + axg(Tvp) h with the background part in FP64
+        const double ebal64 = d.rot[3 * d.nx + ip - NPAD];
+        nu = (real)((double)cu + ((double)(axg(Tvp) - ebal) + ebal64) * h64);
         nvp = cvp + ay * h;
     }
     if (DUST) {
@@ -165,7 +173,7 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
             if (kick) {
                 real Vr = real(0), Vp = real(0), nu, nvp, unused0, unused1;
                 if (d.visc) { Vr = gmem(d.vis)[row + i]; Vp = gmem(d.vis)[N + row + i]; }
-                cell_sources<real, true, false>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), TAB(T_G_R, ip) + rd, u, vp,
+                cell_sources<real, true, false>(d, 0, ip, j, 0.5 * d.ctl->dt, TAB(T_G_R, ip) + rd, u, vp,
                                                 Vr, Vp, real(0), real(0), nu, nvp, unused0, unused1);
                 u = nu;
                 vp = nvp;
@@ -237,9 +245,10 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
         const real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
         const real f = dt * TAB(T_IXV, ip);
         const real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
-        // net_source (riemann.cu:59-62): the deviations' part plus the
-        // background's, T_PSRC = (g0 + g1)/2 (pref[ip] - pref[ip+1])
-        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1]) + TAB(T_PSRC, ip)) * f;
+        // net_source (riemann.cu:59-62): the deviations' part here, the
+        // background's (T_PSRC = (g0 + g1)/2 (pref[ip] - pref[ip+1]), times
+        // dt/volume) below in FP64: it balances the kick's background force
+        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1])) * f;
         const real DL = (s_FL[s] - s_FL[s + 1]) * f;
         const real rb = TAB(T_G_R, ip);
         const real rd = s_r[s], u = s_u[s], vp = s_vp[s];
@@ -258,6 +267,11 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
             // rounding falls on the change, not on the velocity itself
             un = u + (Du - u * Dr) * irn;
             vn = vp + ((DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr) - vp * Dr) * irn;
+            // the background pressure force / (r + Dr), in FP64 (the FP32
+            // reciprocal refined by one Newton step)
+            double ir64 = (double)irn;
+            if (sizeof(real) == 4) ir64 = ir64 * (2.0 - (double)rn * ir64);
+            un = (real)((double)un + d.rot[4 * nx + i] * d.ctl->dt * ir64);
         }
         out[row + i] = rdn;
         out[N + row + i] = un;
@@ -308,7 +322,7 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
                 const PG_GLOBAL real* __restrict__ g0 = gmem(d.gas[gpre]);
                 const real rho = TAB(T_G_R, ip) + g0[row + i], cu = g0[N + row + i], cvp = g0[2 * N + row + i];
                 real ndu, ndvp, unused0, unused1;
-                cell_sources<real, false, true>(d, 0, ip, j, (real)(0.5 * d.ctl->dt), rho, cu, cvp, real(0), real(0),
+                cell_sources<real, false, true>(d, 0, ip, j, 0.5 * d.ctl->dt, rho, cu, cvp, real(0), real(0),
                                                 u, vp, unused0, unused1, ndu, ndvp);
                 u = ndu;
                 vp = ndvp;
@@ -849,9 +863,9 @@ __global__ __launch_bounds__(256) void k_sources(LaunchSet<real> ls, int par, in
         real nu_ = cu, nvp = cvp, ndu = Qu, ndvp = Qvp;
         if (mode != 3) {
             if (d.has_dust && mode != 1)
-                cell_sources<real, true, true>(d, mode, ip, j, h, rho, cu, cvp, Vr, Vp, Qu, Qvp, nu_, nvp, ndu, ndvp);
+                cell_sources<real, true, true>(d, mode, ip, j, 0.5 * d.ctl->dt, rho, cu, cvp, Vr, Vp, Qu, Qvp, nu_, nvp, ndu, ndvp);
             else
-                cell_sources<real, true, false>(d, mode, ip, j, h, rho, cu, cvp, Vr, Vp, Qu, Qvp, nu_, nvp, ndu, ndvp);
+                cell_sources<real, true, false>(d, mode, ip, j, 0.5 * d.ctl->dt, rho, cu, cvp, Vr, Vp, Qu, Qvp, nu_, nvp, ndu, ndvp);
         }
         real nr_ = rd;   // stored form: the deviation from rho_b
         if (mode == 1) {
@@ -944,7 +958,7 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
             const size_t o = (size_t)j * nx + i;
             const real rho = TAB(T_G_R, i + NPAD) + gas[o], cu = gas[N + o], cvp = gas[2 * N + o];
             real nu_, nvp, unused0, unused1;
-            cell_sources<real, true, false>(d, 1, i + NPAD, j, h, rho, cu, cvp, vis[o], vis[N + o], real(0), real(0),
+            cell_sources<real, true, false>(d, 1, i + NPAD, j, 0.5 * d.ctl->dt, rho, cu, cvp, vis[o], vis[N + o], real(0), real(0),
                                             nu_, nvp, unused0, unused1);
             r = rho; u = nu_; v = nvp;
         }
@@ -1009,10 +1023,10 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
             if (d.has_dust) { Qu = dust[N + o]; Qvp = dust[2 * N + o]; }
             real nu_ = cu, nvp = cvp, ndu = Qu, ndvp = Qvp;
             if (d.has_dust)
-                cell_sources<real, true, true>(d, 2, ip, j, h, rho, cu, cvp, vis2[o], vis2[N + o], Qu, Qvp, nu_, nvp,
+                cell_sources<real, true, true>(d, 2, ip, j, 0.5 * d.ctl->dt, rho, cu, cvp, vis2[o], vis2[N + o], Qu, Qvp, nu_, nvp,
                                                ndu, ndvp);
             else
-                cell_sources<real, true, false>(d, 2, ip, j, h, rho, cu, cvp, vis2[o], vis2[N + o], Qu, Qvp, nu_, nvp,
+                cell_sources<real, true, false>(d, 2, ip, j, 0.5 * d.ctl->dt, rho, cu, cvp, vis2[o], vis2[N + o], Qu, Qvp, nu_, nvp,
                                                 ndu, ndvp);
             real nrd = rd;   // the deviation from rho_b, which wave killing relaxes to 0
             if (do_kill) {
