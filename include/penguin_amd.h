@@ -136,6 +136,14 @@ int pg_write_average_snapshot(pg_handle* h, const char* filename, double t);
    with cal_torque and output_gy, output.cu:336-457, output/forces.cu:41-58,
    106-127), FP64.  main.cu:379-388 prints it divided by q^2/h^2 (2D). */
 int pg_torque(pg_handle* h, double* torque);
+/* pg_torque for callers that cannot allocate memory (serving loops,
+   harnesses that forbid allocation): no counterpart in the reference.  The
+   per-tile partial sums live in a fixed device array of the library (16384
+   tiles, one per 64 x 16 cells of a large disk, 64 x 4 of a small one; a
+   larger disk is refused with PG_E_INVALID), one copy per GPU, shared by
+   every call on that GPU: calls that overlap on one GPU race.  Returns the
+   same bits as pg_torque. */
+int pg_torque_noalloc(pg_handle* h, double* torque);
 
 /* ---- introspection ----------------------------------------------------- */
 /* HIP stream the handle launches on (as a void*), for external capture. */

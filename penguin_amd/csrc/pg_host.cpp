@@ -23,6 +23,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -2317,6 +2318,53 @@ int average_get(pg_handle* h, double* gas) {
 
 }  // namespace
 
+// ------------------------------------------------- allocation-free torque
+// pg_torque_noalloc keeps the torque's per-tile partial sums, and their sum
+// in the last slot, in this file-scope device array instead of a buffer the
+// library allocates.  Each GPU holds its own copy until the process ends;
+// every call on one GPU shares it (calls that overlap on a GPU race), and a
+// call overwrites every slot it reads, so nothing needs clearing.
+constexpr int NOALLOC_TILES = 16384;
+static __device__ __align__(256) double scratch[NOALLOC_TILES + 1];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[NOALLOC_MAX_DEVICES] = {};   // the array's address on each device
+
+// The device address of `scratch` on the current device, found once per
+// device (hipGetSymbolAddress with that device current).  The array's name
+// in host code is not a device address and is never passed to a kernel.
+int scratch_on_device(double** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(PG_E_INVALID, "device %d: the allocation-free entry points serve devices 0..%d", dev,
+                    NOALLOC_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PG_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch[dev] = (double*)ptr;
+    }
+    *out = g_scratch[dev];
+    return PG_OK;
+}
+
+// The partials summed by one thread in index order, as pg_torque sums them
+// on the host, so the two entry points return the same bits.
+__global__ void k_sum_in_order(const double* part, int n, double* sum) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += part[k];
+    *sum = s;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pg_write_snapshot(pg_handle* h, const char* filename, double t) {
@@ -2398,6 +2446,33 @@ int pg_torque(pg_handle* h, double* torque) {
     double s = 0.0;
     for (double x : part) s += x;
     *torque = s;
+    return PG_OK;
+}
+
+// pg_torque with the partial sums in the device array `scratch` (at most
+// NOALLOC_TILES tiles of the current schedule): no device or pinned memory
+// is allocated; the result comes back through the caller's pointer.
+int pg_torque_noalloc(pg_handle* h, double* torque) {
+    CHECK_H(h);
+    CHECK_2D(h);
+    if (!torque) return fail(PG_E_INVALID, "null argument");
+    Group& g = h->g;
+    DiskDev* d = g.disks[0];
+    if (d->P.n_planet < 1) return fail(PG_E_STATE, "the disk has no planet");
+    const int n = d->rate_nblk;   // tiles of the current schedule: one partial each
+    if (n > NOALLOC_TILES)
+        return fail(PG_E_INVALID, "pg_torque_noalloc: the disk has %d tiles of %d x %d cells; the allocation-free "
+                    "scratch holds %d (use pg_torque)", n, PT_X, g.pt_rows, NOALLOC_TILES);
+    double* ptr = nullptr;
+    int rc = scratch_on_device(&ptr);
+    if (rc) return rc;
+    hipError_t e = g.prec == 8 ? Launch<double>::torque(g.ls<double>(), g.par(), ptr, g.stream)
+                               : Launch<float>::torque(g.ls<float>(), g.par(), ptr, g.stream);
+    if (e != hipSuccess) return fail(PG_E_HIP, "launch torque failed: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(k_sum_in_order, dim3(1), dim3(64), 0, g.stream, (const double*)ptr, n, ptr + NOALLOC_TILES);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(torque, ptr + NOALLOC_TILES, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
     return PG_OK;
 }
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "pg_average_get": (ctypes.c_int, [_vp, _dp]),
     "pg_write_average_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
     "pg_torque": (ctypes.c_int, [_vp, _dp]),
+    "pg_torque_noalloc": (ctypes.c_int, [_vp, _dp]),
     "pg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pg_batch_get_schedule": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "pg_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
