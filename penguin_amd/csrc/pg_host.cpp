@@ -532,19 +532,15 @@ void build_tables_and_state(DiskDev* h, std::vector<double>& gas, std::vector<do
             T(T_KILL, ip) = k;
         }
     }
-    // orbital advection background rotation (orbit.cu:51-72), R, the
-    // background density and the two background forces that balance in the
-    // unperturbed disk (the kick's centrifugal minus gravity, the x sweep's
-    // pressure force per unit time), FP64
-    rot.assign(5 * (size_t)nx, 0.0);
+    // orbital advection background rotation (orbit.cu:51-72), R and the
+    // background density, FP64
+    rot.assign(3 * (size_t)nx, 0.0);
     for (int i = 0; i < nx; ++i) {
         const int ip = i + NPAD;
         double xc = T(T_XC, ip);
         rot[i] = P.OrbAdv_flag ? pr.v(xc) - xc * P.frame_omega : 0.0;
         rot[nx + i] = xc;
         rot[2 * nx + i] = T(T_G_R, ip);
-        rot[3 * nx + i] = T(T_EBAL, ip);
-        rot[4 * nx + i] = T(T_PSRC, ip) * T(T_IXV, ip);
     }
     h->rho_bg.assign(rot.begin() + 2 * nx, rot.begin() + 3 * nx);
     // initial state (fill_grid, init.cu:502-519) + seeded perturbation

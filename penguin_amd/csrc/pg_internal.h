@@ -134,10 +134,9 @@ struct DiskArgs {
     real gam, sc_h, Stokes, D_G_ratio, frame_omega;
     double CFL;
     double dy_d;         // dy in FP64 for the orbital shift
-    const double* rot;   // 5*nx FP64: [0,nx) vbg(R) - R*frame_omega
+    const double* rot;   // 3*nx FP64: [0,nx) vbg(R) - R*frame_omega
                          // (orbit.cu:65), [nx,2nx) R of the interior radii,
-                         // [2nx,3nx) the background density rho_b(R),
-                         // [3nx,4nx) T_EBAL, [4nx,5nx) T_PSRC * T_IXV
+                         // [2nx,3nx) the background density rho_b(R)
     int* inc;            // nx, integer azimuthal shift of the step
     real* res;           // nx, residual advection speed of the step
     StepCtl* ctl;

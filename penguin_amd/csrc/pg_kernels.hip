@@ -51,13 +51,8 @@ __device__ __forceinline__ double one_minus_exp_neg(double x) { return -expm1(-x
 // dust does not act back on the gas).  mode 0: the first half kick with the
 // planet at t; 1/2: predictor / corrector with the planet at t+dt, the
 // velocity-dependent forces evaluated at the state predicted over h.
-// h64 = dt/2 (FP64).  Gas inputs rho, cu, cvp (vp = deviation from vbg),
-// viscous forces Vr, Vp; dust Qu, Qvp.  GAS / DUST select the outputs
-// computed.  The gas velocity's share of the background force (centrifugal
-// minus gravity at vbg) is added in FP64 from an FP64 table: each step the
-// x sweep's background pressure force takes it back (the two balance in
-// the unperturbed disk), and in FP32 their roundings, the same every step,
-// would add up in the radial velocity, which is ~1e-3 of either.
+// h64 = dt/2.  Gas inputs rho, cu, cvp (vp = deviation from vbg), viscous
+// forces Vr, Vp; dust Qu, Qvp.  GAS / DUST select the outputs computed.
 // Used by k_sources and by the x sweeps, which apply the first half kick to
 // the cells they load.
 template <typename real, bool GAS, bool DUST>
@@ -97,9 +92,7 @@ __device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, 
         }
     }
     if (GAS) {
-        // cu + axg(Tvp) h with the background part in FP64
-        const double ebal64 = d.rot[3 * d.nx + ip - NPAD];
-        nu = (real)((double)cu + ((double)(axg(Tvp) - ebal) + ebal64) * h64);
+        nu = cu + axg(Tvp) * h;
         nvp = cvp + ay * h;
     }
     if (DUST) {
@@ -245,10 +238,9 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
         const real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
         const real f = dt * TAB(T_IXV, ip);
         const real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
-        // net_source (riemann.cu:59-62): the deviations' part here, the
-        // background's (T_PSRC = (g0 + g1)/2 (pref[ip] - pref[ip+1]), times
-        // dt/volume) below in FP64: it balances the kick's background force
-        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1])) * f;
+        // net_source (riemann.cu:59-62): the deviations' part plus the
+        // background's, T_PSRC = (g0 + g1)/2 (pref[ip] - pref[ip+1])
+        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1]) + TAB(T_PSRC, ip)) * f;
         const real DL = (s_FL[s] - s_FL[s + 1]) * f;
         const real rb = TAB(T_G_R, ip);
         const real rd = s_r[s], u = s_u[s], vp = s_vp[s];
@@ -267,11 +259,6 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
             // rounding falls on the change, not on the velocity itself
             un = u + (Du - u * Dr) * irn;
             vn = vp + ((DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr) - vp * Dr) * irn;
-            // the background pressure force / (r + Dr), in FP64 (the FP32
-            // reciprocal refined by one Newton step)
-            double ir64 = (double)irn;
-            if (sizeof(real) == 4) ir64 = ir64 * (2.0 - (double)rn * ir64);
-            un = (real)((double)un + d.rot[4 * nx + i] * d.ctl->dt * ir64);
         }
         out[row + i] = rdn;
         out[N + row + i] = un;
