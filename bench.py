@@ -53,7 +53,7 @@ WORDS_PER_CELL = {
     "predictor_viscosity": 7,    # gas 3 + forces 2 in; forces of the predicted state 2 out
     "corrector_viscosity": 14,   # gas 3 + forces 2 + dust u,vp in; gas 3 (wave killing) + dust 2 + forces 2 out
     "sources_corrector": 12,     # (without viscosity) gas 3 + dust 3 in; gas and dust velocities out
-    "cfl": 6,
+    "cfl": 4,                    # gas u, vp and dust u, vp in (c_s from the per-radius table)
     "prologue": 0,
 }
 
