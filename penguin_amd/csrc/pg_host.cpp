@@ -696,7 +696,8 @@ void build_worklists(const Group& g, int tile_x, int ring_b, int order, int pt_r
         for (int x = 0; x < 8; ++x)
             for (size_t k = 0; k < qp[x].size(); ++k) wp[8 * k + x] = qp[x][k];
     }
-    const int TR = Launch<float>::phi_tile_radii(ring_b), TY = Launch<float>::phi_tile_rows(ring_b);
+    const int TR = g.prec == 8 ? Launch<double>::phi_tile_radii(ring_b) : Launch<float>::phi_tile_radii(ring_b);
+    const int TY = g.prec == 8 ? Launch<double>::phi_tile_rows(ring_b) : Launch<float>::phi_tile_rows(ring_b);
     if (order != 0) {
         for (int k = 0; k < nd; ++k) {
             const int ntr = (g.disks[k]->nx + TR - 1) / TR, nty = (g.disks[k]->ny + TY - 1) / TY;

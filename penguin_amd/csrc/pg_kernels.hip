@@ -407,13 +407,13 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_both(LaunchSet<real>
 // output rows.  The rows of one radius are independent of the other radii,
 // so a tile is TR independent azimuthal strips side by side; TR consecutive
 // radii make each row of the tile one contiguous segment in HBM.
-#ifndef PG_PHI512_TR
-#define PG_PHI512_TR 16
-#endif
-template <int B> struct PhiTile;
-template <> struct PhiTile<256> { static constexpr int TR = 16, TY = 16; };
-template <> struct PhiTile<512> { static constexpr int TR = PG_PHI512_TR, TY = 512 / PG_PHI512_TR; };
-template <> struct PhiTile<1024> { static constexpr int TR = 16, TY = 64; };
+// With 512 threads an FP32 tile is 64 rows high (each thread takes two cells
+// of every stage, which halves the per-cell share of the tile set-up and
+// the 12-row halo); an FP64 tile, twice the LDS per cell, 32 rows.
+template <typename real, int B> struct PhiTile;
+template <typename real> struct PhiTile<real, 256> { static constexpr int TR = 16, TY = 16; };
+template <typename real> struct PhiTile<real, 512> { static constexpr int TR = 16, TY = sizeof(real) == 4 ? 64 : 32; };
+template <typename real> struct PhiTile<real, 1024> { static constexpr int TR = 16, TY = 64; };
 
 // Azimuthal pass of one fluid over a tile, fused with the orbital shift and
 // the residual advection: sweepy_inplace (sweeps.cu:361-456) or
@@ -433,17 +433,19 @@ template <> struct PhiTile<1024> { static constexpr int TR = 16, TY = 64; };
 // the deviations, the interface pressures deviations from c_s^2 rho_b, and
 // the residual advection's mass flux the deviation's (the background's
 // uniform flux cancels in every difference).
-// LDS of one tile: 13 arrays of E cells (state 3, slopes 6, fluxes 4).
-template <int B> struct PhiLds {
-    static constexpr int NK = PhiTile<B>::TY + 4 * NPAD;   // LDS rows per radius
-    static constexpr int E = NK * PhiTile<B>::TR;          // LDS cells per array
-    static constexpr int WORDS = 13 * E;
+// LDS of one tile: 9 arrays of E cells (state 3, slopes 6).  A flux stage
+// keeps its interface fluxes in registers until every thread has read the
+// slopes, then stores them over the slope arrays for the update.
+template <typename real, int B> struct PhiLds {
+    static constexpr int NK = PhiTile<real, B>::TY + 4 * NPAD;   // LDS rows per radius
+    static constexpr int E = NK * PhiTile<real, B>::TR;          // LDS cells per array
+    static constexpr int WORDS = 9 * E;
 };
 template <typename real, bool DUST, int B>
 __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int par, int flags, real* lds) {
-    constexpr int TR = PhiTile<B>::TR, TY = PhiTile<B>::TY;
-    constexpr int NK = PhiLds<B>::NK;
-    constexpr int E = PhiLds<B>::E;
+    constexpr int TR = PhiTile<real, B>::TR, TY = PhiTile<real, B>::TY;
+    constexpr int NK = PhiLds<real, B>::NK;
+    constexpr int E = PhiLds<real, B>::E;
     real* const c0 = lds;
     real* const c1 = lds + E;
     real* const c2 = lds + 2 * E;
@@ -453,10 +455,10 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
     real* const sr1 = lds + 6 * E;
     real* const sl2 = lds + 7 * E;
     real* const sr2 = lds + 8 * E;
-    real* const F0 = lds + 9 * E;
-    real* const F1 = lds + 10 * E;
-    real* const F2 = lds + 11 * E;
-    real* const F3 = lds + 12 * E;
+    real* const F0 = sl0;   // fluxes, once the slopes are consumed
+    real* const F1 = sr0;
+    real* const F2 = sl1;
+    real* const F3 = sr1;
     if (w.y < 0) return;   // padding entry of the work list (whole block)
     const DiskArgs<real>& d = ls.disks[w.x];
     const int nx = d.nx, ny = d.ny;
@@ -523,13 +525,19 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
         sl2[e] = l; sr2[e] = r;
     }
     __syncthreads();
-    for (int e = t + NPAD * TR; e < (NK - 2) * TR; e += B) {
+    // interfaces [NPAD, NK-2) of every radius: at most NFA per thread
+    constexpr int FA0 = NPAD * TR, FA1 = (NK - 2) * TR, NFA = (FA1 - FA0 + B - 1) / B;
+    real fa[NFA][4];
+#pragma unroll
+    for (int k = 0; k < NFA; ++k) {
+        const int e = t + FA0 + k * B;
+        if (e >= FA1) break;
         const int m = e - TR;   // the cell left of interface e
         if (DUST) {
-            real Fr, Fu, Fw;
             dust_iface(dts, idy, c0[m], sl0[m], sr0[m], c1[m], sl1[m], sr1[m], c2[m], sl2[m], sr2[m],
-                       idy, c0[e], sl0[e], sr0[e], c1[e], sl1[e], sr1[e], c2[e], sl2[e], sr2[e], Fr, Fu, Fw);
-            F0[e] = Fr; F1[e] = Fu; F2[e] = Fw;
+                       idy, c0[e], sl0[e], sr0[e], c1[e], sl1[e], sr1[e], c2[e], sl2[e], sr2[e],
+                       fa[k][0], fa[k][1], fa[k][2]);
+            fa[k][3] = real(0);
         } else {
             const real cs2 = TAB(T_CS2, ip), cs0 = TAB(T_C0, ip), ic0 = TAB(T_IC0, ip);
             real rl, rld, ul, vl, rr, rrd, ur, vr;
@@ -538,10 +546,16 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
                           sr2[m], rl, rld, ul, vl);
             trace_right_cd(cs0, ic0, dts, idy, rbg, c0[e], c1[e], sl0[e], sr0[e], sl1[e], sr1[e], sl2[e], c2[e],
                            sr2[e], rr, rrd, ur, vr);
-            real Fr, Fu, Fw, presd;
-            hllc_cd(cs0, ic0, cs2, rbg, rl, rld, ul, vl, rr, rrd, ur, vr, Fr, Fu, Fw, presd);
-            F0[e] = Fr; F1[e] = Fu; F2[e] = Fw; F3[e] = presd;
+            hllc_cd(cs0, ic0, cs2, rbg, rl, rld, ul, vl, rr, rrd, ur, vr, fa[k][0], fa[k][1], fa[k][2], fa[k][3]);
         }
+    }
+    __syncthreads();   // every slope read: the fluxes go over them
+#pragma unroll
+    for (int k = 0; k < NFA; ++k) {
+        const int e = t + FA0 + k * B;
+        if (e >= FA1) break;
+        F0[e] = fa[k][0]; F1[e] = fa[k][1]; F2[e] = fa[k][2];
+        if (!DUST) F3[e] = fa[k][3];
     }
     __syncthreads();
     for (int e = t + NPAD * TR; e < (NK - NPAD) * TR; e += B) {
@@ -599,7 +613,12 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
         sl2[e] = l; sr2[e] = r;
     }
     __syncthreads();
-    for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY + 1) * TR; e += B) {
+    constexpr int FB0 = 2 * NPAD * TR, FB1 = (2 * NPAD + TY + 1) * TR, NFB = (FB1 - FB0 + B - 1) / B;
+    real fb[NFB][3];
+#pragma unroll
+    for (int k = 0; k < NFB; ++k) {
+        const int e = t + FB0 + k * B;
+        if (e >= FB1) break;
         real Dr, Dv, Du;   // Dr: upwind mean of the stored density (gas: its deviation)
         if (speed > real(0)) {
             const int m = e - TR;
@@ -613,7 +632,14 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
             Dv = aveL(q, sl1[e], c1[e], sr1[e]) * ra;
             Du = aveL(q, sl2[e], c2[e], sr2[e]) * ra;
         }
-        F0[e] = Dr * speed; F1[e] = Dv * speed; F2[e] = Du * speed;
+        fb[k][0] = Dr * speed; fb[k][1] = Dv * speed; fb[k][2] = Du * speed;
+    }
+    __syncthreads();   // every slope read: the fluxes go over them
+#pragma unroll
+    for (int k = 0; k < NFB; ++k) {
+        const int e = t + FB0 + k * B;
+        if (e >= FB1) break;
+        F0[e] = fb[k][0]; F1[e] = fb[k][1]; F2[e] = fb[k][2];
     }
     __syncthreads();
     for (int e = t + 2 * NPAD * TR; e < (2 * NPAD + TY) * TR; e += B) {
@@ -648,12 +674,12 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
 // fluids read and write different buffers).
 template <typename real, bool DUST, int B>
 __global__ __launch_bounds__(B) void k_phi(LaunchSet<real> ls, int par, int flags) {
-    __shared__ real lds[PhiLds<B>::WORDS];
+    __shared__ real lds[PhiLds<real, B>::WORDS];
     phi_tile<real, DUST, B>(ls, ls.wl_ring[blockIdx.x], par, flags, lds);
 }
 template <typename real, int B>
 __global__ __launch_bounds__(B) void k_phi_both(LaunchSet<real> ls, int par, int flags) {
-    __shared__ real lds[PhiLds<B>::WORDS];
+    __shared__ real lds[PhiLds<real, B>::WORDS];
     const int b = blockIdx.x;
     if (b < ls.n_ring) phi_tile<real, false, B>(ls, ls.wl_ring[b], par, flags, lds);
     else phi_tile<real, true, B>(ls, ls.wl_ring[b - ls.n_ring], par, flags, lds);
@@ -1348,8 +1374,12 @@ static hipError_t ring_any(const LaunchSet<real>& ls, int par, int flags, int wh
 
 template <typename real>
 struct Launch {
-    static int phi_tile_radii(int B) { return B == 256 ? PhiTile<256>::TR : B == 1024 ? PhiTile<1024>::TR : PhiTile<512>::TR; }
-    static int phi_tile_rows(int B) { return B == 256 ? PhiTile<256>::TY : B == 1024 ? PhiTile<1024>::TY : PhiTile<512>::TY; }
+    static int phi_tile_radii(int B) {
+        return B == 256 ? PhiTile<real, 256>::TR : B == 1024 ? PhiTile<real, 1024>::TR : PhiTile<real, 512>::TR;
+    }
+    static int phi_tile_rows(int B) {
+        return B == 256 ? PhiTile<real, 256>::TY : B == 1024 ? PhiTile<real, 1024>::TY : PhiTile<real, 512>::TY;
+    }
     static hipError_t sweep_x_gas(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, 0, kick, s); }
     static hipError_t sweep_x_dust(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, 1, kick, s); }
     static hipError_t sweep_x_both(const LaunchSet<real>& ls, int par, int kick, hipStream_t s) { return sweep_x_any(ls, par, 2, kick, s); }
