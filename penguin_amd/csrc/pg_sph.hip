@@ -79,9 +79,12 @@ struct SphK {
     const real* sza;
     const real* sinz;
     const real* cotz;
-    // per (padded radius, interior polar layer): background rotation, nu
+    // per (padded radius, interior polar layer): background rotation, nu,
+    // and the radial force balance of the background rotation
+    // ebal = vbg^2 / r - 1 / r^2 (formed in FP64)
     const real* vbg;
     const real* nu;
+    const real* ebal;
     // radial ghosts: [f][g][k], g = 0..2 (ip = 0..2), 3..5 (ip = nx+3..nx+5)
     const real* xg;
     // per azimuthal row j: sin and cos of phi_j / 2 (phi_j the row centre)
@@ -215,15 +218,32 @@ __device__ __forceinline__ Par<real> params(real am2, real am1, real a0, real ap
     pem_select(p);
     return p;
 }
+// the same parameters about a zero mean: avgL / avgR of them is the sub-cell
+// mean less the cell's own (both are affine in the mean with unit weight)
+template <typename real>
+__device__ __forceinline__ Par<real> zero_mean(Par<real> p) {
+    p.a = real(0);
+    return p;
+}
 
 // set_state (states.cu:1-135) for an adiabatic gas: the state right of the
 // cell's left edge (from the cell's means over its first fractions, R side)
 // and left of its right edge (L side).  P: parameters of rho, p, u, v, w;
 // rc, pc, uc: the cell's values; dts: the sweep's time step over its length
 // unit (dt / rad); idx = 1 / width.
+// The traced pressure S[1] is returned as its deviation from the edge's
+// reference pressure pref (the mean of the two cells' pressures), given the
+// cell's own deviation pdev = p - pref: the pressure's sub-cell means enter
+// as their differences from the cell's pressure, so the pressure
+// differences of the update are formed from numbers of their own size.
+// S[1] is p0 exp_lim(gam u_) - pref = (p0 - pref) + p0 (exp_lim(gam u_) - 1).
+template <typename real>
+__device__ __forceinline__ real exp_lim_m1(real x) {   // exp_lim(x) - 1
+    return fmax(x, real(1.0e-6) - real(1));
+}
 template <int RECON, typename real>
-__device__ __forceinline__ void trace_R(const Par<real>* P, real rc, real pc, real uc, real gam, real dts, real idx,
-                                        real* S) {
+__device__ __forceinline__ void trace_R(const Par<real>* P, real rc, real pc, real uc, real pdev, real gam, real dts,
+                                        real idx, real* S) {
     const real hdt = real(0.5) * dts;
     real cs = FM<real>::sqrt(FM<real>::div(gam * pc, rc));
     real q = lim01((cs - fmin(uc, real(0))) * dts * idx);
@@ -231,24 +251,28 @@ __device__ __forceinline__ void trace_R(const Par<real>* P, real rc, real pc, re
     const real ur = avgL<RECON>(q, P[2]);
     const real dis = fmin(ur, real(0)) * dts;
     q = lim01(-dis * idx);
-    real r0 = avgL<RECON>(q, P[0]), p0 = avgL<RECON>(q, P[1]), u0 = avgL<RECON>(q, P[2]);
+    const Par<real> Pz = zero_mean(P[1]);
+    const real dp0 = avgL<RECON>(q, Pz);
+    real r0 = avgL<RECON>(q, P[0]), p0 = pc + dp0, u0 = avgL<RECON>(q, P[2]);
     r0 = fmax(r0, rmin);
+    const bool pfl = p0 < pmin;
     p0 = fmax(p0, pmin);
     cs = FM<real>::sqrt(FM<real>::div(gam * p0, r0));
     q = lim01((cs * dts - dis) * idx);
-    real u_ = avgL<RECON>(q, P[2]), p_ = avgL<RECON>(q, P[1]);
+    const real dp_ = avgL<RECON>(q, Pz);
+    real u_ = avgL<RECON>(q, P[2]);
     const real f = FM<real>::div(real(2), cs * dts) * (-hdt);
     u_ = (u_ - u0) * f;
-    p_ = (p_ - p0) * f;
+    const real p_ = (pfl ? (pc + dp_) - p0 : dp_ - dp0) * f;
     S[0] = r0 * exp_lim(u_);
-    S[1] = p0 * exp_lim(gam * u_);
+    S[1] = (pfl ? p0 - (pc - pdev) : pdev + dp0) + p0 * exp_lim_m1(gam * u_);
     S[2] = u0 + FM<real>::div(p_, r0);
     S[3] = avgL<RECON>(q, P[3]);
     S[4] = avgL<RECON>(q, P[4]);
 }
 template <int RECON, typename real>
-__device__ __forceinline__ void trace_L(const Par<real>* P, real rc, real pc, real uc, real gam, real dts, real idx,
-                                        real* S) {
+__device__ __forceinline__ void trace_L(const Par<real>* P, real rc, real pc, real uc, real pdev, real gam, real dts,
+                                        real idx, real* S) {
     const real hdt = real(0.5) * dts;
     real cs = FM<real>::sqrt(FM<real>::div(gam * pc, rc));
     real q = lim01(real(1) - (fmax(uc, real(0)) + cs) * dts * idx);
@@ -256,17 +280,21 @@ __device__ __forceinline__ void trace_L(const Par<real>* P, real rc, real pc, re
     const real ul = avgR<RECON>(q, P[2]);
     const real dis = fmax(ul, real(0)) * dts;
     q = lim01(real(1) - dis * idx);
-    real r0 = avgR<RECON>(q, P[0]), p0 = avgR<RECON>(q, P[1]), u0 = avgR<RECON>(q, P[2]);
+    const Par<real> Pz = zero_mean(P[1]);
+    const real dp0 = avgR<RECON>(q, Pz);
+    real r0 = avgR<RECON>(q, P[0]), p0 = pc + dp0, u0 = avgR<RECON>(q, P[2]);
     r0 = fmax(r0, rmin);
+    const bool pfl = p0 < pmin;
     p0 = fmax(p0, pmin);
     cs = FM<real>::sqrt(FM<real>::div(gam * p0, r0));
     q = lim01(real(1) - (dis + cs * dts) * idx);
-    real u_ = avgR<RECON>(q, P[2]), p_ = avgR<RECON>(q, P[1]);
+    const real dp_ = avgR<RECON>(q, Pz);
+    real u_ = avgR<RECON>(q, P[2]);
     const real f = FM<real>::div(real(2), cs * dts) * (-hdt);
     u_ = (u0 - u_) * f;
-    p_ = (p0 - p_) * f;
+    const real p_ = (pfl ? p0 - (pc + dp_) : dp0 - dp_) * f;
     S[0] = r0 * exp_lim(u_);
-    S[1] = p0 * exp_lim(gam * u_);
+    S[1] = (pfl ? p0 - (pc - pdev) : pdev + dp0) + p0 * exp_lim_m1(gam * u_);
     S[2] = u0 + FM<real>::div(p_, r0);
     S[3] = avgR<RECON>(q, P[3]);
     S[4] = avgR<RECON>(q, P[4]);
@@ -275,14 +303,21 @@ __device__ __forceinline__ void trace_L(const Par<real>* P, real rc, real pc, re
 // wave_speeds (wave_speeds.cu:24-35,288-333) and HLLC_fluxes (HLLC.cu:2-77)
 // with EOS_flag 2, internal_e_flag 1: mass flux; internal energy, normal and
 // transverse momenta upwinded times the mass flux; interface pressure p* and
-// velocity u* (the p dV work, riemann.cu:65-73).
+// velocity u* (the p dV work, riemann.cu:65-73).  L[1], R[1] and the
+// returned p* are deviations from the interface's reference pressure pref:
+// the contact speed's pressure term and p* are formed from them, the sound
+// speeds and fluxes from the full pressures.
 template <typename real>
-__device__ __forceinline__ void hllc_adi(const real* L, const real* R, real gam, real* F, real& pres, real& uprs) {
-    const real rl = L[0], pl = L[1], ul = L[2], rr = R[0], pr = R[1], ur = R[2];
+__device__ __forceinline__ void hllc_adi(const real* L, const real* R, real pref, real gam, real* F, real& pres,
+                                         real& uprs) {
+    const real rl = L[0], pld = L[1], ul = L[2], rr = R[0], prd = R[1], ur = R[2];
+    const real pl = pref + pld, pr = pref + prd;
     const real cl = FM<real>::sqrt(FM<real>::div(gam * pl, rl));
     const real cr = FM<real>::sqrt(FM<real>::div(gam * pr, rr));
     const real c_ = real(0.5) * (cr + cl), p_ = real(0.5) * (pr + pl);
-    const real pm = p_ * exp_lim(FM<real>::div(real(0.5) * gam * (ul - ur), c_));
+    const real xm = FM<real>::div(real(0.5) * gam * (ul - ur), c_);
+    const real pm = p_ * exp_lim(xm);
+    const real pmd = real(0.5) * (pld + prd) + p_ * exp_lim_m1(xm);   // pm - pref
     const real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
     const real isum = FM<real>::rcp(srl + srr);
     const real ubar = (srl * ul + srr * ur) * isum;
@@ -290,12 +325,12 @@ __device__ __forceinline__ void hllc_adi(const real* L, const real* R, real gam,
     const real cbar = FM<real>::sqrt((cl * cl * srl + cr * cr * srr) * isum + n2 * (ur - ul) * (ur - ul));
     const real sl = ubar - cbar, sr = ubar + cbar;
     real sm = FM<real>::div(rl * ul * cl + rr * ur * cr, rl * cl + rr * cr);
-    sm += FM<real>::div(pl - pr, (rl + rr) * (real(0.5) * (cl + cr)));
+    sm += FM<real>::div(pld - prd, (rl + rr) * (real(0.5) * (cl + cr)));
     // pick (HLLC.cu:2-8): sl >= 0 -> left, sr < 0 -> right, else the star
     // state on the side of the contact
     const bool supL = !signbit(sl), supR = signbit(sr), left = supL || (!supR && !signbit(sm));
     const bool star = !supL && !supR;
-    const real rK = left ? rl : rr, uK = left ? ul : ur, pK = left ? pl : pr;
+    const real rK = left ? rl : rr, uK = left ? ul : ur, pK = left ? pl : pr, pKd = left ? pld : prd;
     const real sK = left ? sl : sr;
     const real vK = left ? L[3] : R[3], wK = left ? L[4] : R[4];
     const real fm = star ? FM<real>::div(sm * rK * (sK - uK), sK - sm) : uK * rK;
@@ -304,7 +339,7 @@ __device__ __forceinline__ void hllc_adi(const real* L, const real* R, real gam,
     F[2] = uK * fm;
     F[3] = vK * fm;
     F[4] = wK * fm;
-    pres = star ? FM<real>::div(sK * pm - sm * pK, sK - sm) : pK;
+    pres = star ? FM<real>::div(sK * pmd - sm * pKd, sK - sm) : pKd;
     uprs = star ? sm * FM<real>::div(sK * pm - uK * pK, sK * pm - sm * pK) : uK;
 }
 
@@ -502,9 +537,14 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<re
             const real* x = q + f * RS;
             P[f] = params<RECON>(x[-2 * R], x[-R], x[0], x[R], x[2 * R], wl, flat);
         }
+        // the cell's pressure less the reference pressures of its left and
+        // right edges (the means of the two cells' pressures at each): the
+        // differences of neighbouring pressures are exact in the working
+        // precision, so these carry no rounding of the pressure itself
+        const real pdl = real(0.5) * (q[RS] - q[RS - R]), pdr = real(0.5) * (q[RS] - q[RS + R]);
         real SR[NF], SL[NF];
-        trace_R<RECON>(P, q[0], q[RS], q[2 * RS], a.gam, dts, idx, SR);
-        trace_L<RECON>(P, q[0], q[RS], q[2 * RS], a.gam, dts, idx, SL);
+        trace_R<RECON>(P, q[0], q[RS], q[2 * RS], pdl, a.gam, dts, idx, SR);
+        trace_L<RECON>(P, q[0], q[RS], q[2 * RS], pdr, a.gam, dts, idx, SL);
         for (int f = 0; f < NF; ++f) {
             TRs[f * RS + e] = SR[f];
             TLs[f * RS + e] = SL[f];
@@ -524,7 +564,8 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<re
             Ls[f] = TLs[f * RS + e - R];
             Rs[f] = TRs[f * RS + e];
         }
-        hllc_adi(Ls, Rs, a.gam, Fr[m], Fr[m][NF], Fr[m][NF + 1]);
+        const real pref = real(0.5) * (raw[RS + e - R] + raw[RS + e]);
+        hllc_adi(Ls, Rs, pref, a.gam, Fr[m], Fr[m][NF], Fr[m][NF + 1]);
     }
     __syncthreads();
 #pragma unroll
@@ -563,8 +604,12 @@ __global__ __launch_bounds__(DIR == 0 ? SPH_BTX : SPH_BT2) void k3_sweep(SphK<re
         }
         const real* F0 = TLs + e;
         const real* F1 = TLs + e + R;
-        const real p0s = TRs[e], p1s = TRs[e + R];
-        const real du = real(0.5) * (g0 + g1) * (p0s - p1s);
+        // p* at the two edges: deviations from the edges' references, whose
+        // own difference is half that of the neighbours' pressures
+        const real* pq = raw + RS + e;
+        const real p0d = TRs[e], p1d = TRs[e + R];
+        const real du = real(0.5) * (g0 + g1) * ((p0d - p1d) + real(0.5) * (pq[-R] - pq[R]));
+        const real p0s = real(0.5) * (pq[-R] + pq[0]) + p0d, p1s = real(0.5) * (pq[0] + pq[R]) + p1d;
         const real de = real(0.5) * (p0s + p1s) * (TRs[RS + e] * g0 - TRs[RS + e + R] * g1);
         real D[NF];
         D[0] = (F0[0] * g0 - F1[0] * g1) * fac;
@@ -694,10 +739,12 @@ __global__ __launch_bounds__(SPH_BT) void k3_orbadv(SphK<real> a, double dt, int
 // (forces.cu:110-327): mode 0 the half kick (in -> out, forces at the state
 // itself), modes 1/2 predictor and corrector (forces re-evaluated at the
 // midpoint state, in -> out).
+// gx, gy, gz: the planet's pull and the indirect term (the star's gravity
+// -1/r^2 is part of the tabulated background balance ebal)
 template <typename real>
 __device__ __forceinline__ void sph_forces(const SphK<real>& a, const PlanetS<real>& pl, bool planet, real rad, int j,
                                            int kp, real& gx, real& gy, real& gz) {
-    gx = -FM<real>::rcp(rad * rad);
+    gx = real(0);
     gy = real(0);
     gz = real(0);
     if (!planet) return;
@@ -755,20 +802,26 @@ __device__ __forceinline__ void source_cell(const SphK<real>& a, const PlanetS<r
     const real rad = a.xc[ip];
     real gx, gy, gz;
     sph_forces(a, pl, planet, rad, j, kp, gx, gy, gz);
-    const real vb = a.vbg[(size_t)(kp - NPAD) * (a.nx + 2 * NPAD) + ip];
+    const size_t tk = (size_t)(kp - NPAD) * (a.nx + 2 * NPAD) + ip;
+    const real vb = a.vbg[tk], eb = a.ebal[tk];
     const real vis0 = FM<real>::div(visr, rho), vis1 = FM<real>::div(visp, rho);
     const real ir = FM<real>::rcp(rad), cot = a.cotz[kp];
-    // get_g_forces (sweeps.cu:555-576): fictitious forces of the state T
-    auto forces = [&](real Tv, real Tw, real& fx, real& fy, real& fz) {
+    // get_g_forces (sweeps.cu:555-576): fictitious forces of the state T at
+    // azimuthal velocity vb + Tvp.  The radial centrifugal force less the
+    // star's gravity is the tabulated balance of the background plus the
+    // deviation's part, (vb + Tvp)^2/r - 1/r^2 = ebal + Tvp (2 vb + Tvp)/r:
+    // the two large terms cancel in FP64, not in the working precision.
+    auto forces = [&](real Tvp, real Tw, real& fx, real& fy, real& fz) {
+        const real Tv = vb + Tvp;
         fz = Tv * Tv * ir * cot;
         fy = vis1;
-        fx = Tv * Tv * ir + Tw * Tw * ir + vis0;
+        fx = eb + Tvp * (real(2) * vb + Tvp) * ir + Tw * Tw * ir + vis0;
     };
     real fx, fy, fz;
-    forces(vp + vb, w, fx, fy, fz);
+    forces(vp, w, fx, fy, fz);
     if (mdt != real(0)) {
-        const real Tv = vp + vb + fy * mdt + gy * mdt, Tw = w + fz * mdt + gz * mdt;
-        forces(Tv, Tw, fx, fy, fz);
+        const real Tvp = vp + fy * mdt + gy * mdt, Tw = w + fz * mdt + gz * mdt;
+        forces(Tvp, Tw, fx, fy, fz);
     }
     un = u + fx * dt + gx * dt;
     vn = vp + fy * dt + gy * dt;
@@ -1028,7 +1081,8 @@ struct Sph {
 
 namespace {
 
-enum { TB_XA, TB_XV, TB_XC, TB_XW, TB_ZA, TB_ZV, TB_ZC, TB_ZW, TB_SZA, TB_SINZ, TB_COTZ, TB_VBG, TB_NU, TB_XG, TB_HPHI, TB_N };
+enum { TB_XA, TB_XV, TB_XC, TB_XW, TB_ZA, TB_ZV, TB_ZC, TB_ZW, TB_SZA, TB_SINZ, TB_COTZ, TB_VBG, TB_NU, TB_XG, TB_HPHI,
+       TB_EBAL, TB_N };
 
 template <typename real>
 SphK<real> kargs(Sph* s, int in, int out) {
@@ -1044,6 +1098,7 @@ SphK<real> kargs(Sph* s, int in, int out) {
     a.zw = T + s->tab_off[TB_ZW]; a.sza = T + s->tab_off[TB_SZA]; a.sinz = T + s->tab_off[TB_SINZ];
     a.cotz = T + s->tab_off[TB_COTZ];
     a.vbg = T + s->tab_off[TB_VBG]; a.nu = T + s->tab_off[TB_NU]; a.xg = T + s->tab_off[TB_XG];
+    a.ebal = T + s->tab_off[TB_EBAL];
     a.hphi = T + s->tab_off[TB_HPHI];
     a.rot = s->dtab;
     a.rcyl = s->dtab + (size_t)s->nx * s->nz;
@@ -1291,14 +1346,16 @@ int upload_tables(Sph* s) {
         cotz[k] = std::cos(zc[k]) / std::sin(zc[k]);
     }
     s->vbg.assign((size_t)nxe * nz, 0.0);
-    std::vector<double> nu((size_t)nxe * nz, 0.0), xg((size_t)NF * 6 * nz, 0.0);
+    std::vector<double> nu((size_t)nxe * nz, 0.0), xg((size_t)NF * 6 * nz, 0.0), ebal((size_t)nxe * nz, 0.0);
     s->rot.assign((size_t)nx * nz, 0.0);
     s->rcyl.assign((size_t)nx * nz, 0.0);
     for (int k = 0; k < nz; ++k) {
         const double z = zc[k + NPAD];
         for (int ip = 0; ip < nxe; ++ip) {
-            s->vbg[(size_t)k * nxe + ip] = pr.v(xc[ip], z);
+            const double vb = pr.v(xc[ip], z);
+            s->vbg[(size_t)k * nxe + ip] = vb;
             nu[(size_t)k * nxe + ip] = pr.nu(xc[ip], z);
+            ebal[(size_t)k * nxe + ip] = vb * vb / xc[ip] - 1.0 / (xc[ip] * xc[ip]);
         }
         for (int i = 0; i < nx; ++i) {
             const double rc = xc[i + NPAD] * std::sin(z);
@@ -1326,7 +1383,7 @@ int upload_tables(Sph* s) {
     parts[TB_XA] = &xa; parts[TB_XV] = &xv; parts[TB_XC] = &xc; parts[TB_XW] = &xw;
     parts[TB_ZA] = &za; parts[TB_ZV] = &zv; parts[TB_ZC] = &zc; parts[TB_ZW] = &zw;
     parts[TB_SZA] = &sza; parts[TB_SINZ] = &sinz; parts[TB_COTZ] = &cotz;
-    parts[TB_VBG] = &s->vbg; parts[TB_NU] = &nu; parts[TB_XG] = &xg; parts[TB_HPHI] = &hphi;
+    parts[TB_VBG] = &s->vbg; parts[TB_NU] = &nu; parts[TB_XG] = &xg; parts[TB_HPHI] = &hphi; parts[TB_EBAL] = &ebal;
     size_t tot = 0;
     for (int q = 0; q < TB_N; ++q) {
         s->tab_off[q] = tot;

@@ -7,8 +7,9 @@ symmetry) on the device.
 
 Tolerances: FP64 agrees to rounding (both sides take PEM's float
 logarithm correctly rounded), compared in relative L1 at 1e-10 and pointwise
-at 1e-6; FP32 runs in relative L1 at 1e-5 on density and the momentum vector
-after 50 steps."""
+at 1e-6; FP32 runs in relative L1 at 1e-5 after 50 steps on density,
+pressure, the radial and azimuthal momenta and the momentum vector, and at
+2e-5 on the polar momentum (see test_steps_match_oracle_fp32)."""
 import numpy as np
 import pytest
 
@@ -131,17 +132,25 @@ def test_steps_match_oracle_fp32():
         o.solve(t, float(h))
         t += float(h)
     a, b = d.get_state()[0], o.get_state()[0]
-    # the gate of the 2D parity tests: density and the momentum vector
-    # rho (u, v, w) in relative L1; its small radial and polar components
-    # are reported on their own
+    # density, pressure, each momentum component and the momentum vector in
+    # relative L1.  The polar momentum's bound is 2e-5: the polar pressure
+    # force and the centrifugal force's polar part each change rho w per step
+    # by about |rho w| itself (mean |d(rho w)| of one polar sweep 1.06e-3
+    # against mean |rho w| 1.24e-3 on this state), so the FP32 rounding of
+    # the stored pressure and of those two forces reaches rho w undivided
+    # (1.6e-5 after 50 steps; FP64 agrees to 1e-12)
+    err = {}
     for k, name in ((0, "rho"), (1, "p"), (2, "rho u"), (3, "rho v"), (4, "rho w")):
         x, y = (a[0] * a[k], b[0] * b[k]) if k >= 2 else (a[k], b[k])
-        print(f"FP32 50 steps {name}: rel L1 {np.abs(x - y).sum() / np.abs(y).sum():.2e}", flush=True)
-    rho = np.abs(a[0] - b[0]).sum() / np.abs(b[0]).sum()
+        err[name] = np.abs(x - y).sum() / np.abs(y).sum()
+        print(f"FP32 50 steps {name}: rel L1 {err[name]:.2e}", flush=True)
     mom = sum(np.abs(a[0] * a[k] - b[0] * b[k]).sum() for k in (2, 3, 4)) / \
         sum(np.abs(b[0] * b[k]).sum() for k in (2, 3, 4))
-    print(f"FP32 50 steps: rho {rho:.2e} momentum {mom:.2e}", flush=True)
-    assert rho <= 1e-5 and mom <= 1e-5
+    print(f"FP32 50 steps: momentum {mom:.2e}", flush=True)
+    for name in ("rho", "p", "rho u", "rho v"):
+        assert err[name] <= 1e-5, (name, err[name])
+    assert err["rho w"] <= 2e-5
+    assert mom <= 1e-5
 
 
 def test_polar_sweep_conserves_column_mass_on_device():
