@@ -107,10 +107,19 @@ __device__ __forceinline__ void cell_sources(const DiskArgs<real>& d, int mode, 
 // slot in each phase), or the TX + 2 NPAD slots rounded up to whole waves
 // for the tiles sized to fill them (122 + 6 = two waves, 250 + 6 = four,
 // 506 + 6 = eight: one slot per thread, no phase run twice by one wave).
+// A 1024-cell tile has 512 threads, two slots each (the first NPAD*2 a
+// third): a 1024-radius row in one workgroup, the per-wave set-up shared by
+// two cells, and no workgroup for the few radii a 506-cell tiling leaves.
 template <int TX>
 struct XBlock {
-    static constexpr int BT = TX % 64 == 0 ? TX : (TX + 2 * NPAD + 63) / 64 * 64;
+    static constexpr int BT = TX == 1024 ? 512 : TX % 64 == 0 ? TX : (TX + 2 * NPAD + 63) / 64 * 64;
+    static constexpr int S = TX + 2 * NPAD;                                // LDS slots
+    static constexpr int NFX = (S - 2 - NPAD + BT - 1) / BT;               // interfaces per thread, at most
 };
+// LDS arrays of S slots: the gas tile 12, the dust tile 10 (their fluxes
+// are held in registers until the slopes are consumed, then stored over
+// the slope arrays)
+constexpr int XLDS_GAS = 12, XLDS_DUST = 10;
 
 // ------------------------------------------------------------ x sweep, gas
 // One block = TX interior cells of one azimuthal row (the whole row when
@@ -131,18 +140,17 @@ struct XBlock {
 template <typename real, int TX>
 __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2 w, int par, int kick, real* sm) {
     const DiskArgs<real>& d = ls.disks[w.x];
-    constexpr int S = TX + 2 * NPAD;
-    real* s_r = sm;             // rho - rho_b
-    real* s_p = sm + S;         // p - p_b = c_s^2 (rho - rho_b)
-    real* s_u = sm + 2 * S;
-    real* s_L = sm + 3 * S;
-    real* s_vp = sm + 4 * S;
-    real* s_sl = sm + 5 * S;    // 4 fields: r, p, u, L
-    real* s_sr = sm + 9 * S;
-    real* s_Fr = sm + 13 * S;
-    real* s_Fu = sm + 14 * S;
-    real* s_FL = sm + 15 * S;
-    real* s_pr = sm + 16 * S;   // interface pressure - its background reference
+    constexpr int S = XBlock<TX>::S;
+    real* s_r = sm;             // rho - rho_b (p - p_b = c_s^2 (rho - rho_b) is formed where it is read)
+    real* s_u = sm + S;
+    real* s_L = sm + 2 * S;
+    real* s_vp = sm + 3 * S;
+    real* s_sl = sm + 4 * S;    // 4 fields: r, p, u, L
+    real* s_sr = sm + 8 * S;
+    real* s_Fr = s_sl;          // the fluxes, once the parabolas are consumed
+    real* s_Fu = s_sl + S;
+    real* s_FL = s_sl + 2 * S;
+    real* s_pr = s_sl + 3 * S;  // interface pressure - its background reference
     const int nx = d.nx;
     const size_t N = (size_t)nx * d.ny;
     const int t = threadIdx.x;
@@ -173,7 +181,6 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
             }
         }
         s_r[s] = rd;
-        s_p[s] = TAB(T_CS2, ip) * rd;
         s_u[s] = u;
         s_vp[s] = vp;
         s_L[s] = TAB(T_XC, ip) * (TAB(T_VBG, ip) + vp);
@@ -191,7 +198,9 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
         const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
         const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
         // differences across the edges ip-1 .. ip+2 (edge e between cells e-1, e)
-        const real p0 = s_p[s - 2], p1 = s_p[s - 1], p2 = s_p[s], p3 = s_p[s + 1], p4 = s_p[s + 2];
+        const real p0 = TAB(T_CS2, ip - 2) * s_r[s - 2], p1 = TAB(T_CS2, ip - 1) * s_r[s - 1];
+        const real p2 = TAB(T_CS2, ip) * s_r[s], p3 = TAB(T_CS2, ip + 1) * s_r[s + 1];
+        const real p4 = TAB(T_CS2, ip + 2) * s_r[s + 2];
         const real dp0 = TAB(T_TBP, ip - 1) + (p1 - p0), dp1 = TAB(T_TBP, ip) + (p2 - p1);
         const real dp2 = TAB(T_TBP, ip + 1) + (p3 - p2), dp3 = TAB(T_TBP, ip + 2) + (p4 - p3);
         const real r0 = s_r[s - 2], r1 = s_r[s - 1], r2 = s_r[s], r3 = s_r[s + 1], r4 = s_r[s + 2];
@@ -210,25 +219,40 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
         s_sl[3 * S + s] = l; s_sr[3 * S + s] = r;
     }
     __syncthreads();
-    for (int k = t + NPAD; k < ns - 2; k += XBlock<TX>::BT) {
+    real fx[XBlock<TX>::NFX][4];
+#pragma unroll
+    for (int q = 0; q < XBlock<TX>::NFX; ++q) {
+        const int k = t + NPAD + q * XBlock<TX>::BT;
+        if (k >= ns - 2) break;
         const int ipk = x0 + k;
         const real rbl = TAB(T_G_R, ipk - 1), rbr = TAB(T_G_R, ipk);
-        const real pbl = TAB(T_CS2, ipk - 1) * rbl, pbr = TAB(T_CS2, ipk) * rbr;
+        const real cs2l = TAB(T_CS2, ipk - 1), cs2r = TAB(T_CS2, ipk);
+        const real pbl = cs2l * rbl, pbr = cs2r * rbr;
         real rl, rld, pld, ul, vl, rr, rrd, prd, ur, vr, cl, cr;
-        trace_left_d(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), rbl, pbl, s_r[k - 1], s_p[k - 1], s_u[k - 1],
+        trace_left_d(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), rbl, pbl, s_r[k - 1], cs2l * s_r[k - 1], s_u[k - 1],
                      s_sl[k - 1], s_sr[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
                      s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
                      s_sl[3 * S + k - 1], s_L[k - 1], s_sr[3 * S + k - 1], rl, rld, pld, ul, vl, cl);
-        trace_right_d(TAB(T_C0, ipk), dt, TAB(T_IDX, ipk), rbr, pbr, s_r[k], s_p[k], s_u[k],
+        trace_right_d(TAB(T_C0, ipk), dt, TAB(T_IDX, ipk), rbr, pbr, s_r[k], cs2r * s_r[k], s_u[k],
                       s_sl[k], s_sr[k], s_sl[S + k], s_sr[S + k], s_sl[2 * S + k], s_sr[2 * S + k],
                       s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, rrd, prd, ur, vr, cr);
         real Fr, Fu, FL, presd;
         hllc_iso_d(cl, cr, pbl, TAB(T_TBP, ipk), rl, pld, ul, vl, rr, prd, ur, vr, Fr, Fu, FL, presd);
         const real g = TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
-        s_Fr[k] = Fr * g;
-        s_Fu[k] = Fu * g;
-        s_FL[k] = FL * g;
-        s_pr[k] = presd;
+        fx[q][0] = Fr * g;
+        fx[q][1] = Fu * g;
+        fx[q][2] = FL * g;
+        fx[q][3] = presd;
+    }
+    __syncthreads();   // every parabola read: the fluxes go over them
+#pragma unroll
+    for (int q = 0; q < XBlock<TX>::NFX; ++q) {
+        const int k = t + NPAD + q * XBlock<TX>::BT;
+        if (k >= ns - 2) break;
+        s_Fr[k] = fx[q][0];
+        s_Fu[k] = fx[q][1];
+        s_FL[k] = fx[q][2];
+        s_pr[k] = fx[q][3];
     }
     __syncthreads();
     PG_GLOBAL real* __restrict__ out = gmem(d.gas[gsel(par) ^ 1]);
@@ -275,16 +299,16 @@ template <typename real, int TX>
 __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int2 w, int par, int gpre, int kick,
                                                   real* sm) {
     const DiskArgs<real>& d = ls.disks[w.x];
-    constexpr int S = TX + 2 * NPAD;
+    constexpr int S = XBlock<TX>::S;
     real* s_r = sm;
     real* s_u = sm + S;
     real* s_L = sm + 2 * S;
     real* s_vp = sm + 3 * S;
     real* s_sl = sm + 4 * S;  // 3 fields: r, u, L
     real* s_sr = sm + 7 * S;
-    real* s_Fr = sm + 10 * S;
-    real* s_Fu = sm + 11 * S;
-    real* s_FL = sm + 12 * S;
+    real* s_Fr = s_sl;        // the fluxes, once the parabolas are consumed
+    real* s_Fu = s_sl + S;
+    real* s_FL = s_sl + 2 * S;
     const int nx = d.nx;
     const size_t N = (size_t)nx * d.ny;
     const int t = threadIdx.x;
@@ -334,7 +358,11 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
         s_sl[2 * S + s] = l; s_sr[2 * S + s] = r;
     }
     __syncthreads();
-    for (int k = t + NPAD; k < ns - 2; k += XBlock<TX>::BT) {
+    real fx[XBlock<TX>::NFX][3];
+#pragma unroll
+    for (int q = 0; q < XBlock<TX>::NFX; ++q) {
+        const int k = t + NPAD + q * XBlock<TX>::BT;
+        if (k >= ns - 2) break;
         const int ipk = x0 + k;
         real Fr, Fu, FL;
         dust_iface(dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_sl[k - 1], s_sr[k - 1],
@@ -343,9 +371,18 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
                    TAB(T_IDX, ipk), s_r[k], s_sl[k], s_sr[k], s_u[k], s_sl[S + k], s_sr[S + k],
                    s_L[k], s_sl[2 * S + k], s_sr[2 * S + k], Fr, Fu, FL);
         const real g = TAB(T_XA, ipk);
-        s_Fr[k] = Fr * g;
-        s_Fu[k] = Fu * g;
-        s_FL[k] = FL * g;
+        fx[q][0] = Fr * g;
+        fx[q][1] = Fu * g;
+        fx[q][2] = FL * g;
+    }
+    __syncthreads();   // every parabola read: the fluxes go over them
+#pragma unroll
+    for (int q = 0; q < XBlock<TX>::NFX; ++q) {
+        const int k = t + NPAD + q * XBlock<TX>::BT;
+        if (k >= ns - 2) break;
+        s_Fr[k] = fx[q][0];
+        s_Fu[k] = fx[q][1];
+        s_FL[k] = fx[q][2];
     }
     __syncthreads();
     PG_GLOBAL real* __restrict__ out = gmem(d.dust[dsel(par) ^ 1]);
@@ -1306,19 +1343,19 @@ static hipError_t set_lds(Kern k, size_t lds) {
 // which: 0 the gas, 1 the dust, 2 both in one launch
 template <typename real, int TX>
 static hipError_t launch_sweep_x(const LaunchSet<real>& ls, int par, int which, int kick, hipStream_t s) {
-    const size_t S = TX + 2 * NPAD;
+    const size_t S = XBlock<TX>::S;
     if (which == 1) {
-        size_t lds = 13 * S * sizeof(real);
+        size_t lds = XLDS_DUST * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_dust<real, TX>, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_sweep_x_dust<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
     } else if (which == 0) {
-        size_t lds = 17 * S * sizeof(real);
+        size_t lds = XLDS_GAS * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_gas<real, TX>, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_sweep_x_gas<real, TX>), dim3(ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
     } else {
-        size_t lds = 17 * S * sizeof(real);
+        size_t lds = XLDS_GAS * S * sizeof(real);
         hipError_t e = set_lds(k_sweep_x_both<real, TX>, lds);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_sweep_x_both<real, TX>), dim3(2 * ls.n_x), dim3(XBlock<TX>::BT), lds, s, ls, par, kick);
