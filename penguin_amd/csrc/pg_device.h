@@ -368,10 +368,11 @@ PG_HD void trace_right_d(real c, real dt, real idx, real rb, real pb, real rd, r
     if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
     if (p0 < pmin) { p0 = pmin; p0d = pmin - pb; }
     real u0 = aveL(q, usL, u, usR);
-    real ir0 = FM<real>::rcp(r0);
-    real w = p0 * ir0;
-    real ics = FM<real>::rsqrt(w);
-    real cs = w * ics;
+    // y = 1/sqrt(p0 r0): the sound speed sqrt(p0/r0) = p0 y, its reciprocal
+    // r0 y and 1/(r0 cs) = y, from one reciprocal square root
+    const real y = FM<real>::rsqrt(p0 * r0);
+    const real ics = r0 * y;
+    const real cs = p0 * y;
     q = lim01((cs * dt - dis) * idx);
     real u_ = aveL(q, usL, u, usR);
     real p_d = aveL(q, psL, pd, psR);
@@ -379,7 +380,7 @@ PG_HD void trace_right_d(real c, real dt, real idx, real rb, real pb, real rd, r
     rr = fma(r0, xe, r0);
     rrd = fma(r0, xe, r0d);
     prd = fma(p0, xe, p0d);
-    ur = u0 + (p0d - p_d) * ics * ir0;
+    ur = u0 + (p0d - p_d) * y;
     vr = aveL(q, vsL, v, vsR);
     cout = cs;
 }
@@ -399,10 +400,9 @@ PG_HD void trace_left_d(real c, real dt, real idx, real rb, real pb, real rd, re
     if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
     if (p0 < pmin) { p0 = pmin; p0d = pmin - pb; }
     real u0 = aveR(x, usL, u, usR);
-    real ir0 = FM<real>::rcp(r0);
-    real w = p0 * ir0;
-    real ics = FM<real>::rsqrt(w);
-    real cs = w * ics;
+    const real y = FM<real>::rsqrt(p0 * r0);
+    const real ics = r0 * y;
+    const real cs = p0 * y;
     x = lim01((dis + cs * dt) * idx);
     real u_ = aveR(x, usL, u, usR);
     real p_d = aveR(x, psL, pd, psR);
@@ -410,7 +410,7 @@ PG_HD void trace_left_d(real c, real dt, real idx, real rb, real pb, real rd, re
     rl = fma(r0, xe, r0);
     rld = fma(r0, xe, r0d);
     pld = fma(p0, xe, p0d);
-    ul = u0 + (p_d - p0d) * ics * ir0;
+    ul = u0 + (p_d - p0d) * y;
     vl = aveR(x, vsL, v, vsR);
     cout = cs;
 }
@@ -428,19 +428,23 @@ PG_HD void hllc_iso_d(real cl, real cr, real pbl, real tbp, real rl, real pld, r
     const real h = real(0.5) * tbp;
     const real pref = pbl + h;
     const real pl = pld - h, pr = prd + h;    // deviations from pref
-    real csum = cl + cr;
-    real ics = FM<real>::rcp(csum);
+    // Roe averages through a = sqrt(rr/rl) (one reciprocal square root):
+    // weights 1/(1 + a) and a/(1 + a), n2 = a/(2 (1 + a)^2).  The three
+    // reciprocals 1/(cl + cr), 1/(1 + a) and 1/(den rs) come from the
+    // reciprocal of their product (all three factors are positive).
+    const real a = rr * FM<real>::rsqrt(rl * rr);
+    const real csum = cl + cr, opa = real(1) + a;
+    const real den = rl * cl + rr * cr, rs = rl + rr;
+    const real dr = den * rs, co = csum * opa;
+    const real Z = FM<real>::rcp(co * dr);
+    const real ics = opa * dr * Z, iss = csum * dr * Z, X = co * Z;
     const real xm = exp_lim_m1((ul - ur) * ics);
     const real pm = fma(real(0.5) * (pr + pl), real(1) + xm, pref * xm);
-    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
-    real iss = FM<real>::rcp(srl + srr);
-    real ubar = (srl * ul + srr * ur) * iss;
-    real n2 = real(0.5) * srl * srr * iss * iss;
+    real ubar = (ul + a * ur) * iss;
+    real n2 = real(0.5) * a * iss * iss;
     real du = ur - ul;
-    real cbar = FM<real>::sqrt((cl * cl * srl + cr * cr * srr) * iss + n2 * du * du);
+    real cbar = FM<real>::sqrt((cl * cl + a * (cr * cr)) * iss + n2 * du * du);
     real sl = ubar - cbar, sr = ubar + cbar;
-    const real den = rl * cl + rr * cr, rs = rl + rr;
-    const real X = FM<real>::rcp(den * rs);
     real sm = (rl * ul * cl + rr * ur * cr) * rs * X + real(2) * (pl - pr) * ics * den * X;
     // the star pressure (s pm - sm p)/(s - sm) is affine in (pm, p) with
     // weights summing to one, so it carries the common reference unchanged
@@ -503,17 +507,21 @@ PG_HD void hllc_cd(real c, real ic, real c2, real rb, real rl, real rld, real ul
     const real pl = c2 * rld, pr = c2 * rrd;
     const real xm = exp_lim_m1(real(0.5) * (ul - ur) * ic);
     const real pm = c2 * fma(real(0.5) * (rld + rrd), real(1) + xm, rb * xm);
-    real srl = FM<real>::sqrt(rl), srr = FM<real>::sqrt(rr);
-    real iss = FM<real>::rcp(srl + srr);
-    real ubar = (srl * ul + srr * ur) * iss;
-    real n2 = real(0.5) * srl * srr * iss * iss;
+    // Roe averages through a = sqrt(rr/rl); 1/(1 + a) and 1/(rl + rr) from
+    // the reciprocal of their product
+    const real a = rr * FM<real>::rsqrt(rl * rr);
+    const real opa = real(1) + a, rs = rl + rr;
+    const real Z = FM<real>::rcp(opa * rs);
+    const real iss = rs * Z;
+    real ubar = (ul + a * ur) * iss;
+    real n2 = real(0.5) * a * iss * iss;
     real du = ur - ul;
     real cbar = FM<real>::sqrt(c2 + n2 * du * du);
     real sl = ubar - cbar, sr = ubar + cbar;
     // sm = (rl (ul + c) + rr (ur - c)) / (rl + rr) with c (rl - rr) formed
     // from the deviations: the terms of order c rho cancel to order u rho,
     // and in FP32 their rounding would be ~1e-3 of the contact speed
-    real sm = FM<real>::div(rl * ul + rr * ur + c * (rld - rrd), rl + rr);
+    real sm = (rl * ul + rr * ur + c * (rld - rrd)) * (opa * Z);
     hllc_select(pm, sl, sm, sr, rl, pl, ul, vl, rr, pr, ur, vr, Fr, us, vs, presd);
 }
 
@@ -619,7 +627,7 @@ __device__ __forceinline__ void planet_acc(const PlanetK& pk, const real* pt, re
     const real h = pt[2] * chp - pt[3] * shp;    // sin((phi - phi_p)/2)
     const real px = (real)pk.x, m = (real)pk.m, rs = (real)pk.rs;
     const real Rp2 = (R - px) * (R - px) + real(4) * px * R * h * h + rs * rs;
-    const real iRp = FM<real>::rcp(FM<real>::sqrt(Rp2));
+    const real iRp = FM<real>::rsqrt(Rp2);
     const real k = iRp * iRp * iRp;
     const real ind = (real)pk.m_over_x2;
     ax = -m * (R - px * c) * k - c * ind;
