@@ -159,6 +159,11 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
     const int x0 = (w.y - j * tiles) * TX;
     const int nt = min(TX, nx - x0);
     const int ns = nt + 2 * NPAD;
+    // a wave with no slot in any phase (the last, narrow tile of a row)
+    // ends here: a barrier waits on the waves of the workgroup that have
+    // not ended, and the ended waves free their slots on the CU for other
+    // workgroups (the condition is uniform over each wave)
+    if ((t & ~63) >= ns) return;
     const real dt = (real)d.ctl->dt;
     const PG_GLOBAL real* __restrict__ in = gmem(d.gas[gsel(par)]);
     const size_t row = (size_t)j * nx;
@@ -317,6 +322,7 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
     const int x0 = (w.y - j * tiles) * TX;
     const int nt = min(TX, nx - x0);
     const int ns = nt + 2 * NPAD;
+    if ((t & ~63) >= ns) return;   // as in the gas tile
     const real dt = (real)d.ctl->dt;
     const PG_GLOBAL real* __restrict__ in = gmem(d.dust[dsel(par)]);
     const size_t row = (size_t)j * nx;
