@@ -167,6 +167,33 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
     const real dt = (real)d.ctl->dt;
     const PG_GLOBAL real* __restrict__ in = gmem(d.gas[gsel(par)]);
     const size_t row = (size_t)j * nx;
+    // The narrow tiles of one slot per thread in every phase (122, 250):
+    // the per-radius table values of the thread's slots are loaded here,
+    // with the state, so that no later phase waits on a table load.  The
+    // thread's parabola slot is radius ia = x0 + t + 2, its interface and
+    // its update slot ia + 1 (indices ia - 2 .. ia + 2, inside the padded
+    // tables for every slot of the tile).  Measured: the x sweep of 64
+    // uneven disks on 122-radius tiles 321 -> 282 us; the 506-radius tiles
+    // of a 1024-radius disk, whose eight waves hide the table loads behind
+    // each other, 91 -> 99 us with the 24 more VGPRs, so they load as they go.
+    constexpr bool ONE = XBlock<TX>::BT >= XBlock<TX>::S && TX <= 250;
+    real hw[6] = {}, hcs[5] = {}, htp[4] = {}, htr[4] = {}, hgr[2] = {}, hc0[2] = {}, hix[2] = {}, hxa[2] = {};
+    real hixv = real(0), hps = real(0), hir = real(0), hvb = real(0);
+    if (ONE && t < ns - 4) {
+        const int ia = x0 + t + 2;
+        hw[0] = TAB(T_WL0, ia); hw[1] = TAB(T_WL1, ia); hw[2] = TAB(T_WL2, ia);
+        hw[3] = TAB(T_WR0, ia); hw[4] = TAB(T_WR1, ia); hw[5] = TAB(T_WR2, ia);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) hcs[m] = TAB(T_CS2, ia - 2 + m);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) { htp[m] = TAB(T_TBP, ia - 1 + m); htr[m] = TAB(T_TBR, ia - 1 + m); }
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            hgr[m] = TAB(T_G_R, ia + m); hc0[m] = TAB(T_C0, ia + m); hix[m] = TAB(T_IDX, ia + m);
+            hxa[m] = TAB(T_XA, ia + 1 + m);
+        }
+        hixv = TAB(T_IXV, ia + 1); hps = TAB(T_PSRC, ia + 1); hir = TAB(T_IR, ia + 1); hvb = TAB(T_VBG, ia + 1);
+    }
 
     for (int s = t; s < ns; s += XBlock<TX>::BT) {
         const int i = x0 - NPAD + s;
@@ -200,18 +227,27 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
     }
     for (int s = t + 2; s < ns - 2; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
-        const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
-        const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
+        const real wl0 = ONE ? hw[0] : TAB(T_WL0, ip), wl1 = ONE ? hw[1] : TAB(T_WL1, ip);
+        const real wl2 = ONE ? hw[2] : TAB(T_WL2, ip), wr0 = ONE ? hw[3] : TAB(T_WR0, ip);
+        const real wr1 = ONE ? hw[4] : TAB(T_WR1, ip), wr2 = ONE ? hw[5] : TAB(T_WR2, ip);
+        real cs[5], tp[4], tr[4];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) cs[m] = ONE ? hcs[m] : TAB(T_CS2, ip - 2 + m);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            tp[m] = ONE ? htp[m] : TAB(T_TBP, ip - 1 + m);
+            tr[m] = ONE ? htr[m] : TAB(T_TBR, ip - 1 + m);
+        }
         // differences across the edges ip-1 .. ip+2 (edge e between cells e-1, e)
-        const real p0 = TAB(T_CS2, ip - 2) * s_r[s - 2], p1 = TAB(T_CS2, ip - 1) * s_r[s - 1];
-        const real p2 = TAB(T_CS2, ip) * s_r[s], p3 = TAB(T_CS2, ip + 1) * s_r[s + 1];
-        const real p4 = TAB(T_CS2, ip + 2) * s_r[s + 2];
-        const real dp0 = TAB(T_TBP, ip - 1) + (p1 - p0), dp1 = TAB(T_TBP, ip) + (p2 - p1);
-        const real dp2 = TAB(T_TBP, ip + 1) + (p3 - p2), dp3 = TAB(T_TBP, ip + 2) + (p4 - p3);
+        const real p0 = cs[0] * s_r[s - 2], p1 = cs[1] * s_r[s - 1];
+        const real p2 = cs[2] * s_r[s], p3 = cs[3] * s_r[s + 1];
+        const real p4 = cs[4] * s_r[s + 2];
+        const real dp0 = tp[0] + (p1 - p0), dp1 = tp[1] + (p2 - p1);
+        const real dp2 = tp[2] + (p3 - p2), dp3 = tp[3] + (p4 - p3);
         const real r0 = s_r[s - 2], r1 = s_r[s - 1], r2 = s_r[s], r3 = s_r[s + 1], r4 = s_r[s + 2];
-        const real dr0 = TAB(T_TBR, ip - 1) + (r1 - r0), dr1 = TAB(T_TBR, ip) + (r2 - r1);
-        const real dr2 = TAB(T_TBR, ip + 1) + (r3 - r2), dr3 = TAB(T_TBR, ip + 2) + (r4 - r3);
-        const real pc = TAB(T_CS2, ip) * TAB(T_G_R, ip) + p2;
+        const real dr0 = tr[0] + (r1 - r0), dr1 = tr[1] + (r2 - r1);
+        const real dr2 = tr[2] + (r3 - r2), dr3 = tr[3] + (r4 - r3);
+        const real pc = cs[2] * (ONE ? hgr[0] : TAB(T_G_R, ip)) + p2;
         const real fl = flatten_d(dp2, dp1 + dp2 + dp3, dp1, dp0 + dp1 + dp2, pc, s_u[s - 1], s_u[s], s_u[s + 1]);
         real l, r;
         slopes_d(dr0, dr1, dr2, dr3, wl0, wl1, wl2, wr0, wr1, wr2, fl, l, r);
@@ -230,20 +266,20 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
         const int k = t + NPAD + q * XBlock<TX>::BT;
         if (k >= ns - 2) break;
         const int ipk = x0 + k;
-        const real rbl = TAB(T_G_R, ipk - 1), rbr = TAB(T_G_R, ipk);
-        const real cs2l = TAB(T_CS2, ipk - 1), cs2r = TAB(T_CS2, ipk);
+        const real rbl = ONE ? hgr[0] : TAB(T_G_R, ipk - 1), rbr = ONE ? hgr[1] : TAB(T_G_R, ipk);
+        const real cs2l = ONE ? hcs[2] : TAB(T_CS2, ipk - 1), cs2r = ONE ? hcs[3] : TAB(T_CS2, ipk);
         const real pbl = cs2l * rbl, pbr = cs2r * rbr;
         real rl, rld, pld, ul, vl, rr, rrd, prd, ur, vr, cl, cr;
-        trace_left_d(TAB(T_C0, ipk - 1), dt, TAB(T_IDX, ipk - 1), rbl, pbl, s_r[k - 1], cs2l * s_r[k - 1], s_u[k - 1],
+        trace_left_d(ONE ? hc0[0] : TAB(T_C0, ipk - 1), dt, ONE ? hix[0] : TAB(T_IDX, ipk - 1), rbl, pbl, s_r[k - 1], cs2l * s_r[k - 1], s_u[k - 1],
                      s_sl[k - 1], s_sr[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
                      s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
                      s_sl[3 * S + k - 1], s_L[k - 1], s_sr[3 * S + k - 1], rl, rld, pld, ul, vl, cl);
-        trace_right_d(TAB(T_C0, ipk), dt, TAB(T_IDX, ipk), rbr, pbr, s_r[k], cs2r * s_r[k], s_u[k],
+        trace_right_d(ONE ? hc0[1] : TAB(T_C0, ipk), dt, ONE ? hix[1] : TAB(T_IDX, ipk), rbr, pbr, s_r[k], cs2r * s_r[k], s_u[k],
                       s_sl[k], s_sr[k], s_sl[S + k], s_sr[S + k], s_sl[2 * S + k], s_sr[2 * S + k],
                       s_sl[3 * S + k], s_L[k], s_sr[3 * S + k], rr, rrd, prd, ur, vr, cr);
         real Fr, Fu, FL, presd;
-        hllc_iso_d(cl, cr, pbl, TAB(T_TBP, ipk), rl, pld, ul, vl, rr, prd, ur, vr, Fr, Fu, FL, presd);
-        const real g = TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
+        hllc_iso_d(cl, cr, pbl, ONE ? htp[2] : TAB(T_TBP, ipk), rl, pld, ul, vl, rr, prd, ur, vr, Fr, Fu, FL, presd);
+        const real g = ONE ? hxa[0] : TAB(T_XA, ipk);  // eval_gfac(1, xa) = edge radius
         fx[q][0] = Fr * g;
         fx[q][1] = Fu * g;
         fx[q][2] = FL * g;
@@ -264,14 +300,15 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
     for (int s = t + NPAD; s < ns - NPAD; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
         const int i = ip - NPAD;
-        const real g0 = TAB(T_XA, ip), g1 = TAB(T_XA, ip + 1);
-        const real f = dt * TAB(T_IXV, ip);
+        const real g0 = ONE ? hxa[0] : TAB(T_XA, ip), g1 = ONE ? hxa[1] : TAB(T_XA, ip + 1);
+        const real f = dt * (ONE ? hixv : TAB(T_IXV, ip));
         const real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
         // net_source (riemann.cu:59-62): the deviations' part plus the
         // background's, T_PSRC = (g0 + g1)/2 (pref[ip] - pref[ip+1])
-        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1]) + TAB(T_PSRC, ip)) * f;
+        const real Du = (s_Fu[s] - s_Fu[s + 1] + real(0.5) * (g0 + g1) * (s_pr[s] - s_pr[s + 1]) +
+                         (ONE ? hps : TAB(T_PSRC, ip))) * f;
         const real DL = (s_FL[s] - s_FL[s + 1]) * f;
-        const real rb = TAB(T_G_R, ip);
+        const real rb = ONE ? hgr[1] : TAB(T_G_R, ip);
         const real rd = s_r[s], u = s_u[s], vp = s_vp[s];
         const real r = rb + rd;
         const real rn = r + Dr;
@@ -287,7 +324,7 @@ __device__ __forceinline__ void sweep_x_gas_tile(const LaunchSet<real>& ls, int2
             // (r u + Du) / (r + Dr) as u + (Du - u Dr) / (r + Dr): the
             // rounding falls on the change, not on the velocity itself
             un = u + (Du - u * Dr) * irn;
-            vn = vp + ((DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr) - vp * Dr) * irn;
+            vn = vp + ((DL * (ONE ? hir : TAB(T_IR, ip)) - (ONE ? hvb : TAB(T_VBG, ip)) * Dr) - vp * Dr) * irn;
         }
         out[row + i] = rdn;
         out[N + row + i] = un;
