@@ -236,6 +236,21 @@ def copy_probe(mib: int):
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
 
 
+def persistent_probe(npass: int = 200) -> dict:
+    """What one persistent launch of the step would pay per pass boundary:
+    a grid-wide barrier of a cooperative launch, against the gap between two
+    kernel nodes of a captured graph (the batch's step has five boundaries:
+    prologue, x sweep, azimuthal pass, predictor, corrector)."""
+    from penguin_amd.disk import probe_grid_barrier
+
+    out = {"passes": npass, "threads_per_block": 256, "us_per_grid_barrier": {}, "us_per_graph_node": {}}
+    for blocks in (256, 1024, 2048):
+        ub, un = probe_grid_barrier(blocks, npass)
+        out["us_per_grid_barrier"][str(blocks)] = round(ub, 3)
+        out["us_per_graph_node"][str(blocks)] = round(un, 3)
+    return out
+
+
 def kernel_times(d, cells: int, word: int, steps: int) -> dict:
     """Per-kernel durations from HIP events around every launch of `steps`
     uncaptured steps of disk d."""
@@ -415,6 +430,7 @@ def main():
 
     batched = batched_leg(a.batch_disks, a.batch_steps, a.precision) if a.batch_disks > 0 else None
     if batched:
+        batched["persistent_probe"] = persistent_probe()
         note(f"batched: speedup {batched['speedup']}")
     adjoint = adjoint_leg(a.adjoint_steps, a.precision) if a.adjoint_steps > 0 else None
     if adjoint:

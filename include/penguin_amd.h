@@ -144,6 +144,15 @@ int pg_torque(pg_handle* h, double* torque);
    every call on that GPU: calls that overlap on one GPU race.  Returns the
    same bits as pg_torque. */
 int pg_torque_noalloc(pg_handle* h, double* torque);
+/* Measurement, no counterpart in the reference: what a persistent step
+   kernel would pay per pass boundary.  Runs npass (2 .. 100000) passes of a
+   data-carrying pass over `blocks` workgroups of 256 threads (at most the
+   co-resident count, else PG_E_INVALID) once as one cooperative launch with
+   a grid-wide barrier after each pass and once as npass kernel nodes of a
+   captured graph; returns the microseconds per barrier and per node (pass
+   included in both) and checks the words the passes leave (PG_E_STATE if
+   they are wrong). */
+int pg_probe_grid_barrier(int blocks, int npass, double* us_per_barrier, double* us_per_node);
 
 /* ---- introspection ----------------------------------------------------- */
 /* HIP stream the handle launches on (as a void*), for external capture. */

@@ -23,7 +23,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpenguin_amd.so")
-SOURCES = ["pg_kernels.hip", "pg_adjoint.hip", "pg_sph.hip", "pg_host.cpp"]
+SOURCES = ["pg_kernels.hip", "pg_adjoint.hip", "pg_sph.hip", "pg_probe.hip", "pg_host.cpp"]
 HEADERS = ["pg_internal.h", "pg_launch.h", "pg_device.h", "pg_adjoint_fn.h", "../../include/penguin_amd.h", "../../include/pg_params.h"]
 
 

@@ -2449,6 +2449,12 @@ int pg_torque(pg_handle* h, double* torque) {
 // pg_torque with the partial sums in the device array `scratch` (at most
 // NOALLOC_TILES tiles of the current schedule): no device or pinned memory
 // is allocated; the result comes back through the caller's pointer.
+int pg_probe_grid_barrier(int blocks, int npass, double* us_per_barrier, double* us_per_node) {
+    std::string msg;
+    const int rc = probe_grid_barrier(blocks, npass, us_per_barrier, us_per_node, msg);
+    return rc == PG_OK ? PG_OK : fail(rc, "%s", msg.c_str());
+}
+
 int pg_torque_noalloc(pg_handle* h, double* torque) {
     CHECK_H(h);
     CHECK_2D(h);

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "pg_internal.h"
 
 namespace pg {
@@ -49,5 +51,10 @@ struct AdjLaunch {
     static hipError_t vis(const AdjArgs<real>& A, int mode, int nx, int ny, hipStream_t s);
     static hipError_t src(const AdjArgs<real>& A, int nx, int ny, hipStream_t s);
 };
+
+// Grid-barrier probe (pg_probe.hip): microseconds per grid-wide barrier of
+// one cooperative launch and per kernel node of a captured graph, for the
+// same data-carrying pass over `blocks` workgroups of 256 threads.
+int probe_grid_barrier(int blocks, int npass, double* us_per_barrier, double* us_per_node, std::string& msg);
 
 }  // namespace pg

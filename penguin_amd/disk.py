@@ -68,6 +68,7 @@ SIGNATURES = {
     "pg_write_average_snapshot": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double]),
     "pg_torque": (ctypes.c_int, [_vp, _dp]),
     "pg_torque_noalloc": (ctypes.c_int, [_vp, _dp]),
+    "pg_probe_grid_barrier": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "pg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pg_batch_get_schedule": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "pg_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -102,6 +103,15 @@ def load_library(path: str = LIB_PATH):
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def probe_grid_barrier(blocks: int, npass: int = 200):
+    """Microseconds per grid-wide barrier of a cooperative launch and per
+    kernel node of a captured graph, for the same pass over `blocks`
+    workgroups of 256 threads (pg_probe_grid_barrier)."""
+    ub, un = ctypes.c_double(), ctypes.c_double()
+    _check(load_library().pg_probe_grid_barrier(blocks, npass, ctypes.byref(ub), ctypes.byref(un)))
+    return ub.value, un.value
 
 
 def _check(rc: int):
