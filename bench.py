@@ -81,6 +81,9 @@ def parse():
                     help="uncaptured steps with HIP events around every launch, for the per-kernel durations")
     ap.add_argument("--batch-disks", type=int, default=64, help="disks of the batched leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the grid-barrier probe of the batched leg (its cooperative launch is not traced: "
+                         "rocprofv3's kernel tracing ends the process in a host SIGSEGV there)")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="steps of the adjoint leg (0 skips it)")
     ap.add_argument("--sph-steps", type=int, default=20, help="steps of the 3D disk leg (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -430,7 +433,8 @@ def main():
 
     batched = batched_leg(a.batch_disks, a.batch_steps, a.precision) if a.batch_disks > 0 else None
     if batched:
-        batched["persistent_probe"] = persistent_probe()
+        if not a.no_probe:
+            batched["persistent_probe"] = persistent_probe()
         note(f"batched: speedup {batched['speedup']}")
     adjoint = adjoint_leg(a.adjoint_steps, a.precision) if a.adjoint_steps > 0 else None
     if adjoint:
