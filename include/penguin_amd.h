@@ -61,6 +61,21 @@ int pg_destroy(pg_handle* h);
 int pg_get_state(pg_handle* h, double* gas, double* dust);
 int pg_set_state(pg_handle* h, const double* gas, const double* dust);
 
+/* Ghost fills, one direction each (boundx / boundx_dust, boundy /
+   boundy_dust, boundz; include/boundary.h:4-10, boundary.cu:13-374,
+   462-718).  No ghost cells are stored on the device: the kernels read the
+   radial ghosts from the analytic tables (type 0), the azimuthal ones by
+   periodic indexing (type 3) and the polar ones by mirroring with w
+   negated (type 2).  These return what those reads see: the current state
+   with the 3 ghost cells on each side of the named direction filled as the
+   reference fills them, field-major in pg_get_state's field order, that
+   direction's extent n + 6 (x fastest; e.g. pg_bound_x: (xres+6)*yres*zres
+   per field).  dust may be NULL (PG_E_STATE if given for a disk without
+   dust); pg_bound_z needs a 3D disk (PG_E_INVALID otherwise). */
+int pg_bound_x(pg_handle* h, double* gas, double* dust);
+int pg_bound_y(pg_handle* h, double* gas, double* dust);
+int pg_bound_z(pg_handle* h, double* gas, double* dust);
+
 /* CFL step: dt = min(CFL / max signal rate, 1.1*dt_prev)
    (global_dt, timestep.cu:113-140). */
 int pg_global_dt(pg_handle* h, double dt_prev, double* dt_out);

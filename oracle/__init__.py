@@ -59,6 +59,7 @@ def _bind(L):
         "orc_get_orbadv": (None, [vp, _ip, _dp]),
         "orc_killwave": (None, [vp, ctypes.c_double]),
         "orc_write_snapshot": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_double]),
+        "orc_bound": (None, [vp, ctypes.c_int, _dp, _dp]),
         "orc_evolve_planet": (None, [vp, ctypes.c_double]),
         "orc_fourth_parameters": (None, [ctypes.c_int, _dp, _dp, _dp]),
         "orc_ppm_adjust": (None, [ctypes.c_double, _dp]),
@@ -235,6 +236,15 @@ class Oracle:
 
     def evolve_planet(self, t):
         self._L.orc_evolve_planet(self._h, t)
+
+    def bound(self, direction: int, dust: bool = False):
+        """The state with the ghost cells of one direction filled (boundx/y/z)."""
+        shape = list(self.shape)
+        shape[len(shape) - 1 - direction] += 6
+        g = np.zeros((5,) + tuple(shape))
+        d = np.zeros((4,) + tuple(shape)) if dust else None
+        self._L.orc_bound(self._h, direction, ptr(g), ptr(d) if dust else None)
+        return g, d
 
     def write_snapshot(self, filename: str, t: float):
         if self._L.orc_write_snapshot(self._h, filename.encode(), t) != 0:

@@ -1650,6 +1650,38 @@ void orc_killwave(orc_grid* g, double dt) { killwave(*g, dt); }
 // write_check_point (output.cu:125-180) for ndim 2, with restructure_data's
 // field order (output.cu:95-123); the dust w field is written too (ndim > 1
 // guard of output.cu:171)
+void orc_bound(orc_grid* g, int dir, double* gas, double* dust) {
+    orc_grid& G = *g;
+    if (dir == 0) {
+        fill_x_ghosts(G, G.C);
+        if (dust && G.P.dust_flag) fill_x_ghosts_dust(G, G.CD);
+    } else if (dir == 1) {
+        fill_y_ghosts(G, G.C);
+        if (dust && G.P.dust_flag) fill_y_ghosts(G, G.CD);
+    } else {
+        fill_z_ghosts(G, G.C);
+    }
+    // the block: every cell of the padded direction, the interior of the others
+    const int ni = dir == 0 ? G.xarr : G.xres, nj = dir == 1 ? G.yarr : G.yres, nk = dir == 2 ? G.zarr : G.zres;
+    const int i0 = dir == 0 ? 0 : G.xpad, j0 = dir == 1 ? 0 : G.ypad, k0 = dir == 2 ? 0 : G.zpad;
+    const size_t N = (size_t)ni * nj * nk;
+    for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const size_t o = (size_t)i + (size_t)ni * (j + (size_t)nj * k);
+            const size_t c = G.ind(i + i0, j + j0, k + k0);
+            if (gas) {
+                const Cell& C = G.C[c];
+                gas[o] = val(C.r); gas[N + o] = val(C.p); gas[2 * N + o] = val(C.u); gas[3 * N + o] = val(C.v);
+                gas[4 * N + o] = val(C.w);
+            }
+            if (dust && G.P.dust_flag) {
+                const Dust& D = G.CD[c];
+                dust[o] = val(D.r); dust[N + o] = val(D.u); dust[2 * N + o] = val(D.v); dust[3 * N + o] = val(D.w);
+            }
+        }
+}
+
 int orc_write_snapshot(const orc_grid* g, const char* filename, double t) {
     FILE* f = std::fopen(filename, "wb");
     if (!f) return -1;

@@ -57,6 +57,12 @@ void orc_orbital_advection(orc_grid* g, double dt);
 void orc_get_orbadv(const orc_grid* g, int* inc_xres, double* res_xres);
 void orc_killwave(orc_grid* g, double dt);
 int  orc_write_snapshot(const orc_grid* g, const char* filename, double t);
+/* ghost fills of one direction (dir 0: boundx/boundx_dust, 1: boundy/
+   boundy_dust, 2: boundz; boundary.cu:13-374, 462-718) applied to the
+   current state, which is returned with the 2*pad ghost cells of that
+   direction: field-major as orc_get_state, the padded direction's extent
+   n + 2 pad (x fastest).  dust may be NULL; 3D has none. */
+void orc_bound(orc_grid* g, int dir, double* gas, double* dust);
 void orc_evolve_planet(orc_grid* g, double t);
 
 /* unit-level functions, on caller-supplied row arrays */

@@ -1980,6 +1980,73 @@ int pg_destroy(pg_handle* h) {
     return PG_OK;
 }
 
+// Ghost fills of one direction on the host, from the handle's state: what
+// the kernels read in place of stored ghost cells.  dir 0: radial ghosts,
+// the analytic initial state (boundary type 0: the tables of the 2D disk,
+// the profiles of the 3D one); 1: azimuthal, periodic; 2: polar, mirrored
+// with w negated (type 2).
+static int bound_dir(pg_handle* h, int dir, double* gas, double* dust) {
+    CHECK_H(h);
+    if (!gas && !dust) return fail(PG_E_INVALID, "null gas and dust");
+    int nx, ny, nz;
+    if (h->sph) pg::sph_dims(h->sph, &nx, &ny, &nz);
+    else { nx = h->g.disks[0]->nx; ny = h->g.disks[0]->ny; nz = 1; }
+    if (dir == 2 && !h->sph) return fail(PG_E_INVALID, "pg_bound_z: the disk is two-dimensional");
+    if (dust && (h->sph || !h->g.disks[0]->P.dust_flag)) return fail(PG_E_STATE, "the disk has no dust");
+    const size_t N = (size_t)nx * ny * nz;
+    std::vector<double> g5(5 * N), d4(dust ? 4 * N : 0);
+    int rc = pg_get_state(h, g5.data(), dust ? d4.data() : nullptr);
+    if (rc) return rc;
+    const int ni = nx + (dir == 0) * 2 * NPAD, nj = ny + (dir == 1) * 2 * NPAD, nk = nz + (dir == 2) * 2 * NPAD;
+    const size_t M = (size_t)ni * nj * nk;
+    for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const size_t o = (size_t)i + (size_t)ni * (j + (size_t)nj * k);
+            int ic = i, jc = j, kc = k;   // the interior cell a ghost copies, if any
+            double sgn_w = 1.0;
+            bool analytic = false;
+            if (dir == 0) { ic = i - NPAD; analytic = ic < 0 || ic >= nx; }
+            if (dir == 1) jc = ((j - NPAD) % ny + ny) % ny;
+            if (dir == 2) {
+                kc = k - NPAD;
+                if (kc < 0 || kc >= nz) { kc = kc < 0 ? -1 - kc : 2 * nz - 1 - kc; sgn_w = -1.0; }
+            }
+            if (analytic) {
+                const int ip = i;   // padded radius
+                if (h->sph) {
+                    double f[5];
+                    pg::sph_ghost_x(h->sph, ic < 0 ? ip : ip - nx, k, f);
+                    if (gas) for (int q = 0; q < 5; ++q) gas[q * M + o] = f[q];
+                } else {
+                    const DiskDev* dd = h->g.disks[0];
+                    const double rho = tabv(dd, T_G_R, ip);
+                    if (gas) {
+                        gas[o] = rho; gas[M + o] = tabv(dd, T_CS2, ip) * rho; gas[2 * M + o] = tabv(dd, T_G_U, ip);
+                        gas[3 * M + o] = tabv(dd, T_VBG, ip) + tabv(dd, T_G_V, ip); gas[4 * M + o] = 0.0;
+                    }
+                    if (dust) {
+                        dust[o] = tabv(dd, T_D_R, ip); dust[M + o] = tabv(dd, T_D_U, ip);
+                        dust[2 * M + o] = tabv(dd, T_VBG, ip) + tabv(dd, T_D_V, ip); dust[3 * M + o] = 0.0;
+                    }
+                }
+                continue;
+            }
+            const size_t c = (size_t)ic + (size_t)nx * (jc + (size_t)ny * kc);
+            if (gas) {
+                for (int q = 0; q < 4; ++q) gas[q * M + o] = g5[q * N + c];
+                gas[4 * M + o] = sgn_w * g5[4 * N + c];
+            }
+            if (dust)
+                for (int q = 0; q < 4; ++q) dust[q * M + o] = d4[q * N + c];
+        }
+    return PG_OK;
+}
+
+int pg_bound_x(pg_handle* h, double* gas, double* dust) { return bound_dir(h, 0, gas, dust); }
+int pg_bound_y(pg_handle* h, double* gas, double* dust) { return bound_dir(h, 1, gas, dust); }
+int pg_bound_z(pg_handle* h, double* gas, double* dust) { return bound_dir(h, 2, gas, dust); }
+
 int pg_get_state(pg_handle* h, double* gas, double* dust) {
     CHECK_H(h);
     if (h->sph) return pg::sph_get_state(h->sph, gas);

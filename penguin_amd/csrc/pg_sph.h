@@ -22,6 +22,8 @@ int sph_validate(const pg_params& P);
 int sph_create(const pg_params& P, Sph** out);
 void sph_destroy(Sph* s);
 int sph_get_state(Sph* s, double* gas);
+void sph_ghost_x(const Sph* s, int g, int k, double* f5);
+void sph_dims(const Sph* s, int* nx, int* ny, int* nz);
 int sph_set_state(Sph* s, const double* gas);
 int sph_global_dt(Sph* s, double dt_prev, double* dt_out);
 int sph_solve(Sph* s, double t, double dt);

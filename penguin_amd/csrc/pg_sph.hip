@@ -1071,6 +1071,7 @@ struct Sph {
     void* rate_h = nullptr;     // pinned
     size_t tab_off[16] = {0};
     std::vector<double> xa, xv, ya, yv, za, zv, vbg, rot, rcyl;
+    std::vector<double> xgh;    // radial ghost states (rho, p, u, v, w) [field][ghost 0..5][polar k], FP64
     Body planet{};              // planet at the time of the last evolve
     double yw[6] = {0};
     size_t bytes = 0;
@@ -1349,6 +1350,7 @@ int upload_tables(Sph* s) {
     std::vector<double> nu((size_t)nxe * nz, 0.0), xg((size_t)NF * 6 * nz, 0.0), ebal((size_t)nxe * nz, 0.0);
     s->rot.assign((size_t)nx * nz, 0.0);
     s->rcyl.assign((size_t)nx * nz, 0.0);
+    s->xgh.assign((size_t)NF * 6 * nz, 0.0);
     for (int k = 0; k < nz; ++k) {
         const double z = zc[k + NPAD];
         for (int ip = 0; ip < nxe; ++ip) {
@@ -1368,6 +1370,8 @@ int upload_tables(Sph* s) {
             const double x = xc[ip];
             const double f[NF] = {pr.rho(x, z), pr.p(x, z), pr.u(x, z), pr.v(x, z) - s->vbg[(size_t)k * nxe + ip], 0.0};
             for (int q = 0; q < NF; ++q) xg[((size_t)q * 6 + g) * nz + k] = f[q];
+            const double fh[NF] = {f[0], f[1], f[2], pr.v(x, z), 0.0};
+            for (int q = 0; q < NF; ++q) s->xgh[((size_t)q * 6 + g) * nz + k] = fh[q];
         }
     }
     std::vector<double> hphi(2 * (size_t)ny);
@@ -1564,6 +1568,15 @@ void sph_destroy(Sph* s) {
     if (s->rate_h) (void)hipHostFree(s->rate_h);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
+}
+
+// The analytic state of radial ghost cell g (0..2 left of the disk, 3..5
+// right) at polar layer k: rho, p, u, v, w (boundary type 0).
+void sph_ghost_x(const Sph* s, int g, int k, double* f5) {
+    for (int q = 0; q < NF; ++q) f5[q] = s->xgh[((size_t)q * 6 + g) * s->nz + k];
+}
+void sph_dims(const Sph* s, int* nx, int* ny, int* nz) {
+    *nx = s->nx; *ny = s->ny; *nz = s->nz;
 }
 
 int sph_get_state(Sph* s, double* gas) {

@@ -69,6 +69,9 @@ SIGNATURES = {
     "pg_torque": (ctypes.c_int, [_vp, _dp]),
     "pg_torque_noalloc": (ctypes.c_int, [_vp, _dp]),
     "pg_probe_grid_barrier": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "pg_bound_x": (ctypes.c_int, [_vp, _dp, _dp]),
+    "pg_bound_y": (ctypes.c_int, [_vp, _dp, _dp]),
+    "pg_bound_z": (ctypes.c_int, [_vp, _dp, _dp]),
     "pg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pg_batch_get_schedule": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "pg_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -175,6 +178,17 @@ class Disk:
                                            _ptr(d) if d is not None else None))
 
     # ---- stepping (timestep.cu:113-140, solve.cu:279-292, main.cu:320-359)
+    def bound(self, direction: int, dust: bool = False):
+        """The state with the ghost cells of one direction filled (pg_bound_x/y/z):
+        arrays shaped as get_state's with that axis 6 longer."""
+        shape = list(self.shape)
+        shape[len(shape) - 1 - direction] += 6
+        g = np.zeros((5,) + tuple(shape))
+        dd = np.zeros((4,) + tuple(shape)) if dust else None
+        f = ("pg_bound_x", "pg_bound_y", "pg_bound_z")[direction]
+        _check(getattr(load_library(), f)(self._h, _ptr(g), _ptr(dd) if dust else None))
+        return g, dd
+
     def global_dt(self, dt_prev: float) -> float:
         out = ctypes.c_double()
         _check(load_library().pg_global_dt(self._h, dt_prev, ctypes.byref(out)))
