@@ -372,6 +372,19 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
     const real dt = (real)d.ctl->dt;
     const PG_GLOBAL real* __restrict__ in = gmem(d.dust[dsel(par)]);
     const size_t row = (size_t)j * nx;
+    // the narrow tiles load the table values of the thread's slots here, as
+    // the gas tile does (parabola slot ia = x0 + t + 2, interface and update
+    // slot ia + 1)
+    constexpr bool ONE = XBlock<TX>::BT >= XBlock<TX>::S && TX <= 250;
+    real hw[6] = {}, hix[2] = {};
+    real hxa = real(0), hixv = real(0), hir = real(0), hvb = real(0);
+    if (ONE && t < ns - 4) {
+        const int ia = x0 + t + 2;
+        hw[0] = TAB(T_WL0, ia); hw[1] = TAB(T_WL1, ia); hw[2] = TAB(T_WL2, ia);
+        hw[3] = TAB(T_WR0, ia); hw[4] = TAB(T_WR1, ia); hw[5] = TAB(T_WR2, ia);
+        hix[0] = TAB(T_IDX, ia); hix[1] = TAB(T_IDX, ia + 1);
+        hxa = TAB(T_XA, ia + 1); hixv = TAB(T_IXV, ia + 1); hir = TAB(T_IR, ia + 1); hvb = TAB(T_VBG, ia + 1);
+    }
 
     for (int s = t; s < ns; s += XBlock<TX>::BT) {
         const int i = x0 - NPAD + s;
@@ -399,8 +412,9 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
     __syncthreads();
     for (int s = t + 2; s < ns - 2; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
-        const real wl0 = TAB(T_WL0, ip), wl1 = TAB(T_WL1, ip), wl2 = TAB(T_WL2, ip);
-        const real wr0 = TAB(T_WR0, ip), wr1 = TAB(T_WR1, ip), wr2 = TAB(T_WR2, ip);
+        const real wl0 = ONE ? hw[0] : TAB(T_WL0, ip), wl1 = ONE ? hw[1] : TAB(T_WL1, ip);
+        const real wl2 = ONE ? hw[2] : TAB(T_WL2, ip), wr0 = ONE ? hw[3] : TAB(T_WR0, ip);
+        const real wr1 = ONE ? hw[4] : TAB(T_WR1, ip), wr2 = ONE ? hw[5] : TAB(T_WR2, ip);
         real l, r;
         slopes(s_r[s - 2], s_r[s - 1], s_r[s], s_r[s + 1], s_r[s + 2], wl0, wl1, wl2, wr0, wr1, wr2, real(0), l, r);
         s_sl[s] = l; s_sr[s] = r;
@@ -417,12 +431,12 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
         if (k >= ns - 2) break;
         const int ipk = x0 + k;
         real Fr, Fu, FL;
-        dust_iface(dt, TAB(T_IDX, ipk - 1), s_r[k - 1], s_sl[k - 1], s_sr[k - 1],
+        dust_iface(dt, ONE ? hix[0] : TAB(T_IDX, ipk - 1), s_r[k - 1], s_sl[k - 1], s_sr[k - 1],
                    s_u[k - 1], s_sl[S + k - 1], s_sr[S + k - 1],
                    s_L[k - 1], s_sl[2 * S + k - 1], s_sr[2 * S + k - 1],
-                   TAB(T_IDX, ipk), s_r[k], s_sl[k], s_sr[k], s_u[k], s_sl[S + k], s_sr[S + k],
+                   ONE ? hix[1] : TAB(T_IDX, ipk), s_r[k], s_sl[k], s_sr[k], s_u[k], s_sl[S + k], s_sr[S + k],
                    s_L[k], s_sl[2 * S + k], s_sr[2 * S + k], Fr, Fu, FL);
-        const real g = TAB(T_XA, ipk);
+        const real g = ONE ? hxa : TAB(T_XA, ipk);
         fx[q][0] = Fr * g;
         fx[q][1] = Fu * g;
         fx[q][2] = FL * g;
@@ -442,7 +456,7 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
     for (int s = t + NPAD; s < ns - NPAD; s += XBlock<TX>::BT) {
         const int ip = x0 + s;
         const int i = ip - NPAD;
-        const real f = dt * TAB(T_IXV, ip);
+        const real f = dt * (ONE ? hixv : TAB(T_IXV, ip));
         const real Dr = (s_Fr[s] - s_Fr[s + 1]) * f;
         const real Du = (s_Fu[s] - s_Fu[s + 1]) * f;
         const real DL = (s_FL[s] - s_FL[s + 1]) * f;
@@ -459,7 +473,7 @@ __device__ __forceinline__ void sweep_x_dust_tile(const LaunchSet<real>& ls, int
             // (r u + Du) / (r + Dr) as u + (Du - u Dr) / (r + Dr): the
             // rounding falls on the change, not on the velocity itself
             un = u + (Du - u * Dr) * irn;
-            vn = vp + ((DL * TAB(T_IR, ip) - TAB(T_VBG, ip) * Dr) - vp * Dr) * irn;
+            vn = vp + ((DL * (ONE ? hir : TAB(T_IR, ip)) - (ONE ? hvb : TAB(T_VBG, ip)) * Dr) - vp * Dr) * irn;
         }
         out[row + i] = rn;
         out[N + row + i] = un;
