@@ -453,20 +453,22 @@ PG_HD void hllc_iso_d(real cl, real cr, real pbl, real tbp, real rl, real pld, r
 
 // trace_right_c / trace_left_c with the ring's background density rb and
 // the deviation rd of the cell's mean; returns the traced density (positive)
-// and its deviation.
+// and its deviation.  dti = dt idx, constant along the ring: the fractions
+// (c - min(u, 0)) dt idx, -dis idx and (c dt - dis) idx with dis = w dt,
+// w = min(ua, 0), are (c - min(u, 0)) dti, -w dti and (c - w) dti.
 template <typename real>
-PG_HD void trace_right_cd(real c, real ic, real dt, real idx, real rb, real rd, real u,
+PG_HD void trace_right_cd(real c, real ic, real dti, real rb, real rd, real u,
                           real rsL, real rsR, real usL, real usR, real vsL, real v, real vsR,
                           real& rr, real& rrd, real& ur, real& vr) {
-    real q = lim01((c - fmin(u, real(0))) * dt * idx);
+    real q = lim01((c - fmin(u, real(0))) * dti);
     real ua = aveL(q, usL, u, usR);
-    real dis = fmin(ua, real(0)) * dt;
-    q = lim01(-dis * idx);
+    const real w = fmin(ua, real(0));
+    q = lim01(-w * dti);
     real r0d = aveL(q, rsL, rd, rsR), r0 = rb + r0d;
     const real rmin = (rb + rd) * real(1.0e-10);
     if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
     real u0 = aveL(q, usL, u, usR);
-    q = lim01((c * dt - dis) * idx);
+    q = lim01((c - w) * dti);
     real u_ = aveL(q, usL, u, usR);
     real rad = aveL(q, rsL, rd, rsR);
     const real xe = exp_lim_m1((u0 - u_) * ic);
@@ -477,18 +479,18 @@ PG_HD void trace_right_cd(real c, real ic, real dt, real idx, real rb, real rd, 
 }
 
 template <typename real>
-PG_HD void trace_left_cd(real c, real ic, real dt, real idx, real rb, real rd, real u,
+PG_HD void trace_left_cd(real c, real ic, real dti, real rb, real rd, real u,
                          real rsL, real rsR, real usL, real usR, real vsL, real v, real vsR,
                          real& rl, real& rld, real& ul, real& vl) {
-    real x = lim01((fmax(u, real(0)) + c) * dt * idx);
+    real x = lim01((fmax(u, real(0)) + c) * dti);
     real ua = aveR(x, usL, u, usR);
-    real dis = fmax(ua, real(0)) * dt;
-    x = lim01(dis * idx);
+    const real w = fmax(ua, real(0));   // dis = w dt
+    x = lim01(w * dti);
     real r0d = aveR(x, rsL, rd, rsR), r0 = rb + r0d;
     const real rmin = (rb + rd) * real(1.0e-10);
     if (r0 < rmin) { r0 = rmin; r0d = rmin - rb; }
     real u0 = aveR(x, usL, u, usR);
-    x = lim01((dis + c * dt) * idx);
+    x = lim01((w + c) * dti);
     real u_ = aveR(x, usL, u, usR);
     real rad = aveR(x, rsL, rd, rsR);
     const real xe = exp_lim_m1((u_ - u0) * ic);

@@ -584,12 +584,16 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
     const real dts = dt * iR;                  // set_state / dust_flow receive dt/rad
     const real idy = d.idy;
     const real fup = dt * idy * iR;            // Del.multiply(dt/yv/rad_cyl), sweeps.cu:412
+    const real dti = dts * idy;                // dt idx of the traced states
     const int inc = orbadv ? gmem(d.inc)[ic] : 0;
     const PG_GLOBAL real* __restrict__ in = gmem(DUST ? d.dust[dsel(par)] : d.gas[gsel(par)]);
     PG_GLOBAL real* __restrict__ out = gmem(DUST ? d.dust[dsel(par) ^ 1] : d.gas[gsel(par) ^ 1]);
     const real wy[6] = {d.wLy[0], d.wLy[1], d.wLy[2], d.wRy[0], d.wRy[1], d.wRy[2]};
     const real rfloor = d.D_G_ratio * real(1.0e-10);
     const real rbg = DUST ? real(0) : TAB(T_G_R, ip);   // what the stored density is relative to
+    // the sound speed of the radius, read once (the gas interfaces use them)
+    const real cs2 = DUST ? real(0) : TAB(T_CS2, ip), cs0 = DUST ? real(0) : TAB(T_C0, ip);
+    const real ic0 = DUST ? real(0) : TAB(T_IC0, ip);
     // B is a multiple of TR, so lane t keeps radius ri in every loop below
     static_assert(B % TR == 0, "tile");
 
@@ -642,12 +646,11 @@ __device__ __forceinline__ void phi_tile(const LaunchSet<real>& ls, int2 w, int 
                        fa[k][0], fa[k][1], fa[k][2]);
             fa[k][3] = real(0);
         } else {
-            const real cs2 = TAB(T_CS2, ip), cs0 = TAB(T_C0, ip), ic0 = TAB(T_IC0, ip);
             real rl, rld, ul, vl, rr, rrd, ur, vr;
             // the pressure parabola is c_s^2 times the density parabola
-            trace_left_cd(cs0, ic0, dts, idy, rbg, c0[m], c1[m], sl0[m], sr0[m], sl1[m], sr1[m], sl2[m], c2[m],
+            trace_left_cd(cs0, ic0, dti, rbg, c0[m], c1[m], sl0[m], sr0[m], sl1[m], sr1[m], sl2[m], c2[m],
                           sr2[m], rl, rld, ul, vl);
-            trace_right_cd(cs0, ic0, dts, idy, rbg, c0[e], c1[e], sl0[e], sr0[e], sl1[e], sr1[e], sl2[e], c2[e],
+            trace_right_cd(cs0, ic0, dti, rbg, c0[e], c1[e], sl0[e], sr0[e], sl1[e], sr1[e], sl2[e], c2[e],
                            sr2[e], rr, rrd, ur, vr);
             hllc_cd(cs0, ic0, cs2, rbg, rl, rld, ul, vl, rr, rrd, ur, vr, fa[k][0], fa[k][1], fa[k][2], fa[k][3]);
         }
