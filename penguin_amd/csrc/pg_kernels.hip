@@ -1113,6 +1113,7 @@ __global__ __launch_bounds__(256) void k_predvis(LaunchSet<real> ls, int par) {
             for (int ly = tl / PT_X; ly < th + 2; ly += PT_TY) pred(lx, ly, c.x, tb, rb, gu, gv);
         }
         {   // the two halo columns: one slot for each of the threads 2 PT_X .. 2 PT_X + 2 (TH + 2) - 1
+            static_assert(2 * PT_X + 2 * (TH + 2) <= PT_X * PT_TY, "the halo columns need a thread per slot");
             const int hs = tl - 2 * PT_X;
             if (hs >= 0 && hs < 2 * (th + 2)) {
                 const int side = hs >= th + 2, ly = hs - side * (th + 2), lx = side ? PT_X + 1 : 0;
@@ -1241,6 +1242,7 @@ __global__ __launch_bounds__(256) void k_corrvis(LaunchSet<real> ls, int par, in
         for (int ly = tl / PT_X; ly < th + 2; ly += PT_TY) corr(lx, ly, c);
     }
     {   // the two halo columns: one slot for each of the threads 2 PT_X .. 2 PT_X + 2 (TH + 2) - 1
+        static_assert(2 * PT_X + 2 * (TH + 2) <= PT_X * PT_TY, "the halo columns need a thread per slot");
         const int hs = tl - 2 * PT_X;
         if (hs >= 0 && hs < 2 * (th + 2)) {
             const int side = hs >= th + 2, ly = hs - side * (th + 2), lx = side ? PT_X + 1 : 0;
