@@ -244,7 +244,7 @@ int pg_batch_set_state(pg_batch* b, int disk, const double* gas, const double* d
 int pg_batch_run(pg_batch* b, int nsteps, double* t_io, double* dt_io, double* dts, int use_graph);
 /* launch schedule: x-sweep tile width (128, 256, 384, 512, 1024, or 122,
    250, 506, whose tiles with their 6 halo cells fill 2, 4, 8 waves); threads
-   per azimuthal tile (256: 16 radii x 16 rows, 512: 16 x 32, 1024: 16 x 64);
+   per azimuthal tile (256: 16 radii x 16 rows, 512: 8 x 128, FP64 8 x 64, 1024: 16 x 64);
    azimuthal tile order (0: each XCD a contiguous band of tile rows of every
    disk, 1: row by row, 2: radius strip by strip); rows of the pointwise and
    stencil tiles (4, 8 or 16; 0 = chosen from the batch size) */

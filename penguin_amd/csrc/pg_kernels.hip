@@ -510,12 +510,14 @@ __global__ __launch_bounds__(XBlock<TX>::BT) void k_sweep_x_both(LaunchSet<real>
 // output rows.  The rows of one radius are independent of the other radii,
 // so a tile is TR independent azimuthal strips side by side; TR consecutive
 // radii make each row of the tile one contiguous segment in HBM.
-// With 512 threads an FP32 tile is 64 rows high (each thread takes two cells
-// of every stage, which halves the per-cell share of the tile set-up and
-// the 12-row halo); an FP64 tile, twice the LDS per cell, 32 rows.
+// With 512 threads an FP32 tile is 8 radii x 128 rows (each thread takes two
+// cells of every stage; the 12-row halo is 9 % of the rows swept, against
+// 19 % for 16 x 64 in the same 40 KB of LDS: 73 us against 80 on the
+// 1024x3072 disk; 4 x 256, with 16-byte row segments in HBM, 96 us); an
+// FP64 tile, twice the LDS per cell, 8 x 64.
 template <typename real, int B> struct PhiTile;
 template <typename real> struct PhiTile<real, 256> { static constexpr int TR = 16, TY = 16; };
-template <typename real> struct PhiTile<real, 512> { static constexpr int TR = 16, TY = sizeof(real) == 4 ? 64 : 32; };
+template <typename real> struct PhiTile<real, 512> { static constexpr int TR = 8, TY = sizeof(real) == 4 ? 128 : 64; };
 template <typename real> struct PhiTile<real, 1024> { static constexpr int TR = 16, TY = 64; };
 
 // Azimuthal pass of one fluid over a tile, fused with the orbital shift and
