@@ -998,9 +998,11 @@ __global__ __launch_bounds__(256) void k_adj_src(AdjArgs<real> A) {
 // ------------------------------------------------------------- launchers
 template <typename real>
 struct AdjLaunch {
-    // 244-cell tiles: with the 12 halo slots, 256 slots fill four waves (a
-    // 256-cell tile's 268 slots took a fifth, mostly idle, wave)
-    static constexpr int TXA = 244;
+    // 116-cell tiles of two waves (ADJ_TX): a 512-radius row takes four and
+    // a 48-radius one, 9 wave passes with one idle wave, where 244-cell
+    // tiles of four waves took 12 with three idle; and a CU holds more of
+    // the smaller workgroups
+    static constexpr int TXA = ADJ_TX;
     static hipError_t x(const AdjArgs<real>& A, int nx, int ny, hipStream_t s) {
         hipLaunchKernelGGL((k_adj_x<real, TXA>), dim3((nx + TXA - 1) / TXA, ny), dim3(AdjXBlock<TXA>::B), 0, s, A);
         return hipGetLastError();

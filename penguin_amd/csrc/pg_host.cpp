@@ -1401,7 +1401,7 @@ namespace {
 // rows; source adjoints: 256-radius segments x rows; azimuthal adjoints:
 // 8-radius x 16-row tiles at most; viscosity: 64 x 8 tiles at most)
 int adj_slots(int nx, int ny) {
-    const int a = ((nx + 127) / 128) * ny;
+    const int a = ((nx + ADJ_TX - 1) / ADJ_TX) * ny;   // x-sweep adjoint tiles
     const int b = ((nx + 2 + 255) / 256) * ny;
     const int c = ((nx + 7) / 8) * ((ny + 15) / 16);
     const int v = ((nx + 63) / 64) * ((ny + 7) / 8);   // viscosity tiles (the smaller, FP64)

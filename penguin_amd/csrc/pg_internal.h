@@ -37,6 +37,10 @@ constexpr int NPAD = 3;
 // keep the one-cell halo of the stencil kernels small; short ones give a
 // small disk more workgroups.
 constexpr int PT_X = 64, PT_TY = 4, PT_YMAX = 16;
+// Radii per tile of the x-sweep adjoint: with its 12 halo slots, 128 slots
+// fill two waves, one thread per slot (the host sizes the per-workgroup
+// parameter sums of the adjoint from it).
+constexpr int ADJ_TX = 116;
 constexpr int NGAS = 3;
 constexpr int NDUST = 3;
 constexpr int RING_THREADS = 512;   // default threads per azimuthal-pass block

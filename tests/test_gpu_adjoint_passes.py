@@ -86,7 +86,7 @@ def _perturbed(g, dirn, e):
     return h
 
 
-# 520 radii: the x adjoint's rows span three 244-cell tiles (the last one
+# 520 radii: the x adjoint's rows span five 116-cell tiles (the last one
 # partial) and the azimuthal adjoint's 8-radius strips 65 tiles
 @pytest.mark.parametrize("size", [(64, 96), (520, 48)])
 @pytest.mark.parametrize("name", ["sweep_x", "sweep_y", "orbital_advection", "viscosity", "kick", "corrector"])
