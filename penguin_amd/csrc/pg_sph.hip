@@ -51,7 +51,7 @@ namespace {
 constexpr int NF = 5;            // rho, p, u, vp, w
 constexpr int SPH_BT = 256;      // threads per block of the pointwise kernels and the radial sweep
 constexpr int SPH_BT2 = 512;     // threads per block of the strided sweeps (about one element each)
-constexpr int SPH_BTX = 512;     // most threads per radial-sweep block (one per LDS slot)
+constexpr int SPH_BTX = 256;     // most threads per radial-sweep block (one per LDS slot)
 constexpr int SPH_R = 8;         // rows per strided-sweep block
 constexpr int SPH_TY = 64;       // azimuthal segment per block
 constexpr int SPH_ZMAX = 100;    // polar cells: the whole column is one tile (FP64: 102 KB of LDS)
@@ -103,7 +103,10 @@ struct SphK {
 // Radial segment of the radial sweep: the row cut into the fewest equal
 // parts whose slots (segment + 2 NPAD) fit one block of SPH_BTX threads, so
 // every thread holds one slot and no block runs half empty (a 384-radius row
-// is one 390-slot block of 448 threads).
+// is two 198-slot blocks of 256 threads: a CU holds more of them than of one
+// 390-slot block of 448 threads, and one block's loads hide behind the
+// others' arithmetic: 1343 -> 1219 us on the 384x1536x64 disk; 128-thread
+// blocks of 102 slots 1240-1256 us).
 inline int sph_tx(int nx) {
     const int parts = (nx + SPH_BTX - 2 * NPAD - 1) / (SPH_BTX - 2 * NPAD);
     return (nx + parts - 1) / parts;
