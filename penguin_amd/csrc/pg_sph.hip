@@ -53,7 +53,11 @@ constexpr int SPH_BT = 256;      // threads per block of the pointwise kernels a
 constexpr int SPH_BT2 = 512;     // threads per block of the strided sweeps (about one element each)
 constexpr int SPH_BTX = 256;     // most threads per radial-sweep block (one per LDS slot)
 constexpr int SPH_R = 8;         // rows per strided-sweep block
-constexpr int SPH_TY = 64;       // azimuthal segment per block
+// azimuthal segment per block: with its 6 halo rows, 8 radii x 64 slots fill
+// a 512-thread block (and two passes of a 256-thread one) exactly (64 rows:
+// 560 slots, a partly filled extra pass; 1091 -> 1050 us for the azimuthal
+// sweep, 591 -> 555 us for the orbital advection of the 384x1536x64 disk)
+constexpr int SPH_TY = 58;
 constexpr int SPH_ZMAX = 100;    // polar cells: the whole column is one tile (FP64: 102 KB of LDS)
 
 template <typename real>
