@@ -825,31 +825,37 @@ __device__ __forceinline__ void vis_cell(const DiskArgs<real>& d, const VisTab<r
     const real i20 = tb.i20, i21 = tb.i21, i10 = tb.i10;
     const real ir = tb.ir;
     const real idy = d.idy;
-    const real hdy = real(0.5) * idy;            // 1/(t2 - t0)
+    // scale factors of the differences, per radius (formed once per thread
+    // when the stencil runs over several rows of one radius)
+    const real hir = real(0.5) * idy * ir;        // 1/((t2 - t0) R)
+    const real yy = (idy * idy) * (ir * ir);      // 1/(dy R)^2
+    const real xy = i20 * hir;
+    const real i20x2 = real(2) * i20;
     // radial derivatives
-    real dudx = (u[2] - u[0]) * i20;
-    real d2udx2 = ((u[2] - u[1]) * i21 - (u[1] - u[0]) * i10) * (real(2) * i20);
-    real dvdx = tb.dvbg + (v[2] - v[0]) * i20;
-    real d2vdx2 = tb.d2vbg + ((v[2] - v[1]) * i21 - (v[1] - v[0]) * i10) * (real(2) * i20);
-    real dndx = (nr[2] - nr[0]) * i20;
+    const real dudx = (u[2] - u[0]) * i20;
+    const real d2udx2 = ((u[2] - u[1]) * i21 - (u[1] - u[0]) * i10) * i20x2;
+    const real dvdx = tb.dvbg + (v[2] - v[0]) * i20;
+    const real d2vdx2 = tb.d2vbg + ((v[2] - v[1]) * i21 - (v[1] - v[0]) * i10) * i20x2;
+    const real dndx = (nr[2] - nr[0]) * i20;
     // azimuthal and mixed derivatives
-    real dvdy = (v[7] - v[4]) * hdy * ir;
-    real d2vdy2 = (v[7] - real(2) * v[1] + v[4]) * (idy * idy) * ir * ir;
-    real d2vdxy = (v[8] - v[6] - v[5] + v[3]) * i20 * hdy * ir;
-    real dudy = (u[7] - u[4]) * hdy * ir;
-    real d2udy2 = (u[7] - real(2) * u[1] + u[4]) * (idy * idy) * ir * ir;
-    real d2udxy = (u[8] - u[6] - u[5] + u[3]) * i20 * hdy * ir;
-    real dndy = (nr[7] - nr[4]) * hdy * ir;
-    const real v1 = tb.vbg + v[1];
-    const real n1 = nr[1];
-    real sxx = real(2) * n1 * dudx;
-    real fx = real(2) * dndx * dudx + real(2) * n1 * d2udx2 + sxx * ir;
-    real syy = real(2) * n1 * (dvdy + u[1] * ir);
-    real sxy = n1 * (dudy + dvdx - v1 * ir);
-    real fy = real(2) * dndy * dvdy + real(2) * n1 * d2vdy2 + real(2) * n1 * dudy * ir + real(2) * dndy * u[1] * ir;
-    fx += dndy * (dudy + dvdx) + n1 * (d2udy2 + d2vdxy) - dndy * v1 * ir - n1 * dvdy * ir - syy * ir;
-    fy += dndx * (dudy + dvdx) + n1 * (d2udxy + d2vdx2) - dndx * v1 * ir - n1 * dvdx * ir + n1 * v1 * ir * ir +
-          real(2) * sxy * ir;
+    const real dvdy = (v[7] - v[4]) * hir;
+    const real d2vdy2 = (v[7] - real(2) * v[1] + v[4]) * yy;
+    const real d2vdxy = (v[8] - v[6] - v[5] + v[3]) * xy;
+    const real dudy = (u[7] - u[4]) * hir;
+    const real d2udy2 = (u[7] - real(2) * u[1] + u[4]) * yy;
+    const real d2udxy = (u[8] - u[6] - u[5] + u[3]) * xy;
+    const real dndy = (nr[7] - nr[4]) * hir;
+    // stresses sxx = 2 n dudx, syy = 2 n (dvdy + u/R), sxy = n (dudy + dvdx -
+    // v/R) and their divergence in polar coordinates, the terms over R
+    // gathered
+    const real n1 = nr[1], n2 = real(2) * n1;
+    const real vir = (tb.vbg + v[1]) * ir, uir = u[1] * ir;
+    const real t = dudy + dvdx - vir;
+    const real sxx = n2 * dudx, syy = n2 * (dvdy + uir), sxy = n1 * t;
+    const real fx = real(2) * dndx * dudx + n2 * d2udx2 + dndy * t + n1 * (d2udy2 + d2vdxy) +
+                    ir * (sxx - n1 * dvdy - syy);
+    const real fy = real(2) * dndy * (dvdy + uir) + n2 * d2vdy2 + dndx * t + n1 * (d2udxy + d2vdx2) +
+                    ir * (n2 * dudy - n1 * dvdx + n1 * vir + real(2) * sxy);
     fx_out = fx;
     fy_out = fy;
 }
