@@ -940,15 +940,22 @@ __global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a, PlanetS<real> 
     }
     const real dy = a.dy;
     const real ir1 = FM<real>::rcp(r[1]);
+    // reciprocal spacings and derivative scales of this thread's radius,
+    // formed once for all its rows
+    const real i20 = FM<real>::rcp(r[2] - r[0]), i21 = FM<real>::rcp(r[2] - r[1]), i10 = FM<real>::rcp(r[1] - r[0]);
+    const real i20x2 = real(2) * i20;                      // 1 / ((r2 - r0) / 2)
+    const real hir = FM<real>::rcp(real(2) * dy) * ir1;    // 1 / (2 dy r)
+    const real idy = FM<real>::rcp(dy);
+    const real yy = (idy * idy) * (ir1 * ir1);             // 1 / (dy r)^2
+    const real xy = i20 * hir;
     auto d1 = [&](const real* f, real& dx, real& dxx) {
-        dx = FM<real>::div(f[2] - f[0], r[2] - r[0]);
-        dxx = FM<real>::div(FM<real>::div(f[2] - f[1], r[2] - r[1]) - FM<real>::div(f[1] - f[0], r[1] - r[0]),
-                            real(0.5) * (r[2] - r[0]));
+        dx = (f[2] - f[0]) * i20;
+        dxx = ((f[2] - f[1]) * i21 - (f[1] - f[0]) * i10) * i20x2;
     };
     auto d2 = [&](const real* f, real& dyv, real& dyy, real& dxy) {
-        dyv = FM<real>::div(f[7] - f[4], real(2) * dy) * ir1;
-        dyy = FM<real>::div(FM<real>::div(f[7] - f[1], dy) - FM<real>::div(f[1] - f[4], dy), dy) * ir1 * ir1;
-        dxy = FM<real>::div(f[8] - f[6] - f[5] + f[3], (r[2] - r[0]) * (real(2) * dy)) * ir1;
+        dyv = (f[7] - f[4]) * hir;
+        dyy = ((f[7] - f[1]) - (f[1] - f[4])) * yy;
+        dxy = (f[8] - f[6] - f[5] + f[3]) * xy;
     };
     for (int ty = t / VX; ty < VY; ty += 256 / VX) {
         const int j = j0 + ty;
@@ -963,20 +970,21 @@ __global__ __launch_bounds__(256) void k3_viscosity(SphK<real> a, PlanetS<real> 
         }
         real dudx, d2udx2, dvdx, d2vdx2, dvdy, d2vdy2, d2vdxy, dudy, d2udy2, d2udxy;
         d1(u, dudx, d2udx2);
-        const real dndx = FM<real>::div(nr[2] - nr[0], r[2] - r[0]);
+        const real dndx = (nr[2] - nr[0]) * i20;
         d1(v, dvdx, d2vdx2);
         d2(v, dvdy, d2vdy2, d2vdxy);
         d2(u, dudy, d2udy2, d2udxy);
-        const real dndy = FM<real>::div(nr[7] - nr[4], real(2) * dy) * ir1;
-        const real sxx = real(2) * nr[1] * dudx;
-        real fx = real(2) * dndx * dudx + real(2) * nr[1] * d2udx2 + sxx * ir1;
-        const real syy = real(2) * nr[1] * (dvdy + u[1] * ir1);
-        const real sxy = nr[1] * (dudy + dvdx - v[1] * ir1);
-        real fy = real(2) * dndy * dvdy + real(2) * nr[1] * d2vdy2 + real(2) * nr[1] * dudy * ir1 +
-                  real(2) * dndy * u[1] * ir1;
-        fx += dndy * (dudy + dvdx) + nr[1] * (d2udy2 + d2vdxy) - dndy * v[1] * ir1 - nr[1] * dvdy * ir1 - syy * ir1;
-        fy += dndx * (dudy + dvdx) + nr[1] * (d2udxy + d2vdx2) - dndx * v[1] * ir1 - nr[1] * dvdx * ir1 +
-              nr[1] * v[1] * ir1 * ir1 + real(2) * sxy * ir1;
+        const real dndy = (nr[7] - nr[4]) * hir;
+        // stresses sxx = 2 n dudx, syy = 2 n (dvdy + u/r), sxy = n (dudy +
+        // dvdx - v/r) and their divergence, the terms over r gathered
+        const real n1 = nr[1], n2 = real(2) * n1;
+        const real vir = v[1] * ir1, uir = u[1] * ir1;
+        const real tt = dudy + dvdx - vir;
+        const real sxx = n2 * dudx, syy = n2 * (dvdy + uir), sxy = n1 * tt;
+        const real fx = real(2) * dndx * dudx + n2 * d2udx2 + dndy * tt + n1 * (d2udy2 + d2vdxy) +
+                        ir1 * (sxx - n1 * dvdy - syy);
+        const real fy = real(2) * dndy * (dvdy + uir) + n2 * d2vdy2 + dndx * tt + n1 * (d2udxy + d2vdx2) +
+                        ir1 * (n2 * dudy - n1 * dvdx + n1 * vir + real(2) * sxy);
         const size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
         a.vis_out[c] = fx;
         a.vis_out[N + c] = fy;
